@@ -1,0 +1,165 @@
+/*
+ * vanadis_gpu.h -- C ABI of libvanadis_gpu.so, the MI355X (gfx950) solve path
+ * for Vanadis's element-by-element directional-residual (DR) HEX8 operator and
+ * its Jacobi-preconditioned BiCG.
+ *
+ * Plain C types only: host pointers unless a name ends in _dev, sizes as int /
+ * int64_t, every call returns an int status (0 = VGPU_OK) except the legacy
+ * Fortran entry, which keeps its reference signature.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *   kernel_dbcg_simple_  <- src_v2026_2_1/cudatest.cu:301-305 (atomic scatter)
+ *                           src_v2026_2_1/cudatest_coloring_gtx1060_optimized.cu:434-438
+ *                           (8-colour scatter); called from VANADIS.FOR:809-810
+ *   vgpu_create / vgpu_set_coeffs <- the per-element data that crosses into
+ *                           el3d (VANADIS.FOR:456-636) via data3d (398-419),
+ *                           atmosfera (188-224) and licz_alfa (168-186)
+ *   vgpu_apply           <- matvec_0 / mttvec_0 (VANADIS.FOR:2149-2245)
+ *   vgpu_assemble_rhs    <- the RHS/diagonal part of przest3d (VANADIS.FOR:723-769,
+ *                           799-805)
+ *   vgpu_solve           <- dbcg_simple (VANADIS.FOR:977-1055) / kernel_dbcg_simple_
+ *   vgpu_mass            <- integrate_mass_hex8 (VANADIS.FOR:2493-2561)
+ */
+#ifndef VANADIS_GPU_H
+#define VANADIS_GPU_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define VGPU_OK 0
+#define VGPU_ERR_ARG 1          /* bad argument (null, size, enum)            */
+#define VGPU_ERR_HIP 2          /* a HIP runtime call failed                  */
+#define VGPU_ERR_MESH 3         /* coords/ISS are not the structured HEX8 grid */
+#define VGPU_ERR_STATE 4        /* call order (e.g. solve before set_coeffs)  */
+#define VGPU_NOT_CONVERGED 5    /* itmax reached with err > rtol              */
+#define VGPU_BREAKDOWN 6        /* BiCG breakdown: a zero/NaN scalar          */
+#define VGPU_ERR_NOMEM 7
+
+/* ---- enums ---- */
+/* scatter modes of the element operator */
+#define VGPU_SCATTER_OWNER 0    /* owner-computes tiles, deterministic, 1 pass */
+#define VGPU_SCATTER_COLORED 1  /* 8 brick colours, RMW, deterministic        */
+#define VGPU_SCATTER_ATOMIC 2   /* hardware FP64/FP32 atomics on tile faces   */
+/* DR variants (gauss_point_hex8DR): V1 VANADIS.FOR:1868-2023,
+ * V2 src_updates/gauss_point_hex8DR.for, V3 vanadis_full_DR_OpenMP_timing.for:1567-1716 */
+#define VGPU_DR_V1 1
+#define VGPU_DR_V2 2
+#define VGPU_DR_V3 3
+/* element-system schemes: A = cH*H + cC*C */
+#define VGPU_SCHEME_THETA 0     /* A = 2H + (3/dt)C   (VANADIS.FOR:743-748)   */
+#define VGPU_SCHEME_STEADY 1    /* A = H                                      */
+#define VGPU_SCHEME_EULER 2     /* A = H + C/dt                               */
+/* precisions */
+#define VGPU_FP64 0
+#define VGPU_FP32 1             /* FP32 element math, FP64 node accumulation  */
+#define VGPU_MIXED 2            /* solve: FP32 inner BiCG + FP64 refinement   */
+
+typedef struct vgpu_ctx vgpu_ctx;
+
+/* Last error message of the calling thread (static storage, never NULL). */
+const char *vgpu_last_error(void);
+/* Library version string. */
+const char *vgpu_version(void);
+/* Number of visible HIP devices (0 when none); never initialises a context. */
+int vgpu_device_count(int *count);
+
+/* ------------------------------------------------------------------------ *
+ * Legacy drop-in (src_v2026_2_1/cudatest.cu:301-305).  Every argument by
+ * reference, Fortran default INTEGER / REAL*8.  Semantics of the reference:
+ * Jacobi-PBiCG on (I-D)A(I-D) with A the assembled 8x8 element matrices BB
+ * (row-major per element, BB[64e+8i+j]), ISS column-major 1-based, stop when
+ * ||r||/||b|| <= TOL on the recursive residual; X in/out, zeroed at masked
+ * nodes.  Differences, by design: ITER/ERR are also written when x0 already
+ * satisfies TOL (ITER=0), and a HIP failure sets ERR=+Inf so the host takes
+ * its "stopping criterion not satisfied" path.  Scatter mode: environment
+ * VGPU_LEGACY_SCATTER=atomic|colored (default colored) or
+ * vgpu_legacy_set_scatter().  Device buffers persist between calls with the
+ * same sizes (one cache per process; not re-entrant, like the reference).
+ * ------------------------------------------------------------------------ */
+void kernel_dbcg_simple_(int *N, double *B, double *X, double *TOL, int *ITMAX,
+                         int *ITER, double *ERR, double *BB, int *ISS,
+                         double *DIAG, int *NAX, int *NAY, int *NAZ,
+                         int *bc_dirichlet);
+int vgpu_legacy_set_scatter(int scatter);
+/* Release the legacy path's cached device buffers. */
+int vgpu_legacy_release(void);
+
+/* ------------------------------------------------------------------------ *
+ * Context API: matrix-free operator, device-resident across time steps.
+ * ------------------------------------------------------------------------ */
+/* Build a context on the current device from the host's node coordinates
+ * xx/yy/zz[nax*nay*naz] (siatka order, VANADIS.FOR:226-244), the
+ * connectivity iss[8*E] (w_el layout, VANADIS.FOR:246-276; NULL = implied)
+ * and the Dirichlet mask bc[N] in {0,1}.  The coordinates must form a tensor
+ * grid and iss must equal the structured connectivity (VGPU_ERR_MESH). */
+int vgpu_create(int nax, int nay, int naz, const double *xx, const double *yy,
+                const double *zz, const int *iss, const int *bc,
+                vgpu_ctx **out);
+int vgpu_destroy(vgpu_ctx *ctx);
+int vgpu_get_sizes(const vgpu_ctx *ctx, int64_t *n_nodes, int64_t *n_elems);
+
+/* Per-element coefficients (atmosfera hook): isotropic diffusivity lam[E],
+ * velocity vel[3*E] (SoA: vx[0..E), vy[E..2E), vz[2E..3E)), decay P, Robin
+ * alpha on the x-min face (face 3, VANADIS.FOR:579-627; 0 disables), DR
+ * variant, scheme and dt.  tau is computed on the device per element. */
+int vgpu_set_coeffs(vgpu_ctx *ctx, const double *lam, const double *vel,
+                    double P, double alpha, int dr_variant, int scheme,
+                    double dt);
+/* Change only the scheme / dt (e.g. steady vs implicit Euler). */
+int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
+/* Options: scatter mode for the solver and z-chunk of the owner tiles
+ * (0 = library default). */
+int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk);
+
+/* y = (I-D) A (I-D) x  (transpose=1: A^T), host vectors of length N.
+ * precision VGPU_FP64 or VGPU_FP32 (x,y still FP64 on the host). */
+int vgpu_apply(vgpu_ctx *ctx, int scatter, int precision, int transpose,
+               const double *x, double *y);
+/* Device-pointer variant on the context's stream (x_dev/y_dev FP64 [N]). */
+int vgpu_apply_dev(vgpu_ctx *ctx, int scatter, int transpose,
+                   const double *x_dev, double *y_dev);
+/* Jacobi diagonal diag[N] = sum_e A_e(a,a), 1 on masked nodes
+ * (VANADIS.FOR:761-769, 799-805). */
+int vgpu_diag(vgpu_ctx *ctx, double *diag);
+/* RHS of the element system, masked (VANADIS.FOR:723-741, 756-758, 799-805):
+ * theta: b = 3F - (H - (3/dt)C) t_old;  steady: b = F;
+ * Euler: b = F + (C/dt) t_old.  q_in[E] per-element source [mass/time];
+ * t_old may be NULL (zero). */
+int vgpu_assemble_rhs(vgpu_ctx *ctx, const double *q_in, const double *t_old,
+                      double *b);
+/* Solve A x = b.  x is in/out (warm start, zeroed at masked nodes).
+ * precision VGPU_FP64 or VGPU_MIXED (inner FP32 BiCG to inner_rtol, FP64
+ * true-residual refinement to rtol).  iters = total BiCG iterations,
+ * err = final relative residual (recursive for FP64, true for MIXED). */
+int vgpu_solve(vgpu_ctx *ctx, int precision, const double *b, double *x,
+               double rtol, int itmax, double inner_rtol, int *iters,
+               double *err);
+/* Same with b, x already on the device (FP64 [N]); diagnostic timings. */
+int vgpu_solve_dev(vgpu_ctx *ctx, int precision, const double *b_dev,
+                   double *x_dev, double rtol, int itmax, double inner_rtol,
+                   int *iters, double *err, double *seconds);
+/* ||true residual||/||b|| for the current coefficients (FP64). */
+int vgpu_true_residual(vgpu_ctx *ctx, const double *b, const double *x,
+                       double *rel);
+/* Mass integral (2x2x2 Gauss, exact for trilinear c on boxes). */
+int vgpu_mass(vgpu_ctx *ctx, const double *c, double *mass);
+
+/* Device buffer helpers for callers that keep data resident (bench, batch). */
+int vgpu_dev_alloc(vgpu_ctx *ctx, int64_t bytes, void **ptr);
+int vgpu_dev_free(vgpu_ctx *ctx, void *ptr);
+int vgpu_memcpy_h2d(vgpu_ctx *ctx, void *dst_dev, const void *src, int64_t bytes);
+int vgpu_memcpy_d2h(vgpu_ctx *ctx, void *dst, const void *src_dev, int64_t bytes);
+
+/* Timing of the dominant kernel with HIP events on the context's stream:
+ * `reps` back-to-back operator launches; avg_ms per launch.
+ * what: 0 = single apply (A x), 1 = dual apply (A p and A^T pp, as in BiCG),
+ *       2 = device copy of N doubles (bandwidth calibration). */
+int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
+                     int reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,285 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so) and of the
+flang-built reference (oracle/_ref/*.so, when present).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+REF_DIR = os.path.join(HERE, "_ref")
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+
+
+def build(force: bool = False) -> None:
+    """Compile liboracle.so (and the reference build when its sources exist)."""
+    if force or not os.path.exists(LIB):
+        subprocess.run(["make", "-B" if force else "-s", "-C", HERE, "liboracle.so"],
+                       check=True, stdout=subprocess.DEVNULL)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.orc_el3d.argtypes = [C.c_int, _dp, _dp, _dp, _dp, C.c_double, _dp, C.c_double,
+                               C.c_int, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_gauss_point.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                      _dp, _dp, _dp, _pd, _dp, _dp, _dp, _dp, _dp, _pd]
+        L.orc_assemble.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                   _dp, _dp, _dp, _ip, _ip, _dp, C.c_double, _dp, _dp,
+                                   C.c_double, C.c_double, C.c_void_p, _dp, _dp, _dp]
+        L.orc_assemble.restype = C.c_int
+        for f in (L.orc_matvec_0, L.orc_mttvec_0):
+            f.argtypes = [_dp, _dp, _ip, _dp, C.c_int, C.c_int, C.c_int, _ip]
+        L.orc_dbcg_simple.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _pi, _pd, _dp,
+                                      _ip, _dp, C.c_int, C.c_int, C.c_int, _ip]
+        L.orc_dbcg_simple.restype = C.c_int
+        L.orc_integrate_mass_hex8.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp]
+        L.orc_integrate_mass_hex8.restype = C.c_double
+        L.orc_diagonal_dominance_factor.argtypes = [C.c_int, C.c_int, _ip, _dp]
+        L.orc_diagonal_dominance_factor.restype = C.c_double
+        L.orc_is_diagonally_dominant.argtypes = [C.c_int, C.c_int, _ip, _dp]
+        L.orc_is_diagonally_dominant.restype = C.c_int
+        L.orc_w_el.argtypes = [C.c_int, C.c_int, C.c_int, _ip]
+        L.orc_kod_el_dr.argtypes = [C.c_int, C.c_int, C.c_int, _ip]
+        L.orc_build_bc_nodes.argtypes = [C.c_int, _ip, _ip, C.c_int, C.c_int, C.c_int, _ip]
+        L.orc_siatka.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
+        L.orc_generate_hex8_colors.argtypes = [C.c_int, C.c_int, C.c_int, _ip]
+        L.orc_build_color_element_lists.argtypes = [C.c_int, _ip, _ip, _ip, _ip]
+        _lib = L
+    return _lib
+
+
+# ---- thin numpy-facing helpers --------------------------------------------
+def w_el(nax, nay, naz):
+    E = (nax - 1) * (nay - 1) * (naz - 1)
+    iss = np.zeros(8 * E, np.int32)
+    lib().orc_w_el(nax, nay, naz, iss)
+    return iss
+
+
+def kod_el_dr(nax, nay, naz):
+    E = (nax - 1) * (nay - 1) * (naz - 1)
+    kod = np.zeros(6 * E, np.int32)
+    lib().orc_kod_el_dr(nax, nay, naz, kod)
+    return kod
+
+
+def build_bc_nodes(facemap, iss, kod, nax, nay, naz):
+    m = np.zeros(nax * nay * naz, np.int32)
+    lib().orc_build_bc_nodes(facemap, iss, kod, nax, nay, naz, m)
+    return m
+
+
+def el3d(variant, xe, ye, ze, lam3, P, v3, q_in=0.0, robin=0, alpha=0.0, qs=0.0):
+    H = np.zeros(64)
+    Cm = np.zeros(64)
+    F = np.zeros(8)
+    lib().orc_el3d(variant, np.ascontiguousarray(xe, np.float64),
+                   np.ascontiguousarray(ye, np.float64), np.ascontiguousarray(ze, np.float64),
+                   np.ascontiguousarray(lam3, np.float64), P,
+                   np.ascontiguousarray(v3, np.float64), q_in, robin, alpha, qs, H, Cm, F)
+    return H.reshape(8, 8), Cm.reshape(8, 8), F
+
+
+def assemble(p, t_old=None, scheme=None, dt=None):
+    """BB [64E], b [N], diag [N] of problem p (no mask imposition)."""
+    N, E = p.N, p.E
+    bb = np.zeros(64 * E)
+    b = np.zeros(N)
+    d = np.zeros(N)
+    t = None if t_old is None else np.ascontiguousarray(t_old, np.float64)
+    lib().orc_assemble(p.variant, p.scheme if scheme is None else scheme,
+                       p.dt if dt is None else dt, p.nax, p.nay, p.naz,
+                       p.xx, p.yy, p.zz, p.iss, p.kod, np.ascontiguousarray(p.lam, np.float64),
+                       p.P, np.ascontiguousarray(p.vel, np.float64),
+                       np.ascontiguousarray(p.q_in, np.float64), p.alpha, p.qs,
+                       None if t is None else t.ctypes.data_as(C.c_void_p), bb, b, d)
+    return bb, b, d
+
+
+def impose_mask(mask, b, diag, x):
+    """VANADIS.FOR:799-805: b = 0, diag = 1, x = 0 on masked nodes."""
+    m = mask == 1
+    b = b.copy()
+    diag = diag.copy()
+    x = x.copy()
+    b[m] = 0.0
+    diag[m] = 1.0
+    x[m] = 0.0
+    return b, diag, x
+
+
+def matvec_0(x, iss, bb, nax, nay, naz, mask, transpose=False):
+    xin = np.array(x, np.float64, copy=True)
+    y = np.zeros_like(xin)
+    f = lib().orc_mttvec_0 if transpose else lib().orc_matvec_0
+    f(xin, y, iss, bb, nax, nay, naz, mask)
+    return y
+
+
+def dbcg_simple(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
+    x = np.array(x0, np.float64, copy=True)
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    rc = lib().orc_dbcg_simple(len(b), np.ascontiguousarray(b, np.float64), x, tol, itmax,
+                               C.byref(it), C.byref(err), bb, iss,
+                               np.ascontiguousarray(diag, np.float64), nax, nay, naz, mask)
+    return x, it.value, err.value, rc
+
+
+def mass(iss, xx, yy, zz, c, nelem):
+    return lib().orc_integrate_mass_hex8(nelem, iss, xx, yy, zz,
+                                         np.ascontiguousarray(c, np.float64))
+
+
+def ddf(nelem, nnode, iss, bb):
+    return lib().orc_diagonal_dominance_factor(nelem, nnode, iss, bb)
+
+
+def is_dd(nelem, nnode, iss, bb):
+    return bool(lib().orc_is_diagonally_dominant(nelem, nnode, iss, bb))
+
+
+def colors(nax, nay, naz):
+    E = (nax - 1) * (nay - 1) * (naz - 1)
+    c = np.zeros(E, np.int32)
+    lib().orc_generate_hex8_colors(nax, nay, naz, c)
+    el = np.zeros(E, np.int32)
+    off = np.zeros(9, np.int32)
+    cnt = np.zeros(8, np.int32)
+    lib().orc_build_color_element_lists(E, c, el, off, cnt)
+    return c, el, off, cnt
+
+
+# ---- the flang-built reference (oracle/_ref) ------------------------------
+class RefLib:
+    """Direct calls into the reference's own subroutines (VANADIS.FOR for V1,
+    vanadis_full_DR_OpenMP_timing.for for V3), built by `make -C oracle ref`.
+    COMMON blocks are read and written as the globals the compiler emits."""
+
+    def __init__(self, which: str):
+        path = os.path.join(REF_DIR, f"libvanadis_ref_{which}.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        self.L = C.CDLL(path)
+        self.which = which
+        self.L.par_3d_()
+
+    @staticmethod
+    def available(which: str) -> bool:
+        return os.path.exists(os.path.join(REF_DIR, f"libvanadis_ref_{which}.so"))
+
+    def _common(self, name, n, ctype=C.c_double):
+        return (ctype * n).in_dll(self.L, name)
+
+    def set_cfg(self, Q, K, P, V, alpha):
+        cfg = self._common("cfg_", 7)
+        for i, v in enumerate((Q, K, P, V[0], V[1], V[2], alpha)):
+            cfg[i] = v
+
+    def set_sizes(self, nelem, nnode):
+        cal = self._common("cal_el_", 10, C.c_int)
+        cal[0] = nelem
+        cal[1] = nnode
+
+    def set_dt(self, idt: int):
+        self._common("d_time_", 1, C.c_int)[0] = idt
+
+    def el3d(self, kel, xe, ye, ze, kod, nax, nay, naz):
+        """Run the reference el3d for element kel (1-based) with the given
+        node coordinates; returns (H, C, F) row-major like the oracle."""
+        wsp = self._common("wsp_lok_", 24)
+        for j in range(8):
+            wsp[j] = xe[j]
+            wsp[8 + j] = ye[j]
+            wsp[16 + j] = ze[j]
+        t = (C.c_double * 6)()
+        ic = C.c_int(100)
+        k = C.c_int(kel)
+        nx, ny, nz = C.c_int(nax), C.c_int(nay), C.c_int(naz)
+        kod_c = np.ascontiguousarray(kod, np.int32)
+        self.L.el3d_(C.byref(ic), C.byref(k), t, kod_c.ctypes.data_as(_pi), C.byref(nx),
+                     C.byref(ny), C.byref(nz))
+        skl = np.ctypeslib.as_array(self._common("skl_el_", 64 * 11 + 8))
+        hma = skl[0:64].reshape(8, 8, order="F").copy()
+        fmv = skl[128:136].copy()
+        cma_n = skl[136 + 64 * 8: 136 + 64 * 9].reshape(8, 8, order="F").copy()
+        return hma, cma_n, fmv
+
+    def matvec_0(self, x, iss, bb, nax, nay, naz, mask, transpose=False):
+        xin = np.array(x, np.float64, copy=True)
+        y = np.zeros_like(xin)
+        nx, ny, nz = C.c_int(nax), C.c_int(nay), C.c_int(naz)
+        f = self.L.mttvec_0_ if transpose else self.L.matvec_0_
+        f(xin.ctypes.data_as(_pd), y.ctypes.data_as(_pd), iss.ctypes.data_as(_pi),
+          bb.ctypes.data_as(_pd), C.byref(nx), C.byref(ny), C.byref(nz),
+          mask.ctypes.data_as(_pi))
+        return y
+
+    def dbcg_simple(self, b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
+        """The reference CPU BiCG; run on a thread with a large stack because
+        its work vectors are automatic arrays."""
+        out = {}
+
+        def run():
+            x = np.array(x0, np.float64, copy=True)
+            n = C.c_int(len(b))
+            t = C.c_double(tol)
+            im = C.c_int(itmax)
+            it = C.c_int(0)
+            err = C.c_double(0.0)
+            nx, ny, nz = C.c_int(nax), C.c_int(nay), C.c_int(naz)
+            bc = np.ascontiguousarray(b, np.float64)
+            dg = np.ascontiguousarray(diag, np.float64)
+            self.L.dbcg_simple_(C.byref(n), bc.ctypes.data_as(_pd), x.ctypes.data_as(_pd),
+                                C.byref(t), C.byref(im), C.byref(it), C.byref(err),
+                                bb.ctypes.data_as(_pd), iss.ctypes.data_as(_pi),
+                                dg.ctypes.data_as(_pd), C.byref(nx), C.byref(ny),
+                                C.byref(nz), mask.ctypes.data_as(_pi))
+            out["r"] = (x, it.value, err.value)
+
+        old = threading.stack_size()
+        need = max(64 << 20, 6 * 8 * len(b) + (16 << 20))
+        threading.stack_size(min(need, 1 << 30))
+        try:
+            th = threading.Thread(target=run)
+            th.start()
+            th.join()
+        finally:
+            threading.stack_size(old)
+        return out["r"]
+
+    def mass(self, iss, xx, yy, zz, c, nelem):
+        m = C.c_double(0)
+        ne = C.c_int(nelem)
+        cc = np.ascontiguousarray(c, np.float64)
+        self.L.integrate_mass_hex8_(C.byref(ne), iss.ctypes.data_as(_pi),
+                                    xx.ctypes.data_as(_pd), yy.ctypes.data_as(_pd),
+                                    zz.ctypes.data_as(_pd), cc.ctypes.data_as(_pd), C.byref(m))
+        return m.value
+
+    def ddf(self, nelem, nnode, iss, bb):
+        f = self.L.diagonal_dominance_factor_
+        f.restype = C.c_double
+        ne, nn = C.c_int(nelem), C.c_int(nnode)
+        return f(C.byref(ne), C.byref(nn), iss.ctypes.data_as(_pi), bb.ctypes.data_as(_pd))

@@ -1,0 +1,897 @@
+// C ABI of libvanadis_gpu.so: context lifecycle, coefficients, operator
+// apply, RHS/diagonal assembly, FP64 and mixed-precision solves, timing.
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "vgpu_bicg.hpp"
+#include "vgpu_internal.hpp"
+
+namespace vgpu {
+
+static thread_local std::string g_err = "";
+void set_error(const std::string &msg) { g_err = msg; }
+
+namespace {
+
+template <typename TA, typename TB>
+__global__ void k_convert(int64_t n, const TA *a, TB *b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = TB(a[i]);
+}
+
+// out = (mask ? 0 : a*u + c*v)
+__global__ void k_lincomb(int64_t n, const uint8_t *mask, double a,
+                          const double *u, double c, const double *v,
+                          double *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = mask[i] ? 0.0 : a * u[i] + (v ? c * v[i] : 0.0);
+}
+
+// r = b - r; partials of r.r
+__global__ __launch_bounds__(VB) void k_resid(int64_t n, const double *b,
+                                              double *r, double *part) {
+  __shared__ double red[VB / 64];
+  double s = 0;
+  for (int64_t i = blockIdx.x * (int64_t)VB + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * VB) {
+    const double ri = b[i] - r[i];
+    r[i] = ri;
+    s += ri * ri;
+  }
+  block_partial(s, part, red);
+}
+
+// f = float(r * scale)
+__global__ void k_scale_to_f32(int64_t n, const double *r, double scale,
+                               float *f, float *x0) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    f[i] = float(r[i] * scale);
+    x0[i] = 0.0f;
+  }
+}
+
+// x += s * double(d)
+__global__ void k_refine(int64_t n, double *x, double s, const float *d) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] += s * (double)d[i];
+}
+
+__global__ void k_copy(int64_t n, const double *a, double *b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+inline dim3 grid_for(int64_t n, int bs = 256) {
+  int64_t g = (n + bs - 1) / bs;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return dim3((unsigned)g);
+}
+
+}  // namespace
+}  // namespace vgpu
+
+using namespace vgpu;
+
+struct vgpu_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  Grid3 g{};
+  double *xc = nullptr, *yc = nullptr, *zc = nullptr;
+  uint8_t *mask = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr, *lam = nullptr,
+         *tau = nullptr;
+  float *fvx = nullptr, *fvy = nullptr, *fvz = nullptr, *flam = nullptr,
+        *ftau = nullptr;
+  bool have_coeffs = false;
+  double P = 0, alpha = 0, dt = 1;
+  int variant = VGPU_DR_V1, scheme = VGPU_SCHEME_STEADY;
+  double cH = 1, cC = 0;
+  int scatter = VGPU_SCATTER_OWNER, zchunk = 8;
+  double *diag = nullptr;
+  float *fdiag = nullptr;
+  bool diag_valid = false;
+  double *vb = nullptr, *vx_ = nullptr, *vt = nullptr, *vt2 = nullptr,
+         *red = nullptr;
+  float *fb = nullptr, *fx = nullptr;
+  Bicg<double> bicg64;
+  Bicg<float> bicg32;
+  uint64_t version = 1;
+  std::vector<void *> user_allocs;
+
+  OpData<double> op64(double cH_, double cC_) const {
+    OpData<double> o{};
+    o.g = g;
+    o.xc = xc;
+    o.yc = yc;
+    o.zc = zc;
+    o.mask = mask;
+    o.vx = vx;
+    o.vy = vy;
+    o.vz = vz;
+    o.lam = lam;
+    o.tau = tau;
+    o.cH = cH_;
+    o.mu = cH_ * P + cC_;
+    o.ra = cH_ * alpha;
+    o.done = nullptr;
+    return o;
+  }
+  OpData<float> op32() const {
+    OpData<float> o{};
+    o.g = g;
+    o.xc = xc;
+    o.yc = yc;
+    o.zc = zc;
+    o.mask = mask;
+    o.vx = fvx;
+    o.vy = fvy;
+    o.vz = fvz;
+    o.lam = flam;
+    o.tau = ftau;
+    o.cH = cH;
+    o.mu = cH * P + cC;
+    o.ra = cH * alpha;
+    o.done = nullptr;
+    return o;
+  }
+};
+
+namespace {
+
+int scheme_coeffs(int scheme, double dt, double *cH, double *cC) {
+  switch (scheme) {
+    case VGPU_SCHEME_THETA:  // A = 2H + (3/dt) C   (VANADIS.FOR:743-748)
+      if (!(dt > 0)) break;
+      *cH = 2.0;
+      *cC = 3.0 / dt;
+      return VGPU_OK;
+    case VGPU_SCHEME_STEADY:
+      *cH = 1.0;
+      *cC = 0.0;
+      return VGPU_OK;
+    case VGPU_SCHEME_EULER:
+      if (!(dt > 0)) break;
+      *cH = 1.0;
+      *cC = 1.0 / dt;
+      return VGPU_OK;
+    default:
+      break;
+  }
+  set_error("invalid scheme/dt");
+  return VGPU_ERR_ARG;
+}
+
+template <typename T>
+void free_dev(T *&p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+int ensure_diag(vgpu_ctx *c) {
+  if (c->diag_valid) return VGPU_OK;
+  OpData<double> op = c->op64(c->cH, c->cC);
+  int rc = launch_diag(op, c->diag, c->stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_convert<double, float>), grid_for(c->g.N), dim3(256), 0,
+                     c->stream, c->g.N, c->diag, c->fdiag);
+  VGPU_HIP(hipGetLastError());
+  c->diag_valid = true;
+  return VGPU_OK;
+}
+
+double host_sum(const double *dev, int n, hipStream_t s, int *rc) {
+  std::vector<double> h(n);
+  if (hipMemcpyAsync(h.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost,
+                     s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess) {
+    set_error("reduction readback failed");
+    *rc = VGPU_ERR_HIP;
+    return 0;
+  }
+  double t = 0;
+  for (int i = 0; i < n; ++i) t += h[i];
+  return t;
+}
+
+int nparts_for(int64_t n) {
+  return (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));
+}
+
+// FP64 dual-operator callback for the BiCG on the context's operator
+DualOp<double> make_dual64(vgpu_ctx *c, int scatter) {
+  return [c, scatter](const double *p, double *z, const double *pp, double *zz,
+                      double *pa, int *na, const int *done, hipStream_t s) {
+    OpData<double> op = c->op64(c->cH, c->cC);
+    op.done = done;
+    if (scatter == VGPU_SCATTER_OWNER) {
+      *na = owner_num_tiles(c->g, c->zchunk);
+      return launch_apply<double, double>(op, scatter, c->zchunk, p, z, pp, zz,
+                                          pa, s);
+    }
+    int rc = launch_apply<double, double>(op, scatter, c->zchunk, p, z, pp, zz,
+                                          nullptr, s);
+    if (rc) return rc;
+    const int nb = nparts_for(c->g.N);
+    *na = nb;
+    SolveStatus *st = reinterpret_cast<SolveStatus *>(
+        const_cast<int *>(done));  // done is the first member
+    hipLaunchKernelGGL((k_dot<double>), dim3(nb), dim3(VB), 0, s, c->g.N, st, z,
+                       pp, pa);
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  };
+}
+
+DualOp<float> make_dual32(vgpu_ctx *c, int scatter) {
+  return [c, scatter](const float *p, float *z, const float *pp, float *zz,
+                      double *pa, int *na, const int *done, hipStream_t s) {
+    OpData<float> op = c->op32();
+    op.done = done;
+    if (scatter == VGPU_SCATTER_OWNER) {
+      *na = owner_num_tiles(c->g, c->zchunk);
+      return launch_apply<float, float>(op, scatter, c->zchunk, p, z, pp, zz,
+                                        pa, s);
+    }
+    int rc = launch_apply<float, float>(op, scatter, c->zchunk, p, z, pp, zz,
+                                        nullptr, s);
+    if (rc) return rc;
+    const int nb = nparts_for(c->g.N);
+    *na = nb;
+    SolveStatus *st = reinterpret_cast<SolveStatus *>(const_cast<int *>(done));
+    hipLaunchKernelGGL((k_dot<float>), dim3(nb), dim3(VB), 0, s, c->g.N, st, z,
+                       pp, pa);
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  };
+}
+
+int solve_fp64_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
+                   int itmax, int *iters, double *err, int *status) {
+  const int scatter = c->scatter;
+  if (c->bicg64.n != c->g.N) {
+    int rc = c->bicg64.alloc(c->g.N, owner_num_tiles(c->g, 1) + 256);
+    if (rc) return rc;
+  }
+  DualOp<double> dual = make_dual64(c, scatter);
+  SingleOp<double> single = [c, scatter](const double *xx, double *y,
+                                         hipStream_t s) {
+    OpData<double> op = c->op64(c->cH, c->cC);
+    return launch_apply<double, double>(op, scatter, c->zchunk, xx, y, nullptr,
+                                        nullptr, nullptr, s);
+  };
+  const uint64_t key = (c->version << 8) | (uint64_t)(scatter * 16 + c->zchunk % 16);
+  return c->bicg64.solve(key, dual, single, c->mask, b, x, c->diag, rtol, itmax,
+                         scatter != VGPU_SCATTER_OWNER, c->stream, iters, err,
+                         status);
+}
+
+// Mixed precision: FP64 true residual + refinement, FP32 inner BiCG
+int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
+                    int itmax, double inner_rtol, int *iters, double *err,
+                    int *status) {
+  const int64_t N = c->g.N;
+  const int scatter = c->scatter;
+  if (c->bicg32.n != N) {
+    int rc = c->bicg32.alloc(N, owner_num_tiles(c->g, 1) + 256);
+    if (rc) return rc;
+  }
+  if (!c->fb) {
+    VGPU_HIP(hipMalloc(&c->fb, N * sizeof(float)));
+    VGPU_HIP(hipMalloc(&c->fx, N * sizeof(float)));
+  }
+  const int np = nparts_for(N);
+  int rc = VGPU_OK;
+  // ||b||
+  hipLaunchKernelGGL((k_dot<double>), dim3(np), dim3(VB), 0, c->stream, N,
+                     nullptr, b, b, c->red);
+  VGPU_HIP(hipGetLastError());
+  const double bn = std::sqrt(host_sum(c->red, np, c->stream, &rc));
+  if (rc) return rc;
+  DualOp<float> dual = make_dual32(c, scatter);
+  SingleOp<float> single = [c, scatter](const float *xx, float *y,
+                                        hipStream_t s) {
+    OpData<float> op = c->op32();
+    return launch_apply<float, float>(op, scatter, c->zchunk, xx, y, nullptr,
+                                      nullptr, nullptr, s);
+  };
+  const uint64_t key = (c->version << 8) | (uint64_t)(scatter * 16 + c->zchunk % 16) | (1ull << 7);
+  int total = 0, st_inner = 0;
+  double rel = 0;
+  *status = 2;
+  // x masked like the FP64 path
+  hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N, c->mask,
+                     1.0, x, 0.0, nullptr, x);
+  for (int outer = 0; outer < 64; ++outer) {
+    OpData<double> op = c->op64(c->cH, c->cC);
+    rc = launch_apply<double, double>(op, VGPU_SCATTER_OWNER, c->zchunk, x, c->vt,
+                                      nullptr, nullptr, nullptr, c->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_resid, dim3(np), dim3(VB), 0, c->stream, N, b, c->vt,
+                       c->red);
+    VGPU_HIP(hipGetLastError());
+    const double rn = std::sqrt(host_sum(c->red, np, c->stream, &rc));
+    if (rc) return rc;
+    rel = rn / bn;
+    if (rel <= rtol) {
+      *status = 1;
+      break;
+    }
+    if (!(rel == rel)) {
+      *status = 3;
+      break;
+    }
+    if (total >= itmax) break;
+    hipLaunchKernelGGL(k_scale_to_f32, grid_for(N), dim3(256), 0, c->stream, N,
+                       c->vt, 1.0 / rn, c->fb, c->fx);
+    VGPU_HIP(hipGetLastError());
+    int it = 0;
+    double e = 0;
+    rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
+                         inner_rtol, itmax - total, scatter != VGPU_SCATTER_OWNER,
+                         c->stream, &it, &e, &st_inner);
+    if (rc) return rc;
+    total += it;
+    if (st_inner == 3 && it == 0) {
+      *status = 3;
+      break;
+    }
+    hipLaunchKernelGGL(k_refine, grid_for(N), dim3(256), 0, c->stream, N, x, rn,
+                       c->fx);
+    VGPU_HIP(hipGetLastError());
+  }
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  *iters = total;
+  *err = rel;
+  return VGPU_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *vgpu_last_error(void) { return g_err.c_str(); }
+const char *vgpu_version(void) { return "vanadis-mi355x 0.1 (gfx950)"; }
+
+int vgpu_device_count(int *count) {
+  if (!count) return VGPU_ERR_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+  *count = n;
+  return VGPU_OK;
+}
+
+int vgpu_create(int nax, int nay, int naz, const double *xx, const double *yy,
+                const double *zz, const int *iss, const int *bc,
+                vgpu_ctx **out) {
+  if (!out || !xx || !yy || !zz || !bc || nax < 2 || nay < 2 || naz < 2) {
+    set_error("vgpu_create: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  *out = nullptr;
+  const int64_t N = (int64_t)nax * nay * naz;
+  const int64_t E = (int64_t)(nax - 1) * (nay - 1) * (naz - 1);
+  if (E > INT32_MAX / 8) {
+    set_error("vgpu_create: mesh too large for 32-bit connectivity");
+    return VGPU_ERR_ARG;
+  }
+  // axis coordinates and tensor-grid check (siatka, VANADIS.FOR:226-244)
+  std::vector<double> ax(nax), ay(nay), az(naz);
+  for (int i = 0; i < nax; ++i) ax[i] = xx[i];
+  for (int j = 0; j < nay; ++j) ay[j] = yy[(int64_t)nax * j];
+  for (int k = 0; k < naz; ++k) az[k] = zz[(int64_t)nax * nay * k];
+  for (int64_t n = 0; n < N; ++n) {
+    const int i = (int)(n % nax), j = (int)((n / nax) % nay),
+              k = (int)(n / ((int64_t)nax * nay));
+    if (xx[n] != ax[i] || yy[n] != ay[j] || zz[n] != az[k]) {
+      set_error("vgpu_create: node coordinates are not a tensor grid");
+      return VGPU_ERR_MESH;
+    }
+  }
+  for (int i = 0; i + 1 < nax; ++i)
+    if (!(ax[i + 1] > ax[i])) {
+      set_error("vgpu_create: x coordinates not increasing");
+      return VGPU_ERR_MESH;
+    }
+  for (int j = 0; j + 1 < nay; ++j)
+    if (!(ay[j + 1] > ay[j])) {
+      set_error("vgpu_create: y coordinates not increasing");
+      return VGPU_ERR_MESH;
+    }
+  for (int k = 0; k + 1 < naz; ++k)
+    if (!(az[k + 1] > az[k])) {
+      set_error("vgpu_create: z coordinates not increasing");
+      return VGPU_ERR_MESH;
+    }
+  // connectivity must be the structured w_el numbering (VANADIS.FOR:246-276)
+  if (iss) {
+    const int ex = nax - 1, ey = nay - 1;
+    for (int64_t e = 0; e < E; ++e) {
+      const int i = (int)(e % ex), j = (int)((e / ex) % ey),
+                k = (int)(e / ((int64_t)ex * ey));
+      const int64_t n0 = 1 + i + (int64_t)nax * j + (int64_t)nax * nay * k;
+      const int64_t off[8] = {0,
+                              1,
+                              1 + nax,
+                              nax,
+                              (int64_t)nax * nay,
+                              1 + (int64_t)nax * nay,
+                              1 + (int64_t)nax * nay + nax,
+                              (int64_t)nax * nay + nax};
+      for (int c = 0; c < 8; ++c)
+        if (iss[c * E + e] != n0 + off[c]) {
+          set_error("vgpu_create: ISS is not the structured HEX8 connectivity");
+          return VGPU_ERR_MESH;
+        }
+    }
+  }
+  std::vector<uint8_t> m(N);
+  for (int64_t n = 0; n < N; ++n) {
+    if (bc[n] != 0 && bc[n] != 1) {
+      set_error("vgpu_create: bc_dirichlet must be 0/1");
+      return VGPU_ERR_ARG;
+    }
+    m[n] = (uint8_t)bc[n];
+  }
+  vgpu_ctx *c = new vgpu_ctx();
+  auto fail = [&](int rc) {
+    vgpu_destroy(c);
+    return rc;
+  };
+#define CK(call)                                                         \
+  do {                                                                   \
+    hipError_t e_ = (call);                                              \
+    if (e_ != hipSuccess) {                                              \
+      set_error(std::string(#call) + ": " + hipGetErrorString(e_));      \
+      return fail(VGPU_ERR_HIP);                                         \
+    }                                                                    \
+  } while (0)
+  CK(hipGetDevice(&c->device));
+  CK(hipStreamCreate(&c->stream));
+  c->g = Grid3{nax, nay, naz, nax - 1, nay - 1, naz - 1, N, E};
+  CK(hipMalloc(&c->xc, nax * sizeof(double)));
+  CK(hipMalloc(&c->yc, nay * sizeof(double)));
+  CK(hipMalloc(&c->zc, naz * sizeof(double)));
+  CK(hipMalloc(&c->mask, N));
+  CK(hipMemcpy(c->xc, ax.data(), nax * sizeof(double), hipMemcpyHostToDevice));
+  CK(hipMemcpy(c->yc, ay.data(), nay * sizeof(double), hipMemcpyHostToDevice));
+  CK(hipMemcpy(c->zc, az.data(), naz * sizeof(double), hipMemcpyHostToDevice));
+  CK(hipMemcpy(c->mask, m.data(), N, hipMemcpyHostToDevice));
+  for (double **p : {&c->vx, &c->vy, &c->vz, &c->lam, &c->tau})
+    CK(hipMalloc(p, E * sizeof(double)));
+  for (float **p : {&c->fvx, &c->fvy, &c->fvz, &c->flam, &c->ftau})
+    CK(hipMalloc(p, E * sizeof(float)));
+  CK(hipMalloc(&c->diag, N * sizeof(double)));
+  CK(hipMalloc(&c->fdiag, N * sizeof(float)));
+  for (double **p : {&c->vb, &c->vx_, &c->vt, &c->vt2})
+    CK(hipMalloc(p, N * sizeof(double)));
+  CK(hipMalloc(&c->red, 4096 * sizeof(double)));
+#undef CK
+  *out = c;
+  return VGPU_OK;
+}
+
+int vgpu_destroy(vgpu_ctx *c) {
+  if (!c) return VGPU_OK;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  c->bicg64.release();
+  c->bicg32.release();
+  free_dev(c->xc);
+  free_dev(c->yc);
+  free_dev(c->zc);
+  free_dev(c->mask);
+  for (double **p : {&c->vx, &c->vy, &c->vz, &c->lam, &c->tau, &c->diag, &c->vb,
+                     &c->vx_, &c->vt, &c->vt2, &c->red})
+    free_dev(*p);
+  for (float **p : {&c->fvx, &c->fvy, &c->fvz, &c->flam, &c->ftau, &c->fdiag,
+                    &c->fb, &c->fx})
+    free_dev(*p);
+  for (void *p : c->user_allocs) hipFree(p);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return VGPU_OK;
+}
+
+int vgpu_get_sizes(const vgpu_ctx *c, int64_t *n_nodes, int64_t *n_elems) {
+  if (!c || !n_nodes || !n_elems) return VGPU_ERR_ARG;
+  *n_nodes = c->g.N;
+  *n_elems = c->g.E;
+  return VGPU_OK;
+}
+
+int vgpu_set_coeffs(vgpu_ctx *c, const double *lam, const double *vel,
+                    double P, double alpha, int dr_variant, int scheme,
+                    double dt) {
+  if (!c || !lam || !vel) {
+    set_error("vgpu_set_coeffs: null argument");
+    return VGPU_ERR_ARG;
+  }
+  if (dr_variant < VGPU_DR_V1 || dr_variant > VGPU_DR_V3) {
+    set_error("vgpu_set_coeffs: dr_variant must be 1, 2 or 3");
+    return VGPU_ERR_ARG;
+  }
+  double cH, cC;
+  int rc = scheme_coeffs(scheme, dt, &cH, &cC);
+  if (rc) return rc;
+  const int64_t E = c->g.E;
+  VGPU_HIP(hipSetDevice(c->device));
+  VGPU_HIP(hipMemcpyAsync(c->lam, lam, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  VGPU_HIP(hipMemcpyAsync(c->vx, vel, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  VGPU_HIP(hipMemcpyAsync(c->vy, vel + E, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  VGPU_HIP(hipMemcpyAsync(c->vz, vel + 2 * E, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  rc = launch_tau(c->g, c->xc, c->yc, c->zc, c->lam, c->vx, c->vy, c->vz,
+                  dr_variant, c->tau, c->stream);
+  if (rc) return rc;
+  const double *src[5] = {c->vx, c->vy, c->vz, c->lam, c->tau};
+  float *dst[5] = {c->fvx, c->fvy, c->fvz, c->flam, c->ftau};
+  for (int i = 0; i < 5; ++i)
+    hipLaunchKernelGGL((k_convert<double, float>), grid_for(E), dim3(256), 0,
+                       c->stream, E, src[i], dst[i]);
+  VGPU_HIP(hipGetLastError());
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  c->P = P;
+  c->alpha = alpha;
+  c->variant = dr_variant;
+  c->scheme = scheme;
+  c->dt = dt;
+  c->cH = cH;
+  c->cC = cC;
+  c->have_coeffs = true;
+  c->diag_valid = false;
+  c->version++;
+  return VGPU_OK;
+}
+
+int vgpu_set_scheme(vgpu_ctx *c, int scheme, double dt) {
+  if (!c) return VGPU_ERR_ARG;
+  double cH, cC;
+  int rc = scheme_coeffs(scheme, dt, &cH, &cC);
+  if (rc) return rc;
+  c->scheme = scheme;
+  c->dt = dt;
+  c->cH = cH;
+  c->cC = cC;
+  c->diag_valid = false;
+  c->version++;
+  return VGPU_OK;
+}
+
+int vgpu_set_options(vgpu_ctx *c, int scatter, int zchunk) {
+  if (!c || scatter < 0 || scatter > 2 || zchunk < 0 || zchunk > 256) {
+    set_error("vgpu_set_options: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  c->scatter = scatter;
+  if (zchunk > 0) c->zchunk = zchunk;
+  c->version++;
+  return VGPU_OK;
+}
+
+int vgpu_apply_dev(vgpu_ctx *c, int scatter, int transpose, const double *x,
+                   double *y) {
+  if (!c || !x || !y || scatter < 0 || scatter > 2) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) {
+    set_error("vgpu_apply: coefficients not set");
+    return VGPU_ERR_STATE;
+  }
+  OpData<double> op = c->op64(c->cH, c->cC);
+  if (!transpose)
+    return launch_apply<double, double>(op, scatter, c->zchunk, x, y, nullptr,
+                                        nullptr, nullptr, c->stream);
+  // transpose alone: dual apply with the forward result in scratch
+  return launch_apply<double, double>(op, scatter, c->zchunk, x, c->vt2, x, y,
+                                      nullptr, c->stream);
+}
+
+int vgpu_apply(vgpu_ctx *c, int scatter, int precision, int transpose,
+               const double *x, double *y) {
+  if (!c || !x || !y || scatter < 0 || scatter > 2) {
+    set_error("vgpu_apply: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  if (!c->have_coeffs) {
+    set_error("vgpu_apply: coefficients not set");
+    return VGPU_ERR_STATE;
+  }
+  const int64_t N = c->g.N;
+  VGPU_HIP(hipSetDevice(c->device));
+  VGPU_HIP(hipMemcpyAsync(c->vx_, x, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  int rc;
+  if (precision == VGPU_FP64) {
+    rc = vgpu_apply_dev(c, scatter, transpose, c->vx_, c->vb);
+  } else if (precision == VGPU_FP32) {
+    OpData<float> op = c->op32();
+    if (!transpose)
+      rc = launch_apply<float, double>(op, scatter, c->zchunk, c->vx_, c->vb,
+                                       nullptr, nullptr, nullptr, c->stream);
+    else
+      rc = launch_apply<float, double>(op, scatter, c->zchunk, c->vx_, c->vt2,
+                                       c->vx_, c->vb, nullptr, c->stream);
+  } else {
+    set_error("vgpu_apply: precision must be FP64 or FP32");
+    return VGPU_ERR_ARG;
+  }
+  if (rc) return rc;
+  VGPU_HIP(hipMemcpyAsync(y, c->vb, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  return VGPU_OK;
+}
+
+int vgpu_diag(vgpu_ctx *c, double *diag) {
+  if (!c || !diag) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) {
+    set_error("vgpu_diag: coefficients not set");
+    return VGPU_ERR_STATE;
+  }
+  int rc = ensure_diag(c);
+  if (rc) return rc;
+  VGPU_HIP(hipMemcpyAsync(diag, c->diag, c->g.N * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  return VGPU_OK;
+}
+
+int vgpu_assemble_rhs(vgpu_ctx *c, const double *q_in, const double *t_old,
+                      double *b) {
+  if (!c || !q_in || !b) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) {
+    set_error("vgpu_assemble_rhs: coefficients not set");
+    return VGPU_ERR_STATE;
+  }
+  const int64_t N = c->g.N, E = c->g.E;
+  double *q_dev = nullptr;
+  VGPU_HIP(hipMalloc(&q_dev, E * sizeof(double)));
+  int rc = VGPU_OK;
+  do {
+    if (hipMemcpyAsync(q_dev, q_in, E * sizeof(double), hipMemcpyHostToDevice,
+                       c->stream) != hipSuccess) {
+      rc = VGPU_ERR_HIP;
+      break;
+    }
+    OpData<double> op = c->op64(c->cH, c->cC);
+    rc = launch_source(op, q_dev, c->vt, c->stream);  // F
+    if (rc) break;
+    double fa = 1.0, ya = 0.0;
+    bool use_t = t_old != nullptr && c->scheme != VGPU_SCHEME_STEADY;
+    if (use_t) {
+      if (hipMemcpyAsync(c->vx_, t_old, N * sizeof(double),
+                         hipMemcpyHostToDevice, c->stream) != hipSuccess) {
+        rc = VGPU_ERR_HIP;
+        break;
+      }
+      OpData<double> o2;
+      if (c->scheme == VGPU_SCHEME_THETA) {
+        // b = 3F - (H - (3/dt) C) t_old   (VANADIS.FOR:725-741)
+        o2 = c->op64(1.0, -3.0 / c->dt);
+        fa = 3.0;
+        ya = -1.0;
+      } else {
+        // b = F + (C/dt) t_old
+        o2 = c->op64(0.0, 1.0 / c->dt);
+        ya = 1.0;
+      }
+      rc = launch_apply<double, double>(o2, VGPU_SCATTER_OWNER, c->zchunk,
+                                        c->vx_, c->vt2, nullptr, nullptr,
+                                        nullptr, c->stream);
+      if (rc) break;
+    } else if (c->scheme == VGPU_SCHEME_THETA) {
+      fa = 3.0;
+    }
+    hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N,
+                       c->mask, fa, c->vt, ya, use_t ? c->vt2 : nullptr, c->vb);
+    if (hipGetLastError() != hipSuccess ||
+        hipMemcpyAsync(b, c->vb, N * sizeof(double), hipMemcpyDeviceToHost,
+                       c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+      set_error("vgpu_assemble_rhs: HIP failure");
+      rc = VGPU_ERR_HIP;
+    }
+  } while (0);
+  hipFree(q_dev);
+  return rc;
+}
+
+int vgpu_solve_dev(vgpu_ctx *c, int precision, const double *b, double *x,
+                   double rtol, int itmax, double inner_rtol, int *iters,
+                   double *err, double *seconds) {
+  if (!c || !b || !x || !iters || !err || itmax < 0 || !(rtol > 0)) {
+    set_error("vgpu_solve: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  if (!c->have_coeffs) {
+    set_error("vgpu_solve: coefficients not set");
+    return VGPU_ERR_STATE;
+  }
+  VGPU_HIP(hipSetDevice(c->device));
+  int rc = ensure_diag(c);
+  if (rc) return rc;
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  int status = 0;
+  if (precision == VGPU_FP64)
+    rc = solve_fp64_dev(c, b, x, rtol, itmax, iters, err, &status);
+  else if (precision == VGPU_MIXED)
+    rc = solve_mixed_dev(c, b, x, rtol, itmax,
+                         inner_rtol > 0 ? inner_rtol : 1e-4, iters, err, &status);
+  else {
+    set_error("vgpu_solve: precision must be FP64 or MIXED");
+    return VGPU_ERR_ARG;
+  }
+  if (rc) return rc;
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (status == 1) return VGPU_OK;
+  if (status == 3) {
+    set_error("BiCG breakdown (zero or non-finite scalar)");
+    return VGPU_BREAKDOWN;
+  }
+  set_error("BiCG: itmax reached before rtol");
+  return VGPU_NOT_CONVERGED;
+}
+
+int vgpu_solve(vgpu_ctx *c, int precision, const double *b, double *x,
+               double rtol, int itmax, double inner_rtol, int *iters,
+               double *err) {
+  if (!c || !b || !x) return VGPU_ERR_ARG;
+  const int64_t N = c->g.N;
+  VGPU_HIP(hipSetDevice(c->device));
+  VGPU_HIP(hipMemcpyAsync(c->vb, b, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  VGPU_HIP(hipMemcpyAsync(c->vx_, x, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int rc = vgpu_solve_dev(c, precision, c->vb, c->vx_, rtol, itmax,
+                                inner_rtol, iters, err, nullptr);
+  if (rc == VGPU_OK || rc == VGPU_NOT_CONVERGED || rc == VGPU_BREAKDOWN) {
+    VGPU_HIP(hipMemcpyAsync(x, c->vx_, N * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    VGPU_HIP(hipStreamSynchronize(c->stream));
+  }
+  return rc;
+}
+
+int vgpu_true_residual(vgpu_ctx *c, const double *b, const double *x,
+                       double *rel) {
+  if (!c || !b || !x || !rel) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) return VGPU_ERR_STATE;
+  const int64_t N = c->g.N;
+  VGPU_HIP(hipMemcpyAsync(c->vb, b, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  VGPU_HIP(hipMemcpyAsync(c->vx_, x, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  OpData<double> op = c->op64(c->cH, c->cC);
+  int rc = launch_apply<double, double>(op, VGPU_SCATTER_OWNER, c->zchunk,
+                                        c->vx_, c->vt, nullptr, nullptr, nullptr,
+                                        c->stream);
+  if (rc) return rc;
+  const int np = nparts_for(N);
+  hipLaunchKernelGGL(k_resid, dim3(np), dim3(VB), 0, c->stream, N, c->vb, c->vt,
+                     c->red);
+  hipLaunchKernelGGL((k_dot<double>), dim3(np), dim3(VB), 0, c->stream, N,
+                     nullptr, c->vb, c->vb, c->red + np);
+  VGPU_HIP(hipGetLastError());
+  const double rn = host_sum(c->red, np, c->stream, &rc);
+  if (rc) return rc;
+  const double bn = host_sum(c->red + np, np, c->stream, &rc);
+  if (rc) return rc;
+  *rel = std::sqrt(rn / bn);
+  return VGPU_OK;
+}
+
+int vgpu_mass(vgpu_ctx *c, const double *cvec, double *mass) {
+  if (!c || !cvec || !mass) return VGPU_ERR_ARG;
+  const int64_t N = c->g.N;
+  VGPU_HIP(hipMemcpyAsync(c->vx_, cvec, N * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  const int np = 512;
+  int rc = launch_mass(c->g, c->xc, c->yc, c->zc, c->vx_, c->red, np, c->stream);
+  if (rc) return rc;
+  *mass = host_sum(c->red, np, c->stream, &rc);
+  return rc;
+}
+
+int vgpu_dev_alloc(vgpu_ctx *c, int64_t bytes, void **ptr) {
+  if (!c || !ptr || bytes <= 0) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(c->device));
+  VGPU_HIP(hipMalloc(ptr, bytes));
+  c->user_allocs.push_back(*ptr);
+  return VGPU_OK;
+}
+
+int vgpu_dev_free(vgpu_ctx *c, void *ptr) {
+  if (!c || !ptr) return VGPU_ERR_ARG;
+  for (size_t i = 0; i < c->user_allocs.size(); ++i)
+    if (c->user_allocs[i] == ptr) {
+      c->user_allocs.erase(c->user_allocs.begin() + i);
+      VGPU_HIP(hipFree(ptr));
+      return VGPU_OK;
+    }
+  set_error("vgpu_dev_free: pointer not allocated by this context");
+  return VGPU_ERR_ARG;
+}
+
+int vgpu_memcpy_h2d(vgpu_ctx *c, void *dst, const void *src, int64_t bytes) {
+  if (!c || !dst || !src || bytes < 0) return VGPU_ERR_ARG;
+  VGPU_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  return VGPU_OK;
+}
+
+int vgpu_memcpy_d2h(vgpu_ctx *c, void *dst, const void *src, int64_t bytes) {
+  if (!c || !dst || !src || bytes < 0) return VGPU_ERR_ARG;
+  VGPU_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  return VGPU_OK;
+}
+
+int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
+                     int reps, double *avg_ms) {
+  if (!c || !avg_ms || reps < 1 || what < 0 || what > 2) return VGPU_ERR_ARG;
+  if (what != 2 && !c->have_coeffs) return VGPU_ERR_STATE;
+  const int64_t N = c->g.N;
+  VGPU_HIP(hipSetDevice(c->device));
+  // deterministic non-trivial inputs
+  hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N, c->mask,
+                     0.0, c->vb, 0.0, nullptr, c->vt);
+  if (precision == VGPU_FP32 && !c->fb) {
+    VGPU_HIP(hipMalloc(&c->fb, N * sizeof(float)));
+    VGPU_HIP(hipMalloc(&c->fx, N * sizeof(float)));
+  }
+  float *f1 = nullptr, *f2 = nullptr, *f3 = nullptr, *f4 = nullptr;
+  if (precision == VGPU_FP32) {
+    f1 = c->fb;
+    f2 = c->fx;
+    f3 = reinterpret_cast<float *>(c->vt);
+    f4 = reinterpret_cast<float *>(c->vt2);
+  }
+  auto launch = [&]() -> int {
+    if (what == 2) {
+      hipLaunchKernelGGL(k_copy, grid_for(N, 256), dim3(256), 0, c->stream, N,
+                         c->vx_, c->vb);
+      VGPU_HIP(hipGetLastError());
+      return VGPU_OK;
+    }
+    if (precision == VGPU_FP32) {
+      OpData<float> op = c->op32();
+      return launch_apply<float, float>(op, scatter, c->zchunk, f1, f2,
+                                        what == 1 ? f3 : nullptr,
+                                        what == 1 ? f4 : nullptr, c->red,
+                                        c->stream);
+    }
+    OpData<double> op = c->op64(c->cH, c->cC);
+    return launch_apply<double, double>(op, scatter, c->zchunk, c->vx_, c->vb,
+                                        what == 1 ? c->vt : nullptr,
+                                        what == 1 ? c->vt2 : nullptr,
+                                        what == 1 ? c->red : nullptr, c->stream);
+  };
+  // red holds 4096 partials: enough for owner tiles at zc >= 1 up to ~8M nodes
+  if (what == 1 && scatter == VGPU_SCATTER_OWNER &&
+      owner_num_tiles(c->g, c->zchunk) > 4096) {
+    set_error("vgpu_time_kernel: mesh too large for the timing partial buffer");
+    return VGPU_ERR_ARG;
+  }
+  for (int i = 0; i < 3; ++i) {
+    int rc = launch();
+    if (rc) return rc;
+  }
+  hipEvent_t e0, e1;
+  VGPU_HIP(hipEventCreate(&e0));
+  VGPU_HIP(hipEventCreate(&e1));
+  VGPU_HIP(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i) {
+    int rc = launch();
+    if (rc) return rc;
+  }
+  VGPU_HIP(hipEventRecord(e1, c->stream));
+  VGPU_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  VGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  *avg_ms = ms / reps;
+  return VGPU_OK;
+}
+
+}  // extern "C"

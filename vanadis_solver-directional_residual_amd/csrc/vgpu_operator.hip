@@ -1,0 +1,643 @@
+// Matrix-free DR-HEX8 operator kernels for gfx950.
+//
+// Every kernel marches a 32 x 8 element-column tile up the z axis, one element
+// layer per step: the node plane above the layer is staged in LDS (x-fastest,
+// coalesced), each of the 256 threads applies one element (Haar-space closed
+// form, vgpu_internal.hpp), the 8 nodal results go to an LDS staging array and
+// each node sums its (up to) 4 in-plane contributions in a fixed order.
+//
+//  * OWNER   (deterministic, one launch): tiles overlap by one element column,
+//            each tile owns 31 x 7 node columns and writes them once with plain
+//            stores -- no atomics, no pre-zeroing, no read-modify-write.
+//  * COLORED (deterministic): disjoint 32 x 8 x zc tiles in 8 parity colours,
+//            one launch per colour, tile-face nodes read-modify-written.
+//  * ATOMIC  : disjoint tiles in one launch, tile-face nodes added with
+//            hardware atomics (global_atomic_add_f64 / _f32).
+// Reference semantics: y = (I-D) A (I-D) x as matvec_0 / mttvec_0
+// (VANADIS.FOR:2149-2245) with A the element systems of przest3d (723-748).
+#include "vgpu_internal.hpp"
+
+namespace vgpu {
+
+namespace {
+
+template <typename T>
+struct Coef {
+  T lam, vx, vy, vz, tau;
+};
+
+template <typename T>
+__device__ __forceinline__ Coef<T> load_coef(const OpData<T> &op, bool valid,
+                                             int64_t e) {
+  Coef<T> c;
+  if (valid) {
+    c.lam = op.lam[e];
+    c.vx = op.vx[e];
+    c.vy = op.vy[e];
+    c.vz = op.vz[e];
+    c.tau = op.tau[e];
+  } else {
+    c.lam = c.vx = c.vy = c.vz = c.tau = T(0);
+  }
+  return c;
+}
+
+// load one tile node plane value (masked, zero outside the grid)
+template <typename T, typename TIO>
+__device__ __forceinline__ T load_node(const OpData<T> &op, const TIO *x, int gx,
+                                       int gy, int gz) {
+  if (gx < 0 || gy < 0 || gz < 0 || gx >= op.g.nax || gy >= op.g.nay ||
+      gz >= op.g.naz)
+    return T(0);
+  const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * gz);
+  return op.mask[n] ? T(0) : T(x[n]);
+}
+
+// Element apply for one thread: gathers from the two LDS planes, writes the 8
+// results (and 8 transposed results when DUAL) to the staging arrays.
+template <typename T, bool DUAL>
+__device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
+                                          bool robin, T hx, T hy, T hz,
+                                          const Coef<T> &cf, const T *pb,
+                                          const T *pt, const T *pb2,
+                                          const T *pt2, int tx, int ty,
+                                          T *stage, T *stage2, int tid) {
+  T y[8], y2[8];
+  if (valid) {
+    ElemConst<T> k;
+    elem_setup<T>(hx, hy, hz, cf.lam, cf.vx, cf.vy, cf.vz, cf.tau, T(op.cH),
+                  T(op.mu), k);
+    const int i00 = ty * PLX + tx;
+    T x[8], xh[8], yh[8];
+    x[0] = pb[i00];
+    x[1] = pb[i00 + 1];
+    x[2] = pb[i00 + PLX + 1];
+    x[3] = pb[i00 + PLX];
+    x[4] = pt[i00];
+    x[5] = pt[i00 + 1];
+    x[6] = pt[i00 + PLX + 1];
+    x[7] = pt[i00 + PLX];
+    haar_fwd<T>(x, xh);
+    haar_apply<T, false>(k, xh, yh);
+    haar_inv<T>(yh, y);
+    if (robin) robin_add<T, false>(k, hx, cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x, y);
+    if (DUAL) {
+      x[0] = pb2[i00];
+      x[1] = pb2[i00 + 1];
+      x[2] = pb2[i00 + PLX + 1];
+      x[3] = pb2[i00 + PLX];
+      x[4] = pt2[i00];
+      x[5] = pt2[i00 + 1];
+      x[6] = pt2[i00 + PLX + 1];
+      x[7] = pt2[i00 + PLX];
+      haar_fwd<T>(x, xh);
+      haar_apply<T, true>(k, xh, yh);
+      haar_inv<T>(yh, y2);
+      if (robin) robin_add<T, true>(k, hx, cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x, y2);
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) y[c] = y2[c] = T(0);
+  }
+#pragma unroll
+  for (int c = 0; c < 8; ++c) stage[c * TPB + tid] = y[c];
+  if (DUAL) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) stage2[c * TPB + tid] = y2[c];
+  }
+}
+
+// Sum of the four in-plane contributions to tile node (lx, ly); `top` selects
+// the upper face (local nodes 4..7) of the layer.  Elements outside the tile
+// contribute nothing.  Fixed order -> deterministic.
+template <typename T>
+__device__ __forceinline__ double node_sum(const T *stage, int lx, int ly,
+                                           bool top) {
+  const int o = top ? 4 : 0;
+  double s = 0.0;
+  if (lx >= 1 && ly >= 1) s += (double)stage[(o + 2) * TPB + (ly - 1) * TXE + lx - 1];
+  if (lx < TXE && ly >= 1) s += (double)stage[(o + 3) * TPB + (ly - 1) * TXE + lx];
+  if (lx >= 1 && ly < TYE) s += (double)stage[(o + 1) * TPB + ly * TXE + lx - 1];
+  if (lx < TXE && ly < TYE) s += (double)stage[(o + 0) * TPB + ly * TXE + lx];
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// OWNER kernel.  Grid: (ceil(nax/31), ceil(nay/7), ceil(naz/zc)).
+// Tile (bx,by,bz) owns nodes [31bx,31bx+31) x [7by,7by+7) x [zc bz, zc bz+zc)
+// and computes element columns 31bx-1 .. 31bx+30 (x), 7by-1 .. 7by+6 (y),
+// element layers zc bz - 1 .. zc bz + zc - 1.
+// ---------------------------------------------------------------------------
+template <typename T, typename TIO, bool DUAL>
+__global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
+                                                const TIO *__restrict__ x1,
+                                                TIO *__restrict__ y1,
+                                                const TIO *__restrict__ x2,
+                                                TIO *__restrict__ y2,
+                                                double *__restrict__ dot_part) {
+  __shared__ T pl1[2][PLN];
+  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? PLN : 1];
+  __shared__ T st1[8 * TPB];
+  __shared__ T st2[DUAL ? 8 * TPB : 1];
+  __shared__ double red[TPB / 64];
+  if (op.done && *op.done) return;
+
+  const int tid = threadIdx.x;
+  const int tx = tid & (TXE - 1), ty = tid / TXE;
+  const int gx0 = blockIdx.x * OWN_X - 1;  // tile node/element origin
+  const int gy0 = blockIdx.y * OWN_Y - 1;
+  const int kz0 = blockIdx.z * zc;          // first owned node plane
+  const int exg = gx0 + tx, eyg = gy0 + ty;
+  const bool colok = exg >= 0 && exg < op.g.ex && eyg >= 0 && eyg < op.g.ey;
+  const T hx = colok ? T(op.xc[exg + 1] - op.xc[exg]) : T(1);
+  const T hy = colok ? T(op.yc[eyg + 1] - op.yc[eyg]) : T(1);
+  const bool robin = colok && exg == 0 && op.ra != 0.0;
+
+  // plane loader: thread handles plane nodes q = tid and tid + 256
+  const int q0 = tid, q1 = tid + TPB;
+  const int lx0 = q0 % PLX, ly0 = q0 / PLX;
+  const int lx1 = q1 % PLX, ly1 = q1 / PLX;
+  const bool has1 = q1 < PLN;
+
+  // owner node of this thread
+  const int ox = tid & (TXE - 1), oy = tid / TXE;
+  const bool own = ox < OWN_X && oy < OWN_Y;
+  const int ngx = gx0 + 1 + ox, ngy = gy0 + 1 + oy;
+  const bool owng = own && ngx < op.g.nax && ngy < op.g.nay;
+
+  // prologue: plane kz0-1 -> LDS buffer 0, plane kz0 -> registers
+  int kl = kz0 - 1;
+  {
+    pl1[0][q0] = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl);
+    if (has1) pl1[0][q1] = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl);
+    if (DUAL) {
+      pl2[0][q0] = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl);
+      if (has1) pl2[0][q1] = load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl);
+    }
+  }
+  T r0 = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl + 1);
+  T r1 = has1 ? load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl + 1) : T(0);
+  T s0 = T(0), s1 = T(0);
+  if (DUAL) {
+    s0 = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl + 1);
+    s1 = has1 ? load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl + 1) : T(0);
+  }
+  auto evalid = [&](int k) { return colok && k >= 0 && k < op.g.ez; };
+  auto eidx = [&](int k) {
+    return exg + (int64_t)op.g.ex * (eyg + (int64_t)op.g.ey * k);
+  };
+  Coef<T> cn = load_coef<T>(op, evalid(kl), eidx(kl));
+
+  double acc1 = 0.0, acc2 = 0.0, dot = 0.0;
+  for (int it = 0; it <= zc; ++it, ++kl) {
+    const int cb = it & 1, ct = cb ^ 1;
+    pl1[ct][q0] = r0;
+    if (has1) pl1[ct][q1] = r1;
+    if (DUAL) {
+      pl2[ct][q0] = s0;
+      if (has1) pl2[ct][q1] = s1;
+    }
+    const Coef<T> cf = cn;
+    const bool ev = evalid(kl);
+    const T hz = ev ? T(op.zc[kl + 1] - op.zc[kl]) : T(1);
+    if (it < zc) {  // prefetch plane kl+2 and layer kl+1 coefficients
+      r0 = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl + 2);
+      r1 = has1 ? load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl + 2) : T(0);
+      if (DUAL) {
+        s0 = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl + 2);
+        s1 = has1 ? load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl + 2) : T(0);
+      }
+      cn = load_coef<T>(op, evalid(kl + 1), eidx(kl + 1));
+    }
+    __syncthreads();
+    // x2 at this thread's owned node of plane kl, read before the buffer can
+    // be refilled by a thread that runs ahead into the next layer
+    const double xo2 =
+        (DUAL && own) ? (double)pl2[DUAL ? cb : 0][(oy + 1) * PLX + ox + 1] : 0.0;
+    elem_step<T, DUAL>(op, ev, robin, hx, hy, hz, cf, pl1[cb], pl1[ct],
+                       DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
+                       ty, st1, DUAL ? st2 : nullptr, tid);
+    __syncthreads();
+    if (own) {
+      const double b1 = node_sum<T>(st1, ox + 1, oy + 1, false);
+      const double v1 = acc1 + b1;
+      acc1 = node_sum<T>(st1, ox + 1, oy + 1, true);
+      double v2 = 0.0;
+      if (DUAL) {
+        const double b2 = node_sum<T>(st2, ox + 1, oy + 1, false);
+        v2 = acc2 + b2;
+        acc2 = node_sum<T>(st2, ox + 1, oy + 1, true);
+      }
+      if (owng && it >= 1 && kl < op.g.naz) {
+        const int64_t n = ngx + (int64_t)op.g.nax * (ngy + (int64_t)op.g.nay * kl);
+        const bool m = op.mask[n] != 0;
+        const TIO o1 = m ? TIO(0) : TIO(v1);
+        y1[n] = o1;
+        if (DUAL) {
+          y2[n] = m ? TIO(0) : TIO(v2);
+          if (dot_part) dot += (double)o1 * xo2;
+        }
+      }
+    }
+  }
+  if (dot_part) {
+    // deterministic block reduction of y1 . x2 (masked x2 from LDS)
+    for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
+    if ((tid & 63) == 0) red[tid >> 6] = dot;
+    __syncthreads();
+    if (tid == 0) {
+      double s = 0.0;
+      for (int w = 0; w < TPB / 64; ++w) s += red[w];
+      const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
+      dot_part[b] = s;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// COLORED / ATOMIC kernel: disjoint tiles of 32 x 8 elements x zc layers.
+// Nodes strictly inside the tile (x and y) of planes strictly inside the
+// chunk are stored; all other tile nodes are added (RMW for COLORED, atomic
+// for ATOMIC).  The output must be zero on entry.  COLORED runs one launch
+// per colour: tile parity (bx&1) | (by&1)<<1 | (bz&1)<<2 == color.
+// ---------------------------------------------------------------------------
+template <typename T, typename TIO, bool DUAL, bool ATOM>
+__global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
+                                                const TIO *__restrict__ x1,
+                                                TIO *__restrict__ y1,
+                                                const TIO *__restrict__ x2,
+                                                TIO *__restrict__ y2) {
+  __shared__ T pl1[2][PLN];
+  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? PLN : 1];
+  __shared__ T st1[8 * TPB];
+  __shared__ T st2[DUAL ? 8 * TPB : 1];
+
+  if (op.done && *op.done) return;
+  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
+  if (!ATOM) {
+    bx = 2 * bx + (color & 1);
+    by = 2 * by + ((color >> 1) & 1);
+    bz = 2 * bz + ((color >> 2) & 1);
+  }
+  const int kz0 = bz * zc;
+  if (bx * TXE >= op.g.ex || by * TYE >= op.g.ey || kz0 >= op.g.ez) return;
+  const int kz1 = min(kz0 + zc, op.g.ez);  // layers [kz0, kz1)
+
+  const int tid = threadIdx.x;
+  const int tx = tid & (TXE - 1), ty = tid / TXE;
+  const int gx0 = bx * TXE, gy0 = by * TYE;
+  const int exg = gx0 + tx, eyg = gy0 + ty;
+  const bool colok = exg < op.g.ex && eyg < op.g.ey;
+  const T hx = colok ? T(op.xc[exg + 1] - op.xc[exg]) : T(1);
+  const T hy = colok ? T(op.yc[eyg + 1] - op.yc[eyg]) : T(1);
+  const bool robin = colok && exg == 0 && op.ra != 0.0;
+
+  const int q0 = tid, q1 = tid + TPB;
+  const int lx0 = q0 % PLX, ly0 = q0 / PLX;
+  const int lx1 = q1 % PLX, ly1 = q1 / PLX;
+  const bool has1 = q1 < PLN;
+
+  auto eidx = [&](int k) {
+    return exg + (int64_t)op.g.ex * (eyg + (int64_t)op.g.ey * k);
+  };
+  // output of tile node (lx,ly) in plane kz: interior nodes are plain stores
+  auto emit = [&](TIO *y, int lx, int ly, int kz, double v, bool complete_z) {
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    if (gx >= op.g.nax || gy >= op.g.nay || kz >= op.g.naz) return;
+    const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * kz);
+    if (op.mask[n]) return;  // stays zero
+    const bool inner = complete_z && lx >= 1 && lx < TXE && ly >= 1 && ly < TYE &&
+                       gx < op.g.nax - 0 && gy < op.g.nay;
+    if (inner) {
+      y[n] = TIO(v);
+    } else if (ATOM) {
+      atomicAdd(&y[n], TIO(v));
+    } else {
+      y[n] = TIO((double)y[n] + v);
+    }
+  };
+
+  pl1[0][q0] = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kz0);
+  if (has1) pl1[0][q1] = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kz0);
+  if (DUAL) {
+    pl2[0][q0] = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kz0);
+    if (has1) pl2[0][q1] = load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kz0);
+  }
+  double a1[2] = {0.0, 0.0}, a2[2] = {0.0, 0.0};
+  for (int kl = kz0; kl < kz1; ++kl) {
+    const int cb = (kl - kz0) & 1, ct = cb ^ 1;
+    pl1[ct][q0] = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl + 1);
+    if (has1) pl1[ct][q1] = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl + 1);
+    if (DUAL) {
+      pl2[ct][q0] = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl + 1);
+      if (has1) pl2[ct][q1] = load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl + 1);
+    }
+    const Coef<T> cf = load_coef<T>(op, colok, eidx(kl));
+    const T hz = T(op.zc[kl + 1] - op.zc[kl]);
+    __syncthreads();
+    elem_step<T, DUAL>(op, colok, robin, hx, hy, hz, cf, pl1[cb], pl1[ct],
+                       DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
+                       ty, st1, DUAL ? st2 : nullptr, tid);
+    __syncthreads();
+    const bool zin = kl > kz0;  // plane kl has both layers inside the chunk
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int q = h ? q1 : q0;
+      if (h && !has1) break;
+      const int lx = h ? lx1 : lx0, ly = h ? ly1 : ly0;
+      emit(y1, lx, ly, kl, a1[h] + node_sum<T>(st1, lx, ly, false), zin);
+      a1[h] = node_sum<T>(st1, lx, ly, true);
+      if (DUAL) {
+        emit(y2, lx, ly, kl, a2[h] + node_sum<T>(st2, lx, ly, false), zin);
+        a2[h] = node_sum<T>(st2, lx, ly, true);
+      }
+      (void)q;
+    }
+  }
+  // top plane of the chunk: shared with the next chunk
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h && !has1) break;
+    const int lx = h ? lx1 : lx0, ly = h ? ly1 : ly0;
+    emit(y1, lx, ly, kz1, a1[h], false);
+    if (DUAL) emit(y2, lx, ly, kz1, a2[h], false);
+  }
+}
+
+template <typename TIO>
+__global__ void k_zero2(const int *done, TIO *a, TIO *b, int64_t n) {
+  if (done && *done) return;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    a[i] = TIO(0);
+    if (b) b[i] = TIO(0);
+  }
+}
+
+}  // namespace
+
+int owner_num_tiles(const Grid3 &g, int zc) {
+  const int nx = (g.nax + OWN_X - 1) / OWN_X, ny = (g.nay + OWN_Y - 1) / OWN_Y;
+  const int nz = (g.naz + zc - 1) / zc;
+  return nx * ny * nz;
+}
+
+template <typename T, typename TIO>
+int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
+                 TIO *y1, const TIO *x2, TIO *y2, double *dot_part,
+                 hipStream_t s) {
+  if (zc <= 0) zc = 8;
+  const bool dual = x2 != nullptr;
+  const Grid3 &g = op.g;
+  if (scatter == VGPU_SCATTER_OWNER) {
+    dim3 grid((g.nax + OWN_X - 1) / OWN_X, (g.nay + OWN_Y - 1) / OWN_Y,
+              (g.naz + zc - 1) / zc);
+    if (dual)
+      hipLaunchKernelGGL((k_owner<T, TIO, true>), grid, dim3(TPB), 0, s, op, zc,
+                         x1, y1, x2, y2, dot_part);
+    else
+      hipLaunchKernelGGL((k_owner<T, TIO, false>), grid, dim3(TPB), 0, s, op, zc,
+                         x1, y1, x2, y2, nullptr);
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  }
+  // brick modes need zeroed outputs
+  hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
+                     dual ? y2 : nullptr, g.N);
+  const int ntx = (g.ex + TXE - 1) / TXE, nty = (g.ey + TYE - 1) / TYE;
+  const int ntz = (g.ez + zc - 1) / zc;
+  if (scatter == VGPU_SCATTER_ATOMIC) {
+    dim3 grid(ntx, nty, ntz);
+    if (dual)
+      hipLaunchKernelGGL((k_brick<T, TIO, true, true>), grid, dim3(TPB), 0, s,
+                         op, zc, 0, x1, y1, x2, y2);
+    else
+      hipLaunchKernelGGL((k_brick<T, TIO, false, true>), grid, dim3(TPB), 0, s,
+                         op, zc, 0, x1, y1, x2, y2);
+  } else if (scatter == VGPU_SCATTER_COLORED) {
+    dim3 grid((ntx + 1) / 2, (nty + 1) / 2, (ntz + 1) / 2);
+    for (int c = 0; c < 8; ++c) {
+      if (dual)
+        hipLaunchKernelGGL((k_brick<T, TIO, true, false>), grid, dim3(TPB), 0, s,
+                           op, zc, c, x1, y1, x2, y2);
+      else
+        hipLaunchKernelGGL((k_brick<T, TIO, false, false>), grid, dim3(TPB), 0,
+                           s, op, zc, c, x1, y1, x2, y2);
+    }
+  } else {
+    set_error("unknown scatter mode " + std::to_string(scatter));
+    return VGPU_ERR_ARG;
+  }
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
+template int launch_apply<double, double>(const OpData<double> &, int, int,
+                                          const double *, double *,
+                                          const double *, double *, double *,
+                                          hipStream_t);
+template int launch_apply<float, float>(const OpData<float> &, int, int,
+                                        const float *, float *, const float *,
+                                        float *, double *, hipStream_t);
+template int launch_apply<float, double>(const OpData<float> &, int, int,
+                                         const double *, double *,
+                                         const double *, double *, double *,
+                                         hipStream_t);
+
+// ---------------------------------------------------------------------------
+// Per-element tau of the DR test function (closed form of
+// gauss_point_hex8DR for a box: J = diag(h/2)).
+//   V1 (VANADIS.FOR:1952-2006): h_s = 2/|J^-1 e|, lambda_s = lam e.e,
+//       beta = series / 1-1/Pe / coth-1/Pe.
+//   V2 (gauss_point_hex8DR.for:106-157): h_s as V1, lambda_s = lam,
+//       coth from exp.
+//   V3 (omp_timing.for:1650-1688): h_s = 2/(sum |v_k|/h_k + 1e-14),
+//       lambda_s = lam, coth from exp.
+// ---------------------------------------------------------------------------
+__global__ void k_tau(Grid3 g, const double *xc, const double *yc,
+                      const double *zc, const double *lam, const double *vx,
+                      const double *vy, const double *vz, int variant,
+                      double *tau) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(e % g.ex);
+    const int iy = (int)((e / g.ex) % g.ey);
+    const int iz = (int)(e / ((int64_t)g.ex * g.ey));
+    const double hx = xc[ix + 1] - xc[ix], hy = yc[iy + 1] - yc[iy],
+                 hz = zc[iz + 1] - zc[iz];
+    const double v1 = vx[e], v2 = vy[e], v3 = vz[e], l = lam[e];
+    const double eps = 1.0e-14;
+    const double vmag = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
+    double hs, ls;
+    if (variant == VGPU_DR_V3) {
+      hs = vmag > eps ? 2.0 / (fabs(v1) / hx + fabs(v2) / hy + fabs(v3) / hz + eps) : 0.0;
+      ls = l;
+    } else {
+      double ex = 0, ey = 0, ez = 0;
+      if (vmag > eps) {
+        ex = v1 / vmag;
+        ey = v2 / vmag;
+        ez = v3 / vmag;
+      }
+      const double gx = 2.0 / hx * ex, gy = 2.0 / hy * ey, gz = 2.0 / hz * ez;
+      const double gn = sqrt(gx * gx + gy * gy + gz * gz);
+      hs = gn > eps ? 2.0 / gn : 0.0;
+      ls = (variant == VGPU_DR_V1) ? l * ex * ex + l * ey * ey + l * ez * ez : l;
+    }
+    double Pe = 0.0;
+    if (vmag > eps && ls > eps) Pe = vmag * hs / (2.0 * ls);
+    double t = 0.0;
+    if (Pe > 1.0e-12 && vmag > eps) {
+      double beta;
+      if (variant == VGPU_DR_V1) {
+        if (Pe < 1.0e-3)
+          beta = Pe / 3.0 - (Pe * Pe * Pe) / 45.0 + 2.0 * ((Pe * Pe) * (Pe * Pe) * Pe) / 945.0;
+        else if (Pe > 50.0)
+          beta = 1.0 - 1.0 / Pe;
+        else
+          beta = 1.0 / tanh(Pe) - 1.0 / Pe;
+      } else {
+        const double ep = exp(Pe), en = exp(-Pe);
+        beta = (0.5 * (ep + en)) / (0.5 * (ep - en)) - 1.0 / Pe;
+      }
+      t = 0.5 * hs / vmag * beta;
+    }
+    tau[e] = t;
+  }
+}
+
+int launch_tau(const Grid3 &g, const double *xc, const double *yc,
+               const double *zc, const double *lam, const double *vx,
+               const double *vy, const double *vz, int variant, double *tau,
+               hipStream_t s) {
+  hipLaunchKernelGGL(k_tau, dim3(2048), dim3(256), 0, s, g, xc, yc, zc, lam, vx,
+                     vy, vz, variant, tau);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Jacobi diagonal: node-centric, deterministic.  diag(n) = sum over the (up
+// to 8) elements around n of A_e(a,a), a = local index of n; 1 on the mask
+// (VANADIS.FOR:761-769, 799-805).  A_e(a,a) via the element apply of e_a.
+// ---------------------------------------------------------------------------
+__device__ double elem_entry(const OpData<double> &op, int ix, int iy, int iz,
+                             int a) {
+  const int64_t e = ix + (int64_t)op.g.ex * (iy + (int64_t)op.g.ey * iz);
+  const double hx = op.xc[ix + 1] - op.xc[ix], hy = op.yc[iy + 1] - op.yc[iy],
+               hz = op.zc[iz + 1] - op.zc[iz];
+  ElemConst<double> k;
+  elem_setup<double>(hx, hy, hz, op.lam[e], op.vx[e], op.vy[e], op.vz[e],
+                     op.tau[e], op.cH, op.mu, k);
+  double x[8], xh[8], yh[8], y[8];
+  for (int c = 0; c < 8; ++c) x[c] = (c == a) ? 1.0 : 0.0;
+  haar_fwd<double>(x, xh);
+  haar_apply<double, false>(k, xh, yh);
+  haar_inv<double>(yh, y);
+  if (ix == 0 && op.ra != 0.0)
+    robin_add<double, false>(k, hx, op.tau[e], op.vx[e], op.vy[e], op.vz[e],
+                             op.ra, x, y);
+  return y[a];
+}
+
+__global__ void k_diag(OpData<double> op, double *diag) {
+  // local index of a node at corner offset (cx,cy,cz) of its element
+  const int LOC[8] = {0, 1, 3, 2, 4, 5, 7, 6};  // index cx + 2cy + 4cz -> local
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < op.g.N;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    if (op.mask[n]) {
+      diag[n] = 1.0;
+      continue;
+    }
+    const int gx = (int)(n % op.g.nax);
+    const int gy = (int)((n / op.g.nax) % op.g.nay);
+    const int gz = (int)(n / ((int64_t)op.g.nax * op.g.nay));
+    double s = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+      // element whose corner (cx,cy,cz) is this node
+      const int ix = gx - cx, iy = gy - cy, iz = gz - cz;
+      if (ix < 0 || iy < 0 || iz < 0 || ix >= op.g.ex || iy >= op.g.ey || iz >= op.g.ez)
+        continue;
+      s += elem_entry(op, ix, iy, iz, LOC[c]);
+    }
+    diag[n] = s;
+  }
+}
+
+int launch_diag(const OpData<double> &op, double *diag, hipStream_t s) {
+  hipLaunchKernelGGL(k_diag, dim3(2048), dim3(256), 0, s, op, diag);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
+// Element source F_a = q_in [1/8 + tau sum_k v_k s_k(a) / (4 h_k)]
+// (el3d fmv, VANADIS.FOR:525, 551-553: qv = q_in/vol, int w_a), node-centric.
+__global__ void k_source(OpData<double> op, const double *q_in, double *f) {
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < op.g.N;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const int gx = (int)(n % op.g.nax);
+    const int gy = (int)((n / op.g.nax) % op.g.nay);
+    const int gz = (int)(n / ((int64_t)op.g.nax * op.g.nay));
+    double s = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const int cx = c & 1, cy = (c >> 1) & 1, cz = c >> 2;
+      const int ix = gx - cx, iy = gy - cy, iz = gz - cz;
+      if (ix < 0 || iy < 0 || iz < 0 || ix >= op.g.ex || iy >= op.g.ey || iz >= op.g.ez)
+        continue;
+      const int64_t e = ix + (int64_t)op.g.ex * (iy + (int64_t)op.g.ey * iz);
+      const double q = q_in[e];
+      if (q == 0.0) continue;
+      const double hx = op.xc[ix + 1] - op.xc[ix], hy = op.yc[iy + 1] - op.yc[iy],
+                   hz = op.zc[iz + 1] - op.zc[iz];
+      const double sx = cx ? 1.0 : -1.0, sy = cy ? 1.0 : -1.0, sz = cz ? 1.0 : -1.0;
+      const double t = op.tau[e];
+      s += q * (0.125 + t * (op.vx[e] * sx / (4.0 * hx) + op.vy[e] * sy / (4.0 * hy) +
+                             op.vz[e] * sz / (4.0 * hz)));
+    }
+    f[n] = s;
+  }
+}
+
+int launch_source(const OpData<double> &op, const double *q_in, double *f,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(k_source, dim3(2048), dim3(256), 0, s, op, q_in, f);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
+// Mass integral: for trilinear c on a box the 2x2x2 rule is exact, so
+// int_e c = vol_e * mean of the 8 nodal values (integrate_mass_hex8,
+// VANADIS.FOR:2493-2561).  Per-block partials, summed on the host in order.
+__global__ void k_mass(Grid3 g, const double *xc, const double *yc,
+                       const double *zc, const double *c, double *partials) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int ix = (int)(e % g.ex);
+    const int iy = (int)((e / g.ex) % g.ey);
+    const int iz = (int)(e / ((int64_t)g.ex * g.ey));
+    const int64_t n0 = ix + (int64_t)g.nax * (iy + (int64_t)g.nay * iz);
+    const int64_t px = 1, py = g.nax, pz = (int64_t)g.nax * g.nay;
+    const double sum = c[n0] + c[n0 + px] + c[n0 + px + py] + c[n0 + py] + c[n0 + pz] +
+                       c[n0 + pz + px] + c[n0 + pz + px + py] + c[n0 + pz + py];
+    const double vol = (xc[ix + 1] - xc[ix]) * (yc[iy + 1] - yc[iy]) * (zc[iz + 1] - zc[iz]);
+    s += 0.125 * sum * vol;
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) partials[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+int launch_mass(const Grid3 &g, const double *xc, const double *yc,
+                const double *zc, const double *c, double *partials, int nparts,
+                hipStream_t s) {
+  hipLaunchKernelGGL(k_mass, dim3(nparts), dim3(256), 0, s, g, xc, yc, zc, c,
+                     partials);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
+}  // namespace vgpu
