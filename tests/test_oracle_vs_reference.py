@@ -1,0 +1,87 @@
+"""The oracle against the reference's own Fortran, built by `make -C oracle ref`
+with flang from the sources (V1: VANADIS.FOR, V3: the OpenMP timing build).
+Skipped when oracle/_ref has not been built (e.g. no reference tree)."""
+import numpy as np
+import pytest
+
+
+def ref(po, which):
+    if not po.RefLib.available(which):
+        pytest.skip(f"oracle/_ref/libvanadis_ref_{which}.so not built")
+    return po.RefLib(which)
+
+
+@pytest.fixture(scope="module")
+def default_case(pkg):
+    return pkg.problem.reference_default_case()
+
+
+@pytest.mark.parametrize("which,variant,facemap", [("v1", 1, 1), ("v3", 3, 3)])
+def test_el3d_every_element_default_mesh(pkg, po, which, variant, facemap):
+    """hma, cma_n, fmv of el3d for all 2166 elements of the shipped case."""
+    R = ref(po, which)
+    p = pkg.problem.reference_default_case(variant=variant, facemap=facemap)
+    R.set_cfg(1.0, 40.0, 7e-4, (0.0, -5.0, 0.0), 0.01)
+    R.set_sizes(p.E, p.N)
+    R.set_dt(100)
+    iss = p.iss.reshape(8, p.E) - 1
+    kod = p.kod
+    worst = 0.0
+    for e in range(p.E):
+        nd = iss[:, e]
+        Hr, Cr, Fr = R.el3d(e + 1, p.xx[nd], p.yy[nd], p.zz[nd], kod, p.nax, p.nay, p.naz)
+        q = 1.0 if e + 1 == 1083 else 0.0
+        Ho, Co, Fo = po.el3d(variant, p.xx[nd], p.yy[nd], p.zz[nd], np.full(3, 40.0), 7e-4,
+                             np.array([0.0, -5.0, 0.0]), q, int(kod[2 * p.E + e] == 3), 0.01,
+                             0.0)
+        worst = max(worst, np.abs(Ho - Hr).max() / np.abs(Hr).max(),
+                    np.abs(Co - Cr).max() / np.abs(Cr).max())
+        if q:
+            assert np.abs(Fo - Fr).max() <= 1e-15 * np.abs(Fr).max()
+    assert worst <= 1e-15, worst
+
+
+@pytest.mark.parametrize("which", ["v1", "v3"])
+def test_matvec_bicg_mass_ddf(pkg, po, which):
+    """matvec_0/mttvec_0, dbcg_simple, integrate_mass_hex8 and the DDF of the
+    reference against the oracle on the same assembled inputs."""
+    R = ref(po, which)
+    variant, facemap = (1, 1) if which == "v1" else (3, 3)
+    p = pkg.problem.reference_default_case(variant=variant, facemap=facemap)
+    R.set_sizes(p.E, p.N)
+    bb, b, d = po.assemble(p)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(p.N)
+    for tr in (False, True):
+        yo = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+        yr = R.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+        assert np.array_equal(yo, yr)
+    b, d, x0 = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    xo, ito, erro, _ = po.dbcg_simple(b, x0, 1e-8, 1000, bb, p.iss, d, p.nax, p.nay, p.naz,
+                                      p.mask)
+    xr, itr, errr = R.dbcg_simple(b, x0, 1e-8, 1000, bb, p.iss, d, p.nax, p.nay, p.naz, p.mask)
+    assert ito == itr
+    assert np.linalg.norm(xo - xr) <= 1e-12 * np.linalg.norm(xr)
+    assert po.mass(p.iss, p.xx, p.yy, p.zz, xo, p.E) == pytest.approx(
+        R.mass(p.iss, p.xx, p.yy, p.zz, xo, p.E), rel=1e-15)
+    assert po.ddf(p.E, p.N, p.iss, bb) == R.ddf(p.E, p.N, p.iss, bb)
+
+
+def test_el3d_unit_box_fields(pkg, po):
+    """Per-element velocity/diffusivity through the reference el3d (cfg set per
+    element) on a unit-box slab: the same element routine the synthetic
+    workloads use."""
+    R = ref(po, "v1")
+    p = pkg.problem.unit_box(6, 5, 4, vel_seed=0, lam=1e-3)
+    R.set_sizes(p.E, p.N)
+    iss = p.iss.reshape(8, p.E) - 1
+    worst = 0.0
+    for e in range(0, p.E, 7):
+        nd = iss[:, e]
+        v = p.vel[[e, p.E + e, 2 * p.E + e]]
+        R.set_cfg(0.0, p.lam[e], 0.0, v, 0.0)
+        Hr, Cr, _ = R.el3d(e + 1, p.xx[nd], p.yy[nd], p.zz[nd], p.kod, p.nax, p.nay, p.naz)
+        Ho, Co, _ = po.el3d(1, p.xx[nd], p.yy[nd], p.zz[nd], np.full(3, p.lam[e]), 0.0, v, 0.0,
+                            0, 0.0, 0.0)
+        worst = max(worst, np.abs(Ho - Hr).max() / np.abs(Hr).max())
+    assert worst <= 1e-15, worst

@@ -1,0 +1,262 @@
+"""ctypes binding of libvanadis_gpu.so (C ABI declared in include/vanadis_gpu.h).
+
+The library is built in place next to this file by the package Makefile.  If it
+is missing or fails to load, every entry point raises: there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libvanadis_gpu.so")
+
+# status codes (vanadis_gpu.h)
+OK, ERR_ARG, ERR_HIP, ERR_MESH, ERR_STATE, NOT_CONVERGED, BREAKDOWN, ERR_NOMEM = range(8)
+SCATTER_OWNER, SCATTER_COLORED, SCATTER_ATOMIC = 0, 1, 2
+DR_V1, DR_V2, DR_V3 = 1, 2, 3
+SCHEME_THETA, SCHEME_STEADY, SCHEME_EULER = 0, 1, 2
+FP64, FP32, MIXED = 0, 1, 2
+
+_SCATTER_NAMES = {"owner": SCATTER_OWNER, "colored": SCATTER_COLORED, "atomic": SCATTER_ATOMIC}
+
+# every symbol include/vanadis_gpu.h declares
+EXPORTED = (
+    "vgpu_last_error", "vgpu_version", "vgpu_device_count", "kernel_dbcg_simple_",
+    "vgpu_legacy_set_scatter", "vgpu_legacy_release", "vgpu_create", "vgpu_destroy",
+    "vgpu_get_sizes", "vgpu_set_coeffs", "vgpu_set_scheme", "vgpu_set_options", "vgpu_apply",
+    "vgpu_apply_dev", "vgpu_diag", "vgpu_assemble_rhs", "vgpu_solve", "vgpu_solve_dev",
+    "vgpu_true_residual", "vgpu_mass", "vgpu_dev_alloc", "vgpu_dev_free", "vgpu_memcpy_h2d",
+    "vgpu_memcpy_d2h", "vgpu_time_kernel",
+)
+
+
+class VgpuError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"vgpu error {code}: {msg}")
+        self.code = code
+
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+_vp = C.c_void_p
+
+_lib = None
+
+
+def scatter_code(s) -> int:
+    return _SCATTER_NAMES[s] if isinstance(s, str) else int(s)
+
+
+def load():
+    """Load the HIP library (raises if it is absent: no fallback exists)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} not built: run __graft_entry__.build() "
+                          "(hipcc --offload-arch=gfx950); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    L.vgpu_last_error.restype = C.c_char_p
+    L.vgpu_version.restype = C.c_char_p
+    L.vgpu_device_count.argtypes = [_pi]
+    L.kernel_dbcg_simple_.argtypes = [_pi, _dp, _dp, _pd, _pi, _pi, _pd, _dp, _ip, _dp, _pi,
+                                      _pi, _pi, _ip]
+    L.kernel_dbcg_simple_.restype = None
+    L.vgpu_legacy_set_scatter.argtypes = [C.c_int]
+    L.vgpu_create.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _vp, _ip,
+                              C.POINTER(_vp)]
+    L.vgpu_destroy.argtypes = [_vp]
+    L.vgpu_get_sizes.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.vgpu_set_coeffs.argtypes = [_vp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
+                                  C.c_double]
+    L.vgpu_set_scheme.argtypes = [_vp, C.c_int, C.c_double]
+    L.vgpu_set_options.argtypes = [_vp, C.c_int, C.c_int]
+    L.vgpu_apply.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+    L.vgpu_apply_dev.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp]
+    L.vgpu_diag.argtypes = [_vp, _dp]
+    L.vgpu_assemble_rhs.argtypes = [_vp, _dp, _vp, _dp]
+    L.vgpu_solve.argtypes = [_vp, C.c_int, _dp, _dp, C.c_double, C.c_int, C.c_double, _pi, _pd]
+    L.vgpu_solve_dev.argtypes = [_vp, C.c_int, _vp, _vp, C.c_double, C.c_int, C.c_double,
+                                 _pi, _pd, _pd]
+    L.vgpu_true_residual.argtypes = [_vp, _dp, _dp, _pd]
+    L.vgpu_mass.argtypes = [_vp, _dp, _pd]
+    L.vgpu_dev_alloc.argtypes = [_vp, C.c_int64, C.POINTER(_vp)]
+    L.vgpu_dev_free.argtypes = [_vp, _vp]
+    L.vgpu_memcpy_h2d.argtypes = [_vp, _vp, _vp, C.c_int64]
+    L.vgpu_memcpy_d2h.argtypes = [_vp, _vp, _vp, C.c_int64]
+    L.vgpu_time_kernel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _pd]
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str = "") -> int:
+    if rc not in (OK,):
+        msg = load().vgpu_last_error().decode(errors="replace")
+        raise VgpuError(rc, f"{what}: {msg}" if what else msg)
+    return rc
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+# --------------------------------------------------------------------------
+# reference entry point: CALL kernel_dbcg_simple(NON, FMV3, TNW2, TOL, IL_I,
+# ITER, ERR, BB, ISS, DIAG, NAX, NAY, NAZ, bc_dirichlet)  (VANADIS.FOR:809-810)
+# --------------------------------------------------------------------------
+def kernel_dbcg_simple(non, fmv3, tnw2, tol, il_i, bb, iss, diag, nax, nay, naz,
+                       bc_dirichlet, scatter="colored"):
+    """Legacy drop-in: returns (x, iter, err).  tnw2 is not modified (the
+    Fortran host passes it in/out; here the updated copy is returned)."""
+    L = load()
+    check(L.vgpu_legacy_set_scatter(scatter_code(scatter)), "legacy scatter")
+    x = np.array(tnw2, dtype=np.float64, copy=True)
+    n = C.c_int(int(non))
+    t = C.c_double(float(tol))
+    im = C.c_int(int(il_i))
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    nx, ny, nz = C.c_int(nax), C.c_int(nay), C.c_int(naz)
+    L.kernel_dbcg_simple_(C.byref(n), _f64(fmv3), x, C.byref(t), C.byref(im), C.byref(it),
+                          C.byref(err), _f64(bb), _i32(iss), _f64(diag), C.byref(nx),
+                          C.byref(ny), C.byref(nz), _i32(bc_dirichlet))
+    return x, it.value, err.value
+
+
+class Context:
+    """Device-resident matrix-free operator and solver for one structured mesh
+    (the vgpu_* context API)."""
+
+    def __init__(self, nax, nay, naz, xx, yy, zz, iss, bc):
+        L = load()
+        h = _vp()
+        iss_arr = None if iss is None else _i32(iss)
+        check(L.vgpu_create(nax, nay, naz, _f64(xx), _f64(yy), _f64(zz),
+                            None if iss_arr is None else iss_arr.ctypes.data_as(_vp),
+                            _i32(bc), C.byref(h)), "vgpu_create")
+        self._keep = iss_arr
+        self.h = h
+        self.nax, self.nay, self.naz = nax, nay, naz
+        self.N = nax * nay * naz
+        self.E = (nax - 1) * (nay - 1) * (naz - 1)
+
+    @classmethod
+    def from_problem(cls, p, coeffs=True):
+        c = cls(p.nax, p.nay, p.naz, p.xx, p.yy, p.zz, p.iss, p.mask)
+        if coeffs:
+            c.set_coeffs(p.lam, p.vel, p.P, p.alpha, p.variant, p.scheme, p.dt)
+        return c
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().vgpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_coeffs(self, lam, vel, P=0.0, alpha=0.0, variant=1, scheme=1, dt=1.0):
+        check(load().vgpu_set_coeffs(self.h, _f64(lam), _f64(vel), float(P), float(alpha),
+                                     int(variant), int(scheme), float(dt)), "set_coeffs")
+
+    def set_scheme(self, scheme, dt):
+        check(load().vgpu_set_scheme(self.h, int(scheme), float(dt)), "set_scheme")
+
+    def set_options(self, scatter="owner", zchunk=0):
+        check(load().vgpu_set_options(self.h, scatter_code(scatter), int(zchunk)), "set_options")
+
+    def apply(self, x, scatter="owner", precision=FP64, transpose=False):
+        y = np.zeros(self.N)
+        check(load().vgpu_apply(self.h, scatter_code(scatter), int(precision), int(transpose),
+                                _f64(x), y), "apply")
+        return y
+
+    def diag(self):
+        d = np.zeros(self.N)
+        check(load().vgpu_diag(self.h, d), "diag")
+        return d
+
+    def assemble_rhs(self, q_in, t_old=None):
+        b = np.zeros(self.N)
+        t = None if t_old is None else _f64(t_old)
+        check(load().vgpu_assemble_rhs(self.h, _f64(q_in),
+                                       None if t is None else t.ctypes.data_as(_vp), b),
+              "assemble_rhs")
+        return b
+
+    def solve(self, b, x0=None, rtol=1e-8, itmax=20000, precision=FP64, inner_rtol=1e-4,
+              raise_on_fail=True):
+        """Returns (x, iters, err, status code)."""
+        x = np.zeros(self.N) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        it = C.c_int(0)
+        err = C.c_double(0.0)
+        rc = load().vgpu_solve(self.h, int(precision), _f64(b), x, float(rtol), int(itmax),
+                               float(inner_rtol), C.byref(it), C.byref(err))
+        if raise_on_fail and rc != OK:
+            check(rc, "solve")
+        return x, it.value, err.value, rc
+
+    def true_residual(self, b, x):
+        r = C.c_double(0)
+        check(load().vgpu_true_residual(self.h, _f64(b), _f64(x), C.byref(r)), "true_residual")
+        return r.value
+
+    def mass(self, c):
+        m = C.c_double(0)
+        check(load().vgpu_mass(self.h, _f64(c), C.byref(m)), "mass")
+        return m.value
+
+    # ---- device-resident helpers (bench) ----
+    def dev_alloc(self, nbytes):
+        p = _vp()
+        check(load().vgpu_dev_alloc(self.h, int(nbytes), C.byref(p)), "dev_alloc")
+        return p
+
+    def h2d(self, dst, arr):
+        a = np.ascontiguousarray(arr)
+        check(load().vgpu_memcpy_h2d(self.h, dst, a.ctypes.data_as(_vp), a.nbytes), "h2d")
+
+    def d2h(self, src, n, dtype=np.float64):
+        a = np.zeros(n, dtype=dtype)
+        check(load().vgpu_memcpy_d2h(self.h, a.ctypes.data_as(_vp), src, a.nbytes), "d2h")
+        return a
+
+    def solve_dev(self, b_dev, x_dev, rtol=1e-8, itmax=20000, precision=FP64,
+                  inner_rtol=1e-4):
+        it = C.c_int(0)
+        err = C.c_double(0.0)
+        sec = C.c_double(0.0)
+        rc = load().vgpu_solve_dev(self.h, int(precision), b_dev, x_dev, float(rtol),
+                                   int(itmax), float(inner_rtol), C.byref(it), C.byref(err),
+                                   C.byref(sec))
+        return it.value, err.value, sec.value, rc
+
+    def time_kernel(self, what=0, scatter="owner", precision=FP64, reps=20):
+        ms = C.c_double(0)
+        check(load().vgpu_time_kernel(self.h, int(what), scatter_code(scatter), int(precision),
+                                      int(reps), C.byref(ms)), "time_kernel")
+        return ms.value
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    load().vgpu_device_count(C.byref(n))
+    return n.value

@@ -1,0 +1,85 @@
+! ISO_C_BINDING interface to libvanadis_gpu.so for a Fortran host.
+!
+! The legacy entry kernel_dbcg_simple_ needs no interface: the host's
+! implicit-external CALL kernel_dbcg_simple(...) (VANADIS.FOR:809-810) already
+! binds to it with every argument by reference.  This module declares the
+! context API of include/vanadis_gpu.h for hosts that keep the operator and
+! the solver state on the device across time steps.
+module vanadis_gpu
+  use iso_c_binding
+  implicit none
+
+  integer(c_int), parameter :: VGPU_OK = 0
+  integer(c_int), parameter :: VGPU_NOT_CONVERGED = 5
+  integer(c_int), parameter :: VGPU_SCATTER_OWNER = 0, VGPU_SCATTER_COLORED = 1, &
+                               VGPU_SCATTER_ATOMIC = 2
+  integer(c_int), parameter :: VGPU_DR_V1 = 1, VGPU_DR_V2 = 2, VGPU_DR_V3 = 3
+  integer(c_int), parameter :: VGPU_SCHEME_THETA = 0, VGPU_SCHEME_STEADY = 1, &
+                               VGPU_SCHEME_EULER = 2
+  integer(c_int), parameter :: VGPU_FP64 = 0, VGPU_FP32 = 1, VGPU_MIXED = 2
+
+  interface
+    integer(c_int) function vgpu_create(nax, nay, naz, xx, yy, zz, iss, bc, ctx) &
+        bind(C, name="vgpu_create")
+      import :: c_int, c_double, c_ptr
+      integer(c_int), value :: nax, nay, naz
+      real(c_double), intent(in) :: xx(*), yy(*), zz(*)
+      integer(c_int), intent(in) :: iss(*), bc(*)
+      type(c_ptr), intent(out) :: ctx
+    end function
+
+    integer(c_int) function vgpu_destroy(ctx) bind(C, name="vgpu_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+
+    integer(c_int) function vgpu_set_coeffs(ctx, lam, vel, p, alpha, variant, scheme, dt) &
+        bind(C, name="vgpu_set_coeffs")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: lam(*), vel(*)
+      real(c_double), value :: p, alpha, dt
+      integer(c_int), value :: variant, scheme
+    end function
+
+    integer(c_int) function vgpu_set_options(ctx, scatter, zchunk) &
+        bind(C, name="vgpu_set_options")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: scatter, zchunk
+    end function
+
+    integer(c_int) function vgpu_assemble_rhs(ctx, q_in, t_old, b) &
+        bind(C, name="vgpu_assemble_rhs")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: q_in(*), t_old(*)
+      real(c_double), intent(out) :: b(*)
+    end function
+
+    integer(c_int) function vgpu_diag(ctx, diag) bind(C, name="vgpu_diag")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: diag(*)
+    end function
+
+    integer(c_int) function vgpu_solve(ctx, precision, b, x, rtol, itmax, inner_rtol, &
+                                       iters, err) bind(C, name="vgpu_solve")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, itmax
+      real(c_double), intent(in) :: b(*)
+      real(c_double), intent(inout) :: x(*)
+      real(c_double), value :: rtol, inner_rtol
+      integer(c_int), intent(out) :: iters
+      real(c_double), intent(out) :: err
+    end function
+
+    integer(c_int) function vgpu_mass(ctx, c, mass) bind(C, name="vgpu_mass")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(in) :: c(*)
+      real(c_double), intent(out) :: mass
+    end function
+  end interface
+end module vanadis_gpu
