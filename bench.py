@@ -1,0 +1,203 @@
+"""Benchmark of the Vanadis DR-HEX8 solve path on one MI355X.
+
+Workload: 128x128x64 HEX8 unit box (1,081,665 nodes, 1,048,576 elements),
+steady advection-diffusion with DR stabilisation (V1), lambda = 1e-3, seeded
+velocity, Gaussian source, Dirichlet inflow faces; data generated on the host,
+resident in HBM before timing.
+
+A step is one plain-FP64 Jacobi-BiCG solve to rtol 1e-8 from x0 = 0
+(time-to-rtol).  `value` is the operator throughput in GDOF/s (nodes / time of
+one FP64 matrix-free apply, owner-computes scatter), measured with HIP events
+on the library's stream.  The line also carries time-to-rtol of FP64 and
+mixed precision, the other scatter modes, the roofline of the dominant kernel
+(the dual apply A p, A^T pp of every BiCG iteration) and the reference's CPU
+operator timed on the host.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel-only] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "BiCG time-to-rtol 1e-8 and operator GDOF/s; mixed/learned vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+
+def progress(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes(N, E, dual, fp32=False):
+    """Compulsory bytes of one matrix-free apply with this library's layout:
+    input/output vectors, the uint8 Dirichlet mask and five per-element
+    fields (vx, vy, vz, lambda, tau); axis coordinates are O(n^(1/3))."""
+    w = 4 if fp32 else 8
+    vec = (4 if dual else 2) * w * N
+    return vec + N + 5 * w * E
+
+
+def cpu_baseline(pkg, seconds=12.0):
+    """The reference's own CPU operator (matvec_0 of VANADIS.FOR built by
+    flang into oracle/_ref; the C oracle port when that build is absent) on a
+    128x128x8-element slab of the workload, single thread."""
+    import __graft_entry__ as ge
+    po = ge.load_oracle()
+    p = pkg.problem.unit_box(128, 128, 8, vel_seed=0, lam=1e-3, extent=(1.0, 1.0, 0.125))
+    t0 = time.time()
+    bb, _, _ = po.assemble(p)
+    t_asm = time.time() - t0
+    kind = "port"
+    apply = lambda x: po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask)  # noqa: E731
+    if po.RefLib.available("v1"):
+        R = po.RefLib("v1")
+        R.set_sizes(p.E, p.N)
+        apply = lambda x: R.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask)  # noqa: E731
+        kind = "reference"
+    x = np.random.default_rng(0).standard_normal(p.N)
+    apply(x)
+    reps, t0 = 0, time.time()
+    while time.time() - t0 < seconds:
+        apply(x)
+        reps += 1
+    dt = (time.time() - t0) / reps
+    return {"value": round(p.N / dt / 1e9, 5), "unit": "GDOF/s", "cores": 1, "kind": kind,
+            "sample": f"matvec_0 on stored element matrices, 128x128x8 elements "
+                      f"({p.N} nodes), {reps} applies in {seconds:.0f} s after "
+                      f"{t_asm:.1f} s assembly; serial like the reference solver",
+            "ms_per_apply": round(dt * 1e3, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--kernel-only", action="store_true",
+                    help="only launch the dual apply and copy kernels (counter runs)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--size", default="128,128,64")
+    a = ap.parse_args()
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the slab decomposition is a design, see the sharding notes)", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as ge
+    pkg = ge.load_package()
+    pkg.load()
+    ex, ey, ez = (int(v) for v in a.size.split(","))
+    progress(f"generating {ex}x{ey}x{ez} problem")
+    p = pkg.problem.unit_box(ex, ey, ez, vel_seed=0, lam=1e-3)
+    N, E = p.N, p.E
+    ctx = pkg.Context.from_problem(p)
+    ctx.set_options("owner", 0)
+    b = ctx.assemble_rhs(p.q_in)
+    b_dev = ctx.dev_alloc(8 * N)
+    x_dev = ctx.dev_alloc(8 * N)
+    ctx.h2d(b_dev, b)
+    zeros = np.zeros(N)
+
+    if a.kernel_only:
+        for _ in range(3):
+            ctx.time_kernel(1, "owner", pkg.FP64, reps=5)
+            ctx.time_kernel(2, "owner", pkg.FP64, reps=5)
+        progress("kernel-only done")
+        print(json.dumps({"kernel_only": True, "N": N, "E": E}))
+        return
+
+    def solve(precision, rtol=1e-8):
+        ctx.h2d(x_dev, zeros)
+        it, err, sec, rc = ctx.solve_dev(b_dev, x_dev, rtol=rtol, itmax=50000,
+                                         precision=precision)
+        if rc != pkg.OK:
+            raise RuntimeError(f"solve failed rc={rc} it={it} err={err}")
+        return it, err, sec
+
+    progress("warmup")
+    for _ in range(a.warmup):
+        solve(pkg.FP64)
+    progress(f"timing {a.steps} FP64 solves")
+    secs, iters = [], []
+    t0 = time.time()
+    for _ in range(a.steps):
+        it, err, sec = solve(pkg.FP64)
+        secs.append(sec)
+        iters.append(it)
+    wall = time.time() - t0
+    x64 = ctx.d2h(x_dev, N)
+    ms_step = 1e3 * float(np.mean(secs))
+    progress(f"fp64: {iters[-1]} iterations, {ms_step:.2f} ms to rtol")
+
+    # mixed precision
+    solve(pkg.MIXED)
+    msec, mits = [], 0
+    for _ in range(max(1, min(3, a.steps))):
+        it, err_m, sec = solve(pkg.MIXED)
+        msec.append(sec)
+        mits = it
+    xm = ctx.d2h(x_dev, N)
+    rel_l2_mixed = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+    true_res_mixed = ctx.true_residual(b, xm)
+    progress(f"mixed: {mits} inner iterations, {1e3 * np.mean(msec):.2f} ms")
+
+    # kernels, HIP events on the library stream
+    reps = 50
+    t_single = ctx.time_kernel(0, "owner", pkg.FP64, reps)
+    t_dual = ctx.time_kernel(1, "owner", pkg.FP64, reps)
+    t_copy = ctx.time_kernel(2, "owner", pkg.FP64, reps)
+    modes = {}
+    for sc in ("owner", "colored", "atomic"):
+        for prec, pn in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32")):
+            ms = ctx.time_kernel(0, sc, prec, 20)
+            modes[f"{sc}_{pn}"] = {"ms": round(ms, 5), "GDOF/s": round(N / ms / 1e6, 2)}
+    gdofs = N / t_single / 1e6
+    dual_bytes = algorithmic_bytes(N, E, dual=True)
+    achieved = dual_bytes / (t_dual * 1e-3) / 1e9
+    copy_gbs = 16.0 * N / (t_copy * 1e-3) / 1e9
+    progress(f"apply {t_single * 1e3:.1f} us ({gdofs:.1f} GDOF/s), dual {t_dual * 1e3:.1f} us "
+             f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
+
+    cpu = None
+    if not a.no_cpu:
+        progress("cpu baseline")
+        cpu = cpu_baseline(pkg)
+
+    out = {
+        "metric": METRIC, "value": round(gdofs, 3), "unit": "GDOF/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded velocity/source fields, host-generated)",
+        "config": {"workload": f"HEX8 unit box {ex}x{ey}x{ez} elements, {N} nodes, steady "
+                               "DR advection-diffusion lambda=1e-3, V1 tau, FP64 Jacobi-BiCG "
+                               "to rtol 1e-8 from x0=0, owner-computes scatter",
+                   "elements": E, "nodes": N},
+        "time_to_rtol_s": {"fp64": round(float(np.mean(secs)), 5),
+                           "mixed": round(float(np.mean(msec)), 5)},
+        "iterations": {"fp64": iters[-1], "mixed_inner_total": mits},
+        "speedup_mixed_vs_fp64": round(float(np.mean(secs)) / float(np.mean(msec)), 3),
+        "mixed_rel_l2_vs_fp64": rel_l2_mixed, "mixed_true_rel_residual": true_res_mixed,
+        "wall_s_timed_steps": round(wall, 3),
+        "apply_ms": {"single_fp64": round(t_single, 5), "dual_fp64": round(t_dual, 5)},
+        "scatter_modes": modes,
+        "copy_GBps": round(copy_gbs, 1),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_owner<double,double,dual>",
+                     "algorithmic_bytes_per_launch": dual_bytes},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

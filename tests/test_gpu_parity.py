@@ -1,0 +1,177 @@
+"""GPU parity through the C ABI against the CPU oracle (same inputs).
+
+Bars: FP64 operator apply <= 1e-13 relative (2-norm) to the oracle's
+matvec_0 / mttvec_0 on the oracle-assembled element matrices, every scatter
+mode; FP32 operator <= 2e-6; deterministic modes bitwise run to run; solves
+to the oracle's solution within the solver tolerance; the legacy drop-in
+reproduces the reference transcript masses."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SCATTERS = ["owner", "colored", "atomic"]
+
+
+def log(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+CASES = {
+    # odd sizes straddle the 32x8 / 31x7 tiles and the z chunks
+    "box37x11x9_steady_v1": dict(kind="box", dims=(37, 11, 9), scheme=1, variant=1, lam=3e-3),
+    "box20x17x13_theta_v2": dict(kind="box", dims=(20, 17, 13), scheme=0, variant=2, lam=1e-2,
+                                 dt=0.05, P=0.3),
+    "box16x40x10_euler_v3": dict(kind="box", dims=(16, 40, 10), scheme=2, variant=3, lam=5e-3,
+                                 dt=0.01),
+    "reference_default_theta_v1": dict(kind="ref"),
+    "reference_default_theta_v3": dict(kind="ref", variant=3, facemap=3),
+}
+
+
+def make_problem(pkg, name):
+    c = CASES[name]
+    if c["kind"] == "ref":
+        return pkg.problem.reference_default_case(variant=c.get("variant", 1),
+                                                  facemap=c.get("facemap", 1))
+    ex, ey, ez = c["dims"]
+    p = pkg.problem.unit_box(ex, ey, ez, vel_seed=7, lam=c["lam"], scheme=c["scheme"],
+                             dt=c.get("dt", 1.0), variant=c["variant"], P=c.get("P", 0.0))
+    return p
+
+
+@pytest.fixture(scope="module", params=list(CASES))
+def case(request, pkg, po, gpu):
+    name = request.param
+    p = make_problem(pkg, name)
+    bb, b, d = po.assemble(p)
+    ctx = pkg.Context.from_problem(p)
+    yield name, p, bb, b, d, ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("scatter", SCATTERS)
+@pytest.mark.parametrize("transpose", [False, True])
+def test_apply_fp64_matches_oracle(case, po, scatter, transpose):
+    name, p, bb, _, _, ctx = case
+    log(f"apply fp64 {name} {scatter} T={transpose}")
+    x = np.random.default_rng(11).standard_normal(p.N)
+    ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, transpose)
+    y = ctx.apply(x, scatter=scatter, transpose=transpose)
+    assert rel(y, ref) <= 1e-13, rel(y, ref)
+    assert np.all(y[p.mask == 1] == 0.0)
+
+
+@pytest.mark.parametrize("scatter", SCATTERS)
+def test_apply_fp32_matches_oracle(case, po, pkg, scatter):
+    name, p, bb, _, _, ctx = case
+    log(f"apply fp32 {name} {scatter}")
+    x = np.random.default_rng(12).standard_normal(p.N)
+    ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask)
+    y = ctx.apply(x, scatter=scatter, precision=pkg.FP32)
+    assert rel(y, ref) <= 2e-6, rel(y, ref)
+
+
+@pytest.mark.parametrize("scatter", ["owner", "colored"])
+def test_deterministic_modes_bitwise(case, scatter):
+    name, p, _, _, _, ctx = case
+    log(f"determinism {name} {scatter}")
+    x = np.random.default_rng(13).standard_normal(p.N)
+    y1 = ctx.apply(x, scatter=scatter)
+    y2 = ctx.apply(x, scatter=scatter)
+    assert np.array_equal(y1, y2)
+
+
+def test_diag_matches_oracle(case, po):
+    name, p, bb, b, d, ctx = case
+    log(f"diag {name}")
+    _, dref, _ = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    assert rel(ctx.diag(), dref) <= 1e-14
+
+
+def test_rhs_matches_oracle(case, po):
+    name, p, _, _, _, ctx = case
+    log(f"rhs {name}")
+    t_old = np.random.default_rng(14).standard_normal(p.N) * (1 - p.mask)
+    _, bref, dref = po.assemble(p, t_old=t_old)
+    bref, _, _ = po.impose_mask(p.mask, bref, dref, t_old)
+    b = ctx.assemble_rhs(p.q_in, t_old)
+    assert rel(b, bref) <= 1e-13, rel(b, bref)
+
+
+def test_mass_matches_oracle(case, po):
+    name, p, _, _, _, ctx = case
+    c = np.random.default_rng(15).random(p.N)
+    assert ctx.mass(c) == pytest.approx(po.mass(p.iss, p.xx, p.yy, p.zz, c, p.E), rel=1e-13)
+
+
+@pytest.mark.parametrize("scatter", SCATTERS)
+def test_solve_fp64_matches_oracle(case, po, scatter):
+    name, p, bb, b, d, ctx = case
+    log(f"solve fp64 {name} {scatter}")
+    b, d, x0 = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    if np.linalg.norm(b) == 0.0:
+        b = np.random.default_rng(16).standard_normal(p.N) * (1 - p.mask)
+    xo, ito, erro, rco = po.dbcg_simple(b, x0, 1e-10, 5000, bb, p.iss, d, p.nax, p.nay, p.naz,
+                                        p.mask)
+    ctx.set_options(scatter=scatter)
+    x, it, err, rc = ctx.solve(b, x0, rtol=1e-10, itmax=5000)
+    ctx.set_options(scatter="owner")
+    log(f"  iters gpu {it} oracle {ito}  err {err:.2e}")
+    assert err <= 1e-10
+    assert abs(it - ito) <= max(2, ito // 20)
+    assert rel(x, xo) <= 1e-7
+    assert ctx.true_residual(b, x) <= 1e-9
+
+
+def test_legacy_drop_in_reproduces_transcript(pkg, po, gpu, golden_log):
+    """20 time steps of the shipped case, element matrices assembled by the
+    oracle (the host's role), solved by kernel_dbcg_simple_ on the GPU."""
+    p = pkg.problem.reference_default_case()
+    for scatter in ("colored", "atomic"):
+        log(f"legacy 20 steps {scatter}")
+        t = np.zeros(p.N)
+        for k, s in enumerate(golden_log["steps"]):
+            bb, b, d = po.assemble(p, t_old=t)
+            b, d, t = po.impose_mask(p.mask, b, d, t)
+            t, it, err = pkg.kernel_dbcg_simple(p.N, b, t, 1e-8, 1000, bb, p.iss, d, p.nax,
+                                                p.nay, p.naz, p.mask, scatter=scatter)
+            m = po.mass(p.iss, p.xx, p.yy, p.zz, t, p.E)
+            assert err <= 1e-8
+            assert abs(it - s["iters"]) <= 1, (k, it, s["iters"])
+            assert m == pytest.approx(float(s["mass_text"]), rel=1e-9), k
+
+
+def test_mixed_precision_reaches_fp64_solution(pkg, po, gpu):
+    p = pkg.problem.config_problem("c1")
+    log("mixed c1")
+    with pkg.Context.from_problem(p) as ctx:
+        b = ctx.assemble_rhs(p.q_in)
+        x64, it64, e64, _ = ctx.solve(b, rtol=1e-12, itmax=20000)
+        xm, itm, em, _ = ctx.solve(b, rtol=1e-8, itmax=20000, precision=pkg.MIXED,
+                                   inner_rtol=1e-4)
+        log(f"  fp64 iters {it64}, mixed iters {itm}, mixed true rel res {em:.2e}")
+        assert em <= 1e-8
+        assert ctx.true_residual(b, xm) <= 1e-8
+        assert rel(xm, x64) <= 1e-6
+
+
+def test_context_rejects_non_tensor_grid(pkg, gpu):
+    p = pkg.problem.unit_box(4, 3, 3)
+    xx = p.xx.copy()
+    xx[5] += 1e-3
+    with pytest.raises(pkg.VgpuError) as ei:
+        pkg.Context(p.nax, p.nay, p.naz, xx, p.yy, p.zz, p.iss, p.mask)
+    assert ei.value.code == pkg.ERR_MESH
+    iss = p.iss.copy()
+    iss[3] += 1
+    with pytest.raises(pkg.VgpuError) as ei:
+        pkg.Context(p.nax, p.nay, p.naz, p.xx, p.yy, p.zz, iss, p.mask)
+    assert ei.value.code == pkg.ERR_MESH

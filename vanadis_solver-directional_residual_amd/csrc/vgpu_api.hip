@@ -335,7 +335,7 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
     int it = 0;
     double e = 0;
     rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
-                         inner_rtol, itmax - total, scatter != VGPU_SCATTER_OWNER,
+                         inner_rtol, itmax, scatter != VGPU_SCATTER_OWNER,
                          c->stream, &it, &e, &st_inner);
     if (rc) return rc;
     total += it;

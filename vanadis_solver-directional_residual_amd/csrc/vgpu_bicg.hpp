@@ -130,8 +130,9 @@ __global__ __launch_bounds__(VB) void k_dir(int64_t n, int nb, int q,
   for (int64_t i = blockIdx.x * (int64_t)VB + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * VB) {
     const TV zi = z[i], zzi = zz[i];
-    p[i] = zi + bt * p[i];
-    pp[i] = zzi + bt * pp[i];
+    // iteration 1 copies (cudatest.cu:504-505): no 0 * stale-value products
+    p[i] = (k == 1) ? zi : zi + bt * p[i];
+    pp[i] = (k == 1) ? zzi : zzi + bt * pp[i];
     if (zero_out) {
       z[i] = TV(0);
       zz[i] = TV(0);
