@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dual apply and copy kernels (counter runs)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--size", default="128,128,64")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -137,17 +138,24 @@ def main():
     ms_step = 1e3 * float(np.mean(secs))
     progress(f"fp64: {iters[-1]} iterations, {ms_step:.2f} ms to rtol")
 
-    # mixed precision
-    solve(pkg.MIXED)
-    msec, mits = [], 0
-    for _ in range(max(1, min(3, a.steps))):
-        it, err_m, sec = solve(pkg.MIXED)
-        msec.append(sec)
-        mits = it
-    xm = ctx.d2h(x_dev, N)
-    rel_l2_mixed = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
-    true_res_mixed = ctx.true_residual(b, xm)
-    progress(f"mixed: {mits} inner iterations, {1e3 * np.mean(msec):.2f} ms")
+    # mixed precision (reported, never fatal for the line)
+    msec, mits, rel_l2_mixed, true_res_mixed, mixed_status = [], 0, None, None, "ok"
+    if not a.no_mixed:
+        try:
+            solve(pkg.MIXED)
+            for _ in range(max(1, min(3, a.steps))):
+                it, err_m, sec = solve(pkg.MIXED)
+                msec.append(sec)
+                mits = it
+            xm = ctx.d2h(x_dev, N)
+            rel_l2_mixed = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+            true_res_mixed = ctx.true_residual(b, xm)
+            progress(f"mixed: {mits} inner iterations, {1e3 * np.mean(msec):.2f} ms")
+        except RuntimeError as exc:
+            mixed_status = f"failed: {exc}"
+            progress(f"mixed {mixed_status}")
+    else:
+        mixed_status = "skipped"
 
     # kernels, HIP events on the library stream
     reps = 50
@@ -181,9 +189,11 @@ def main():
                                "to rtol 1e-8 from x0=0, owner-computes scatter",
                    "elements": E, "nodes": N},
         "time_to_rtol_s": {"fp64": round(float(np.mean(secs)), 5),
-                           "mixed": round(float(np.mean(msec)), 5)},
+                           "mixed": round(float(np.mean(msec)), 5) if msec else None},
         "iterations": {"fp64": iters[-1], "mixed_inner_total": mits},
-        "speedup_mixed_vs_fp64": round(float(np.mean(secs)) / float(np.mean(msec)), 3),
+        "speedup_mixed_vs_fp64": (round(float(np.mean(secs)) / float(np.mean(msec)), 3)
+                                  if msec else None),
+        "mixed_status": mixed_status,
         "mixed_rel_l2_vs_fp64": rel_l2_mixed, "mixed_true_rel_residual": true_res_mixed,
         "wall_s_timed_steps": round(wall, 3),
         "apply_ms": {"single_fp64": round(t_single, 5), "dual_fp64": round(t_dual, 5)},

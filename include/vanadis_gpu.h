@@ -130,8 +130,10 @@ int vgpu_diag(vgpu_ctx *ctx, double *diag);
 int vgpu_assemble_rhs(vgpu_ctx *ctx, const double *q_in, const double *t_old,
                       double *b);
 /* Solve A x = b.  x is in/out (warm start, zeroed at masked nodes).
- * precision VGPU_FP64 or VGPU_MIXED (inner FP32 BiCG to inner_rtol, FP64
- * true-residual refinement to rtol).  iters = total BiCG iterations,
+ * precision VGPU_FP64, VGPU_FP32 (FP32 vectors and operator, FP64 scalars)
+ * or VGPU_MIXED (inner FP32 BiCG to inner_rtol, 0 = 1e-2, with FP64
+ * true-residual refinement to rtol; if a refinement step fails to halve the
+ * true residual the solve finishes in FP64 BiCG).  iters = total BiCG iterations,
  * err = final relative residual (recursive for FP64, true for MIXED). */
 int vgpu_solve(vgpu_ctx *ctx, int precision, const double *b, double *x,
                double rtol, int itmax, double inner_rtol, int *iters,

@@ -202,7 +202,7 @@ class Context:
               "assemble_rhs")
         return b
 
-    def solve(self, b, x0=None, rtol=1e-8, itmax=20000, precision=FP64, inner_rtol=1e-4,
+    def solve(self, b, x0=None, rtol=1e-8, itmax=20000, precision=FP64, inner_rtol=0.0,
               raise_on_fail=True):
         """Returns (x, iters, err, status code)."""
         x = np.zeros(self.N) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
@@ -240,7 +240,7 @@ class Context:
         return a
 
     def solve_dev(self, b_dev, x_dev, rtol=1e-8, itmax=20000, precision=FP64,
-                  inner_rtol=1e-4):
+                  inner_rtol=0.0):
         it = C.c_int(0)
         err = C.c_double(0.0)
         sec = C.c_double(0.0)

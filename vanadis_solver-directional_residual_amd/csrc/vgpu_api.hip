@@ -85,6 +85,8 @@ struct vgpu_ctx {
   hipStream_t stream = nullptr;
   Grid3 g{};
   double *xc = nullptr, *yc = nullptr, *zc = nullptr;
+  double *ah[3] = {nullptr, nullptr, nullptr}, *aih[3] = {nullptr, nullptr, nullptr};
+  float *fah[3] = {nullptr, nullptr, nullptr}, *faih[3] = {nullptr, nullptr, nullptr};
   uint8_t *mask = nullptr;
   double *vx = nullptr, *vy = nullptr, *vz = nullptr, *lam = nullptr,
          *tau = nullptr;
@@ -113,6 +115,12 @@ struct vgpu_ctx {
     o.yc = yc;
     o.zc = zc;
     o.mask = mask;
+    o.hx = ah[0];
+    o.hy = ah[1];
+    o.hz = ah[2];
+    o.ihx = aih[0];
+    o.ihy = aih[1];
+    o.ihz = aih[2];
     o.vx = vx;
     o.vy = vy;
     o.vz = vz;
@@ -131,6 +139,12 @@ struct vgpu_ctx {
     o.yc = yc;
     o.zc = zc;
     o.mask = mask;
+    o.hx = fah[0];
+    o.hy = fah[1];
+    o.hz = fah[2];
+    o.ihx = faih[0];
+    o.ihy = faih[1];
+    o.ihz = faih[2];
     o.vx = fvx;
     o.vy = fvy;
     o.vz = fvz;
@@ -273,6 +287,42 @@ int solve_fp64_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                          status);
 }
 
+// Pure FP32 BiCG (FP32 vectors and operator, FP64 scalars): b, x FP64 in/out
+int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
+                   int itmax, int *iters, double *err, int *status) {
+  const int64_t N = c->g.N;
+  const int scatter = c->scatter;
+  if (c->bicg32.n != N) {
+    int rc = c->bicg32.alloc(N, owner_num_tiles(c->g, 1) + 256);
+    if (rc) return rc;
+  }
+  if (!c->fb) {
+    VGPU_HIP(hipMalloc(&c->fb, N * sizeof(float)));
+    VGPU_HIP(hipMalloc(&c->fx, N * sizeof(float)));
+  }
+  hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
+                     c->stream, N, b, c->fb);
+  hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
+                     c->stream, N, x, c->fx);
+  VGPU_HIP(hipGetLastError());
+  DualOp<float> dual = make_dual32(c, scatter);
+  SingleOp<float> single = [c, scatter](const float *xx, float *y,
+                                        hipStream_t s) {
+    OpData<float> op = c->op32();
+    return launch_apply<float, float>(op, scatter, c->zchunk, xx, y, nullptr,
+                                      nullptr, nullptr, s);
+  };
+  const uint64_t key = (c->version << 8) | (uint64_t)(scatter * 16 + c->zchunk % 16) | (1ull << 7);
+  int rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
+                           rtol, itmax, scatter != VGPU_SCATTER_OWNER, c->stream,
+                           iters, err, status);
+  if (rc) return rc;
+  hipLaunchKernelGGL((k_convert<float, double>), grid_for(N), dim3(256), 0,
+                     c->stream, N, c->fx, x);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
 // Mixed precision: FP64 true residual + refinement, FP32 inner BiCG
 int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                     int itmax, double inner_rtol, int *iters, double *err,
@@ -304,7 +354,7 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
   };
   const uint64_t key = (c->version << 8) | (uint64_t)(scatter * 16 + c->zchunk % 16) | (1ull << 7);
   int total = 0, st_inner = 0;
-  double rel = 0;
+  double rel = 0, prev_rel = 0;
   *status = 2;
   // x masked like the FP64 path
   hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N, c->mask,
@@ -329,6 +379,19 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
       break;
     }
     if (total >= itmax) break;
+    if (outer > 0 && rel > 0.5 * prev_rel) {
+      // the FP32 inner solve stopped reducing the true residual: finish in
+      // FP64 BiCG from the current iterate (the fallback of the mixed mode)
+      int it64 = 0, st64 = 0;
+      double e64 = 0;
+      rc = solve_fp64_dev(c, b, x, rtol, itmax - total, &it64, &e64, &st64);
+      if (rc) return rc;
+      total += it64;
+      *status = st64;
+      rel = e64;
+      break;
+    }
+    prev_rel = rel;
     hipLaunchKernelGGL(k_scale_to_f32, grid_for(N), dim3(256), 0, c->stream, N,
                        c->vt, 1.0 / rn, c->fb, c->fx);
     VGPU_HIP(hipGetLastError());
@@ -464,6 +527,30 @@ int vgpu_create(int nax, int nay, int naz, const double *xx, const double *yy,
   CK(hipMemcpy(c->yc, ay.data(), nay * sizeof(double), hipMemcpyHostToDevice));
   CK(hipMemcpy(c->zc, az.data(), naz * sizeof(double), hipMemcpyHostToDevice));
   CK(hipMemcpy(c->mask, m.data(), N, hipMemcpyHostToDevice));
+  {
+    // element sizes and reciprocals per axis (FP64 and FP32 copies)
+    const std::vector<double> *axes[3] = {&ax, &ay, &az};
+    for (int d = 0; d < 3; ++d) {
+      const std::vector<double> &a = *axes[d];
+      const size_t ne = a.size() - 1;
+      std::vector<double> h(ne), ih(ne);
+      std::vector<float> fh(ne), fih(ne);
+      for (size_t i = 0; i < ne; ++i) {
+        h[i] = a[i + 1] - a[i];
+        ih[i] = 1.0 / h[i];
+        fh[i] = (float)h[i];
+        fih[i] = (float)ih[i];
+      }
+      CK(hipMalloc(&c->ah[d], ne * sizeof(double)));
+      CK(hipMalloc(&c->aih[d], ne * sizeof(double)));
+      CK(hipMalloc(&c->fah[d], ne * sizeof(float)));
+      CK(hipMalloc(&c->faih[d], ne * sizeof(float)));
+      CK(hipMemcpy(c->ah[d], h.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+      CK(hipMemcpy(c->aih[d], ih.data(), ne * sizeof(double), hipMemcpyHostToDevice));
+      CK(hipMemcpy(c->fah[d], fh.data(), ne * sizeof(float), hipMemcpyHostToDevice));
+      CK(hipMemcpy(c->faih[d], fih.data(), ne * sizeof(float), hipMemcpyHostToDevice));
+    }
+  }
   for (double **p : {&c->vx, &c->vy, &c->vz, &c->lam, &c->tau})
     CK(hipMalloc(p, E * sizeof(double)));
   for (float **p : {&c->fvx, &c->fvy, &c->fvz, &c->flam, &c->ftau})
@@ -488,6 +575,12 @@ int vgpu_destroy(vgpu_ctx *c) {
   free_dev(c->yc);
   free_dev(c->zc);
   free_dev(c->mask);
+  for (int d = 0; d < 3; ++d) {
+    free_dev(c->ah[d]);
+    free_dev(c->aih[d]);
+    free_dev(c->fah[d]);
+    free_dev(c->faih[d]);
+  }
   for (double **p : {&c->vx, &c->vy, &c->vz, &c->lam, &c->tau, &c->diag, &c->vb,
                      &c->vx_, &c->vt, &c->vt2, &c->red})
     free_dev(*p);
@@ -718,11 +811,13 @@ int vgpu_solve_dev(vgpu_ctx *c, int precision, const double *b, double *x,
   int status = 0;
   if (precision == VGPU_FP64)
     rc = solve_fp64_dev(c, b, x, rtol, itmax, iters, err, &status);
+  else if (precision == VGPU_FP32)
+    rc = solve_fp32_dev(c, b, x, rtol, itmax, iters, err, &status);
   else if (precision == VGPU_MIXED)
     rc = solve_mixed_dev(c, b, x, rtol, itmax,
-                         inner_rtol > 0 ? inner_rtol : 1e-4, iters, err, &status);
+                         inner_rtol > 0 ? inner_rtol : 1e-2, iters, err, &status);
   else {
-    set_error("vgpu_solve: precision must be FP64 or MIXED");
+    set_error("vgpu_solve: precision must be FP64, FP32 or MIXED");
     return VGPU_ERR_ARG;
   }
   if (rc) return rc;

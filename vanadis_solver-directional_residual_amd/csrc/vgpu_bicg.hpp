@@ -94,6 +94,10 @@ __global__ __launch_bounds__(VB) void k_init(int64_t n, const uint8_t *mask,
   block_partial(sn, pbn, red);
 }
 
+// Elements per thread staged in registers before the block reductions, so
+// the first chunk of vector traffic overlaps the partial-sum latency.
+constexpr int PF = 4;
+
 // K1: convergence test of iteration k-1, then p = z + beta p, pp = zz + beta pp.
 // q = k & 1.  zero_out: also clear z, zz (brick scatter modes accumulate).
 template <typename TV>
@@ -106,6 +110,18 @@ __global__ __launch_bounds__(VB) void k_dir(int64_t n, int nb, int q,
   __shared__ double red[VB / 64];
   if (st->done) return;
   const int k = st->iter + 1;
+  const int64_t stride = (int64_t)gridDim.x * VB;
+  const int64_t i0 = blockIdx.x * (int64_t)VB + threadIdx.x;
+  TV zr[PF], zzr[PF], pr0[PF], ppr0[PF];
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int64_t i = i0 + j * stride;
+    const bool ok = i < n;
+    zr[j] = ok ? z[i] : TV(0);
+    zzr[j] = ok ? zz[i] : TV(0);
+    pr0[j] = ok ? p[i] : TV(0);
+    ppr0[j] = ok ? pp[i] : TV(0);
+  }
   const double rn2 = block_sum(pr + (q ^ 1) * nb, nb, red);
   const double bn2 = block_sum(pbn, nb, red);
   const double err = sqrt(rn2 / bn2);
@@ -127,12 +143,24 @@ __global__ __launch_bounds__(VB) void k_dir(int64_t n, int nb, int q,
   }
   if (done) return;
   const TV bt = TV(beta);
-  for (int64_t i = blockIdx.x * (int64_t)VB + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * VB) {
+  // iteration 1 copies (cudatest.cu:504-505): no 0 * stale-value products
+  const bool first = k == 1;
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int64_t i = i0 + j * stride;
+    if (i < n) {
+      p[i] = first ? zr[j] : zr[j] + bt * pr0[j];
+      pp[i] = first ? zzr[j] : zzr[j] + bt * ppr0[j];
+      if (zero_out) {
+        z[i] = TV(0);
+        zz[i] = TV(0);
+      }
+    }
+  }
+  for (int64_t i = i0 + PF * stride; i < n; i += stride) {
     const TV zi = z[i], zzi = zz[i];
-    // iteration 1 copies (cudatest.cu:504-505): no 0 * stale-value products
-    p[i] = (k == 1) ? zi : zi + bt * p[i];
-    pp[i] = (k == 1) ? zzi : zzi + bt * pp[i];
+    p[i] = first ? zi : zi + bt * p[i];
+    pp[i] = first ? zzi : zzi + bt * pp[i];
     if (zero_out) {
       z[i] = TV(0);
       zz[i] = TV(0);
@@ -157,12 +185,44 @@ __global__ __launch_bounds__(VB) void k_dot(int64_t n, const SolveStatus *st,
 // K3: alpha = bknum/akden; x += a p; r -= a z; rr -= a zz; z = r/d; zz = rr/d;
 // partials of r.r and z.rr into parity q.
 template <typename TV>
+__device__ __forceinline__ void update_one(int64_t i, TV a, TV xi, TV pi, TV ri,
+                                           TV rri, TV zi, TV zzi, TV di, TV *x,
+                                           TV *r, TV *rr, TV *z, TV *zz,
+                                           double &sr, double &sb) {
+  x[i] = xi + a * pi;
+  const TV rn = ri - a * zi;
+  const TV rrn = rri - a * zzi;
+  r[i] = rn;
+  rr[i] = rrn;
+  const TV zn = rn / di, zzn = rrn / di;
+  z[i] = zn;
+  zz[i] = zzn;
+  sr += (double)rn * (double)rn;
+  sb += (double)zn * (double)rrn;
+}
+
+template <typename TV>
 __global__ __launch_bounds__(VB) void k_update(
     int64_t n, int nb, int na, int q, const double *pa, double *pr, double *pb,
     SolveStatus *st, TV *x, const TV *p, TV *r, TV *rr, TV *z, TV *zz,
     const TV *diag) {
   __shared__ double red[VB / 64];
   if (st->done) return;
+  const int64_t stride = (int64_t)gridDim.x * VB;
+  const int64_t i0 = blockIdx.x * (int64_t)VB + threadIdx.x;
+  TV xv[PF], pv[PF], rv[PF], rrv[PF], zv[PF], zzv[PF], dv[PF];
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int64_t i = i0 + j * stride;
+    const bool ok = i < n;
+    xv[j] = ok ? x[i] : TV(0);
+    pv[j] = ok ? p[i] : TV(0);
+    rv[j] = ok ? r[i] : TV(0);
+    rrv[j] = ok ? rr[i] : TV(0);
+    zv[j] = ok ? z[i] : TV(0);
+    zzv[j] = ok ? zz[i] : TV(0);
+    dv[j] = ok ? diag[i] : TV(1);
+  }
   const double akden = block_sum(pa, na, red);
   const double bknum = block_sum(pb + (q ^ 1) * nb, nb, red);
   const double ak = bknum / akden;
@@ -172,20 +232,16 @@ __global__ __launch_bounds__(VB) void k_update(
   }
   const TV a = TV(ak);
   double sr = 0, sb = 0;
-  for (int64_t i = blockIdx.x * (int64_t)VB + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * VB) {
-    x[i] = x[i] + a * p[i];
-    const TV ri = r[i] - a * z[i];
-    const TV rri = rr[i] - a * zz[i];
-    r[i] = ri;
-    rr[i] = rri;
-    const TV d = diag[i];
-    const TV zi = ri / d, zzi = rri / d;
-    z[i] = zi;
-    zz[i] = zzi;
-    sr += (double)ri * (double)ri;
-    sb += (double)zi * (double)rri;
+#pragma unroll
+  for (int j = 0; j < PF; ++j) {
+    const int64_t i = i0 + j * stride;
+    if (i < n)
+      update_one<TV>(i, a, xv[j], pv[j], rv[j], rrv[j], zv[j], zzv[j], dv[j], x, r,
+                     rr, z, zz, sr, sb);
   }
+  for (int64_t i = i0 + PF * stride; i < n; i += stride)
+    update_one<TV>(i, a, x[i], p[i], r[i], rr[i], z[i], zz[i], diag[i], x, r, rr,
+                   z, zz, sr, sb);
   block_partial(sr, pr + q * nb, red);
   block_partial(sb, pb + q * nb, red);
   if (blockIdx.x == 0 && threadIdx.x == 0) st->iter = st->iter + 1;

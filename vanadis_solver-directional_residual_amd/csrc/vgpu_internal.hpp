@@ -44,6 +44,8 @@ template <typename T>
 struct OpData {
   Grid3 g;
   const double *xc, *yc, *zc;  // axis coordinates (device, FP64)
+  const T *hx, *hy, *hz;       // element sizes along each axis [ex], [ey], [ez]
+  const T *ihx, *ihy, *ihz;    // their reciprocals
   const uint8_t *mask;         // 1 = Dirichlet node
   const T *vx, *vy, *vz, *lam, *tau;  // per element, SoA
   double cH;     // weight of H (diffusion + advection + DR + Robin)
@@ -63,6 +65,14 @@ constexpr int PLN = PLX * PLY;  // 297 nodes per tile plane
 constexpr int OWN_X = TXE - 1;  // owner mode: nodes owned per tile in x
 constexpr int OWN_Y = TYE - 1;
 
+// Workgroup barrier that waits only for LDS (and scalar) traffic: global
+// loads issued before it stay in flight, which is what lets a z-marching
+// kernel prefetch the next layer while it computes the current one.  The
+// compiler still inserts the vmcnt wait before any use of a prefetched value.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 // ---------------------------------------------------------------------------
 // Element math.  Local node order (VANADIS.FOR:255-262, par_3d 332-357):
 // 0:(0,0,0) 1:(1,0,0) 2:(1,1,0) 3:(0,1,0) 4:(0,0,1) 5:(1,0,1) 6:(1,1,1)
@@ -70,41 +80,49 @@ constexpr int OWN_Y = TYE - 1;
 // ---------------------------------------------------------------------------
 template <typename T>
 struct ElemConst {
-  T m[3][2];  // 1-D mass eigenvalues / 2: h/4 (s), h/12 (d)
-  T dg[8];    // diagonal of the Haar-space operator (already / 8)
-  T gc[3];    // A   : d -> s coupling per axis (cH v_k / 2)
-  T cc[3];    // A   : s -> d coupling per axis (mu tau v_k / 2)
-  T tc[3];    // symmetric DR cross terms, indexed by the free axis (/ 4)
+  T m[3][2];  // 1-D mass eigenvalues / 2: h/4 (s mode), h/12 (d mode)
+  T W[8];     // mass weight of each Haar mode: m_x(bx) m_y(by) m_z(bz)
+  T D[8];     // diagonal multiplier: mu + sum over d-mode axes of bra_k
+  T g[3];     // d -> s coupling per axis on u = W xh: 6 cH v_k / h_k
+  T c[3];     // s -> d coupling per axis: 2 mu tau v_k / h_k
+  T t[3];     // symmetric DR cross terms, by free axis: 12 cH tau v_k v_l / (h_k h_l)
 };
 
+// Per-element constants from the sizes h, reciprocals ih and the fields.
+// bra_k = 12 cH (lam + tau v_k^2) / h_k^2 is the K/M eigenvalue ratio of the
+// d mode times the diffusion plus DR streamline-diffusion coefficient.
 template <typename T>
-__device__ __forceinline__ void elem_setup(T hx, T hy, T hz, T lam, T vx, T vy,
-                                           T vz, T tau, T cH, T mu,
-                                           ElemConst<T> &k) {
+__device__ __forceinline__ void elem_setup(T hx, T hy, T hz, T ihx, T ihy,
+                                           T ihz, T lam, T vx, T vy, T vz,
+                                           T tau, T cH, T mu, ElemConst<T> &k) {
   const T h[3] = {hx, hy, hz};
+  const T ih[3] = {ihx, ihy, ihz};
   const T v[3] = {vx, vy, vz};
-  T bra[3];
+  const T c12 = T(12) * cH, c6 = T(6) * cH, m2 = T(2) * mu;
+  T bra[3], tv[3];
 #pragma unroll
   for (int d = 0; d < 3; ++d) {
     k.m[d][0] = h[d] * T(0.25);
     k.m[d][1] = h[d] * T(1.0 / 12.0);
-    // K/M ratio on the d-mode is 12/h^2; DR streamline diffusion tau v_k^2
-    bra[d] = cH * T(12.0) / (h[d] * h[d]) * (lam + tau * v[d] * v[d]);
-    k.gc[d] = cH * v[d] * T(0.5);
-    k.cc[d] = mu * tau * v[d] * T(0.5);
+    tv[d] = tau * v[d];
+    bra[d] = c12 * (ih[d] * ih[d]) * (lam + tv[d] * v[d]);
+    k.g[d] = c6 * v[d] * ih[d];
+    k.c[d] = m2 * tv[d] * ih[d];
   }
-  k.tc[0] = cH * tau * vy * vz * T(0.25);  // y-z coupling, free axis x
-  k.tc[1] = cH * tau * vx * vz * T(0.25);  // x-z coupling, free axis y
-  k.tc[2] = cH * tau * vx * vy * T(0.25);  // x-y coupling, free axis z
+  k.t[0] = c12 * tv[1] * vz * (ih[1] * ih[2]);  // y-z, free axis x
+  k.t[1] = c12 * tv[0] * vz * (ih[0] * ih[2]);  // x-z, free axis y
+  k.t[2] = c12 * tv[0] * vy * (ih[0] * ih[1]);  // x-y, free axis z
+  T wxy[4];
 #pragma unroll
-  for (int b = 0; b < 8; ++b) {
-    const int bx = b & 1, by = (b >> 1) & 1, bz = b >> 2;
-    T s = mu;
-    if (bx) s += bra[0];
-    if (by) s += bra[1];
-    if (bz) s += bra[2];
-    k.dg[b] = k.m[0][bx] * k.m[1][by] * k.m[2][bz] * s;
-  }
+  for (int q = 0; q < 4; ++q) wxy[q] = k.m[0][q & 1] * k.m[1][q >> 1];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) k.W[b] = wxy[b & 3] * k.m[2][b >> 2];
+  k.D[0] = mu;
+  k.D[1] = mu + bra[0];
+  k.D[2] = mu + bra[1];
+  k.D[3] = k.D[1] + bra[1];
+#pragma unroll
+  for (int b = 0; b < 4; ++b) k.D[b + 4] = k.D[b] + bra[2];
 }
 
 // forward per-axis butterflies: nodal x[8] -> Haar h[8]
@@ -151,30 +169,31 @@ __device__ __forceinline__ void haar_inv(const T h[8], T y[8]) {
   y[6] = d6 + d7;
 }
 
-// Haar-space operator; TR = transpose (swap the d->s / s->d couplings).
+// Haar-space operator on u = W xh; TR = transpose (swap the d->s / s->d
+// couplings; the diagonal and the DR cross terms are symmetric).
 template <typename T, bool TR>
 __device__ __forceinline__ void haar_apply(const ElemConst<T> &k, const T xh[8],
                                            T yh[8]) {
+  T u[8];
 #pragma unroll
-  for (int b = 0; b < 8; ++b) yh[b] = k.dg[b] * xh[b];
-  // axis pairs: axis d, the other two modes (p, q)
+  for (int b = 0; b < 8; ++b) {
+    u[b] = k.W[b] * xh[b];
+    yh[b] = k.D[b] * u[b];
+  }
 #pragma unroll
   for (int ax = 0; ax < 3; ++ax) {
     const int bit = 1 << ax;
     const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
-    const T g = TR ? k.cc[ax] : k.gc[ax];
-    const T c = TR ? k.gc[ax] : k.cc[ax];
+    const T g = TR ? k.c[ax] : k.g[ax];
+    const T c = TR ? k.g[ax] : k.c[ax];
 #pragma unroll
     for (int pq = 0; pq < 4; ++pq) {
-      const int p = pq & 1, q = pq >> 1;
-      const int b0 = (p << o1) | (q << o2);  // mode s on axis ax
-      const int b1 = b0 | bit;               // mode d on axis ax
-      const T w = k.m[o1][p] * k.m[o2][q];
-      yh[b0] += g * w * xh[b1];
-      yh[b1] += c * w * xh[b0];
+      const int b0 = ((pq & 1) << o1) | ((pq >> 1) << o2);  // s on axis ax
+      const int b1 = b0 | bit;                             // d on axis ax
+      yh[b0] += g * u[b1];
+      yh[b1] += c * u[b0];
     }
   }
-  // DR cross terms (symmetric): axes (k,l), free axis j
 #pragma unroll
   for (int j = 0; j < 3; ++j) {
     const int kx = (j == 0) ? 1 : 0, lx = (j == 2) ? 1 : 2;
@@ -182,12 +201,10 @@ __device__ __forceinline__ void haar_apply(const ElemConst<T> &k, const T xh[8],
 #pragma unroll
     for (int f = 0; f < 2; ++f) {
       const int base = f ? bj : 0;
-      const T w = k.tc[j] * k.m[j][f];
-      // s on k, d on l  <->  d on k, s on l
-      const int i1 = base | bl, i2 = base | bk;
-      const T x1 = xh[i1], x2 = xh[i2];
-      yh[i2] += w * x1;
-      yh[i1] += w * x2;
+      const int i1 = base | bl, i2 = base | bk;  // (s,d) and (d,s) on (k,l)
+      const T u1 = u[i1], u2 = u[i2];
+      yh[i2] += k.t[j] * u1;
+      yh[i1] += k.t[j] * u2;
     }
   }
 }

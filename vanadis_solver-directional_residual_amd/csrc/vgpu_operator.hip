@@ -23,12 +23,13 @@ namespace {
 
 template <typename T>
 struct Coef {
-  T lam, vx, vy, vz, tau;
+  T lam, vx, vy, vz, tau, hz, ihz;
 };
 
+// per-element fields and the layer's z size (zero / unit outside the grid)
 template <typename T>
 __device__ __forceinline__ Coef<T> load_coef(const OpData<T> &op, bool valid,
-                                             int64_t e) {
+                                             int64_t e, int kz) {
   Coef<T> c;
   if (valid) {
     c.lam = op.lam[e];
@@ -36,28 +37,37 @@ __device__ __forceinline__ Coef<T> load_coef(const OpData<T> &op, bool valid,
     c.vy = op.vy[e];
     c.vz = op.vz[e];
     c.tau = op.tau[e];
+    c.hz = op.hz[kz];
+    c.ihz = op.ihz[kz];
   } else {
     c.lam = c.vx = c.vy = c.vz = c.tau = T(0);
+    c.hz = c.ihz = T(1);
   }
   return c;
 }
 
-// load one tile node plane value (masked, zero outside the grid)
+// One tile-plane node: masked value, and its Dirichlet flag in m (nodes
+// outside the grid read as masked zeros).  Value and flag load in parallel.
 template <typename T, typename TIO>
 __device__ __forceinline__ T load_node(const OpData<T> &op, const TIO *x, int gx,
-                                       int gy, int gz) {
+                                       int gy, int gz, uint8_t &m) {
   if (gx < 0 || gy < 0 || gz < 0 || gx >= op.g.nax || gy >= op.g.nay ||
-      gz >= op.g.naz)
+      gz >= op.g.naz) {
+    m = 1;
     return T(0);
+  }
   const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * gz);
-  return op.mask[n] ? T(0) : T(x[n]);
+  const uint8_t mk = op.mask[n];
+  const T v = T(x[n]);
+  m = mk;
+  return mk ? T(0) : v;
 }
 
 // Element apply for one thread: gathers from the two LDS planes, writes the 8
 // results (and 8 transposed results when DUAL) to the staging arrays.
 template <typename T, bool DUAL>
 __device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
-                                          bool robin, T hx, T hy, T hz,
+                                          bool robin, T hx, T hy, T ihx, T ihy,
                                           const Coef<T> &cf, const T *pb,
                                           const T *pt, const T *pb2,
                                           const T *pt2, int tx, int ty,
@@ -65,8 +75,8 @@ __device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
   T y[8], y2[8];
   if (valid) {
     ElemConst<T> k;
-    elem_setup<T>(hx, hy, hz, cf.lam, cf.vx, cf.vy, cf.vz, cf.tau, T(op.cH),
-                  T(op.mu), k);
+    elem_setup<T>(hx, hy, cf.hz, ihx, ihy, cf.ihz, cf.lam, cf.vx, cf.vy, cf.vz,
+                  cf.tau, T(op.cH), T(op.mu), k);
     const int i00 = ty * PLX + tx;
     T x[8], xh[8], yh[8];
     x[0] = pb[i00];
@@ -122,11 +132,51 @@ __device__ __forceinline__ double node_sum(const T *stage, int lx, int ly,
   return s;
 }
 
+// Plane traffic of one thread: tile-plane nodes q0 = tid and q1 = tid + 256
+// (q1 < 297 only), value of each input and the Dirichlet flag.
+template <typename T, bool DUAL>
+struct PlaneRegs {
+  T a0, a1, b0, b1;
+  uint8_t m0, m1;
+};
+
+template <typename T, typename TIO, bool DUAL>
+__device__ __forceinline__ void plane_load(const OpData<T> &op, const TIO *x1,
+                                           const TIO *x2, int gx0, int gy0,
+                                           int gz, int lx0, int ly0, int lx1,
+                                           int ly1, bool has1,
+                                           PlaneRegs<T, DUAL> &r) {
+  uint8_t dm;
+  r.a0 = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, gz, r.m0);
+  r.b0 = DUAL ? load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, gz, dm) : T(0);
+  r.m1 = 1;
+  r.a1 = r.b1 = T(0);
+  if (has1) {
+    r.a1 = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, gz, r.m1);
+    r.b1 = DUAL ? load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, gz, dm) : T(0);
+  }
+}
+
+template <typename T, bool DUAL>
+__device__ __forceinline__ void plane_store(const PlaneRegs<T, DUAL> &r, T *p1,
+                                            T *p2, uint8_t *pm, int q0, int q1,
+                                            bool has1) {
+  p1[q0] = r.a0;
+  pm[q0] = r.m0;
+  if (DUAL) p2[q0] = r.b0;
+  if (has1) {
+    p1[q1] = r.a1;
+    pm[q1] = r.m1;
+    if (DUAL) p2[q1] = r.b1;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // OWNER kernel.  Grid: (ceil(nax/31), ceil(nay/7), ceil(naz/zc)).
 // Tile (bx,by,bz) owns nodes [31bx,31bx+31) x [7by,7by+7) x [zc bz, zc bz+zc)
 // and computes element columns 31bx-1 .. 31bx+30 (x), 7by-1 .. 7by+6 (y),
-// element layers zc bz - 1 .. zc bz + zc - 1.
+// element layers zc bz - 1 .. zc bz + zc - 1.  The next node plane, the next
+// layer's fields and sizes are loaded one layer ahead.
 // ---------------------------------------------------------------------------
 template <typename T, typename TIO, bool DUAL>
 __global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
@@ -137,6 +187,7 @@ __global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
                                                 double *__restrict__ dot_part) {
   __shared__ T pl1[2][PLN];
   __shared__ T pl2[DUAL ? 2 : 1][DUAL ? PLN : 1];
+  __shared__ uint8_t pm[2][PLN];
   __shared__ T st1[8 * TPB];
   __shared__ T st2[DUAL ? 8 * TPB : 1];
   __shared__ double red[TPB / 64];
@@ -149,92 +200,67 @@ __global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
   const int kz0 = blockIdx.z * zc;          // first owned node plane
   const int exg = gx0 + tx, eyg = gy0 + ty;
   const bool colok = exg >= 0 && exg < op.g.ex && eyg >= 0 && eyg < op.g.ey;
-  const T hx = colok ? T(op.xc[exg + 1] - op.xc[exg]) : T(1);
-  const T hy = colok ? T(op.yc[eyg + 1] - op.yc[eyg]) : T(1);
+  const T hx = colok ? op.hx[exg] : T(1), ihx = colok ? op.ihx[exg] : T(1);
+  const T hy = colok ? op.hy[eyg] : T(1), ihy = colok ? op.ihy[eyg] : T(1);
   const bool robin = colok && exg == 0 && op.ra != 0.0;
 
-  // plane loader: thread handles plane nodes q = tid and tid + 256
   const int q0 = tid, q1 = tid + TPB;
   const int lx0 = q0 % PLX, ly0 = q0 / PLX;
   const int lx1 = q1 % PLX, ly1 = q1 / PLX;
   const bool has1 = q1 < PLN;
 
-  // owner node of this thread
-  const int ox = tid & (TXE - 1), oy = tid / TXE;
+  const int ox = tx, oy = ty;  // owner node of this thread
   const bool own = ox < OWN_X && oy < OWN_Y;
   const int ngx = gx0 + 1 + ox, ngy = gy0 + 1 + oy;
   const bool owng = own && ngx < op.g.nax && ngy < op.g.nay;
+  const int oq = (oy + 1) * PLX + ox + 1;
 
-  // prologue: plane kz0-1 -> LDS buffer 0, plane kz0 -> registers
-  int kl = kz0 - 1;
-  {
-    pl1[0][q0] = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl);
-    if (has1) pl1[0][q1] = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl);
-    if (DUAL) {
-      pl2[0][q0] = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl);
-      if (has1) pl2[0][q1] = load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl);
-    }
-  }
-  T r0 = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl + 1);
-  T r1 = has1 ? load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl + 1) : T(0);
-  T s0 = T(0), s1 = T(0);
-  if (DUAL) {
-    s0 = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl + 1);
-    s1 = has1 ? load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl + 1) : T(0);
-  }
   auto evalid = [&](int k) { return colok && k >= 0 && k < op.g.ez; };
   auto eidx = [&](int k) {
     return exg + (int64_t)op.g.ex * (eyg + (int64_t)op.g.ey * k);
   };
-  Coef<T> cn = load_coef<T>(op, evalid(kl), eidx(kl));
+
+  int kl = kz0 - 1;
+  PlaneRegs<T, DUAL> pr;
+  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl, lx0, ly0, lx1, ly1, has1, pr);
+  plane_store<T, DUAL>(pr, pl1[0], DUAL ? pl2[0] : nullptr, pm[0], q0, q1, has1);
+  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 1, lx0, ly0, lx1, ly1, has1, pr);
+  Coef<T> cn = load_coef<T>(op, evalid(kl), eidx(kl), kl);
 
   double acc1 = 0.0, acc2 = 0.0, dot = 0.0;
   for (int it = 0; it <= zc; ++it, ++kl) {
     const int cb = it & 1, ct = cb ^ 1;
-    pl1[ct][q0] = r0;
-    if (has1) pl1[ct][q1] = r1;
-    if (DUAL) {
-      pl2[ct][q0] = s0;
-      if (has1) pl2[ct][q1] = s1;
-    }
+    plane_store<T, DUAL>(pr, pl1[ct], DUAL ? pl2[ct] : nullptr, pm[ct], q0, q1, has1);
     const Coef<T> cf = cn;
     const bool ev = evalid(kl);
-    const T hz = ev ? T(op.zc[kl + 1] - op.zc[kl]) : T(1);
-    if (it < zc) {  // prefetch plane kl+2 and layer kl+1 coefficients
-      r0 = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl + 2);
-      r1 = has1 ? load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl + 2) : T(0);
-      if (DUAL) {
-        s0 = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl + 2);
-        s1 = has1 ? load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl + 2) : T(0);
-      }
-      cn = load_coef<T>(op, evalid(kl + 1), eidx(kl + 1));
+    if (it < zc) {  // prefetch plane kl+2 and layer kl+1
+      plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
+                               has1, pr);
+      cn = load_coef<T>(op, evalid(kl + 1), eidx(kl + 1), kl + 1);
     }
-    __syncthreads();
-    // x2 at this thread's owned node of plane kl, read before the buffer can
-    // be refilled by a thread that runs ahead into the next layer
-    const double xo2 =
-        (DUAL && own) ? (double)pl2[DUAL ? cb : 0][(oy + 1) * PLX + ox + 1] : 0.0;
-    elem_step<T, DUAL>(op, ev, robin, hx, hy, hz, cf, pl1[cb], pl1[ct],
+    lds_barrier();
+    // this thread's owned node of plane kl, read before a faster thread can
+    // refill the buffer with plane kl+2
+    const uint8_t om = pm[cb][oq];
+    const double xo2 = (DUAL && own) ? (double)pl2[DUAL ? cb : 0][oq] : 0.0;
+    elem_step<T, DUAL>(op, ev, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
                        DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
                        ty, st1, DUAL ? st2 : nullptr, tid);
-    __syncthreads();
+    lds_barrier();
     if (own) {
-      const double b1 = node_sum<T>(st1, ox + 1, oy + 1, false);
-      const double v1 = acc1 + b1;
+      const double v1 = acc1 + node_sum<T>(st1, ox + 1, oy + 1, false);
       acc1 = node_sum<T>(st1, ox + 1, oy + 1, true);
       double v2 = 0.0;
       if (DUAL) {
-        const double b2 = node_sum<T>(st2, ox + 1, oy + 1, false);
-        v2 = acc2 + b2;
+        v2 = acc2 + node_sum<T>(st2, ox + 1, oy + 1, false);
         acc2 = node_sum<T>(st2, ox + 1, oy + 1, true);
       }
       if (owng && it >= 1 && kl < op.g.naz) {
         const int64_t n = ngx + (int64_t)op.g.nax * (ngy + (int64_t)op.g.nay * kl);
-        const bool m = op.mask[n] != 0;
-        const TIO o1 = m ? TIO(0) : TIO(v1);
+        const TIO o1 = om ? TIO(0) : TIO(v1);
         y1[n] = o1;
         if (DUAL) {
-          y2[n] = m ? TIO(0) : TIO(v2);
+          y2[n] = om ? TIO(0) : TIO(v2);
           if (dot_part) dot += (double)o1 * xo2;
         }
       }
@@ -269,6 +295,7 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
                                                 TIO *__restrict__ y2) {
   __shared__ T pl1[2][PLN];
   __shared__ T pl2[DUAL ? 2 : 1][DUAL ? PLN : 1];
+  __shared__ uint8_t pm[2][PLN];
   __shared__ T st1[8 * TPB];
   __shared__ T st2[DUAL ? 8 * TPB : 1];
 
@@ -288,8 +315,8 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
   const int gx0 = bx * TXE, gy0 = by * TYE;
   const int exg = gx0 + tx, eyg = gy0 + ty;
   const bool colok = exg < op.g.ex && eyg < op.g.ey;
-  const T hx = colok ? T(op.xc[exg + 1] - op.xc[exg]) : T(1);
-  const T hy = colok ? T(op.yc[eyg + 1] - op.yc[eyg]) : T(1);
+  const T hx = colok ? op.hx[exg] : T(1), ihx = colok ? op.ihx[exg] : T(1);
+  const T hy = colok ? op.hy[eyg] : T(1), ihy = colok ? op.ihy[eyg] : T(1);
   const bool robin = colok && exg == 0 && op.ra != 0.0;
 
   const int q0 = tid, q1 = tid + TPB;
@@ -300,14 +327,13 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
   auto eidx = [&](int k) {
     return exg + (int64_t)op.g.ex * (eyg + (int64_t)op.g.ey * k);
   };
-  // output of tile node (lx,ly) in plane kz: interior nodes are plain stores
-  auto emit = [&](TIO *y, int lx, int ly, int kz, double v, bool complete_z) {
+  // tile node (lx,ly) of plane kz: interior nodes are plain stores
+  auto emit = [&](TIO *y, int lx, int ly, int kz, double v, bool masked,
+                  bool complete_z) {
     const int gx = gx0 + lx, gy = gy0 + ly;
-    if (gx >= op.g.nax || gy >= op.g.nay || kz >= op.g.naz) return;
+    if (masked || gx >= op.g.nax || gy >= op.g.nay || kz >= op.g.naz) return;
     const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * kz);
-    if (op.mask[n]) return;  // stays zero
-    const bool inner = complete_z && lx >= 1 && lx < TXE && ly >= 1 && ly < TYE &&
-                       gx < op.g.nax - 0 && gy < op.g.nay;
+    const bool inner = complete_z && lx >= 1 && lx < TXE && ly >= 1 && ly < TYE;
     if (inner) {
       y[n] = TIO(v);
     } else if (ATOM) {
@@ -317,50 +343,51 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
     }
   };
 
-  pl1[0][q0] = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kz0);
-  if (has1) pl1[0][q1] = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kz0);
-  if (DUAL) {
-    pl2[0][q0] = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kz0);
-    if (has1) pl2[0][q1] = load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kz0);
-  }
+  PlaneRegs<T, DUAL> pr;
+  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kz0, lx0, ly0, lx1, ly1, has1, pr);
+  plane_store<T, DUAL>(pr, pl1[0], DUAL ? pl2[0] : nullptr, pm[0], q0, q1, has1);
+  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kz0 + 1, lx0, ly0, lx1, ly1, has1, pr);
+  Coef<T> cn = load_coef<T>(op, colok, eidx(kz0), kz0);
   double a1[2] = {0.0, 0.0}, a2[2] = {0.0, 0.0};
   for (int kl = kz0; kl < kz1; ++kl) {
     const int cb = (kl - kz0) & 1, ct = cb ^ 1;
-    pl1[ct][q0] = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, kl + 1);
-    if (has1) pl1[ct][q1] = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, kl + 1);
-    if (DUAL) {
-      pl2[ct][q0] = load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, kl + 1);
-      if (has1) pl2[ct][q1] = load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, kl + 1);
+    plane_store<T, DUAL>(pr, pl1[ct], DUAL ? pl2[ct] : nullptr, pm[ct], q0, q1, has1);
+    const Coef<T> cf = cn;
+    if (kl + 1 < kz1) {
+      plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
+                               has1, pr);
+      cn = load_coef<T>(op, colok, eidx(kl + 1), kl + 1);
     }
-    const Coef<T> cf = load_coef<T>(op, colok, eidx(kl));
-    const T hz = T(op.zc[kl + 1] - op.zc[kl]);
-    __syncthreads();
-    elem_step<T, DUAL>(op, colok, robin, hx, hy, hz, cf, pl1[cb], pl1[ct],
+    lds_barrier();
+    const uint8_t mk0 = pm[cb][q0], mk1 = has1 ? pm[cb][q1] : uint8_t(1);
+    elem_step<T, DUAL>(op, colok, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
                        DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
                        ty, st1, DUAL ? st2 : nullptr, tid);
-    __syncthreads();
+    lds_barrier();
     const bool zin = kl > kz0;  // plane kl has both layers inside the chunk
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
-      const int q = h ? q1 : q0;
       if (h && !has1) break;
       const int lx = h ? lx1 : lx0, ly = h ? ly1 : ly0;
-      emit(y1, lx, ly, kl, a1[h] + node_sum<T>(st1, lx, ly, false), zin);
+      const bool mk = h ? mk1 : mk0;
+      emit(y1, lx, ly, kl, a1[h] + node_sum<T>(st1, lx, ly, false), mk, zin);
       a1[h] = node_sum<T>(st1, lx, ly, true);
       if (DUAL) {
-        emit(y2, lx, ly, kl, a2[h] + node_sum<T>(st2, lx, ly, false), zin);
+        emit(y2, lx, ly, kl, a2[h] + node_sum<T>(st2, lx, ly, false), mk, zin);
         a2[h] = node_sum<T>(st2, lx, ly, true);
       }
-      (void)q;
     }
   }
-  // top plane of the chunk: shared with the next chunk
+  // top plane of the chunk (shared with the next chunk); its flags are in the
+  // buffer written last, which no thread refills
+  const int cl = (kz1 - kz0) & 1;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
     if (h && !has1) break;
     const int lx = h ? lx1 : lx0, ly = h ? ly1 : ly0;
-    emit(y1, lx, ly, kz1, a1[h], false);
-    if (DUAL) emit(y2, lx, ly, kz1, a2[h], false);
+    const bool mk = pm[cl][h ? q1 : q0] != 0;
+    emit(y1, lx, ly, kz1, a1[h], mk, false);
+    if (DUAL) emit(y2, lx, ly, kz1, a2[h], mk, false);
   }
 }
 
@@ -524,10 +551,9 @@ int launch_tau(const Grid3 &g, const double *xc, const double *yc,
 __device__ double elem_entry(const OpData<double> &op, int ix, int iy, int iz,
                              int a) {
   const int64_t e = ix + (int64_t)op.g.ex * (iy + (int64_t)op.g.ey * iz);
-  const double hx = op.xc[ix + 1] - op.xc[ix], hy = op.yc[iy + 1] - op.yc[iy],
-               hz = op.zc[iz + 1] - op.zc[iz];
   ElemConst<double> k;
-  elem_setup<double>(hx, hy, hz, op.lam[e], op.vx[e], op.vy[e], op.vz[e],
+  elem_setup<double>(op.hx[ix], op.hy[iy], op.hz[iz], op.ihx[ix], op.ihy[iy],
+                     op.ihz[iz], op.lam[e], op.vx[e], op.vy[e], op.vz[e],
                      op.tau[e], op.cH, op.mu, k);
   double x[8], xh[8], yh[8], y[8];
   for (int c = 0; c < 8; ++c) x[c] = (c == a) ? 1.0 : 0.0;
@@ -535,8 +561,8 @@ __device__ double elem_entry(const OpData<double> &op, int ix, int iy, int iz,
   haar_apply<double, false>(k, xh, yh);
   haar_inv<double>(yh, y);
   if (ix == 0 && op.ra != 0.0)
-    robin_add<double, false>(k, hx, op.tau[e], op.vx[e], op.vy[e], op.vz[e],
-                             op.ra, x, y);
+    robin_add<double, false>(k, op.hx[ix], op.tau[e], op.vx[e], op.vy[e],
+                             op.vz[e], op.ra, x, y);
   return y[a];
 }
 
