@@ -109,9 +109,10 @@ int vgpu_set_coeffs(vgpu_ctx *ctx, const double *lam, const double *vel,
                     double dt);
 /* Change only the scheme / dt (e.g. steady vs implicit Euler). */
 int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
-/* Options: scatter mode for the solver and z-chunk of the owner tiles
- * (0 = library default). */
-int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk);
+/* Options: scatter mode for the solver, z-chunk of the z-marching tiles
+ * (0 = keep) and owner tile shape (0 = 32x8, 1 = 16x16, 2 = 64x4 element
+ * columns; -1 = keep). */
+int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk, int tile);
 
 /* y = (I-D) A (I-D) x  (transpose=1: A^T), host vectors of length N.
  * precision VGPU_FP64 or VGPU_FP32 (x,y still FP64 on the host). */

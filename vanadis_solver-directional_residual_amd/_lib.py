@@ -75,7 +75,7 @@ def load():
     L.vgpu_set_coeffs.argtypes = [_vp, _dp, _dp, C.c_double, C.c_double, C.c_int, C.c_int,
                                   C.c_double]
     L.vgpu_set_scheme.argtypes = [_vp, C.c_int, C.c_double]
-    L.vgpu_set_options.argtypes = [_vp, C.c_int, C.c_int]
+    L.vgpu_set_options.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
     L.vgpu_apply.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
     L.vgpu_apply_dev.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp]
     L.vgpu_diag.argtypes = [_vp, _dp]
@@ -180,8 +180,9 @@ class Context:
     def set_scheme(self, scheme, dt):
         check(load().vgpu_set_scheme(self.h, int(scheme), float(dt)), "set_scheme")
 
-    def set_options(self, scatter="owner", zchunk=0):
-        check(load().vgpu_set_options(self.h, scatter_code(scatter), int(zchunk)), "set_options")
+    def set_options(self, scatter="owner", zchunk=0, tile=-1):
+        check(load().vgpu_set_options(self.h, scatter_code(scatter), int(zchunk), int(tile)),
+              "set_options")
 
     def apply(self, x, scatter="owner", precision=FP64, transpose=False):
         y = np.zeros(self.N)

@@ -1,5 +1,6 @@
 // C ABI of libvanadis_gpu.so: context lifecycle, coefficients, operator
 // apply, RHS/diagonal assembly, FP64 and mixed-precision solves, timing.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -96,7 +97,7 @@ struct vgpu_ctx {
   double P = 0, alpha = 0, dt = 1;
   int variant = VGPU_DR_V1, scheme = VGPU_SCHEME_STEADY;
   double cH = 1, cC = 0;
-  int scatter = VGPU_SCATTER_OWNER, zchunk = 8;
+  int scatter = VGPU_SCATTER_OWNER, zchunk = 8, tile = 0;
   double *diag = nullptr;
   float *fdiag = nullptr;
   bool diag_valid = false;
@@ -130,6 +131,7 @@ struct vgpu_ctx {
     o.mu = cH_ * P + cC_;
     o.ra = cH_ * alpha;
     o.done = nullptr;
+    o.tile = tile;
     return o;
   }
   OpData<float> op32() const {
@@ -154,6 +156,7 @@ struct vgpu_ctx {
     o.mu = cH * P + cC;
     o.ra = cH * alpha;
     o.done = nullptr;
+    o.tile = tile;
     return o;
   }
 };
@@ -215,6 +218,13 @@ double host_sum(const double *dev, int n, hipStream_t s, int *rc) {
   return t;
 }
 
+// capacity of owner-mode dot partials over every tile shape and z chunk
+int max_owner_tiles(const Grid3 &g) {
+  int m = 0;
+  for (int t = 0; t < 3; ++t) m = std::max(m, owner_num_tiles(g, 1, t));
+  return m;
+}
+
 int nparts_for(int64_t n) {
   return (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));
 }
@@ -226,7 +236,7 @@ DualOp<double> make_dual64(vgpu_ctx *c, int scatter) {
     OpData<double> op = c->op64(c->cH, c->cC);
     op.done = done;
     if (scatter == VGPU_SCATTER_OWNER) {
-      *na = owner_num_tiles(c->g, c->zchunk);
+      *na = owner_num_tiles(c->g, c->zchunk, c->tile);
       return launch_apply<double, double>(op, scatter, c->zchunk, p, z, pp, zz,
                                           pa, s);
     }
@@ -250,7 +260,7 @@ DualOp<float> make_dual32(vgpu_ctx *c, int scatter) {
     OpData<float> op = c->op32();
     op.done = done;
     if (scatter == VGPU_SCATTER_OWNER) {
-      *na = owner_num_tiles(c->g, c->zchunk);
+      *na = owner_num_tiles(c->g, c->zchunk, c->tile);
       return launch_apply<float, float>(op, scatter, c->zchunk, p, z, pp, zz,
                                         pa, s);
     }
@@ -271,7 +281,7 @@ int solve_fp64_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                    int itmax, int *iters, double *err, int *status) {
   const int scatter = c->scatter;
   if (c->bicg64.n != c->g.N) {
-    int rc = c->bicg64.alloc(c->g.N, owner_num_tiles(c->g, 1) + 256);
+    int rc = c->bicg64.alloc(c->g.N, max_owner_tiles(c->g) + 256);
     if (rc) return rc;
   }
   DualOp<double> dual = make_dual64(c, scatter);
@@ -293,7 +303,7 @@ int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
   const int64_t N = c->g.N;
   const int scatter = c->scatter;
   if (c->bicg32.n != N) {
-    int rc = c->bicg32.alloc(N, owner_num_tiles(c->g, 1) + 256);
+    int rc = c->bicg32.alloc(N, max_owner_tiles(c->g) + 256);
     if (rc) return rc;
   }
   if (!c->fb) {
@@ -330,7 +340,7 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
   const int64_t N = c->g.N;
   const int scatter = c->scatter;
   if (c->bicg32.n != N) {
-    int rc = c->bicg32.alloc(N, owner_num_tiles(c->g, 1) + 256);
+    int rc = c->bicg32.alloc(N, max_owner_tiles(c->g) + 256);
     if (rc) return rc;
   }
   if (!c->fb) {
@@ -657,13 +667,14 @@ int vgpu_set_scheme(vgpu_ctx *c, int scheme, double dt) {
   return VGPU_OK;
 }
 
-int vgpu_set_options(vgpu_ctx *c, int scatter, int zchunk) {
-  if (!c || scatter < 0 || scatter > 2 || zchunk < 0 || zchunk > 256) {
+int vgpu_set_options(vgpu_ctx *c, int scatter, int zchunk, int tile) {
+  if (!c || scatter < 0 || scatter > 2 || zchunk < 0 || zchunk > 256 || tile > 2) {
     set_error("vgpu_set_options: bad arguments");
     return VGPU_ERR_ARG;
   }
   c->scatter = scatter;
   if (zchunk > 0) c->zchunk = zchunk;
+  if (tile >= 0) c->tile = tile;
   c->version++;
   return VGPU_OK;
 }
@@ -963,7 +974,7 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   };
   // red holds 4096 partials: enough for owner tiles at zc >= 1 up to ~8M nodes
   if (what == 1 && scatter == VGPU_SCATTER_OWNER &&
-      owner_num_tiles(c->g, c->zchunk) > 4096) {
+      owner_num_tiles(c->g, c->zchunk, c->tile) > 4096) {
     set_error("vgpu_time_kernel: mesh too large for the timing partial buffer");
     return VGPU_ERR_ARG;
   }

@@ -65,7 +65,7 @@ __device__ __forceinline__ T load_node(const OpData<T> &op, const TIO *x, int gx
 
 // Element apply for one thread: gathers from the two LDS planes, writes the 8
 // results (and 8 transposed results when DUAL) to the staging arrays.
-template <typename T, bool DUAL>
+template <typename T, bool DUAL, int TX, int TY>
 __device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
                                           bool robin, T hx, T hy, T ihx, T ihy,
                                           const Coef<T> &cf, const T *pb,
@@ -77,16 +77,16 @@ __device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
     ElemConst<T> k;
     elem_setup<T>(hx, hy, cf.hz, ihx, ihy, cf.ihz, cf.lam, cf.vx, cf.vy, cf.vz,
                   cf.tau, T(op.cH), T(op.mu), k);
-    const int i00 = ty * PLX + tx;
+    const int i00 = ty * (TX + 1) + tx;
     T x[8], xh[8], yh[8];
     x[0] = pb[i00];
     x[1] = pb[i00 + 1];
-    x[2] = pb[i00 + PLX + 1];
-    x[3] = pb[i00 + PLX];
+    x[2] = pb[i00 + (TX + 1) + 1];
+    x[3] = pb[i00 + (TX + 1)];
     x[4] = pt[i00];
     x[5] = pt[i00 + 1];
-    x[6] = pt[i00 + PLX + 1];
-    x[7] = pt[i00 + PLX];
+    x[6] = pt[i00 + (TX + 1) + 1];
+    x[7] = pt[i00 + (TX + 1)];
     haar_fwd<T>(x, xh);
     haar_apply<T, false>(k, xh, yh);
     haar_inv<T>(yh, y);
@@ -94,12 +94,12 @@ __device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
     if (DUAL) {
       x[0] = pb2[i00];
       x[1] = pb2[i00 + 1];
-      x[2] = pb2[i00 + PLX + 1];
-      x[3] = pb2[i00 + PLX];
+      x[2] = pb2[i00 + (TX + 1) + 1];
+      x[3] = pb2[i00 + (TX + 1)];
       x[4] = pt2[i00];
       x[5] = pt2[i00 + 1];
-      x[6] = pt2[i00 + PLX + 1];
-      x[7] = pt2[i00 + PLX];
+      x[6] = pt2[i00 + (TX + 1) + 1];
+      x[7] = pt2[i00 + (TX + 1)];
       haar_fwd<T>(x, xh);
       haar_apply<T, true>(k, xh, yh);
       haar_inv<T>(yh, y2);
@@ -110,25 +110,25 @@ __device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
     for (int c = 0; c < 8; ++c) y[c] = y2[c] = T(0);
   }
 #pragma unroll
-  for (int c = 0; c < 8; ++c) stage[c * TPB + tid] = y[c];
+  for (int c = 0; c < 8; ++c) stage[c * (TX * TY) + tid] = y[c];
   if (DUAL) {
 #pragma unroll
-    for (int c = 0; c < 8; ++c) stage2[c * TPB + tid] = y2[c];
+    for (int c = 0; c < 8; ++c) stage2[c * (TX * TY) + tid] = y2[c];
   }
 }
 
 // Sum of the four in-plane contributions to tile node (lx, ly); `top` selects
 // the upper face (local nodes 4..7) of the layer.  Elements outside the tile
 // contribute nothing.  Fixed order -> deterministic.
-template <typename T>
+template <typename T, int TX, int TY>
 __device__ __forceinline__ double node_sum(const T *stage, int lx, int ly,
                                            bool top) {
   const int o = top ? 4 : 0;
   double s = 0.0;
-  if (lx >= 1 && ly >= 1) s += (double)stage[(o + 2) * TPB + (ly - 1) * TXE + lx - 1];
-  if (lx < TXE && ly >= 1) s += (double)stage[(o + 3) * TPB + (ly - 1) * TXE + lx];
-  if (lx >= 1 && ly < TYE) s += (double)stage[(o + 1) * TPB + ly * TXE + lx - 1];
-  if (lx < TXE && ly < TYE) s += (double)stage[(o + 0) * TPB + ly * TXE + lx];
+  if (lx >= 1 && ly >= 1) s += (double)stage[(o + 2) * (TX * TY) + (ly - 1) * TX + lx - 1];
+  if (lx < TX && ly >= 1) s += (double)stage[(o + 3) * (TX * TY) + (ly - 1) * TX + lx];
+  if (lx >= 1 && ly < TY) s += (double)stage[(o + 1) * (TX * TY) + ly * TX + lx - 1];
+  if (lx < TX && ly < TY) s += (double)stage[(o + 0) * (TX * TY) + ly * TX + lx];
   return s;
 }
 
@@ -178,25 +178,26 @@ __device__ __forceinline__ void plane_store(const PlaneRegs<T, DUAL> &r, T *p1,
 // element layers zc bz - 1 .. zc bz + zc - 1.  The next node plane, the next
 // layer's fields and sizes are loaded one layer ahead.
 // ---------------------------------------------------------------------------
-template <typename T, typename TIO, bool DUAL>
-__global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
+template <typename T, typename TIO, bool DUAL, int TX, int TY>
+__global__ __launch_bounds__(TX * TY)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k_owner(OpData<T> op, int zc,
                                                 const TIO *__restrict__ x1,
                                                 TIO *__restrict__ y1,
                                                 const TIO *__restrict__ x2,
                                                 TIO *__restrict__ y2,
                                                 double *__restrict__ dot_part) {
-  __shared__ T pl1[2][PLN];
-  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? PLN : 1];
-  __shared__ uint8_t pm[2][PLN];
-  __shared__ T st1[8 * TPB];
-  __shared__ T st2[DUAL ? 8 * TPB : 1];
-  __shared__ double red[TPB / 64];
+  __shared__ T pl1[2][((TX + 1) * (TY + 1))];
+  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? ((TX + 1) * (TY + 1)) : 1];
+  __shared__ uint8_t pm[2][((TX + 1) * (TY + 1))];
+  __shared__ T st1[8 * (TX * TY)];
+  __shared__ T st2[DUAL ? 8 * (TX * TY) : 1];
+  __shared__ double red[(TX * TY) / 64];
   if (op.done && *op.done) return;
 
   const int tid = threadIdx.x;
-  const int tx = tid & (TXE - 1), ty = tid / TXE;
-  const int gx0 = blockIdx.x * OWN_X - 1;  // tile node/element origin
-  const int gy0 = blockIdx.y * OWN_Y - 1;
+  const int tx = tid & (TX - 1), ty = tid / TX;
+  const int gx0 = blockIdx.x * (TX - 1) - 1;  // tile node/element origin
+  const int gy0 = blockIdx.y * (TY - 1) - 1;
   const int kz0 = blockIdx.z * zc;          // first owned node plane
   const int exg = gx0 + tx, eyg = gy0 + ty;
   const bool colok = exg >= 0 && exg < op.g.ex && eyg >= 0 && eyg < op.g.ey;
@@ -204,16 +205,16 @@ __global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
   const T hy = colok ? op.hy[eyg] : T(1), ihy = colok ? op.ihy[eyg] : T(1);
   const bool robin = colok && exg == 0 && op.ra != 0.0;
 
-  const int q0 = tid, q1 = tid + TPB;
-  const int lx0 = q0 % PLX, ly0 = q0 / PLX;
-  const int lx1 = q1 % PLX, ly1 = q1 / PLX;
-  const bool has1 = q1 < PLN;
+  const int q0 = tid, q1 = tid + (TX * TY);
+  const int lx0 = q0 % (TX + 1), ly0 = q0 / (TX + 1);
+  const int lx1 = q1 % (TX + 1), ly1 = q1 / (TX + 1);
+  const bool has1 = q1 < ((TX + 1) * (TY + 1));
 
   const int ox = tx, oy = ty;  // owner node of this thread
-  const bool own = ox < OWN_X && oy < OWN_Y;
+  const bool own = ox < (TX - 1) && oy < (TY - 1);
   const int ngx = gx0 + 1 + ox, ngy = gy0 + 1 + oy;
   const bool owng = own && ngx < op.g.nax && ngy < op.g.nay;
-  const int oq = (oy + 1) * PLX + ox + 1;
+  const int oq = (oy + 1) * (TX + 1) + ox + 1;
 
   auto evalid = [&](int k) { return colok && k >= 0 && k < op.g.ez; };
   auto eidx = [&](int k) {
@@ -243,17 +244,17 @@ __global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
     // refill the buffer with plane kl+2
     const uint8_t om = pm[cb][oq];
     const double xo2 = (DUAL && own) ? (double)pl2[DUAL ? cb : 0][oq] : 0.0;
-    elem_step<T, DUAL>(op, ev, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
+    elem_step<T, DUAL, TX, TY>(op, ev, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
                        DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
                        ty, st1, DUAL ? st2 : nullptr, tid);
     lds_barrier();
     if (own) {
-      const double v1 = acc1 + node_sum<T>(st1, ox + 1, oy + 1, false);
-      acc1 = node_sum<T>(st1, ox + 1, oy + 1, true);
+      const double v1 = acc1 + node_sum<T, TX, TY>(st1, ox + 1, oy + 1, false);
+      acc1 = node_sum<T, TX, TY>(st1, ox + 1, oy + 1, true);
       double v2 = 0.0;
       if (DUAL) {
-        v2 = acc2 + node_sum<T>(st2, ox + 1, oy + 1, false);
-        acc2 = node_sum<T>(st2, ox + 1, oy + 1, true);
+        v2 = acc2 + node_sum<T, TX, TY>(st2, ox + 1, oy + 1, false);
+        acc2 = node_sum<T, TX, TY>(st2, ox + 1, oy + 1, true);
       }
       if (owng && it >= 1 && kl < op.g.naz) {
         const int64_t n = ngx + (int64_t)op.g.nax * (ngy + (int64_t)op.g.nay * kl);
@@ -273,7 +274,7 @@ __global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
     __syncthreads();
     if (tid == 0) {
       double s = 0.0;
-      for (int w = 0; w < TPB / 64; ++w) s += red[w];
+      for (int w = 0; w < (TX * TY) / 64; ++w) s += red[w];
       const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
       dot_part[b] = s;
     }
@@ -287,17 +288,17 @@ __global__ __launch_bounds__(TPB) void k_owner(OpData<T> op, int zc,
 // for ATOMIC).  The output must be zero on entry.  COLORED runs one launch
 // per colour: tile parity (bx&1) | (by&1)<<1 | (bz&1)<<2 == color.
 // ---------------------------------------------------------------------------
-template <typename T, typename TIO, bool DUAL, bool ATOM>
-__global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
+template <typename T, typename TIO, bool DUAL, bool ATOM, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int color,
                                                 const TIO *__restrict__ x1,
                                                 TIO *__restrict__ y1,
                                                 const TIO *__restrict__ x2,
                                                 TIO *__restrict__ y2) {
-  __shared__ T pl1[2][PLN];
-  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? PLN : 1];
-  __shared__ uint8_t pm[2][PLN];
-  __shared__ T st1[8 * TPB];
-  __shared__ T st2[DUAL ? 8 * TPB : 1];
+  __shared__ T pl1[2][((TX + 1) * (TY + 1))];
+  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? ((TX + 1) * (TY + 1)) : 1];
+  __shared__ uint8_t pm[2][((TX + 1) * (TY + 1))];
+  __shared__ T st1[8 * (TX * TY)];
+  __shared__ T st2[DUAL ? 8 * (TX * TY) : 1];
 
   if (op.done && *op.done) return;
   int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
@@ -307,22 +308,22 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
     bz = 2 * bz + ((color >> 2) & 1);
   }
   const int kz0 = bz * zc;
-  if (bx * TXE >= op.g.ex || by * TYE >= op.g.ey || kz0 >= op.g.ez) return;
+  if (bx * TX >= op.g.ex || by * TY >= op.g.ey || kz0 >= op.g.ez) return;
   const int kz1 = min(kz0 + zc, op.g.ez);  // layers [kz0, kz1)
 
   const int tid = threadIdx.x;
-  const int tx = tid & (TXE - 1), ty = tid / TXE;
-  const int gx0 = bx * TXE, gy0 = by * TYE;
+  const int tx = tid & (TX - 1), ty = tid / TX;
+  const int gx0 = bx * TX, gy0 = by * TY;
   const int exg = gx0 + tx, eyg = gy0 + ty;
   const bool colok = exg < op.g.ex && eyg < op.g.ey;
   const T hx = colok ? op.hx[exg] : T(1), ihx = colok ? op.ihx[exg] : T(1);
   const T hy = colok ? op.hy[eyg] : T(1), ihy = colok ? op.ihy[eyg] : T(1);
   const bool robin = colok && exg == 0 && op.ra != 0.0;
 
-  const int q0 = tid, q1 = tid + TPB;
-  const int lx0 = q0 % PLX, ly0 = q0 / PLX;
-  const int lx1 = q1 % PLX, ly1 = q1 / PLX;
-  const bool has1 = q1 < PLN;
+  const int q0 = tid, q1 = tid + (TX * TY);
+  const int lx0 = q0 % (TX + 1), ly0 = q0 / (TX + 1);
+  const int lx1 = q1 % (TX + 1), ly1 = q1 / (TX + 1);
+  const bool has1 = q1 < ((TX + 1) * (TY + 1));
 
   auto eidx = [&](int k) {
     return exg + (int64_t)op.g.ex * (eyg + (int64_t)op.g.ey * k);
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
     const int gx = gx0 + lx, gy = gy0 + ly;
     if (masked || gx >= op.g.nax || gy >= op.g.nay || kz >= op.g.naz) return;
     const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * kz);
-    const bool inner = complete_z && lx >= 1 && lx < TXE && ly >= 1 && ly < TYE;
+    const bool inner = complete_z && lx >= 1 && lx < TX && ly >= 1 && ly < TY;
     if (inner) {
       y[n] = TIO(v);
     } else if (ATOM) {
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
     }
     lds_barrier();
     const uint8_t mk0 = pm[cb][q0], mk1 = has1 ? pm[cb][q1] : uint8_t(1);
-    elem_step<T, DUAL>(op, colok, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
+    elem_step<T, DUAL, TX, TY>(op, colok, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
                        DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
                        ty, st1, DUAL ? st2 : nullptr, tid);
     lds_barrier();
@@ -370,11 +371,11 @@ __global__ __launch_bounds__(TPB) void k_brick(OpData<T> op, int zc, int color,
       if (h && !has1) break;
       const int lx = h ? lx1 : lx0, ly = h ? ly1 : ly0;
       const bool mk = h ? mk1 : mk0;
-      emit(y1, lx, ly, kl, a1[h] + node_sum<T>(st1, lx, ly, false), mk, zin);
-      a1[h] = node_sum<T>(st1, lx, ly, true);
+      emit(y1, lx, ly, kl, a1[h] + node_sum<T, TX, TY>(st1, lx, ly, false), mk, zin);
+      a1[h] = node_sum<T, TX, TY>(st1, lx, ly, true);
       if (DUAL) {
-        emit(y2, lx, ly, kl, a2[h] + node_sum<T>(st2, lx, ly, false), mk, zin);
-        a2[h] = node_sum<T>(st2, lx, ly, true);
+        emit(y2, lx, ly, kl, a2[h] + node_sum<T, TX, TY>(st2, lx, ly, false), mk, zin);
+        a2[h] = node_sum<T, TX, TY>(st2, lx, ly, true);
       }
     }
   }
@@ -403,11 +404,39 @@ __global__ void k_zero2(const int *done, TIO *a, TIO *b, int64_t n) {
 
 }  // namespace
 
-int owner_num_tiles(const Grid3 &g, int zc) {
-  const int nx = (g.nax + OWN_X - 1) / OWN_X, ny = (g.nay + OWN_Y - 1) / OWN_Y;
+// Owner tile shapes (element columns per workgroup): 0 = 32x8, 1 = 16x16,
+// 2 = 64x4; each owns (TX-1) x (TY-1) node columns.
+static void owner_shape(int tile, int *tx, int *ty) {
+  switch (tile) {
+    case 1: *tx = 16; *ty = 16; break;
+    case 2: *tx = 64; *ty = 4; break;
+    default: *tx = 32; *ty = 8; break;
+  }
+}
+
+int owner_num_tiles(const Grid3 &g, int zc, int tile) {
+  int tx, ty;
+  owner_shape(tile, &tx, &ty);
+  const int nx = (g.nax + tx - 2) / (tx - 1), ny = (g.nay + ty - 2) / (ty - 1);
   const int nz = (g.naz + zc - 1) / zc;
   return nx * ny * nz;
 }
+
+namespace {
+template <typename T, typename TIO, int TX, int TY>
+void launch_owner(const OpData<T> &op, int zc, const TIO *x1, TIO *y1,
+                  const TIO *x2, TIO *y2, double *dot_part, hipStream_t s) {
+  const Grid3 &g = op.g;
+  dim3 grid((g.nax + TX - 2) / (TX - 1), (g.nay + TY - 2) / (TY - 1),
+            (g.naz + zc - 1) / zc);
+  if (x2)
+    hipLaunchKernelGGL((k_owner<T, TIO, true, TX, TY>), grid, dim3(TX * TY), 0, s,
+                       op, zc, x1, y1, x2, y2, dot_part);
+  else
+    hipLaunchKernelGGL((k_owner<T, TIO, false, TX, TY>), grid, dim3(TX * TY), 0,
+                       s, op, zc, x1, y1, x2, y2, nullptr);
+}
+}  // namespace
 
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
@@ -417,39 +446,37 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
   const bool dual = x2 != nullptr;
   const Grid3 &g = op.g;
   if (scatter == VGPU_SCATTER_OWNER) {
-    dim3 grid((g.nax + OWN_X - 1) / OWN_X, (g.nay + OWN_Y - 1) / OWN_Y,
-              (g.naz + zc - 1) / zc);
-    if (dual)
-      hipLaunchKernelGGL((k_owner<T, TIO, true>), grid, dim3(TPB), 0, s, op, zc,
-                         x1, y1, x2, y2, dot_part);
-    else
-      hipLaunchKernelGGL((k_owner<T, TIO, false>), grid, dim3(TPB), 0, s, op, zc,
-                         x1, y1, x2, y2, nullptr);
+    switch (op.tile) {
+      case 1: launch_owner<T, TIO, 16, 16>(op, zc, x1, y1, x2, y2, dot_part, s); break;
+      case 2: launch_owner<T, TIO, 64, 4>(op, zc, x1, y1, x2, y2, dot_part, s); break;
+      default: launch_owner<T, TIO, 32, 8>(op, zc, x1, y1, x2, y2, dot_part, s); break;
+    }
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
   }
-  // brick modes need zeroed outputs
+  // brick modes need zeroed outputs; bricks are 32 x 8 element columns
+  constexpr int BX = 32, BY = 8;
   hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
                      dual ? y2 : nullptr, g.N);
-  const int ntx = (g.ex + TXE - 1) / TXE, nty = (g.ey + TYE - 1) / TYE;
+  const int ntx = (g.ex + BX - 1) / BX, nty = (g.ey + BY - 1) / BY;
   const int ntz = (g.ez + zc - 1) / zc;
   if (scatter == VGPU_SCATTER_ATOMIC) {
     dim3 grid(ntx, nty, ntz);
     if (dual)
-      hipLaunchKernelGGL((k_brick<T, TIO, true, true>), grid, dim3(TPB), 0, s,
-                         op, zc, 0, x1, y1, x2, y2);
+      hipLaunchKernelGGL((k_brick<T, TIO, true, true, BX, BY>), grid, dim3(BX * BY),
+                         0, s, op, zc, 0, x1, y1, x2, y2);
     else
-      hipLaunchKernelGGL((k_brick<T, TIO, false, true>), grid, dim3(TPB), 0, s,
-                         op, zc, 0, x1, y1, x2, y2);
+      hipLaunchKernelGGL((k_brick<T, TIO, false, true, BX, BY>), grid,
+                         dim3(BX * BY), 0, s, op, zc, 0, x1, y1, x2, y2);
   } else if (scatter == VGPU_SCATTER_COLORED) {
     dim3 grid((ntx + 1) / 2, (nty + 1) / 2, (ntz + 1) / 2);
     for (int c = 0; c < 8; ++c) {
       if (dual)
-        hipLaunchKernelGGL((k_brick<T, TIO, true, false>), grid, dim3(TPB), 0, s,
-                           op, zc, c, x1, y1, x2, y2);
+        hipLaunchKernelGGL((k_brick<T, TIO, true, false, BX, BY>), grid,
+                           dim3(BX * BY), 0, s, op, zc, c, x1, y1, x2, y2);
       else
-        hipLaunchKernelGGL((k_brick<T, TIO, false, false>), grid, dim3(TPB), 0,
-                           s, op, zc, c, x1, y1, x2, y2);
+        hipLaunchKernelGGL((k_brick<T, TIO, false, false, BX, BY>), grid,
+                           dim3(BX * BY), 0, s, op, zc, c, x1, y1, x2, y2);
     }
   } else {
     set_error("unknown scatter mode " + std::to_string(scatter));

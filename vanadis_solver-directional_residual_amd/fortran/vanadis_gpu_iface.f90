@@ -42,11 +42,11 @@ module vanadis_gpu
       integer(c_int), value :: variant, scheme
     end function
 
-    integer(c_int) function vgpu_set_options(ctx, scatter, zchunk) &
+    integer(c_int) function vgpu_set_options(ctx, scatter, zchunk, tile) &
         bind(C, name="vgpu_set_options")
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
-      integer(c_int), value :: scatter, zchunk
+      integer(c_int), value :: scatter, zchunk, tile
     end function
 
     integer(c_int) function vgpu_assemble_rhs(ctx, q_in, t_old, b) &
