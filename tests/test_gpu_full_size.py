@@ -1,0 +1,67 @@
+"""Parity at the benchmark size (128x128x64 elements, 1,081,665 nodes): the
+operator against the oracle's matvec_0/mttvec_0 on oracle-assembled element
+matrices, and the BiCG solution against the oracle's own solve (summary in
+tests/golden/oracle_c2.json, made by tests/golden/make_oracle_c2.py)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.fixture(scope="module")
+def c2(pkg, po, gpu):
+    p = pkg.problem.config_problem("c2")
+    bb, b, d = po.assemble(p)
+    ctx = pkg.Context.from_problem(p)
+    yield p, bb, ctx
+    ctx.close()
+
+
+@pytest.mark.parametrize("scatter,tile", [("owner", 0), ("owner", 1), ("owner", 2),
+                                          ("atomic", 0), ("colored", 0)])
+def test_apply_full_size(c2, po, scatter, tile):
+    p, bb, ctx = c2
+    print(f"full-size apply {scatter} tile {tile}", flush=True)
+    ctx.set_options(scatter, 8, tile)
+    x = np.random.default_rng(21).standard_normal(p.N)
+    for tr in (False, True):
+        ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+        y = ctx.apply(x, scatter=scatter, transpose=tr)
+        r = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+        assert r <= 1e-13, (tr, r)
+    ctx.set_options("owner", 8, 0)
+
+
+@pytest.mark.parametrize("scatter", ["owner", "atomic", "colored"])
+def test_solve_full_size_matches_oracle(c2, pkg, po, scatter):
+    p, bb, ctx = c2
+    g = json.load(open(os.path.join(HERE, "golden", "oracle_c2.json")))
+    print(f"full-size solve {scatter}", flush=True)
+    ctx.set_options(scatter, 8, 0)
+    b = ctx.assemble_rhs(p.q_in)
+    assert np.linalg.norm(b) == pytest.approx(g["b_norm2"], rel=1e-12)
+    x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=5000)
+    ctx.set_options("owner", 8, 0)
+    print(f"  iterations gpu {it} oracle {g['iterations']}", flush=True)
+    assert err <= 1e-8
+    # same algorithm, different summation order: iteration counts within 20 %
+    assert abs(it - g["iterations"]) <= 0.2 * g["iterations"]
+    # both iterates meet rtol 1e-8 (true residuals ~1e-8); they agree to ~5e-10
+    assert ctx.true_residual(b, x) <= 2e-8
+    assert np.linalg.norm(x) == pytest.approx(g["x_norm2"], rel=1e-8)
+    assert ctx.mass(x) == pytest.approx(g["mass"], rel=1e-8)
+    assert x.sum() == pytest.approx(g["x_sum"], rel=1e-8)
+
+
+def test_mixed_full_size(c2, pkg):
+    p, bb, ctx = c2
+    g = json.load(open(os.path.join(HERE, "golden", "oracle_c2.json")))
+    b = ctx.assemble_rhs(p.q_in)
+    x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=20000, precision=pkg.MIXED)
+    print(f"  mixed inner iterations {it}, true rel residual {err:.2e}", flush=True)
+    assert err <= 1e-8 and ctx.true_residual(b, x) <= 1e-8
+    assert np.linalg.norm(x) == pytest.approx(g["x_norm2"], rel=1e-8)

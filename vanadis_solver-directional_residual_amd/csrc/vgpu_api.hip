@@ -333,6 +333,9 @@ int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
   return VGPU_OK;
 }
 
+// Inner iterations allowed per refinement step of the mixed solve.
+constexpr int kInnerCap = 1000;
+
 // Mixed precision: FP64 true residual + refinement, FP32 inner BiCG
 int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                     int itmax, double inner_rtol, int *iters, double *err,
@@ -388,8 +391,7 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
       *status = 3;
       break;
     }
-    if (total >= itmax) break;
-    if (outer > 0 && rel > 0.5 * prev_rel) {
+    if (outer > 0 && (rel > 0.5 * prev_rel || st_inner >= 3)) {
       // the FP32 inner solve stopped reducing the true residual: finish in
       // FP64 BiCG from the current iterate (the fallback of the mixed mode)
       int it64 = 0, st64 = 0;
@@ -402,20 +404,21 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
       break;
     }
     prev_rel = rel;
+    if (total >= itmax) break;
     hipLaunchKernelGGL(k_scale_to_f32, grid_for(N), dim3(256), 0, c->stream, N,
                        c->vt, 1.0 / rn, c->fb, c->fx);
     VGPU_HIP(hipGetLastError());
     int it = 0;
     double e = 0;
+    // inner FP32 BiCG on the scaled residual: capped, and stopped if its
+    // recursive residual grows 100-fold (FP32 BiCG loses its recurrences
+    // after large residual peaks); a stopped inner solve is not applied
     rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
-                         inner_rtol, itmax, scatter != VGPU_SCATTER_OWNER,
-                         c->stream, &it, &e, &st_inner);
+                         inner_rtol, kInnerCap, scatter != VGPU_SCATTER_OWNER,
+                         c->stream, &it, &e, &st_inner, 1e2);
     if (rc) return rc;
     total += it;
-    if (st_inner == 3 && it == 0) {
-      *status = 3;
-      break;
-    }
+    if (st_inner >= 3) continue;  // next pass falls back to FP64
     hipLaunchKernelGGL(k_refine, grid_for(N), dim3(256), 0, c->stream, N, x, rn,
                        c->fx);
     VGPU_HIP(hipGetLastError());

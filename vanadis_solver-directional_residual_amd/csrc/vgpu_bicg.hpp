@@ -25,7 +25,7 @@
 namespace vgpu {
 
 struct SolveStatus {
-  int done;   // 0 running, 1 converged, 2 itmax, 3 breakdown
+  int done;   // 0 running, 1 converged, 2 itmax, 3 breakdown, 4 diverged
   int iter;   // completed iterations
   double err; // ||r||/||b|| of the last test
   double bnrm2;
@@ -104,9 +104,9 @@ template <typename TV>
 __global__ __launch_bounds__(VB) void k_dir(int64_t n, int nb, int q,
                                             const double *pr, const double *pb,
                                             const double *pbn, SolveStatus *st,
-                                            double tol, int itmax, TV *p,
-                                            TV *pp, TV *z, TV *zz,
-                                            int zero_out) {
+                                            double tol, int itmax,
+                                            double divtol, TV *p, TV *pp,
+                                            TV *z, TV *zz, int zero_out) {
   __shared__ double red[VB / 64];
   if (st->done) return;
   const int k = st->iter + 1;
@@ -129,6 +129,7 @@ __global__ __launch_bounds__(VB) void k_dir(int64_t n, int nb, int q,
   if (err <= tol) done = 1;
   else if (k > itmax) done = 2;
   else if (!(err == err)) done = 3;
+  else if (err > divtol) done = 4;
   double beta = 0.0;
   if (!done && k > 1) {
     const double bknum = block_sum(pb + (q ^ 1) * nb, nb, red);
@@ -331,7 +332,7 @@ struct Bicg {
   int solve(uint64_t op_key, const DualOp<TV> &dual, const SingleOp<TV> &single,
             const uint8_t *mask, const TV *b, TV *x, const TV *diag,
             double tol, int itmax, bool zero_out, hipStream_t s, int *iters,
-            double *err, int *status) {
+            double *err, int *status, double divtol = 1e300) {
     int rc = single(x, r, s);  // r = A x0 (x0 masked on load)
     if (rc) return rc;
     hipLaunchKernelGGL((k_init<TV>), dim3(nb), dim3(VB), 0, s, n, mask, b, x,
@@ -341,14 +342,15 @@ struct Bicg {
     VGPU_HIP(hipMemcpyAsync(st, &s0, sizeof s0, hipMemcpyHostToDevice, s));
     const bool same = key_valid && key_op == op_key && key_b == b &&
                       key_x == x && key_diag == diag && key_tol == tol &&
-                      key_itmax == itmax && key_zero == zero_out;
+                      key_itmax == itmax && key_zero == zero_out &&
+                      key_div == divtol;
     if (!same) {
       drop_graph();
       VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
       int crc = VGPU_OK;
       for (int pi = 0; pi < pairs_per_graph && crc == VGPU_OK; ++pi)
         for (int qq = 1; qq >= 0 && crc == VGPU_OK; --qq)
-          crc = enqueue_iteration(dual, diag, x, tol, itmax, zero_out, qq, s);
+          crc = enqueue_iteration(dual, diag, x, tol, itmax, divtol, zero_out, qq, s);
       hipError_t ce = hipStreamEndCapture(s, &graph);
       if (crc) {
         drop_graph();
@@ -364,6 +366,7 @@ struct Bicg {
       key_tol = tol;
       key_itmax = itmax;
       key_zero = zero_out;
+      key_div = divtol;
     }
     // replay until the device reports done, at most two replays in flight
     const int max_replays = itmax / (2 * pairs_per_graph) + 2;
@@ -412,12 +415,13 @@ struct Bicg {
   double key_tol = 0;
   int key_itmax = 0;
   bool key_zero = false;
+  double key_div = 0;
 
   int enqueue_iteration(const DualOp<TV> &dual, const TV *diag, TV *x,
-                        double tol, int itmax, bool zero_out, int q,
-                        hipStream_t s) {
+                        double tol, int itmax, double divtol, bool zero_out,
+                        int q, hipStream_t s) {
     hipLaunchKernelGGL((k_dir<TV>), dim3(nb), dim3(VB), 0, s, n, nb, q, pr, pb,
-                       pbn, st, tol, itmax, p, pp, z, zz, (int)zero_out);
+                       pbn, st, tol, itmax, divtol, p, pp, z, zz, (int)zero_out);
     int na = 0;
     int rc = dual(p, z, pp, zz, pa, &na, &st->done, s);
     if (rc) return rc;
