@@ -89,10 +89,8 @@ struct vgpu_ctx {
   double *ah[3] = {nullptr, nullptr, nullptr}, *aih[3] = {nullptr, nullptr, nullptr};
   float *fah[3] = {nullptr, nullptr, nullptr}, *faih[3] = {nullptr, nullptr, nullptr};
   uint8_t *mask = nullptr;
-  double *vx = nullptr, *vy = nullptr, *vz = nullptr, *lam = nullptr,
-         *tau = nullptr;
-  float *fvx = nullptr, *fvy = nullptr, *fvz = nullptr, *flam = nullptr,
-        *ftau = nullptr;
+  double *rec = nullptr;  // per-element records vx, vy, vz, lambda, tau
+  float *frec = nullptr;
   bool have_coeffs = false;
   double P = 0, alpha = 0, dt = 1;
   int variant = VGPU_DR_V1, scheme = VGPU_SCHEME_STEADY;
@@ -122,11 +120,7 @@ struct vgpu_ctx {
     o.ihx = aih[0];
     o.ihy = aih[1];
     o.ihz = aih[2];
-    o.vx = vx;
-    o.vy = vy;
-    o.vz = vz;
-    o.lam = lam;
-    o.tau = tau;
+    o.rec = rec;
     o.cH = cH_;
     o.mu = cH_ * P + cC_;
     o.ra = cH_ * alpha;
@@ -147,11 +141,7 @@ struct vgpu_ctx {
     o.ihx = faih[0];
     o.ihy = faih[1];
     o.ihz = faih[2];
-    o.vx = fvx;
-    o.vy = fvy;
-    o.vz = fvz;
-    o.lam = flam;
-    o.tau = ftau;
+    o.rec = frec;
     o.cH = cH;
     o.mu = cH * P + cC;
     o.ra = cH * alpha;
@@ -564,10 +554,8 @@ int vgpu_create(int nax, int nay, int naz, const double *xx, const double *yy,
       CK(hipMemcpy(c->faih[d], fih.data(), ne * sizeof(float), hipMemcpyHostToDevice));
     }
   }
-  for (double **p : {&c->vx, &c->vy, &c->vz, &c->lam, &c->tau})
-    CK(hipMalloc(p, E * sizeof(double)));
-  for (float **p : {&c->fvx, &c->fvy, &c->fvz, &c->flam, &c->ftau})
-    CK(hipMalloc(p, E * sizeof(float)));
+  CK(hipMalloc(&c->rec, 5 * E * sizeof(double)));
+  CK(hipMalloc(&c->frec, 5 * E * sizeof(float)));
   CK(hipMalloc(&c->diag, N * sizeof(double)));
   CK(hipMalloc(&c->fdiag, N * sizeof(float)));
   for (double **p : {&c->vb, &c->vx_, &c->vt, &c->vt2})
@@ -594,10 +582,9 @@ int vgpu_destroy(vgpu_ctx *c) {
     free_dev(c->fah[d]);
     free_dev(c->faih[d]);
   }
-  for (double **p : {&c->vx, &c->vy, &c->vz, &c->lam, &c->tau, &c->diag, &c->vb,
-                     &c->vx_, &c->vt, &c->vt2, &c->red})
+  for (double **p : {&c->rec, &c->diag, &c->vb, &c->vx_, &c->vt, &c->vt2, &c->red})
     free_dev(*p);
-  for (float **p : {&c->fvx, &c->fvy, &c->fvz, &c->flam, &c->ftau, &c->fdiag,
+  for (float **p : {&c->frec, &c->fdiag,
                     &c->fb, &c->fx})
     free_dev(*p);
   for (void *p : c->user_allocs) hipFree(p);
@@ -629,18 +616,21 @@ int vgpu_set_coeffs(vgpu_ctx *c, const double *lam, const double *vel,
   if (rc) return rc;
   const int64_t E = c->g.E;
   VGPU_HIP(hipSetDevice(c->device));
-  VGPU_HIP(hipMemcpyAsync(c->lam, lam, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  VGPU_HIP(hipMemcpyAsync(c->vx, vel, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  VGPU_HIP(hipMemcpyAsync(c->vy, vel + E, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  VGPU_HIP(hipMemcpyAsync(c->vz, vel + 2 * E, E * sizeof(double), hipMemcpyHostToDevice, c->stream));
-  rc = launch_tau(c->g, c->xc, c->yc, c->zc, c->lam, c->vx, c->vy, c->vz,
-                  dr_variant, c->tau, c->stream);
+  {
+    std::vector<double> r(5 * (size_t)E);
+    for (int64_t e = 0; e < E; ++e) {
+      r[5 * e + 0] = vel[e];
+      r[5 * e + 1] = vel[E + e];
+      r[5 * e + 2] = vel[2 * E + e];
+      r[5 * e + 3] = lam[e];
+      r[5 * e + 4] = 0.0;
+    }
+    VGPU_HIP(hipMemcpy(c->rec, r.data(), 5 * E * sizeof(double), hipMemcpyHostToDevice));
+  }
+  rc = launch_tau(c->g, c->xc, c->yc, c->zc, c->rec, dr_variant, c->stream);
   if (rc) return rc;
-  const double *src[5] = {c->vx, c->vy, c->vz, c->lam, c->tau};
-  float *dst[5] = {c->fvx, c->fvy, c->fvz, c->flam, c->ftau};
-  for (int i = 0; i < 5; ++i)
-    hipLaunchKernelGGL((k_convert<double, float>), grid_for(E), dim3(256), 0,
-                       c->stream, E, src[i], dst[i]);
+  hipLaunchKernelGGL((k_convert<double, float>), grid_for(5 * E), dim3(256), 0,
+                     c->stream, 5 * E, c->rec, c->frec);
   VGPU_HIP(hipGetLastError());
   VGPU_HIP(hipStreamSynchronize(c->stream));
   c->P = P;

@@ -40,7 +40,7 @@ struct OpData {
   const T *hx, *hy, *hz;       // element sizes along each axis [ex], [ey], [ez]
   const T *ihx, *ihy, *ihz;    // their reciprocals
   const uint8_t *mask;         // 1 = Dirichlet node
-  const T *vx, *vy, *vz, *lam, *tau;  // per element, SoA
+  const T *rec;  // per-element records [5E]: vx, vy, vz, lambda, tau
   double cH;     // weight of H (diffusion + advection + DR + Robin)
   double mu;     // weight of C = M_w: cH*P + cC
   double ra;     // Robin weight cH*alpha on x-min elements (0 = none)
@@ -62,6 +62,17 @@ __device__ __forceinline__ void lds_barrier() {
 
 namespace vgpu {
 
+// Workgroups are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8),
+// each XCD with its own L2.  Map block b to a logical tile so that every XCD
+// works on a contiguous range of tiles: neighbouring tiles, which share halo
+// node planes and element columns, then meet in the same L2.  Bijective on
+// [0, nb) for any nb.
+__device__ __forceinline__ int xcd_tile(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7;
+  const int x = b & 7, slot = b >> 3;
+  return x * q + min(x, r) + slot;
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers (vgpu_operator.hip)
 // ---------------------------------------------------------------------------
@@ -75,10 +86,9 @@ int launch_apply(const OpData<T> &op, int scatter, int zchunk,
 // number of dot partials the owner launch writes for this geometry
 int owner_num_tiles(const Grid3 &g, int zchunk, int tile);
 
+// Fills the tau field of the per-element records from the other fields.
 int launch_tau(const Grid3 &g, const double *xc, const double *yc,
-               const double *zc, const double *lam, const double *vx,
-               const double *vy, const double *vz, int variant, double *tau,
-               hipStream_t s);
+               const double *zc, double *rec, int variant, hipStream_t s);
 int launch_diag(const OpData<double> &op, double *diag, hipStream_t s);
 int launch_source(const OpData<double> &op, const double *q_in, double *f,
                   hipStream_t s);

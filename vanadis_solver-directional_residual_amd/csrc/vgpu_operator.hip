@@ -32,11 +32,12 @@ __device__ __forceinline__ Coef<T> load_coef(const OpData<T> &op, bool valid,
                                              int64_t e, int kz) {
   Coef<T> c;
   if (valid) {
-    c.lam = op.lam[e];
-    c.vx = op.vx[e];
-    c.vy = op.vy[e];
-    c.vz = op.vz[e];
-    c.tau = op.tau[e];
+    const T *r = op.rec + 5 * e;
+    c.vx = r[0];
+    c.vy = r[1];
+    c.vz = r[2];
+    c.lam = r[3];
+    c.tau = r[4];
     c.hz = op.hz[kz];
     c.ihz = op.ihz[kz];
   } else {
@@ -196,9 +197,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
 
   const int tid = threadIdx.x;
   const int tx = tid & (TX - 1), ty = tid / TX;
-  const int gx0 = blockIdx.x * (TX - 1) - 1;  // tile node/element origin
-  const int gy0 = blockIdx.y * (TY - 1) - 1;
-  const int kz0 = blockIdx.z * zc;          // first owned node plane
+  const int ntx = (op.g.nax + TX - 2) / (TX - 1), nty = (op.g.nay + TY - 2) / (TY - 1);
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
+  const int gy0 = ((tile / ntx) % nty) * (TY - 1) - 1;
+  const int kz0 = (tile / (ntx * nty)) * zc;  // first owned node plane
   const int exg = gx0 + tx, eyg = gy0 + ty;
   const bool colok = exg >= 0 && exg < op.g.ex && eyg >= 0 && eyg < op.g.ey;
   const T hx = colok ? op.hx[exg] : T(1), ihx = colok ? op.ihx[exg] : T(1);
@@ -275,8 +278,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     if (tid == 0) {
       double s = 0.0;
       for (int w = 0; w < (TX * TY) / 64; ++w) s += red[w];
-      const int64_t b = blockIdx.x + (int64_t)gridDim.x * (blockIdx.y + (int64_t)gridDim.y * blockIdx.z);
-      dot_part[b] = s;
+      dot_part[blockIdx.x] = s;
     }
   }
 }
@@ -301,11 +303,17 @@ __global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int col
   __shared__ T st2[DUAL ? 8 * (TX * TY) : 1];
 
   if (op.done && *op.done) return;
-  int bx = blockIdx.x, by = blockIdx.y, bz = blockIdx.z;
-  if (!ATOM) {
-    bx = 2 * bx + (color & 1);
-    by = 2 * by + ((color >> 1) & 1);
-    bz = 2 * bz + ((color >> 2) & 1);
+  int bx, by, bz;
+  if (ATOM) {  // 1-D grid, XCD-aware
+    const int ntx = (op.g.ex + TX - 1) / TX, nty = (op.g.ey + TY - 1) / TY;
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    bx = t % ntx;
+    by = (t / ntx) % nty;
+    bz = t / (ntx * nty);
+  } else {
+    bx = 2 * blockIdx.x + (color & 1);
+    by = 2 * blockIdx.y + ((color >> 1) & 1);
+    bz = 2 * blockIdx.z + ((color >> 2) & 1);
   }
   const int kz0 = bz * zc;
   if (bx * TX >= op.g.ex || by * TY >= op.g.ey || kz0 >= op.g.ez) return;
@@ -427,8 +435,8 @@ template <typename T, typename TIO, int TX, int TY>
 void launch_owner(const OpData<T> &op, int zc, const TIO *x1, TIO *y1,
                   const TIO *x2, TIO *y2, double *dot_part, hipStream_t s) {
   const Grid3 &g = op.g;
-  dim3 grid((g.nax + TX - 2) / (TX - 1), (g.nay + TY - 2) / (TY - 1),
-            (g.naz + zc - 1) / zc);
+  dim3 grid(((g.nax + TX - 2) / (TX - 1)) * ((g.nay + TY - 2) / (TY - 1)) *
+            ((g.naz + zc - 1) / zc));
   if (x2)
     hipLaunchKernelGGL((k_owner<T, TIO, true, TX, TY>), grid, dim3(TX * TY), 0, s,
                        op, zc, x1, y1, x2, y2, dot_part);
@@ -461,7 +469,7 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
   const int ntx = (g.ex + BX - 1) / BX, nty = (g.ey + BY - 1) / BY;
   const int ntz = (g.ez + zc - 1) / zc;
   if (scatter == VGPU_SCATTER_ATOMIC) {
-    dim3 grid(ntx, nty, ntz);
+    dim3 grid(ntx * nty * ntz);
     if (dual)
       hipLaunchKernelGGL((k_brick<T, TIO, true, true, BX, BY>), grid, dim3(BX * BY),
                          0, s, op, zc, 0, x1, y1, x2, y2);
@@ -509,9 +517,7 @@ template int launch_apply<float, double>(const OpData<float> &, int, int,
 //       lambda_s = lam, coth from exp.
 // ---------------------------------------------------------------------------
 __global__ void k_tau(Grid3 g, const double *xc, const double *yc,
-                      const double *zc, const double *lam, const double *vx,
-                      const double *vy, const double *vz, int variant,
-                      double *tau) {
+                      const double *zc, double *rec, int variant) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.E;
        e += (int64_t)gridDim.x * blockDim.x) {
     const int ix = (int)(e % g.ex);
@@ -519,7 +525,8 @@ __global__ void k_tau(Grid3 g, const double *xc, const double *yc,
     const int iz = (int)(e / ((int64_t)g.ex * g.ey));
     const double hx = xc[ix + 1] - xc[ix], hy = yc[iy + 1] - yc[iy],
                  hz = zc[iz + 1] - zc[iz];
-    const double v1 = vx[e], v2 = vy[e], v3 = vz[e], l = lam[e];
+    double *r = rec + 5 * e;
+    const double v1 = r[0], v2 = r[1], v3 = r[2], l = r[3];
     const double eps = 1.0e-14;
     const double vmag = sqrt(v1 * v1 + v2 * v2 + v3 * v3);
     double hs, ls;
@@ -556,16 +563,14 @@ __global__ void k_tau(Grid3 g, const double *xc, const double *yc,
       }
       t = 0.5 * hs / vmag * beta;
     }
-    tau[e] = t;
+    r[4] = t;
   }
 }
 
 int launch_tau(const Grid3 &g, const double *xc, const double *yc,
-               const double *zc, const double *lam, const double *vx,
-               const double *vy, const double *vz, int variant, double *tau,
-               hipStream_t s) {
-  hipLaunchKernelGGL(k_tau, dim3(2048), dim3(256), 0, s, g, xc, yc, zc, lam, vx,
-                     vy, vz, variant, tau);
+               const double *zc, double *rec, int variant, hipStream_t s) {
+  hipLaunchKernelGGL(k_tau, dim3(2048), dim3(256), 0, s, g, xc, yc, zc, rec,
+                     variant);
   VGPU_HIP(hipGetLastError());
   return VGPU_OK;
 }
@@ -579,17 +584,16 @@ __device__ double elem_entry(const OpData<double> &op, int ix, int iy, int iz,
                              int a) {
   const int64_t e = ix + (int64_t)op.g.ex * (iy + (int64_t)op.g.ey * iz);
   ElemConst<double> k;
+  const double *r = op.rec + 5 * e;
   elem_setup<double>(op.hx[ix], op.hy[iy], op.hz[iz], op.ihx[ix], op.ihy[iy],
-                     op.ihz[iz], op.lam[e], op.vx[e], op.vy[e], op.vz[e],
-                     op.tau[e], op.cH, op.mu, k);
+                     op.ihz[iz], r[3], r[0], r[1], r[2], r[4], op.cH, op.mu, k);
   double x[8], xh[8], yh[8], y[8];
   for (int c = 0; c < 8; ++c) x[c] = (c == a) ? 1.0 : 0.0;
   haar_fwd<double>(x, xh);
   haar_apply<double, false>(k, xh, yh);
   haar_inv<double>(yh, y);
   if (ix == 0 && op.ra != 0.0)
-    robin_add<double, false>(k, op.hx[ix], op.tau[e], op.vx[e], op.vy[e],
-                             op.vz[e], op.ra, x, y);
+    robin_add<double, false>(k, op.hx[ix], r[4], r[0], r[1], r[2], op.ra, x, y);
   return y[a];
 }
 
@@ -644,9 +648,9 @@ __global__ void k_source(OpData<double> op, const double *q_in, double *f) {
       const double hx = op.xc[ix + 1] - op.xc[ix], hy = op.yc[iy + 1] - op.yc[iy],
                    hz = op.zc[iz + 1] - op.zc[iz];
       const double sx = cx ? 1.0 : -1.0, sy = cy ? 1.0 : -1.0, sz = cz ? 1.0 : -1.0;
-      const double t = op.tau[e];
-      s += q * (0.125 + t * (op.vx[e] * sx / (4.0 * hx) + op.vy[e] * sy / (4.0 * hy) +
-                             op.vz[e] * sz / (4.0 * hz)));
+      const double *r = op.rec + 5 * e;
+      s += q * (0.125 + r[4] * (r[0] * sx / (4.0 * hx) + r[1] * sy / (4.0 * hy) +
+                                r[2] * sz / (4.0 * hz)));
     }
     f[n] = s;
   }
