@@ -110,8 +110,10 @@ int vgpu_set_coeffs(vgpu_ctx *ctx, const double *lam, const double *vel,
 /* Change only the scheme / dt (e.g. steady vs implicit Euler). */
 int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
 /* Options: scatter mode for the solver, z-chunk of the z-marching tiles
- * (0 = keep) and owner tile shape (0 = 32x8, 1 = 16x16, 2 = 64x4 element
- * columns; -1 = keep). */
+ * (-1 = keep; 0 = owner launches size their grid to the device's resident
+ * workgroups and split the node planes evenly over them, brick launches use
+ * 8; n > 0 = about n node planes per workgroup) and owner tile shape
+ * (0 = 32x8, 1 = 16x16, 2 = 64x4 element columns; -1 = keep). */
 int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk, int tile);
 
 /* y = (I-D) A (I-D) x  (transpose=1: A^T), host vectors of length N.

@@ -180,7 +180,9 @@ class Context:
     def set_scheme(self, scheme, dt):
         check(load().vgpu_set_scheme(self.h, int(scheme), float(dt)), "set_scheme")
 
-    def set_options(self, scatter="owner", zchunk=0, tile=-1):
+    def set_options(self, scatter="owner", zchunk=-1, tile=-1):
+        """Scatter mode; z chunk (-1 keep, 0 balanced over resident workgroups,
+        n > 0 about n node planes per workgroup); owner tile shape (-1 keep)."""
         check(load().vgpu_set_options(self.h, scatter_code(scatter), int(zchunk), int(tile)),
               "set_options")
 

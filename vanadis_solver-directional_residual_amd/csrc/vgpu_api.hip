@@ -95,7 +95,7 @@ struct vgpu_ctx {
   double P = 0, alpha = 0, dt = 1;
   int variant = VGPU_DR_V1, scheme = VGPU_SCHEME_STEADY;
   double cH = 1, cC = 0;
-  int scatter = VGPU_SCATTER_OWNER, zchunk = 8, tile = 0;
+  int scatter = VGPU_SCATTER_OWNER, zchunk = 0, tile = 0;
   double *diag = nullptr;
   float *fdiag = nullptr;
   bool diag_valid = false;
@@ -209,11 +209,7 @@ double host_sum(const double *dev, int n, hipStream_t s, int *rc) {
 }
 
 // capacity of owner-mode dot partials over every tile shape and z chunk
-int max_owner_tiles(const Grid3 &g) {
-  int m = 0;
-  for (int t = 0; t < 3; ++t) m = std::max(m, owner_num_tiles(g, 1, t));
-  return m;
-}
+int max_owner_tiles(const Grid3 &g) { return owner_max_blocks(g); }
 
 int nparts_for(int64_t n) {
   return (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));
@@ -226,7 +222,7 @@ DualOp<double> make_dual64(vgpu_ctx *c, int scatter) {
     OpData<double> op = c->op64(c->cH, c->cC);
     op.done = done;
     if (scatter == VGPU_SCATTER_OWNER) {
-      *na = owner_num_tiles(c->g, c->zchunk, c->tile);
+      *na = owner_num_blocks(c->g, c->zchunk, c->tile, 8, 8, true, op.ra != 0.0);
       return launch_apply<double, double>(op, scatter, c->zchunk, p, z, pp, zz,
                                           pa, s);
     }
@@ -250,7 +246,7 @@ DualOp<float> make_dual32(vgpu_ctx *c, int scatter) {
     OpData<float> op = c->op32();
     op.done = done;
     if (scatter == VGPU_SCATTER_OWNER) {
-      *na = owner_num_tiles(c->g, c->zchunk, c->tile);
+      *na = owner_num_blocks(c->g, c->zchunk, c->tile, 4, 4, true, op.ra != 0.0);
       return launch_apply<float, float>(op, scatter, c->zchunk, p, z, pp, zz,
                                         pa, s);
     }
@@ -362,15 +358,22 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
   // x masked like the FP64 path
   hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N, c->mask,
                      1.0, x, 0.0, nullptr, x);
-  for (int outer = 0; outer < 64; ++outer) {
+  // true relative residual |b - A x| / |b| of the current iterate (r in vt)
+  auto true_rel = [&](double *rn_out) -> int {
     OpData<double> op = c->op64(c->cH, c->cC);
-    rc = launch_apply<double, double>(op, VGPU_SCATTER_OWNER, c->zchunk, x, c->vt,
-                                      nullptr, nullptr, nullptr, c->stream);
-    if (rc) return rc;
+    int r = launch_apply<double, double>(op, VGPU_SCATTER_OWNER, c->zchunk, x, c->vt,
+                                         nullptr, nullptr, nullptr, c->stream);
+    if (r) return r;
     hipLaunchKernelGGL(k_resid, dim3(np), dim3(VB), 0, c->stream, N, b, c->vt,
                        c->red);
     VGPU_HIP(hipGetLastError());
-    const double rn = std::sqrt(host_sum(c->red, np, c->stream, &rc));
+    *rn_out = std::sqrt(host_sum(c->red, np, c->stream, &r));
+    return r;
+  };
+  bool fp64 = false;
+  for (int outer = 0; outer < 64; ++outer) {
+    double rn = 0;
+    rc = true_rel(&rn);
     if (rc) return rc;
     rel = rn / bn;
     if (rel <= rtol) {
@@ -381,20 +384,31 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
       *status = 3;
       break;
     }
-    if (outer > 0 && (rel > 0.5 * prev_rel || st_inner >= 3)) {
-      // the FP32 inner solve stopped reducing the true residual: finish in
-      // FP64 BiCG from the current iterate (the fallback of the mixed mode)
+    if (total >= itmax) {
+      *status = 2;
+      break;
+    }
+    // the FP32 inner solve stopped reducing the true residual: continue with
+    // FP64 BiCG passes from the current iterate (the fallback of the mixed
+    // mode); each pass restarts from the true residual checked above
+    if (!fp64 && outer > 0 && (rel > 0.5 * prev_rel || st_inner >= 3)) fp64 = true;
+    prev_rel = rel;
+    if (fp64) {
       int it64 = 0, st64 = 0;
       double e64 = 0;
       rc = solve_fp64_dev(c, b, x, rtol, itmax - total, &it64, &e64, &st64);
       if (rc) return rc;
       total += it64;
-      *status = st64;
-      rel = e64;
-      break;
+      if (st64 >= 3) {  // breakdown or divergence of the FP64 pass
+        *status = st64;
+        double rn2 = 0;
+        rc = true_rel(&rn2);
+        if (rc) return rc;
+        rel = rn2 / bn;
+        break;
+      }
+      continue;
     }
-    prev_rel = rel;
-    if (total >= itmax) break;
     hipLaunchKernelGGL(k_scale_to_f32, grid_for(N), dim3(256), 0, c->stream, N,
                        c->vt, 1.0 / rn, c->fb, c->fx);
     VGPU_HIP(hipGetLastError());
@@ -412,6 +426,12 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
     hipLaunchKernelGGL(k_refine, grid_for(N), dim3(256), 0, c->stream, N, x, rn,
                        c->fx);
     VGPU_HIP(hipGetLastError());
+  }
+  if (*status == 2) {  // pass limit or itmax: report the final iterate
+    double rn = 0;
+    rc = true_rel(&rn);
+    if (rc) return rc;
+    rel = rn / bn;
   }
   VGPU_HIP(hipStreamSynchronize(c->stream));
   *iters = total;
@@ -661,12 +681,12 @@ int vgpu_set_scheme(vgpu_ctx *c, int scheme, double dt) {
 }
 
 int vgpu_set_options(vgpu_ctx *c, int scatter, int zchunk, int tile) {
-  if (!c || scatter < 0 || scatter > 2 || zchunk < 0 || zchunk > 256 || tile > 2) {
+  if (!c || scatter < 0 || scatter > 2 || zchunk < -1 || zchunk > 256 || tile > 2) {
     set_error("vgpu_set_options: bad arguments");
     return VGPU_ERR_ARG;
   }
   c->scatter = scatter;
-  if (zchunk > 0) c->zchunk = zchunk;
+  if (zchunk >= 0) c->zchunk = zchunk;
   if (tile >= 0) c->tile = tile;
   c->version++;
   return VGPU_OK;
@@ -965,9 +985,10 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                                         what == 1 ? c->vt2 : nullptr,
                                         what == 1 ? c->red : nullptr, c->stream);
   };
-  // red holds 4096 partials: enough for owner tiles at zc >= 1 up to ~8M nodes
+  // red holds 4096 partials
+  const int eb = precision == VGPU_FP32 ? 4 : 8;
   if (what == 1 && scatter == VGPU_SCATTER_OWNER &&
-      owner_num_tiles(c->g, c->zchunk, c->tile) > 4096) {
+      owner_num_blocks(c->g, c->zchunk, c->tile, eb, eb, true, c->alpha != 0.0) > 4096) {
     set_error("vgpu_time_kernel: mesh too large for the timing partial buffer");
     return VGPU_ERR_ARG;
   }

@@ -32,41 +32,70 @@ struct ElemConst {
   T t[3];     // symmetric DR cross terms, by free axis: 12 cH tau v_k v_l / (h_k h_l)
 };
 
-// Per-element constants from the sizes h, reciprocals ih and the fields.
-// bra_k = 12 cH (lam + tau v_k^2) / h_k^2 is the K/M eigenvalue ratio of the
-// d mode times the diffusion plus DR streamline-diffusion coefficient.
+// Factors of an element column that stay fixed while a kernel marches up z:
+// the halved x and y mass eigenvalues of the s mode and the reciprocal sizes.
 template <typename T>
-VGPU_HD void elem_setup(T hx, T hy, T hz, T ihx, T ihy,
-                                           T ihz, T lam, T vx, T vy, T vz,
-                                           T tau, T cH, T mu, ElemConst<T> &k) {
-  const T h[3] = {hx, hy, hz};
-  const T ih[3] = {ihx, ihy, ihz};
-  const T v[3] = {vx, vy, vz};
+struct ColConst {
+  T mx0, my0;  // hx / 4, hy / 4
+  T ihx, ihy;  // 1 / hx, 1 / hy
+};
+
+template <typename T>
+VGPU_HD void col_setup(T hx, T hy, T ihx, T ihy, ColConst<T> &q) {
+  q.mx0 = hx * T(0.25);
+  q.my0 = hy * T(0.25);
+  q.ihx = ihx;
+  q.ihy = ihy;
+}
+
+// Per-element constants of one layer from the column factors, the z size and
+// the element's fields.  bra_k = 12 cH (lam + tau v_k^2) / h_k^2 is the K/M
+// eigenvalue ratio of the d mode times the diffusion plus DR streamline
+// diffusion coefficient.
+template <typename T>
+VGPU_HD void elem_layer_setup(const ColConst<T> &q, T hz, T ihz, T lam, T vx,
+                              T vy, T vz, T tau, T cH, T mu,
+                              ElemConst<T> &k) {
+  const T third = T(1.0 / 3.0);
+  k.m[0][0] = q.mx0;
+  k.m[0][1] = q.mx0 * third;
+  k.m[1][0] = q.my0;
+  k.m[1][1] = q.my0 * third;
+  k.m[2][0] = hz * T(0.25);
+  k.m[2][1] = hz * T(1.0 / 12.0);
+  const T tvx = tau * vx, tvy = tau * vy, tvz = tau * vz;
   const T c12 = T(12) * cH, c6 = T(6) * cH, m2 = T(2) * mu;
-  T bra[3], tv[3];
+  const T brx = (c12 * (q.ihx * q.ihx)) * (lam + tvx * vx);
+  const T bry = (c12 * (q.ihy * q.ihy)) * (lam + tvy * vy);
+  const T brz = (c12 * (ihz * ihz)) * (lam + tvz * vz);
+  k.g[0] = (c6 * q.ihx) * vx;
+  k.g[1] = (c6 * q.ihy) * vy;
+  k.g[2] = (c6 * ihz) * vz;
+  k.c[0] = (m2 * q.ihx) * tvx;
+  k.c[1] = (m2 * q.ihy) * tvy;
+  k.c[2] = (m2 * ihz) * tvz;
+  const T c12z = c12 * ihz;
+  k.t[0] = (c12z * q.ihy) * tvy * vz;          // y-z, free axis x
+  k.t[1] = (c12z * q.ihx) * tvx * vz;          // x-z, free axis y
+  k.t[2] = (c12 * (q.ihx * q.ihy)) * tvx * vy;  // x-y, free axis z
 #pragma unroll
-  for (int d = 0; d < 3; ++d) {
-    k.m[d][0] = h[d] * T(0.25);
-    k.m[d][1] = h[d] * T(1.0 / 12.0);
-    tv[d] = tau * v[d];
-    bra[d] = c12 * (ih[d] * ih[d]) * (lam + tv[d] * v[d]);
-    k.g[d] = c6 * v[d] * ih[d];
-    k.c[d] = m2 * tv[d] * ih[d];
-  }
-  k.t[0] = c12 * tv[1] * vz * (ih[1] * ih[2]);  // y-z, free axis x
-  k.t[1] = c12 * tv[0] * vz * (ih[0] * ih[2]);  // x-z, free axis y
-  k.t[2] = c12 * tv[0] * vy * (ih[0] * ih[1]);  // x-y, free axis z
-  T wxy[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) wxy[q] = k.m[0][q & 1] * k.m[1][q >> 1];
-#pragma unroll
-  for (int b = 0; b < 8; ++b) k.W[b] = wxy[b & 3] * k.m[2][b >> 2];
+  for (int b = 0; b < 8; ++b)
+    k.W[b] = (k.m[0][b & 1] * k.m[1][(b >> 1) & 1]) * k.m[2][b >> 2];
   k.D[0] = mu;
-  k.D[1] = mu + bra[0];
-  k.D[2] = mu + bra[1];
-  k.D[3] = k.D[1] + bra[1];
+  k.D[1] = mu + brx;
+  k.D[2] = mu + bry;
+  k.D[3] = k.D[1] + bry;
 #pragma unroll
-  for (int b = 0; b < 4; ++b) k.D[b + 4] = k.D[b] + bra[2];
+  for (int b = 0; b < 4; ++b) k.D[b + 4] = k.D[b] + brz;
+}
+
+// All per-element constants from the sizes h, reciprocals ih and the fields.
+template <typename T>
+VGPU_HD void elem_setup(T hx, T hy, T hz, T ihx, T ihy, T ihz, T lam, T vx, T vy,
+                        T vz, T tau, T cH, T mu, ElemConst<T> &k) {
+  ColConst<T> q;
+  col_setup<T>(hx, hy, ihx, ihy, q);
+  elem_layer_setup<T>(q, hz, ihz, lam, vx, vy, vz, tau, cH, mu, k);
 }
 
 // forward per-axis butterflies: nodal x[8] -> Haar h[8]

@@ -84,7 +84,13 @@ int launch_apply(const OpData<T> &op, int scatter, int zchunk,
                  const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
                  double *dot_part, hipStream_t s);
 // number of dot partials the owner launch writes for this geometry
-int owner_num_tiles(const Grid3 &g, int zchunk, int tile);
+// Workgroups of an owner launch (= dot partials it writes): zchunk > 0 gives
+// about zchunk node planes per workgroup, zchunk = 0 one workgroup per
+// resident slot of the device.  elem_bytes / io_bytes: sizeof(T) / sizeof(TIO).
+int owner_num_blocks(const Grid3 &g, int zchunk, int tile, int elem_bytes,
+                     int io_bytes, bool dual, bool robin);
+// Upper bound of owner_num_blocks over every option (partial buffer size).
+int owner_max_blocks(const Grid3 &g);
 
 // Fills the tau field of the per-element records from the other fields.
 int launch_tau(const Grid3 &g, const double *xc, const double *yc,

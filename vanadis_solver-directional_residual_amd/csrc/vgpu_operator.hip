@@ -17,6 +17,10 @@
 // (VANADIS.FOR:2149-2245) with A the element systems of przest3d (723-748).
 #include "vgpu_internal.hpp"
 
+#include <algorithm>
+#include <map>
+#include <mutex>
+
 namespace vgpu {
 
 namespace {
@@ -26,11 +30,15 @@ struct Coef {
   T lam, vx, vy, vz, tau, hz, ihz;
 };
 
-// per-element fields and the layer's z size (zero / unit outside the grid)
+// per-element fields (zero outside the grid) and the layer's z size
 template <typename T>
 __device__ __forceinline__ Coef<T> load_coef(const OpData<T> &op, bool valid,
                                              int64_t e, int kz) {
   Coef<T> c;
+  // kz is uniform over the workgroup: scalar loads of the layer size
+  const int kc = min(max(kz, 0), op.g.ez - 1);
+  c.hz = op.hz[kc];
+  c.ihz = op.ihz[kc];
   if (valid) {
     const T *r = op.rec + 5 * e;
     c.vx = r[0];
@@ -38,84 +46,10 @@ __device__ __forceinline__ Coef<T> load_coef(const OpData<T> &op, bool valid,
     c.vz = r[2];
     c.lam = r[3];
     c.tau = r[4];
-    c.hz = op.hz[kz];
-    c.ihz = op.ihz[kz];
   } else {
     c.lam = c.vx = c.vy = c.vz = c.tau = T(0);
-    c.hz = c.ihz = T(1);
   }
   return c;
-}
-
-// One tile-plane node: masked value, and its Dirichlet flag in m (nodes
-// outside the grid read as masked zeros).  Value and flag load in parallel.
-template <typename T, typename TIO>
-__device__ __forceinline__ T load_node(const OpData<T> &op, const TIO *x, int gx,
-                                       int gy, int gz, uint8_t &m) {
-  if (gx < 0 || gy < 0 || gz < 0 || gx >= op.g.nax || gy >= op.g.nay ||
-      gz >= op.g.naz) {
-    m = 1;
-    return T(0);
-  }
-  const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * gz);
-  const uint8_t mk = op.mask[n];
-  const T v = T(x[n]);
-  m = mk;
-  return mk ? T(0) : v;
-}
-
-// Element apply for one thread: gathers from the two LDS planes, writes the 8
-// results (and 8 transposed results when DUAL) to the staging arrays.
-template <typename T, bool DUAL, int TX, int TY>
-__device__ __forceinline__ void elem_step(const OpData<T> &op, bool valid,
-                                          bool robin, T hx, T hy, T ihx, T ihy,
-                                          const Coef<T> &cf, const T *pb,
-                                          const T *pt, const T *pb2,
-                                          const T *pt2, int tx, int ty,
-                                          T *stage, T *stage2, int tid) {
-  T y[8], y2[8];
-  if (valid) {
-    ElemConst<T> k;
-    elem_setup<T>(hx, hy, cf.hz, ihx, ihy, cf.ihz, cf.lam, cf.vx, cf.vy, cf.vz,
-                  cf.tau, T(op.cH), T(op.mu), k);
-    const int i00 = ty * (TX + 1) + tx;
-    T x[8], xh[8], yh[8];
-    x[0] = pb[i00];
-    x[1] = pb[i00 + 1];
-    x[2] = pb[i00 + (TX + 1) + 1];
-    x[3] = pb[i00 + (TX + 1)];
-    x[4] = pt[i00];
-    x[5] = pt[i00 + 1];
-    x[6] = pt[i00 + (TX + 1) + 1];
-    x[7] = pt[i00 + (TX + 1)];
-    haar_fwd<T>(x, xh);
-    haar_apply<T, false>(k, xh, yh);
-    haar_inv<T>(yh, y);
-    if (robin) robin_add<T, false>(k, hx, cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x, y);
-    if (DUAL) {
-      x[0] = pb2[i00];
-      x[1] = pb2[i00 + 1];
-      x[2] = pb2[i00 + (TX + 1) + 1];
-      x[3] = pb2[i00 + (TX + 1)];
-      x[4] = pt2[i00];
-      x[5] = pt2[i00 + 1];
-      x[6] = pt2[i00 + (TX + 1) + 1];
-      x[7] = pt2[i00 + (TX + 1)];
-      haar_fwd<T>(x, xh);
-      haar_apply<T, true>(k, xh, yh);
-      haar_inv<T>(yh, y2);
-      if (robin) robin_add<T, true>(k, hx, cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x, y2);
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) y[c] = y2[c] = T(0);
-  }
-#pragma unroll
-  for (int c = 0; c < 8; ++c) stage[c * (TX * TY) + tid] = y[c];
-  if (DUAL) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) stage2[c * (TX * TY) + tid] = y2[c];
-  }
 }
 
 // Sum of the four in-plane contributions to tile node (lx, ly); `top` selects
@@ -133,142 +67,282 @@ __device__ __forceinline__ double node_sum(const T *stage, int lx, int ly,
   return s;
 }
 
-// Plane traffic of one thread: tile-plane nodes q0 = tid and q1 = tid + 256
-// (q1 < 297 only), value of each input and the Dirichlet flag.
-template <typename T, bool DUAL>
+// Plane traffic of one thread: tile-plane nodes q0 = tid and q1 = tid + NT
+// (q1 < (TX+1)(TY+1) only): raw values of each input and the Dirichlet flag
+// (1 also outside the grid).  The loads are only issued here; the mask is
+// applied when the plane is written to LDS one layer later, so no wait on
+// the loads is needed before then.
+template <typename TIO, bool DUAL>
 struct PlaneRegs {
-  T a0, a1, b0, b1;
+  TIO a0, a1, b0, b1;
   uint8_t m0, m1;
 };
+
+template <typename TIO, bool DUAL>
+__device__ __forceinline__ void node_raw(const Grid3 &g, const uint8_t *mask,
+                                         const TIO *x1, const TIO *x2, int gx,
+                                         int gy, int gz, TIO &a, TIO &b,
+                                         uint8_t &m) {
+  a = b = TIO(0);
+  m = 1;
+  if (gx >= 0 && gy >= 0 && gz >= 0 && gx < g.nax && gy < g.nay && gz < g.naz) {
+    const int64_t n = gx + (int64_t)g.nax * (gy + (int64_t)g.nay * gz);
+    m = mask[n];
+    a = x1[n];
+    if (DUAL) b = x2[n];
+  }
+}
 
 template <typename T, typename TIO, bool DUAL>
 __device__ __forceinline__ void plane_load(const OpData<T> &op, const TIO *x1,
                                            const TIO *x2, int gx0, int gy0,
                                            int gz, int lx0, int ly0, int lx1,
                                            int ly1, bool has1,
-                                           PlaneRegs<T, DUAL> &r) {
-  uint8_t dm;
-  r.a0 = load_node<T, TIO>(op, x1, gx0 + lx0, gy0 + ly0, gz, r.m0);
-  r.b0 = DUAL ? load_node<T, TIO>(op, x2, gx0 + lx0, gy0 + ly0, gz, dm) : T(0);
+                                           PlaneRegs<TIO, DUAL> &r) {
+  node_raw<TIO, DUAL>(op.g, op.mask, x1, x2, gx0 + lx0, gy0 + ly0, gz, r.a0, r.b0, r.m0);
+  r.a1 = r.b1 = TIO(0);
   r.m1 = 1;
-  r.a1 = r.b1 = T(0);
-  if (has1) {
-    r.a1 = load_node<T, TIO>(op, x1, gx0 + lx1, gy0 + ly1, gz, r.m1);
-    r.b1 = DUAL ? load_node<T, TIO>(op, x2, gx0 + lx1, gy0 + ly1, gz, dm) : T(0);
-  }
+  if (has1)
+    node_raw<TIO, DUAL>(op.g, op.mask, x1, x2, gx0 + lx1, gy0 + ly1, gz, r.a1, r.b1, r.m1);
 }
 
-template <typename T, bool DUAL>
-__device__ __forceinline__ void plane_store(const PlaneRegs<T, DUAL> &r, T *p1,
+template <typename T, typename TIO, bool DUAL>
+__device__ __forceinline__ void plane_store(const PlaneRegs<TIO, DUAL> &r, T *p1,
                                             T *p2, uint8_t *pm, int q0, int q1,
                                             bool has1) {
-  p1[q0] = r.a0;
+  p1[q0] = r.m0 ? T(0) : T(r.a0);
   pm[q0] = r.m0;
-  if (DUAL) p2[q0] = r.b0;
+  if (DUAL) p2[q0] = r.m0 ? T(0) : T(r.b0);
   if (has1) {
-    p1[q1] = r.a1;
+    p1[q1] = r.m1 ? T(0) : T(r.a1);
     pm[q1] = r.m1;
-    if (DUAL) p2[q1] = r.b1;
+    if (DUAL) p2[q1] = r.m1 ? T(0) : T(r.b1);
   }
 }
 
 // ---------------------------------------------------------------------------
-// OWNER kernel.  Grid: (ceil(nax/31), ceil(nay/7), ceil(naz/zc)).
-// Tile (bx,by,bz) owns nodes [31bx,31bx+31) x [7by,7by+7) x [zc bz, zc bz+zc)
-// and computes element columns 31bx-1 .. 31bx+30 (x), 7by-1 .. 7by+6 (y),
-// element layers zc bz - 1 .. zc bz + zc - 1.  The next node plane, the next
-// layer's fields and sizes are loaded one layer ahead.
+// Element apply in registers: nodal inputs x1 (and x2) in local node order,
+// outputs y1 = A_e x1 (and y2 = A_e^T x2).  Invalid elements give zeros.
 // ---------------------------------------------------------------------------
-template <typename T, typename TIO, bool DUAL, int TX, int TY>
+template <typename T, bool DUAL, bool ROBIN>
+__device__ __forceinline__ void elem_eval(const OpData<T> &op, bool valid,
+                                          bool robin, const ColConst<T> &q,
+                                          const Coef<T> &cf, const T x1[8],
+                                          const T x2[8], T y1[8], T y2[8]) {
+  ElemConst<T> k;
+  elem_layer_setup<T>(q, cf.hz, cf.ihz, cf.lam, cf.vx, cf.vy, cf.vz, cf.tau,
+                      T(op.cH), T(op.mu), k);
+  T xh[8], yh[8];
+  haar_fwd<T>(x1, xh);
+  haar_apply<T, false>(k, xh, yh);
+  haar_inv<T>(yh, y1);
+  if (ROBIN && robin)
+    robin_add<T, false>(k, q.mx0 * T(4), cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x1, y1);
+  if (DUAL) {
+    haar_fwd<T>(x2, xh);
+    haar_apply<T, true>(k, xh, yh);
+    haar_inv<T>(yh, y2);
+    if (ROBIN && robin)
+      robin_add<T, true>(k, q.mx0 * T(4), cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x2, y2);
+  }
+  if (!valid) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) y1[c] = y2[c] = T(0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// OWNER kernel (persistent, balanced).  The node columns are cut into
+// overlapping TX x TY element-column tiles that own (TX-1) x (TY-1) node
+// columns each; a work unit is one node plane of one tile, units are ordered
+// tile-major (z fastest) and block b takes the contiguous range
+// [b U / G, (b+1) U / G) of the U = tiles * naz units, split into one march
+// per tile it touches.  Producing planes k0..k1-1 of a tile computes element
+// layers k0-1..k1-1.
+//
+// Per layer: one barrier.  Node planes live in three LDS buffers (plane p in
+// buffer p mod 3), so the buffer written for plane kl+1 was last read two
+// layers ago.  Every thread sums the top-face result of its element column's
+// previous layer with the bottom-face result of the current one in registers
+// and stages the 4 column contributions of plane kl; the owner of a node adds
+// the 4 columns around it one layer later (double-buffered stage).  The next
+// plane and the next layer's fields are loaded one layer ahead.
+// ---------------------------------------------------------------------------
+template <typename T, typename TIO, bool DUAL, bool ROBIN, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k_owner(OpData<T> op, int zc,
-                                                const TIO *__restrict__ x1,
-                                                TIO *__restrict__ y1,
-                                                const TIO *__restrict__ x2,
-                                                TIO *__restrict__ y2,
-                                                double *__restrict__ dot_part) {
-  __shared__ T pl1[2][((TX + 1) * (TY + 1))];
-  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? ((TX + 1) * (TY + 1)) : 1];
-  __shared__ uint8_t pm[2][((TX + 1) * (TY + 1))];
-  __shared__ T st1[8 * (TX * TY)];
-  __shared__ T st2[DUAL ? 8 * (TX * TY) : 1];
-  __shared__ double red[(TX * TY) / 64];
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k_owner(
+    OpData<T> op, const TIO *__restrict__ x1, TIO *__restrict__ y1,
+    const TIO *__restrict__ x2, TIO *__restrict__ y2,
+    double *__restrict__ dot_part) {
+  constexpr int NP = (TX + 1) * (TY + 1);
+  constexpr int NT = TX * TY;
+  __shared__ T pl1[3][NP];
+  __shared__ T pl2[DUAL ? 3 : 1][DUAL ? NP : 1];
+  __shared__ uint8_t pm[3][NP];
+  __shared__ T st1[2][4 * NT];
+  __shared__ T st2[DUAL ? 2 : 1][DUAL ? 4 * NT : 1];
+  __shared__ double red[NT / 64];
   if (op.done && *op.done) return;
 
   const int tid = threadIdx.x;
-  const int tx = tid & (TX - 1), ty = tid / TX;
+  const int tx = tid % TX, ty = tid / TX;
+  const int naz = op.g.naz;
   const int ntx = (op.g.nax + TX - 2) / (TX - 1), nty = (op.g.nay + TY - 2) / (TY - 1);
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
-  const int gy0 = ((tile / ntx) % nty) * (TY - 1) - 1;
-  const int kz0 = (tile / (ntx * nty)) * zc;  // first owned node plane
-  const int exg = gx0 + tx, eyg = gy0 + ty;
-  const bool colok = exg >= 0 && exg < op.g.ex && eyg >= 0 && eyg < op.g.ey;
-  const T hx = colok ? op.hx[exg] : T(1), ihx = colok ? op.ihx[exg] : T(1);
-  const T hy = colok ? op.hy[eyg] : T(1), ihy = colok ? op.ihy[eyg] : T(1);
-  const bool robin = colok && exg == 0 && op.ra != 0.0;
+  const int64_t U = (int64_t)ntx * nty * naz;
+  const int64_t G = gridDim.x, lb = xcd_tile(blockIdx.x, gridDim.x);
+  int64_t u = U * lb / G;
+  const int64_t uend = U * (lb + 1) / G;
 
-  const int q0 = tid, q1 = tid + (TX * TY);
+  const int q0 = tid, q1 = tid + NT;
   const int lx0 = q0 % (TX + 1), ly0 = q0 / (TX + 1);
   const int lx1 = q1 % (TX + 1), ly1 = q1 / (TX + 1);
-  const bool has1 = q1 < ((TX + 1) * (TY + 1));
+  const bool has1 = q1 < NP;
+  const bool own = tx < (TX - 1) && ty < (TY - 1);  // owns node (tx+1, ty+1)
+  const int oq = (ty + 1) * (TX + 1) + tx + 1;
+  const int c00 = ty * (TX + 1) + tx;  // tile node of local corner 0
+  double dot = 0.0;
 
-  const int ox = tx, oy = ty;  // owner node of this thread
-  const bool own = ox < (TX - 1) && oy < (TY - 1);
-  const int ngx = gx0 + 1 + ox, ngy = gy0 + 1 + oy;
-  const bool owng = own && ngx < op.g.nax && ngy < op.g.nay;
-  const int oq = (oy + 1) * (TX + 1) + ox + 1;
+  while (u < uend) {
+    const int tile = (int)(u / naz);
+    const int k0 = (int)(u - (int64_t)tile * naz);
+    const int k1 = (int)min<int64_t>((int64_t)naz, k0 + (uend - u));
+    u += k1 - k0;
 
-  auto evalid = [&](int k) { return colok && k >= 0 && k < op.g.ez; };
-  auto eidx = [&](int k) {
-    return exg + (int64_t)op.g.ex * (eyg + (int64_t)op.g.ey * k);
-  };
+    const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
+    const int gy0 = (tile / ntx) * (TY - 1) - 1;
+    const int exg = gx0 + tx, eyg = gy0 + ty;
+    const bool colok = exg >= 0 && exg < op.g.ex && eyg >= 0 && eyg < op.g.ey;
+    ColConst<T> cq;
+    col_setup<T>(colok ? op.hx[exg] : T(1), colok ? op.hy[eyg] : T(1),
+                 colok ? op.ihx[exg] : T(1), colok ? op.ihy[eyg] : T(1), cq);
+    const bool robin = ROBIN && colok && exg == 0;
+    const int ngx = gx0 + 1 + tx, ngy = gy0 + 1 + ty;
+    const bool owng = own && ngx < op.g.nax && ngy < op.g.nay;
+    const int64_t ncol = ngx + (int64_t)op.g.nax * ngy;
+    const int64_t plane = (int64_t)op.g.nax * op.g.nay;
+    const int64_t ecol = exg + (int64_t)op.g.ex * eyg;
+    const int64_t eplane = (int64_t)op.g.ex * op.g.ey;
 
-  int kl = kz0 - 1;
-  PlaneRegs<T, DUAL> pr;
-  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl, lx0, ly0, lx1, ly1, has1, pr);
-  plane_store<T, DUAL>(pr, pl1[0], DUAL ? pl2[0] : nullptr, pm[0], q0, q1, has1);
-  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 1, lx0, ly0, lx1, ly1, has1, pr);
-  Coef<T> cn = load_coef<T>(op, evalid(kl), eidx(kl), kl);
+    // prologue: plane k0-1 into its buffer, plane k0 and layer k0-1 ahead
+    int kl = k0 - 1;
+    int bb = (kl + 3) % 3, bt = (kl + 4) % 3, bn = (kl + 5) % 3;
+    PlaneRegs<TIO, DUAL> pr;
+    plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl, lx0, ly0, lx1, ly1, has1, pr);
+    plane_store<T, TIO, DUAL>(pr, pl1[bb], DUAL ? pl2[bb] : nullptr, pm[bb], q0, q1, has1);
+    plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 1, lx0, ly0, lx1, ly1, has1, pr);
+    const bool ev0 = colok && kl >= 0;
+    Coef<T> cn = load_coef<T>(op, ev0, ecol + eplane * (ev0 ? kl : 0), max(kl, 0));
 
-  double acc1 = 0.0, acc2 = 0.0, dot = 0.0;
-  for (int it = 0; it <= zc; ++it, ++kl) {
-    const int cb = it & 1, ct = cb ^ 1;
-    plane_store<T, DUAL>(pr, pl1[ct], DUAL ? pl2[ct] : nullptr, pm[ct], q0, q1, has1);
-    const Coef<T> cf = cn;
-    const bool ev = evalid(kl);
-    if (it < zc) {  // prefetch plane kl+2 and layer kl+1
-      plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
-                               has1, pr);
-      cn = load_coef<T>(op, evalid(kl + 1), eidx(kl + 1), kl + 1);
-    }
-    lds_barrier();
-    // this thread's owned node of plane kl, read before a faster thread can
-    // refill the buffer with plane kl+2
-    const uint8_t om = pm[cb][oq];
-    const double xo2 = (DUAL && own) ? (double)pl2[DUAL ? cb : 0][oq] : 0.0;
-    elem_step<T, DUAL, TX, TY>(op, ev, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
-                       DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
-                       ty, st1, DUAL ? st2 : nullptr, tid);
-    lds_barrier();
-    if (own) {
-      const double v1 = acc1 + node_sum<T, TX, TY>(st1, ox + 1, oy + 1, false);
-      acc1 = node_sum<T, TX, TY>(st1, ox + 1, oy + 1, true);
-      double v2 = 0.0;
-      if (DUAL) {
-        v2 = acc2 + node_sum<T, TX, TY>(st2, ox + 1, oy + 1, false);
-        acc2 = node_sum<T, TX, TY>(st2, ox + 1, oy + 1, true);
+    T yt1[4], yt2[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) yt1[c] = yt2[c] = T(0);
+    uint8_t om_prev = 1;
+    double xo_prev = 0.0;
+    for (; kl < k1; ++kl) {
+      // plane kl+1 into buffer bt; buffer bt held plane kl-2, last read in
+      // the layer before the previous barrier
+      plane_store<T, TIO, DUAL>(pr, pl1[bt], DUAL ? pl2[bt] : nullptr, pm[bt], q0, q1, has1);
+      const Coef<T> cf = cn;
+      const bool ev = colok && kl >= 0 && kl < op.g.ez;
+      if (kl + 1 < k1) {  // prefetch plane kl+2 and layer kl+1
+        plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
+                                 has1, pr);
+        const bool evn = colok && kl + 1 < op.g.ez;
+        cn = load_coef<T>(op, evn, ecol + eplane * (evn ? kl + 1 : 0), kl + 1);
       }
-      if (owng && it >= 1 && kl < op.g.naz) {
-        const int64_t n = ngx + (int64_t)op.g.nax * (ngy + (int64_t)op.g.nay * kl);
-        const TIO o1 = om ? TIO(0) : TIO(v1);
+      lds_barrier();
+      // owner sums of plane kl-1 (staged in the previous layer)
+      const bool emit = kl - 1 >= k0;
+      double v1 = 0.0, v2 = 0.0;
+      if (emit && own) {
+        const T *s = st1[(kl - 1) & 1];
+        v1 = (double)s[2 * NT + tid] + (double)s[3 * NT + tid + 1] +
+             (double)s[1 * NT + tid + TX] + (double)s[tid + TX + 1];
+        if (DUAL) {
+          const T *s2 = st2[DUAL ? ((kl - 1) & 1) : 0];
+          v2 = (double)s2[2 * NT + tid] + (double)s2[3 * NT + tid + 1] +
+               (double)s2[1 * NT + tid + TX] + (double)s2[tid + TX + 1];
+        }
+      }
+      // this layer's corner values from the two LDS planes
+      T xa[8], xc[8];
+      const uint8_t om = pm[bb][oq];
+      const double xo = (DUAL && own) ? (double)pl2[DUAL ? bb : 0][oq] : 0.0;
+      xa[0] = pl1[bb][c00];
+      xa[1] = pl1[bb][c00 + 1];
+      xa[2] = pl1[bb][c00 + TX + 2];
+      xa[3] = pl1[bb][c00 + TX + 1];
+      xa[4] = pl1[bt][c00];
+      xa[5] = pl1[bt][c00 + 1];
+      xa[6] = pl1[bt][c00 + TX + 2];
+      xa[7] = pl1[bt][c00 + TX + 1];
+      if (DUAL) {
+        xc[0] = pl2[DUAL ? bb : 0][c00];
+        xc[1] = pl2[DUAL ? bb : 0][c00 + 1];
+        xc[2] = pl2[DUAL ? bb : 0][c00 + TX + 2];
+        xc[3] = pl2[DUAL ? bb : 0][c00 + TX + 1];
+        xc[4] = pl2[DUAL ? bt : 0][c00];
+        xc[5] = pl2[DUAL ? bt : 0][c00 + 1];
+        xc[6] = pl2[DUAL ? bt : 0][c00 + TX + 2];
+        xc[7] = pl2[DUAL ? bt : 0][c00 + TX + 1];
+      }
+      T ya[8], yc[8];
+      elem_eval<T, DUAL, ROBIN>(op, ev, robin, cq, cf, xa, xc, ya, yc);
+      // column contributions to plane kl
+      if (kl >= k0) {
+        T *s = st1[kl & 1];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) s[c * NT + tid] = yt1[c] + ya[c];
+        if (DUAL) {
+          T *s2 = st2[DUAL ? (kl & 1) : 0];
+#pragma unroll
+          for (int c = 0; c < 4; ++c) s2[c * NT + tid] = yt2[c] + yc[c];
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        yt1[c] = ya[c + 4];
+        yt2[c] = yc[c + 4];
+      }
+      if (emit && owng) {
+        const int64_t n = ncol + plane * (kl - 1);
+        const TIO o1 = om_prev ? TIO(0) : TIO(v1);
         y1[n] = o1;
         if (DUAL) {
-          y2[n] = om ? TIO(0) : TIO(v2);
-          if (dot_part) dot += (double)o1 * xo2;
+          y2[n] = om_prev ? TIO(0) : TIO(v2);
+          if (dot_part) dot += (double)o1 * xo_prev;
+        }
+      }
+      om_prev = om;
+      xo_prev = xo;
+      const int bfree = bb;
+      bb = bt;
+      bt = bn;
+      bn = bfree;
+    }
+    // epilogue: plane k1-1
+    lds_barrier();
+    if (own && k1 - 1 >= k0) {
+      const T *s = st1[(k1 - 1) & 1];
+      const double v1 = (double)s[2 * NT + tid] + (double)s[3 * NT + tid + 1] +
+                        (double)s[1 * NT + tid + TX] + (double)s[tid + TX + 1];
+      double v2 = 0.0;
+      if (DUAL) {
+        const T *s2 = st2[DUAL ? ((k1 - 1) & 1) : 0];
+        v2 = (double)s2[2 * NT + tid] + (double)s2[3 * NT + tid + 1] +
+             (double)s2[1 * NT + tid + TX] + (double)s2[tid + TX + 1];
+      }
+      if (owng) {
+        const int64_t n = ncol + plane * (k1 - 1);
+        const TIO o1 = om_prev ? TIO(0) : TIO(v1);
+        y1[n] = o1;
+        if (DUAL) {
+          y2[n] = om_prev ? TIO(0) : TIO(v2);
+          if (dot_part) dot += (double)o1 * xo_prev;
         }
       }
     }
+    lds_barrier();  // buffers are refilled by the next march
   }
   if (dot_part) {
     // deterministic block reduction of y1 . x2 (masked x2 from LDS)
@@ -277,9 +351,37 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     __syncthreads();
     if (tid == 0) {
       double s = 0.0;
-      for (int w = 0; w < (TX * TY) / 64; ++w) s += red[w];
+      for (int w = 0; w < NT / 64; ++w) s += red[w];
       dot_part[blockIdx.x] = s;
     }
+  }
+}
+// Element apply of the brick kernels: corners gathered from the two LDS
+// planes, the 8 nodal results (8 transposed results when DUAL) written to the
+// staging arrays, corner-major.
+template <typename T, bool DUAL, int TX, int TY>
+__device__ __forceinline__ void elem_stage(const OpData<T> &op, bool valid,
+                                           bool robin, const ColConst<T> &q,
+                                           const Coef<T> &cf, const T *pb,
+                                           const T *pt, const T *pb2,
+                                           const T *pt2, int tx, int ty,
+                                           T *stage, T *stage2, int tid) {
+  const int c00 = ty * (TX + 1) + tx;
+  const int off[4] = {0, 1, TX + 2, TX + 1};
+  T xa[8], xc[8], ya[8], yc[8];
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    xa[c] = pb[c00 + off[c]];
+    xa[c + 4] = pt[c00 + off[c]];
+    xc[c] = DUAL ? pb2[c00 + off[c]] : T(0);
+    xc[c + 4] = DUAL ? pt2[c00 + off[c]] : T(0);
+  }
+  elem_eval<T, DUAL, true>(op, valid, robin, q, cf, xa, xc, ya, yc);
+#pragma unroll
+  for (int c = 0; c < 8; ++c) stage[c * (TX * TY) + tid] = ya[c];
+  if (DUAL) {
+#pragma unroll
+    for (int c = 0; c < 8; ++c) stage2[c * (TX * TY) + tid] = yc[c];
   }
 }
 
@@ -324,8 +426,9 @@ __global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int col
   const int gx0 = bx * TX, gy0 = by * TY;
   const int exg = gx0 + tx, eyg = gy0 + ty;
   const bool colok = exg < op.g.ex && eyg < op.g.ey;
-  const T hx = colok ? op.hx[exg] : T(1), ihx = colok ? op.ihx[exg] : T(1);
-  const T hy = colok ? op.hy[eyg] : T(1), ihy = colok ? op.ihy[eyg] : T(1);
+  ColConst<T> cq;
+  col_setup<T>(colok ? op.hx[exg] : T(1), colok ? op.hy[eyg] : T(1),
+               colok ? op.ihx[exg] : T(1), colok ? op.ihy[eyg] : T(1), cq);
   const bool robin = colok && exg == 0 && op.ra != 0.0;
 
   const int q0 = tid, q1 = tid + (TX * TY);
@@ -352,15 +455,15 @@ __global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int col
     }
   };
 
-  PlaneRegs<T, DUAL> pr;
+  PlaneRegs<TIO, DUAL> pr;
   plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kz0, lx0, ly0, lx1, ly1, has1, pr);
-  plane_store<T, DUAL>(pr, pl1[0], DUAL ? pl2[0] : nullptr, pm[0], q0, q1, has1);
+  plane_store<T, TIO, DUAL>(pr, pl1[0], DUAL ? pl2[0] : nullptr, pm[0], q0, q1, has1);
   plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kz0 + 1, lx0, ly0, lx1, ly1, has1, pr);
   Coef<T> cn = load_coef<T>(op, colok, eidx(kz0), kz0);
   double a1[2] = {0.0, 0.0}, a2[2] = {0.0, 0.0};
   for (int kl = kz0; kl < kz1; ++kl) {
     const int cb = (kl - kz0) & 1, ct = cb ^ 1;
-    plane_store<T, DUAL>(pr, pl1[ct], DUAL ? pl2[ct] : nullptr, pm[ct], q0, q1, has1);
+    plane_store<T, TIO, DUAL>(pr, pl1[ct], DUAL ? pl2[ct] : nullptr, pm[ct], q0, q1, has1);
     const Coef<T> cf = cn;
     if (kl + 1 < kz1) {
       plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
@@ -369,7 +472,7 @@ __global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int col
     }
     lds_barrier();
     const uint8_t mk0 = pm[cb][q0], mk1 = has1 ? pm[cb][q1] : uint8_t(1);
-    elem_step<T, DUAL, TX, TY>(op, colok, robin, hx, hy, ihx, ihy, cf, pl1[cb], pl1[ct],
+    elem_stage<T, DUAL, TX, TY>(op, colok, robin, cq, cf, pl1[cb], pl1[ct],
                        DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx,
                        ty, st1, DUAL ? st2 : nullptr, tid);
     lds_barrier();
@@ -422,27 +525,107 @@ static void owner_shape(int tile, int *tx, int *ty) {
   }
 }
 
-int owner_num_tiles(const Grid3 &g, int zc, int tile) {
+namespace {
+template <typename T, typename TIO, bool DUAL, bool ROBIN, int TX, int TY>
+const void *owner_kernel() {
+  return reinterpret_cast<const void *>(&k_owner<T, TIO, DUAL, ROBIN, TX, TY>);
+}
+
+template <typename T, typename TIO, int TX, int TY>
+const void *owner_kernel_for(bool dual, bool robin) {
+  if (dual)
+    return robin ? owner_kernel<T, TIO, true, true, TX, TY>()
+                 : owner_kernel<T, TIO, true, false, TX, TY>();
+  return robin ? owner_kernel<T, TIO, false, true, TX, TY>()
+               : owner_kernel<T, TIO, false, false, TX, TY>();
+}
+
+// Resident workgroups of a kernel on the whole device (cached per kernel).
+int resident_blocks(const void *fn, int threads) {
+  static std::mutex mu;
+  static std::map<std::pair<int, const void *>, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 0;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find({dev, fn});
+  if (it != cache.end()) return it->second;
+  int per_cu = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    return 0;
+  const int r = std::max(1, per_cu) * std::max(1, cus);
+  cache[{dev, fn}] = r;
+  return r;
+}
+}  // namespace
+
+int owner_num_blocks(const Grid3 &g, int zc, int tile, int elem_bytes,
+                     int io_bytes, bool dual, bool robin) {
   int tx, ty;
   owner_shape(tile, &tx, &ty);
-  const int nx = (g.nax + tx - 2) / (tx - 1), ny = (g.nay + ty - 2) / (ty - 1);
-  const int nz = (g.naz + zc - 1) / zc;
-  return nx * ny * nz;
+  const int64_t ntiles = (int64_t)((g.nax + tx - 2) / (tx - 1)) *
+                         ((g.nay + ty - 2) / (ty - 1));
+  const int64_t units = ntiles * g.naz;
+  if (zc > 0) return (int)std::min<int64_t>(units, ntiles * ((g.naz + zc - 1) / zc));
+  const void *fn = nullptr;
+  const bool f64 = elem_bytes == 8, io64 = io_bytes == 8;
+  switch (tile) {
+    case 1:
+      fn = f64 ? owner_kernel_for<double, double, 16, 16>(dual, robin)
+               : (io64 ? owner_kernel_for<float, double, 16, 16>(dual, robin)
+                       : owner_kernel_for<float, float, 16, 16>(dual, robin));
+      break;
+    case 2:
+      fn = f64 ? owner_kernel_for<double, double, 64, 4>(dual, robin)
+               : (io64 ? owner_kernel_for<float, double, 64, 4>(dual, robin)
+                       : owner_kernel_for<float, float, 64, 4>(dual, robin));
+      break;
+    default:
+      fn = f64 ? owner_kernel_for<double, double, 32, 8>(dual, robin)
+               : (io64 ? owner_kernel_for<float, double, 32, 8>(dual, robin)
+                       : owner_kernel_for<float, float, 32, 8>(dual, robin));
+      break;
+  }
+  int r = resident_blocks(fn, tx * ty);
+  if (r <= 0) r = 1024;
+  // at least 4 planes per workgroup so the one-layer prologue stays small
+  return (int)std::max<int64_t>(1, std::min<int64_t>(r, units / 4));
+}
+
+int owner_max_blocks(const Grid3 &g) {
+  int64_t m = 1;
+  for (int t = 0; t < 3; ++t) {
+    int tx, ty;
+    owner_shape(t, &tx, &ty);
+    m = std::max<int64_t>(m, (int64_t)((g.nax + tx - 2) / (tx - 1)) *
+                                 ((g.nay + ty - 2) / (ty - 1)) * g.naz);
+  }
+  return (int)std::min<int64_t>(m, 1 << 30);
 }
 
 namespace {
 template <typename T, typename TIO, int TX, int TY>
-void launch_owner(const OpData<T> &op, int zc, const TIO *x1, TIO *y1,
-                  const TIO *x2, TIO *y2, double *dot_part, hipStream_t s) {
-  const Grid3 &g = op.g;
-  dim3 grid(((g.nax + TX - 2) / (TX - 1)) * ((g.nay + TY - 2) / (TY - 1)) *
-            ((g.naz + zc - 1) / zc));
-  if (x2)
-    hipLaunchKernelGGL((k_owner<T, TIO, true, TX, TY>), grid, dim3(TX * TY), 0, s,
-                       op, zc, x1, y1, x2, y2, dot_part);
-  else
-    hipLaunchKernelGGL((k_owner<T, TIO, false, TX, TY>), grid, dim3(TX * TY), 0,
-                       s, op, zc, x1, y1, x2, y2, nullptr);
+int launch_owner(const OpData<T> &op, int zc, const TIO *x1, TIO *y1,
+                 const TIO *x2, TIO *y2, double *dot_part, hipStream_t s) {
+  const bool dual = x2 != nullptr, robin = op.ra != 0.0;
+  const int nb = owner_num_blocks(op.g, zc, op.tile, sizeof(T), sizeof(TIO), dual, robin);
+  dim3 grid(nb), block(TX * TY);
+  if (dual) {
+    if (robin)
+      hipLaunchKernelGGL((k_owner<T, TIO, true, true, TX, TY>), grid, block, 0, s, op,
+                         x1, y1, x2, y2, dot_part);
+    else
+      hipLaunchKernelGGL((k_owner<T, TIO, true, false, TX, TY>), grid, block, 0, s, op,
+                         x1, y1, x2, y2, dot_part);
+  } else {
+    if (robin)
+      hipLaunchKernelGGL((k_owner<T, TIO, false, true, TX, TY>), grid, block, 0, s, op,
+                         x1, y1, x2, y2, nullptr);
+    else
+      hipLaunchKernelGGL((k_owner<T, TIO, false, false, TX, TY>), grid, block, 0, s, op,
+                         x1, y1, x2, y2, nullptr);
+  }
+  return nb;
 }
 }  // namespace
 
@@ -450,7 +633,6 @@ template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
                  TIO *y1, const TIO *x2, TIO *y2, double *dot_part,
                  hipStream_t s) {
-  if (zc <= 0) zc = 8;
   const bool dual = x2 != nullptr;
   const Grid3 &g = op.g;
   if (scatter == VGPU_SCATTER_OWNER) {
@@ -464,6 +646,7 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
   }
   // brick modes need zeroed outputs; bricks are 32 x 8 element columns
   constexpr int BX = 32, BY = 8;
+  if (zc <= 0) zc = 8;
   hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
                      dual ? y2 : nullptr, g.N);
   const int ntx = (g.ex + BX - 1) / BX, nty = (g.ey + BY - 1) / BY;
