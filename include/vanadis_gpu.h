@@ -164,6 +164,41 @@ int vgpu_memcpy_d2h(vgpu_ctx *ctx, void *dst, const void *src_dev, int64_t bytes
 int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
                      int reps, double *avg_ms);
 
+/* Autotuner (no counterpart in the reference, whose CUDA launches use fixed
+ * 256-thread blocks, cudatest.cu:339-347): times the dual apply of `precision`
+ * for every owner tile shape and z chunk in {0, 4, 8, 16, 32} (`reps` launches
+ * each), keeps the fastest on the context and returns it.  Every owner node is
+ * summed by one thread in a fixed order, so the choice does not change any
+ * result bit. */
+int vgpu_autotune(vgpu_ctx *ctx, int precision, int reps, int *scatter,
+                  int *tile, int *zchunk, double *best_ms);
+
+/* ---- Batched solves (no counterpart in the reference, whose host solves one
+ * system per call, VANADIS.FOR:809-810) ----
+ * A batch groups nm contexts on one device (each with its own mesh, mask,
+ * coefficients and scheme; several right-hand sides of one operator are
+ * several contexts).  Every BiCG iteration runs one persistent owner launch
+ * over all systems plus two vector launches, captured in one hipGraph; the
+ * systems stop independently.  The batch uses the owner tile shape and z
+ * chunk of ctxs[0]; with the same tile shape each system's iterates are
+ * bitwise those of vgpu_solve on its context.  Contexts must outlive the
+ * batch; coefficient or option changes are picked up at the next solve. */
+typedef struct vgpu_batch vgpu_batch;
+int vgpu_batch_create(int nm, vgpu_ctx *const *ctxs, vgpu_batch **out);
+int vgpu_batch_destroy(vgpu_batch *batch);
+/* b[m], x[m]: host vectors of system m (x in/out).  precision VGPU_FP64 or
+ * VGPU_FP32.  iters/errs/status [nm]: status 1 converged, 2 itmax,
+ * 3 breakdown.  Returns VGPU_OK when every system converged. */
+int vgpu_batch_solve(vgpu_batch *batch, int precision, const double *const *b,
+                     double *const *x, double rtol, int itmax, int *iters,
+                     double *errs, int *status);
+/* Device-resident form: b, x in the batch layout (system m at offsets[m]
+ * of a vector of `total` doubles; padding between systems is ignored). */
+int vgpu_batch_layout(vgpu_batch *batch, int64_t *offsets, int64_t *total);
+int vgpu_batch_solve_dev(vgpu_batch *batch, int precision, const double *b,
+                         double *x, double rtol, int itmax, int *iters,
+                         double *errs, int *status, double *seconds);
+
 #ifdef __cplusplus
 }
 #endif

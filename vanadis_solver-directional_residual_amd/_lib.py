@@ -29,7 +29,8 @@ EXPORTED = (
     "vgpu_get_sizes", "vgpu_set_coeffs", "vgpu_set_scheme", "vgpu_set_options", "vgpu_apply",
     "vgpu_apply_dev", "vgpu_diag", "vgpu_assemble_rhs", "vgpu_solve", "vgpu_solve_dev",
     "vgpu_true_residual", "vgpu_mass", "vgpu_dev_alloc", "vgpu_dev_free", "vgpu_memcpy_h2d",
-    "vgpu_memcpy_d2h", "vgpu_time_kernel",
+    "vgpu_memcpy_d2h", "vgpu_time_kernel", "vgpu_autotune", "vgpu_batch_create",
+    "vgpu_batch_destroy", "vgpu_batch_solve", "vgpu_batch_layout", "vgpu_batch_solve_dev",
 )
 
 
@@ -90,6 +91,14 @@ def load():
     L.vgpu_memcpy_h2d.argtypes = [_vp, _vp, _vp, C.c_int64]
     L.vgpu_memcpy_d2h.argtypes = [_vp, _vp, _vp, C.c_int64]
     L.vgpu_time_kernel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _pd]
+    L.vgpu_autotune.argtypes = [_vp, C.c_int, C.c_int, _pi, _pi, _pi, _pd]
+    L.vgpu_batch_create.argtypes = [C.c_int, C.POINTER(_vp), C.POINTER(_vp)]
+    L.vgpu_batch_destroy.argtypes = [_vp]
+    L.vgpu_batch_solve.argtypes = [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double,
+                                   C.c_int, _pi, _pd, _pi]
+    L.vgpu_batch_layout.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
+    L.vgpu_batch_solve_dev.argtypes = [_vp, C.c_int, _vp, _vp, C.c_double, C.c_int, _pi, _pd,
+                                       _pi, _pd]
     _lib = L
     return L
 
@@ -257,6 +266,81 @@ class Context:
         check(load().vgpu_time_kernel(self.h, int(what), scatter_code(scatter), int(precision),
                                       int(reps), C.byref(ms)), "time_kernel")
         return ms.value
+
+    def autotune(self, precision=FP64, reps=20):
+        """Time every owner tile shape / z chunk, keep the fastest; returns
+        {"scatter", "tile", "zchunk", "ms"}."""
+        sc, t, z = C.c_int(0), C.c_int(0), C.c_int(0)
+        ms = C.c_double(0)
+        check(load().vgpu_autotune(self.h, int(precision), int(reps), C.byref(sc), C.byref(t),
+                                   C.byref(z), C.byref(ms)), "autotune")
+        return {"scatter": sc.value, "tile": t.value, "zchunk": z.value, "ms": ms.value}
+
+
+class Batch:
+    """Several contexts solved together (vgpu_batch_*): one persistent owner
+    launch per BiCG iteration over all systems, one hipGraph per batch."""
+
+    def __init__(self, contexts):
+        self.ctxs = list(contexts)
+        arr = (_vp * len(self.ctxs))(*[c.h for c in self.ctxs])
+        h = _vp()
+        check(load().vgpu_batch_create(len(self.ctxs), arr, C.byref(h)), "batch_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().vgpu_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def layout(self):
+        off = (C.c_int64 * len(self.ctxs))()
+        tot = C.c_int64(0)
+        check(load().vgpu_batch_layout(self.h, off, C.byref(tot)), "batch_layout")
+        return list(off), tot.value
+
+    def solve(self, bs, x0s=None, rtol=1e-8, itmax=20000, precision=FP64,
+              raise_on_fail=True):
+        """Returns (xs, iters, errs, status, rc); xs[m] is system m's solution."""
+        nm = len(self.ctxs)
+        bs = [_f64(b) for b in bs]
+        xs = [np.zeros(c.N) if x0s is None else np.array(x0s[i], dtype=np.float64, copy=True)
+              for i, c in enumerate(self.ctxs)]
+        bp = (_vp * nm)(*[b.ctypes.data_as(_vp) for b in bs])
+        xp = (_vp * nm)(*[x.ctypes.data_as(_vp) for x in xs])
+        it = np.zeros(nm, dtype=np.int32)
+        er = np.zeros(nm)
+        st = np.zeros(nm, dtype=np.int32)
+        rc = load().vgpu_batch_solve(self.h, int(precision), bp, xp, float(rtol), int(itmax),
+                                     it.ctypes.data_as(_pi), er.ctypes.data_as(_pd),
+                                     st.ctypes.data_as(_pi))
+        if raise_on_fail and rc != OK:
+            check(rc, "batch_solve")
+        return xs, it, er, st, rc
+
+    def solve_dev(self, b_dev, x_dev, rtol=1e-8, itmax=20000, precision=FP64):
+        nm = len(self.ctxs)
+        it = np.zeros(nm, dtype=np.int32)
+        er = np.zeros(nm)
+        st = np.zeros(nm, dtype=np.int32)
+        sec = C.c_double(0)
+        rc = load().vgpu_batch_solve_dev(self.h, int(precision), b_dev, x_dev, float(rtol),
+                                         int(itmax), it.ctypes.data_as(_pi),
+                                         er.ctypes.data_as(_pd), st.ctypes.data_as(_pi),
+                                         C.byref(sec))
+        return it, er, st, sec.value, rc
 
 
 def device_count() -> int:

@@ -6,7 +6,7 @@
 #include <cstring>
 #include <vector>
 
-#include "vgpu_bicg.hpp"
+#include "vgpu_bbicg.hpp"
 #include "vgpu_internal.hpp"
 
 namespace vgpu {
@@ -63,6 +63,12 @@ __global__ void k_refine(int64_t n, double *x, double s, const float *d) {
     x[i] += s * (double)d[i];
 }
 
+__global__ void k_fill(int64_t n, double v, double *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = v;
+}
+
 __global__ void k_copy(int64_t n, const double *a, double *b) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -102,8 +108,12 @@ struct vgpu_ctx {
   double *vb = nullptr, *vx_ = nullptr, *vt = nullptr, *vt2 = nullptr,
          *red = nullptr;
   float *fb = nullptr, *fx = nullptr;
-  Bicg<double> bicg64;
+  double *tpart = nullptr;  // dot partials of vgpu_time_kernel
+  int64_t tpart_n = 0;
+  Bicg<double> bicg64;  // brick scatter modes
   Bicg<float> bicg32;
+  BBicg<double> bb64;   // owner scatter: batched engine, one system
+  BBicg<float> bb32;
   uint64_t version = 1;
   std::vector<void *> user_allocs;
 
@@ -208,8 +218,8 @@ double host_sum(const double *dev, int n, hipStream_t s, int *rc) {
   return t;
 }
 
-// capacity of owner-mode dot partials over every tile shape and z chunk
-int max_owner_tiles(const Grid3 &g) { return owner_max_blocks(g); }
+// dot-partial capacity of the brick-mode solver (k_dot partials)
+int max_owner_tiles(const Grid3 &) { return 256; }
 
 int nparts_for(int64_t n) {
   return (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));
@@ -221,11 +231,6 @@ DualOp<double> make_dual64(vgpu_ctx *c, int scatter) {
                       double *pa, int *na, const int *done, hipStream_t s) {
     OpData<double> op = c->op64(c->cH, c->cC);
     op.done = done;
-    if (scatter == VGPU_SCATTER_OWNER) {
-      *na = owner_num_blocks(c->g, c->zchunk, c->tile, 8, 8, true, op.ra != 0.0);
-      return launch_apply<double, double>(op, scatter, c->zchunk, p, z, pp, zz,
-                                          pa, s);
-    }
     int rc = launch_apply<double, double>(op, scatter, c->zchunk, p, z, pp, zz,
                                           nullptr, s);
     if (rc) return rc;
@@ -245,11 +250,6 @@ DualOp<float> make_dual32(vgpu_ctx *c, int scatter) {
                       double *pa, int *na, const int *done, hipStream_t s) {
     OpData<float> op = c->op32();
     op.done = done;
-    if (scatter == VGPU_SCATTER_OWNER) {
-      *na = owner_num_blocks(c->g, c->zchunk, c->tile, 4, 4, true, op.ra != 0.0);
-      return launch_apply<float, float>(op, scatter, c->zchunk, p, z, pp, zz,
-                                        pa, s);
-    }
     int rc = launch_apply<float, float>(op, scatter, c->zchunk, p, z, pp, zz,
                                         nullptr, s);
     if (rc) return rc;
@@ -263,9 +263,49 @@ DualOp<float> make_dual32(vgpu_ctx *c, int scatter) {
   };
 }
 
+template <typename TV>
+OpData<TV> op_for(const vgpu_ctx *c);
+template <>
+OpData<double> op_for<double>(const vgpu_ctx *c) {
+  return c->op64(c->cH, c->cC);
+}
+template <>
+OpData<float> op_for<float>(const vgpu_ctx *c) {
+  return c->op32();
+}
+
+// One system on the batched engine (owner scatter).  status pointer of the
+// dual callback doubles as OpData::done (SolveStatus for owner launches).
+template <typename TV>
+int solve_owner(vgpu_ctx *c, BBicg<TV> &eng, const TV *b, TV *x, const TV *diag,
+                double tol, int itmax, int *iters, double *err, int *status,
+                double divtol = 1e300) {
+  int rc = eng.alloc(make_segments({c->g.N}, {owner_units(c->g, c->tile)}));
+  if (rc) return rc;
+  BDualOp<TV> dual = [c](const TV *p, TV *z, const TV *pp, TV *zz, double *pa,
+                         const SolveStatus *st, hipStream_t s) {
+    OpData<TV> op = op_for<TV>(c);
+    op.done = reinterpret_cast<const int *>(st);
+    return launch_apply<TV, TV>(op, VGPU_SCATTER_OWNER, c->zchunk, p, z, pp, zz,
+                                pa, s);
+  };
+  BSingleOp<TV> single = [c](const TV *xx, TV *y, hipStream_t s) {
+    OpData<TV> op = op_for<TV>(c);
+    return launch_apply<TV, TV>(op, VGPU_SCATTER_OWNER, c->zchunk, xx, y, nullptr,
+                                nullptr, nullptr, s);
+  };
+  const uint64_t key = (c->version << 8) | (uint64_t)(c->zchunk % 64) |
+                       (sizeof(TV) == 4 ? (1ull << 7) : 0);
+  return eng.solve(key, dual, single, c->mask, b, x, diag, tol, itmax, c->stream,
+                   iters, err, status, divtol);
+}
+
 int solve_fp64_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                    int itmax, int *iters, double *err, int *status) {
   const int scatter = c->scatter;
+  if (scatter == VGPU_SCATTER_OWNER)
+    return solve_owner<double>(c, c->bb64, b, x, c->diag, rtol, itmax, iters, err,
+                               status);
   if (c->bicg64.n != c->g.N) {
     int rc = c->bicg64.alloc(c->g.N, max_owner_tiles(c->g) + 256);
     if (rc) return rc;
@@ -301,17 +341,22 @@ int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
   hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
                      c->stream, N, x, c->fx);
   VGPU_HIP(hipGetLastError());
-  DualOp<float> dual = make_dual32(c, scatter);
-  SingleOp<float> single = [c, scatter](const float *xx, float *y,
-                                        hipStream_t s) {
-    OpData<float> op = c->op32();
-    return launch_apply<float, float>(op, scatter, c->zchunk, xx, y, nullptr,
-                                      nullptr, nullptr, s);
-  };
-  const uint64_t key = (c->version << 8) | (uint64_t)(scatter * 16 + c->zchunk % 16) | (1ull << 7);
-  int rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
-                           rtol, itmax, scatter != VGPU_SCATTER_OWNER, c->stream,
-                           iters, err, status);
+  int rc = VGPU_OK;
+  if (scatter == VGPU_SCATTER_OWNER) {
+    rc = solve_owner<float>(c, c->bb32, c->fb, c->fx, c->fdiag, rtol, itmax, iters,
+                            err, status);
+  } else {
+    DualOp<float> dual = make_dual32(c, scatter);
+    SingleOp<float> single = [c, scatter](const float *xx, float *y,
+                                          hipStream_t s) {
+      OpData<float> op = c->op32();
+      return launch_apply<float, float>(op, scatter, c->zchunk, xx, y, nullptr,
+                                        nullptr, nullptr, s);
+    };
+    const uint64_t key = (c->version << 8) | (uint64_t)(scatter * 16 + c->zchunk % 16) | (1ull << 7);
+    rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag, rtol,
+                         itmax, true, c->stream, iters, err, status);
+  }
   if (rc) return rc;
   hipLaunchKernelGGL((k_convert<float, double>), grid_for(N), dim3(256), 0,
                      c->stream, N, c->fx, x);
@@ -417,9 +462,13 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
     // inner FP32 BiCG on the scaled residual: capped, and stopped if its
     // recursive residual grows 100-fold (FP32 BiCG loses its recurrences
     // after large residual peaks); a stopped inner solve is not applied
-    rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
-                         inner_rtol, kInnerCap, scatter != VGPU_SCATTER_OWNER,
-                         c->stream, &it, &e, &st_inner, 1e2);
+    if (scatter == VGPU_SCATTER_OWNER)
+      rc = solve_owner<float>(c, c->bb32, c->fb, c->fx, c->fdiag, inner_rtol,
+                              kInnerCap, &it, &e, &st_inner, 1e2);
+    else
+      rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
+                           inner_rtol, kInnerCap, true, c->stream, &it, &e,
+                           &st_inner, 1e2);
     if (rc) return rc;
     total += it;
     if (st_inner >= 3) continue;  // next pass falls back to FP64
@@ -592,6 +641,8 @@ int vgpu_destroy(vgpu_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   c->bicg64.release();
   c->bicg32.release();
+  c->bb64.release();
+  c->bb32.release();
   free_dev(c->xc);
   free_dev(c->yc);
   free_dev(c->zc);
@@ -602,7 +653,8 @@ int vgpu_destroy(vgpu_ctx *c) {
     free_dev(c->fah[d]);
     free_dev(c->faih[d]);
   }
-  for (double **p : {&c->rec, &c->diag, &c->vb, &c->vx_, &c->vt, &c->vt2, &c->red})
+  for (double **p : {&c->rec, &c->diag, &c->vb, &c->vx_, &c->vt, &c->vt2, &c->red,
+                     &c->tpart})
     free_dev(*p);
   for (float **p : {&c->frec, &c->fdiag,
                     &c->fb, &c->fx})
@@ -874,6 +926,259 @@ int vgpu_solve(vgpu_ctx *c, int precision, const double *b, double *x,
   return rc;
 }
 
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched solves: several contexts (own mesh, coefficients, right-hand side)
+// on one device, solved by one engine: one persistent owner launch per
+// iteration over all systems, one hipGraph for the batch.
+// ---------------------------------------------------------------------------
+struct vgpu_batch {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<vgpu_ctx *> m;
+  std::vector<uint64_t> versions;
+  std::vector<Segment> seg;
+  int tile = 0, zchunk = 0;
+  bool robin = false;
+  int64_t ntot = 0, utot = 0;
+  Member<double> *md64 = nullptr;
+  Member<float> *md32 = nullptr;
+  uint8_t *mask = nullptr;
+  double *diag = nullptr, *b = nullptr, *x = nullptr;
+  float *fdiag = nullptr, *fb = nullptr, *fx = nullptr;
+  BBicg<double> e64;
+  BBicg<float> e32;
+  uint64_t gen = 0;
+
+  ~vgpu_batch() {
+    e64.release();
+    e32.release();
+    void *ptrs[] = {md64, md32, mask, diag, b, x, fdiag, fb, fx};
+    for (void *q : ptrs)
+      if (q) hipFree(q);
+    if (stream) hipStreamDestroy(stream);
+  }
+
+  // rebuild descriptors and concatenated mask / diagonals after any member
+  // changed its coefficients, scheme or options
+  int refresh() {
+    bool stale = !md64;
+    for (size_t i = 0; i < m.size(); ++i)
+      stale = stale || versions[i] != m[i]->version || !m[i]->diag_valid;
+    if (!stale) return VGPU_OK;
+    const int nm = (int)m.size();
+    std::vector<Member<double>> h64(nm);
+    std::vector<Member<float>> h32(nm);
+    robin = false;
+    for (int i = 0; i < nm; ++i) {
+      vgpu_ctx *c = m[i];
+      if (!c->have_coeffs) {
+        set_error("vgpu_batch: member " + std::to_string(i) + " has no coefficients");
+        return VGPU_ERR_STATE;
+      }
+      int rc = ensure_diag(c);
+      if (rc) return rc;
+      VGPU_HIP(hipStreamSynchronize(c->stream));
+      h64[i].op = c->op64(c->cH, c->cC);
+      h32[i].op = c->op32();
+      h64[i].op.tile = h32[i].op.tile = tile;
+      h64[i].seg = h32[i].seg = seg[i];
+      robin = robin || h64[i].op.ra != 0.0;
+      const int64_t o = seg[i].off, n = seg[i].n;
+      VGPU_HIP(hipMemcpyAsync(mask + o, c->mask, n, hipMemcpyDeviceToDevice, stream));
+      VGPU_HIP(hipMemcpyAsync(diag + o, c->diag, n * sizeof(double),
+                              hipMemcpyDeviceToDevice, stream));
+      VGPU_HIP(hipMemcpyAsync(fdiag + o, c->fdiag, n * sizeof(float),
+                              hipMemcpyDeviceToDevice, stream));
+      versions[i] = c->version;
+    }
+    VGPU_HIP(hipMemcpyAsync(md64, h64.data(), nm * sizeof(Member<double>),
+                            hipMemcpyHostToDevice, stream));
+    VGPU_HIP(hipMemcpyAsync(md32, h32.data(), nm * sizeof(Member<float>),
+                            hipMemcpyHostToDevice, stream));
+    VGPU_HIP(hipStreamSynchronize(stream));
+    ++gen;
+    return VGPU_OK;
+  }
+
+  template <typename TV>
+  int solve_tv(BBicg<TV> &eng, const Member<TV> *md, const TV *bb, TV *xx,
+               const TV *dg, double rtol, int itmax, int *iters, double *errs,
+               int *status) {
+    int rc = eng.alloc(seg);
+    if (rc) return rc;
+    const int nm = (int)m.size();
+    BDualOp<TV> dual = [this, md, nm](const TV *p, TV *z, const TV *pp, TV *zz,
+                                      double *pa, const SolveStatus *st,
+                                      hipStream_t s) {
+      return launch_owner_batch<TV, TV>(md, nm, utot, tile, zchunk, robin, st, p, z,
+                                        pp, zz, pa, s);
+    };
+    BSingleOp<TV> single = [this, md, nm](const TV *xi, TV *y, hipStream_t s) {
+      return launch_owner_batch<TV, TV>(md, nm, utot, tile, zchunk, robin, nullptr, xi,
+                                        y, nullptr, nullptr, nullptr, s);
+    };
+    const uint64_t key = (gen << 8) | (uint64_t)(zchunk % 64) |
+                         (sizeof(TV) == 4 ? (1ull << 7) : 0);
+    return eng.solve(key, dual, single, mask, bb, xx, dg, rtol, itmax, stream, iters,
+                     errs, status);
+  }
+};
+
+extern "C" {
+
+int vgpu_batch_create(int nm, vgpu_ctx *const *ctxs, vgpu_batch **out) {
+  if (!out || nm < 1 || !ctxs) {
+    set_error("vgpu_batch_create: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  *out = nullptr;
+  for (int i = 0; i < nm; ++i)
+    if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) {
+      set_error("vgpu_batch_create: null context or contexts on different devices");
+      return VGPU_ERR_ARG;
+    }
+  auto *bt = new (std::nothrow) vgpu_batch();
+  if (!bt) return VGPU_ERR_NOMEM;
+  bt->device = ctxs[0]->device;
+  bt->m.assign(ctxs, ctxs + nm);
+  bt->versions.assign(nm, 0);
+  bt->tile = ctxs[0]->tile;
+  bt->zchunk = ctxs[0]->zchunk;
+  std::vector<int64_t> n(nm), nu(nm);
+  for (int i = 0; i < nm; ++i) {
+    n[i] = ctxs[i]->g.N;
+    nu[i] = owner_units(ctxs[i]->g, bt->tile);
+  }
+  bt->seg = make_segments(n, nu);
+  bt->ntot = bt->seg.back().off + seg_pad(bt->seg.back().n);
+  bt->utot = bt->seg.back().u0 + bt->seg.back().nu;
+  auto fail = [&](const char *what, hipError_t e) {
+    set_error(std::string("vgpu_batch_create: ") + what + ": " + hipGetErrorString(e));
+    delete bt;
+    return e == hipErrorOutOfMemory ? VGPU_ERR_NOMEM : VGPU_ERR_HIP;
+  };
+  hipError_t e;
+  if ((e = hipSetDevice(bt->device)) != hipSuccess) return fail("device", e);
+  if ((e = hipStreamCreateWithFlags(&bt->stream, hipStreamNonBlocking)) != hipSuccess)
+    return fail("stream", e);
+  const size_t N = (size_t)bt->ntot;
+  if ((e = hipMalloc(&bt->md64, nm * sizeof(Member<double>))) != hipSuccess ||
+      (e = hipMalloc(&bt->md32, nm * sizeof(Member<float>))) != hipSuccess ||
+      (e = hipMalloc(&bt->mask, N)) != hipSuccess ||
+      (e = hipMalloc(&bt->diag, N * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&bt->b, N * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&bt->x, N * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&bt->fdiag, N * sizeof(float))) != hipSuccess ||
+      (e = hipMalloc(&bt->fb, N * sizeof(float))) != hipSuccess ||
+      (e = hipMalloc(&bt->fx, N * sizeof(float))) != hipSuccess)
+    return fail("allocation", e);
+  // padding between systems: masked, unit diagonal, zero vectors
+  if ((e = hipMemset(bt->mask, 1, N)) != hipSuccess ||
+      (e = hipMemset(bt->b, 0, N * sizeof(double))) != hipSuccess ||
+      (e = hipMemset(bt->x, 0, N * sizeof(double))) != hipSuccess)
+    return fail("memset", e);
+  hipLaunchKernelGGL(k_fill, grid_for(N), dim3(256), 0, bt->stream, (int64_t)N, 1.0,
+                     bt->diag);
+  hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0, bt->stream,
+                     (int64_t)N, bt->diag, bt->fdiag);
+  if ((e = hipStreamSynchronize(bt->stream)) != hipSuccess) return fail("init", e);
+  *out = bt;
+  return VGPU_OK;
+}
+
+int vgpu_batch_destroy(vgpu_batch *bt) {
+  if (!bt) return VGPU_ERR_ARG;
+  hipSetDevice(bt->device);
+  delete bt;
+  return VGPU_OK;
+}
+
+int vgpu_batch_layout(vgpu_batch *bt, int64_t *offsets, int64_t *total) {
+  if (!bt || !offsets || !total) return VGPU_ERR_ARG;
+  for (size_t i = 0; i < bt->seg.size(); ++i) offsets[i] = bt->seg[i].off;
+  *total = bt->ntot;
+  return VGPU_OK;
+}
+
+int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double *x,
+                         double rtol, int itmax, int *iters, double *errs,
+                         int *status, double *seconds) {
+  if (!bt || !b || !x || !iters || !errs || !status || itmax < 0 || !(rtol > 0)) {
+    set_error("vgpu_batch_solve: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  if (precision != VGPU_FP64 && precision != VGPU_FP32) {
+    set_error("vgpu_batch_solve: precision must be FP64 or FP32");
+    return VGPU_ERR_ARG;
+  }
+  VGPU_HIP(hipSetDevice(bt->device));
+  int rc = bt->refresh();
+  if (rc) return rc;
+  const int64_t N = bt->ntot;
+  VGPU_HIP(hipStreamSynchronize(bt->stream));
+  const auto t0 = std::chrono::steady_clock::now();
+  if (precision == VGPU_FP64) {
+    rc = bt->solve_tv<double>(bt->e64, bt->md64, b, x, bt->diag, rtol, itmax, iters,
+                              errs, status);
+  } else {
+    hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
+                       bt->stream, N, b, bt->fb);
+    hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
+                       bt->stream, N, x, bt->fx);
+    VGPU_HIP(hipGetLastError());
+    rc = bt->solve_tv<float>(bt->e32, bt->md32, bt->fb, bt->fx, bt->fdiag, rtol,
+                             itmax, iters, errs, status);
+    if (rc) return rc;
+    hipLaunchKernelGGL((k_convert<float, double>), grid_for(N), dim3(256), 0,
+                       bt->stream, N, bt->fx, x);
+    VGPU_HIP(hipGetLastError());
+  }
+  if (rc) return rc;
+  VGPU_HIP(hipStreamSynchronize(bt->stream));
+  const auto t1 = std::chrono::steady_clock::now();
+  if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  for (size_t i = 0; i < bt->m.size(); ++i) {
+    if (status[i] == 3) {
+      set_error("batched BiCG: breakdown in system " + std::to_string(i));
+      return VGPU_BREAKDOWN;
+    }
+  }
+  for (size_t i = 0; i < bt->m.size(); ++i)
+    if (status[i] != 1) {
+      set_error("batched BiCG: system " + std::to_string(i) + " did not reach rtol");
+      return VGPU_NOT_CONVERGED;
+    }
+  return VGPU_OK;
+}
+
+int vgpu_batch_solve(vgpu_batch *bt, int precision, const double *const *b,
+                     double *const *x, double rtol, int itmax, int *iters,
+                     double *errs, int *status) {
+  if (!bt || !b || !x) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(bt->device));
+  for (size_t i = 0; i < bt->m.size(); ++i) {
+    if (!b[i] || !x[i]) return VGPU_ERR_ARG;
+    const Segment &sg = bt->seg[i];
+    VGPU_HIP(hipMemcpyAsync(bt->b + sg.off, b[i], sg.n * sizeof(double),
+                            hipMemcpyHostToDevice, bt->stream));
+    VGPU_HIP(hipMemcpyAsync(bt->x + sg.off, x[i], sg.n * sizeof(double),
+                            hipMemcpyHostToDevice, bt->stream));
+  }
+  const int rc = vgpu_batch_solve_dev(bt, precision, bt->b, bt->x, rtol, itmax, iters,
+                                      errs, status, nullptr);
+  if (rc == VGPU_OK || rc == VGPU_NOT_CONVERGED || rc == VGPU_BREAKDOWN) {
+    for (size_t i = 0; i < bt->m.size(); ++i) {
+      const Segment &sg = bt->seg[i];
+      VGPU_HIP(hipMemcpyAsync(x[i], bt->x + sg.off, sg.n * sizeof(double),
+                              hipMemcpyDeviceToHost, bt->stream));
+    }
+    VGPU_HIP(hipStreamSynchronize(bt->stream));
+  }
+  return rc;
+}
+
 int vgpu_true_residual(vgpu_ctx *c, const double *b, const double *x,
                        double *rel) {
   if (!c || !b || !x || !rel) return VGPU_ERR_ARG;
@@ -945,6 +1250,53 @@ int vgpu_memcpy_d2h(vgpu_ctx *c, void *dst, const void *src, int64_t bytes) {
   return VGPU_OK;
 }
 
+int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
+                  int *tile, int *zchunk, double *best_ms) {
+  if (!c || reps < 1 || (precision != VGPU_FP64 && precision != VGPU_FP32)) {
+    set_error("vgpu_autotune: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  if (!c->have_coeffs) return VGPU_ERR_STATE;
+  static const int zcs[] = {0, 4, 8, 16, 32};
+  const int save_tile = c->tile, save_zc = c->zchunk, save_sc = c->scatter;
+  double best = 1e300;
+  int bt = -1, bz = 0;
+  for (int t = 0; t < 3; ++t)
+    for (int zc : zcs) {
+      c->tile = t;
+      c->zchunk = zc;
+      double ms = 0;
+      const int rc = vgpu_time_kernel(c, 1, VGPU_SCATTER_OWNER, precision, reps, &ms);
+      if (rc == VGPU_ERR_ARG) continue;  // more partials than the timing buffer
+      if (rc) {
+        c->tile = save_tile;
+        c->zchunk = save_zc;
+        c->scatter = save_sc;
+        return rc;
+      }
+      if (ms < best) {
+        best = ms;
+        bt = t;
+        bz = zc;
+      }
+    }
+  if (bt < 0) {
+    c->tile = save_tile;
+    c->zchunk = save_zc;
+    set_error("vgpu_autotune: no candidate could be timed");
+    return VGPU_ERR_ARG;
+  }
+  c->scatter = VGPU_SCATTER_OWNER;
+  c->tile = bt;
+  c->zchunk = bz;
+  c->version++;
+  if (scatter) *scatter = VGPU_SCATTER_OWNER;
+  if (tile) *tile = bt;
+  if (zchunk) *zchunk = bz;
+  if (best_ms) *best_ms = best;
+  return VGPU_OK;
+}
+
 int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                      int reps, double *avg_ms) {
   if (!c || !avg_ms || reps < 1 || what < 0 || what > 2) return VGPU_ERR_ARG;
@@ -976,21 +1328,22 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
       OpData<float> op = c->op32();
       return launch_apply<float, float>(op, scatter, c->zchunk, f1, f2,
                                         what == 1 ? f3 : nullptr,
-                                        what == 1 ? f4 : nullptr, c->red,
+                                        what == 1 ? f4 : nullptr, c->tpart,
                                         c->stream);
     }
     OpData<double> op = c->op64(c->cH, c->cC);
     return launch_apply<double, double>(op, scatter, c->zchunk, c->vx_, c->vb,
                                         what == 1 ? c->vt : nullptr,
                                         what == 1 ? c->vt2 : nullptr,
-                                        what == 1 ? c->red : nullptr, c->stream);
+                                        what == 1 ? c->tpart : nullptr, c->stream);
   };
-  // red holds 4096 partials
-  const int eb = precision == VGPU_FP32 ? 4 : 8;
-  if (what == 1 && scatter == VGPU_SCATTER_OWNER &&
-      owner_num_blocks(c->g, c->zchunk, c->tile, eb, eb, true, c->alpha != 0.0) > 4096) {
-    set_error("vgpu_time_kernel: mesh too large for the timing partial buffer");
-    return VGPU_ERR_ARG;
+  // dot partials: one per owner unit (brick modes: none)
+  const int64_t nparts = std::max<int64_t>(owner_units(c->g, c->tile), 4096);
+  if (c->tpart_n < nparts) {
+    free_dev(c->tpart);
+    c->tpart_n = 0;
+    VGPU_HIP(hipMalloc(&c->tpart, nparts * sizeof(double)));
+    c->tpart_n = nparts;
   }
   for (int i = 0; i < 3; ++i) {
     int rc = launch();

@@ -24,14 +24,6 @@
 
 namespace vgpu {
 
-struct SolveStatus {
-  int done;   // 0 running, 1 converged, 2 itmax, 3 breakdown, 4 diverged
-  int iter;   // completed iterations
-  double err; // ||r||/||b|| of the last test
-  double bnrm2;
-  double pad;
-};
-
 constexpr int VB = 1024;  // threads per vector block
 
 // Block-wide sum of parts[0..n) in a fixed order; result valid in all threads.

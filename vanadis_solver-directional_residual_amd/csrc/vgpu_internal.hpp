@@ -73,24 +73,56 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
   return x * q + min(x, r) + slot;
 }
 
+// Status of one BiCG system on the device; its kernels return at once when
+// done != 0.
+struct SolveStatus {
+  int done;   // 0 running, 1 converged, 2 itmax, 3 breakdown, 4 diverged
+  int iter;   // completed iterations
+  double err; // ||r||/||b|| of the last test
+  double bnrm2;
+  double pad;
+};
+
+// One system of a batch: its node range in the concatenated batch vectors,
+// its vector workgroups and its owner work units (tile-planes).  A single
+// solve is a batch of one.
+struct Segment {
+  int64_t off, n;  // nodes [off, off + n) of the batch vectors
+  int64_t u0, nu;  // owner units [u0, u0 + nu) (dot partial slots)
+  int vb0, nb;     // vector workgroups [vb0, vb0 + nb) (= partial slots)
+};
+
+// Operator data of one batch member for the owner kernel (device array).
+template <typename T>
+struct Member {
+  OpData<T> op;
+  Segment seg;
+};
+
 // ---------------------------------------------------------------------------
 // Host-side launchers (vgpu_operator.hip)
 // ---------------------------------------------------------------------------
+// Owner work units (tile-planes) of one system for a tile shape.
+int64_t owner_units(const Grid3 &g, int tile);
+// Owner operator over a batch of nm systems with utot units in total: for
+// every member m whose status is not done, y1 = A_m x1 (and y2 = A_m^T x2) on
+// the member's node range of the batch vectors.  dot (dual only, optional):
+// per-unit partials of y1 . x2 at dot[u0 + unit]; they depend on the tile
+// shape only, not on the grid.  zchunk: as owner_num_blocks.
+template <typename T, typename TIO>
+int launch_owner_batch(const Member<T> *md, int nm, int64_t utot, int tile,
+                       int zchunk, bool robin, const SolveStatus *st,
+                       const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
+                       double *dot, hipStream_t s);
 // Apply y1 = A x1 (and, if x2, y2 = A^T x2) with masking; TIO is the vector
 // type (double or float), T the element arithmetic type.  dot_part (optional,
-// owner mode only): per-workgroup partials of y1 . x2.  Returns # partials.
+// owner mode, dual only): owner_units(g, op.tile) partials of y1 . x2, one per
+// owner unit.  op.done, if set, points at a SolveStatus (owner mode) or its
+// done flag (brick modes).  zchunk: as owner_num_blocks / brick z chunk.
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zchunk,
                  const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
                  double *dot_part, hipStream_t s);
-// number of dot partials the owner launch writes for this geometry
-// Workgroups of an owner launch (= dot partials it writes): zchunk > 0 gives
-// about zchunk node planes per workgroup, zchunk = 0 one workgroup per
-// resident slot of the device.  elem_bytes / io_bytes: sizeof(T) / sizeof(TIO).
-int owner_num_blocks(const Grid3 &g, int zchunk, int tile, int elem_bytes,
-                     int io_bytes, bool dual, bool robin);
-// Upper bound of owner_num_blocks over every option (partial buffer size).
-int owner_max_blocks(const Grid3 &g);
 
 // Fills the tau field of the per-element records from the other fields.
 int launch_tau(const Grid3 &g, const double *xc, const double *yc,

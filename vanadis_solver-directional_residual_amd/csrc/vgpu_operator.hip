@@ -152,13 +152,13 @@ __device__ __forceinline__ void elem_eval(const OpData<T> &op, bool valid,
 }
 
 // ---------------------------------------------------------------------------
-// OWNER kernel (persistent, balanced).  The node columns are cut into
-// overlapping TX x TY element-column tiles that own (TX-1) x (TY-1) node
-// columns each; a work unit is one node plane of one tile, units are ordered
-// tile-major (z fastest) and block b takes the contiguous range
-// [b U / G, (b+1) U / G) of the U = tiles * naz units, split into one march
-// per tile it touches.  Producing planes k0..k1-1 of a tile computes element
-// layers k0-1..k1-1.
+// OWNER kernel (persistent, balanced, batched).  The node columns of each
+// system are cut into overlapping TX x TY element-column tiles that own
+// (TX-1) x (TY-1) node columns each; a work unit is one node plane of one
+// tile, units are numbered system by system, tile-major (z fastest), and
+// block b takes the contiguous range [b U / G, (b+1) U / G) of all U units,
+// split into one march per (system, tile) it touches.  Producing planes
+// k0..k1-1 of a tile computes element layers k0-1..k1-1.
 //
 // Per layer: one barrier.  Node planes live in three LDS buffers (plane p in
 // buffer p mod 3), so the buffer written for plane kl+1 was last read two
@@ -166,32 +166,33 @@ __device__ __forceinline__ void elem_eval(const OpData<T> &op, bool valid,
 // previous layer with the bottom-face result of the current one in registers
 // and stages the 4 column contributions of plane kl; the owner of a node adds
 // the 4 columns around it one layer later (double-buffered stage).  The next
-// plane and the next layer's fields are loaded one layer ahead.
+// plane and the next layer's fields are loaded one layer ahead.  The dual
+// mode also reduces y1 . x2 over each unit's owned nodes (dot[u], fixed
+// order), so the partials do not depend on the grid.
 // ---------------------------------------------------------------------------
-template <typename T, typename TIO, bool DUAL, bool ROBIN, int TX, int TY>
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k_owner(
-    OpData<T> op, const TIO *__restrict__ x1, TIO *__restrict__ y1,
-    const TIO *__restrict__ x2, TIO *__restrict__ y2,
-    double *__restrict__ dot_part) {
+    Member<T> m0, const Member<T> *__restrict__ md, int nm, int64_t U,
+    const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
+    TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
+    double *__restrict__ dot) {
   constexpr int NP = (TX + 1) * (TY + 1);
   constexpr int NT = TX * TY;
+  constexpr int NW = NT / 64;
   __shared__ T pl1[3][NP];
   __shared__ T pl2[DUAL ? 3 : 1][DUAL ? NP : 1];
   __shared__ uint8_t pm[3][NP];
   __shared__ T st1[2][4 * NT];
   __shared__ T st2[DUAL ? 2 : 1][DUAL ? 4 * NT : 1];
-  __shared__ double red[NT / 64];
-  if (op.done && *op.done) return;
+  __shared__ double red[2][NW];
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
-  const int naz = op.g.naz;
-  const int ntx = (op.g.nax + TX - 2) / (TX - 1), nty = (op.g.nay + TY - 2) / (TY - 1);
-  const int64_t U = (int64_t)ntx * nty * naz;
   const int64_t G = gridDim.x, lb = xcd_tile(blockIdx.x, gridDim.x);
   int64_t u = U * lb / G;
   const int64_t uend = U * (lb + 1) / G;
+  const bool want_dot = DUAL && dot != nullptr;
 
   const int q0 = tid, q1 = tid + NT;
   const int lx0 = q0 % (TX + 1), ly0 = q0 / (TX + 1);
@@ -200,13 +201,35 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
   const bool own = tx < (TX - 1) && ty < (TY - 1);  // owns node (tx+1, ty+1)
   const int oq = (ty + 1) * (TX + 1) + tx + 1;
   const int c00 = ty * (TX + 1) + tx;  // tile node of local corner 0
-  double dot = 0.0;
+
+  // members: md[0..nm) sorted by u0 (BATCH), or the single member m0
+  int m = 0;
+  if (BATCH) {
+    int lo = 0, hi = nm - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (md[mid].seg.u0 <= u) lo = mid; else hi = mid - 1;
+    }
+    m = lo;
+  }
 
   while (u < uend) {
-    const int tile = (int)(u / naz);
-    const int k0 = (int)(u - (int64_t)tile * naz);
+    if (BATCH)
+      while (u >= md[m].seg.u0 + md[m].seg.nu) ++m;
+    const OpData<T> op = BATCH ? md[m].op : m0.op;
+    const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
+    const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
+    const int naz = op.g.naz;
+    const int ntx = (op.g.nax + TX - 2) / (TX - 1);
+    const int64_t lu = u - ub;
+    const int tile = (int)(lu / naz);
+    const int k0 = (int)(lu - (int64_t)tile * naz);
     const int k1 = (int)min<int64_t>((int64_t)naz, k0 + (uend - u));
     u += k1 - k0;
+    if (st && st[m].done) continue;
+    const TIO *xm1 = x1 + off, *xm2 = DUAL ? x2 + off : nullptr;
+    TIO *ym1 = y1 + off, *ym2 = DUAL ? y2 + off : nullptr;
+    double *dm = want_dot ? dot + ub + (int64_t)tile * naz : nullptr;
 
     const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
     const int gy0 = (tile / ntx) * (TY - 1) - 1;
@@ -215,7 +238,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     ColConst<T> cq;
     col_setup<T>(colok ? op.hx[exg] : T(1), colok ? op.hy[eyg] : T(1),
                  colok ? op.ihx[exg] : T(1), colok ? op.ihy[eyg] : T(1), cq);
-    const bool robin = ROBIN && colok && exg == 0;
+    const bool robin = ROBIN && op.ra != 0.0 && colok && exg == 0;
     const int ngx = gx0 + 1 + tx, ngy = gy0 + 1 + ty;
     const bool owng = own && ngx < op.g.nax && ngy < op.g.nay;
     const int64_t ncol = ngx + (int64_t)op.g.nax * ngy;
@@ -223,13 +246,27 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     const int64_t ecol = exg + (int64_t)op.g.ex * eyg;
     const int64_t eplane = (int64_t)op.g.ex * op.g.ey;
 
+    // block sum of this thread's dot term of plane k into red[k & 1]
+    auto dot_stage = [&](double d, int k) {
+      for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o, 64);
+      if ((tid & 63) == 0) red[k & 1][tid >> 6] = d;
+    };
+    // thread 0: unit partial of plane k (staged before the last barrier)
+    auto dot_flush = [&](int k) {
+      if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NW; ++w) t += red[k & 1][w];
+        dm[k] = t;
+      }
+    };
+
     // prologue: plane k0-1 into its buffer, plane k0 and layer k0-1 ahead
     int kl = k0 - 1;
     int bb = (kl + 3) % 3, bt = (kl + 4) % 3, bn = (kl + 5) % 3;
     PlaneRegs<TIO, DUAL> pr;
-    plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl, lx0, ly0, lx1, ly1, has1, pr);
+    plane_load<T, TIO, DUAL>(op, xm1, xm2, gx0, gy0, kl, lx0, ly0, lx1, ly1, has1, pr);
     plane_store<T, TIO, DUAL>(pr, pl1[bb], DUAL ? pl2[bb] : nullptr, pm[bb], q0, q1, has1);
-    plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 1, lx0, ly0, lx1, ly1, has1, pr);
+    plane_load<T, TIO, DUAL>(op, xm1, xm2, gx0, gy0, kl + 1, lx0, ly0, lx1, ly1, has1, pr);
     const bool ev0 = colok && kl >= 0;
     Coef<T> cn = load_coef<T>(op, ev0, ecol + eplane * (ev0 ? kl : 0), max(kl, 0));
 
@@ -245,12 +282,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
       const Coef<T> cf = cn;
       const bool ev = colok && kl >= 0 && kl < op.g.ez;
       if (kl + 1 < k1) {  // prefetch plane kl+2 and layer kl+1
-        plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
+        plane_load<T, TIO, DUAL>(op, xm1, xm2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
                                  has1, pr);
         const bool evn = colok && kl + 1 < op.g.ez;
         cn = load_coef<T>(op, evn, ecol + eplane * (evn ? kl + 1 : 0), kl + 1);
       }
       lds_barrier();
+      if (want_dot && kl - 2 >= k0) dot_flush(kl - 2);
       // owner sums of plane kl-1 (staged in the previous layer)
       const bool emit = kl - 1 >= k0;
       double v1 = 0.0, v2 = 0.0;
@@ -304,14 +342,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
         yt1[c] = ya[c + 4];
         yt2[c] = yc[c + 4];
       }
-      if (emit && owng) {
-        const int64_t n = ncol + plane * (kl - 1);
-        const TIO o1 = om_prev ? TIO(0) : TIO(v1);
-        y1[n] = o1;
-        if (DUAL) {
-          y2[n] = om_prev ? TIO(0) : TIO(v2);
-          if (dot_part) dot += (double)o1 * xo_prev;
+      if (emit) {
+        double d = 0.0;
+        if (owng) {
+          const int64_t n = ncol + plane * (kl - 1);
+          const TIO o1 = om_prev ? TIO(0) : TIO(v1);
+          ym1[n] = o1;
+          if (DUAL) {
+            ym2[n] = om_prev ? TIO(0) : TIO(v2);
+            d = (double)o1 * xo_prev;
+          }
         }
+        if (want_dot) dot_stage(d, kl - 1);
       }
       om_prev = om;
       xo_prev = xo;
@@ -322,7 +364,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     }
     // epilogue: plane k1-1
     lds_barrier();
-    if (own && k1 - 1 >= k0) {
+    if (want_dot && k1 - 2 >= k0) dot_flush(k1 - 2);
+    double d = 0.0;
+    if (own) {
       const T *s = st1[(k1 - 1) & 1];
       const double v1 = (double)s[2 * NT + tid] + (double)s[3 * NT + tid + 1] +
                         (double)s[1 * NT + tid + TX] + (double)s[tid + TX + 1];
@@ -335,27 +379,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
       if (owng) {
         const int64_t n = ncol + plane * (k1 - 1);
         const TIO o1 = om_prev ? TIO(0) : TIO(v1);
-        y1[n] = o1;
+        ym1[n] = o1;
         if (DUAL) {
-          y2[n] = om_prev ? TIO(0) : TIO(v2);
-          if (dot_part) dot += (double)o1 * xo_prev;
+          ym2[n] = om_prev ? TIO(0) : TIO(v2);
+          d = (double)o1 * xo_prev;
         }
       }
     }
+    if (want_dot) dot_stage(d, k1 - 1);
     lds_barrier();  // buffers are refilled by the next march
-  }
-  if (dot_part) {
-    // deterministic block reduction of y1 . x2 (masked x2 from LDS)
-    for (int off = 32; off > 0; off >>= 1) dot += __shfl_down(dot, off, 64);
-    if ((tid & 63) == 0) red[tid >> 6] = dot;
-    __syncthreads();
-    if (tid == 0) {
-      double s = 0.0;
-      for (int w = 0; w < NT / 64; ++w) s += red[w];
-      dot_part[blockIdx.x] = s;
-    }
+    if (want_dot) dot_flush(k1 - 1);
   }
 }
+
 // Element apply of the brick kernels: corners gathered from the two LDS
 // planes, the 8 nodal results (8 transposed results when DUAL) written to the
 // staging arrays, corner-major.
@@ -525,19 +561,16 @@ static void owner_shape(int tile, int *tx, int *ty) {
   }
 }
 
-namespace {
-template <typename T, typename TIO, bool DUAL, bool ROBIN, int TX, int TY>
-const void *owner_kernel() {
-  return reinterpret_cast<const void *>(&k_owner<T, TIO, DUAL, ROBIN, TX, TY>);
+int64_t owner_units(const Grid3 &g, int tile) {
+  int tx, ty;
+  owner_shape(tile, &tx, &ty);
+  return (int64_t)((g.nax + tx - 2) / (tx - 1)) * ((g.nay + ty - 2) / (ty - 1)) * g.naz;
 }
 
-template <typename T, typename TIO, int TX, int TY>
-const void *owner_kernel_for(bool dual, bool robin) {
-  if (dual)
-    return robin ? owner_kernel<T, TIO, true, true, TX, TY>()
-                 : owner_kernel<T, TIO, true, false, TX, TY>();
-  return robin ? owner_kernel<T, TIO, false, true, TX, TY>()
-               : owner_kernel<T, TIO, false, false, TX, TY>();
+namespace {
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
+const void *owner_kernel() {
+  return reinterpret_cast<const void *>(&k_owner<T, TIO, DUAL, ROBIN, BATCH, TX, TY>);
 }
 
 // Resident workgroups of a kernel on the whole device (cached per kernel).
@@ -557,77 +590,86 @@ int resident_blocks(const void *fn, int threads) {
   cache[{dev, fn}] = r;
   return r;
 }
-}  // namespace
 
-int owner_num_blocks(const Grid3 &g, int zc, int tile, int elem_bytes,
-                     int io_bytes, bool dual, bool robin) {
-  int tx, ty;
-  owner_shape(tile, &tx, &ty);
-  const int64_t ntiles = (int64_t)((g.nax + tx - 2) / (tx - 1)) *
-                         ((g.nay + ty - 2) / (ty - 1));
-  const int64_t units = ntiles * g.naz;
-  if (zc > 0) return (int)std::min<int64_t>(units, ntiles * ((g.naz + zc - 1) / zc));
-  const void *fn = nullptr;
-  const bool f64 = elem_bytes == 8, io64 = io_bytes == 8;
-  switch (tile) {
-    case 1:
-      fn = f64 ? owner_kernel_for<double, double, 16, 16>(dual, robin)
-               : (io64 ? owner_kernel_for<float, double, 16, 16>(dual, robin)
-                       : owner_kernel_for<float, float, 16, 16>(dual, robin));
-      break;
-    case 2:
-      fn = f64 ? owner_kernel_for<double, double, 64, 4>(dual, robin)
-               : (io64 ? owner_kernel_for<float, double, 64, 4>(dual, robin)
-                       : owner_kernel_for<float, float, 64, 4>(dual, robin));
-      break;
-    default:
-      fn = f64 ? owner_kernel_for<double, double, 32, 8>(dual, robin)
-               : (io64 ? owner_kernel_for<float, double, 32, 8>(dual, robin)
-                       : owner_kernel_for<float, float, 32, 8>(dual, robin));
-      break;
-  }
-  int r = resident_blocks(fn, tx * ty);
+// Grid of an owner launch over utot units: zchunk > 0 gives about zchunk
+// units per workgroup, zchunk = 0 one workgroup per resident slot (at least 4
+// units each, so the one-layer prologue of a march stays small).
+int owner_grid(const void *fn, int threads, int64_t utot, int zchunk) {
+  if (zchunk > 0) return (int)std::max<int64_t>(1, (utot + zchunk - 1) / zchunk);
+  int r = resident_blocks(fn, threads);
   if (r <= 0) r = 1024;
-  // at least 4 planes per workgroup so the one-layer prologue stays small
-  return (int)std::max<int64_t>(1, std::min<int64_t>(r, units / 4));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(r, utot / 4));
 }
 
-int owner_max_blocks(const Grid3 &g) {
-  int64_t m = 1;
-  for (int t = 0; t < 3; ++t) {
-    int tx, ty;
-    owner_shape(t, &tx, &ty);
-    m = std::max<int64_t>(m, (int64_t)((g.nax + tx - 2) / (tx - 1)) *
-                                 ((g.nay + ty - 2) / (ty - 1)) * g.naz);
-  }
-  return (int)std::min<int64_t>(m, 1 << 30);
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
+void launch_owner_k(const Member<T> &m0, const Member<T> *md, int nm, int64_t utot,
+                    int zchunk, const SolveStatus *st, const TIO *x1, TIO *y1,
+                    const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
+  const void *fn = owner_kernel<T, TIO, DUAL, ROBIN, BATCH, TX, TY>();
+  const int nb = owner_grid(fn, TX * TY, utot, zchunk);
+  hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, TX, TY>), dim3(nb),
+                     dim3(TX * TY), 0, s, m0, md, nm, utot, st, x1, y1, x2, y2, dot);
 }
 
-namespace {
-template <typename T, typename TIO, int TX, int TY>
-int launch_owner(const OpData<T> &op, int zc, const TIO *x1, TIO *y1,
-                 const TIO *x2, TIO *y2, double *dot_part, hipStream_t s) {
-  const bool dual = x2 != nullptr, robin = op.ra != 0.0;
-  const int nb = owner_num_blocks(op.g, zc, op.tile, sizeof(T), sizeof(TIO), dual, robin);
-  dim3 grid(nb), block(TX * TY);
-  if (dual) {
+template <typename T, typename TIO, bool BATCH, int TX, int TY>
+void launch_owner_shape(const Member<T> &m0, const Member<T> *md, int nm,
+                        int64_t utot, int zchunk, bool robin,
+                        const SolveStatus *st, const TIO *x1, TIO *y1,
+                        const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
+  if (x2) {
     if (robin)
-      hipLaunchKernelGGL((k_owner<T, TIO, true, true, TX, TY>), grid, block, 0, s, op,
-                         x1, y1, x2, y2, dot_part);
+      launch_owner_k<T, TIO, true, true, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, dot, s);
     else
-      hipLaunchKernelGGL((k_owner<T, TIO, true, false, TX, TY>), grid, block, 0, s, op,
-                         x1, y1, x2, y2, dot_part);
+      launch_owner_k<T, TIO, true, false, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, dot, s);
   } else {
     if (robin)
-      hipLaunchKernelGGL((k_owner<T, TIO, false, true, TX, TY>), grid, block, 0, s, op,
-                         x1, y1, x2, y2, nullptr);
+      launch_owner_k<T, TIO, false, true, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
     else
-      hipLaunchKernelGGL((k_owner<T, TIO, false, false, TX, TY>), grid, block, 0, s, op,
-                         x1, y1, x2, y2, nullptr);
+      launch_owner_k<T, TIO, false, false, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
   }
-  return nb;
+}
+
+template <typename T, typename TIO, bool BATCH>
+int launch_owner_any(const Member<T> &m0, const Member<T> *md, int nm,
+                     int64_t utot, int tile, int zchunk, bool robin,
+                     const SolveStatus *st, const TIO *x1, TIO *y1,
+                     const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
+  if (utot <= 0) return VGPU_OK;
+  switch (tile) {
+    case 1:
+      launch_owner_shape<T, TIO, BATCH, 16, 16>(m0, md, nm, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      break;
+    case 2:
+      launch_owner_shape<T, TIO, BATCH, 64, 4>(m0, md, nm, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      break;
+    default:
+      launch_owner_shape<T, TIO, BATCH, 32, 8>(m0, md, nm, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      break;
+  }
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
 }
 }  // namespace
+
+template <typename T, typename TIO>
+int launch_owner_batch(const Member<T> *md, int nm, int64_t utot, int tile,
+                       int zchunk, bool robin, const SolveStatus *st,
+                       const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
+                       double *dot, hipStream_t s) {
+  Member<T> none{};
+  return launch_owner_any<T, TIO, true>(none, md, nm, utot, tile, zchunk, robin, st,
+                                        x1, y1, x2, y2, dot, s);
+}
+
+template int launch_owner_batch<double, double>(const Member<double> *, int, int64_t, int,
+                                                int, bool, const SolveStatus *,
+                                                const double *, double *,
+                                                const double *, double *, double *,
+                                                hipStream_t);
+template int launch_owner_batch<float, float>(const Member<float> *, int, int64_t, int,
+                                              int, bool, const SolveStatus *,
+                                              const float *, float *, const float *,
+                                              float *, double *, hipStream_t);
 
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
@@ -636,13 +678,15 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
   const bool dual = x2 != nullptr;
   const Grid3 &g = op.g;
   if (scatter == VGPU_SCATTER_OWNER) {
-    switch (op.tile) {
-      case 1: launch_owner<T, TIO, 16, 16>(op, zc, x1, y1, x2, y2, dot_part, s); break;
-      case 2: launch_owner<T, TIO, 64, 4>(op, zc, x1, y1, x2, y2, dot_part, s); break;
-      default: launch_owner<T, TIO, 32, 8>(op, zc, x1, y1, x2, y2, dot_part, s); break;
-    }
-    VGPU_HIP(hipGetLastError());
-    return VGPU_OK;
+    Member<T> m0{};
+    m0.op = op;
+    m0.seg.off = 0;
+    m0.seg.n = g.N;
+    m0.seg.u0 = 0;
+    m0.seg.nu = owner_units(g, op.tile);
+    const SolveStatus *st = reinterpret_cast<const SolveStatus *>(op.done);
+    return launch_owner_any<T, TIO, false>(m0, nullptr, 1, m0.seg.nu, op.tile, zc,
+                                    op.ra != 0.0, st, x1, y1, x2, y2, dot_part, s);
   }
   // brick modes need zeroed outputs; bricks are 32 x 8 element columns
   constexpr int BX = 32, BY = 8;

@@ -260,3 +260,37 @@ def config_problem(name: str, **kw) -> Problem:
         return unit_box(ex, ey, ez, vel_seed=0, lam_range=(1e-4, 1e-2), lam_seed=1,
                         name=name, **kw)
     raise KeyError(name)
+
+
+def _batch_fields(seed: int):
+    """Scalar diffusivity log-uniform in [1e-4, 1e-2], source centre in
+    [0.2, 0.8]^3 and sigma in [0.05, 0.2], drawn from `seed`."""
+    u = uniform01(seed, 5, stream=3)
+    lam = 10.0 ** (-4.0 + 2.0 * u[0])
+    centre = tuple(0.2 + 0.6 * u[1:4])
+    sigma = 0.05 + 0.15 * u[4]
+    return lam, centre, sigma
+
+
+def batch_rhs_problems(shape=(96, 96, 48), count=32, seed0=100, **kw):
+    """`count` systems sharing the mesh and the velocity field (seed 0); system
+    b has its own scalar diffusivity and Gaussian source from seed seed0 + b."""
+    out = []
+    for b in range(count):
+        lam, c, sig = _batch_fields(seed0 + b)
+        out.append(unit_box(*shape, vel_seed=0, lam=lam, src_center=c, src_sigma=sig,
+                            name=f"rhs{b}", **kw))
+    return out
+
+
+def batch_subdomain_problems(count=16, seed0=200, lo=24, hi=96, **kw):
+    """`count` independent boxes of n^3 elements, n uniform in {lo..hi}, each
+    with the field recipe of batch_rhs_problems, all drawn from seed0 + b."""
+    out = []
+    for b in range(count):
+        n = lo + int(uniform01(seed0 + b, 1, stream=4)[0] * (hi - lo + 1))
+        lam, c, sig = _batch_fields(seed0 + b)
+        out.append(unit_box(n, n, n, vel_seed=0, lam=lam, src_center=c, src_sigma=sig,
+                            name=f"sub{b}-{n}", **kw))
+    return out
+
