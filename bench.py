@@ -76,6 +76,66 @@ def cpu_baseline(pkg, seconds=12.0):
             "ms_per_apply": round(dt * 1e3, 3)}
 
 
+def measured_traffic(tuned):
+    """HBM bytes per dual-apply launch from the committed rocprofv3 --pmc
+    summary of the same kernel and tile (FETCH_SIZE and WRITE_SIZE, each scaled
+    by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_dual_apply_fp64_128x128x64.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        t = json.load(f)
+    tile = 0 if tuned is None else tuned["tile"]
+    e = t.get("tiles", {}).get(str(tile))
+    if not e:
+        return None
+    return {"bytes_per_launch": e["bytes_per_launch"],
+            "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json tile {tile}"}
+
+
+def batched_solves(pkg, tuned):
+    """Batched FP64 solves to rtol 1e-8 against the same systems solved one by
+    one: 32 right-hand sides (own diffusivity and source) on one 96x96x48 mesh,
+    and 16 boxes of n^3 elements, n uniform in 24..96."""
+    res = {}
+    for name, gen in (("rhs32_96x96x48", lambda: pkg.problem.batch_rhs_problems()),
+                      ("subdomains16_24to96", lambda: pkg.problem.batch_subdomain_problems())):
+        progress(f"batched {name}: generating")
+        probs = gen()
+        ctxs = [pkg.Context.from_problem(q) for q in probs]
+        for c in ctxs:
+            c.set_options("owner", 0 if tuned is None else tuned["zchunk"],
+                          0 if tuned is None else tuned["tile"])
+        bs = [c.assemble_rhs(q.q_in) for c, q in zip(ctxs, probs)]
+        itmax = 20000
+        with pkg.Batch(ctxs) as bt:
+            bt.solve(bs, rtol=1e-8, itmax=itmax, raise_on_fail=False)  # warm-up, capture
+            t0 = time.time()
+            xs, its, errs, st, rc = bt.solve(bs, rtol=1e-8, itmax=itmax, raise_on_fail=False)
+            t_batch = time.time() - t0
+        for c, bb in zip(ctxs, bs):
+            c.solve(bb, rtol=1e-8, itmax=itmax, raise_on_fail=False)  # warm-up
+        t0 = time.time()
+        same = True
+        for m, (c, bb) in enumerate(zip(ctxs, bs)):
+            x, it, err, _ = c.solve(bb, rtol=1e-8, itmax=itmax, raise_on_fail=False)
+            same = same and it == its[m] and bool(np.array_equal(x, xs[m]))
+        t_seq = time.time() - t0
+        nodes = sum(q.N for q in probs)
+        res[name] = {"systems": len(probs), "nodes": nodes,
+                     "iterations_min_max": [int(its.min()), int(its.max())],
+                     "all_converged": bool((st == 1).all()),
+                     "status_counts": {str(k): int((st == k).sum()) for k in sorted(set(st))},
+                     "batched_s": round(t_batch, 4), "one_by_one_s": round(t_seq, 4),
+                     "speedup": round(t_seq / t_batch, 3),
+                     "bitwise_equal_to_single": same,
+                     "note": "host-to-device copies of b and x included in both"}
+        progress(f"batched {name}: {t_batch * 1e3:.1f} ms vs {t_seq * 1e3:.1f} ms one by one")
+        for c in ctxs:
+            c.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +145,12 @@ def main():
                     help="only launch the dual apply and copy kernels (counter runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-tune", action="store_true",
+                    help="keep the default owner tile (32x8) instead of the autotuner's")
+    ap.add_argument("--tile", type=int, default=-1,
+                    help="owner tile shape 0/1/2 (default: autotuner, or 0 with --kernel-only)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched solves (32 right-hand sides, 16 subdomains)")
     ap.add_argument("--size", default="128,128,64")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -100,7 +166,15 @@ def main():
     p = pkg.problem.unit_box(ex, ey, ez, vel_seed=0, lam=1e-3)
     N, E = p.N, p.E
     ctx = pkg.Context.from_problem(p)
-    ctx.set_options("owner", 0)
+    ctx.set_options("owner", 0, max(a.tile, 0))
+    tuned = None
+    if a.tile >= 0:
+        tuned = {"tile": a.tile, "zchunk": 0, "ms": None, "source": "--tile"}
+    elif not a.no_tune and not a.kernel_only:
+        tuned = ctx.autotune(pkg.FP64, reps=20)
+        tuned["source"] = "autotuner"
+        progress(f"autotuner: tile {tuned['tile']} zchunk {tuned['zchunk']} "
+                 f"({tuned['ms'] * 1e3:.1f} us dual apply)")
     b = ctx.assemble_rhs(p.q_in)
     b_dev = ctx.dev_alloc(8 * N)
     x_dev = ctx.dev_alloc(8 * N)
@@ -174,6 +248,14 @@ def main():
     progress(f"apply {t_single * 1e3:.1f} us ({gdofs:.1f} GDOF/s), dual {t_dual * 1e3:.1f} us "
              f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
 
+    batch = None
+    if not a.no_batch:
+        try:
+            batch = batched_solves(pkg, tuned)
+        except Exception as exc:  # reported, never fatal for the line
+            batch = {"status": f"failed: {exc}"}
+            progress(f"batched {batch['status']}")
+
     cpu = None
     if not a.no_cpu:
         progress("cpu baseline")
@@ -204,7 +286,13 @@ def main():
                      "traffic": None, "kernel": "k_owner<double,double,dual>",
                      "algorithmic_bytes_per_launch": dual_bytes},
         "cpu_baseline": cpu,
+        "autotune": tuned,
+        "batched": batch,
     }
+    traffic = measured_traffic(tuned)
+    if traffic:
+        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
+        out["roofline"]["traffic_source"] = traffic["source"]
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -283,13 +283,13 @@ int solve_owner(vgpu_ctx *c, BBicg<TV> &eng, const TV *b, TV *x, const TV *diag,
   int rc = eng.alloc(make_segments({c->g.N}, {owner_units(c->g, c->tile)}));
   if (rc) return rc;
   BDualOp<TV> dual = [c](const TV *p, TV *z, const TV *pp, TV *zz, double *pa,
-                         const SolveStatus *st, hipStream_t s) {
+                         const SolveStatus *st, const ActiveSet &, hipStream_t s) {
     OpData<TV> op = op_for<TV>(c);
     op.done = reinterpret_cast<const int *>(st);
     return launch_apply<TV, TV>(op, VGPU_SCATTER_OWNER, c->zchunk, p, z, pp, zz,
                                 pa, s);
   };
-  BSingleOp<TV> single = [c](const TV *xx, TV *y, hipStream_t s) {
+  BSingleOp<TV> single = [c](const TV *xx, TV *y, const ActiveSet &, hipStream_t s) {
     OpData<TV> op = op_for<TV>(c);
     return launch_apply<TV, TV>(op, VGPU_SCATTER_OWNER, c->zchunk, xx, y, nullptr,
                                 nullptr, nullptr, s);
@@ -1008,16 +1008,16 @@ struct vgpu_batch {
                int *status) {
     int rc = eng.alloc(seg);
     if (rc) return rc;
-    const int nm = (int)m.size();
-    BDualOp<TV> dual = [this, md, nm](const TV *p, TV *z, const TV *pp, TV *zz,
-                                      double *pa, const SolveStatus *st,
-                                      hipStream_t s) {
-      return launch_owner_batch<TV, TV>(md, nm, utot, tile, zchunk, robin, st, p, z,
-                                        pp, zz, pa, s);
+    BDualOp<TV> dual = [this, md](const TV *p, TV *z, const TV *pp, TV *zz,
+                                  double *pa, const SolveStatus *st,
+                                  const ActiveSet &as, hipStream_t s) {
+      return launch_owner_batch<TV, TV>(md, as, tile, zchunk, robin, st, p, z, pp, zz,
+                                        pa, s);
     };
-    BSingleOp<TV> single = [this, md, nm](const TV *xi, TV *y, hipStream_t s) {
-      return launch_owner_batch<TV, TV>(md, nm, utot, tile, zchunk, robin, nullptr, xi,
-                                        y, nullptr, nullptr, nullptr, s);
+    BSingleOp<TV> single = [this, md](const TV *xi, TV *y, const ActiveSet &as,
+                                      hipStream_t s) {
+      return launch_owner_batch<TV, TV>(md, as, tile, zchunk, robin, nullptr, xi, y,
+                                        nullptr, nullptr, nullptr, s);
     };
     const uint64_t key = (gen << 8) | (uint64_t)(zchunk % 64) |
                          (sizeof(TV) == 4 ? (1ull << 7) : 0);

@@ -25,28 +25,50 @@
 
 namespace vgpu {
 
-// system of a vector workgroup (segments sorted by vb0)
-__device__ __forceinline__ int seg_of_block(const Segment *seg, int nm, int b) {
-  int lo = 0, hi = nm - 1;
+// Active system of vector workgroup b and the workgroup's index within it.
+struct BlockOf {
+  int m, j;
+};
+__device__ __forceinline__ BlockOf block_of(const int *act, const int *actvb, int nact,
+                                            int b) {
+  int lo = 0, hi = nact - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (seg[mid].vb0 <= b) lo = mid; else hi = mid - 1;
+    if (actvb[mid] <= b) lo = mid; else hi = mid - 1;
   }
-  return lo;
+  return {act[lo], b - actvb[lo]};
 }
+
+// Block-wide sum of a per-thread value, written by thread 0 to out[slot].
+__device__ __forceinline__ void block_partial_at(double v, double *out, int slot,
+                                                 double *red) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    const int nw = blockDim.x >> 6;
+    for (int w = 0; w < nw; ++w) t += red[w];
+    out[slot] = t;
+  }
+}
+
+#define VGPU_ACT_ARGS const int *act, const int *actvb, int nact
 
 // r = b - r (r holds A x0), rr = r, z = r/d, zz = rr/d, x zeroed on the mask;
 // partials of r.r, z.rr and b.b (cudatest.cu:464-491, VANADIS.FOR:991-1000).
 template <typename TV>
-__global__ __launch_bounds__(VB) void kb_init(const Segment *seg, int nm,
+__global__ __launch_bounds__(VB) void kb_init(const Segment *seg, VGPU_ACT_ARGS,
                                               const uint8_t *mask, const TV *b,
                                               TV *x, TV *r, TV *rr, TV *z,
                                               TV *zz, const TV *diag,
                                               double *pr, double *pb,
                                               double *pbn) {
   __shared__ double red[VB / 64];
-  const Segment sg = seg[seg_of_block(seg, nm, blockIdx.x)];
-  const int j = blockIdx.x - sg.vb0;
+  const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
+  const Segment sg = seg[bo.m];
+  const int j = bo.j;
   double sr = 0, sb = 0, sn = 0;
   for (int64_t li = j * (int64_t)VB + threadIdx.x; li < sg.n;
        li += (int64_t)sg.nb * VB) {
@@ -63,15 +85,15 @@ __global__ __launch_bounds__(VB) void kb_init(const Segment *seg, int nm,
     sb += (double)zi * (double)ri;
     sn += bi * bi;
   }
-  block_partial(sr, pr, red);
-  block_partial(sb, pb, red);
-  block_partial(sn, pbn, red);
+  block_partial_at(sr, pr, sg.vb0 + j, red);
+  block_partial_at(sb, pb, sg.vb0 + j, red);
+  block_partial_at(sn, pbn, sg.vb0 + j, red);
 }
 
 // K1: convergence test of iteration k-1, then p = z + beta p, pp = zz + beta pp
 // (iteration 1 copies, cudatest.cu:504-505).  q = k & 1; NB = all workgroups.
 template <typename TV>
-__global__ __launch_bounds__(VB) void kb_dir(const Segment *seg, int nm, int NB,
+__global__ __launch_bounds__(VB) void kb_dir(const Segment *seg, VGPU_ACT_ARGS, int NB,
                                              int q, const double *pr,
                                              const double *pb, const double *pbn,
                                              SolveStatus *stv, double tol,
@@ -79,11 +101,11 @@ __global__ __launch_bounds__(VB) void kb_dir(const Segment *seg, int nm, int NB,
                                              TV *pp, const TV *z,
                                              const TV *zz) {
   __shared__ double red[VB / 64];
-  const int m = seg_of_block(seg, nm, blockIdx.x);
-  SolveStatus *st = stv + m;
+  const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
+  SolveStatus *st = stv + bo.m;
   if (st->done) return;
-  const Segment sg = seg[m];
-  const int j = blockIdx.x - sg.vb0;
+  const Segment sg = seg[bo.m];
+  const int j = bo.j;
   const int k = st->iter + 1;
   const int64_t stride = (int64_t)sg.nb * VB;
   const int64_t i0 = j * (int64_t)VB + threadIdx.x;
@@ -142,18 +164,18 @@ __global__ __launch_bounds__(VB) void kb_dir(const Segment *seg, int nm, int NB,
 // unit order); x += a p; r -= a z; rr -= a zz; z = r/d; zz = rr/d; partials of
 // r.r and z.rr into parity q.
 template <typename TV>
-__global__ __launch_bounds__(VB) void kb_update(const Segment *seg, int nm,
+__global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARGS,
                                                 int NB, int q, const double *pa,
                                                 double *pr, double *pb,
                                                 SolveStatus *stv, TV *x,
                                                 const TV *p, TV *r, TV *rr,
                                                 TV *z, TV *zz, const TV *diag) {
   __shared__ double red[VB / 64];
-  const int m = seg_of_block(seg, nm, blockIdx.x);
-  SolveStatus *st = stv + m;
+  const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
+  SolveStatus *st = stv + bo.m;
   if (st->done) return;
-  const Segment sg = seg[m];
-  const int j = blockIdx.x - sg.vb0;
+  const Segment sg = seg[bo.m];
+  const int j = bo.j;
   const int64_t stride = (int64_t)sg.nb * VB;
   const int64_t i0 = j * (int64_t)VB + threadIdx.x;
   TV xv[PF], pv[PF], rv[PF], rrv[PF], zv[PF], zzv[PF], dv[PF];
@@ -191,8 +213,8 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, int nm,
     update_one<TV>(i, a, x[i], p[i], r[i], rr[i], z[i], zz[i], diag[i], x, r, rr,
                    z, zz, sr, sb);
   }
-  block_partial(sr, pr + q * NB, red);
-  block_partial(sb, pb + q * NB, red);
+  block_partial_at(sr, pr + q * NB, sg.vb0 + j, red);
+  block_partial_at(sb, pb + q * NB, sg.vb0 + j, red);
   if (j == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
@@ -202,9 +224,10 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, int nm,
 template <typename TV>
 using BDualOp = std::function<int(const TV *p, TV *z, const TV *pp, TV *zz,
                                   double *pa, const SolveStatus *st,
-                                  hipStream_t s)>;
+                                  const ActiveSet &as, hipStream_t s)>;
 template <typename TV>
-using BSingleOp = std::function<int(const TV *x, TV *y, hipStream_t s)>;
+using BSingleOp = std::function<int(const TV *x, TV *y, const ActiveSet &as,
+                                    hipStream_t s)>;
 
 // Nodes of a system rounded up so that every system starts on a 256-node
 // boundary of the batch vectors.
@@ -230,12 +253,59 @@ inline std::vector<Segment> make_segments(const std::vector<int64_t> &n,
   return s;
 }
 
+// Device copy of an active set (member list and prefix sums).
+struct ActiveBuf {
+  int *act = nullptr, *actvb = nullptr;
+  int64_t *actu = nullptr;
+  ActiveSet view;
+  std::vector<int> members;
+
+  int alloc(int nm) {
+    VGPU_HIP(hipMalloc(&act, nm * sizeof(int)));
+    VGPU_HIP(hipMalloc(&actvb, (nm + 1) * sizeof(int)));
+    VGPU_HIP(hipMalloc(&actu, (nm + 1) * sizeof(int64_t)));
+    return VGPU_OK;
+  }
+  // members (increasing) of segments s; synchronous upload
+  int set(const std::vector<Segment> &s, const std::vector<int> &mem) {
+    members = mem;
+    std::vector<int> vb(mem.size() + 1, 0);
+    std::vector<int64_t> u(mem.size() + 1, 0);
+    for (size_t i = 0; i < mem.size(); ++i) {
+      vb[i + 1] = vb[i] + s[mem[i]].nb;
+      u[i + 1] = u[i] + s[mem[i]].nu;
+    }
+    if (!mem.empty()) {
+      VGPU_HIP(hipMemcpy(act, mem.data(), mem.size() * sizeof(int), hipMemcpyHostToDevice));
+      VGPU_HIP(hipMemcpy(actvb, vb.data(), vb.size() * sizeof(int), hipMemcpyHostToDevice));
+      VGPU_HIP(hipMemcpy(actu, u.data(), u.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    }
+    view.act = act;
+    view.actvb = actvb;
+    view.actu = actu;
+    view.nact = (int)mem.size();
+    view.nvb = vb.back();
+    view.nu = u.back();
+    return VGPU_OK;
+  }
+  void release() {
+    if (act) hipFree(act);
+    if (actvb) hipFree(actvb);
+    if (actu) hipFree(actu);
+    act = actvb = nullptr;
+    actu = nullptr;
+    view = ActiveSet{};
+    members.clear();
+  }
+};
+
 template <typename TV>
 struct BBicg {
   int nm = 0, NB = 0;
   int64_t ntot = 0, utot = 0;
   std::vector<Segment> hseg;
   Segment *seg = nullptr;
+  ActiveBuf full, cur;  // every system / the systems the graph launches
   TV *r = nullptr, *rr = nullptr, *p = nullptr, *pp = nullptr, *z = nullptr,
      *zz = nullptr;
   double *pr = nullptr, *pb = nullptr, *pbn = nullptr, *pa = nullptr;
@@ -264,6 +334,14 @@ struct BBicg {
     utot = s.back().u0 + s.back().nu;
     VGPU_HIP(hipMalloc(&seg, nm * sizeof(Segment)));
     VGPU_HIP(hipMemcpy(seg, s.data(), nm * sizeof(Segment), hipMemcpyHostToDevice));
+    int rc = full.alloc(nm);
+    if (!rc) rc = cur.alloc(nm);
+    if (rc) return rc;
+    std::vector<int> all(nm);
+    for (int m = 0; m < nm; ++m) all[m] = m;
+    rc = full.set(hseg, all);
+    if (!rc) rc = cur.set(hseg, all);
+    if (rc) return rc;
     TV **v[] = {&r, &rr, &p, &pp, &z, &zz};
     for (auto pv : v) {
       VGPU_HIP(hipMalloc(pv, ntot * sizeof(TV)));
@@ -301,6 +379,8 @@ struct BBicg {
       }
     if (seg) hipFree(seg);
     seg = nullptr;
+    full.release();
+    cur.release();
     if (st) hipFree(st);
     st = nullptr;
     if (st_host) hipHostFree(st_host);
@@ -312,45 +392,37 @@ struct BBicg {
 
   // Solve every system with x, b, diag, mask on the device (batch layout).
   // op_key identifies the operators; the graph is reused while the key,
-  // buffers, tol, itmax and divtol are unchanged.  Per-system results go to
-  // iters/err/status [nm].
+  // buffers, tol, itmax, divtol and the active set are unchanged.  When a
+  // quarter of the launched systems have finished, the launches are narrowed
+  // to the running ones (graph re-captured between replays).  Per-system
+  // results go to iters/err/status [nm].
   int solve(uint64_t op_key, const BDualOp<TV> &dual, const BSingleOp<TV> &single,
             const uint8_t *mask, const TV *b, TV *x, const TV *diag, double tol,
             int itmax, hipStream_t s, int *iters, double *err, int *status,
             double divtol = 1e300) {
-    int rc = single(x, r, s);  // r = A x0 (x0 masked on load)
+    int rc = single(x, r, full.view, s);  // r = A x0 (x0 masked on load)
     if (rc) return rc;
-    hipLaunchKernelGGL((kb_init<TV>), dim3(NB), dim3(VB), 0, s, seg, nm, mask, b,
-                       x, r, rr, z, zz, diag, pr, pb, pbn);
+    hipLaunchKernelGGL((kb_init<TV>), dim3(full.view.nvb), dim3(VB), 0, s, seg,
+                       full.view.act, full.view.actvb, full.view.nact, mask, b, x, r, rr,
+                       z, zz, diag, pr, pb, pbn);
     VGPU_HIP(hipGetLastError());
     std::vector<SolveStatus> s0(nm, SolveStatus{});
     VGPU_HIP(hipMemcpyAsync(st, s0.data(), nm * sizeof(SolveStatus),
                             hipMemcpyHostToDevice, s));
+    if ((int)cur.members.size() != nm) {  // start with every system launched
+      VGPU_HIP(hipStreamSynchronize(s));
+      drop_graph();
+      std::vector<int> all(nm);
+      for (int m = 0; m < nm; ++m) all[m] = m;
+      rc = cur.set(hseg, all);
+      if (rc) return rc;
+    }
     const bool same = key_valid && key_op == op_key && key_b == b && key_x == x &&
                       key_diag == diag && key_tol == tol && key_itmax == itmax &&
                       key_div == divtol;
     if (!same) {
-      drop_graph();
-      VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      int crc = VGPU_OK;
-      for (int pi = 0; pi < pairs_per_graph && crc == VGPU_OK; ++pi)
-        for (int q = 1; q >= 0 && crc == VGPU_OK; --q)
-          crc = enqueue_iteration(dual, diag, x, tol, itmax, divtol, q, s);
-      hipError_t ce = hipStreamEndCapture(s, &graph);
-      if (crc) {
-        drop_graph();
-        return crc;
-      }
-      VGPU_HIP(ce);
-      VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-      key_valid = true;
-      key_op = op_key;
-      key_b = b;
-      key_x = x;
-      key_diag = diag;
-      key_tol = tol;
-      key_itmax = itmax;
-      key_div = divtol;
+      rc = capture(op_key, dual, diag, b, x, tol, itmax, divtol, s);
+      if (rc) return rc;
     }
     // replay until every system is done, at most two replays in flight
     const int max_replays = itmax / (2 * pairs_per_graph) + 2;
@@ -377,9 +449,26 @@ struct BBicg {
         break;
       }
       ++completed;
-      bool all = true;
-      for (int m = 0; m < nm; ++m) all = all && st_host[m].done != 0;
-      if (all) break;
+      int running = 0;
+      for (int m = 0; m < nm; ++m) running += st_host[m].done == 0;
+      if (running == 0) break;
+      if (4 * running <= 3 * (int)cur.members.size() && launched < max_replays) {
+        // drain, then launch only the running systems
+        if (hipStreamSynchronize(s) != hipSuccess) {
+          set_error("batched BiCG stream sync failed");
+          result = VGPU_ERR_HIP;
+          break;
+        }
+        completed = launched;
+        std::vector<int> run;
+        for (int m = 0; m < nm; ++m)
+          if (st_host[m].done == 0) run.push_back(m);
+        if (run.empty()) break;
+        result = cur.set(hseg, run);
+        if (!result)
+          result = capture(op_key, dual, diag, b, x, tol, itmax, divtol, s);
+        if (result) break;
+      }
     }
     if (hipStreamSynchronize(s) != hipSuccess && !result) {
       set_error("batched BiCG stream sync failed");
@@ -404,14 +493,41 @@ struct BBicg {
   int key_itmax = 0;
   double key_div = 0;
 
+  int capture(uint64_t op_key, const BDualOp<TV> &dual, const TV *diag, const TV *b,
+              TV *x, double tol, int itmax, double divtol, hipStream_t s) {
+    drop_graph();
+    VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int crc = VGPU_OK;
+    for (int pi = 0; pi < pairs_per_graph && crc == VGPU_OK; ++pi)
+      for (int q = 1; q >= 0 && crc == VGPU_OK; --q)
+        crc = enqueue_iteration(dual, diag, x, tol, itmax, divtol, q, s);
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (crc) {
+      drop_graph();
+      return crc;
+    }
+    VGPU_HIP(ce);
+    VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    key_valid = true;
+    key_op = op_key;
+    key_b = b;
+    key_x = x;
+    key_diag = diag;
+    key_tol = tol;
+    key_itmax = itmax;
+    key_div = divtol;
+    return VGPU_OK;
+  }
+
   int enqueue_iteration(const BDualOp<TV> &dual, const TV *diag, TV *x, double tol,
                         int itmax, double divtol, int q, hipStream_t s) {
-    hipLaunchKernelGGL((kb_dir<TV>), dim3(NB), dim3(VB), 0, s, seg, nm, NB, q, pr,
-                       pb, pbn, st, tol, itmax, divtol, p, pp, z, zz);
-    int rc = dual(p, z, pp, zz, pa, st, s);
+    const ActiveSet &a = cur.view;
+    hipLaunchKernelGGL((kb_dir<TV>), dim3(a.nvb), dim3(VB), 0, s, seg, a.act, a.actvb,
+                       a.nact, NB, q, pr, pb, pbn, st, tol, itmax, divtol, p, pp, z, zz);
+    int rc = dual(p, z, pp, zz, pa, st, a, s);
     if (rc) return rc;
-    hipLaunchKernelGGL((kb_update<TV>), dim3(NB), dim3(VB), 0, s, seg, nm, NB, q,
-                       pa, pr, pb, st, x, p, r, rr, z, zz, diag);
+    hipLaunchKernelGGL((kb_update<TV>), dim3(a.nvb), dim3(VB), 0, s, seg, a.act,
+                       a.actvb, a.nact, NB, q, pa, pr, pb, st, x, p, r, rr, z, zz, diag);
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
   }

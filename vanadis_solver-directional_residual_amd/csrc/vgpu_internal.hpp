@@ -99,18 +99,58 @@ struct Member {
   Segment seg;
 };
 
+// The systems of a batch that launches cover: member indices act[0..nact)
+// (increasing) with the prefix sums of their vector workgroups actvb[0..nact]
+// and owner units actu[0..nact] (device arrays).  Partial slots stay at each
+// member's own vb0 / u0, so dropping finished systems from the launches does
+// not move anything the others read.
+struct ActiveSet {
+  const int *act = nullptr;
+  const int *actvb = nullptr;
+  const int64_t *actu = nullptr;
+  int nact = 0;
+  int nvb = 0;     // actvb[nact]
+  int64_t nu = 0;  // actu[nact]
+};
+
+// Sum of v over the 64 lanes of a wave, in a fixed order, valid in lane 63:
+// DPP adds across lane pairs (quad_perm 1,0,3,2), quads (quad_perm 2,3,0,1),
+// half rows (row_half_mirror) and rows (row_mirror), then row r += lane 15 of
+// row r-1 (row_bcast:15 on rows 1, 3) and rows 2, 3 += lane 31
+// (row_bcast:31).  VALU moves only: no LDS round trips on the critical path.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const uint64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xf, false);
+  return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+
+__device__ __forceinline__ double wave_sum_lane63(double v) {
+  v += dpp_f64<0xB1, 0xf>(v);   // quad_perm [1,0,3,2]
+  v += dpp_f64<0x4E, 0xf>(v);   // quad_perm [2,3,0,1]
+  v += dpp_f64<0x141, 0xf>(v);  // row_half_mirror
+  v += dpp_f64<0x140, 0xf>(v);  // row_mirror
+  v += dpp_f64<0x142, 0xa>(v);  // row_bcast:15 -> rows 1, 3
+  v += dpp_f64<0x143, 0xc>(v);  // row_bcast:31 -> rows 2, 3
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers (vgpu_operator.hip)
 // ---------------------------------------------------------------------------
-// Owner work units (tile-planes) of one system for a tile shape.
+// Node planes per owner work unit: a unit is this many consecutive planes of
+// one tile, the granularity of the workgroup split and of the dot partials.
+constexpr int kUnitPlanes = 1;
+// Owner work units of one system for a tile shape.
 int64_t owner_units(const Grid3 &g, int tile);
-// Owner operator over a batch of nm systems with utot units in total: for
-// every member m whose status is not done, y1 = A_m x1 (and y2 = A_m^T x2) on
-// the member's node range of the batch vectors.  dot (dual only, optional):
+// Owner operator over the active systems of a batch (units split evenly over
+// as.nu): for every active member m whose status is not done, y1 = A_m x1 (and
+// y2 = A_m^T x2) on the member's node range of the batch vectors.  dot (dual only, optional):
 // per-unit partials of y1 . x2 at dot[u0 + unit]; they depend on the tile
 // shape only, not on the grid.  zchunk: as owner_num_blocks.
 template <typename T, typename TIO>
-int launch_owner_batch(const Member<T> *md, int nm, int64_t utot, int tile,
+int launch_owner_batch(const Member<T> *md, const ActiveSet &as, int tile,
                        int zchunk, bool robin, const SolveStatus *st,
                        const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
                        double *dot, hipStream_t s);

@@ -154,8 +154,9 @@ __device__ __forceinline__ void elem_eval(const OpData<T> &op, bool valid,
 // ---------------------------------------------------------------------------
 // OWNER kernel (persistent, balanced, batched).  The node columns of each
 // system are cut into overlapping TX x TY element-column tiles that own
-// (TX-1) x (TY-1) node columns each; a work unit is one node plane of one
-// tile, units are numbered system by system, tile-major (z fastest), and
+// (TX-1) x (TY-1) node columns each; a work unit is kUnitPlanes consecutive
+// node planes of one tile, units are numbered system by system, tile-major
+// (z fastest), and
 // block b takes the contiguous range [b U / G, (b+1) U / G) of all U units,
 // split into one march per (system, tile) it touches.  Producing planes
 // k0..k1-1 of a tile computes element layers k0-1..k1-1.
@@ -167,13 +168,15 @@ __device__ __forceinline__ void elem_eval(const OpData<T> &op, bool valid,
 // and stages the 4 column contributions of plane kl; the owner of a node adds
 // the 4 columns around it one layer later (double-buffered stage).  The next
 // plane and the next layer's fields are loaded one layer ahead.  The dual
-// mode also reduces y1 . x2 over each unit's owned nodes (dot[u], fixed
-// order), so the partials do not depend on the grid.
+// mode also reduces y1 . x2 over each unit's owned nodes (dot[u]: per thread
+// over the unit's planes in order, then over the block in a fixed order), so
+// the partials do not depend on the grid.
 // ---------------------------------------------------------------------------
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k_owner(
-    Member<T> m0, const Member<T> *__restrict__ md, int nm, int64_t U,
+    Member<T> m0, const Member<T> *__restrict__ md, const int *__restrict__ act,
+    const int64_t *__restrict__ actu, int nact, int64_t U,
     const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
     TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
     double *__restrict__ dot) {
@@ -202,34 +205,38 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
   const int oq = (ty + 1) * (TX + 1) + tx + 1;
   const int c00 = ty * (TX + 1) + tx;  // tile node of local corner 0
 
-  // members: md[0..nm) sorted by u0 (BATCH), or the single member m0
-  int m = 0;
+  // BATCH: active members act[i], their units at [actu[i], actu[i+1]) of
+  // the split and at md[act[i]].seg.u0 of the partials; else the member m0
+  int ia = 0;
   if (BATCH) {
-    int lo = 0, hi = nm - 1;
+    int lo = 0, hi = nact - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (md[mid].seg.u0 <= u) lo = mid; else hi = mid - 1;
+      if (actu[mid] <= u) lo = mid; else hi = mid - 1;
     }
-    m = lo;
+    ia = lo;
   }
 
   while (u < uend) {
     if (BATCH)
-      while (u >= md[m].seg.u0 + md[m].seg.nu) ++m;
+      while (u >= actu[ia + 1]) ++ia;
+    const int m = BATCH ? act[ia] : 0;
     const OpData<T> op = BATCH ? md[m].op : m0.op;
     const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
     const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
     const int naz = op.g.naz;
+    const int ngz = (naz + kUnitPlanes - 1) / kUnitPlanes;  // units per tile
     const int ntx = (op.g.nax + TX - 2) / (TX - 1);
-    const int64_t lu = u - ub;
-    const int tile = (int)(lu / naz);
-    const int k0 = (int)(lu - (int64_t)tile * naz);
-    const int k1 = (int)min<int64_t>((int64_t)naz, k0 + (uend - u));
-    u += k1 - k0;
+    const int64_t lu = u - (BATCH ? actu[ia] : (int64_t)0);
+    const int tile = (int)(lu / ngz);
+    const int g0 = (int)(lu - (int64_t)tile * ngz);
+    const int g1 = (int)min<int64_t>((int64_t)ngz, g0 + (uend - u));
+    u += g1 - g0;
     if (st && st[m].done) continue;
+    const int k0 = g0 * kUnitPlanes, k1 = min(naz, g1 * kUnitPlanes);
     const TIO *xm1 = x1 + off, *xm2 = DUAL ? x2 + off : nullptr;
     TIO *ym1 = y1 + off, *ym2 = DUAL ? y2 + off : nullptr;
-    double *dm = want_dot ? dot + ub + (int64_t)tile * naz : nullptr;
+    double *dm = want_dot ? dot + ub + (int64_t)tile * ngz : nullptr;
 
     const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
     const int gy0 = (tile / ntx) * (TY - 1) - 1;
@@ -246,18 +253,28 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     const int64_t ecol = exg + (int64_t)op.g.ex * eyg;
     const int64_t eplane = (int64_t)op.g.ex * op.g.ey;
 
-    // block sum of this thread's dot term of plane k into red[k & 1]
-    auto dot_stage = [&](double d, int k) {
-      for (int o = 32; o > 0; o >>= 1) d += __shfl_down(d, o, 64);
-      if ((tid & 63) == 0) red[k & 1][tid >> 6] = d;
-    };
-    // thread 0: unit partial of plane k (staged before the last barrier)
-    auto dot_flush = [&](int k) {
-      if (tid == 0) {
-        double t = 0.0;
-        for (int w = 0; w < NW; ++w) t += red[k & 1][w];
-        dm[k] = t;
+    // dot terms: each thread sums its owned nodes of the current unit (its
+    // planes in order); at the unit's last plane the block sum goes to
+    // red[unit & 1] and thread 0 stores it after the next barrier
+    double dacc = 0.0;
+    int pend = -1;  // unit whose block sum waits for the next barrier
+    auto dot_emit = [&](double d, int k) {
+      dacc += d;
+      if (k % kUnitPlanes == kUnitPlanes - 1 || k == naz - 1) {
+        const double v = wave_sum_lane63(dacc);
+        if ((tid & 63) == 63) red[(k / kUnitPlanes) & 1][tid >> 6] = v;
+        dacc = 0.0;
+        pend = k / kUnitPlanes;
       }
+    };
+    // every thread reads the wave sums (LDS broadcast, no branch around the
+    // reads, so they issue with the layer's other LDS reads); thread 0 stores
+    auto dot_flush = [&]() {
+      const int pb = pend < 0 ? 0 : pend;
+      double t = 0.0;
+      for (int w = 0; w < NW; ++w) t += red[pb & 1][w];
+      if (pend >= 0 && tid == 0) dm[pend] = t;
+      pend = -1;
     };
 
     // prologue: plane k0-1 into its buffer, plane k0 and layer k0-1 ahead
@@ -288,7 +305,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
         cn = load_coef<T>(op, evn, ecol + eplane * (evn ? kl + 1 : 0), kl + 1);
       }
       lds_barrier();
-      if (want_dot && kl - 2 >= k0) dot_flush(kl - 2);
+      if (want_dot) dot_flush();
       // owner sums of plane kl-1 (staged in the previous layer)
       const bool emit = kl - 1 >= k0;
       double v1 = 0.0, v2 = 0.0;
@@ -353,7 +370,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
             d = (double)o1 * xo_prev;
           }
         }
-        if (want_dot) dot_stage(d, kl - 1);
+        if (want_dot) dot_emit(d, kl - 1);
       }
       om_prev = om;
       xo_prev = xo;
@@ -364,7 +381,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     }
     // epilogue: plane k1-1
     lds_barrier();
-    if (want_dot && k1 - 2 >= k0) dot_flush(k1 - 2);
+    if (want_dot) dot_flush();
     double d = 0.0;
     if (own) {
       const T *s = st1[(k1 - 1) & 1];
@@ -386,9 +403,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
         }
       }
     }
-    if (want_dot) dot_stage(d, k1 - 1);
+    if (want_dot) dot_emit(d, k1 - 1);
     lds_barrier();  // buffers are refilled by the next march
-    if (want_dot) dot_flush(k1 - 1);
+    if (want_dot) dot_flush();
   }
 }
 
@@ -564,7 +581,8 @@ static void owner_shape(int tile, int *tx, int *ty) {
 int64_t owner_units(const Grid3 &g, int tile) {
   int tx, ty;
   owner_shape(tile, &tx, &ty);
-  return (int64_t)((g.nax + tx - 2) / (tx - 1)) * ((g.nay + ty - 2) / (ty - 1)) * g.naz;
+  return (int64_t)((g.nax + tx - 2) / (tx - 1)) * ((g.nay + ty - 2) / (ty - 1)) *
+         ((g.naz + kUnitPlanes - 1) / kUnitPlanes);
 }
 
 namespace {
@@ -592,58 +610,62 @@ int resident_blocks(const void *fn, int threads) {
 }
 
 // Grid of an owner launch over utot units: zchunk > 0 gives about zchunk
-// units per workgroup, zchunk = 0 one workgroup per resident slot (at least 4
-// units each, so the one-layer prologue of a march stays small).
+// node planes (zchunk / kUnitPlanes units, at least one) per workgroup,
+// zchunk = 0 one workgroup per resident slot.
 int owner_grid(const void *fn, int threads, int64_t utot, int zchunk) {
-  if (zchunk > 0) return (int)std::max<int64_t>(1, (utot + zchunk - 1) / zchunk);
+  if (zchunk > 0) {
+    const int64_t upb = std::max(1, zchunk / kUnitPlanes);
+    return (int)std::max<int64_t>(1, (utot + upb - 1) / upb);
+  }
   int r = resident_blocks(fn, threads);
   if (r <= 0) r = 1024;
-  return (int)std::max<int64_t>(1, std::min<int64_t>(r, utot / 4));
+  return (int)std::max<int64_t>(1, std::min<int64_t>(r, utot));
 }
 
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
-void launch_owner_k(const Member<T> &m0, const Member<T> *md, int nm, int64_t utot,
-                    int zchunk, const SolveStatus *st, const TIO *x1, TIO *y1,
-                    const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
+void launch_owner_k(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
+                    int64_t utot, int zchunk, const SolveStatus *st, const TIO *x1,
+                    TIO *y1, const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
   const void *fn = owner_kernel<T, TIO, DUAL, ROBIN, BATCH, TX, TY>();
   const int nb = owner_grid(fn, TX * TY, utot, zchunk);
   hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, TX, TY>), dim3(nb),
-                     dim3(TX * TY), 0, s, m0, md, nm, utot, st, x1, y1, x2, y2, dot);
+                     dim3(TX * TY), 0, s, m0, md, as.act, as.actu, as.nact, utot, st, x1,
+                     y1, x2, y2, dot);
 }
 
 template <typename T, typename TIO, bool BATCH, int TX, int TY>
-void launch_owner_shape(const Member<T> &m0, const Member<T> *md, int nm,
+void launch_owner_shape(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
                         int64_t utot, int zchunk, bool robin,
                         const SolveStatus *st, const TIO *x1, TIO *y1,
                         const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
   if (x2) {
     if (robin)
-      launch_owner_k<T, TIO, true, true, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, dot, s);
+      launch_owner_k<T, TIO, true, true, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, dot, s);
     else
-      launch_owner_k<T, TIO, true, false, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, dot, s);
+      launch_owner_k<T, TIO, true, false, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, dot, s);
   } else {
     if (robin)
-      launch_owner_k<T, TIO, false, true, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
+      launch_owner_k<T, TIO, false, true, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
     else
-      launch_owner_k<T, TIO, false, false, BATCH, TX, TY>(m0, md, nm, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
+      launch_owner_k<T, TIO, false, false, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
   }
 }
 
 template <typename T, typename TIO, bool BATCH>
-int launch_owner_any(const Member<T> &m0, const Member<T> *md, int nm,
+int launch_owner_any(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
                      int64_t utot, int tile, int zchunk, bool robin,
                      const SolveStatus *st, const TIO *x1, TIO *y1,
                      const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
   if (utot <= 0) return VGPU_OK;
   switch (tile) {
     case 1:
-      launch_owner_shape<T, TIO, BATCH, 16, 16>(m0, md, nm, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, 16, 16>(m0, md, as, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
       break;
     case 2:
-      launch_owner_shape<T, TIO, BATCH, 64, 4>(m0, md, nm, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, 64, 4>(m0, md, as, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
       break;
     default:
-      launch_owner_shape<T, TIO, BATCH, 32, 8>(m0, md, nm, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, 32, 8>(m0, md, as, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
       break;
   }
   VGPU_HIP(hipGetLastError());
@@ -652,22 +674,22 @@ int launch_owner_any(const Member<T> &m0, const Member<T> *md, int nm,
 }  // namespace
 
 template <typename T, typename TIO>
-int launch_owner_batch(const Member<T> *md, int nm, int64_t utot, int tile,
+int launch_owner_batch(const Member<T> *md, const ActiveSet &as, int tile,
                        int zchunk, bool robin, const SolveStatus *st,
                        const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
                        double *dot, hipStream_t s) {
   Member<T> none{};
-  return launch_owner_any<T, TIO, true>(none, md, nm, utot, tile, zchunk, robin, st,
+  return launch_owner_any<T, TIO, true>(none, md, as, as.nu, tile, zchunk, robin, st,
                                         x1, y1, x2, y2, dot, s);
 }
 
-template int launch_owner_batch<double, double>(const Member<double> *, int, int64_t, int,
-                                                int, bool, const SolveStatus *,
+template int launch_owner_batch<double, double>(const Member<double> *, const ActiveSet &,
+                                                int, int, bool, const SolveStatus *,
                                                 const double *, double *,
                                                 const double *, double *, double *,
                                                 hipStream_t);
-template int launch_owner_batch<float, float>(const Member<float> *, int, int64_t, int,
-                                              int, bool, const SolveStatus *,
+template int launch_owner_batch<float, float>(const Member<float> *, const ActiveSet &,
+                                              int, int, bool, const SolveStatus *,
                                               const float *, float *, const float *,
                                               float *, double *, hipStream_t);
 
@@ -685,7 +707,7 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
     m0.seg.u0 = 0;
     m0.seg.nu = owner_units(g, op.tile);
     const SolveStatus *st = reinterpret_cast<const SolveStatus *>(op.done);
-    return launch_owner_any<T, TIO, false>(m0, nullptr, 1, m0.seg.nu, op.tile, zc,
+    return launch_owner_any<T, TIO, false>(m0, nullptr, ActiveSet{}, m0.seg.nu, op.tile, zc,
                                     op.ra != 0.0, st, x1, y1, x2, y2, dot_part, s);
   }
   // brick modes need zeroed outputs; bricks are 32 x 8 element columns
