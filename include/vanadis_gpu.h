@@ -173,6 +173,27 @@ int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
 int vgpu_autotune(vgpu_ctx *ctx, int precision, int reps, int *scatter,
                   int *tile, int *zchunk, double *best_ms);
 
+/* Device-resident forms for time stepping: b_dev = right-hand side of the
+ * scheme from element sources q_dev [E] and the previous solution t_old_dev
+ * [N] (may be NULL), as vgpu_assemble_rhs; and a Gaussian source
+ * exp(-|x - c|^2 / (2 sigma^2)) with centre c: at element centroids times the
+ * element volume into q_elem_dev [E] and at the nodes into src_node_dev [N]
+ * (either may be NULL).  Both run on the context's stream. */
+int vgpu_assemble_rhs_dev(vgpu_ctx *ctx, const double *q_dev, const double *t_old_dev,
+                          double *b_dev);
+int vgpu_gaussian_dev(vgpu_ctx *ctx, double cx, double cy, double cz, double sigma,
+                      double *q_elem_dev, double *src_node_dev);
+/* Device vector helpers on the context's stream: copy; out = a x + b y (zero on
+ * Dirichlet nodes; y may be NULL when b = 0); |b - A x| / |b| (FP64 owner
+ * apply, synchronous). */
+int vgpu_copy_dev(vgpu_ctx *ctx, void *dst, const void *src, int64_t bytes);
+int vgpu_axpby_dev(vgpu_ctx *ctx, double a, const double *x, double b, const double *y,
+                   double *out);
+int vgpu_residual_norm_dev(vgpu_ctx *ctx, const double *b, const double *x, double *rel);
+/* `sweeps` damped Jacobi sweeps x += omega D^-1 (b - A x) on the device (D the
+ * solver's Jacobi diagonal): smooths the rough part of an initial guess. */
+int vgpu_jacobi_dev(vgpu_ctx *ctx, const double *b, double *x, int sweeps, double omega);
+
 /* ---- Batched solves (no counterpart in the reference, whose host solves one
  * system per call, VANADIS.FOR:809-810) ----
  * A batch groups nm contexts on one device (each with its own mesh, mask,
@@ -198,6 +219,50 @@ int vgpu_batch_layout(vgpu_batch *batch, int64_t *offsets, int64_t *total);
 int vgpu_batch_solve_dev(vgpu_batch *batch, int precision, const double *b,
                          double *x, double rtol, int itmax, int *iters,
                          double *errs, int *status, double *seconds);
+
+/* ---- Learned initial guess (no counterpart in the reference, whose host
+ * restarts BiCG from the previous step's solution, VANADIS.FOR:809-810) ----
+ * A 5-layer 3x3x3 convolution network on the context's node grid: 16
+ * channels, zero "same" padding, leaky ReLU (slope 0.1) between layers, BF16
+ * operands on MFMA
+ * with FP32 accumulation.  Inputs: x_{n-1}, x_{n-1} - x_{n-2}, a nodal source
+ * and the nodal velocity (vx, vy, vz), each times in_scale[c]; the prediction is
+ * guess = x_{n-1} + out_scale * (output channel 0).
+ * Parameters: w [5][27][16][16] (layer, tap dx + 3 dy + 9 dz at offset d-1,
+ * output channel, input channel), b [5][16], FP32 on the host. */
+typedef struct vgpu_net vgpu_net;
+int vgpu_net_create(vgpu_ctx *ctx, vgpu_net **out);
+int vgpu_net_destroy(vgpu_net *net);
+int vgpu_net_set_params(vgpu_net *net, const float *w, const float *b);
+int vgpu_net_get_params(vgpu_net *net, float *w, float *b);
+/* gradients of the last vgpu_net_train_step_dev (same layout as w, b) */
+int vgpu_net_get_grads(vgpu_net *net, float *dw, float *db);
+/* nodal velocity (host, [N] each), the channel scales and the base of the
+ * prediction: guess = x1 + extrap (x1 - x2) + out_scale * net (extrap 0: the
+ * previous step, 1: linear extrapolation) */
+int vgpu_net_set_static(vgpu_net *net, const double *vx, const double *vy,
+                        const double *vz, const float *in_scale, double out_scale,
+                        double extrap);
+/* device vectors [N]: guess = x1 + extrap (x1 - x2) + out_scale * net(...) */
+int vgpu_net_predict_dev(vgpu_net *net, const double *x1, const double *x2,
+                         const double *src, double *guess);
+/* one training step on one sample: loss = mean over free nodes of
+ * (net - (target - x1 - extrap (x1 - x2)) / out_scale)^2, gradients by the
+ * backward kernels,
+ * then Adam with learning rate lr (lr <= 0: gradients only). */
+int vgpu_net_train_step_dev(vgpu_net *net, const double *x1, const double *x2,
+                            const double *src, const double *target, float lr,
+                            double *loss);
+int vgpu_net_reset_optimizer(vgpu_net *net);
+/* training objective: 0 = the increment loss above (default); 1 = the initial
+ * residual BiCG starts from, |A (target - guess)|^2 / |A (target - base)|^2
+ * with A the context's masked FP64 operator (gradient through A^T). */
+int vgpu_net_set_loss(vgpu_net *net, int kind);
+/* passes (0..8, default 0) of a [1 4 1]^3 / 216 low-pass filter on the network
+ * output, in prediction and training alike: it damps the grid-scale modes
+ * that BF16 rounding leaves in the correction, the slowest modes of a
+ * mass-dominated operator for BiCG. */
+int vgpu_net_set_smoothing(vgpu_net *net, int passes);
 
 #ifdef __cplusplus
 }

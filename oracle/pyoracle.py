@@ -65,6 +65,12 @@ def lib():
         L.orc_siatka.argtypes = [_dp, _dp, _dp, C.c_int, C.c_int, C.c_int, _dp, _dp, _dp]
         L.orc_generate_hex8_colors.argtypes = [C.c_int, C.c_int, C.c_int, _ip]
         L.orc_build_color_element_lists.argtypes = [C.c_int, _ip, _ip, _ip, _ip]
+        _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+        L.orc_net_grads.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp,
+                                    _dp, _ip, _fp, C.c_double, C.c_double, _fp, _fp, C.c_int,
+                                    _pd, _dp,
+                                    _dp, _dp]
+        L.orc_net_grads.restype = C.c_int
         _lib = L
     return _lib
 
@@ -283,3 +289,26 @@ class RefLib:
         f.restype = C.c_double
         ne, nn = C.c_int(nelem), C.c_int(nnode)
         return f(C.byref(ne), C.byref(nn), iss.ctypes.data_as(_pi), bb.ctypes.data_as(_pd))
+
+
+def net_grads(nx, ny, nz, x1, x2, src, vx, vy, vz, target, mask, in_scale, out_scale, w, b,
+              quant=True, extrap=0.0):
+    """Loss, weight/bias gradients and output channel 0 of the learned-guess
+    network (netoracle.c; quant mirrors the GPU's BF16 storage points)."""
+    N = nx * ny * nz
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float64)  # noqa: E731
+    loss = C.c_double(0)
+    dw = np.zeros(5 * 27 * 256)
+    db = np.zeros(5 * 16)
+    out = np.zeros(N)
+    rc = lib().orc_net_grads(nx, ny, nz, f(x1), f(x2), f(src), f(vx), f(vy), f(vz), f(target),
+                             np.ascontiguousarray(mask, dtype=np.int32),
+                             np.ascontiguousarray(in_scale, dtype=np.float32), float(out_scale),
+                             float(extrap),
+                             np.ascontiguousarray(w, dtype=np.float32),
+                             np.ascontiguousarray(b, dtype=np.float32), int(bool(quant)),
+                             C.byref(loss), dw, db, out)
+    if rc:
+        raise MemoryError("orc_net_grads")
+    return loss.value, dw, db, out
+

@@ -1,0 +1,149 @@
+"""Learned-guess network on the GPU against the CPU oracle (oracle/netoracle.c,
+pinned by finite differences in test_net_oracle.py), and the gain of the
+trained network over the previous-step guess on implicit-Euler stepping."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+L, T, C = 5, 27, 16
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def _setup(pkg, shape, seed):
+    p = pkg.problem.unit_box(*shape, lam=1e-3)
+    ctx = pkg.Context.from_problem(p)
+    net = pkg.Net(ctx)
+    rng = np.random.default_rng(seed)
+    vec = {k: rng.standard_normal(p.N) for k in ("x1", "x2", "src", "vx", "vy", "vz", "tgt")}
+    dev = {}
+    for k in ("x1", "x2", "src", "tgt"):
+        dev[k] = ctx.dev_alloc(8 * p.N)
+        ctx.h2d(dev[k], vec[k])
+    dev["guess"] = ctx.dev_alloc(8 * p.N)
+    return p, ctx, net, vec, dev
+
+
+def test_forward_known_answer_shift(pkg, po):
+    p, ctx, net, vec, dev = _setup(pkg, (19, 9, 6), 1)
+    x1 = np.abs(vec["x1"]) + 0.1
+    ctx.h2d(dev["x1"], x1)
+    w = np.zeros((L, T, C, C), np.float32)
+    for layer in range(L - 1):
+        w[layer, 13] = np.eye(C, dtype=np.float32)
+    w[L - 1, 14, 0, 0] = 1.0
+    b = np.zeros((L, C), np.float32)
+    scale = np.array([2.0, 1, 1, 1, 1, 1], np.float32)
+    net.set_params(w, b)
+    net.set_static(vec["vx"], vec["vy"], vec["vz"], scale, 1.0, 0.0)
+    net.predict_dev(dev["x1"], dev["x2"], dev["src"], dev["guess"])
+    out = ctx.d2h(dev["guess"], p.N) - x1
+    _, _, _, ref = po.net_grads(p.nax, p.nay, p.naz, x1, vec["x2"], vec["src"], vec["vx"],
+                                vec["vy"], vec["vz"], vec["tgt"], p.mask, scale, 1.0, w, b,
+                                quant=True)
+    assert np.max(np.abs(out - ref)) <= 1e-14 * max(1.0, np.abs(ref).max())
+    net.close()
+    ctx.close()
+
+
+@pytest.mark.parametrize("extrap", [0.0, 1.0])
+def test_gradients_match_oracle(pkg, po, extrap):
+    p, ctx, net, vec, dev = _setup(pkg, (20, 17, 9), 2)
+    rng = np.random.default_rng(3)
+    w = (rng.standard_normal((L, T, C, C)) * 0.12).astype(np.float32)
+    b = (rng.standard_normal((L, C)) * 0.05).astype(np.float32)
+    scale = np.array([1.0, 0.8, 1.2, 1, 1, 1], np.float32)
+    net.set_params(w, b)
+    net.set_static(vec["vx"], vec["vy"], vec["vz"], scale, 0.7, extrap)
+    loss = net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 0.0)
+    gw, gb = net.get_grads()
+    lo, dw, db, out = po.net_grads(p.nax, p.nay, p.naz, vec["x1"], vec["x2"], vec["src"],
+                                   vec["vx"], vec["vy"], vec["vz"], vec["tgt"], p.mask, scale,
+                                   0.7, w, b, quant=True, extrap=extrap)
+    log(f"loss gpu {loss:.6e} oracle {lo:.6e}")
+    assert loss == pytest.approx(lo, rel=2e-3)
+    gw = gw.reshape(L, -1)
+    dw = dw.reshape(L, -1)
+    for layer in range(L):
+        r = np.linalg.norm(gw[layer] - dw[layer]) / np.linalg.norm(dw[layer])
+        log(f"layer {layer}: |dW gpu - oracle| / |dW| = {r:.2e}")
+        assert r <= 2e-2, (layer, r)
+    rb = np.linalg.norm(gb - db) / np.linalg.norm(db)
+    assert rb <= 2e-2, rb
+    # the prediction is the oracle's forward pass
+    net.predict_dev(dev["x1"], dev["x2"], dev["src"], dev["guess"])
+    g = ctx.d2h(dev["guess"], p.N)
+    base = vec["x1"] + extrap * (vec["x1"] - vec["x2"])
+    pred = (g - base) / 0.7
+    assert np.linalg.norm(pred - out) <= 1e-2 * np.linalg.norm(out)
+    # deterministic: a second pass gives the same gradients bitwise
+    net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 0.0)
+    gw2, gb2 = net.get_grads()
+    assert np.array_equal(gw2.reshape(L, -1), gw) and np.array_equal(gb2, gb)
+    net.close()
+    ctx.close()
+
+
+def test_residual_loss_gradient_matches_finite_difference(pkg):
+    """Initial-residual loss |A (t - g)|^2 / |A (t - base)|^2: its gradient
+    w.r.t. the output bias (the sum of dL/dout) against central differences."""
+    p = pkg.problem.unit_box(18, 14, 8, lam=1e-3, scheme=pkg.SCHEME_EULER, dt=0.01)
+    ctx = pkg.Context.from_problem(p)
+    net = pkg.Net(ctx)
+    rng = np.random.default_rng(7)
+    dev = {}
+    for k in ("x1", "x2", "src", "tgt"):
+        dev[k] = ctx.dev_alloc(8 * p.N)
+        ctx.h2d(dev[k], rng.standard_normal(p.N) * (1 - p.mask))
+    w, b = pkg.learned.init_params(2)
+    w[L - 1] = rng.standard_normal(w[L - 1].shape).astype(np.float32) * 0.05
+    v = rng.standard_normal(p.N)
+    net.set_static(v, v, v, np.ones(6, np.float32), 0.3, 1.0)
+    net.set_loss(1)
+    net.set_smoothing(2)
+    net.set_params(w, b)
+    l0 = net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 0.0)
+    _, gb = net.get_grads()
+    h = 1e-3
+    lp = []
+    for sgn in (1, -1):
+        bb = b.copy()
+        bb[L - 1, 0] += sgn * h
+        net.set_params(w, bb)
+        lp.append(net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 0.0))
+    fd = (lp[0] - lp[1]) / (2 * h)
+    log(f"residual loss {l0:.5e}: db fd {fd:.5e} kernel {gb[(L - 1) * C]:.5e}")
+    assert gb[(L - 1) * C] == pytest.approx(fd, rel=2e-2)
+    net.close()
+    ctx.close()
+
+
+def test_adam_steps_reduce_loss(pkg):
+    p, ctx, net, vec, dev = _setup(pkg, (16, 16, 8), 4)
+    w, b = pkg.learned.init_params(1)
+    w[L - 1] = np.random.default_rng(5).standard_normal(w[L - 1].shape) * 0.01
+    net.set_params(w, b)
+    net.set_static(vec["vx"], vec["vy"], vec["vz"], np.ones(6, np.float32), 1.0, 0.0)
+    losses = [net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 3e-3)
+              for _ in range(40)]
+    log(f"loss {losses[0]:.4e} -> {losses[-1]:.4e}")
+    assert losses[-1] < 0.9 * losses[0]
+    net.close()
+    ctx.close()
+
+
+def test_learned_guess_beats_previous_step(pkg):
+    """Small implicit-Euler run: train on 8 seeds x 40 steps, evaluate on the
+    unseen seed 0; every step is solved to rtol 1e-8 whatever the guess.  The
+    trained guess (filtered, then 2 Jacobi sweeps) needs fewer BiCG
+    iterations than the previous-step guess of the plain method."""
+    res, params = pkg.learned.experiment(pkg, shape=(32, 32, 16), dt=0.01, steps=40,
+                                         train_seeds=range(1, 9), eval_seed=0, epochs=10,
+                                         lr=5e-4, log=log)
+    for mode in ("previous", "extrap", "extrap+jacobi", "net", "net+jacobi"):
+        log(f"{mode}: {res[mode]['iterations']} iterations")
+    assert res["loss_per_epoch"][-1] < 0.2 * res["loss_per_epoch"][0]
+    assert res["net+jacobi"]["iterations"] < 0.9 * res["previous"]["iterations"]

@@ -31,6 +31,11 @@ EXPORTED = (
     "vgpu_true_residual", "vgpu_mass", "vgpu_dev_alloc", "vgpu_dev_free", "vgpu_memcpy_h2d",
     "vgpu_memcpy_d2h", "vgpu_time_kernel", "vgpu_autotune", "vgpu_batch_create",
     "vgpu_batch_destroy", "vgpu_batch_solve", "vgpu_batch_layout", "vgpu_batch_solve_dev",
+    "vgpu_assemble_rhs_dev", "vgpu_gaussian_dev", "vgpu_net_create", "vgpu_net_destroy",
+    "vgpu_net_set_params", "vgpu_net_get_params", "vgpu_net_get_grads", "vgpu_net_set_static",
+    "vgpu_net_predict_dev", "vgpu_net_train_step_dev", "vgpu_net_reset_optimizer",
+    "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
+    "vgpu_jacobi_dev", "vgpu_net_set_smoothing",
 )
 
 
@@ -99,6 +104,25 @@ def load():
     L.vgpu_batch_layout.argtypes = [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.vgpu_batch_solve_dev.argtypes = [_vp, C.c_int, _vp, _vp, C.c_double, C.c_int, _pi, _pd,
                                        _pi, _pd]
+    L.vgpu_assemble_rhs_dev.argtypes = [_vp, _vp, _vp, _vp]
+    L.vgpu_gaussian_dev.argtypes = [_vp, C.c_double, C.c_double, C.c_double, C.c_double, _vp,
+                                    _vp]
+    _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
+    L.vgpu_net_create.argtypes = [_vp, C.POINTER(_vp)]
+    L.vgpu_net_destroy.argtypes = [_vp]
+    L.vgpu_net_set_params.argtypes = [_vp, _fp, _fp]
+    L.vgpu_net_get_params.argtypes = [_vp, _fp, _fp]
+    L.vgpu_net_get_grads.argtypes = [_vp, _fp, _fp]
+    L.vgpu_net_set_static.argtypes = [_vp, _dp, _dp, _dp, _fp, C.c_double, C.c_double]
+    L.vgpu_copy_dev.argtypes = [_vp, _vp, _vp, C.c_int64]
+    L.vgpu_axpby_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp, _vp]
+    L.vgpu_residual_norm_dev.argtypes = [_vp, _vp, _vp, _pd]
+    L.vgpu_jacobi_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double]
+    L.vgpu_net_predict_dev.argtypes = [_vp, _vp, _vp, _vp, _vp]
+    L.vgpu_net_train_step_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_float, _pd]
+    L.vgpu_net_reset_optimizer.argtypes = [_vp]
+    L.vgpu_net_set_loss.argtypes = [_vp, C.c_int]
+    L.vgpu_net_set_smoothing.argtypes = [_vp, C.c_int]
     _lib = L
     return L
 
@@ -267,6 +291,28 @@ class Context:
                                       int(reps), C.byref(ms)), "time_kernel")
         return ms.value
 
+    def assemble_rhs_dev(self, q_dev, t_old_dev, b_dev):
+        check(load().vgpu_assemble_rhs_dev(self.h, q_dev, t_old_dev, b_dev), "assemble_rhs_dev")
+
+    def copy_dev(self, dst, src, nbytes):
+        check(load().vgpu_copy_dev(self.h, dst, src, int(nbytes)), "copy_dev")
+
+    def axpby_dev(self, a, x, b, y, out):
+        check(load().vgpu_axpby_dev(self.h, float(a), x, float(b), y, out), "axpby_dev")
+
+    def residual_norm_dev(self, b, x):
+        r = C.c_double(0)
+        check(load().vgpu_residual_norm_dev(self.h, b, x, C.byref(r)), "residual_norm_dev")
+        return r.value
+
+    def jacobi_dev(self, b, x, sweeps=1, omega=0.8):
+        check(load().vgpu_jacobi_dev(self.h, b, x, int(sweeps), float(omega)), "jacobi_dev")
+
+    def gaussian_dev(self, centre, sigma, q_elem_dev=None, src_node_dev=None):
+        check(load().vgpu_gaussian_dev(self.h, float(centre[0]), float(centre[1]),
+                                       float(centre[2]), float(sigma), q_elem_dev, src_node_dev),
+              "gaussian_dev")
+
     def autotune(self, precision=FP64, reps=20):
         """Time every owner tile shape / z chunk, keep the fastest; returns
         {"scatter", "tile", "zchunk", "ms"}."""
@@ -347,3 +393,72 @@ def device_count() -> int:
     n = C.c_int(0)
     load().vgpu_device_count(C.byref(n))
     return n.value
+
+
+NET_LAYERS, NET_TAPS, NET_CH = 5, 27, 16
+NET_W = NET_LAYERS * NET_TAPS * NET_CH * NET_CH
+NET_B = NET_LAYERS * NET_CH
+
+
+class Net:
+    """The learned-guess network on a context's node grid (vgpu_net_*)."""
+
+    def __init__(self, ctx):
+        h = _vp()
+        check(load().vgpu_net_create(ctx.h, C.byref(h)), "net_create")
+        self.h = h
+        self.ctx = ctx
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().vgpu_net_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, w, b):
+        check(load().vgpu_net_set_params(self.h, np.ascontiguousarray(w, np.float32).ravel(),
+                                         np.ascontiguousarray(b, np.float32).ravel()),
+              "net_set_params")
+
+    def get_params(self):
+        w = np.zeros(NET_W, np.float32)
+        b = np.zeros(NET_B, np.float32)
+        check(load().vgpu_net_get_params(self.h, w, b), "net_get_params")
+        return w, b
+
+    def get_grads(self):
+        w = np.zeros(NET_W, np.float32)
+        b = np.zeros(NET_B, np.float32)
+        check(load().vgpu_net_get_grads(self.h, w, b), "net_get_grads")
+        return w, b
+
+    def set_static(self, vx, vy, vz, in_scale, out_scale, extrap=0.0):
+        check(load().vgpu_net_set_static(self.h, _f64(vx), _f64(vy), _f64(vz),
+                                         np.ascontiguousarray(in_scale, np.float32),
+                                         float(out_scale), float(extrap)), "net_set_static")
+
+    def predict_dev(self, x1, x2, src, guess):
+        check(load().vgpu_net_predict_dev(self.h, x1, x2, src, guess), "net_predict")
+
+    def train_step_dev(self, x1, x2, src, target, lr):
+        loss = C.c_double(0)
+        check(load().vgpu_net_train_step_dev(self.h, x1, x2, src, target, float(lr),
+                                             C.byref(loss)), "net_train_step")
+        return loss.value
+
+    def reset_optimizer(self):
+        check(load().vgpu_net_reset_optimizer(self.h), "net_reset_optimizer")
+
+    def set_smoothing(self, passes):
+        """Passes of the [1 4 1]^3 / 216 low-pass filter on the output."""
+        check(load().vgpu_net_set_smoothing(self.h, int(passes)), "net_set_smoothing")
+
+    def set_loss(self, kind):
+        """0: increment L2 loss; 1: initial-residual loss through the operator."""
+        check(load().vgpu_net_set_loss(self.h, int(kind)), "net_set_loss")
+

@@ -63,6 +63,44 @@ __global__ void k_refine(int64_t n, double *x, double s, const float *d) {
     x[i] += s * (double)d[i];
 }
 
+// Gaussian source exp(-|c - x0|^2 / (2 sigma^2)) at element centroids times
+// the element volume (the q_in of a unit box), and at the nodes.
+__global__ void k_gauss_elem(Grid3 g, const double *xc, const double *yc,
+                             const double *zc, double cx, double cy, double cz,
+                             double inv2s2, double *q) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < g.E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % g.ex), j = (int)((e / g.ex) % g.ey);
+    const int k = (int)(e / ((int64_t)g.ex * g.ey));
+    const double x = 0.5 * (xc[i] + xc[i + 1]), y = 0.5 * (yc[j] + yc[j + 1]),
+                 z = 0.5 * (zc[k] + zc[k + 1]);
+    const double vol = (xc[i + 1] - xc[i]) * (yc[j + 1] - yc[j]) * (zc[k + 1] - zc[k]);
+    const double d2 = (x - cx) * (x - cx) + (y - cy) * (y - cy) + (z - cz) * (z - cz);
+    q[e] = exp(-d2 * inv2s2) * vol;
+  }
+}
+
+__global__ void k_gauss_node(Grid3 g, const double *xc, const double *yc,
+                             const double *zc, double cx, double cy, double cz,
+                             double inv2s2, double *s) {
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < g.N;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(n % g.nax), j = (int)((n / g.nax) % g.nay);
+    const int k = (int)(n / ((int64_t)g.nax * g.nay));
+    const double d2 = (xc[i] - cx) * (xc[i] - cx) + (yc[j] - cy) * (yc[j] - cy) +
+                      (zc[k] - cz) * (zc[k] - cz);
+    s[n] = exp(-d2 * inv2s2);
+  }
+}
+
+// x += omega (b - A x) / diag on free nodes (Ax holds A x)
+__global__ void k_jacobi(int64_t n, const uint8_t *mask, const double *b, const double *ax,
+                         const double *diag, double omega, double *x) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    x[i] = mask[i] ? 0.0 : x[i] + omega * (b[i] - ax[i]) / diag[i];
+}
+
 __global__ void k_fill(int64_t n, double v, double *out) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
@@ -87,79 +125,7 @@ inline dim3 grid_for(int64_t n, int bs = 256) {
 
 using namespace vgpu;
 
-struct vgpu_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  Grid3 g{};
-  double *xc = nullptr, *yc = nullptr, *zc = nullptr;
-  double *ah[3] = {nullptr, nullptr, nullptr}, *aih[3] = {nullptr, nullptr, nullptr};
-  float *fah[3] = {nullptr, nullptr, nullptr}, *faih[3] = {nullptr, nullptr, nullptr};
-  uint8_t *mask = nullptr;
-  double *rec = nullptr;  // per-element records vx, vy, vz, lambda, tau
-  float *frec = nullptr;
-  bool have_coeffs = false;
-  double P = 0, alpha = 0, dt = 1;
-  int variant = VGPU_DR_V1, scheme = VGPU_SCHEME_STEADY;
-  double cH = 1, cC = 0;
-  int scatter = VGPU_SCATTER_OWNER, zchunk = 0, tile = 0;
-  double *diag = nullptr;
-  float *fdiag = nullptr;
-  bool diag_valid = false;
-  double *vb = nullptr, *vx_ = nullptr, *vt = nullptr, *vt2 = nullptr,
-         *red = nullptr;
-  float *fb = nullptr, *fx = nullptr;
-  double *tpart = nullptr;  // dot partials of vgpu_time_kernel
-  int64_t tpart_n = 0;
-  Bicg<double> bicg64;  // brick scatter modes
-  Bicg<float> bicg32;
-  BBicg<double> bb64;   // owner scatter: batched engine, one system
-  BBicg<float> bb32;
-  uint64_t version = 1;
-  std::vector<void *> user_allocs;
-
-  OpData<double> op64(double cH_, double cC_) const {
-    OpData<double> o{};
-    o.g = g;
-    o.xc = xc;
-    o.yc = yc;
-    o.zc = zc;
-    o.mask = mask;
-    o.hx = ah[0];
-    o.hy = ah[1];
-    o.hz = ah[2];
-    o.ihx = aih[0];
-    o.ihy = aih[1];
-    o.ihz = aih[2];
-    o.rec = rec;
-    o.cH = cH_;
-    o.mu = cH_ * P + cC_;
-    o.ra = cH_ * alpha;
-    o.done = nullptr;
-    o.tile = tile;
-    return o;
-  }
-  OpData<float> op32() const {
-    OpData<float> o{};
-    o.g = g;
-    o.xc = xc;
-    o.yc = yc;
-    o.zc = zc;
-    o.mask = mask;
-    o.hx = fah[0];
-    o.hy = fah[1];
-    o.hz = fah[2];
-    o.ihx = faih[0];
-    o.ihy = faih[1];
-    o.ihz = faih[2];
-    o.rec = frec;
-    o.cH = cH;
-    o.mu = cH * P + cC;
-    o.ra = cH * alpha;
-    o.done = nullptr;
-    o.tile = tile;
-    return o;
-  }
-};
+#include "vgpu_ctx.hpp"
 
 namespace {
 
@@ -808,6 +774,44 @@ int vgpu_diag(vgpu_ctx *c, double *diag) {
   return VGPU_OK;
 }
 
+int vgpu_assemble_rhs_dev(vgpu_ctx *c, const double *q_dev, const double *t_old_dev,
+                          double *b_dev) {
+  if (!c || !q_dev || !b_dev) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) {
+    set_error("vgpu_assemble_rhs: coefficients not set");
+    return VGPU_ERR_STATE;
+  }
+  VGPU_HIP(hipSetDevice(c->device));
+  const int64_t N = c->g.N;
+  OpData<double> op = c->op64(c->cH, c->cC);
+  int rc = launch_source(op, q_dev, c->vt, c->stream);  // F
+  if (rc) return rc;
+  double fa = 1.0, ya = 0.0;
+  const bool use_t = t_old_dev != nullptr && c->scheme != VGPU_SCHEME_STEADY;
+  if (use_t) {
+    OpData<double> o2;
+    if (c->scheme == VGPU_SCHEME_THETA) {
+      // b = 3F - (H - (3/dt) C) t_old   (VANADIS.FOR:725-741)
+      o2 = c->op64(1.0, -3.0 / c->dt);
+      fa = 3.0;
+      ya = -1.0;
+    } else {
+      // b = F + (C/dt) t_old
+      o2 = c->op64(0.0, 1.0 / c->dt);
+      ya = 1.0;
+    }
+    rc = launch_apply<double, double>(o2, VGPU_SCATTER_OWNER, c->zchunk, t_old_dev,
+                                      c->vt2, nullptr, nullptr, nullptr, c->stream);
+    if (rc) return rc;
+  } else if (c->scheme == VGPU_SCHEME_THETA) {
+    fa = 3.0;
+  }
+  hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N, c->mask, fa,
+                     c->vt, ya, use_t ? c->vt2 : nullptr, b_dev);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
 int vgpu_assemble_rhs(vgpu_ctx *c, const double *q_in, const double *t_old,
                       double *b) {
   if (!c || !q_in || !b) return VGPU_ERR_ARG;
@@ -815,49 +819,24 @@ int vgpu_assemble_rhs(vgpu_ctx *c, const double *q_in, const double *t_old,
     set_error("vgpu_assemble_rhs: coefficients not set");
     return VGPU_ERR_STATE;
   }
+  VGPU_HIP(hipSetDevice(c->device));
   const int64_t N = c->g.N, E = c->g.E;
   double *q_dev = nullptr;
   VGPU_HIP(hipMalloc(&q_dev, E * sizeof(double)));
   int rc = VGPU_OK;
   do {
+    const bool use_t = t_old != nullptr && c->scheme != VGPU_SCHEME_STEADY;
     if (hipMemcpyAsync(q_dev, q_in, E * sizeof(double), hipMemcpyHostToDevice,
-                       c->stream) != hipSuccess) {
+                       c->stream) != hipSuccess ||
+        (use_t && hipMemcpyAsync(c->vx_, t_old, N * sizeof(double),
+                                 hipMemcpyHostToDevice, c->stream) != hipSuccess)) {
+      set_error("vgpu_assemble_rhs: upload failed");
       rc = VGPU_ERR_HIP;
       break;
     }
-    OpData<double> op = c->op64(c->cH, c->cC);
-    rc = launch_source(op, q_dev, c->vt, c->stream);  // F
+    rc = vgpu_assemble_rhs_dev(c, q_dev, use_t ? c->vx_ : nullptr, c->vb);
     if (rc) break;
-    double fa = 1.0, ya = 0.0;
-    bool use_t = t_old != nullptr && c->scheme != VGPU_SCHEME_STEADY;
-    if (use_t) {
-      if (hipMemcpyAsync(c->vx_, t_old, N * sizeof(double),
-                         hipMemcpyHostToDevice, c->stream) != hipSuccess) {
-        rc = VGPU_ERR_HIP;
-        break;
-      }
-      OpData<double> o2;
-      if (c->scheme == VGPU_SCHEME_THETA) {
-        // b = 3F - (H - (3/dt) C) t_old   (VANADIS.FOR:725-741)
-        o2 = c->op64(1.0, -3.0 / c->dt);
-        fa = 3.0;
-        ya = -1.0;
-      } else {
-        // b = F + (C/dt) t_old
-        o2 = c->op64(0.0, 1.0 / c->dt);
-        ya = 1.0;
-      }
-      rc = launch_apply<double, double>(o2, VGPU_SCATTER_OWNER, c->zchunk,
-                                        c->vx_, c->vt2, nullptr, nullptr,
-                                        nullptr, c->stream);
-      if (rc) break;
-    } else if (c->scheme == VGPU_SCHEME_THETA) {
-      fa = 3.0;
-    }
-    hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N,
-                       c->mask, fa, c->vt, ya, use_t ? c->vt2 : nullptr, c->vb);
-    if (hipGetLastError() != hipSuccess ||
-        hipMemcpyAsync(b, c->vb, N * sizeof(double), hipMemcpyDeviceToHost,
+    if (hipMemcpyAsync(b, c->vb, N * sizeof(double), hipMemcpyDeviceToHost,
                        c->stream) != hipSuccess ||
         hipStreamSynchronize(c->stream) != hipSuccess) {
       set_error("vgpu_assemble_rhs: HIP failure");
@@ -866,6 +845,80 @@ int vgpu_assemble_rhs(vgpu_ctx *c, const double *q_in, const double *t_old,
   } while (0);
   hipFree(q_dev);
   return rc;
+}
+
+int vgpu_copy_dev(vgpu_ctx *c, void *dst, const void *src, int64_t bytes) {
+  if (!c || !dst || !src || bytes < 0) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(c->device));
+  VGPU_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c->stream));
+  return VGPU_OK;
+}
+
+int vgpu_axpby_dev(vgpu_ctx *c, double a, const double *x, double b, const double *y,
+                   double *out) {
+  if (!c || !x || !out || (b != 0.0 && !y)) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(c->device));
+  const int64_t N = c->g.N;
+  hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N, c->mask, a, x,
+                     b, b != 0.0 ? y : nullptr, out);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
+int vgpu_residual_norm_dev(vgpu_ctx *c, const double *b, const double *x, double *rel) {
+  if (!c || !b || !x || !rel) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) return VGPU_ERR_STATE;
+  VGPU_HIP(hipSetDevice(c->device));
+  const int64_t N = c->g.N;
+  OpData<double> op = c->op64(c->cH, c->cC);
+  int rc = launch_apply<double, double>(op, VGPU_SCATTER_OWNER, c->zchunk, x, c->vt,
+                                        nullptr, nullptr, nullptr, c->stream);
+  if (rc) return rc;
+  const int np = nparts_for(N);
+  hipLaunchKernelGGL(k_resid, dim3(np), dim3(VB), 0, c->stream, N, b, c->vt, c->red);
+  hipLaunchKernelGGL((k_dot<double>), dim3(np), dim3(VB), 0, c->stream, N, nullptr, b, b,
+                     c->red + np);
+  VGPU_HIP(hipGetLastError());
+  const double rn = host_sum(c->red, np, c->stream, &rc);
+  if (rc) return rc;
+  const double bn = host_sum(c->red + np, np, c->stream, &rc);
+  if (rc) return rc;
+  *rel = bn > 0 ? std::sqrt(rn / bn) : std::sqrt(rn);
+  return VGPU_OK;
+}
+
+int vgpu_jacobi_dev(vgpu_ctx *c, const double *b, double *x, int sweeps, double omega) {
+  if (!c || !b || !x || sweeps < 0) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) return VGPU_ERR_STATE;
+  VGPU_HIP(hipSetDevice(c->device));
+  int rc = ensure_diag(c);
+  if (rc) return rc;
+  const int64_t N = c->g.N;
+  OpData<double> op = c->op64(c->cH, c->cC);
+  for (int k = 0; k < sweeps; ++k) {
+    rc = launch_apply<double, double>(op, VGPU_SCATTER_OWNER, c->zchunk, x, c->vt, nullptr,
+                                      nullptr, nullptr, c->stream);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_jacobi, grid_for(N), dim3(256), 0, c->stream, N, c->mask, b, c->vt,
+                       c->diag, omega, x);
+    VGPU_HIP(hipGetLastError());
+  }
+  return VGPU_OK;
+}
+
+int vgpu_gaussian_dev(vgpu_ctx *c, double cx, double cy, double cz, double sigma,
+                      double *q_elem_dev, double *src_node_dev) {
+  if (!c || !(sigma > 0)) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(c->device));
+  const double inv2s2 = 0.5 / (sigma * sigma);
+  if (q_elem_dev)
+    hipLaunchKernelGGL(k_gauss_elem, grid_for(c->g.E), dim3(256), 0, c->stream, c->g,
+                       c->xc, c->yc, c->zc, cx, cy, cz, inv2s2, q_elem_dev);
+  if (src_node_dev)
+    hipLaunchKernelGGL(k_gauss_node, grid_for(c->g.N), dim3(256), 0, c->stream, c->g,
+                       c->xc, c->yc, c->zc, cx, cy, cz, inv2s2, src_node_dev);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
 }
 
 int vgpu_solve_dev(vgpu_ctx *c, int precision, const double *b, double *x,

@@ -1,0 +1,681 @@
+// Learned initial guess for implicit time stepping: a 5-layer 3x3x3
+// convolution network on the node grid (16 channels, "same" zero padding,
+// leaky ReLU (slope 0.1) between layers), BF16 operands on the MFMA units with FP32
+// accumulation, trained on the device with hand-written backward kernels and
+// Adam.  No counterpart in the reference (its host restarts BiCG from the
+// previous step's solution, VANADIS.FOR:809-810 with TNW2 in/out); the guess
+// only changes x0, BiCG still runs to the same rtol.
+//
+// Inputs (channels 0..5, 6..15 zero): x_{n-1}, the increment x_{n-1} - x_{n-2}
+// (differenced in FP64: separately rounded states would lose it), the nodal
+// source, vx, vy, vz, each times a scale; output channel 0 of layer 5, times
+// out_scale, is the predicted correction of a base guess:
+// guess = x_{n-1} + extrap (x_{n-1} - x_{n-2}) + out_scale * net.
+//
+// Tensors are node-major, 16 BF16 channels per node ([N][16], 32 bytes).
+// One wave computes 16 consecutive nodes of an x row for all 16 output
+// channels: per tap one v_mfma_f32_16x16x16_bf16 with A = the tap's weights
+// (output x input channels, held in registers) and B = the 16 nodes' input
+// channels (one 8-byte load per lane).  MFMA operand layout (16x16x16,
+// lane l): A[l%16][4(l/16)+i], B[4(l/16)+i][l%16], D[4(l/16)+i][l%16].
+#include <cmath>
+#include <cstring>
+
+#include "vgpu_ctx.hpp"
+
+namespace vgpu {
+namespace {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+constexpr int NC = 16;            // channels
+constexpr int NL = 5;             // layers
+constexpr int NTAP = 27;          // 3x3x3 taps, t = dx + 3 dy + 9 dz, offset d-1
+constexpr int WL = NTAP * NC * NC;  // weights per layer
+constexpr int NWAVE_W = 1024;     // waves of the weight-gradient kernel
+constexpr float SLOPE = 0.1f;     // leaky ReLU slope for negative inputs
+
+// round to nearest even (NaN kept quiet)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u)
+    return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) {
+  return __uint_as_float((uint32_t)h << 16);
+}
+
+__device__ __forceinline__ v4s load4(const uint16_t *p) {
+  return *reinterpret_cast<const v4s *>(p);
+}
+
+struct RowSeg {
+  int x0, y, z;
+};
+__device__ __forceinline__ RowSeg row_seg(const Grid3 &g, int64_t s) {
+  const int nsx = (g.nax + 15) >> 4;
+  const int64_t r = s / nsx;
+  return {(int)(s % nsx) * 16, (int)(r % g.nay), (int)(r / g.nay)};
+}
+
+__device__ __forceinline__ bool inside(const Grid3 &g, int x, int y, int z) {
+  return x >= 0 && y >= 0 && z >= 0 && x < g.nax && y < g.nay && z < g.naz;
+}
+
+// Forward layer: Y[n][co] = act(b[co] + sum_t sum_ci W[t][co][ci] X[n+off_t][ci]).
+// LAST: no activation, channel 0 to out[n] in FP32.
+template <bool RELU, bool LAST>
+__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__restrict__ X,
+                                                  const uint16_t *__restrict__ W,
+                                                  const float *__restrict__ bias,
+                                                  uint16_t *__restrict__ Y,
+                                                  float *__restrict__ out) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nseg = (int64_t)((g.nax + 15) >> 4) * g.nay * g.naz;
+  if (wave >= nseg) return;  // whole waves only
+  const RowSeg rs = row_seg(g, wave);
+  const int col = lane & 15, cg = lane >> 4;
+  const int x = rs.x0 + col;
+  v4s a[NTAP];
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t) a[t] = load4(W + (t * NC + col) * NC + 4 * cg);
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t) {
+    const int xx = x + t % 3 - 1, yy = rs.y + (t / 3) % 3 - 1, zz = rs.z + t / 9 - 1;
+    v4s b = {0, 0, 0, 0};
+    if (inside(g, xx, yy, zz))
+      b = load4(X + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
+  }
+  if (x >= g.nax) return;
+  const int64_t n = x + (int64_t)g.nax * (rs.y + (int64_t)g.nay * rs.z);
+  if (LAST) {
+    if (cg == 0) out[n] = acc[0] + bias[0];
+    return;
+  }
+  v4s o;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    float v = acc[i] + bias[4 * cg + i];
+    if (RELU && !(v > 0.f)) v *= SLOPE;
+    o[i] = (short)f2bf(v);
+  }
+  *reinterpret_cast<v4s *>(Y + n * NC + 4 * cg) = o;
+}
+
+// Backward through a layer's input: dX[n][ci] = sum_t sum_co W[t][co][ci]
+// dZ[n-off_t][co] (WT = W transposed per tap, [t][ci][co]); through the
+// leaky ReLU of the previous layer: dZprev = dX where Yprev > 0, else SLOPE dX.
+__global__ __launch_bounds__(256) void k_conv_bwd_data(Grid3 g,
+                                                       const uint16_t *__restrict__ dZ,
+                                                       const uint16_t *__restrict__ WT,
+                                                       const uint16_t *__restrict__ Yprev,
+                                                       uint16_t *__restrict__ dZprev) {
+  const int lane = threadIdx.x & 63;
+  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
+  const int64_t nseg = (int64_t)((g.nax + 15) >> 4) * g.nay * g.naz;
+  if (wave >= nseg) return;
+  const RowSeg rs = row_seg(g, wave);
+  const int col = lane & 15, cg = lane >> 4;
+  const int x = rs.x0 + col;
+  v4s a[NTAP];
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t) a[t] = load4(WT + (t * NC + col) * NC + 4 * cg);
+  v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t) {
+    const int xx = x - (t % 3 - 1), yy = rs.y - ((t / 3) % 3 - 1), zz = rs.z - (t / 9 - 1);
+    v4s b = {0, 0, 0, 0};
+    if (inside(g, xx, yy, zz))
+      b = load4(dZ + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
+  }
+  if (x >= g.nax) return;
+  const int64_t n = x + (int64_t)g.nax * (rs.y + (int64_t)g.nay * rs.z);
+  const v4s yp = load4(Yprev + n * NC + 4 * cg);
+  v4s o;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+    o[i] = (short)f2bf(bf2f((uint16_t)yp[i]) > 0.f ? acc[i] : SLOPE * acc[i]);
+  *reinterpret_cast<v4s *>(dZprev + n * NC + 4 * cg) = o;
+}
+
+// Weight and bias gradients of a layer, per wave over a fixed set of row
+// segments (s = wave, wave + NWAVE_W, ...):
+//   dW[t][co][ci] = sum_n dZ[n][co] X[n+off_t][ci],  db[co] = sum_n dZ[n][co]
+// as MFMAs over 16 nodes (K).  part[w][t][co][ci], partb[w][lane].
+__global__ __launch_bounds__(256) void k_conv_bwd_w(Grid3 g, const uint16_t *__restrict__ X,
+                                                    const uint16_t *__restrict__ dZ,
+                                                    float *__restrict__ part,
+                                                    float *__restrict__ partb) {
+  const int lane = threadIdx.x & 63;
+  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int64_t nseg = (int64_t)((g.nax + 15) >> 4) * g.nay * g.naz;
+  const int col = lane & 15, cg = lane >> 4;
+  v4f acc[NTAP];
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
+  float bsum = 0.f;
+  for (int64_t s = w; s < nseg; s += NWAVE_W) {
+    const RowSeg rs = row_seg(g, s);
+    // A[co = col][k = node 4cg+i]
+    v4s a;
+    int xi[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      xi[i] = rs.x0 + 4 * cg + i;
+      uint16_t h = 0;
+      if (xi[i] < g.nax)
+        h = dZ[(xi[i] + (int64_t)g.nax * (rs.y + (int64_t)g.nay * rs.z)) * NC + col];
+      a[i] = (short)h;
+      bsum += bf2f(h);
+    }
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) {
+      const int dx = t % 3 - 1, yy = rs.y + (t / 3) % 3 - 1, zz = rs.z + t / 9 - 1;
+      v4s b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int xx = xi[i] + dx;
+        uint16_t h = 0;
+        if (xi[i] < g.nax && inside(g, xx, yy, zz))
+          h = X[(xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + col];
+        b[i] = (short)h;
+      }
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+    }
+  }
+  float *pw = part + (int64_t)w * WL;
+#pragma unroll
+  for (int t = 0; t < NTAP; ++t)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) pw[(t * NC + 4 * cg + i) * NC + col] = acc[t][i];
+  partb[(int64_t)w * 64 + lane] = bsum;
+}
+
+// dW[i] = sum over waves in order; db[co] = sum over waves and lane groups
+__global__ void k_wsum(const float *__restrict__ part, const float *__restrict__ partb,
+                       float *__restrict__ dW, float *__restrict__ db) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < WL) {
+    float s = 0.f;
+    for (int w = 0; w < NWAVE_W; ++w) s += part[(int64_t)w * WL + i];
+    dW[i] = s;
+  } else if (i < WL + NC) {
+    const int co = i - WL;
+    float s = 0.f;
+    for (int w = 0; w < NWAVE_W; ++w)
+      for (int c = 0; c < 4; ++c) s += partb[(int64_t)w * 64 + c * 16 + co];
+    db[co] = s;
+  }
+}
+
+// 16-channel input tensor from FP64 node vectors (channels 6..15 zero); the
+// step increment x1 - x2 is formed in FP64 before the BF16 rounding
+__global__ void k_pack(int64_t n, const double *x1, const double *x2, const double *src,
+                       const double *vx, const double *vy, const double *vz,
+                       float s0, float s1, float s2, float s3, float s4, float s5,
+                       uint16_t *X0) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    v4s o0, o1;
+    o0[0] = (short)f2bf((float)(x1[i] * (double)s0));
+    o0[1] = (short)f2bf((float)((x1[i] - x2[i]) * (double)s1));
+    o0[2] = (short)f2bf((float)(src[i] * (double)s2));
+    o0[3] = (short)f2bf((float)(vx[i] * (double)s3));
+    o1[0] = (short)f2bf((float)(vy[i] * (double)s4));
+    o1[1] = (short)f2bf((float)(vz[i] * (double)s5));
+    o1[2] = o1[3] = 0;
+    v4s z = {0, 0, 0, 0};
+    v4s *p = reinterpret_cast<v4s *>(X0 + i * NC);
+    p[0] = o0;
+    p[1] = o1;
+    p[2] = z;
+    p[3] = z;
+  }
+}
+
+// guess = x1 + ex (x1 - x2) + os * out
+__global__ void k_guess(int64_t n, const double *x1, const double *x2, const float *out,
+                        double ex, double os, double *guess) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    guess[i] = x1[i] + ex * (x1[i] - x2[i]) + os * (double)out[i];
+}
+
+// Loss = sum over free nodes of (out - t)^2 / count with
+// t = (target - x1 - ex (x1 - x2)) / os; gout = dLoss/dout = 2 (out - t) /
+// count; block partials.
+__global__ __launch_bounds__(256) void k_loss(int64_t n, const float *out, const double *x1,
+                                              const double *x2, const double *target,
+                                              const uint8_t *mask, double ex, double inv_os,
+                                              float inv_count, float *gout,
+                                              double *lpart) {
+  __shared__ double red[4];
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    float d = 0.f;
+    if (!mask[i]) d = out[i] - (float)((target[i] - x1[i] - ex * (x1[i] - x2[i])) * inv_os);
+    s += (double)d * (double)d;
+    gout[i] = 2.f * d * inv_count;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) lpart[blockIdx.x] = (red[0] + red[1]) + (red[2] + red[3]);
+}
+
+// Residual loss, step 1: base = x1 + ex (x1 - x2), g = base + os out,
+// e = target - g and e0 = target - base on free nodes (0 on Dirichlet nodes).
+__global__ void k_res_prep(int64_t n, const double *x1, const double *x2, const float *out,
+                           const double *target, const uint8_t *mask, double ex, double os,
+                           double *e, double *e0) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double base = x1[i] + ex * (x1[i] - x2[i]);
+    const bool m = mask[i] != 0;
+    e[i] = m ? 0.0 : target[i] - (base + os * (double)out[i]);
+    e0[i] = m ? 0.0 : target[i] - base;
+  }
+}
+
+// block partials of a.a and b.b
+__global__ __launch_bounds__(256) void k_sq2(int64_t n, const double *a, const double *b,
+                                             double *pa, double *pb) {
+  __shared__ double ra[4], rb[4];
+  double sa = 0.0, sb = 0.0;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    sa += a[i] * a[i];
+    sb += b[i] * b[i];
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    sa += __shfl_down(sa, o, 64);
+    sb += __shfl_down(sb, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    ra[threadIdx.x >> 6] = sa;
+    rb[threadIdx.x >> 6] = sb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pa[blockIdx.x] = (ra[0] + ra[1]) + (ra[2] + ra[3]);
+    pb[blockIdx.x] = (rb[0] + rb[1]) + (rb[2] + rb[3]);
+  }
+}
+
+// Residual loss, step 2: dL/dout = coef * u (u = A^T A e) on free nodes
+__global__ void k_res_grad(int64_t n, const double *u, const uint8_t *mask, double coef,
+                           float *gout) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    gout[i] = mask[i] ? 0.f : (float)(coef * u[i]);
+}
+
+// One axis of the [1 4 1] / 6 filter (zero padding, so the filter is
+// symmetric and its own transpose): damps the grid-scale (checkerboard)
+// modes, the slowest ones of a mass-dominated operator, by 1/3 per axis.
+__global__ void k_smooth_axis(Grid3 g, int axis, const float *in, float *out) {
+  const int64_t n = g.N;
+  const int64_t st = axis == 0 ? 1 : (axis == 1 ? (int64_t)g.nax : (int64_t)g.nax * g.nay);
+  const int len = axis == 0 ? g.nax : (axis == 1 ? g.nay : g.naz);
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int c = (int)((i / st) % len);
+    float v = 4.f * in[i];
+    if (c > 0) v += in[i - st];
+    if (c < len - 1) v += in[i + st];
+    out[i] = v * (1.f / 6.f);
+  }
+}
+
+__global__ void k_copy_f(int64_t n, const float *a, float *b) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
+// dZ4[n][0] = bf16(g[n]) (other channels stay zero)
+__global__ void k_pack_grad(int64_t n, const float *g, uint16_t *dZ4) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    dZ4[i * NC] = f2bf(g[i]);
+}
+
+// Adam (bias-corrected) on n parameters
+__global__ void k_adam(int n, float *p, const float *gr, float *m, float *v, float lr,
+                       float b1, float b2, float eps, float c1, float c2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float g = gr[i];
+  const float mi = b1 * m[i] + (1.f - b1) * g;
+  const float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// BF16 copies of the weights: W [l][t][co][ci] and per-tap transposes
+__global__ void k_cast_w(const float *W, uint16_t *Wb, uint16_t *WbT) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= NL * WL) return;
+  const int l = i / WL, r = i % WL, t = r / (NC * NC), co = (r / NC) % NC, ci = r % NC;
+  const uint16_t h = f2bf(W[i]);
+  Wb[i] = h;
+  WbT[l * WL + (t * NC + ci) * NC + co] = h;
+}
+
+inline dim3 grid1(int64_t n, int bs = 256) {
+  int64_t gsz = (n + bs - 1) / bs;
+  if (gsz > 4096) gsz = 4096;
+  if (gsz < 1) gsz = 1;
+  return dim3((unsigned)gsz);
+}
+
+}  // namespace
+}  // namespace vgpu
+
+struct vgpu_net {
+  vgpu_ctx *c = nullptr;
+  int64_t N = 0, nseg = 0, count = 0;
+  uint16_t *X0 = nullptr, *Y[NL - 1] = {}, *dZ4 = nullptr, *dZa = nullptr, *dZb = nullptr;
+  float *out = nullptr;
+  float *W = nullptr, *B = nullptr, *mW = nullptr, *vW = nullptr, *mB = nullptr,
+        *vB = nullptr, *dW = nullptr, *dB = nullptr, *part = nullptr, *partb = nullptr;
+  uint16_t *Wb = nullptr, *WbT = nullptr;
+  double *vx = nullptr, *vy = nullptr, *vz = nullptr, *lpart = nullptr;
+  float in_scale[6] = {1, 1, 1, 1, 1, 1};
+  double out_scale = 1.0, extrap = 0.0;
+  int adam_t = 0;
+  int loss_kind = 0;  // 0: increment L2, 1: operator residual
+  int smooth = 0;     // passes of the [1 4 1]^3 / 216 filter on the output
+  double *e = nullptr, *e0 = nullptr, *r = nullptr, *r0 = nullptr;
+  float *fo = nullptr, *ft = nullptr;  // filtered output / gradient, scratch
+
+  ~vgpu_net() {
+    void *ptrs[] = {X0, Y[0], Y[1], Y[2], Y[3], dZ4, dZa, dZb, out, W,  B, mW, vW, mB,
+                    vB, dW, dB, part, partb, Wb, WbT, vx, vy, vz, lpart, e, e0, r, r0,
+                    fo, ft};
+    for (void *p : ptrs)
+      if (p) hipFree(p);
+  }
+
+  int forward(const double *x1, const double *x2, const double *src) {
+    hipStream_t s = c->stream;
+    const Grid3 &g = c->g;
+    hipLaunchKernelGGL(k_pack, grid1(N), dim3(256), 0, s, N, x1, x2, src, vx, vy, vz,
+                       in_scale[0], in_scale[1], in_scale[2], in_scale[3], in_scale[4],
+                       in_scale[5], X0);
+    const dim3 grid((unsigned)((nseg + 3) / 4)), blk(256);
+    const uint16_t *in = X0;
+    for (int l = 0; l < NL - 1; ++l) {
+      hipLaunchKernelGGL((k_conv_fwd<true, false>), grid, blk, 0, s, g, in, Wb + l * WL,
+                         B + l * NC, Y[l], nullptr);
+      in = Y[l];
+    }
+    hipLaunchKernelGGL((k_conv_fwd<false, true>), grid, blk, 0, s, g, in,
+                       Wb + (NL - 1) * WL, B + (NL - 1) * NC, nullptr, out);
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  }
+
+  // v <- S^passes v (S = the three axis filters); uses ft as scratch
+  int filter(float *v) {
+    hipStream_t s = c->stream;
+    for (int p = 0; p < smooth; ++p)
+      for (int ax = 0; ax < 3; ++ax) {
+        hipLaunchKernelGGL(k_smooth_axis, grid1(N), dim3(256), 0, s, c->g, ax, v, ft);
+        hipLaunchKernelGGL(k_copy_f, grid1(N), dim3(256), 0, s, N, ft, v);
+      }
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  }
+  // the output the guess uses: out, or S^passes out in fo
+  const float *final_out() {
+    if (!smooth) return out;
+    hipLaunchKernelGGL(k_copy_f, grid1(N), dim3(256), 0, c->stream, N, out, fo);
+    filter(fo);
+    return fo;
+  }
+
+  int weight_grad(int l, const uint16_t *Xin, const uint16_t *dZl) {
+    hipStream_t s = c->stream;
+    hipLaunchKernelGGL(k_conv_bwd_w, dim3(NWAVE_W / 4), dim3(256), 0, s, c->g, Xin, dZl,
+                       part, partb);
+    hipLaunchKernelGGL(k_wsum, dim3((WL + NC + 255) / 256), dim3(256), 0, s, part, partb,
+                       dW + l * WL, dB + l * NC);
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  }
+};
+
+extern "C" {
+
+int vgpu_net_create(vgpu_ctx *c, vgpu_net **out) {
+  if (!c || !out) return VGPU_ERR_ARG;
+  *out = nullptr;
+  VGPU_HIP(hipSetDevice(c->device));
+  auto *n = new (std::nothrow) vgpu_net();
+  if (!n) return VGPU_ERR_NOMEM;
+  n->c = c;
+  n->N = c->g.N;
+  n->nseg = (int64_t)((c->g.nax + 15) / 16) * c->g.nay * c->g.naz;
+  const size_t T = (size_t)n->N * NC * sizeof(uint16_t);
+  hipError_t e = hipSuccess;
+  auto A = [&](void **p, size_t bytes) {
+    if (e == hipSuccess) e = hipMalloc(p, bytes);
+    if (e == hipSuccess) e = hipMemset(*p, 0, bytes);
+  };
+  A((void **)&n->X0, T);
+  for (int l = 0; l < NL - 1; ++l) A((void **)&n->Y[l], T);
+  A((void **)&n->dZ4, T);
+  A((void **)&n->dZa, T);
+  A((void **)&n->dZb, T);
+  A((void **)&n->out, n->N * sizeof(float));
+  A((void **)&n->fo, n->N * sizeof(float));
+  A((void **)&n->ft, n->N * sizeof(float));
+  for (float **p : {&n->W, &n->mW, &n->vW, &n->dW}) A((void **)p, NL * WL * sizeof(float));
+  for (float **p : {&n->B, &n->mB, &n->vB, &n->dB}) A((void **)p, NL * NC * sizeof(float));
+  A((void **)&n->part, (size_t)NWAVE_W * WL * sizeof(float));
+  A((void **)&n->partb, (size_t)NWAVE_W * 64 * sizeof(float));
+  A((void **)&n->Wb, NL * WL * sizeof(uint16_t));
+  A((void **)&n->WbT, NL * WL * sizeof(uint16_t));
+  for (double **p : {&n->vx, &n->vy, &n->vz, &n->e, &n->e0, &n->r, &n->r0})
+    A((void **)p, n->N * sizeof(double));
+  A((void **)&n->lpart, 4096 * sizeof(double));
+  if (e != hipSuccess) {
+    set_error(std::string("vgpu_net_create: ") + hipGetErrorString(e));
+    delete n;
+    return e == hipErrorOutOfMemory ? VGPU_ERR_NOMEM : VGPU_ERR_HIP;
+  }
+  // free (non-Dirichlet) nodes: the loss normaliser
+  std::vector<uint8_t> m(n->N);
+  if (hipMemcpy(m.data(), c->mask, n->N, hipMemcpyDeviceToHost) != hipSuccess) {
+    delete n;
+    return VGPU_ERR_HIP;
+  }
+  int64_t cnt = 0;
+  for (uint8_t v : m) cnt += v == 0;
+  n->count = std::max<int64_t>(1, cnt);
+  *out = n;
+  return VGPU_OK;
+}
+
+int vgpu_net_destroy(vgpu_net *n) {
+  if (!n) return VGPU_ERR_ARG;
+  hipSetDevice(n->c->device);
+  hipStreamSynchronize(n->c->stream);
+  delete n;
+  return VGPU_OK;
+}
+
+int vgpu_net_set_params(vgpu_net *n, const float *w, const float *b) {
+  if (!n || !w || !b) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(n->c->device));
+  VGPU_HIP(hipMemcpy(n->W, w, NL * WL * sizeof(float), hipMemcpyHostToDevice));
+  VGPU_HIP(hipMemcpy(n->B, b, NL * NC * sizeof(float), hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_cast_w, dim3((NL * WL + 255) / 256), dim3(256), 0, n->c->stream,
+                     n->W, n->Wb, n->WbT);
+  VGPU_HIP(hipGetLastError());
+  VGPU_HIP(hipStreamSynchronize(n->c->stream));
+  return VGPU_OK;
+}
+
+int vgpu_net_get_params(vgpu_net *n, float *w, float *b) {
+  if (!n || !w || !b) return VGPU_ERR_ARG;
+  VGPU_HIP(hipStreamSynchronize(n->c->stream));
+  VGPU_HIP(hipMemcpy(w, n->W, NL * WL * sizeof(float), hipMemcpyDeviceToHost));
+  VGPU_HIP(hipMemcpy(b, n->B, NL * NC * sizeof(float), hipMemcpyDeviceToHost));
+  return VGPU_OK;
+}
+
+int vgpu_net_get_grads(vgpu_net *n, float *dw, float *db) {
+  if (!n || !dw || !db) return VGPU_ERR_ARG;
+  VGPU_HIP(hipStreamSynchronize(n->c->stream));
+  VGPU_HIP(hipMemcpy(dw, n->dW, NL * WL * sizeof(float), hipMemcpyDeviceToHost));
+  VGPU_HIP(hipMemcpy(db, n->dB, NL * NC * sizeof(float), hipMemcpyDeviceToHost));
+  return VGPU_OK;
+}
+
+int vgpu_net_set_static(vgpu_net *n, const double *vx, const double *vy, const double *vz,
+                        const float *in_scale, double out_scale, double extrap) {
+  if (!n || !vx || !vy || !vz || !in_scale || !(out_scale > 0)) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(n->c->device));
+  VGPU_HIP(hipMemcpy(n->vx, vx, n->N * sizeof(double), hipMemcpyHostToDevice));
+  VGPU_HIP(hipMemcpy(n->vy, vy, n->N * sizeof(double), hipMemcpyHostToDevice));
+  VGPU_HIP(hipMemcpy(n->vz, vz, n->N * sizeof(double), hipMemcpyHostToDevice));
+  std::memcpy(n->in_scale, in_scale, sizeof n->in_scale);
+  n->out_scale = out_scale;
+  n->extrap = extrap;
+  return VGPU_OK;
+}
+
+int vgpu_net_predict_dev(vgpu_net *n, const double *x1, const double *x2,
+                         const double *src, double *guess) {
+  if (!n || !x1 || !x2 || !src || !guess) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(n->c->device));
+  int rc = n->forward(x1, x2, src);
+  if (rc) return rc;
+  const float *o = n->final_out();
+  hipLaunchKernelGGL(k_guess, grid1(n->N), dim3(256), 0, n->c->stream, n->N, x1, x2, o,
+                     n->extrap, n->out_scale, guess);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
+int vgpu_net_train_step_dev(vgpu_net *n, const double *x1, const double *x2,
+                            const double *src, const double *target, float lr,
+                            double *loss) {
+  if (!n || !x1 || !x2 || !src || !target) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(n->c->device));
+  hipStream_t s = n->c->stream;
+  int rc = n->forward(x1, x2, src);
+  if (rc) return rc;
+  const int nlb = 1024;
+  double lval = 0.0;
+  const float *o = n->final_out();  // fo when filtered
+  float *gout = n->fo;              // dLoss/d(final output), then through S^T = S
+  if (n->loss_kind == 0) {
+    hipLaunchKernelGGL(k_loss, dim3(nlb), dim3(256), 0, s, n->N, o, x1, x2, target,
+                       n->c->mask, n->extrap, 1.0 / n->out_scale,
+                       (float)(1.0 / (double)n->count), gout, n->lpart);
+    VGPU_HIP(hipGetLastError());
+    std::vector<double> h(nlb);
+    VGPU_HIP(hipMemcpyAsync(h.data(), n->lpart, nlb * sizeof(double), hipMemcpyDeviceToHost, s));
+    VGPU_HIP(hipStreamSynchronize(s));
+    for (double v : h) lval += v;
+    lval /= (double)n->count;
+  } else {
+    // L = |A e|^2 / |A e0|^2, e = x_n - guess, e0 = x_n - base guess;
+    // dL/dout = -2 os A^T A e / |A e0|^2 (A = the masked FP64 operator)
+    hipLaunchKernelGGL(k_res_prep, grid1(n->N), dim3(256), 0, s, n->N, x1, x2, o,
+                       target, n->c->mask, n->extrap, n->out_scale, n->e, n->e0);
+    VGPU_HIP(hipGetLastError());
+    rc = vgpu_apply_dev(n->c, VGPU_SCATTER_OWNER, 0, n->e, n->r);
+    if (!rc) rc = vgpu_apply_dev(n->c, VGPU_SCATTER_OWNER, 0, n->e0, n->r0);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_sq2, dim3(nlb), dim3(256), 0, s, n->N, n->r, n->r0, n->lpart,
+                       n->lpart + nlb);
+    VGPU_HIP(hipGetLastError());
+    std::vector<double> h(2 * nlb);
+    VGPU_HIP(hipMemcpyAsync(h.data(), n->lpart, 2 * nlb * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    VGPU_HIP(hipStreamSynchronize(s));
+    double ra = 0.0, rb = 0.0;
+    for (int i = 0; i < nlb; ++i) {
+      ra += h[i];
+      rb += h[nlb + i];
+    }
+    const double D = rb > 0.0 ? rb : 1.0;
+    lval = ra / D;
+    rc = vgpu_apply_dev(n->c, VGPU_SCATTER_OWNER, 1, n->r, n->e0);  // u = A^T r
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_res_grad, grid1(n->N), dim3(256), 0, s, n->N, n->e0, n->c->mask,
+                       -2.0 * n->out_scale / D, gout);
+    VGPU_HIP(hipGetLastError());
+  }
+  rc = n->filter(gout);  // the filter is symmetric: S^T g = S g
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_pack_grad, grid1(n->N), dim3(256), 0, s, n->N, gout, n->dZ4);
+  VGPU_HIP(hipGetLastError());
+  // backward: layer 4 (no activation), then 3..0 through the ReLUs
+  const dim3 grid((unsigned)((n->nseg + 3) / 4)), blk(256);
+  const uint16_t *dz = n->dZ4;
+  uint16_t *bufs[2] = {n->dZa, n->dZb};
+  for (int l = NL - 1; l >= 0; --l) {
+    const uint16_t *xin = l == 0 ? n->X0 : n->Y[l - 1];
+    rc = n->weight_grad(l, xin, dz);
+    if (rc) return rc;
+    if (l > 0) {
+      uint16_t *dprev = bufs[l & 1];
+      hipLaunchKernelGGL(k_conv_bwd_data, grid, blk, 0, s, n->c->g, dz, n->WbT + l * WL,
+                         n->Y[l - 1], dprev);
+      dz = dprev;
+    }
+  }
+  VGPU_HIP(hipGetLastError());
+  if (loss) *loss = lval;
+  if (lr > 0.f) {
+    n->adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - std::pow(b1, (float)n->adam_t),
+                c2 = 1.f - std::pow(b2, (float)n->adam_t);
+    hipLaunchKernelGGL(k_adam, dim3((NL * WL + 255) / 256), dim3(256), 0, s, NL * WL, n->W,
+                       n->dW, n->mW, n->vW, lr, b1, b2, 1e-8f, c1, c2);
+    hipLaunchKernelGGL(k_adam, dim3(1), dim3(256), 0, s, NL * NC, n->B, n->dB, n->mB, n->vB,
+                       lr, b1, b2, 1e-8f, c1, c2);
+    hipLaunchKernelGGL(k_cast_w, dim3((NL * WL + 255) / 256), dim3(256), 0, s, n->W, n->Wb,
+                       n->WbT);
+    VGPU_HIP(hipGetLastError());
+  }
+  return VGPU_OK;
+}
+
+int vgpu_net_set_smoothing(vgpu_net *n, int passes) {
+  if (!n || passes < 0 || passes > 8) return VGPU_ERR_ARG;
+  n->smooth = passes;
+  return VGPU_OK;
+}
+
+int vgpu_net_set_loss(vgpu_net *n, int kind) {
+  if (!n || kind < 0 || kind > 1) return VGPU_ERR_ARG;
+  n->loss_kind = kind;
+  return VGPU_OK;
+}
+
+int vgpu_net_reset_optimizer(vgpu_net *n) {
+  if (!n) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(n->c->device));
+  for (float *p : {n->mW, n->vW}) VGPU_HIP(hipMemset(p, 0, NL * WL * sizeof(float)));
+  for (float *p : {n->mB, n->vB}) VGPU_HIP(hipMemset(p, 0, NL * NC * sizeof(float)));
+  n->adam_t = 0;
+  return VGPU_OK;
+}
+
+}  // extern "C"
