@@ -93,6 +93,30 @@ def measured_traffic(tuned):
             "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json tile {tile}"}
 
 
+def learned_guess(pkg):
+    """Implicit Euler (dt 0.01, 200 steps, helix source of seed 0) on the
+    128x128x64 box, every step to rtol 1e-8, from the previous-step guess
+    (the plain method), linear extrapolation + 2 Jacobi sweeps, and the
+    trained network (committed parameters, trained on seeds 1-10 by
+    learned.main) + 2 Jacobi sweeps."""
+    path = os.path.join(ROOT, "vanadis_solver-directional_residual_amd", "data",
+                        "learned_guess_128x128x64.npz")
+    if not os.path.exists(path):
+        return {"status": "no trained parameters"}
+    params = pkg.learned.load_params(path)
+    progress("learned guess: 3 x 200 implicit-Euler steps")
+    res, _ = pkg.learned.experiment(pkg, params=params,
+                                    modes=("previous", "extrap+jacobi", "net+jacobi"))
+    out = {"workload": "HEX8 unit box 128x128x64, implicit Euler dt=0.01, 200 steps, "
+                       "helix source seed 0, BiCG rtol 1e-8 each step"}
+    for m in ("previous", "extrap+jacobi", "net+jacobi"):
+        out[m] = {"iterations": res[m]["iterations"], "seconds": res[m]["seconds"],
+                  "fallbacks": res[m]["fallbacks"]}
+    out["net_speedup_vs_previous"] = res["net+jacobi_speedup_vs_previous"]
+    out["extrap_speedup_vs_previous"] = res["extrap+jacobi_speedup_vs_previous"]
+    return out
+
+
 def batched_solves(pkg, tuned):
     """Batched FP64 solves to rtol 1e-8 against the same systems solved one by
     one: 32 right-hand sides (own diffusivity and source) on one 96x96x48 mesh,
@@ -149,6 +173,8 @@ def main():
                     help="keep the default owner tile (32x8) instead of the autotuner's")
     ap.add_argument("--tile", type=int, default=-1,
                     help="owner tile shape 0/1/2 (default: autotuner, or 0 with --kernel-only)")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="skip the implicit-Euler run with the trained initial guess")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched solves (32 right-hand sides, 16 subdomains)")
     ap.add_argument("--size", default="128,128,64")
@@ -256,6 +282,14 @@ def main():
             batch = {"status": f"failed: {exc}"}
             progress(f"batched {batch['status']}")
 
+    learned = None
+    if not a.no_learned:
+        try:
+            learned = learned_guess(pkg)
+        except Exception as exc:  # reported, never fatal for the line
+            learned = {"status": f"failed: {exc}"}
+            progress(f"learned {learned['status']}")
+
     cpu = None
     if not a.no_cpu:
         progress("cpu baseline")
@@ -288,6 +322,7 @@ def main():
         "cpu_baseline": cpu,
         "autotune": tuned,
         "batched": batch,
+        "learned_guess": learned,
     }
     traffic = measured_traffic(tuned)
     if traffic:

@@ -150,6 +150,17 @@ int vgpu_true_residual(vgpu_ctx *ctx, const double *b, const double *x,
                        double *rel);
 /* Mass integral (2x2x2 Gauss, exact for trilinear c on boxes). */
 int vgpu_mass(vgpu_ctx *ctx, const double *c, double *mass);
+/* Diagonal-dominance diagnostics of the element matrices of the current
+ * scheme (diagonal_dominance_factor, VANADIS.FOR:2434-2489, and
+ * is_diagonally_dominant, 2248-2303): ddf = max over nodes of
+ * sum_e sum_{j != i} |A_e(i,j)| / sum_e |A_e(i,i)|; dominant = 1 when every
+ * node's diagonal sum exceeds its off-diagonal sum (may be NULL). */
+int vgpu_ddf(vgpu_ctx *ctx, double *ddf, int *dominant);
+/* The host's T-file (VANADIS.FOR:835-875): one line per node,
+ * x, y, z, c * 1e6 in Fortran e12.5 separated by two blanks.  Host arrays;
+ * no device work. */
+int vgpu_write_tfile(const char *path, int64_t n, const double *xx, const double *yy,
+                     const double *zz, const double *c);
 
 /* Device buffer helpers for callers that keep data resident (bench, batch). */
 int vgpu_dev_alloc(vgpu_ctx *ctx, int64_t bytes, void **ptr);

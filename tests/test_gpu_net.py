@@ -147,3 +147,22 @@ def test_learned_guess_beats_previous_step(pkg):
         log(f"{mode}: {res[mode]['iterations']} iterations")
     assert res["loss_per_epoch"][-1] < 0.2 * res["loss_per_epoch"][0]
     assert res["net+jacobi"]["iterations"] < 0.9 * res["previous"]["iterations"]
+
+
+def test_trajectory_store_round_trip(pkg, tmp_path):
+    p = pkg.problem.unit_box(12, 10, 8, lam=1e-3, scheme=pkg.SCHEME_EULER, dt=0.01)
+    ctx = pkg.Context.from_problem(p)
+    st = pkg.learned.Stepper(ctx, p, 0.01)
+    buf = st.alloc_states(6)
+    pkg.learned.run(st, 3, 5, "previous", states=buf)
+    path = str(tmp_path / "traj.npy")
+    pkg.learned.save_states(path, st, buf, 6)
+    buf2, n = pkg.learned.load_states(path, st)
+    assert n == 6
+    for i in range(6):
+        a = ctx.d2h(st.ptr(buf, i), p.N)
+        assert np.array_equal(a, ctx.d2h(st.ptr(buf2, i), p.N))
+        assert np.array_equal(a, np.load(path, mmap_mode="r")[i])
+    assert np.abs(ctx.d2h(st.ptr(buf, 5), p.N)).max() > 0
+    ctx.close()
+

@@ -202,3 +202,22 @@ def test_context_rejects_non_tensor_grid(pkg, gpu):
     with pytest.raises(pkg.VgpuError) as ei:
         pkg.Context(p.nax, p.nay, p.naz, p.xx, p.yy, p.zz, iss, p.mask)
     assert ei.value.code == pkg.ERR_MESH
+
+
+def test_ddf_matches_oracle(case, po):
+    """Diagonal-dominance factor and flag of the element matrices on the
+    device against diagonal_dominance_factor / is_diagonally_dominant."""
+    name, p, bb, _, _, ctx = case
+    d, dom = ctx.ddf()
+    ref = po.ddf(p.E, p.N, p.iss, bb)
+    log(f"ddf {name}: gpu {d!r} oracle {ref!r}")
+    assert d == pytest.approx(ref, rel=1e-12)
+    assert dom == po.is_dd(p.E, p.N, p.iss, bb)
+
+
+def test_ddf_reproduces_transcript(pkg, gpu, golden_log):
+    """The shipped case's DDF as printed by the reference (vanadis_log.txt)."""
+    p = pkg.problem.reference_default_case()
+    with pkg.Context.from_problem(p) as ctx:
+        d, _ = ctx.ddf()
+    assert d == pytest.approx(float(golden_log["steps"][0]["ddf_text"]), rel=1e-12)

@@ -35,7 +35,7 @@ EXPORTED = (
     "vgpu_net_set_params", "vgpu_net_get_params", "vgpu_net_get_grads", "vgpu_net_set_static",
     "vgpu_net_predict_dev", "vgpu_net_train_step_dev", "vgpu_net_reset_optimizer",
     "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
-    "vgpu_jacobi_dev", "vgpu_net_set_smoothing",
+    "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
 )
 
 
@@ -118,6 +118,8 @@ def load():
     L.vgpu_axpby_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp, _vp]
     L.vgpu_residual_norm_dev.argtypes = [_vp, _vp, _vp, _pd]
     L.vgpu_jacobi_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double]
+    L.vgpu_ddf.argtypes = [_vp, _pd, _pi]
+    L.vgpu_write_tfile.argtypes = [C.c_char_p, C.c_int64, _dp, _dp, _dp, _dp]
     L.vgpu_net_predict_dev.argtypes = [_vp, _vp, _vp, _vp, _vp]
     L.vgpu_net_train_step_dev.argtypes = [_vp, _vp, _vp, _vp, _vp, C.c_float, _pd]
     L.vgpu_net_reset_optimizer.argtypes = [_vp]
@@ -254,6 +256,13 @@ class Context:
         r = C.c_double(0)
         check(load().vgpu_true_residual(self.h, _f64(b), _f64(x), C.byref(r)), "true_residual")
         return r.value
+
+    def ddf(self):
+        """(diagonal dominance factor, all rows dominant) of the element matrices."""
+        d = C.c_double(0)
+        dom = C.c_int(0)
+        check(load().vgpu_ddf(self.h, C.byref(d), C.byref(dom)), "ddf")
+        return d.value, bool(dom.value)
 
     def mass(self, c):
         m = C.c_double(0)
@@ -461,4 +470,10 @@ class Net:
     def set_loss(self, kind):
         """0: increment L2 loss; 1: initial-residual loss through the operator."""
         check(load().vgpu_net_set_loss(self.h, int(kind)), "net_set_loss")
+
+
+def write_tfile(path, xx, yy, zz, c):
+    """The host's T-file: x, y, z, c*1e6 per node in Fortran e12.5."""
+    xx, yy, zz, c = _f64(xx), _f64(yy), _f64(zz), _f64(c)
+    check(load().vgpu_write_tfile(os.fsencode(path), len(c), xx, yy, zz, c), "write_tfile")
 

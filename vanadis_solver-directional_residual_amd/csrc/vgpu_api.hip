@@ -3,6 +3,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -1255,6 +1257,105 @@ int vgpu_true_residual(vgpu_ctx *c, const double *b, const double *x,
   const double bn = host_sum(c->red + np, np, c->stream, &rc);
   if (rc) return rc;
   *rel = std::sqrt(rn / bn);
+  return VGPU_OK;
+}
+
+}  // extern "C"
+
+namespace {
+// Fortran Ew.d edit descriptor with w = 12, d = 5: [-]0.ddddd E+ee, right
+// justified; exponents beyond 99 drop the E ([-]0.ddddd+eee); zero prints as
+// 0.00000E+00; NaN and infinities right-justified.
+void fortran_e12_5(double v, char out[13]) {
+  char t[40];
+  if (std::isnan(v)) {
+    std::snprintf(out, 13, "%12s", "NaN");
+    return;
+  }
+  if (std::isinf(v)) {
+    std::snprintf(out, 13, "%12s", v < 0 ? "-Infinity" : "Infinity");
+    return;
+  }
+  const bool neg = std::signbit(v);
+  int e10 = 0;
+  char dig[6] = "00000";
+  if (v != 0.0) {
+    std::snprintf(t, sizeof t, "%.4e", std::fabs(v));  // d.dddde+XX
+    dig[0] = t[0];
+    for (int i = 0; i < 4; ++i) dig[i + 1] = t[2 + i];
+    e10 = std::atoi(t + 7) + 1;
+  }
+  if (e10 >= -99 && e10 <= 99)
+    std::snprintf(t, sizeof t, "%s0.%sE%c%02d", neg ? "-" : "", dig, e10 < 0 ? '-' : '+',
+                  e10 < 0 ? -e10 : e10);
+  else
+    std::snprintf(t, sizeof t, "%s0.%s%c%03d", neg ? "-" : "", dig, e10 < 0 ? '-' : '+',
+                  e10 < 0 ? -e10 : e10);
+  std::snprintf(out, 13, "%12s", t);
+}
+}  // namespace
+
+extern "C" {
+
+int vgpu_write_tfile(const char *path, int64_t n, const double *xx, const double *yy,
+                     const double *zz, const double *c) {
+  if (!path || n < 0 || (n > 0 && (!xx || !yy || !zz || !c))) return VGPU_ERR_ARG;
+  FILE *f = std::fopen(path, "w");
+  if (!f) {
+    set_error(std::string("vgpu_write_tfile: cannot open ") + path);
+    return VGPU_ERR_ARG;
+  }
+  char a[13], b[13], d[13], e[13];
+  for (int64_t i = 0; i < n; ++i) {
+    fortran_e12_5(xx[i], a);
+    fortran_e12_5(yy[i], b);
+    fortran_e12_5(zz[i], d);
+    fortran_e12_5(c[i] * 1e6, e);
+    std::fprintf(f, "%s  %s  %s  %s\n", a, b, d, e);
+  }
+  if (std::fclose(f) != 0) {
+    set_error("vgpu_write_tfile: write failed");
+    return VGPU_ERR_ARG;
+  }
+  return VGPU_OK;
+}
+
+int vgpu_ddf(vgpu_ctx *c, double *ddf, int *dominant) {
+  if (!c || !ddf) return VGPU_ERR_ARG;
+  if (!c->have_coeffs) return VGPU_ERR_STATE;
+  VGPU_HIP(hipSetDevice(c->device));
+  constexpr int NB = 1024;
+  double *pm = nullptr;
+  int *pb = nullptr;
+  VGPU_HIP(hipMalloc(&pm, NB * sizeof(double)));
+  if (hipMalloc(&pb, NB * sizeof(int)) != hipSuccess) {
+    hipFree(pm);
+    set_error("vgpu_ddf: allocation failed");
+    return VGPU_ERR_NOMEM;
+  }
+  OpData<double> op = c->op64(c->cH, c->cC);
+  int rc = launch_ddf(op, pm, pb, NB, c->stream);
+  std::vector<double> hm(NB);
+  std::vector<int> hb(NB);
+  if (!rc && (hipMemcpyAsync(hm.data(), pm, NB * sizeof(double), hipMemcpyDeviceToHost,
+                             c->stream) != hipSuccess ||
+              hipMemcpyAsync(hb.data(), pb, NB * sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream) != hipSuccess ||
+              hipStreamSynchronize(c->stream) != hipSuccess)) {
+    set_error("vgpu_ddf: readback failed");
+    rc = VGPU_ERR_HIP;
+  }
+  hipFree(pm);
+  hipFree(pb);
+  if (rc) return rc;
+  double m = 0.0;
+  int64_t bad = 0;
+  for (int i = 0; i < NB; ++i) {
+    m = std::max(m, hm[i]);
+    bad += hb[i];
+  }
+  *ddf = m;
+  if (dominant) *dominant = bad == 0;
   return VGPU_OK;
 }
 

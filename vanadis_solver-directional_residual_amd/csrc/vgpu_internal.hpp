@@ -168,6 +168,10 @@ int launch_apply(const OpData<T> &op, int scatter, int zchunk,
 int launch_tau(const Grid3 &g, const double *xc, const double *yc,
                const double *zc, double *rec, int variant, hipStream_t s);
 int launch_diag(const OpData<double> &op, double *diag, hipStream_t s);
+// Per-block largest off-diagonal / diagonal ratio of the element-matrix rows
+// and the count of rows that are not diagonally dominant (nblocks x 256).
+int launch_ddf(const OpData<double> &op, double *pmax, int *pbad, int nblocks,
+               hipStream_t s);
 int launch_source(const OpData<double> &op, const double *q_in, double *f,
                   hipStream_t s);
 int launch_mass(const Grid3 &g, const double *xc, const double *yc,

@@ -877,6 +877,78 @@ int launch_diag(const OpData<double> &op, double *diag, hipStream_t s) {
   return VGPU_OK;
 }
 
+// Row a of an element matrix: y = A_e^T e_a (closed form, with the Robin term
+// on x-min elements).
+__device__ void elem_row(const OpData<double> &op, int ix, int iy, int iz, int a,
+                         double y[8]) {
+  const int64_t e = ix + (int64_t)op.g.ex * (iy + (int64_t)op.g.ey * iz);
+  ElemConst<double> k;
+  const double *r = op.rec + 5 * e;
+  elem_setup<double>(op.hx[ix], op.hy[iy], op.hz[iz], op.ihx[ix], op.ihy[iy],
+                     op.ihz[iz], r[3], r[0], r[1], r[2], r[4], op.cH, op.mu, k);
+  double x[8], xh[8], yh[8];
+  for (int c = 0; c < 8; ++c) x[c] = (c == a) ? 1.0 : 0.0;
+  haar_fwd<double>(x, xh);
+  haar_apply<double, true>(k, xh, yh);
+  haar_inv<double>(yh, y);
+  if (ix == 0 && op.ra != 0.0)
+    robin_add<double, true>(k, op.hx[ix], r[4], r[0], r[1], r[2], op.ra, x, y);
+}
+
+// Diagonal dominance of the element matrices as diagonal_dominance_factor and
+// is_diagonally_dominant compute it (VANADIS.FOR:2434-2489, 2248-2303): per
+// node i, dg = sum over its elements of |A_e(i,i)|, sr = sum of |A_e(i,j)|,
+// j != i; per block the largest sr/dg (rows with dg > 0) and the number of
+// rows with dg <= sr.
+__global__ __launch_bounds__(256) void k_ddf(OpData<double> op, double *pmax, int *pbad) {
+  const int LOC[8] = {0, 1, 3, 2, 4, 5, 7, 6};  // index cx + 2cy + 4cz -> local
+  __shared__ double rm[4];
+  __shared__ int rb[4];
+  double m = 0.0;
+  int bad = 0;
+  for (int64_t n = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; n < op.g.N;
+       n += (int64_t)gridDim.x * blockDim.x) {
+    const int gx = (int)(n % op.g.nax);
+    const int gy = (int)((n / op.g.nax) % op.g.nay);
+    const int gz = (int)(n / ((int64_t)op.g.nax * op.g.nay));
+    double dg = 0.0, sr = 0.0;
+    for (int c = 0; c < 8; ++c) {
+      const int ix = gx - (c & 1), iy = gy - ((c >> 1) & 1), iz = gz - (c >> 2);
+      if (ix < 0 || iy < 0 || iz < 0 || ix >= op.g.ex || iy >= op.g.ey || iz >= op.g.ez)
+        continue;
+      const int a = LOC[c];
+      double y[8];
+      elem_row(op, ix, iy, iz, a, y);
+      for (int b = 0; b < 8; ++b) {
+        if (b == a) dg += fabs(y[b]);
+        else sr += fabs(y[b]);
+      }
+    }
+    if (dg > 0.0) m = fmax(m, sr / dg);
+    bad += dg <= sr;
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    m = fmax(m, __shfl_down(m, o, 64));
+    bad += __shfl_down(bad, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    rm[threadIdx.x >> 6] = m;
+    rb[threadIdx.x >> 6] = bad;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    pmax[blockIdx.x] = fmax(fmax(rm[0], rm[1]), fmax(rm[2], rm[3]));
+    pbad[blockIdx.x] = rb[0] + rb[1] + rb[2] + rb[3];
+  }
+}
+
+int launch_ddf(const OpData<double> &op, double *pmax, int *pbad, int nblocks,
+               hipStream_t s) {
+  hipLaunchKernelGGL(k_ddf, dim3(nblocks), dim3(256), 0, s, op, pmax, pbad);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
 // Element source F_a = q_in [1/8 + tau sum_k v_k s_k(a) / (4 h_k)]
 // (el3d fmv, VANADIS.FOR:525, 551-553: qv = q_in/vol, int w_a), node-centric.
 __global__ void k_source(OpData<double> op, const double *q_in, double *f) {

@@ -269,6 +269,28 @@ def experiment(pkg, shape=(128, 128, 64), dt=0.01, steps=200, train_seeds=range(
     return res, params
 
 
+def save_states(path, stepper, buf, count):
+    """Binary trajectory store: `count` device state vectors of `buf` as an
+    .npy array of shape (count, N), FP64, written one state at a time."""
+    arr = np.lib.format.open_memmap(path, mode="w+", dtype=np.float64,
+                                    shape=(count, stepper.N))
+    for i in range(count):
+        arr[i] = stepper.ctx.d2h(stepper.ptr(buf, i), stepper.N)
+    arr.flush()
+    del arr
+
+
+def load_states(path, stepper):
+    """Upload a stored trajectory into a new device buffer; returns (buf, count)."""
+    arr = np.load(path, mmap_mode="r")
+    if arr.ndim != 2 or arr.shape[1] != stepper.N:
+        raise ValueError(f"{path}: states of {arr.shape[1:]} nodes, mesh has {stepper.N}")
+    buf = stepper.ctx.dev_alloc(8 * stepper.N * arr.shape[0])
+    for i in range(arr.shape[0]):
+        stepper.ctx.h2d(stepper.ptr(buf, i), np.ascontiguousarray(arr[i]))
+    return buf, arr.shape[0]
+
+
 def save_params(path, params):
     np.savez(path, w=params["w"], b=params["b"], in_scale=params["in_scale"],
              out_scale=np.float64(params["out_scale"]), extrap=np.float64(params["extrap"]),
