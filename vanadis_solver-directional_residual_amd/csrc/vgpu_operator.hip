@@ -106,6 +106,31 @@ __device__ __forceinline__ void plane_load(const OpData<T> &op, const TIO *x1,
     node_raw<TIO, DUAL>(op.g, op.mask, x1, x2, gx0 + lx1, gy0 + ly1, gz, r.a1, r.b1, r.m1);
 }
 
+// The same with the nodes' column offsets gx + nax gy precomputed (-1 for a
+// column outside the grid): one multiply-add per node and layer.
+template <typename TIO, bool DUAL>
+__device__ __forceinline__ void node_raw_c(const uint8_t *mask, const TIO *x1,
+                                           const TIO *x2, int64_t col, int64_t plane,
+                                           int gz, int naz, TIO &a, TIO &b, uint8_t &m) {
+  a = b = TIO(0);
+  m = 1;
+  if (col >= 0 && gz >= 0 && gz < naz) {
+    const int64_t n = col + plane * gz;
+    m = mask[n];
+    a = x1[n];
+    if (DUAL) b = x2[n];
+  }
+}
+
+template <typename TIO, bool DUAL>
+__device__ __forceinline__ void plane_load_c(const uint8_t *mask, const TIO *x1,
+                                             const TIO *x2, int64_t col0, int64_t col1,
+                                             int64_t plane, int gz, int naz,
+                                             PlaneRegs<TIO, DUAL> &r) {
+  node_raw_c<TIO, DUAL>(mask, x1, x2, col0, plane, gz, naz, r.a0, r.b0, r.m0);
+  node_raw_c<TIO, DUAL>(mask, x1, x2, col1, plane, gz, naz, r.a1, r.b1, r.m1);
+}
+
 template <typename T, typename TIO, bool DUAL>
 __device__ __forceinline__ void plane_store(const PlaneRegs<TIO, DUAL> &r, T *p1,
                                             T *p2, uint8_t *pm, int q0, int q1,
@@ -129,25 +154,24 @@ __device__ __forceinline__ void elem_eval(const OpData<T> &op, bool valid,
                                           bool robin, const ColConst<T> &q,
                                           const Coef<T> &cf, const T x1[8],
                                           const T x2[8], T y1[8], T y2[8]) {
+  // an element outside the grid has zero fields (load_coef); with a zero mass
+  // weight and no Robin term every constant vanishes and so do y1, y2
   ElemConst<T> k;
   elem_layer_setup<T>(q, cf.hz, cf.ihz, cf.lam, cf.vx, cf.vy, cf.vz, cf.tau,
-                      T(op.cH), T(op.mu), k);
+                      T(op.cH), valid ? T(op.mu) : T(0), k);
+  const bool rb = ROBIN && robin && valid;
   T xh[8], yh[8];
   haar_fwd<T>(x1, xh);
   haar_apply<T, false>(k, xh, yh);
   haar_inv<T>(yh, y1);
-  if (ROBIN && robin)
+  if (rb)
     robin_add<T, false>(k, q.mx0 * T(4), cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x1, y1);
   if (DUAL) {
     haar_fwd<T>(x2, xh);
     haar_apply<T, true>(k, xh, yh);
     haar_inv<T>(yh, y2);
-    if (ROBIN && robin)
+    if (rb)
       robin_add<T, true>(k, q.mx0 * T(4), cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), x2, y2);
-  }
-  if (!valid) {
-#pragma unroll
-    for (int c = 0; c < 8; ++c) y1[c] = y2[c] = T(0);
   }
 }
 
@@ -252,6 +276,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     const int64_t plane = (int64_t)op.g.nax * op.g.nay;
     const int64_t ecol = exg + (int64_t)op.g.ex * eyg;
     const int64_t eplane = (int64_t)op.g.ex * op.g.ey;
+    // column offsets of this thread's two plane nodes (-1: outside the grid)
+    int64_t pc0 = -1, pc1 = -1;
+    {
+      const int ax = gx0 + lx0, ay = gy0 + ly0;
+      if (ax >= 0 && ay >= 0 && ax < op.g.nax && ay < op.g.nay)
+        pc0 = ax + (int64_t)op.g.nax * ay;
+      const int bx = gx0 + lx1, by = gy0 + ly1;
+      if (has1 && bx >= 0 && by >= 0 && bx < op.g.nax && by < op.g.nay)
+        pc1 = bx + (int64_t)op.g.nax * by;
+    }
 
     // dot terms: each thread sums its owned nodes of the current unit (its
     // planes in order); at the unit's last plane the block sum goes to
@@ -281,9 +315,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     int kl = k0 - 1;
     int bb = (kl + 3) % 3, bt = (kl + 4) % 3, bn = (kl + 5) % 3;
     PlaneRegs<TIO, DUAL> pr;
-    plane_load<T, TIO, DUAL>(op, xm1, xm2, gx0, gy0, kl, lx0, ly0, lx1, ly1, has1, pr);
+    plane_load_c<TIO, DUAL>(op.mask, xm1, xm2, pc0, pc1, plane, kl, naz, pr);
     plane_store<T, TIO, DUAL>(pr, pl1[bb], DUAL ? pl2[bb] : nullptr, pm[bb], q0, q1, has1);
-    plane_load<T, TIO, DUAL>(op, xm1, xm2, gx0, gy0, kl + 1, lx0, ly0, lx1, ly1, has1, pr);
+    plane_load_c<TIO, DUAL>(op.mask, xm1, xm2, pc0, pc1, plane, kl + 1, naz, pr);
     const bool ev0 = colok && kl >= 0;
     Coef<T> cn = load_coef<T>(op, ev0, ecol + eplane * (ev0 ? kl : 0), max(kl, 0));
 
@@ -299,8 +333,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
       const Coef<T> cf = cn;
       const bool ev = colok && kl >= 0 && kl < op.g.ez;
       if (kl + 1 < k1) {  // prefetch plane kl+2 and layer kl+1
-        plane_load<T, TIO, DUAL>(op, xm1, xm2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1,
-                                 has1, pr);
+        plane_load_c<TIO, DUAL>(op.mask, xm1, xm2, pc0, pc1, plane, kl + 2, naz, pr);
         const bool evn = colok && kl + 1 < op.g.ez;
         cn = load_coef<T>(op, evn, ecol + eplane * (evn ? kl + 1 : 0), kl + 1);
       }
