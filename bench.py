@@ -93,6 +93,39 @@ def measured_traffic(tuned):
             "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json tile {tile}"}
 
 
+def config3(pkg):
+    """256x256x64 elements (4.29M nodes), per-element diffusivity log-uniform
+    in [1e-4, 1e-2]: FP64 BiCG and mixed precision (FP32 inner BiCG, rtol 1e-4,
+    FP64 refinement) to rtol 1e-8; rel-L2 of the mixed solution vs FP64."""
+    progress("config 3: 256x256x64")
+    p = pkg.problem.config_problem("c3")
+    ctx = pkg.Context.from_problem(p)
+    ctx.autotune(pkg.FP64, reps=5)
+    b = ctx.assemble_rhs(p.q_in)
+    b_dev, x_dev = ctx.dev_alloc(8 * p.N), ctx.dev_alloc(8 * p.N)
+    ctx.h2d(b_dev, b)
+    zeros = np.zeros(p.N)
+    out = {"workload": "HEX8 unit box 256x256x64 elements, 4293185 nodes, lambda log-uniform "
+                       "[1e-4, 1e-2] per element, V1 tau, rtol 1e-8 from x0=0"}
+    xs = {}
+    for name, prec, inner in (("fp64", pkg.FP64, 0.0), ("mixed", pkg.MIXED, 1e-4)):
+        for rep in range(2):  # first solve captures the graph
+            ctx.h2d(x_dev, zeros)
+            it, err, sec, rc = ctx.solve_dev(b_dev, x_dev, rtol=1e-8, itmax=50000,
+                                             precision=prec, inner_rtol=inner)
+        xs[name] = ctx.d2h(x_dev, p.N)
+        out[name] = {"iterations": it, "seconds": round(sec, 4), "status": rc,
+                     "true_rel_residual": ctx.true_residual(b, xs[name])}
+        progress(f"config 3 {name}: {it} iterations, {sec * 1e3:.1f} ms, rc {rc}")
+    out["mixed_rel_l2_vs_fp64"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
+                                        np.linalg.norm(xs["fp64"]))
+    out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    out["dual_apply_ms"] = {"fp64": round(ctx.time_kernel(1, "owner", pkg.FP64, 20), 5),
+                            "fp32": round(ctx.time_kernel(1, "owner", pkg.FP32, 20), 5)}
+    ctx.close()
+    return out
+
+
 def learned_guess(pkg):
     """Implicit Euler (dt 0.01, 200 steps, helix source of seed 0) on the
     128x128x64 box, every step to rtol 1e-8, from the previous-step guess
@@ -173,6 +206,8 @@ def main():
                     help="keep the default owner tile (32x8) instead of the autotuner's")
     ap.add_argument("--tile", type=int, default=-1,
                     help="owner tile shape 0/1/2 (default: autotuner, or 0 with --kernel-only)")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the 256x256x64 FP64 / mixed-precision solves")
     ap.add_argument("--no-learned", action="store_true",
                     help="skip the implicit-Euler run with the trained initial guess")
     ap.add_argument("--no-batch", action="store_true",
@@ -282,6 +317,14 @@ def main():
             batch = {"status": f"failed: {exc}"}
             progress(f"batched {batch['status']}")
 
+    c3 = None
+    if not a.no_c3:
+        try:
+            c3 = config3(pkg)
+        except Exception as exc:  # reported, never fatal for the line
+            c3 = {"status": f"failed: {exc}"}
+            progress(f"c3 {c3['status']}")
+
     learned = None
     if not a.no_learned:
         try:
@@ -323,6 +366,7 @@ def main():
         "autotune": tuned,
         "batched": batch,
         "learned_guess": learned,
+        "config3": c3,
     }
     traffic = measured_traffic(tuned)
     if traffic:
