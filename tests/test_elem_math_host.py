@@ -18,8 +18,9 @@ def shim(tmp_path_factory):
                     "-o", str(out), os.path.join(HERE, "host", "elem_shim.cpp")], check=True)
     L = C.CDLL(str(out))
     dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
-    L.elem_matrix.argtypes = [dp, C.c_double, dp, C.c_double, C.c_double, C.c_double,
-                              C.c_double, C.c_int, C.c_int, dp]
+    for fn in (L.elem_matrix, L.elem_matrix_faces):
+        fn.argtypes = [dp, C.c_double, dp, C.c_double, C.c_double, C.c_double,
+                       C.c_double, C.c_int, C.c_int, dp]
     return L
 
 
@@ -50,9 +51,13 @@ def test_device_element_matrix_matches_oracle(shim, po, seed, scheme):
     dj = ctypes.c_double(); tau = ctypes.c_double()
     po.lib().orc_gauss_point(1, 0.5, 0.5, 0.5, xe, ye, ze, np.full(3, lam), v,
                              ctypes.byref(dj), n, w, d1, d2, d3, ctypes.byref(tau))
-    for tr in (0, 1):
-        A = np.zeros(64)
-        shim.elem_matrix(h, lam, v, tau.value, cH, cH * P + cC, cH * alpha, robin, tr, A)
-        A = A.reshape(8, 8)
-        ref = Aref.T if tr else Aref
-        assert np.abs(A - ref).max() <= 1e-13 * np.abs(ref).max(), (tr, np.abs(A - ref).max())
+    # the per-element path of the brick kernels and the factored face path of
+    # the marching owner kernel
+    for fn in (shim.elem_matrix, shim.elem_matrix_faces):
+        for tr in (0, 1):
+            A = np.zeros(64)
+            fn(h, lam, v, tau.value, cH, cH * P + cC, cH * alpha, robin, tr, A)
+            A = A.reshape(8, 8)
+            ref = Aref.T if tr else Aref
+            err = np.abs(A - ref).max()
+            assert err <= 1e-13 * np.abs(ref).max(), (fn.__name__, tr, err)

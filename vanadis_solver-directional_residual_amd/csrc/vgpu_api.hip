@@ -559,6 +559,9 @@ int vgpu_create(int nax, int nay, int naz, const double *xx, const double *yy,
   CK(hipGetDevice(&c->device));
   CK(hipStreamCreate(&c->stream));
   c->g = Grid3{nax, nay, naz, nax - 1, nay - 1, naz - 1, N, E};
+  // the owner kernel addresses each vector and the element records with
+  // 32-bit offsets; larger systems use the coloured scatter
+  if (!owner_fits(c->g)) c->scatter = VGPU_SCATTER_COLORED;
   CK(hipMalloc(&c->xc, nax * sizeof(double)));
   CK(hipMalloc(&c->yc, nay * sizeof(double)));
   CK(hipMalloc(&c->zc, naz * sizeof(double)));
@@ -703,6 +706,11 @@ int vgpu_set_scheme(vgpu_ctx *c, int scheme, double dt) {
 int vgpu_set_options(vgpu_ctx *c, int scatter, int zchunk, int tile) {
   if (!c || scatter < 0 || scatter > 2 || zchunk < -1 || zchunk > 256 || tile > 2) {
     set_error("vgpu_set_options: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  if (scatter == VGPU_SCATTER_OWNER && !owner_fits(c->g)) {
+    set_error("vgpu_set_options: system too large for the owner scatter (8 N and 40 E "
+              "must stay below 2^31 bytes); use the coloured or atomic scatter");
     return VGPU_ERR_ARG;
   }
   c->scatter = scatter;
@@ -1089,11 +1097,16 @@ int vgpu_batch_create(int nm, vgpu_ctx *const *ctxs, vgpu_batch **out) {
     return VGPU_ERR_ARG;
   }
   *out = nullptr;
-  for (int i = 0; i < nm; ++i)
+  for (int i = 0; i < nm; ++i) {
     if (!ctxs[i] || ctxs[i]->device != ctxs[0]->device) {
       set_error("vgpu_batch_create: null context or contexts on different devices");
       return VGPU_ERR_ARG;
     }
+    if (!owner_fits(ctxs[i]->g)) {
+      set_error("vgpu_batch_create: system too large for the owner scatter");
+      return VGPU_ERR_ARG;
+    }
+  }
   auto *bt = new (std::nothrow) vgpu_batch();
   if (!bt) return VGPU_ERR_NOMEM;
   bt->device = ctxs[0]->device;

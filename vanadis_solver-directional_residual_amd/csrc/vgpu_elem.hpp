@@ -142,6 +142,134 @@ VGPU_HD void haar_inv(const T h[8], T y[8]) {
   y[6] = d6 + d7;
 }
 
+// The butterflies factor into a 2-D (x, y) part per horizontal face and a
+// z stage per element, so a kernel marching up z transforms each node plane
+// once: the top face of one layer is the bottom face of the next, and the
+// two faces' contributions to a plane are added before one inverse.
+// face_fwd: corners in local order (0,0) (1,0) (1,1) (0,1) -> F[bx + 2 by].
+template <typename T>
+VGPU_HD void face_fwd(const T n[4], T F[4]) {
+  const T as = n[0] + n[1], ad = n[1] - n[0];  // y = 0 edge
+  const T bs = n[3] + n[2], bd = n[2] - n[3];  // y = 1 edge
+  F[0] = as + bs;
+  F[1] = ad + bd;
+  F[2] = bs - as;
+  F[3] = bd - ad;
+}
+
+// transposed 2-D butterfly: face Haar values -> contributions to the corners
+template <typename T>
+VGPU_HD void face_inv(const T S[4], T y[4]) {
+  const T d0 = S[0] - S[2], d2 = S[0] + S[2], d1 = S[1] - S[3], d3 = S[1] + S[3];
+  y[0] = d0 - d1;
+  y[1] = d0 + d1;
+  y[3] = d2 - d3;
+  y[2] = d2 + d3;
+}
+
+// z stage: faces -> element Haar xh[b], b = bx + 2 by + 4 bz
+template <typename T>
+VGPU_HD void zfwd(const T fb[4], const T ft[4], T xh[8]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    xh[i] = fb[i] + ft[i];
+    xh[i + 4] = ft[i] - fb[i];
+  }
+}
+
+// transposed z stage: element Haar output -> bottom and top face values
+template <typename T>
+VGPU_HD void zinv(const T yh[8], T ob[4], T ot[4]) {
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    ob[i] = yh[i] - yh[i + 4];
+    ot[i] = yh[i] + yh[i + 4];
+  }
+}
+
+// Transposed element operator in Haar space.  With A_e = H^T B W H (B the
+// Haar-space map haar_apply uses on u = W xh), A_e^T = H^T W B^T H: apply
+// B^T to xh, then the mass weights.
+template <typename T>
+VGPU_HD void haar_apply_tr(const ElemConst<T> &k, const T xh[8], T yh[8]) {
+  T v[8];
+#pragma unroll
+  for (int b = 0; b < 8; ++b) v[b] = k.D[b] * xh[b];
+#pragma unroll
+  for (int ax = 0; ax < 3; ++ax) {
+    const int bit = 1 << ax;
+    const int o1 = (ax == 0) ? 1 : 0, o2 = (ax == 2) ? 1 : 2;
+#pragma unroll
+    for (int pq = 0; pq < 4; ++pq) {
+      const int b0 = ((pq & 1) << o1) | ((pq >> 1) << o2);
+      const int b1 = b0 | bit;
+      v[b0] += k.c[ax] * xh[b1];
+      v[b1] += k.g[ax] * xh[b0];
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    const int kx = (j == 0) ? 1 : 0, lx = (j == 2) ? 1 : 2;
+    const int bk = 1 << kx, bl = 1 << lx, bj = 1 << j;
+#pragma unroll
+    for (int f = 0; f < 2; ++f) {
+      const int base = f ? bj : 0;
+      const int i1 = base | bl, i2 = base | bk;
+      const T u1 = xh[i1], u2 = xh[i2];
+      v[i2] += k.t[j] * u1;
+      v[i1] += k.t[j] * u2;
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < 8; ++b) yh[b] = k.W[b] * v[b];
+}
+
+// The Robin term of robin_add in Haar space, added to the element's Haar
+// output yh (before zinv).  With i = by + 2 bz, the x-min face's 2-D Haar is
+// w0[i] = (xh[2i] - xh[2i+1]) / 2; nodal face contributions (f0, f1) on the
+// x = 0 and x = 1 faces are yh[2i] += (f0 + f1) / 2, yh[2i+1] += (f1 - f0) / 2.
+template <typename T, bool TR>
+VGPU_HD void robin_haar(const ElemConst<T> &k, T ihx, T tau, T vx, T vy, T vz,
+                        T ra, const T xh[8], T yh[8]) {
+  const T rx = tau * vx * ihx, hr = T(0.5) * ra;
+  const T cy = tau * vy * T(0.5), cz = tau * vz * T(0.5);
+  T w0[4], mm[4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    w0[i] = T(0.5) * (xh[2 * i] - xh[2 * i + 1]);
+    mm[i] = k.m[1][i & 1] * k.m[2][i >> 1];
+  }
+  if (!TR) {
+    // f0 = (1 - rx) M w0 + C w0, f1 = rx M w0
+    T cc[4];
+    cc[0] = T(0);
+    cc[1] = cy * k.m[2][0] * w0[0];
+    cc[2] = cz * k.m[1][0] * w0[0];
+    cc[3] = cy * k.m[2][1] * w0[2] + cz * k.m[1][1] * w0[1];
+    const T r2 = T(2) * rx - T(1);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const T t1 = mm[i] * w0[i];
+      yh[2 * i] += hr * (t1 + cc[i]);
+      yh[2 * i + 1] += hr * (r2 * t1 - cc[i]);
+    }
+  } else {
+    // f0 = M w0 + rx M (w1 - w0) + C^T w0, f1 = 0; w1 - w0 = xh[2i+1]
+    T o0[4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) o0[i] = mm[i] * w0[i] + rx * (mm[i] * xh[2 * i + 1]);
+    o0[0] += cy * k.m[2][0] * w0[1];
+    o0[2] += cy * k.m[2][1] * w0[3];
+    o0[0] += cz * k.m[1][0] * w0[2];
+    o0[1] += cz * k.m[1][1] * w0[3];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      yh[2 * i] += hr * o0[i];
+      yh[2 * i + 1] -= hr * o0[i];
+    }
+  }
+}
+
 // Haar-space operator on u = W xh; TR = transpose (the diagonal and the DR
 // cross terms are symmetric on u; the per-axis couplings are not).
 template <typename T, bool TR>

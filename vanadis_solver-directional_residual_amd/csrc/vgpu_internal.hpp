@@ -121,8 +121,10 @@ struct ActiveSet {
 template <int CTRL, int ROWS>
 __device__ __forceinline__ double dpp_f64(double v) {
   const uint64_t b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, CTRL, ROWS, 0xf, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), CTRL, ROWS, 0xf, false);
+  // lanes of rows a broadcast step leaves out keep an unspecified value (the
+  // destination is not initialised); lane 63 never reads them
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, CTRL, ROWS, 0xf, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), CTRL, ROWS, 0xf, false);
   return __longlong_as_double((long long)(((uint64_t)(uint32_t)hi << 32) | (uint32_t)lo));
 }
 
@@ -144,6 +146,9 @@ __device__ __forceinline__ double wave_sum_lane63(double v) {
 constexpr int kUnitPlanes = 1;
 // Owner work units of one system for a tile shape.
 int64_t owner_units(const Grid3 &g, int tile);
+// Whether the owner kernel can take a system: it addresses each vector
+// (8 N bytes in FP64) and the element records (40 E bytes) with 32-bit offsets.
+bool owner_fits(const Grid3 &g);
 // Owner operator over the active systems of a batch (units split evenly over
 // as.nu): for every active member m whose status is not done, y1 = A_m x1 (and
 // y2 = A_m^T x2) on the member's node range of the batch vectors.  dot (dual only, optional):

@@ -18,6 +18,7 @@
 #include "vgpu_internal.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <map>
 #include <mutex>
 
@@ -106,31 +107,6 @@ __device__ __forceinline__ void plane_load(const OpData<T> &op, const TIO *x1,
     node_raw<TIO, DUAL>(op.g, op.mask, x1, x2, gx0 + lx1, gy0 + ly1, gz, r.a1, r.b1, r.m1);
 }
 
-// The same with the nodes' column offsets gx + nax gy precomputed (-1 for a
-// column outside the grid): one multiply-add per node and layer.
-template <typename TIO, bool DUAL>
-__device__ __forceinline__ void node_raw_c(const uint8_t *mask, const TIO *x1,
-                                           const TIO *x2, int64_t col, int64_t plane,
-                                           int gz, int naz, TIO &a, TIO &b, uint8_t &m) {
-  a = b = TIO(0);
-  m = 1;
-  if (col >= 0 && gz >= 0 && gz < naz) {
-    const int64_t n = col + plane * gz;
-    m = mask[n];
-    a = x1[n];
-    if (DUAL) b = x2[n];
-  }
-}
-
-template <typename TIO, bool DUAL>
-__device__ __forceinline__ void plane_load_c(const uint8_t *mask, const TIO *x1,
-                                             const TIO *x2, int64_t col0, int64_t col1,
-                                             int64_t plane, int gz, int naz,
-                                             PlaneRegs<TIO, DUAL> &r) {
-  node_raw_c<TIO, DUAL>(mask, x1, x2, col0, plane, gz, naz, r.a0, r.b0, r.m0);
-  node_raw_c<TIO, DUAL>(mask, x1, x2, col1, plane, gz, naz, r.a1, r.b1, r.m1);
-}
-
 template <typename T, typename TIO, bool DUAL>
 __device__ __forceinline__ void plane_store(const PlaneRegs<TIO, DUAL> &r, T *p1,
                                             T *p2, uint8_t *pm, int q0, int q1,
@@ -176,29 +152,93 @@ __device__ __forceinline__ void elem_eval(const OpData<T> &op, bool valid,
 }
 
 // ---------------------------------------------------------------------------
+// Raw buffer access for the owner kernel.  A load whose offset lies outside
+// [0, bytes) returns zero and such a store is dropped, so a thread whose node,
+// element or owned column lies outside the grid passes kOob as its vector
+// offset instead of branching.  The per-layer part of an address is a scalar
+// offset (uniform over the workgroup, no vector address arithmetic per layer).
+// Every resource must stay below 2 GiB (checked on the host, owner_fits).
+// ---------------------------------------------------------------------------
+using Res = __amdgpu_buffer_rsrc_t;
+constexpr int kOob = int(0x80000000u);
+
+// a value the compiler may not prove uniform (a strength-reduced induction
+// variable, say) but that is: keep it in a scalar register
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ Res make_res(const void *p, int64_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), (short)0, (int)bytes,
+                                           0x00020000);
+}
+
+template <typename V>
+__device__ __forceinline__ V bload(Res r, int v, int s);
+template <>
+__device__ __forceinline__ double bload<double>(Res r, int v, int s) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, v, s, 0));
+}
+template <>
+__device__ __forceinline__ float bload<float>(Res r, int v, int s) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, v, s, 0));
+}
+template <>
+__device__ __forceinline__ uint8_t bload<uint8_t>(Res r, int v, int s) {
+  return __builtin_amdgcn_raw_buffer_load_b8(r, v, s, 0);
+}
+
+__device__ __forceinline__ void bstore(double x, Res r, int v, int s) {
+  __builtin_amdgcn_raw_buffer_store_b64(
+      __builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, x), r, v, s, 0);
+}
+__device__ __forceinline__ void bstore(float x, Res r, int v, int s) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
+}
+
+// element record (vx, vy, vz, lambda, tau) at byte offset v + s
+template <typename T>
+__device__ __forceinline__ void load_rec(Res r, int v, int s, Coef<T> &c) {
+  c.vx = bload<T>(r, v, s);
+  c.vy = bload<T>(r, v + int(sizeof(T)), s);
+  c.vz = bload<T>(r, v + 2 * int(sizeof(T)), s);
+  c.lam = bload<T>(r, v + 3 * int(sizeof(T)), s);
+  c.tau = bload<T>(r, v + 4 * int(sizeof(T)), s);
+}
+
+// ---------------------------------------------------------------------------
 // OWNER kernel (persistent, balanced, batched).  The node columns of each
 // system are cut into overlapping TX x TY element-column tiles that own
 // (TX-1) x (TY-1) node columns each; a work unit is kUnitPlanes consecutive
 // node planes of one tile, units are numbered system by system, tile-major
-// (z fastest), and
-// block b takes the contiguous range [b U / G, (b+1) U / G) of all U units,
-// split into one march per (system, tile) it touches.  Producing planes
-// k0..k1-1 of a tile computes element layers k0-1..k1-1.
+// (z fastest), and block b takes the contiguous range [b U / G, (b+1) U / G)
+// of all U units, split into one march per (system, tile) it touches.
+// Producing planes k0..k1-1 of a tile computes element layers k0-1..k1-1.
 //
-// Per layer: one barrier.  Node planes live in three LDS buffers (plane p in
-// buffer p mod 3), so the buffer written for plane kl+1 was last read two
-// layers ago.  Every thread sums the top-face result of its element column's
-// previous layer with the bottom-face result of the current one in registers
-// and stages the 4 column contributions of plane kl; the owner of a node adds
-// the 4 columns around it one layer later (double-buffered stage).  The next
-// plane and the next layer's fields are loaded one layer ahead.  The dual
-// mode also reduces y1 . x2 over each unit's owned nodes (dot[u]: per thread
-// over the unit's planes in order, then over the block in a fixed order), so
-// the partials do not depend on the grid.
+// Per layer kl (one barrier): the thread of element column (tx, ty) reads the
+// 4 corners of the layer's top face from LDS and transforms them to 2-D Haar
+// space once; the bottom face's transform is the previous layer's top face,
+// kept in registers.  After the z stage and the Haar-space operator, the
+// layer's bottom-face output is added to the previous layer's top-face output
+// (both in Haar space) and one transposed face butterfly gives the 4 column
+// contributions to plane kl, which are staged in LDS; the owner of a node adds
+// the 4 columns around it one layer later.  Plane kl+1 is read only in layer
+// kl and a stage only in the layer after it is written, so two LDS buffers of
+// each suffice, and the march is unrolled by two so that their indices are
+// constants.  The next plane (values, Dirichlet flags) and the next layer's
+// element fields are loaded one layer ahead; the Dirichlet select is applied
+// when the plane is written to LDS.  The dual mode also reduces y1 . x2 over
+// each unit's owned nodes (dot[u]: per thread over the unit's planes in
+// order, then over the block in a fixed order), so the partials do not depend
+// on the grid.
 // ---------------------------------------------------------------------------
+template <typename TIO>
+struct PlanePair {
+  TIO a0, a1, b0, b1;
+  uint8_t m0, m1;
+};
+
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k_owner(
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 : 4)))) void k_owner(
     Member<T> m0, const Member<T> *__restrict__ md, const int *__restrict__ act,
     const int64_t *__restrict__ actu, int nact, int64_t U,
     const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
@@ -207,9 +247,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
   constexpr int NP = (TX + 1) * (TY + 1);
   constexpr int NT = TX * TY;
   constexpr int NW = NT / 64;
-  __shared__ T pl1[3][NP];
-  __shared__ T pl2[DUAL ? 3 : 1][DUAL ? NP : 1];
-  __shared__ uint8_t pm[3][NP];
+  constexpr int SI = sizeof(TIO), SR = 5 * sizeof(T);
+  __shared__ T pl1[2][NP];
+  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? NP : 1];
+  __shared__ uint8_t pm[2][NP];
   __shared__ T st1[2][4 * NT];
   __shared__ T st2[DUAL ? 2 : 1][DUAL ? 4 * NT : 1];
   __shared__ double red[2][NW];
@@ -258,34 +299,76 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
     u += g1 - g0;
     if (st && st[m].done) continue;
     const int k0 = g0 * kUnitPlanes, k1 = min(naz, g1 * kUnitPlanes);
-    const TIO *xm1 = x1 + off, *xm2 = DUAL ? x2 + off : nullptr;
-    TIO *ym1 = y1 + off, *ym2 = DUAL ? y2 + off : nullptr;
+    const int nax = op.g.nax, nay = op.g.nay, ex = op.g.ex, ey = op.g.ey, ez = op.g.ez;
+    const int plane = nax * nay, eplane = ex * ey;
+    const Res rx1 = make_res(x1 + off, op.g.N * SI);
+    const Res rx2 = make_res(DUAL ? x2 + off : x1 + off, op.g.N * SI);
+    const Res ry1 = make_res(y1 + off, op.g.N * SI);
+    const Res ry2 = make_res(DUAL ? y2 + off : y1 + off, op.g.N * SI);
+    const Res rmk = make_res(op.mask, op.g.N);
+    const Res rrc = make_res(op.rec, (int64_t)eplane * ez * SR);
+    const Res rx1z = make_res(x1, 0);  // every access out of range: zeros
     double *dm = want_dot ? dot + ub + (int64_t)tile * ngz : nullptr;
 
     const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
     const int gy0 = (tile / ntx) * (TY - 1) - 1;
     const int exg = gx0 + tx, eyg = gy0 + ty;
-    const bool colok = exg >= 0 && exg < op.g.ex && eyg >= 0 && eyg < op.g.ey;
+    const bool colok = exg >= 0 && exg < ex && eyg >= 0 && eyg < ey;
     ColConst<T> cq;
     col_setup<T>(colok ? op.hx[exg] : T(1), colok ? op.hy[eyg] : T(1),
                  colok ? op.ihx[exg] : T(1), colok ? op.ihy[eyg] : T(1), cq);
+    // outside the grid the fields load as zero; with a zero mass weight every
+    // element constant and so the element's result vanish
+    const T mu_c = colok ? T(op.mu) : T(0);
     const bool robin = ROBIN && op.ra != 0.0 && colok && exg == 0;
     const int ngx = gx0 + 1 + tx, ngy = gy0 + 1 + ty;
-    const bool owng = own && ngx < op.g.nax && ngy < op.g.nay;
-    const int64_t ncol = ngx + (int64_t)op.g.nax * ngy;
-    const int64_t plane = (int64_t)op.g.nax * op.g.nay;
-    const int64_t ecol = exg + (int64_t)op.g.ex * eyg;
-    const int64_t eplane = (int64_t)op.g.ex * op.g.ey;
-    // column offsets of this thread's two plane nodes (-1: outside the grid)
-    int64_t pc0 = -1, pc1 = -1;
-    {
-      const int ax = gx0 + lx0, ay = gy0 + ly0;
-      if (ax >= 0 && ay >= 0 && ax < op.g.nax && ay < op.g.nay)
-        pc0 = ax + (int64_t)op.g.nax * ay;
-      const int bx = gx0 + lx1, by = gy0 + ly1;
-      if (has1 && bx >= 0 && by >= 0 && bx < op.g.nax && by < op.g.nay)
-        pc1 = bx + (int64_t)op.g.nax * by;
-    }
+    const bool owng = own && ngx < nax && ngy < nay;
+    const int vo = owng ? (ngx + nax * ngy) * SI : kOob;  // owned node column
+    const int vr = colok ? (exg + ex * eyg) * SR : kOob;  // element column
+    auto colofs = [&](int ax, int ay) {
+      return (ax >= 0 && ay >= 0 && ax < nax && ay < nay) ? ax + nax * ay : -1;
+    };
+    const int pc0 = colofs(gx0 + lx0, gy0 + ly0);
+    const int pc1 = has1 ? colofs(gx0 + lx1, gy0 + ly1) : -1;
+    const int vp0 = pc0 >= 0 ? pc0 * SI : kOob, vp1 = pc1 >= 0 ? pc1 * SI : kOob;
+    const int vm0 = pc0 >= 0 ? pc0 : kOob, vm1 = pc1 >= 0 ? pc1 : kOob;
+
+    // this thread's two plane nodes of plane gz (raw values and flags; value
+    // 0 outside the grid: a plane above the top loads through resources of
+    // zero size, at a clamped scalar offset)
+    auto plane_fetch = [&](int gz, PlanePair<TIO> &r) {
+      const bool in = gz < naz;
+      const Res ra = in ? rx1 : rx1z, rb = in ? rx2 : rx1z, rk = in ? rmk : rx1z;
+      const int gc = min(gz, naz - 1);
+      const int sm = uni(gc * plane), sv = uni(gc * plane * SI);
+      r.m0 = bload<uint8_t>(rk, vm0, sm);
+      r.a0 = bload<TIO>(ra, vp0, sv);
+      r.b0 = DUAL ? bload<TIO>(rb, vp0, sv) : TIO(0);
+      r.a1 = r.b1 = TIO(0);
+      r.m1 = 1;
+      if (has1) {
+        r.m1 = bload<uint8_t>(rk, vm1, sm);
+        r.a1 = bload<TIO>(ra, vp1, sv);
+        if (DUAL) r.b1 = bload<TIO>(rb, vp1, sv);
+      }
+    };
+    auto plane_put = [&](const PlanePair<TIO> &r, T *p1, T *p2, uint8_t *pk) {
+      p1[q0] = r.m0 ? T(0) : T(r.a0);
+      if (DUAL) p2[q0] = r.m0 ? T(0) : T(r.b0);
+      pk[q0] = r.m0;
+      if (has1) {
+        p1[q1] = r.m1 ? T(0) : T(r.a1);
+        if (DUAL) p2[q1] = r.m1 ? T(0) : T(r.b1);
+        pk[q1] = r.m1;
+      }
+    };
+    // fields of element layer kz (zero outside the grid) and its z size
+    auto coef_fetch = [&](int kz, Coef<T> &c) {
+      const int kc = min(max(kz, 0), ez - 1);
+      c.hz = op.hz[kc];
+      c.ihz = op.ihz[kc];
+      load_rec<T>(kz == kc ? rrc : rx1z, vr, uni(kc * eplane * SR), c);
+    };
 
     // dot terms: each thread sums its owned nodes of the current unit (its
     // planes in order); at the unit's last plane the block sum goes to
@@ -311,132 +394,171 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : 4))) void k
       pend = -1;
     };
 
-    // prologue: plane k0-1 into its buffer, plane k0 and layer k0-1 ahead
-    int kl = k0 - 1;
-    int bb = (kl + 3) % 3, bt = (kl + 4) % 3, bn = (kl + 5) % 3;
-    PlaneRegs<TIO, DUAL> pr;
-    plane_load_c<TIO, DUAL>(op.mask, xm1, xm2, pc0, pc1, plane, kl, naz, pr);
-    plane_store<T, TIO, DUAL>(pr, pl1[bb], DUAL ? pl2[bb] : nullptr, pm[bb], q0, q1, has1);
-    plane_load_c<TIO, DUAL>(op.mask, xm1, xm2, pc0, pc1, plane, kl + 1, naz, pr);
-    const bool ev0 = colok && kl >= 0;
-    Coef<T> cn = load_coef<T>(op, ev0, ecol + eplane * (ev0 ? kl : 0), max(kl, 0));
-
-    T yt1[4], yt2[4];
+    // prologue: the bottom face of layer k0-1 (plane k0-1) straight from
+    // memory, plane k0 and the fields of layer k0-1 ahead
+    T fb1[4], fb2[4];
+    {
+      T n1[4], n2[4];
+      const int gz = k0 - 1;
+      const bool zin = gz >= 0;
 #pragma unroll
-    for (int c = 0; c < 4; ++c) yt1[c] = yt2[c] = T(0);
-    uint8_t om_prev = 1;
-    double xo_prev = 0.0;
-    for (; kl < k1; ++kl) {
-      // plane kl+1 into buffer bt; buffer bt held plane kl-2, last read in
-      // the layer before the previous barrier
-      plane_store<T, TIO, DUAL>(pr, pl1[bt], DUAL ? pl2[bt] : nullptr, pm[bt], q0, q1, has1);
-      const Coef<T> cf = cn;
-      const bool ev = colok && kl >= 0 && kl < op.g.ez;
-      if (kl + 1 < k1) {  // prefetch plane kl+2 and layer kl+1
-        plane_load_c<TIO, DUAL>(op.mask, xm1, xm2, pc0, pc1, plane, kl + 2, naz, pr);
-        const bool evn = colok && kl + 1 < op.g.ez;
-        cn = load_coef<T>(op, evn, ecol + eplane * (evn ? kl + 1 : 0), kl + 1);
+      for (int c = 0; c < 4; ++c) {
+        const int col = colofs(exg + (c == 1 || c == 2), eyg + (c >= 2));
+        const int vc = col >= 0 ? col : kOob, vcs = col >= 0 ? col * SI : kOob;
+        n1[c] = n2[c] = T(0);
+        if (zin) {
+          const int sm = uni(gz * plane), sv = uni(gz * plane * SI);
+          const uint8_t mk = bload<uint8_t>(rmk, vc, sm);
+          const TIO a = bload<TIO>(rx1, vcs, sv);
+          n1[c] = mk ? T(0) : T(a);
+          if (DUAL) {
+            const TIO b = bload<TIO>(rx2, vcs, sv);
+            n2[c] = mk ? T(0) : T(b);
+          }
+        }
       }
-      lds_barrier();
-      if (want_dot) dot_flush();
-      // owner sums of plane kl-1 (staged in the previous layer)
-      const bool emit = kl - 1 >= k0;
+      face_fwd<T>(n1, fb1);
+      if (DUAL) face_fwd<T>(n2, fb2);
+    }
+    PlanePair<TIO> pr;
+    plane_fetch(k0, pr);
+    Coef<T> cn, cf;
+    coef_fetch(k0 - 1, cn);
+
+    T ot1[4], ot2[4];  // Haar-space top-face outputs of the previous layer
+#pragma unroll
+    for (int c = 0; c < 4; ++c) ot1[c] = ot2[c] = T(0);
+    uint8_t omk[2] = {1, 1};    // flag of the owned node of the plane read 2 layers ago
+    double xok[2] = {0.0, 0.0};  // its x2 value (for the dot)
+    int kl = k0 - 1;
+
+    // output of plane kl-1 (summed from the stage written in the previous
+    // layer, buffer Q) with the flag and x2 value read two layers ago
+    auto emit_plane = [&](auto Qc, auto Pc) {
+      constexpr int Q = decltype(Qc)::value, P = decltype(Pc)::value;
       double v1 = 0.0, v2 = 0.0;
-      if (emit && own) {
-        const T *s = st1[(kl - 1) & 1];
+      if (own) {
+        const T *s = st1[Q];
         v1 = (double)s[2 * NT + tid] + (double)s[3 * NT + tid + 1] +
              (double)s[1 * NT + tid + TX] + (double)s[tid + TX + 1];
         if (DUAL) {
-          const T *s2 = st2[DUAL ? ((kl - 1) & 1) : 0];
+          const T *s2 = st2[DUAL ? Q : 0];
           v2 = (double)s2[2 * NT + tid] + (double)s2[3 * NT + tid + 1] +
                (double)s2[1 * NT + tid + TX] + (double)s2[tid + TX + 1];
         }
       }
-      // this layer's corner values from the two LDS planes
-      T xa[8], xc[8];
-      const uint8_t om = pm[bb][oq];
-      const double xo = (DUAL && own) ? (double)pl2[DUAL ? bb : 0][oq] : 0.0;
-      xa[0] = pl1[bb][c00];
-      xa[1] = pl1[bb][c00 + 1];
-      xa[2] = pl1[bb][c00 + TX + 2];
-      xa[3] = pl1[bb][c00 + TX + 1];
-      xa[4] = pl1[bt][c00];
-      xa[5] = pl1[bt][c00 + 1];
-      xa[6] = pl1[bt][c00 + TX + 2];
-      xa[7] = pl1[bt][c00 + TX + 1];
+      const int so = uni((kl - 1) * plane * SI);
+      const TIO o1 = omk[P] ? TIO(0) : TIO(v1);
+      bstore(o1, ry1, vo, so);
+      double d = 0.0;
       if (DUAL) {
-        xc[0] = pl2[DUAL ? bb : 0][c00];
-        xc[1] = pl2[DUAL ? bb : 0][c00 + 1];
-        xc[2] = pl2[DUAL ? bb : 0][c00 + TX + 2];
-        xc[3] = pl2[DUAL ? bb : 0][c00 + TX + 1];
-        xc[4] = pl2[DUAL ? bt : 0][c00];
-        xc[5] = pl2[DUAL ? bt : 0][c00 + 1];
-        xc[6] = pl2[DUAL ? bt : 0][c00 + TX + 2];
-        xc[7] = pl2[DUAL ? bt : 0][c00 + TX + 1];
+        bstore(omk[P] ? TIO(0) : TIO(v2), ry2, vo, so);
+        d = (double)o1 * xok[P];
       }
-      T ya[8], yc[8];
-      elem_eval<T, DUAL, ROBIN>(op, ev, robin, cq, cf, xa, xc, ya, yc);
+      if (want_dot) dot_emit(d, kl - 1);
+    };
+
+    // one element layer kl with plane kl+1 in buffer P
+    auto layer = [&](auto Pc) {
+      constexpr int P = decltype(Pc)::value;
+      plane_put(pr, pl1[P], DUAL ? pl2[P] : nullptr, pm[P]);
+      cf = cn;
+      plane_fetch(kl + 2, pr);  // prefetch plane kl+2 and layer kl+1
+      coef_fetch(kl + 1, cn);
+      lds_barrier();
+      if (want_dot) dot_flush();
+      // top face of layer kl
+      T xt[4], ft1[4], ft2[4];
+      xt[0] = pl1[P][c00];
+      xt[1] = pl1[P][c00 + 1];
+      xt[2] = pl1[P][c00 + TX + 2];
+      xt[3] = pl1[P][c00 + TX + 1];
+      face_fwd<T>(xt, ft1);
+      if (DUAL) {
+        const T *q = pl2[DUAL ? P : 0];
+        xt[0] = q[c00];
+        xt[1] = q[c00 + 1];
+        xt[2] = q[c00 + TX + 2];
+        xt[3] = q[c00 + TX + 1];
+        face_fwd<T>(xt, ft2);
+      }
+      const uint8_t omn = pm[P][oq];
+      const double xon = (DUAL && own) ? (double)pl2[DUAL ? P : 0][oq] : 0.0;
+      // owner sums and output of plane kl-1
+      if (kl - 1 >= k0) emit_plane(std::integral_constant<int, P ^ 1>(), Pc);
+      omk[P] = omn;
+      xok[P] = xon;
+      // element layer kl; a layer outside the grid has zero fields and a
+      // zero mass weight, so its result is zero
+      const bool lv = kl >= 0 && kl < ez;
+      T ob1[4], on1[4], ob2[4], on2[4];
+      {
+        ElemConst<T> k;
+        elem_layer_setup<T>(cq, cf.hz, cf.ihz, cf.lam, cf.vx, cf.vy, cf.vz, cf.tau,
+                            T(op.cH), lv ? mu_c : T(0), k);
+        T xh[8], yh[8];
+        zfwd<T>(fb1, ft1, xh);
+        haar_apply<T, false>(k, xh, yh);
+        if (ROBIN && robin && lv)
+          robin_haar<T, false>(k, cq.ihx, cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), xh, yh);
+        zinv<T>(yh, ob1, on1);
+        if (DUAL) {
+          zfwd<T>(fb2, ft2, xh);
+          haar_apply_tr<T>(k, xh, yh);
+          if (ROBIN && robin && lv)
+            robin_haar<T, true>(k, cq.ihx, cf.tau, cf.vx, cf.vy, cf.vz, T(op.ra), xh, yh);
+          zinv<T>(yh, ob2, on2);
+        } else {
+#pragma unroll
+          for (int c = 0; c < 4; ++c) ob2[c] = on2[c] = T(0);
+        }
+      }
       // column contributions to plane kl
       if (kl >= k0) {
-        T *s = st1[kl & 1];
+        T s[4], y[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) s[c * NT + tid] = yt1[c] + ya[c];
+        for (int c = 0; c < 4; ++c) s[c] = ot1[c] + ob1[c];
+        face_inv<T>(s, y);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) st1[P][c * NT + tid] = y[c];
         if (DUAL) {
-          T *s2 = st2[DUAL ? (kl & 1) : 0];
 #pragma unroll
-          for (int c = 0; c < 4; ++c) s2[c * NT + tid] = yt2[c] + yc[c];
+          for (int c = 0; c < 4; ++c) s[c] = ot2[c] + ob2[c];
+          face_inv<T>(s, y);
+#pragma unroll
+          for (int c = 0; c < 4; ++c) st2[DUAL ? P : 0][c * NT + tid] = y[c];
         }
       }
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        yt1[c] = ya[c + 4];
-        yt2[c] = yc[c + 4];
-      }
-      if (emit) {
-        double d = 0.0;
-        if (owng) {
-          const int64_t n = ncol + plane * (kl - 1);
-          const TIO o1 = om_prev ? TIO(0) : TIO(v1);
-          ym1[n] = o1;
-          if (DUAL) {
-            ym2[n] = om_prev ? TIO(0) : TIO(v2);
-            d = (double)o1 * xo_prev;
-          }
-        }
-        if (want_dot) dot_emit(d, kl - 1);
-      }
-      om_prev = om;
-      xo_prev = xo;
-      const int bfree = bb;
-      bb = bt;
-      bt = bn;
-      bn = bfree;
-    }
-    // epilogue: plane k1-1
-    lds_barrier();
-    if (want_dot) dot_flush();
-    double d = 0.0;
-    if (own) {
-      const T *s = st1[(k1 - 1) & 1];
-      const double v1 = (double)s[2 * NT + tid] + (double)s[3 * NT + tid + 1] +
-                        (double)s[1 * NT + tid + TX] + (double)s[tid + TX + 1];
-      double v2 = 0.0;
-      if (DUAL) {
-        const T *s2 = st2[DUAL ? ((k1 - 1) & 1) : 0];
-        v2 = (double)s2[2 * NT + tid] + (double)s2[3 * NT + tid + 1] +
-             (double)s2[1 * NT + tid + TX] + (double)s2[tid + TX + 1];
-      }
-      if (owng) {
-        const int64_t n = ncol + plane * (k1 - 1);
-        const TIO o1 = om_prev ? TIO(0) : TIO(v1);
-        ym1[n] = o1;
+        fb1[c] = ft1[c];
+        ot1[c] = on1[c];
         if (DUAL) {
-          ym2[n] = om_prev ? TIO(0) : TIO(v2);
-          d = (double)o1 * xo_prev;
+          fb2[c] = ft2[c];
+          ot2[c] = on2[c];
         }
       }
+    };
+
+    // march: layers k0-1 .. k1-1 (at least two), then plane k1-1
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    for (;;) {
+      layer(I0());
+      if (++kl >= k1) {  // plane k1-1 was staged in buffer 0
+        lds_barrier();
+        if (want_dot) dot_flush();
+        emit_plane(I0(), I1());
+        break;
+      }
+      layer(I1());
+      if (++kl >= k1) {
+        lds_barrier();
+        if (want_dot) dot_flush();
+        emit_plane(I1(), I0());
+        break;
+      }
     }
-    if (want_dot) dot_emit(d, k1 - 1);
     lds_barrier();  // buffers are refilled by the next march
     if (want_dot) dot_flush();
   }
@@ -618,6 +740,11 @@ int64_t owner_units(const Grid3 &g, int tile) {
          ((g.naz + kUnitPlanes - 1) / kUnitPlanes);
 }
 
+bool owner_fits(const Grid3 &g) {
+  const int64_t lim = (int64_t)1 << 31;
+  return 8 * g.N < lim && 40 * g.E < lim;
+}
+
 namespace {
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
 const void *owner_kernel() {
@@ -733,6 +860,11 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
   const bool dual = x2 != nullptr;
   const Grid3 &g = op.g;
   if (scatter == VGPU_SCATTER_OWNER) {
+    if (!owner_fits(g)) {
+      set_error("owner scatter: system too large for 32-bit offsets (8 N and 40 E "
+                "must stay below 2^31 bytes)");
+      return VGPU_ERR_ARG;
+    }
     Member<T> m0{};
     m0.op = op;
     m0.seg.off = 0;
