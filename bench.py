@@ -85,12 +85,12 @@ def measured_traffic(tuned):
         return None
     with open(path) as f:
         t = json.load(f)
-    tile = 0 if tuned is None else tuned["tile"]
-    e = t.get("tiles", {}).get(str(tile))
+    key = "0/0" if tuned is None else f"{tuned['tile']}/{tuned['zchunk']}"
+    e = t.get("configs", {}).get(key)
     if not e:
         return None
     return {"bytes_per_launch": e["bytes_per_launch"],
-            "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json tile {tile}"}
+            "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json tile/zchunk {key}"}
 
 
 def config3(pkg):
@@ -206,6 +206,8 @@ def main():
                     help="keep the default owner tile (32x8) instead of the autotuner's")
     ap.add_argument("--tile", type=int, default=-1,
                     help="owner tile shape 0/1/2 (default: autotuner, or 0 with --kernel-only)")
+    ap.add_argument("--zchunk", type=int, default=0,
+                    help="owner z chunk with --tile (0 = balanced launch)")
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 256x256x64 FP64 / mixed-precision solves")
     ap.add_argument("--no-learned", action="store_true",
@@ -227,10 +229,10 @@ def main():
     p = pkg.problem.unit_box(ex, ey, ez, vel_seed=0, lam=1e-3)
     N, E = p.N, p.E
     ctx = pkg.Context.from_problem(p)
-    ctx.set_options("owner", 0, max(a.tile, 0))
+    ctx.set_options("owner", max(a.zchunk, 0), max(a.tile, 0))
     tuned = None
     if a.tile >= 0:
-        tuned = {"tile": a.tile, "zchunk": 0, "ms": None, "source": "--tile"}
+        tuned = {"tile": a.tile, "zchunk": max(a.zchunk, 0), "ms": None, "source": "--tile"}
     elif not a.no_tune and not a.kernel_only:
         tuned = ctx.autotune(pkg.FP64, reps=20)
         tuned["source"] = "autotuner"

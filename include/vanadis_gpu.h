@@ -112,8 +112,13 @@ int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
 /* Options: scatter mode for the solver, z-chunk of the z-marching tiles
  * (-1 = keep; 0 = owner launches size their grid to the device's resident
  * workgroups and split the node planes evenly over them, brick launches use
- * 8; n > 0 = about n node planes per workgroup) and owner tile shape
- * (0 = 32x8, 1 = 16x16, 2 = 64x4 element columns; -1 = keep). */
+ * 8; n > 0 = owner launches give each (chunk of n node planes, tile) group
+ * one workgroup, groups numbered chunk by chunk so that neighbouring tiles
+ * march the same planes together; brick launches use n planes per brick)
+ * and owner tile shape (0 = 32x8, 1 = 16x16, 2 = 64x4 element columns;
+ * -1 = keep).  The owner results do not depend on the z chunk or the tile
+ * shape (bitwise); the owner scatter needs 8 N and 40 E below 2^31 bytes
+ * (else VGPU_ERR_ARG; such systems default to the coloured scatter). */
 int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk, int tile);
 
 /* y = (I-D) A (I-D) x  (transpose=1: A^T), host vectors of length N.
@@ -177,10 +182,10 @@ int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
 
 /* Autotuner (no counterpart in the reference, whose CUDA launches use fixed
  * 256-thread blocks, cudatest.cu:339-347): times the dual apply of `precision`
- * for every owner tile shape and z chunk in {0, 4, 8, 16, 32} (`reps` launches
- * each), keeps the fastest on the context and returns it.  Every owner node is
- * summed by one thread in a fixed order, so the choice does not change any
- * result bit. */
+ * for every owner tile shape and z chunk in {0, 4, 8, 16} and ceil(naz / k),
+ * k = 1..8 (`reps` launches each), keeps the fastest on the context and
+ * returns it.  Every owner node is summed by one thread in a fixed order, so
+ * the choice does not change any result bit. */
 int vgpu_autotune(vgpu_ctx *ctx, int precision, int reps, int *scatter,
                   int *tile, int *zchunk, double *best_ms);
 

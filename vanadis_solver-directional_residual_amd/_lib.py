@@ -217,7 +217,8 @@ class Context:
 
     def set_options(self, scatter="owner", zchunk=-1, tile=-1):
         """Scatter mode; z chunk (-1 keep, 0 balanced over resident workgroups,
-        n > 0 about n node planes per workgroup); owner tile shape (-1 keep)."""
+        n > 0 one workgroup per (chunk of n node planes, tile) group, chunk by
+        chunk); owner tile shape (-1 keep)."""
         check(load().vgpu_set_options(self.h, scatter_code(scatter), int(zchunk), int(tile)),
               "set_options")
 

@@ -2,8 +2,8 @@
 
 vgpu_autotune times the dual apply for every owner tile shape (32x8, 16x16,
 64x4 element columns) and z chunk (0 = balanced over the resident
-workgroups, 4, 8, 16, 32 node planes per workgroup) and keeps the fastest on
-the context.  The cache maps (library version, mesh shape, precision) to the
+workgroups; 4, 8, 16 and naz / k, k = 1..8, node planes per (chunk, tile)
+group with one workgroup each) and keeps the fastest on the context.  The cache maps (library version, mesh shape, precision) to the
 chosen parameters; it is written with sorted keys so the same choices give the
 same file.  Every owner node is summed by one thread in a fixed order, so the
 parameters change timing only: operator results are bitwise the same.

@@ -1114,12 +1114,13 @@ int vgpu_batch_create(int nm, vgpu_ctx *const *ctxs, vgpu_batch **out) {
   bt->versions.assign(nm, 0);
   bt->tile = ctxs[0]->tile;
   bt->zchunk = ctxs[0]->zchunk;
-  std::vector<int64_t> n(nm), nu(nm);
+  std::vector<int64_t> n(nm), nu(nm), ng(nm);
   for (int i = 0; i < nm; ++i) {
     n[i] = ctxs[i]->g.N;
     nu[i] = owner_units(ctxs[i]->g, bt->tile);
+    ng[i] = owner_groups(ctxs[i]->g, bt->tile, bt->zchunk);
   }
-  bt->seg = make_segments(n, nu);
+  bt->seg = make_segments(n, nu, ng);
   bt->ntot = bt->seg.back().off + seg_pad(bt->seg.back().n);
   bt->utot = bt->seg.back().u0 + bt->seg.back().nu;
   auto fail = [&](const char *what, hipError_t e) {
@@ -1424,12 +1425,19 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
     return VGPU_ERR_ARG;
   }
   if (!c->have_coeffs) return VGPU_ERR_STATE;
-  static const int zcs[] = {0, 4, 8, 16, 32};
+  // balanced launch, and z-chunk launches with chunks of naz / k planes (the
+  // chunkings whose groups fill k rounds of workgroups evenly) and a few
+  // fixed sizes
+  std::vector<int> zcs = {0, 4, 8, 16};
+  for (int k = 1; k <= 8; ++k) zcs.push_back((c->g.naz + k - 1) / k);
+  std::sort(zcs.begin(), zcs.end());
+  zcs.erase(std::unique(zcs.begin(), zcs.end()), zcs.end());
   const int save_tile = c->tile, save_zc = c->zchunk, save_sc = c->scatter;
   double best = 1e300;
   int bt = -1, bz = 0;
   for (int t = 0; t < 3; ++t)
     for (int zc : zcs) {
+      if (zc > 256) continue;
       c->tile = t;
       c->zchunk = zc;
       double ms = 0;

@@ -233,9 +233,11 @@ using BSingleOp = std::function<int(const TV *x, TV *y, const ActiveSet &as,
 // boundary of the batch vectors.
 inline int64_t seg_pad(int64_t n) { return (n + 255) / 256 * 256; }
 
-// Segments of nm systems with n[m] nodes and nu[m] owner units.
+// Segments of nm systems with n[m] nodes, nu[m] owner units and ng[m] owner
+// groups (empty: none).
 inline std::vector<Segment> make_segments(const std::vector<int64_t> &n,
-                                          const std::vector<int64_t> &nu) {
+                                          const std::vector<int64_t> &nu,
+                                          const std::vector<int64_t> &ng = {}) {
   std::vector<Segment> s(n.size());
   int64_t off = 0, u0 = 0;
   int vb0 = 0;
@@ -246,6 +248,7 @@ inline std::vector<Segment> make_segments(const std::vector<int64_t> &n,
     s[m].nu = nu[m];
     s[m].vb0 = vb0;
     s[m].nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n[m] + VB - 1) / VB));
+    s[m].ng = ng.empty() ? 0 : ng[m];
     off += seg_pad(n[m]);
     u0 += nu[m];
     vb0 += s[m].nb;
@@ -256,7 +259,7 @@ inline std::vector<Segment> make_segments(const std::vector<int64_t> &n,
 // Device copy of an active set (member list and prefix sums).
 struct ActiveBuf {
   int *act = nullptr, *actvb = nullptr;
-  int64_t *actu = nullptr;
+  int64_t *actu = nullptr, *actg = nullptr;
   ActiveSet view;
   std::vector<int> members;
 
@@ -264,36 +267,42 @@ struct ActiveBuf {
     VGPU_HIP(hipMalloc(&act, nm * sizeof(int)));
     VGPU_HIP(hipMalloc(&actvb, (nm + 1) * sizeof(int)));
     VGPU_HIP(hipMalloc(&actu, (nm + 1) * sizeof(int64_t)));
+    VGPU_HIP(hipMalloc(&actg, (nm + 1) * sizeof(int64_t)));
     return VGPU_OK;
   }
   // members (increasing) of segments s; synchronous upload
   int set(const std::vector<Segment> &s, const std::vector<int> &mem) {
     members = mem;
     std::vector<int> vb(mem.size() + 1, 0);
-    std::vector<int64_t> u(mem.size() + 1, 0);
+    std::vector<int64_t> u(mem.size() + 1, 0), g(mem.size() + 1, 0);
     for (size_t i = 0; i < mem.size(); ++i) {
       vb[i + 1] = vb[i] + s[mem[i]].nb;
       u[i + 1] = u[i] + s[mem[i]].nu;
+      g[i + 1] = g[i] + s[mem[i]].ng;
     }
     if (!mem.empty()) {
       VGPU_HIP(hipMemcpy(act, mem.data(), mem.size() * sizeof(int), hipMemcpyHostToDevice));
       VGPU_HIP(hipMemcpy(actvb, vb.data(), vb.size() * sizeof(int), hipMemcpyHostToDevice));
       VGPU_HIP(hipMemcpy(actu, u.data(), u.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+      VGPU_HIP(hipMemcpy(actg, g.data(), g.size() * sizeof(int64_t), hipMemcpyHostToDevice));
     }
     view.act = act;
     view.actvb = actvb;
     view.actu = actu;
+    view.actg = actg;
     view.nact = (int)mem.size();
     view.nvb = vb.back();
     view.nu = u.back();
+    view.ng = g.back();
     return VGPU_OK;
   }
   void release() {
     if (act) hipFree(act);
     if (actvb) hipFree(actvb);
     if (actu) hipFree(actu);
+    if (actg) hipFree(actg);
     act = actvb = nullptr;
-    actu = nullptr;
+    actu = actg = nullptr;
     view = ActiveSet{};
     members.clear();
   }
@@ -319,7 +328,8 @@ struct BBicg {
     if ((int)s.size() != nm) return false;
     for (int m = 0; m < nm; ++m)
       if (s[m].off != hseg[m].off || s[m].n != hseg[m].n || s[m].u0 != hseg[m].u0 ||
-          s[m].nu != hseg[m].nu || s[m].vb0 != hseg[m].vb0 || s[m].nb != hseg[m].nb)
+          s[m].nu != hseg[m].nu || s[m].vb0 != hseg[m].vb0 || s[m].nb != hseg[m].nb ||
+          s[m].ng != hseg[m].ng)
         return false;
     return true;
   }

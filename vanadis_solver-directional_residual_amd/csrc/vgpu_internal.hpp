@@ -90,6 +90,7 @@ struct Segment {
   int64_t off, n;  // nodes [off, off + n) of the batch vectors
   int64_t u0, nu;  // owner units [u0, u0 + nu) (dot partial slots)
   int vb0, nb;     // vector workgroups [vb0, vb0 + nb) (= partial slots)
+  int64_t ng;      // owner groups of the system (z-chunk launches), else 0
 };
 
 // Operator data of one batch member for the owner kernel (device array).
@@ -108,9 +109,11 @@ struct ActiveSet {
   const int *act = nullptr;
   const int *actvb = nullptr;
   const int64_t *actu = nullptr;
+  const int64_t *actg = nullptr;  // prefix sums of the members' owner groups
   int nact = 0;
   int nvb = 0;     // actvb[nact]
   int64_t nu = 0;  // actu[nact]
+  int64_t ng = 0;  // actg[nact]
 };
 
 // Sum of v over the 64 lanes of a wave, in a fixed order, valid in lane 63:
@@ -146,6 +149,10 @@ __device__ __forceinline__ double wave_sum_lane63(double v) {
 constexpr int kUnitPlanes = 1;
 // Owner work units of one system for a tile shape.
 int64_t owner_units(const Grid3 &g, int tile);
+// Owner groups of one system for a tile shape and z chunk (> 0): the
+// (z chunk, tile) pairs a z-chunk launch gives one workgroup each; 0 if
+// zchunk <= 0.
+int64_t owner_groups(const Grid3 &g, int tile, int zchunk);
 // Whether the owner kernel can take a system: it addresses each vector
 // (8 N bytes in FP64) and the element records (40 E bytes) with 32-bit offsets.
 bool owner_fits(const Grid3 &g);

@@ -240,8 +240,8 @@ template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, i
 __global__ __launch_bounds__(TX * TY)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 : 4)))) void k_owner(
     Member<T> m0, const Member<T> *__restrict__ md, const int *__restrict__ act,
-    const int64_t *__restrict__ actu, int nact, int64_t U,
-    const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
+    const int64_t *__restrict__ actu, const int64_t *__restrict__ actg, int nact,
+    int64_t U, int zc, const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
     TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
     double *__restrict__ dot) {
   constexpr int NP = (TX + 1) * (TY + 1);
@@ -258,8 +258,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int64_t G = gridDim.x, lb = xcd_tile(blockIdx.x, gridDim.x);
-  int64_t u = U * lb / G;
-  const int64_t uend = U * (lb + 1) / G;
   const bool want_dot = DUAL && dot != nullptr;
 
   const int q0 = tid, q1 = tid + NT;
@@ -271,34 +269,28 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
   const int c00 = ty * (TX + 1) + tx;  // tile node of local corner 0
 
   // BATCH: active members act[i], their units at [actu[i], actu[i+1]) of
-  // the split and at md[act[i]].seg.u0 of the partials; else the member m0
-  int ia = 0;
-  if (BATCH) {
+  // the split, their groups at [actg[i], actg[i+1]) and their partials at
+  // md[act[i]].seg.u0; else the member m0.  Binary search: the last i with
+  // a[i] <= v.
+  auto find = [&](const int64_t *a, int64_t v) {
     int lo = 0, hi = nact - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
-      if (actu[mid] <= u) lo = mid; else hi = mid - 1;
+      if (a[mid] <= v) lo = mid; else hi = mid - 1;
     }
-    ia = lo;
-  }
+    return lo;
+  };
 
-  while (u < uend) {
-    if (BATCH)
-      while (u >= actu[ia + 1]) ++ia;
-    const int m = BATCH ? act[ia] : 0;
+  // One march of member m: node planes k0..k1-1 of tile `tile` (element
+  // layers k0-1..k1-1).  Every thread of the block takes the same path.
+  auto march = [&](int m, int tile, int k0, int k1) {
+    if (st && st[m].done) return;
     const OpData<T> op = BATCH ? md[m].op : m0.op;
     const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
     const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
     const int naz = op.g.naz;
     const int ngz = (naz + kUnitPlanes - 1) / kUnitPlanes;  // units per tile
     const int ntx = (op.g.nax + TX - 2) / (TX - 1);
-    const int64_t lu = u - (BATCH ? actu[ia] : (int64_t)0);
-    const int tile = (int)(lu / ngz);
-    const int g0 = (int)(lu - (int64_t)tile * ngz);
-    const int g1 = (int)min<int64_t>((int64_t)ngz, g0 + (uend - u));
-    u += g1 - g0;
-    if (st && st[m].done) continue;
-    const int k0 = g0 * kUnitPlanes, k1 = min(naz, g1 * kUnitPlanes);
     const int nax = op.g.nax, nay = op.g.nay, ex = op.g.ex, ey = op.g.ey, ez = op.g.ez;
     const int plane = nax * nay, eplane = ex * ey;
     const Res rx1 = make_res(x1 + off, op.g.N * SI);
@@ -561,6 +553,40 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     }
     lds_barrier();  // buffers are refilled by the next march
     if (want_dot) dot_flush();
+  };
+
+  if (zc > 0) {
+    // z-chunk launch: workgroup lb takes group lb, the (chunk, tile) groups
+    // numbered chunk-major per system, so the workgroups of one XCD march
+    // neighbouring tiles over the same planes at the same time and share
+    // their halo in L2
+    const int ia = BATCH ? find(actg, lb) : 0;
+    const int m = BATCH ? act[ia] : 0;
+    const Grid3 g = BATCH ? md[m].op.g : m0.op.g;
+    const int nt = ((g.nax + TX - 2) / (TX - 1)) * ((g.nay + TY - 2) / (TY - 1));
+    const int64_t lg = lb - (BATCH ? actg[ia] : (int64_t)0);
+    const int chunk = (int)(lg / nt), tile = (int)(lg - (int64_t)chunk * nt);
+    const int k0 = chunk * zc;
+    if (k0 < g.naz) march(m, tile, k0, min(g.naz, k0 + zc));
+    return;
+  }
+  // balanced launch: block lb takes the units [lb U / G, (lb+1) U / G),
+  // split into one march per (system, tile) it touches
+  int64_t u = U * lb / G;
+  const int64_t uend = U * (lb + 1) / G;
+  int ia = BATCH ? find(actu, u) : 0;
+  while (u < uend) {
+    if (BATCH)
+      while (u >= actu[ia + 1]) ++ia;
+    const int m = BATCH ? act[ia] : 0;
+    const int naz = BATCH ? md[m].op.g.naz : m0.op.g.naz;
+    const int ngz = (naz + kUnitPlanes - 1) / kUnitPlanes;
+    const int64_t lu = u - (BATCH ? actu[ia] : (int64_t)0);
+    const int tile = (int)(lu / ngz);
+    const int g0 = (int)(lu - (int64_t)tile * ngz);
+    const int g1 = (int)min<int64_t>((int64_t)ngz, g0 + (uend - u));
+    u += g1 - g0;
+    march(m, tile, g0 * kUnitPlanes, min(naz, g1 * kUnitPlanes));
   }
 }
 
@@ -740,6 +766,14 @@ int64_t owner_units(const Grid3 &g, int tile) {
          ((g.naz + kUnitPlanes - 1) / kUnitPlanes);
 }
 
+int64_t owner_groups(const Grid3 &g, int tile, int zchunk) {
+  if (zchunk <= 0) return 0;
+  int tx, ty;
+  owner_shape(tile, &tx, &ty);
+  return (int64_t)((g.nax + tx - 2) / (tx - 1)) * ((g.nay + ty - 2) / (ty - 1)) *
+         ((g.naz + zchunk - 1) / zchunk);
+}
+
 bool owner_fits(const Grid3 &g) {
   const int64_t lim = (int64_t)1 << 31;
   return 8 * g.N < lim && 40 * g.E < lim;
@@ -769,14 +803,11 @@ int resident_blocks(const void *fn, int threads) {
   return r;
 }
 
-// Grid of an owner launch over utot units: zchunk > 0 gives about zchunk
-// node planes (zchunk / kUnitPlanes units, at least one) per workgroup,
-// zchunk = 0 one workgroup per resident slot.
-int owner_grid(const void *fn, int threads, int64_t utot, int zchunk) {
-  if (zchunk > 0) {
-    const int64_t upb = std::max(1, zchunk / kUnitPlanes);
-    return (int)std::max<int64_t>(1, (utot + upb - 1) / upb);
-  }
+// Grid of an owner launch: zchunk > 0 one workgroup per (z chunk, tile)
+// group (gtot of them), zchunk = 0 one workgroup per resident slot over utot
+// units.
+int owner_grid(const void *fn, int threads, int64_t utot, int64_t gtot, int zchunk) {
+  if (zchunk > 0) return (int)std::max<int64_t>(1, gtot);
   int r = resident_blocks(fn, threads);
   if (r <= 0) r = 1024;
   return (int)std::max<int64_t>(1, std::min<int64_t>(r, utot));
@@ -784,48 +815,50 @@ int owner_grid(const void *fn, int threads, int64_t utot, int zchunk) {
 
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
 void launch_owner_k(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
-                    int64_t utot, int zchunk, const SolveStatus *st, const TIO *x1,
-                    TIO *y1, const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
+                    int64_t utot, int64_t gtot, int zchunk, const SolveStatus *st,
+                    const TIO *x1, TIO *y1, const TIO *x2, TIO *y2, double *dot,
+                    hipStream_t s) {
   const void *fn = owner_kernel<T, TIO, DUAL, ROBIN, BATCH, TX, TY>();
-  const int nb = owner_grid(fn, TX * TY, utot, zchunk);
+  const int nb = owner_grid(fn, TX * TY, utot, gtot, zchunk);
   hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, TX, TY>), dim3(nb),
-                     dim3(TX * TY), 0, s, m0, md, as.act, as.actu, as.nact, utot, st, x1,
-                     y1, x2, y2, dot);
+                     dim3(TX * TY), 0, s, m0, md, as.act, as.actu, as.actg, as.nact, utot,
+                     zchunk > 0 ? zchunk : 0, st, x1, y1, x2, y2, dot);
 }
 
 template <typename T, typename TIO, bool BATCH, int TX, int TY>
 void launch_owner_shape(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
-                        int64_t utot, int zchunk, bool robin,
+                        int64_t utot, int64_t gtot, int zchunk, bool robin,
                         const SolveStatus *st, const TIO *x1, TIO *y1,
                         const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
   if (x2) {
     if (robin)
-      launch_owner_k<T, TIO, true, true, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, dot, s);
+      launch_owner_k<T, TIO, true, true, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, dot, s);
     else
-      launch_owner_k<T, TIO, true, false, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, dot, s);
+      launch_owner_k<T, TIO, true, false, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, dot, s);
   } else {
     if (robin)
-      launch_owner_k<T, TIO, false, true, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
+      launch_owner_k<T, TIO, false, true, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, nullptr, s);
     else
-      launch_owner_k<T, TIO, false, false, BATCH, TX, TY>(m0, md, as, utot, zchunk, st, x1, y1, x2, y2, nullptr, s);
+      launch_owner_k<T, TIO, false, false, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, nullptr, s);
   }
 }
 
+// utot units (balanced launch) or gtot groups (z-chunk launch, zchunk > 0)
 template <typename T, typename TIO, bool BATCH>
 int launch_owner_any(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
-                     int64_t utot, int tile, int zchunk, bool robin,
+                     int64_t utot, int64_t gtot, int tile, int zchunk, bool robin,
                      const SolveStatus *st, const TIO *x1, TIO *y1,
                      const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
-  if (utot <= 0) return VGPU_OK;
+  if (zchunk > 0 ? gtot <= 0 : utot <= 0) return VGPU_OK;
   switch (tile) {
     case 1:
-      launch_owner_shape<T, TIO, BATCH, 16, 16>(m0, md, as, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, 16, 16>(m0, md, as, utot, gtot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
       break;
     case 2:
-      launch_owner_shape<T, TIO, BATCH, 64, 4>(m0, md, as, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, 64, 4>(m0, md, as, utot, gtot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
       break;
     default:
-      launch_owner_shape<T, TIO, BATCH, 32, 8>(m0, md, as, utot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, 32, 8>(m0, md, as, utot, gtot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
       break;
   }
   VGPU_HIP(hipGetLastError());
@@ -839,8 +872,8 @@ int launch_owner_batch(const Member<T> *md, const ActiveSet &as, int tile,
                        const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
                        double *dot, hipStream_t s) {
   Member<T> none{};
-  return launch_owner_any<T, TIO, true>(none, md, as, as.nu, tile, zchunk, robin, st,
-                                        x1, y1, x2, y2, dot, s);
+  return launch_owner_any<T, TIO, true>(none, md, as, as.nu, as.ng, tile, zchunk, robin,
+                                        st, x1, y1, x2, y2, dot, s);
 }
 
 template int launch_owner_batch<double, double>(const Member<double> *, const ActiveSet &,
@@ -872,8 +905,10 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
     m0.seg.u0 = 0;
     m0.seg.nu = owner_units(g, op.tile);
     const SolveStatus *st = reinterpret_cast<const SolveStatus *>(op.done);
-    return launch_owner_any<T, TIO, false>(m0, nullptr, ActiveSet{}, m0.seg.nu, op.tile, zc,
-                                    op.ra != 0.0, st, x1, y1, x2, y2, dot_part, s);
+    m0.seg.ng = owner_groups(g, op.tile, zc);
+    return launch_owner_any<T, TIO, false>(m0, nullptr, ActiveSet{}, m0.seg.nu, m0.seg.ng,
+                                           op.tile, zc, op.ra != 0.0, st, x1, y1, x2, y2,
+                                           dot_part, s);
   }
   // brick modes need zeroed outputs; bricks are 32 x 8 element columns
   constexpr int BX = 32, BY = 8;
