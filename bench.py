@@ -10,8 +10,9 @@ A step is one plain-FP64 Jacobi-BiCG solve to rtol 1e-8 from x0 = 0
 one FP64 matrix-free apply, owner-computes scatter), measured with HIP events
 on the library's stream.  The line also carries time-to-rtol of FP64 and
 mixed precision, the other scatter modes, the roofline of the dominant kernel
-(the dual apply A p, A^T pp of every BiCG iteration) and the reference's CPU
-operator timed on the host.
+(the owner launch of every BiCG iteration: direction update p = z + beta p,
+pp = zz + beta pp fused into the dual apply A p, A^T pp), that of the plain
+dual apply, and the reference's CPU operator timed on the host.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel-only] [--no-cpu]
 """
@@ -36,12 +37,14 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def algorithmic_bytes(N, E, dual, fp32=False):
+def algorithmic_bytes(N, E, dual, fp32=False, fused=False):
     """Compulsory bytes of one matrix-free apply with this library's layout:
     input/output vectors, the uint8 Dirichlet mask and five per-element
-    fields (vx, vy, vz, lambda, tau); axis coordinates are O(n^(1/3))."""
+    fields (vx, vy, vz, lambda, tau); axis coordinates are O(n^(1/3)).
+    fused: the iteration launch reads z, zz and the previous directions and
+    writes the new directions and A p, A^T pp (8 vectors)."""
     w = 4 if fp32 else 8
-    vec = (4 if dual else 2) * w * N
+    vec = (8 if fused else 4 if dual else 2) * w * N
     return vec + N + 5 * w * E
 
 
@@ -86,11 +89,12 @@ def measured_traffic(tuned):
     with open(path) as f:
         t = json.load(f)
     key = "0/0" if tuned is None else f"{tuned['tile']}/{tuned['zchunk']}"
-    e = t.get("configs", {}).get(key)
+    e = t.get("iteration_launch", {}).get(key)
     if not e:
         return None
     return {"bytes_per_launch": e["bytes_per_launch"],
-            "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json tile/zchunk {key}"}
+            "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json iteration_launch "
+                      f"tile/zchunk {key}"}
 
 
 def config3(pkg):
@@ -237,7 +241,7 @@ def main():
         tuned = ctx.autotune(pkg.FP64, reps=20)
         tuned["source"] = "autotuner"
         progress(f"autotuner: tile {tuned['tile']} zchunk {tuned['zchunk']} "
-                 f"({tuned['ms'] * 1e3:.1f} us dual apply)")
+                 f"({tuned['ms'] * 1e3:.1f} us iteration launch)")
     b = ctx.assemble_rhs(p.q_in)
     b_dev = ctx.dev_alloc(8 * N)
     x_dev = ctx.dev_alloc(8 * N)
@@ -247,6 +251,7 @@ def main():
     if a.kernel_only:
         for _ in range(3):
             ctx.time_kernel(1, "owner", pkg.FP64, reps=5)
+            ctx.time_kernel(3, "owner", pkg.FP64, reps=5)
             ctx.time_kernel(2, "owner", pkg.FP64, reps=5)
         progress("kernel-only done")
         print(json.dumps({"kernel_only": True, "N": N, "E": E}))
@@ -298,6 +303,7 @@ def main():
     reps = 50
     t_single = ctx.time_kernel(0, "owner", pkg.FP64, reps)
     t_dual = ctx.time_kernel(1, "owner", pkg.FP64, reps)
+    t_iter = ctx.time_kernel(3, "owner", pkg.FP64, reps)
     t_copy = ctx.time_kernel(2, "owner", pkg.FP64, reps)
     modes = {}
     for sc in ("owner", "colored", "atomic"):
@@ -306,9 +312,12 @@ def main():
             modes[f"{sc}_{pn}"] = {"ms": round(ms, 5), "GDOF/s": round(N / ms / 1e6, 2)}
     gdofs = N / t_single / 1e6
     dual_bytes = algorithmic_bytes(N, E, dual=True)
-    achieved = dual_bytes / (t_dual * 1e-3) / 1e9
+    dual_gbs = dual_bytes / (t_dual * 1e-3) / 1e9
+    iter_bytes = algorithmic_bytes(N, E, dual=True, fused=True)
+    achieved = iter_bytes / (t_iter * 1e-3) / 1e9
     copy_gbs = 16.0 * N / (t_copy * 1e-3) / 1e9
     progress(f"apply {t_single * 1e3:.1f} us ({gdofs:.1f} GDOF/s), dual {t_dual * 1e3:.1f} us "
+             f"({dual_gbs:.0f} GB/s), iteration launch {t_iter * 1e3:.1f} us "
              f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
 
     batch = None
@@ -357,13 +366,20 @@ def main():
         "mixed_status": mixed_status,
         "mixed_rel_l2_vs_fp64": rel_l2_mixed, "mixed_true_rel_residual": true_res_mixed,
         "wall_s_timed_steps": round(wall, 3),
-        "apply_ms": {"single_fp64": round(t_single, 5), "dual_fp64": round(t_dual, 5)},
+        "apply_ms": {"single_fp64": round(t_single, 5), "dual_fp64": round(t_dual, 5),
+                     "iteration_launch_fp64": round(t_iter, 5)},
+        "ms_per_iteration_fp64": round(ms_step / iters[-1], 5),
         "scatter_modes": modes,
         "copy_GBps": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None, "kernel": "k_owner<double,double,dual>",
-                     "algorithmic_bytes_per_launch": dual_bytes},
+                     "traffic": None, "kernel": "k_owner<double,double,dual,direction>",
+                     "algorithmic_bytes_per_launch": iter_bytes},
+        "roofline_dual_apply": {"bound": "hbm", "achieved": round(dual_gbs, 1),
+                                "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                "frac": round(dual_gbs / HBM_PEAK_GBS, 4),
+                                "kernel": "k_owner<double,double,dual>",
+                                "algorithmic_bytes_per_launch": dual_bytes},
         "cpu_baseline": cpu,
         "autotune": tuned,
         "batched": batch,

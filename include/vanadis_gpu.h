@@ -175,13 +175,17 @@ int vgpu_memcpy_d2h(vgpu_ctx *ctx, void *dst, const void *src_dev, int64_t bytes
 
 /* Timing of the dominant kernel with HIP events on the context's stream:
  * `reps` back-to-back operator launches; avg_ms per launch.
- * what: 0 = single apply (A x), 1 = dual apply (A p and A^T pp, as in BiCG),
- *       2 = device copy of N doubles (bandwidth calibration). */
+ * what: 0 = single apply (A x), 1 = dual apply (A p and A^T pp),
+ *       2 = device copy of N doubles (bandwidth calibration),
+ *       3 = the owner launch of a BiCG iteration of the solver: convergence
+ *           test, beta and p = z + beta p, pp = zz + beta pp fused into the
+ *           dual apply (owner scatter only). */
 int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
                      int reps, double *avg_ms);
 
 /* Autotuner (no counterpart in the reference, whose CUDA launches use fixed
- * 256-thread blocks, cudatest.cu:339-347): times the dual apply of `precision`
+ * 256-thread blocks, cudatest.cu:339-347): times the solver's owner launch
+ * (vgpu_time_kernel what = 3) of `precision`
  * for every owner tile shape and z chunk in {0, 4, 8, 16} and ceil(naz / k),
  * k = 1..8 (`reps` launches each), keeps the fastest on the context and
  * returns it.  Every owner node is summed by one thread in a fixed order, so

@@ -250,12 +250,14 @@ int solve_owner(vgpu_ctx *c, BBicg<TV> &eng, const TV *b, TV *x, const TV *diag,
                 double divtol = 1e300) {
   int rc = eng.alloc(make_segments({c->g.N}, {owner_units(c->g, c->tile)}));
   if (rc) return rc;
-  BDualOp<TV> dual = [c](const TV *p, TV *z, const TV *pp, TV *zz, double *pa,
-                         const SolveStatus *st, const ActiveSet &, hipStream_t s) {
-    OpData<TV> op = op_for<TV>(c);
-    op.done = reinterpret_cast<const int *>(st);
-    return launch_apply<TV, TV>(op, VGPU_SCATTER_OWNER, c->zchunk, p, z, pp, zz,
-                                pa, s);
+  const Segment sg = eng.hseg[0];
+  BDirOp<TV> dual = [c, sg](const TV *z, const TV *zz, const DirData<TV> &dd, TV *w,
+                            TV *ww, double *pa, const ActiveSet &, hipStream_t s) {
+    Member<TV> m0{};
+    m0.op = op_for<TV>(c);
+    m0.seg = sg;
+    return launch_owner_dir<TV, TV>(m0, nullptr, ActiveSet{}, false, c->tile, c->zchunk,
+                                    m0.op.ra != 0.0, dd, z, zz, w, ww, pa, s);
   };
   BSingleOp<TV> single = [c](const TV *xx, TV *y, const ActiveSet &, hipStream_t s) {
     OpData<TV> op = op_for<TV>(c);
@@ -625,7 +627,7 @@ int vgpu_destroy(vgpu_ctx *c) {
     free_dev(c->faih[d]);
   }
   for (double **p : {&c->rec, &c->diag, &c->vb, &c->vx_, &c->vt, &c->vt2, &c->red,
-                     &c->tpart})
+                     &c->tpart, &c->tdir})
     free_dev(*p);
   for (float **p : {&c->frec, &c->fdiag,
                     &c->fb, &c->fx})
@@ -1071,11 +1073,10 @@ struct vgpu_batch {
                int *status) {
     int rc = eng.alloc(seg);
     if (rc) return rc;
-    BDualOp<TV> dual = [this, md](const TV *p, TV *z, const TV *pp, TV *zz,
-                                  double *pa, const SolveStatus *st,
-                                  const ActiveSet &as, hipStream_t s) {
-      return launch_owner_batch<TV, TV>(md, as, tile, zchunk, robin, st, p, z, pp, zz,
-                                        pa, s);
+    BDirOp<TV> dual = [this, md](const TV *z, const TV *zz, const DirData<TV> &dd, TV *w,
+                                 TV *ww, double *pa, const ActiveSet &as, hipStream_t s) {
+      return launch_owner_dir<TV, TV>(Member<TV>{}, md, as, true, tile, zchunk, robin, dd,
+                                      z, zz, w, ww, pa, s);
     };
     BSingleOp<TV> single = [this, md](const TV *xi, TV *y, const ActiveSet &as,
                                       hipStream_t s) {
@@ -1441,7 +1442,7 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
       c->tile = t;
       c->zchunk = zc;
       double ms = 0;
-      const int rc = vgpu_time_kernel(c, 1, VGPU_SCATTER_OWNER, precision, reps, &ms);
+      const int rc = vgpu_time_kernel(c, 3, VGPU_SCATTER_OWNER, precision, reps, &ms);
       if (rc == VGPU_ERR_ARG) continue;  // more partials than the timing buffer
       if (rc) {
         c->tile = save_tile;
@@ -1474,8 +1475,9 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
 
 int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                      int reps, double *avg_ms) {
-  if (!c || !avg_ms || reps < 1 || what < 0 || what > 2) return VGPU_ERR_ARG;
+  if (!c || !avg_ms || reps < 1 || what < 0 || what > 3) return VGPU_ERR_ARG;
   if (what != 2 && !c->have_coeffs) return VGPU_ERR_STATE;
+  if (what == 3 && scatter != VGPU_SCATTER_OWNER) return VGPU_ERR_ARG;
   const int64_t N = c->g.N;
   VGPU_HIP(hipSetDevice(c->device));
   // deterministic non-trivial inputs
@@ -1492,7 +1494,67 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     f3 = reinterpret_cast<float *>(c->vt);
     f4 = reinterpret_cast<float *>(c->vt2);
   }
+  // fused iteration launch: z, zz = vx_, vb; previous directions vt, vt2;
+  // new directions and operator outputs in tdir; a status that never stops
+  // (tol 0) and unit partials, so beta = 1 and every system runs
+  const Segment sg = make_segments({N}, {owner_units(c->g, c->tile)})[0];
+  const int64_t tail = 2048;
+  if (what == 3) {
+    if (!c->tdir) VGPU_HIP(hipMalloc(&c->tdir, (4 * N + tail) * sizeof(double)));
+    double *parts = c->tdir + 4 * N;
+    hipLaunchKernelGGL(k_fill, grid_for(tail), dim3(256), 0, c->stream, tail, 1.0, parts);
+    SolveStatus s0{};
+    s0.iter = 1;
+    VGPU_HIP(hipMemcpyAsync(parts, &s0, sizeof(SolveStatus), hipMemcpyHostToDevice,
+                            c->stream));
+    VGPU_HIP(hipStreamSynchronize(c->stream));
+  }
+  auto launch_dir = [&]() -> int {
+    double *parts = c->tdir + 4 * N;
+    const int ns = (int)((sizeof(SolveStatus) + 7) / 8);
+    if (precision == VGPU_FP32) {
+      float *v = reinterpret_cast<float *>(c->tdir);
+      DirData<float> dd;
+      dd.po = f3;
+      dd.ppo = f4;
+      dd.pn = v;
+      dd.ppn = v + N;
+      dd.pr = dd.pb = dd.pbn = parts + ns;
+      dd.stv = reinterpret_cast<SolveStatus *>(parts);
+      dd.tol = 0.0;
+      dd.divtol = 1e300;
+      dd.itmax = 1 << 30;
+      dd.NB = sg.nb;
+      dd.q = 0;
+      Member<float> m0{};
+      m0.op = c->op32();
+      m0.seg = sg;
+      return launch_owner_dir<float, float>(m0, nullptr, ActiveSet{}, false, c->tile,
+                                            c->zchunk, m0.op.ra != 0.0, dd, f1, f2,
+                                            v + 2 * N, v + 3 * N, c->tpart, c->stream);
+    }
+    DirData<double> dd;
+    dd.po = c->vt;
+    dd.ppo = c->vt2;
+    dd.pn = c->tdir;
+    dd.ppn = c->tdir + N;
+    dd.pr = dd.pb = dd.pbn = parts + ns;
+    dd.stv = reinterpret_cast<SolveStatus *>(parts);
+    dd.tol = 0.0;
+    dd.divtol = 1e300;
+    dd.itmax = 1 << 30;
+    dd.NB = sg.nb;
+    dd.q = 0;
+    Member<double> m0{};
+    m0.op = c->op64(c->cH, c->cC);
+    m0.seg = sg;
+    return launch_owner_dir<double, double>(m0, nullptr, ActiveSet{}, false, c->tile,
+                                            c->zchunk, m0.op.ra != 0.0, dd, c->vx_, c->vb,
+                                            c->tdir + 2 * N, c->tdir + 3 * N, c->tpart,
+                                            c->stream);
+  };
   auto launch = [&]() -> int {
+    if (what == 3) return launch_dir();
     if (what == 2) {
       hipLaunchKernelGGL(k_copy, grid_for(N, 256), dim3(256), 0, c->stream, N,
                          c->vx_, c->vb);

@@ -12,10 +12,11 @@
 // summed in the same order whatever else is in the batch: a member's iterates
 // are bitwise those of its solve alone (same tile shape).
 //
-// One iteration = K1 (direction), K2 (one persistent owner launch over all
-// systems), K3 (update); pairs of iterations are captured in one hipGraph and
-// replayed until every system reports done (converged, itmax, breakdown or
-// divergence).  Kernels skip the systems that are done.
+// One iteration = K1 (one owner launch over all systems: convergence test,
+// beta and the new directions fused into the dual operator) and K2 (update);
+// pairs of iterations are captured in one hipGraph and replayed until every
+// system reports done (converged, itmax, breakdown or divergence).  Kernels
+// skip the systems that are done.
 #pragma once
 #include <algorithm>
 #include <functional>
@@ -90,86 +91,17 @@ __global__ __launch_bounds__(VB) void kb_init(const Segment *seg, VGPU_ACT_ARGS,
   block_partial_at(sn, pbn, sg.vb0 + j, red);
 }
 
-// K1: convergence test of iteration k-1, then p = z + beta p, pp = zz + beta pp
-// (iteration 1 copies, cudatest.cu:504-505).  q = k & 1; NB = all workgroups.
-template <typename TV>
-__global__ __launch_bounds__(VB) void kb_dir(const Segment *seg, VGPU_ACT_ARGS, int NB,
-                                             int q, const double *pr,
-                                             const double *pb, const double *pbn,
-                                             SolveStatus *stv, double tol,
-                                             int itmax, double divtol, TV *p,
-                                             TV *pp, const TV *z,
-                                             const TV *zz) {
-  __shared__ double red[VB / 64];
-  const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
-  SolveStatus *st = stv + bo.m;
-  if (st->done) return;
-  const Segment sg = seg[bo.m];
-  const int j = bo.j;
-  const int k = st->iter + 1;
-  const int64_t stride = (int64_t)sg.nb * VB;
-  const int64_t i0 = j * (int64_t)VB + threadIdx.x;
-  TV zr[PF], zzr[PF], pr0[PF], ppr0[PF];
-#pragma unroll
-  for (int t = 0; t < PF; ++t) {
-    const int64_t li = i0 + t * stride;
-    const bool ok = li < sg.n;
-    const int64_t i = sg.off + (ok ? li : 0);
-    zr[t] = ok ? z[i] : TV(0);
-    zzr[t] = ok ? zz[i] : TV(0);
-    pr0[t] = ok ? p[i] : TV(0);
-    ppr0[t] = ok ? pp[i] : TV(0);
-  }
-  const double rn2 = block_sum(pr + (q ^ 1) * NB + sg.vb0, sg.nb, red);
-  const double bn2 = block_sum(pbn + sg.vb0, sg.nb, red);
-  const double err = sqrt(rn2 / bn2);
-  int done = 0;
-  if (err <= tol) done = 1;
-  else if (k > itmax) done = 2;
-  else if (!(err == err)) done = 3;
-  else if (err > divtol) done = 4;
-  double beta = 0.0;
-  if (!done && k > 1) {
-    const double bknum = block_sum(pb + (q ^ 1) * NB + sg.vb0, sg.nb, red);
-    const double bkden = block_sum(pb + q * NB + sg.vb0, sg.nb, red);
-    beta = bknum / bkden;
-    if (!(beta == beta) || bkden == 0.0) done = 3;
-  }
-  if (j == 0 && threadIdx.x == 0) {
-    st->err = err;
-    st->bnrm2 = bn2;
-    if (done) st->done = done;
-  }
-  if (done) return;
-  const TV bt = TV(beta);
-  const bool first = k == 1;
-#pragma unroll
-  for (int t = 0; t < PF; ++t) {
-    const int64_t li = i0 + t * stride;
-    if (li < sg.n) {
-      const int64_t i = sg.off + li;
-      p[i] = first ? zr[t] : zr[t] + bt * pr0[t];
-      pp[i] = first ? zzr[t] : zzr[t] + bt * ppr0[t];
-    }
-  }
-  for (int64_t li = i0 + PF * stride; li < sg.n; li += stride) {
-    const int64_t i = sg.off + li;
-    const TV zi = z[i], zzi = zz[i];
-    p[i] = first ? zi : zi + bt * p[i];
-    pp[i] = first ? zzi : zzi + bt * pp[i];
-  }
-}
-
-// K3: alpha = bknum/akden (akden = sum of the system's owner-unit partials in
-// unit order); x += a p; r -= a z; rr -= a zz; z = r/d; zz = rr/d; partials of
-// r.r and z.rr into parity q.
+// K2: alpha = bknum/akden (akden = sum of the system's owner-unit partials in
+// unit order); x += a p; r -= a w; rr -= a ww (w = A p, ww = A^T pp);
+// z = r/d; zz = rr/d; partials of r.r and z.rr into parity q.
 template <typename TV>
 __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARGS,
                                                 int NB, int q, const double *pa,
                                                 double *pr, double *pb,
                                                 SolveStatus *stv, TV *x,
-                                                const TV *p, TV *r, TV *rr,
-                                                TV *z, TV *zz, const TV *diag) {
+                                                const TV *p, const TV *w, const TV *ww,
+                                                TV *r, TV *rr, TV *z, TV *zz,
+                                                const TV *diag) {
   __shared__ double red[VB / 64];
   const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
   SolveStatus *st = stv + bo.m;
@@ -178,7 +110,7 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
   const int j = bo.j;
   const int64_t stride = (int64_t)sg.nb * VB;
   const int64_t i0 = j * (int64_t)VB + threadIdx.x;
-  TV xv[PF], pv[PF], rv[PF], rrv[PF], zv[PF], zzv[PF], dv[PF];
+  TV xv[PF], pv[PF], rv[PF], rrv[PF], wv[PF], wwv[PF], dv[PF];
 #pragma unroll
   for (int t = 0; t < PF; ++t) {
     const int64_t li = i0 + t * stride;
@@ -188,8 +120,8 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
     pv[t] = ok ? p[i] : TV(0);
     rv[t] = ok ? r[i] : TV(0);
     rrv[t] = ok ? rr[i] : TV(0);
-    zv[t] = ok ? z[i] : TV(0);
-    zzv[t] = ok ? zz[i] : TV(0);
+    wv[t] = ok ? w[i] : TV(0);
+    wwv[t] = ok ? ww[i] : TV(0);
     dv[t] = ok ? diag[i] : TV(1);
   }
   const double akden = block_sum(pa + sg.u0, (int)sg.nu, red);
@@ -205,12 +137,12 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
   for (int t = 0; t < PF; ++t) {
     const int64_t li = i0 + t * stride;
     if (li < sg.n)
-      update_one<TV>(sg.off + li, a, xv[t], pv[t], rv[t], rrv[t], zv[t], zzv[t],
+      update_one<TV>(sg.off + li, a, xv[t], pv[t], rv[t], rrv[t], wv[t], wwv[t],
                      dv[t], x, r, rr, z, zz, sr, sb);
   }
   for (int64_t li = i0 + PF * stride; li < sg.n; li += stride) {
     const int64_t i = sg.off + li;
-    update_one<TV>(i, a, x[i], p[i], r[i], rr[i], z[i], zz[i], diag[i], x, r, rr,
+    update_one<TV>(i, a, x[i], p[i], r[i], rr[i], w[i], ww[i], diag[i], x, r, rr,
                    z, zz, sr, sb);
   }
   block_partial_at(sr, pr + q * NB, sg.vb0 + j, red);
@@ -218,13 +150,15 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
   if (j == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
 }
 
-// Operator callbacks of the batched engine: the dual operator writes the
-// per-unit z.pp partials to pa and skips systems whose status is done; the
+// Operator callbacks of the batched engine.  The direction-and-dual operator
+// (K1 of an iteration) tests convergence, forms p = z + beta po and
+// pp = zz + beta ppo into dd.pn / dd.ppn and applies w = A p, ww = A^T pp
+// with the per-unit w.pp partials in pa, skipping systems that are done; the
 // single operator (y = A x for every system) ignores the status.
 template <typename TV>
-using BDualOp = std::function<int(const TV *p, TV *z, const TV *pp, TV *zz,
-                                  double *pa, const SolveStatus *st,
-                                  const ActiveSet &as, hipStream_t s)>;
+using BDirOp = std::function<int(const TV *z, const TV *zz, const DirData<TV> &dd,
+                                 TV *w, TV *ww, double *pa, const ActiveSet &as,
+                                 hipStream_t s)>;
 template <typename TV>
 using BSingleOp = std::function<int(const TV *x, TV *y, const ActiveSet &as,
                                     hipStream_t s)>;
@@ -315,8 +249,10 @@ struct BBicg {
   std::vector<Segment> hseg;
   Segment *seg = nullptr;
   ActiveBuf full, cur;  // every system / the systems the graph launches
-  TV *r = nullptr, *rr = nullptr, *p = nullptr, *pp = nullptr, *z = nullptr,
-     *zz = nullptr;
+  // directions by iteration parity (an iteration reads those of the previous
+  // one while neighbouring tiles still load them), operator outputs w, ww
+  TV *r = nullptr, *rr = nullptr, *p[2] = {nullptr, nullptr}, *pp[2] = {nullptr, nullptr},
+     *z = nullptr, *zz = nullptr, *w = nullptr, *ww = nullptr;
   double *pr = nullptr, *pb = nullptr, *pbn = nullptr, *pa = nullptr;
   SolveStatus *st = nullptr;       // device [nm]
   SolveStatus *st_host = nullptr;  // pinned [nm]
@@ -352,7 +288,7 @@ struct BBicg {
     rc = full.set(hseg, all);
     if (!rc) rc = cur.set(hseg, all);
     if (rc) return rc;
-    TV **v[] = {&r, &rr, &p, &pp, &z, &zz};
+    TV **v[] = {&r, &rr, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
     for (auto pv : v) {
       VGPU_HIP(hipMalloc(pv, ntot * sizeof(TV)));
       VGPU_HIP(hipMemset(*pv, 0, ntot * sizeof(TV)));
@@ -375,7 +311,7 @@ struct BBicg {
   }
   void release() {
     drop_graph();
-    TV **v[] = {&r, &rr, &p, &pp, &z, &zz};
+    TV **v[] = {&r, &rr, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
     for (auto pv : v)
       if (*pv) {
         hipFree(*pv);
@@ -406,7 +342,7 @@ struct BBicg {
   // quarter of the launched systems have finished, the launches are narrowed
   // to the running ones (graph re-captured between replays).  Per-system
   // results go to iters/err/status [nm].
-  int solve(uint64_t op_key, const BDualOp<TV> &dual, const BSingleOp<TV> &single,
+  int solve(uint64_t op_key, const BDirOp<TV> &dual, const BSingleOp<TV> &single,
             const uint8_t *mask, const TV *b, TV *x, const TV *diag, double tol,
             int itmax, hipStream_t s, int *iters, double *err, int *status,
             double divtol = 1e300) {
@@ -503,7 +439,7 @@ struct BBicg {
   int key_itmax = 0;
   double key_div = 0;
 
-  int capture(uint64_t op_key, const BDualOp<TV> &dual, const TV *diag, const TV *b,
+  int capture(uint64_t op_key, const BDirOp<TV> &dual, const TV *diag, const TV *b,
               TV *x, double tol, int itmax, double divtol, hipStream_t s) {
     drop_graph();
     VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
@@ -529,15 +465,28 @@ struct BBicg {
     return VGPU_OK;
   }
 
-  int enqueue_iteration(const BDualOp<TV> &dual, const TV *diag, TV *x, double tol,
+  int enqueue_iteration(const BDirOp<TV> &dual, const TV *diag, TV *x, double tol,
                         int itmax, double divtol, int q, hipStream_t s) {
     const ActiveSet &a = cur.view;
-    hipLaunchKernelGGL((kb_dir<TV>), dim3(a.nvb), dim3(VB), 0, s, seg, a.act, a.actvb,
-                       a.nact, NB, q, pr, pb, pbn, st, tol, itmax, divtol, p, pp, z, zz);
-    int rc = dual(p, z, pp, zz, pa, st, a, s);
+    DirData<TV> dd;
+    dd.po = p[q ^ 1];
+    dd.ppo = pp[q ^ 1];
+    dd.pn = p[q];
+    dd.ppn = pp[q];
+    dd.pr = pr;
+    dd.pb = pb;
+    dd.pbn = pbn;
+    dd.stv = st;
+    dd.tol = tol;
+    dd.divtol = divtol;
+    dd.itmax = itmax;
+    dd.NB = NB;
+    dd.q = q;
+    int rc = dual(z, zz, dd, w, ww, pa, a, s);
     if (rc) return rc;
     hipLaunchKernelGGL((kb_update<TV>), dim3(a.nvb), dim3(VB), 0, s, seg, a.act,
-                       a.actvb, a.nact, NB, q, pa, pr, pb, st, x, p, r, rr, z, zz, diag);
+                       a.actvb, a.nact, NB, q, pa, pr, pb, st, x, p[q], w, ww, r, rr, z,
+                       zz, diag);
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
   }

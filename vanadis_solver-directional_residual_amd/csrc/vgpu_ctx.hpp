@@ -30,6 +30,9 @@ struct vgpu_ctx {
   float *fb = nullptr, *fx = nullptr;
   double *tpart = nullptr;  // dot partials of vgpu_time_kernel
   int64_t tpart_n = 0;
+  // vgpu_time_kernel's fused-iteration launch: four vectors of N doubles
+  // (directions and operator outputs), then status and reduction partials
+  double *tdir = nullptr;
   Bicg<double> bicg64;  // brick scatter modes
   Bicg<float> bicg32;
   BBicg<double> bb64;   // owner scatter: batched engine, one system

@@ -156,6 +156,29 @@ int64_t owner_groups(const Grid3 &g, int tile, int zchunk);
 // Whether the owner kernel can take a system: it addresses each vector
 // (8 N bytes in FP64) and the element records (40 E bytes) with 32-bit offsets.
 bool owner_fits(const Grid3 &g);
+// BiCG direction update fused into the dual owner launch of an iteration
+// (batched engine, owner scatter): per system, the convergence test of the
+// previous iteration and beta from the reduction partials, then
+// p = z + beta po and pp = zz + beta ppo formed as the nodes are loaded
+// (iteration 1: p = z, pp = zz); the new directions of the nodes a tile owns
+// go to pn / ppn, the operator results to the launch's outputs.
+template <typename TIO>
+struct DirData {
+  const TIO *po = nullptr, *ppo = nullptr;  // previous directions
+  TIO *pn = nullptr, *ppn = nullptr;        // new directions
+  const double *pr = nullptr, *pb = nullptr, *pbn = nullptr;  // r.r, z.rr, b.b partials
+  SolveStatus *stv = nullptr;
+  double tol = 0, divtol = 0;
+  int itmax = 0, NB = 0, q = 0;
+};
+// y1 = A p, y2 = A^T pp with the fused direction update (inputs z, zz);
+// batch: the active members of md (else the single member m0); dot partials
+// of y1 . pp per owner unit as launch_owner_batch.
+template <typename T, typename TIO>
+int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
+                     bool batch, int tile, int zchunk, bool robin, const DirData<TIO> &dd,
+                     const TIO *z, const TIO *zz, TIO *y1, TIO *y2, double *dot,
+                     hipStream_t s);
 // Owner operator over the active systems of a batch (units split evenly over
 // as.nu): for every active member m whose status is not done, y1 = A_m x1 (and
 // y2 = A_m^T x2) on the member's node range of the batch vectors.  dot (dual only, optional):

@@ -230,20 +230,27 @@ __device__ __forceinline__ void load_rec(Res r, int v, int s, Coef<T> &c) {
 // order, then over the block in a fixed order), so the partials do not depend
 // on the grid.
 // ---------------------------------------------------------------------------
-template <typename TIO>
+// With DIR (dual only) the launch also does the BiCG direction update of the
+// iteration: x1, x2 are z, zz; the plane loads also read the previous
+// directions, form p = z + beta po, pp = zz + beta ppo (masked in LDS as the
+// operator's inputs), and the nodes of the tile's own columns and planes are
+// stored to pn, ppn.
+template <typename TIO, bool DIR>
 struct PlanePair {
   TIO a0, a1, b0, b1;
+  TIO c0, c1, d0, d1;  // DIR: previous directions of the two nodes
   uint8_t m0, m1;
 };
 
-template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, bool DIR, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
 __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 : 4)))) void k_owner(
     Member<T> m0, const Member<T> *__restrict__ md, const int *__restrict__ act,
     const int64_t *__restrict__ actu, const int64_t *__restrict__ actg, int nact,
     int64_t U, int zc, const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
     TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
-    double *__restrict__ dot) {
+    double *__restrict__ dot, DirData<TIO> dd) {
+  static_assert(DUAL || !DIR, "the direction update needs the dual operator");
   constexpr int NP = (TX + 1) * (TY + 1);
   constexpr int NT = TX * TY;
   constexpr int NW = NT / 64;
@@ -254,6 +261,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
   __shared__ T st1[2][4 * NT];
   __shared__ T st2[DUAL ? 2 : 1][DUAL ? 4 * NT : 1];
   __shared__ double red[2][NW];
+  __shared__ double rsum[NW];
+  __shared__ int sflag[2];
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
@@ -281,10 +290,63 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     return lo;
   };
 
+  // Block-wide sum of parts[0..n) in a fixed order (per thread strided, DPP
+  // within waves, waves in order); the result in every thread.
+  auto blk_sum = [&](const double *parts, int n) {
+    double v = 0.0;
+    for (int i = tid; i < n; i += NT) v += parts[i];
+    v = wave_sum_lane63(v);
+    if ((tid & 63) == 63) rsum[tid >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < NW; ++w) t += rsum[w];
+    __syncthreads();
+    return t;
+  };
+
   // One march of member m: node planes k0..k1-1 of tile `tile` (element
   // layers k0-1..k1-1).  Every thread of the block takes the same path.
   auto march = [&](int m, int tile, int k0, int k1) {
-    if (st && st[m].done) return;
+    double beta = 0.0;
+    bool first = false;
+    if (DIR) {
+      // the status of the previous iteration, read once per block: one block
+      // of this launch may be writing it (its err, and done when this
+      // launch's test ends the solve, which every block then finds too)
+      if (tid == 0) {
+        sflag[0] = dd.stv[m].done;
+        sflag[1] = dd.stv[m].iter;
+      }
+      __syncthreads();
+      const int pdone = sflag[0], k = sflag[1] + 1;
+      __syncthreads();
+      if (pdone) return;
+      const Segment sg = BATCH ? md[m].seg : m0.seg;
+      const int q = dd.q;
+      const double rn2 = blk_sum(dd.pr + (q ^ 1) * dd.NB + sg.vb0, sg.nb);
+      const double bn2 = blk_sum(dd.pbn + sg.vb0, sg.nb);
+      const double err = sqrt(rn2 / bn2);
+      int done = 0;
+      if (err <= dd.tol) done = 1;
+      else if (k > dd.itmax) done = 2;
+      else if (!(err == err)) done = 3;
+      else if (err > dd.divtol) done = 4;
+      if (!done && k > 1) {
+        const double bknum = blk_sum(dd.pb + (q ^ 1) * dd.NB + sg.vb0, sg.nb);
+        const double bkden = blk_sum(dd.pb + q * dd.NB + sg.vb0, sg.nb);
+        beta = bknum / bkden;
+        if (!(beta == beta) || bkden == 0.0) done = 3;
+      }
+      if (tile == 0 && k0 == 0 && tid == 0) {  // one block per system
+        dd.stv[m].err = err;
+        dd.stv[m].bnrm2 = bn2;
+        if (done) dd.stv[m].done = done;
+      }
+      if (done) return;
+      first = k == 1;
+    } else if (st && st[m].done) {
+      return;
+    }
     const OpData<T> op = BATCH ? md[m].op : m0.op;
     const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
     const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
@@ -324,33 +386,71 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const int pc1 = has1 ? colofs(gx0 + lx1, gy0 + ly1) : -1;
     const int vp0 = pc0 >= 0 ? pc0 * SI : kOob, vp1 = pc1 >= 0 ? pc1 * SI : kOob;
     const int vm0 = pc0 >= 0 ? pc0 : kOob, vm1 = pc1 >= 0 ? pc1 : kOob;
+    // DIR: previous and new directions; a plane node of the tile's own node
+    // columns stores its new direction (kOob elsewhere)
+    const Res rpo = make_res(DIR ? dd.po + off : x1 + off, op.g.N * SI);
+    const Res rppo = make_res(DIR ? dd.ppo + off : x1 + off, op.g.N * SI);
+    const Res rpn = make_res(DIR ? dd.pn + off : y1 + off, op.g.N * SI);
+    const Res rppn = make_res(DIR ? dd.ppn + off : y1 + off, op.g.N * SI);
+    auto owned_col = [&](int lx, int ly) {
+      return lx >= 1 && lx <= TX - 1 && ly >= 1 && ly <= TY - 1;
+    };
+    const int vn0 = (pc0 >= 0 && owned_col(lx0, ly0)) ? vp0 : kOob;
+    const int vn1 = (pc1 >= 0 && owned_col(lx1, ly1)) ? vp1 : kOob;
+    const TIO bt = TIO(beta);
+    auto dir = [&](TIO zv, TIO pv) { return first ? zv : zv + bt * pv; };
 
     // this thread's two plane nodes of plane gz (raw values and flags; value
     // 0 outside the grid: a plane above the top loads through resources of
     // zero size, at a clamped scalar offset)
-    auto plane_fetch = [&](int gz, PlanePair<TIO> &r) {
+    auto plane_fetch = [&](int gz, PlanePair<TIO, DIR> &r) {
       const bool in = gz < naz;
       const Res ra = in ? rx1 : rx1z, rb = in ? rx2 : rx1z, rk = in ? rmk : rx1z;
+      const Res rc = in ? rpo : rx1z, rd = in ? rppo : rx1z;
       const int gc = min(gz, naz - 1);
       const int sm = uni(gc * plane), sv = uni(gc * plane * SI);
       r.m0 = bload<uint8_t>(rk, vm0, sm);
       r.a0 = bload<TIO>(ra, vp0, sv);
       r.b0 = DUAL ? bload<TIO>(rb, vp0, sv) : TIO(0);
-      r.a1 = r.b1 = TIO(0);
+      r.c0 = DIR ? bload<TIO>(rc, vp0, sv) : TIO(0);
+      r.d0 = DIR ? bload<TIO>(rd, vp0, sv) : TIO(0);
+      r.a1 = r.b1 = r.c1 = r.d1 = TIO(0);
       r.m1 = 1;
       if (has1) {
         r.m1 = bload<uint8_t>(rk, vm1, sm);
         r.a1 = bload<TIO>(ra, vp1, sv);
         if (DUAL) r.b1 = bload<TIO>(rb, vp1, sv);
+        if (DIR) {
+          r.c1 = bload<TIO>(rc, vp1, sv);
+          r.d1 = bload<TIO>(rd, vp1, sv);
+        }
       }
     };
-    auto plane_put = [&](const PlanePair<TIO> &r, T *p1, T *p2, uint8_t *pk) {
-      p1[q0] = r.m0 ? T(0) : T(r.a0);
-      if (DUAL) p2[q0] = r.m0 ? T(0) : T(r.b0);
+    // plane gz into LDS; DIR: the directions are formed first and those of
+    // the march's own planes stored
+    auto plane_put = [&](const PlanePair<TIO, DIR> &r, int gz, T *p1, T *p2, uint8_t *pk) {
+      TIO a0 = r.a0, b0 = r.b0, a1 = r.a1, b1 = r.b1;
+      if (DIR) {
+        a0 = dir(r.a0, r.c0);
+        b0 = dir(r.b0, r.d0);
+        a1 = dir(r.a1, r.c1);
+        b1 = dir(r.b1, r.d1);
+        if (gz >= k0 && gz < k1) {
+          const int sv = uni(gz * plane * SI);
+          bstore(a0, rpn, vn0, sv);
+          bstore(b0, rppn, vn0, sv);
+          if (has1) {
+            bstore(a1, rpn, vn1, sv);
+            bstore(b1, rppn, vn1, sv);
+          }
+        }
+      }
+      p1[q0] = r.m0 ? T(0) : T(a0);
+      if (DUAL) p2[q0] = r.m0 ? T(0) : T(b0);
       pk[q0] = r.m0;
       if (has1) {
-        p1[q1] = r.m1 ? T(0) : T(r.a1);
-        if (DUAL) p2[q1] = r.m1 ? T(0) : T(r.b1);
+        p1[q1] = r.m1 ? T(0) : T(a1);
+        if (DUAL) p2[q1] = r.m1 ? T(0) : T(b1);
         pk[q1] = r.m1;
       }
     };
@@ -401,10 +501,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
         if (zin) {
           const int sm = uni(gz * plane), sv = uni(gz * plane * SI);
           const uint8_t mk = bload<uint8_t>(rmk, vc, sm);
-          const TIO a = bload<TIO>(rx1, vcs, sv);
+          TIO a = bload<TIO>(rx1, vcs, sv);
+          if (DIR) a = dir(a, bload<TIO>(rpo, vcs, sv));
           n1[c] = mk ? T(0) : T(a);
           if (DUAL) {
-            const TIO b = bload<TIO>(rx2, vcs, sv);
+            TIO b = bload<TIO>(rx2, vcs, sv);
+            if (DIR) b = dir(b, bload<TIO>(rppo, vcs, sv));
             n2[c] = mk ? T(0) : T(b);
           }
         }
@@ -412,7 +514,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       face_fwd<T>(n1, fb1);
       if (DUAL) face_fwd<T>(n2, fb2);
     }
-    PlanePair<TIO> pr;
+    PlanePair<TIO, DIR> pr;
     plane_fetch(k0, pr);
     Coef<T> cn, cf;
     coef_fetch(k0 - 1, cn);
@@ -453,7 +555,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     // one element layer kl with plane kl+1 in buffer P
     auto layer = [&](auto Pc) {
       constexpr int P = decltype(Pc)::value;
-      plane_put(pr, pl1[P], DUAL ? pl2[P] : nullptr, pm[P]);
+      plane_put(pr, kl + 1, pl1[P], DUAL ? pl2[DUAL ? P : 0] : nullptr, pm[P]);
       cf = cn;
       plane_fetch(kl + 2, pr);  // prefetch plane kl+2 and layer kl+1
       coef_fetch(kl + 1, cn);
@@ -780,9 +882,9 @@ bool owner_fits(const Grid3 &g) {
 }
 
 namespace {
-template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, bool DIR, int TX, int TY>
 const void *owner_kernel() {
-  return reinterpret_cast<const void *>(&k_owner<T, TIO, DUAL, ROBIN, BATCH, TX, TY>);
+  return reinterpret_cast<const void *>(&k_owner<T, TIO, DUAL, ROBIN, BATCH, DIR, TX, TY>);
 }
 
 // Resident workgroups of a kernel on the whole device (cached per kernel).
@@ -813,52 +915,61 @@ int owner_grid(const void *fn, int threads, int64_t utot, int64_t gtot, int zchu
   return (int)std::max<int64_t>(1, std::min<int64_t>(r, utot));
 }
 
-template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int TX, int TY>
-void launch_owner_k(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
-                    int64_t utot, int64_t gtot, int zchunk, const SolveStatus *st,
-                    const TIO *x1, TIO *y1, const TIO *x2, TIO *y2, double *dot,
-                    hipStream_t s) {
-  const void *fn = owner_kernel<T, TIO, DUAL, ROBIN, BATCH, TX, TY>();
-  const int nb = owner_grid(fn, TX * TY, utot, gtot, zchunk);
-  hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, TX, TY>), dim3(nb),
-                     dim3(TX * TY), 0, s, m0, md, as.act, as.actu, as.actg, as.nact, utot,
-                     zchunk > 0 ? zchunk : 0, st, x1, y1, x2, y2, dot);
+// Everything one owner launch needs besides the template choices.
+template <typename T, typename TIO>
+struct OwnerLaunch {
+  Member<T> m0;
+  const Member<T> *md;
+  ActiveSet as;
+  int64_t utot, gtot;
+  int zchunk;
+  const SolveStatus *st;
+  const TIO *x1;
+  TIO *y1;
+  const TIO *x2;
+  TIO *y2;
+  double *dot;
+  DirData<TIO> dd;
+};
+
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, bool DIR, int TX, int TY>
+void launch_owner_k(const OwnerLaunch<T, TIO> &L, hipStream_t s) {
+  const void *fn = owner_kernel<T, TIO, DUAL, ROBIN, BATCH, DIR, TX, TY>();
+  const int nb = owner_grid(fn, TX * TY, L.utot, L.gtot, L.zchunk);
+  hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, DIR, TX, TY>), dim3(nb),
+                     dim3(TX * TY), 0, s, L.m0, L.md, L.as.act, L.as.actu, L.as.actg,
+                     L.as.nact, L.utot, L.zchunk > 0 ? L.zchunk : 0, L.st, L.x1, L.y1,
+                     L.x2, L.y2, L.dot, L.dd);
 }
 
-template <typename T, typename TIO, bool BATCH, int TX, int TY>
-void launch_owner_shape(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
-                        int64_t utot, int64_t gtot, int zchunk, bool robin,
-                        const SolveStatus *st, const TIO *x1, TIO *y1,
-                        const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
-  if (x2) {
+template <typename T, typename TIO, bool BATCH, bool DIR, int TX, int TY>
+void launch_owner_shape(const OwnerLaunch<T, TIO> &L, bool robin, hipStream_t s) {
+  if (L.x2) {
     if (robin)
-      launch_owner_k<T, TIO, true, true, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, dot, s);
+      launch_owner_k<T, TIO, true, true, BATCH, DIR, TX, TY>(L, s);
     else
-      launch_owner_k<T, TIO, true, false, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, dot, s);
-  } else {
+      launch_owner_k<T, TIO, true, false, BATCH, DIR, TX, TY>(L, s);
+  } else if (!DIR) {
     if (robin)
-      launch_owner_k<T, TIO, false, true, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, nullptr, s);
+      launch_owner_k<T, TIO, false, true, BATCH, false, TX, TY>(L, s);
     else
-      launch_owner_k<T, TIO, false, false, BATCH, TX, TY>(m0, md, as, utot, gtot, zchunk, st, x1, y1, x2, y2, nullptr, s);
+      launch_owner_k<T, TIO, false, false, BATCH, false, TX, TY>(L, s);
   }
 }
 
 // utot units (balanced launch) or gtot groups (z-chunk launch, zchunk > 0)
-template <typename T, typename TIO, bool BATCH>
-int launch_owner_any(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
-                     int64_t utot, int64_t gtot, int tile, int zchunk, bool robin,
-                     const SolveStatus *st, const TIO *x1, TIO *y1,
-                     const TIO *x2, TIO *y2, double *dot, hipStream_t s) {
-  if (zchunk > 0 ? gtot <= 0 : utot <= 0) return VGPU_OK;
+template <typename T, typename TIO, bool BATCH, bool DIR = false>
+int launch_owner_any(const OwnerLaunch<T, TIO> &L, int tile, bool robin, hipStream_t s) {
+  if (L.zchunk > 0 ? L.gtot <= 0 : L.utot <= 0) return VGPU_OK;
   switch (tile) {
     case 1:
-      launch_owner_shape<T, TIO, BATCH, 16, 16>(m0, md, as, utot, gtot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, DIR, 16, 16>(L, robin, s);
       break;
     case 2:
-      launch_owner_shape<T, TIO, BATCH, 64, 4>(m0, md, as, utot, gtot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, DIR, 64, 4>(L, robin, s);
       break;
     default:
-      launch_owner_shape<T, TIO, BATCH, 32, 8>(m0, md, as, utot, gtot, zchunk, robin, st, x1, y1, x2, y2, dot, s);
+      launch_owner_shape<T, TIO, BATCH, DIR, 32, 8>(L, robin, s);
       break;
   }
   VGPU_HIP(hipGetLastError());
@@ -871,9 +982,19 @@ int launch_owner_batch(const Member<T> *md, const ActiveSet &as, int tile,
                        int zchunk, bool robin, const SolveStatus *st,
                        const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,
                        double *dot, hipStream_t s) {
-  Member<T> none{};
-  return launch_owner_any<T, TIO, true>(none, md, as, as.nu, as.ng, tile, zchunk, robin,
-                                        st, x1, y1, x2, y2, dot, s);
+  OwnerLaunch<T, TIO> L{};
+  L.md = md;
+  L.as = as;
+  L.utot = as.nu;
+  L.gtot = as.ng;
+  L.zchunk = zchunk;
+  L.st = st;
+  L.x1 = x1;
+  L.y1 = y1;
+  L.x2 = x2;
+  L.y2 = y2;
+  L.dot = dot;
+  return launch_owner_any<T, TIO, true>(L, tile, robin, s);
 }
 
 template int launch_owner_batch<double, double>(const Member<double> *, const ActiveSet &,
@@ -885,6 +1006,47 @@ template int launch_owner_batch<float, float>(const Member<float> *, const Activ
                                               int, int, bool, const SolveStatus *,
                                               const float *, float *, const float *,
                                               float *, double *, hipStream_t);
+
+template <typename T, typename TIO>
+int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
+                     bool batch, int tile, int zchunk, bool robin, const DirData<TIO> &dd,
+                     const TIO *z, const TIO *zz, TIO *y1, TIO *y2, double *dot,
+                     hipStream_t s) {
+  OwnerLaunch<T, TIO> L{};
+  L.zchunk = zchunk;
+  L.x1 = z;
+  L.y1 = y1;
+  L.x2 = zz;
+  L.y2 = y2;
+  L.dot = dot;
+  L.dd = dd;
+  if (batch) {
+    L.md = md;
+    L.as = as;
+    L.utot = as.nu;
+    L.gtot = as.ng;
+    return launch_owner_any<T, TIO, true, true>(L, tile, robin, s);
+  }
+  if (!owner_fits(m0.op.g)) {
+    set_error("owner scatter: system too large for 32-bit offsets");
+    return VGPU_ERR_ARG;
+  }
+  L.m0 = m0;
+  L.utot = m0.seg.nu;
+  L.gtot = owner_groups(m0.op.g, tile, zchunk);
+  return launch_owner_any<T, TIO, false, true>(L, tile, robin, s);
+}
+
+template int launch_owner_dir<double, double>(const Member<double> &, const Member<double> *,
+                                              const ActiveSet &, bool, int, int, bool,
+                                              const DirData<double> &, const double *,
+                                              const double *, double *, double *, double *,
+                                              hipStream_t);
+template int launch_owner_dir<float, float>(const Member<float> &, const Member<float> *,
+                                            const ActiveSet &, bool, int, int, bool,
+                                            const DirData<float> &, const float *,
+                                            const float *, float *, float *, double *,
+                                            hipStream_t);
 
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
@@ -905,10 +1067,18 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
     m0.seg.u0 = 0;
     m0.seg.nu = owner_units(g, op.tile);
     const SolveStatus *st = reinterpret_cast<const SolveStatus *>(op.done);
-    m0.seg.ng = owner_groups(g, op.tile, zc);
-    return launch_owner_any<T, TIO, false>(m0, nullptr, ActiveSet{}, m0.seg.nu, m0.seg.ng,
-                                           op.tile, zc, op.ra != 0.0, st, x1, y1, x2, y2,
-                                           dot_part, s);
+    OwnerLaunch<T, TIO> L{};
+    L.m0 = m0;
+    L.utot = m0.seg.nu;
+    L.gtot = owner_groups(g, op.tile, zc);
+    L.zchunk = zc;
+    L.st = st;
+    L.x1 = x1;
+    L.y1 = y1;
+    L.x2 = x2;
+    L.y2 = y2;
+    L.dot = dot_part;
+    return launch_owner_any<T, TIO, false>(L, op.tile, op.ra != 0.0, s);
   }
   // brick modes need zeroed outputs; bricks are 32 x 8 element columns
   constexpr int BX = 32, BY = 8;
