@@ -218,9 +218,10 @@ int vgpu_jacobi_dev(vgpu_ctx *ctx, const double *b, double *x, int sweeps, doubl
  * system per call, VANADIS.FOR:809-810) ----
  * A batch groups nm contexts on one device (each with its own mesh, mask,
  * coefficients and scheme; several right-hand sides of one operator are
- * several contexts).  Every BiCG iteration runs one persistent owner launch
- * over all systems plus two vector launches, captured in one hipGraph; the
- * systems stop independently.  The batch uses the owner tile shape and z
+ * several contexts).  Every BiCG iteration runs one owner launch over all
+ * systems (convergence test, direction update and dual operator) and one
+ * vector launch (update), captured in one hipGraph; the systems stop
+ * independently.  The batch uses the owner tile shape and z
  * chunk of ctxs[0]; with the same tile shape each system's iterates are
  * bitwise those of vgpu_solve on its context.  Contexts must outlive the
  * batch; coefficient or option changes are picked up at the next solve. */

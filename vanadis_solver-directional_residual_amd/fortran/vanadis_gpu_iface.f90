@@ -81,5 +81,59 @@ module vanadis_gpu
       real(c_double), intent(in) :: c(*)
       real(c_double), intent(out) :: mass
     end function
+
+    ! diagonal-dominance factor and flag of the current scheme's element
+    ! matrices (the host's diagonal_dominance_factor / is_diagonally_dominant)
+    integer(c_int) function vgpu_ddf(ctx, ddf, dominant) bind(C, name="vgpu_ddf")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: ddf
+      integer(c_int), intent(out) :: dominant
+    end function
+
+    ! the host's T-file: x, y, z, c*1e6 per node in e12.5; path NUL-terminated
+    integer(c_int) function vgpu_write_tfile(path, n, xx, yy, zz, c) &
+        bind(C, name="vgpu_write_tfile")
+      import :: c_int, c_int64_t, c_double, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int64_t), value :: n
+      real(c_double), intent(in) :: xx(*), yy(*), zz(*), c(*)
+    end function
+
+    ! owner tile shape and z chunk from timings of the solver's iteration launch
+    integer(c_int) function vgpu_autotune(ctx, precision, reps, scatter, tile, zchunk, &
+                                          best_ms) bind(C, name="vgpu_autotune")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: precision, reps
+      integer(c_int), intent(out) :: scatter, tile, zchunk
+      real(c_double), intent(out) :: best_ms
+    end function
+
+    ! several systems (contexts) solved together; ctxs(nm) holds their c_ptr
+    integer(c_int) function vgpu_batch_create(nm, ctxs, batch) &
+        bind(C, name="vgpu_batch_create")
+      import :: c_int, c_ptr
+      integer(c_int), value :: nm
+      type(c_ptr), intent(in) :: ctxs(*)
+      type(c_ptr), intent(out) :: batch
+    end function
+
+    integer(c_int) function vgpu_batch_destroy(batch) bind(C, name="vgpu_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+    end function
+
+    ! b(m), x(m): c_loc of system m's right-hand side and solution
+    integer(c_int) function vgpu_batch_solve(batch, precision, b, x, rtol, itmax, iters, &
+                                             errs, status) bind(C, name="vgpu_batch_solve")
+      import :: c_int, c_double, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: precision, itmax
+      type(c_ptr), intent(in) :: b(*), x(*)
+      real(c_double), value :: rtol
+      integer(c_int), intent(out) :: iters(*), status(*)
+      real(c_double), intent(out) :: errs(*)
+    end function
   end interface
 end module vanadis_gpu
