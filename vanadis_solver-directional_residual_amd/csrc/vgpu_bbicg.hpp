@@ -57,12 +57,14 @@ __device__ __forceinline__ void block_partial_at(double v, double *out, int slot
 
 #define VGPU_ACT_ARGS const int *act, const int *actvb, int nact
 
-// r = b - r (r holds A x0), rr = r, z = r/d, zz = rr/d, x zeroed on the mask;
-// partials of r.r, z.rr and b.b (cudatest.cu:464-491, VANADIS.FOR:991-1000).
+// r = b - A x0 (ax holds A x0), rr = r; stores only the preconditioned
+// z = r/d, zz = rr/d (the engine carries z, zz: r = d z), x zeroed on the
+// mask; partials of r.r, z.rr and b.b (cudatest.cu:464-491,
+// VANADIS.FOR:991-1000).
 template <typename TV>
 __global__ __launch_bounds__(VB) void kb_init(const Segment *seg, VGPU_ACT_ARGS,
                                               const uint8_t *mask, const TV *b,
-                                              TV *x, TV *r, TV *rr, TV *z,
+                                              TV *x, const TV *ax, TV *z,
                                               TV *zz, const TV *diag,
                                               double *pr, double *pb,
                                               double *pbn) {
@@ -76,9 +78,7 @@ __global__ __launch_bounds__(VB) void kb_init(const Segment *seg, VGPU_ACT_ARGS,
     const int64_t i = sg.off + li;
     if (mask[i]) x[i] = TV(0);
     const double bi = b[i];
-    const TV ri = TV(bi - (double)r[i]);
-    r[i] = ri;
-    rr[i] = ri;
+    const TV ri = TV(bi - (double)ax[i]);
     const TV zi = TV((double)ri / (double)diag[i]);
     z[i] = zi;
     zz[i] = zi;
@@ -91,17 +91,32 @@ __global__ __launch_bounds__(VB) void kb_init(const Segment *seg, VGPU_ACT_ARGS,
   block_partial_at(sn, pbn, sg.vb0 + j, red);
 }
 
+// One node of K2: x += a p and the residual update r -= a w, rr -= a ww
+// carried on z = r/d, zz = rr/d (w = A p, ww = A^T pp); partials of r.r and
+// z.rr with r = d z, rr = d zz.
+template <typename TV>
+__device__ __forceinline__ void update_z(int64_t i, TV a, TV xi, TV pi, TV zi, TV zzi,
+                                         TV wi, TV wwi, TV di, TV *x, TV *z, TV *zz,
+                                         double &sr, double &sb) {
+  x[i] = xi + a * pi;
+  const TV zn = zi - a * (wi / di), zzn = zzi - a * (wwi / di);
+  z[i] = zn;
+  zz[i] = zzn;
+  const double rn = (double)zn * (double)di, rrn = (double)zzn * (double)di;
+  sr += rn * rn;
+  sb += (double)zn * rrn;
+}
+
 // K2: alpha = bknum/akden (akden = sum of the system's owner-unit partials in
-// unit order); x += a p; r -= a w; rr -= a ww (w = A p, ww = A^T pp);
-// z = r/d; zz = rr/d; partials of r.r and z.rr into parity q.
+// unit order); x += a p; z -= a w/d; zz -= a ww/d; partials of r.r and z.rr
+// into parity q.
 template <typename TV>
 __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARGS,
                                                 int NB, int q, const double *pa,
                                                 double *pr, double *pb,
                                                 SolveStatus *stv, TV *x,
                                                 const TV *p, const TV *w, const TV *ww,
-                                                TV *r, TV *rr, TV *z, TV *zz,
-                                                const TV *diag) {
+                                                TV *z, TV *zz, const TV *diag) {
   __shared__ double red[VB / 64];
   const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
   SolveStatus *st = stv + bo.m;
@@ -110,7 +125,7 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
   const int j = bo.j;
   const int64_t stride = (int64_t)sg.nb * VB;
   const int64_t i0 = j * (int64_t)VB + threadIdx.x;
-  TV xv[PF], pv[PF], rv[PF], rrv[PF], wv[PF], wwv[PF], dv[PF];
+  TV xv[PF], pv[PF], zv[PF], zzv[PF], wv[PF], wwv[PF], dv[PF];
 #pragma unroll
   for (int t = 0; t < PF; ++t) {
     const int64_t li = i0 + t * stride;
@@ -118,8 +133,8 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
     const int64_t i = sg.off + (ok ? li : 0);
     xv[t] = ok ? x[i] : TV(0);
     pv[t] = ok ? p[i] : TV(0);
-    rv[t] = ok ? r[i] : TV(0);
-    rrv[t] = ok ? rr[i] : TV(0);
+    zv[t] = ok ? z[i] : TV(0);
+    zzv[t] = ok ? zz[i] : TV(0);
     wv[t] = ok ? w[i] : TV(0);
     wwv[t] = ok ? ww[i] : TV(0);
     dv[t] = ok ? diag[i] : TV(1);
@@ -137,13 +152,12 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
   for (int t = 0; t < PF; ++t) {
     const int64_t li = i0 + t * stride;
     if (li < sg.n)
-      update_one<TV>(sg.off + li, a, xv[t], pv[t], rv[t], rrv[t], wv[t], wwv[t],
-                     dv[t], x, r, rr, z, zz, sr, sb);
+      update_z<TV>(sg.off + li, a, xv[t], pv[t], zv[t], zzv[t], wv[t], wwv[t], dv[t], x,
+                   z, zz, sr, sb);
   }
   for (int64_t li = i0 + PF * stride; li < sg.n; li += stride) {
     const int64_t i = sg.off + li;
-    update_one<TV>(i, a, x[i], p[i], r[i], rr[i], w[i], ww[i], diag[i], x, r, rr,
-                   z, zz, sr, sb);
+    update_z<TV>(i, a, x[i], p[i], z[i], zz[i], w[i], ww[i], diag[i], x, z, zz, sr, sb);
   }
   block_partial_at(sr, pr + q * NB, sg.vb0 + j, red);
   block_partial_at(sb, pb + q * NB, sg.vb0 + j, red);
@@ -250,8 +264,10 @@ struct BBicg {
   Segment *seg = nullptr;
   ActiveBuf full, cur;  // every system / the systems the graph launches
   // directions by iteration parity (an iteration reads those of the previous
-  // one while neighbouring tiles still load them), operator outputs w, ww
-  TV *r = nullptr, *rr = nullptr, *p[2] = {nullptr, nullptr}, *pp[2] = {nullptr, nullptr},
+  // one while neighbouring tiles still load them), operator outputs w, ww,
+  // preconditioned residuals z, zz (the residuals are d z, d zz); ax holds
+  // A x0 at the start
+  TV *ax = nullptr, *p[2] = {nullptr, nullptr}, *pp[2] = {nullptr, nullptr},
      *z = nullptr, *zz = nullptr, *w = nullptr, *ww = nullptr;
   double *pr = nullptr, *pb = nullptr, *pbn = nullptr, *pa = nullptr;
   SolveStatus *st = nullptr;       // device [nm]
@@ -288,7 +304,7 @@ struct BBicg {
     rc = full.set(hseg, all);
     if (!rc) rc = cur.set(hseg, all);
     if (rc) return rc;
-    TV **v[] = {&r, &rr, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
+    TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
     for (auto pv : v) {
       VGPU_HIP(hipMalloc(pv, ntot * sizeof(TV)));
       VGPU_HIP(hipMemset(*pv, 0, ntot * sizeof(TV)));
@@ -311,7 +327,7 @@ struct BBicg {
   }
   void release() {
     drop_graph();
-    TV **v[] = {&r, &rr, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
+    TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
     for (auto pv : v)
       if (*pv) {
         hipFree(*pv);
@@ -346,10 +362,10 @@ struct BBicg {
             const uint8_t *mask, const TV *b, TV *x, const TV *diag, double tol,
             int itmax, hipStream_t s, int *iters, double *err, int *status,
             double divtol = 1e300) {
-    int rc = single(x, r, full.view, s);  // r = A x0 (x0 masked on load)
+    int rc = single(x, ax, full.view, s);  // A x0 (x0 masked on load)
     if (rc) return rc;
     hipLaunchKernelGGL((kb_init<TV>), dim3(full.view.nvb), dim3(VB), 0, s, seg,
-                       full.view.act, full.view.actvb, full.view.nact, mask, b, x, r, rr,
+                       full.view.act, full.view.actvb, full.view.nact, mask, b, x, ax,
                        z, zz, diag, pr, pb, pbn);
     VGPU_HIP(hipGetLastError());
     std::vector<SolveStatus> s0(nm, SolveStatus{});
@@ -485,8 +501,8 @@ struct BBicg {
     int rc = dual(z, zz, dd, w, ww, pa, a, s);
     if (rc) return rc;
     hipLaunchKernelGGL((kb_update<TV>), dim3(a.nvb), dim3(VB), 0, s, seg, a.act,
-                       a.actvb, a.nact, NB, q, pa, pr, pb, st, x, p[q], w, ww, r, rr, z,
-                       zz, diag);
+                       a.actvb, a.nact, NB, q, pa, pr, pb, st, x, p[q], w, ww, z, zz,
+                       diag);
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
   }
