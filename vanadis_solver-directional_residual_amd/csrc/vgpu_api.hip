@@ -248,7 +248,8 @@ template <typename TV>
 int solve_owner(vgpu_ctx *c, BBicg<TV> &eng, const TV *b, TV *x, const TV *diag,
                 double tol, int itmax, int *iters, double *err, int *status,
                 double divtol = 1e300) {
-  int rc = eng.alloc(make_segments({c->g.N}, {owner_units(c->g, c->tile)}));
+  int rc = eng.alloc(make_segments({c->g.N}, {owner_units(c->g, c->tile)},
+                                   {owner_groups(c->g, c->tile, c->zchunk)}));
   if (rc) return rc;
   const Segment sg = eng.hseg[0];
   BDirOp<TV> dual = [c, sg](const TV *z, const TV *zz, const DirData<TV> &dd, TV *w,

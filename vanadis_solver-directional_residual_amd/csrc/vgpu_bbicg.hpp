@@ -5,7 +5,7 @@
 //
 // Layout.  The systems' vectors are concatenated (system m at nodes
 // [off_m, off_m + n_m)).  System m owns the vector workgroups
-// [vb0_m, vb0_m + nb_m), nb_m = min(256, ceil(n_m / 1024)), each striding over
+// [vb0_m, vb0_m + nb_m), nb_m = min(1024, ceil(n_m / 1024)), each striding over
 // the system's own range exactly as a solve of that system alone would, and
 // its reduction partials sit at those workgroup indices; the operator writes
 // its z.pp partials per owner unit.  Every scalar of system m is therefore
@@ -30,8 +30,8 @@ namespace vgpu {
 struct BlockOf {
   int m, j;
 };
-__device__ __forceinline__ BlockOf block_of(const int *act, const int *actvb, int nact,
-                                            int b) {
+__device__ __forceinline__ BlockOf block_of(const int *__restrict__ act,
+                                            const int *__restrict__ actvb, int nact, int b) {
   int lo = 0, hi = nact - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -55,26 +55,33 @@ __device__ __forceinline__ void block_partial_at(double v, double *out, int slot
   }
 }
 
-#define VGPU_ACT_ARGS const int *act, const int *actvb, int nact
+#define VGPU_ACT_ARGS \
+  const int *__restrict__ act, const int *__restrict__ actvb, int nact
+
+// Vector workgroups of the batched engine: BVB threads, at most BNB per
+// system, so a large system gets several workgroups per CU whose load and
+// store phases overlap (about four nodes per thread).
+constexpr int BVB = 256;
+constexpr int BNB = 1024;
 
 // r = b - A x0 (ax holds A x0), rr = r; stores only the preconditioned
 // z = r/d, zz = rr/d (the engine carries z, zz: r = d z), x zeroed on the
 // mask; partials of r.r, z.rr and b.b (cudatest.cu:464-491,
 // VANADIS.FOR:991-1000).
 template <typename TV>
-__global__ __launch_bounds__(VB) void kb_init(const Segment *seg, VGPU_ACT_ARGS,
+__global__ __launch_bounds__(BVB) void kb_init(const Segment *__restrict__ seg, VGPU_ACT_ARGS,
                                               const uint8_t *mask, const TV *b,
                                               TV *x, const TV *ax, TV *z,
                                               TV *zz, const TV *diag,
                                               double *pr, double *pb,
                                               double *pbn) {
-  __shared__ double red[VB / 64];
+  __shared__ double red[BVB / 64];
   const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
   const Segment sg = seg[bo.m];
   const int j = bo.j;
   double sr = 0, sb = 0, sn = 0;
-  for (int64_t li = j * (int64_t)VB + threadIdx.x; li < sg.n;
-       li += (int64_t)sg.nb * VB) {
+  for (int64_t li = j * (int64_t)BVB + threadIdx.x; li < sg.n;
+       li += (int64_t)sg.nb * BVB) {
     const int64_t i = sg.off + li;
     if (mask[i]) x[i] = TV(0);
     const double bi = b[i];
@@ -89,6 +96,27 @@ __global__ __launch_bounds__(VB) void kb_init(const Segment *seg, VGPU_ACT_ARGS,
   block_partial_at(sr, pr, sg.vb0 + j, red);
   block_partial_at(sb, pb, sg.vb0 + j, red);
   block_partial_at(sn, pbn, sg.vb0 + j, red);
+}
+
+// Block-wide sums of two per-thread values, each in the order block_sum
+// uses (shuffles within waves, waves in order); results in every thread.
+__device__ __forceinline__ void block_sum_pair(double u, double v, double (*red2)[BVB / 64],
+                                               double &su, double &sv) {
+  for (int off = 32; off > 0; off >>= 1) {
+    u += __shfl_down(u, off, 64);
+    v += __shfl_down(v, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red2[0][threadIdx.x >> 6] = u;
+    red2[1][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  su = sv = 0.0;
+  for (int w = 0; w < BVB / 64; ++w) {
+    su += red2[0][w];
+    sv += red2[1][w];
+  }
+  __syncthreads();
 }
 
 // One node of K2: x += a p and the residual update r -= a w, rr -= a ww
@@ -107,24 +135,50 @@ __device__ __forceinline__ void update_z(int64_t i, TV a, TV xi, TV pi, TV zi, T
   sb += (double)zn * rrn;
 }
 
-// K2: alpha = bknum/akden (akden = sum of the system's owner-unit partials in
-// unit order); x += a p; z -= a w/d; zz -= a ww/d; partials of r.r and z.rr
-// into parity q.
+// K2: alpha = bknum/akden (akden = sum of the system's owner partials in
+// order); x += a p; z -= a w/d; zz -= a ww/d; partials of r.r and z.rr into
+// parity q.
 template <typename TV>
-__global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARGS,
-                                                int NB, int q, const double *pa,
-                                                double *pr, double *pb,
-                                                SolveStatus *stv, TV *x,
-                                                const TV *p, const TV *w, const TV *ww,
-                                                TV *z, TV *zz, const TV *diag) {
-  __shared__ double red[VB / 64];
+__global__ __launch_bounds__(BVB) void kb_update(const Segment *__restrict__ seg,
+                                                 VGPU_ACT_ARGS, int NB, int q,
+                                                 const double *__restrict__ pa,
+                                                 double *__restrict__ pr,
+                                                 double *__restrict__ pb,
+                                                 SolveStatus *__restrict__ stv,
+                                                 TV *__restrict__ x,
+                                                 const TV *__restrict__ p,
+                                                 const TV *__restrict__ w,
+                                                 const TV *__restrict__ ww,
+                                                 TV *__restrict__ z, TV *__restrict__ zz,
+                                                 const TV *__restrict__ diag) {
+  __shared__ double red[BVB / 64];
+  __shared__ double red2[2][BVB / 64];
   const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
   SolveStatus *st = stv + bo.m;
   if (st->done) return;
   const Segment sg = seg[bo.m];
   const int j = bo.j;
-  const int64_t stride = (int64_t)sg.nb * VB;
-  const int64_t i0 = j * (int64_t)VB + threadIdx.x;
+  const int64_t stride = (int64_t)sg.nb * BVB;
+  const int tid = threadIdx.x;
+  const int64_t i0 = j * (int64_t)BVB + tid;
+  // the owner launch's w.pp partials (one per (chunk, tile) group of a
+  // z-chunk launch, else one per unit) and the z.rr partials: their first
+  // loads are issued before the node loads, so alpha is known as soon as
+  // they return (loads complete in order) and the first nodes' stores need
+  // not wait for the last nodes' loads
+  const int na = (int)(sg.ng > 0 ? sg.ng : sg.nu);
+  const double *pas = pa + sg.u0, *pbs = pb + (q ^ 1) * NB + sg.vb0;
+  constexpr int KA = 8, KB = 4;
+  double av[KA], bv[KB];
+  // unconditional loads at clamped indices (na, nb >= 1), then selects
+#pragma unroll
+  for (int k = 0; k < KA; ++k) av[k] = pas[min(tid + k * BVB, na - 1)];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) bv[k] = pbs[min(tid + k * BVB, sg.nb - 1)];
+#pragma unroll
+  for (int k = 0; k < KA; ++k) av[k] = tid + k * BVB < na ? av[k] : 0.0;
+#pragma unroll
+  for (int k = 0; k < KB; ++k) bv[k] = tid + k * BVB < sg.nb ? bv[k] : 0.0;
   TV xv[PF], pv[PF], zv[PF], zzv[PF], wv[PF], wwv[PF], dv[PF];
 #pragma unroll
   for (int t = 0; t < PF; ++t) {
@@ -139,8 +193,16 @@ __global__ __launch_bounds__(VB) void kb_update(const Segment *seg, VGPU_ACT_ARG
     wwv[t] = ok ? ww[i] : TV(0);
     dv[t] = ok ? diag[i] : TV(1);
   }
-  const double akden = block_sum(pa + sg.u0, (int)sg.nu, red);
-  const double bknum = block_sum(pb + (q ^ 1) * NB + sg.vb0, sg.nb, red);
+  // per thread in index order (as block_sum), then over the block
+  double sa = 0.0, sbk = 0.0;
+#pragma unroll
+  for (int k = 0; k < KA; ++k) sa += av[k];
+  for (int i = tid + KA * BVB; i < na; i += BVB) sa += pas[i];
+#pragma unroll
+  for (int k = 0; k < KB; ++k) sbk += bv[k];
+  for (int i = tid + KB * BVB; i < sg.nb; i += BVB) sbk += pbs[i];
+  double akden, bknum;
+  block_sum_pair(sa, sbk, red2, akden, bknum);
   const double ak = bknum / akden;
   if (!(ak == ak) || akden == 0.0) {
     if (j == 0 && threadIdx.x == 0) st->done = 3;
@@ -195,7 +257,7 @@ inline std::vector<Segment> make_segments(const std::vector<int64_t> &n,
     s[m].u0 = u0;
     s[m].nu = nu[m];
     s[m].vb0 = vb0;
-    s[m].nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n[m] + VB - 1) / VB));
+    s[m].nb = (int)std::min<int64_t>(BNB, std::max<int64_t>(1, (n[m] + 4 * BVB - 1) / (4 * BVB)));
     s[m].ng = ng.empty() ? 0 : ng[m];
     off += seg_pad(n[m]);
     u0 += nu[m];
@@ -364,7 +426,7 @@ struct BBicg {
             double divtol = 1e300) {
     int rc = single(x, ax, full.view, s);  // A x0 (x0 masked on load)
     if (rc) return rc;
-    hipLaunchKernelGGL((kb_init<TV>), dim3(full.view.nvb), dim3(VB), 0, s, seg,
+    hipLaunchKernelGGL((kb_init<TV>), dim3(full.view.nvb), dim3(BVB), 0, s, seg,
                        full.view.act, full.view.actvb, full.view.nact, mask, b, x, ax,
                        z, zz, diag, pr, pb, pbn);
     VGPU_HIP(hipGetLastError());
@@ -500,7 +562,7 @@ struct BBicg {
     dd.q = q;
     int rc = dual(z, zz, dd, w, ww, pa, a, s);
     if (rc) return rc;
-    hipLaunchKernelGGL((kb_update<TV>), dim3(a.nvb), dim3(VB), 0, s, seg, a.act,
+    hipLaunchKernelGGL((kb_update<TV>), dim3(a.nvb), dim3(BVB), 0, s, seg, a.act,
                        a.actvb, a.nact, NB, q, pa, pr, pb, st, x, p[q], w, ww, z, zz,
                        diag);
     VGPU_HIP(hipGetLastError());

@@ -30,7 +30,17 @@ constexpr int VB = 1024;  // threads per vector block
 __device__ __forceinline__ double block_sum(const double *__restrict__ parts,
                                             int n, double *red) {
   double s = 0.0;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) s += parts[i];
+  // eight loads in flight per step; the adds keep the sequential order
+  const int bd = blockDim.x;
+  int i = threadIdx.x;
+  for (; i + 7 * bd < n; i += 8 * bd) {
+    double v[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = parts[i + k * bd];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += v[k];
+  }
+  for (; i < n; i += bd) s += parts[i];
   for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
   __syncthreads();
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;

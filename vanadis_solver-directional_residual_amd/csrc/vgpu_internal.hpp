@@ -181,9 +181,10 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
                      hipStream_t s);
 // Owner operator over the active systems of a batch (units split evenly over
 // as.nu): for every active member m whose status is not done, y1 = A_m x1 (and
-// y2 = A_m^T x2) on the member's node range of the batch vectors.  dot (dual only, optional):
-// per-unit partials of y1 . x2 at dot[u0 + unit]; they depend on the tile
-// shape only, not on the grid.  zchunk: as owner_num_blocks.
+// y2 = A_m^T x2) on the member's node range of the batch vectors.  dot (dual
+// only, optional): partials of y1 . x2 at dot[u0 + i], one per unit of a
+// balanced launch (zchunk 0; they depend on the tile shape only) or one per
+// (chunk, tile) group of a z-chunk launch (zchunk > 0).
 template <typename T, typename TIO>
 int launch_owner_batch(const Member<T> *md, const ActiveSet &as, int tile,
                        int zchunk, bool robin, const SolveStatus *st,
@@ -191,9 +192,10 @@ int launch_owner_batch(const Member<T> *md, const ActiveSet &as, int tile,
                        double *dot, hipStream_t s);
 // Apply y1 = A x1 (and, if x2, y2 = A^T x2) with masking; TIO is the vector
 // type (double or float), T the element arithmetic type.  dot_part (optional,
-// owner mode, dual only): owner_units(g, op.tile) partials of y1 . x2, one per
-// owner unit.  op.done, if set, points at a SolveStatus (owner mode) or its
-// done flag (brick modes).  zchunk: as owner_num_blocks / brick z chunk.
+// owner mode, dual only): partials of y1 . x2 as launch_owner_batch (at most
+// owner_units(g, op.tile) of them).  op.done, if set, points at a SolveStatus
+// (owner mode) or its done flag (brick modes).  zchunk: 0 balanced launch,
+// > 0 z-chunk launch / brick z chunk.
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zchunk,
                  const TIO *x1, TIO *y1, const TIO *x2, TIO *y2,

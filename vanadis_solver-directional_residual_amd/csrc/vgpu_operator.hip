@@ -226,9 +226,10 @@ __device__ __forceinline__ void load_rec(Res r, int v, int s, Coef<T> &c) {
 // constants.  The next plane (values, Dirichlet flags) and the next layer's
 // element fields are loaded one layer ahead; the Dirichlet select is applied
 // when the plane is written to LDS.  The dual mode also reduces y1 . x2 over
-// each unit's owned nodes (dot[u]: per thread over the unit's planes in
-// order, then over the block in a fixed order), so the partials do not depend
-// on the grid.
+// each unit's owned nodes (per thread over the unit's planes in order, then
+// over the block in a fixed order): a balanced launch stores one partial per
+// unit (they do not depend on the grid), a z-chunk launch one per (chunk,
+// tile) group, the sum of the group's units in plane order.
 // ---------------------------------------------------------------------------
 // With DIR (dual only) the launch also does the BiCG direction update of the
 // iteration: x1, x2 are z, zz; the plane loads also read the previous
@@ -261,7 +262,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
   __shared__ T st1[2][4 * NT];
   __shared__ T st2[DUAL ? 2 : 1][DUAL ? 4 * NT : 1];
   __shared__ double red[2][NW];
-  __shared__ double rsum[NW];
+  __shared__ double rsum[4][NW];
   __shared__ int sflag[2];
 
   const int tid = threadIdx.x;
@@ -290,63 +291,40 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     return lo;
   };
 
-  // Block-wide sum of parts[0..n) in a fixed order (per thread strided, DPP
-  // within waves, waves in order); the result in every thread.
-  auto blk_sum = [&](const double *parts, int n) {
-    double v = 0.0;
-    for (int i = tid; i < n; i += NT) v += parts[i];
-    v = wave_sum_lane63(v);
-    if ((tid & 63) == 63) rsum[tid >> 6] = v;
+  // Block-wide sums of four partial arrays a[j][0..n) in one pass, each in a
+  // fixed order (per thread strided, DPP within waves, waves in order); the
+  // results in every thread.
+  auto blk_sum4 = [&](const double *const a[4], int n, double out[4]) {
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = tid; i < n; i += NT) {
+      double x[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) x[j] = a[j][i];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) v[j] += x[j];
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      v[j] = wave_sum_lane63(v[j]);
+      if ((tid & 63) == 63) rsum[j][tid >> 6] = v[j];
+    }
     __syncthreads();
-    double t = 0.0;
-    for (int w = 0; w < NW; ++w) t += rsum[w];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      double t = 0.0;
+      for (int w = 0; w < NW; ++w) t += rsum[j][w];
+      out[j] = t;
+    }
     __syncthreads();
-    return t;
   };
 
   // One march of member m: node planes k0..k1-1 of tile `tile` (element
   // layers k0-1..k1-1).  Every thread of the block takes the same path.
-  auto march = [&](int m, int tile, int k0, int k1) {
-    double beta = 0.0;
-    bool first = false;
-    if (DIR) {
-      // the status of the previous iteration, read once per block: one block
-      // of this launch may be writing it (its err, and done when this
-      // launch's test ends the solve, which every block then finds too)
-      if (tid == 0) {
-        sflag[0] = dd.stv[m].done;
-        sflag[1] = dd.stv[m].iter;
-      }
-      __syncthreads();
-      const int pdone = sflag[0], k = sflag[1] + 1;
-      __syncthreads();
-      if (pdone) return;
-      const Segment sg = BATCH ? md[m].seg : m0.seg;
-      const int q = dd.q;
-      const double rn2 = blk_sum(dd.pr + (q ^ 1) * dd.NB + sg.vb0, sg.nb);
-      const double bn2 = blk_sum(dd.pbn + sg.vb0, sg.nb);
-      const double err = sqrt(rn2 / bn2);
-      int done = 0;
-      if (err <= dd.tol) done = 1;
-      else if (k > dd.itmax) done = 2;
-      else if (!(err == err)) done = 3;
-      else if (err > dd.divtol) done = 4;
-      if (!done && k > 1) {
-        const double bknum = blk_sum(dd.pb + (q ^ 1) * dd.NB + sg.vb0, sg.nb);
-        const double bkden = blk_sum(dd.pb + q * dd.NB + sg.vb0, sg.nb);
-        beta = bknum / bkden;
-        if (!(beta == beta) || bkden == 0.0) done = 3;
-      }
-      if (tile == 0 && k0 == 0 && tid == 0) {  // one block per system
-        dd.stv[m].err = err;
-        dd.stv[m].bnrm2 = bn2;
-        if (done) dd.stv[m].done = done;
-      }
-      if (done) return;
-      first = k == 1;
-    } else if (st && st[m].done) {
-      return;
-    }
+  // gidx >= 0 (z-chunk launch): the dot partial of the whole march goes to
+  // slot gidx of the member's partials, the sum of its planes in order;
+  // otherwise one partial per (tile, plane) unit.
+  auto march = [&](int m, int tile, int k0, int k1, int64_t gidx) {
+    if (!DIR && st && st[m].done) return;
     const OpData<T> op = BATCH ? md[m].op : m0.op;
     const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
     const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
@@ -362,7 +340,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const Res rmk = make_res(op.mask, op.g.N);
     const Res rrc = make_res(op.rec, (int64_t)eplane * ez * SR);
     const Res rx1z = make_res(x1, 0);  // every access out of range: zeros
-    double *dm = want_dot ? dot + ub + (int64_t)tile * ngz : nullptr;
+    double *dm = want_dot ? dot + ub + (gidx >= 0 ? gidx : (int64_t)tile * ngz) : nullptr;
+    double gacc = 0.0;  // z-chunk launch: the march's plane sums in order
 
     const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
     const int gy0 = (tile / ntx) * (TY - 1) - 1;
@@ -397,7 +376,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     };
     const int vn0 = (pc0 >= 0 && owned_col(lx0, ly0)) ? vp0 : kOob;
     const int vn1 = (pc1 >= 0 && owned_col(lx1, ly1)) ? vp1 : kOob;
-    const TIO bt = TIO(beta);
+    // DIR: beta and the first-iteration flag, known after the status test
+    // below; dir() is only evaluated after it
+    TIO bt = TIO(0);
+    bool first = false;
     auto dir = [&](TIO zv, TIO pv) { return first ? zv : zv + bt * pv; };
 
     // this thread's two plane nodes of plane gz (raw values and flags; value
@@ -482,42 +464,94 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       const int pb = pend < 0 ? 0 : pend;
       double t = 0.0;
       for (int w = 0; w < NW; ++w) t += red[pb & 1][w];
-      if (pend >= 0 && tid == 0) dm[pend] = t;
+      if (gidx >= 0) {
+        if (pend >= 0) gacc += t;
+      } else if (pend >= 0 && tid == 0) {
+        dm[pend] = t;
+      }
       pend = -1;
     };
 
     // prologue: the bottom face of layer k0-1 (plane k0-1) straight from
-    // memory, plane k0 and the fields of layer k0-1 ahead
-    T fb1[4], fb2[4];
+    // memory, plane k0 and the fields of layer k0-1 ahead; the loads are
+    // issued before the status test so that they overlap its reductions
+    TIO ca[4], cb[4], cc[4], cd[4];
+    uint8_t cm[4];
     {
-      T n1[4], n2[4];
-      const int gz = k0 - 1;
-      const bool zin = gz >= 0;
+      const int gz = max(k0 - 1, 0);
+      const int sm = uni(gz * plane), sv = uni(gz * plane * SI);
+      const bool zin = k0 >= 1;
+      const Res ra = zin ? rx1 : rx1z, rb = zin ? rx2 : rx1z, rk = zin ? rmk : rx1z;
+      const Res rc = zin ? rpo : rx1z, rd = zin ? rppo : rx1z;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int col = colofs(exg + (c == 1 || c == 2), eyg + (c >= 2));
         const int vc = col >= 0 ? col : kOob, vcs = col >= 0 ? col * SI : kOob;
-        n1[c] = n2[c] = T(0);
-        if (zin) {
-          const int sm = uni(gz * plane), sv = uni(gz * plane * SI);
-          const uint8_t mk = bload<uint8_t>(rmk, vc, sm);
-          TIO a = bload<TIO>(rx1, vcs, sv);
-          if (DIR) a = dir(a, bload<TIO>(rpo, vcs, sv));
-          n1[c] = mk ? T(0) : T(a);
-          if (DUAL) {
-            TIO b = bload<TIO>(rx2, vcs, sv);
-            if (DIR) b = dir(b, bload<TIO>(rppo, vcs, sv));
-            n2[c] = mk ? T(0) : T(b);
-          }
-        }
+        cm[c] = bload<uint8_t>(rk, vc, sm);
+        ca[c] = bload<TIO>(ra, vcs, sv);
+        cb[c] = DUAL ? bload<TIO>(rb, vcs, sv) : TIO(0);
+        cc[c] = DIR ? bload<TIO>(rc, vcs, sv) : TIO(0);
+        cd[c] = DIR ? bload<TIO>(rd, vcs, sv) : TIO(0);
       }
-      face_fwd<T>(n1, fb1);
-      if (DUAL) face_fwd<T>(n2, fb2);
     }
     PlanePair<TIO, DIR> pr;
     plane_fetch(k0, pr);
     Coef<T> cn, cf;
     coef_fetch(k0 - 1, cn);
+
+    if (DIR) {
+      // the status of the previous iteration, read once per block: one block
+      // of this launch may be writing it (its err, and done when this
+      // launch's test ends the solve, which every block then finds too)
+      if (tid == 0) {
+        sflag[0] = dd.stv[m].done;
+        sflag[1] = dd.stv[m].iter;
+      }
+      const Segment sg = BATCH ? md[m].seg : m0.seg;
+      const int q = dd.q;
+      const double *const parts[4] = {dd.pr + (q ^ 1) * dd.NB + sg.vb0, dd.pbn + sg.vb0,
+                                      dd.pb + (q ^ 1) * dd.NB + sg.vb0,
+                                      dd.pb + q * dd.NB + sg.vb0};
+      double sums[4];
+      blk_sum4(parts, sg.nb, sums);  // its barriers also publish sflag
+      const int pdone = sflag[0], k = sflag[1] + 1;
+      __syncthreads();  // sflag is rewritten by the block's next march
+      if (pdone) return;
+      const double rn2 = sums[0], bn2 = sums[1];
+      const double err = sqrt(rn2 / bn2);
+      int done = 0;
+      if (err <= dd.tol) done = 1;
+      else if (k > dd.itmax) done = 2;
+      else if (!(err == err)) done = 3;
+      else if (err > dd.divtol) done = 4;
+      double beta = 0.0;
+      if (!done && k > 1) {
+        const double bknum = sums[2], bkden = sums[3];
+        beta = bknum / bkden;
+        if (!(beta == beta) || bkden == 0.0) done = 3;
+      }
+      if (tile == 0 && k0 == 0 && tid == 0) {  // one block per system
+        dd.stv[m].err = err;
+        dd.stv[m].bnrm2 = bn2;
+        if (done) dd.stv[m].done = done;
+      }
+      if (done) return;
+      first = k == 1;
+      bt = TIO(beta);
+    }
+    T fb1[4], fb2[4];
+    {
+      T n1[4], n2[4];
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const TIO a = DIR ? dir(ca[c], cc[c]) : ca[c];
+        const TIO b = DIR ? dir(cb[c], cd[c]) : cb[c];
+        n1[c] = cm[c] ? T(0) : T(a);
+        n2[c] = cm[c] ? T(0) : T(b);
+      }
+      face_fwd<T>(n1, fb1);
+      if (DUAL) face_fwd<T>(n2, fb2);
+    }
 
     T ot1[4], ot2[4];  // Haar-space top-face outputs of the previous layer
 #pragma unroll
@@ -654,7 +688,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       }
     }
     lds_barrier();  // buffers are refilled by the next march
-    if (want_dot) dot_flush();
+    if (want_dot) {
+      dot_flush();
+      if (gidx >= 0 && tid == 0) dm[0] = gacc;
+    }
   };
 
   if (zc > 0) {
@@ -669,7 +706,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const int64_t lg = lb - (BATCH ? actg[ia] : (int64_t)0);
     const int chunk = (int)(lg / nt), tile = (int)(lg - (int64_t)chunk * nt);
     const int k0 = chunk * zc;
-    if (k0 < g.naz) march(m, tile, k0, min(g.naz, k0 + zc));
+    if (k0 < g.naz) march(m, tile, k0, min(g.naz, k0 + zc), lg);
     return;
   }
   // balanced launch: block lb takes the units [lb U / G, (lb+1) U / G),
@@ -688,7 +725,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const int g0 = (int)(lu - (int64_t)tile * ngz);
     const int g1 = (int)min<int64_t>((int64_t)ngz, g0 + (uend - u));
     u += g1 - g0;
-    march(m, tile, g0 * kUnitPlanes, min(naz, g1 * kUnitPlanes));
+    march(m, tile, g0 * kUnitPlanes, min(naz, g1 * kUnitPlanes), -1);
   }
 }
 
