@@ -305,6 +305,7 @@ def main():
     t_dual = ctx.time_kernel(1, "owner", pkg.FP64, reps)
     t_iter = ctx.time_kernel(3, "owner", pkg.FP64, reps)
     t_copy = ctx.time_kernel(2, "owner", pkg.FP64, reps)
+    t_copy8 = ctx.time_kernel(4, "owner", pkg.FP64, 20)
     modes = {}
     for sc in ("owner", "colored", "atomic"):
         for prec, pn in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32")):
@@ -316,9 +317,10 @@ def main():
     iter_bytes = algorithmic_bytes(N, E, dual=True, fused=True)
     achieved = iter_bytes / (t_iter * 1e-3) / 1e9
     copy_gbs = 16.0 * N / (t_copy * 1e-3) / 1e9
+    copy8_gbs = 128.0 * N / (t_copy8 * 1e-3) / 1e9
     progress(f"apply {t_single * 1e3:.1f} us ({gdofs:.1f} GDOF/s), dual {t_dual * 1e3:.1f} us "
              f"({dual_gbs:.0f} GB/s), iteration launch {t_iter * 1e3:.1f} us "
-             f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s")
+             f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s (8 N: {copy8_gbs:.0f} GB/s)")
 
     batch = None
     if not a.no_batch:
@@ -371,6 +373,7 @@ def main():
         "ms_per_iteration_fp64": round(ms_step / iters[-1], 5),
         "scatter_modes": modes,
         "copy_GBps": round(copy_gbs, 1),
+        "copy_8N_GBps": round(copy8_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_owner<double,double,dual,direction>",

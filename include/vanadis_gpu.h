@@ -179,7 +179,9 @@ int vgpu_memcpy_d2h(vgpu_ctx *ctx, void *dst, const void *src_dev, int64_t bytes
  *       2 = device copy of N doubles (bandwidth calibration),
  *       3 = the owner launch of a BiCG iteration of the solver: convergence
  *           test, beta and p = z + beta p, pp = zz + beta pp fused into the
- *           dual apply (owner scatter only). */
+ *           dual apply (owner scatter only),
+ *       4 = device copy of 8 N doubles (the size of an iteration's working
+ *           set; buffers allocated for the call). */
 int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
                      int reps, double *avg_ms);
 

@@ -1476,8 +1476,8 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
 
 int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                      int reps, double *avg_ms) {
-  if (!c || !avg_ms || reps < 1 || what < 0 || what > 3) return VGPU_ERR_ARG;
-  if (what != 2 && !c->have_coeffs) return VGPU_ERR_STATE;
+  if (!c || !avg_ms || reps < 1 || what < 0 || what > 4) return VGPU_ERR_ARG;
+  if (what != 2 && what != 4 && !c->have_coeffs) return VGPU_ERR_STATE;
   if (what == 3 && scatter != VGPU_SCATTER_OWNER) return VGPU_ERR_ARG;
   const int64_t N = c->g.N;
   VGPU_HIP(hipSetDevice(c->device));
@@ -1554,8 +1554,18 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                                             c->tdir + 2 * N, c->tdir + 3 * N, c->tpart,
                                             c->stream);
   };
+  // what 4: a copy of 8 N doubles, the size class of a BiCG iteration's
+  // working set (buffers held for this call only)
+  double *big = nullptr;
+  if (what == 4) VGPU_HIP(hipMalloc(&big, 16 * N * sizeof(double)));
   auto launch = [&]() -> int {
     if (what == 3) return launch_dir();
+    if (what == 4) {
+      hipLaunchKernelGGL(k_copy, grid_for(8 * N, 256), dim3(256), 0, c->stream, 8 * N, big,
+                         big + 8 * N);
+      VGPU_HIP(hipGetLastError());
+      return VGPU_OK;
+    }
     if (what == 2) {
       hipLaunchKernelGGL(k_copy, grid_for(N, 256), dim3(256), 0, c->stream, N,
                          c->vx_, c->vb);
@@ -1601,6 +1611,7 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   VGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  if (big) VGPU_HIP(hipFree(big));
   *avg_ms = ms / reps;
   return VGPU_OK;
 }
