@@ -1496,13 +1496,16 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     f4 = reinterpret_cast<float *>(c->vt2);
   }
   // fused iteration launch: z, zz = vx_, vb; previous directions vt, vt2;
-  // new directions and operator outputs in tdir; a status that never stops
-  // (tol 0) and unit partials, so beta = 1 and every system runs
+  // new directions, operator outputs and x in tdir; a status that never stops
+  // (tol 0, alpha 0) and unit partials, so beta = 1 and every system runs
   const Segment sg = make_segments({N}, {owner_units(c->g, c->tile)})[0];
   const int64_t tail = 2048;
   if (what == 3) {
-    if (!c->tdir) VGPU_HIP(hipMalloc(&c->tdir, (4 * N + tail) * sizeof(double)));
-    double *parts = c->tdir + 4 * N;
+    if (!c->tdir) {
+      VGPU_HIP(hipMalloc(&c->tdir, (5 * N + tail) * sizeof(double)));
+      VGPU_HIP(hipMemset(c->tdir, 0, 5 * N * sizeof(double)));
+    }
+    double *parts = c->tdir + 5 * N;
     hipLaunchKernelGGL(k_fill, grid_for(tail), dim3(256), 0, c->stream, tail, 1.0, parts);
     SolveStatus s0{};
     s0.iter = 1;
@@ -1511,7 +1514,7 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     VGPU_HIP(hipStreamSynchronize(c->stream));
   }
   auto launch_dir = [&]() -> int {
-    double *parts = c->tdir + 4 * N;
+    double *parts = c->tdir + 5 * N;
     const int ns = (int)((sizeof(SolveStatus) + 7) / 8);
     if (precision == VGPU_FP32) {
       float *v = reinterpret_cast<float *>(c->tdir);
@@ -1520,6 +1523,7 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
       dd.ppo = f4;
       dd.pn = v;
       dd.ppn = v + N;
+      dd.x = v + 4 * N;
       dd.pr = dd.pb = dd.pbn = parts + ns;
       dd.stv = reinterpret_cast<SolveStatus *>(parts);
       dd.tol = 0.0;
@@ -1539,6 +1543,7 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     dd.ppo = c->vt2;
     dd.pn = c->tdir;
     dd.ppn = c->tdir + N;
+    dd.x = c->tdir + 4 * N;
     dd.pr = dd.pb = dd.pbn = parts + ns;
     dd.stv = reinterpret_cast<SolveStatus *>(parts);
     dd.tol = 0.0;

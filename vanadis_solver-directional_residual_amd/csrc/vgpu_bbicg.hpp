@@ -119,14 +119,12 @@ __device__ __forceinline__ void block_sum_pair(double u, double v, double (*red2
   __syncthreads();
 }
 
-// One node of K2: x += a p and the residual update r -= a w, rr -= a ww
-// carried on z = r/d, zz = rr/d (w = A p, ww = A^T pp); partials of r.r and
-// z.rr with r = d z, rr = d zz.
+// One node of K2: the residual update r -= a w, rr -= a ww carried on
+// z = r/d, zz = rr/d (w = A p, ww = A^T pp); partials of r.r and z.rr with
+// r = d z, rr = d zz.  (x += a p is applied by the next iteration launch.)
 template <typename TV>
-__device__ __forceinline__ void update_z(int64_t i, TV a, TV xi, TV pi, TV zi, TV zzi,
-                                         TV wi, TV wwi, TV di, TV *x, TV *z, TV *zz,
-                                         double &sr, double &sb) {
-  x[i] = xi + a * pi;
+__device__ __forceinline__ void update_z(int64_t i, TV a, TV zi, TV zzi, TV wi, TV wwi,
+                                         TV di, TV *z, TV *zz, double &sr, double &sb) {
   const TV zn = zi - a * (wi / di), zzn = zzi - a * (wwi / di);
   z[i] = zn;
   zz[i] = zzn;
@@ -136,8 +134,8 @@ __device__ __forceinline__ void update_z(int64_t i, TV a, TV xi, TV pi, TV zi, T
 }
 
 // K2: alpha = bknum/akden (akden = sum of the system's owner partials in
-// order); x += a p; z -= a w/d; zz -= a ww/d; partials of r.r and z.rr into
-// parity q.
+// order); z -= a w/d; zz -= a ww/d; partials of r.r and z.rr into parity q;
+// alpha to the status for the next iteration launch's x += alpha p.
 template <typename TV>
 __global__ __launch_bounds__(BVB) void kb_update(const Segment *__restrict__ seg,
                                                  VGPU_ACT_ARGS, int NB, int q,
@@ -145,19 +143,28 @@ __global__ __launch_bounds__(BVB) void kb_update(const Segment *__restrict__ seg
                                                  double *__restrict__ pr,
                                                  double *__restrict__ pb,
                                                  SolveStatus *__restrict__ stv,
-                                                 TV *__restrict__ x,
-                                                 const TV *__restrict__ p,
                                                  const TV *__restrict__ w,
                                                  const TV *__restrict__ ww,
                                                  TV *__restrict__ z, TV *__restrict__ zz,
                                                  const TV *__restrict__ diag) {
   __shared__ double red[BVB / 64];
   __shared__ double red2[2][BVB / 64];
+  __shared__ int sst[2];
   const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
   SolveStatus *st = stv + bo.m;
-  if (st->done) return;
   const Segment sg = seg[bo.m];
   const int j = bo.j;
+  // the status, read once per block (block 0 of the system may write it
+  // below); a result the iteration launch found becomes `done` here
+  if (threadIdx.x == 0) {
+    sst[0] = st->done;
+    sst[1] = st->pend;
+  }
+  __syncthreads();
+  if (sst[0] || sst[1]) {
+    if (!sst[0] && j == 0 && threadIdx.x == 0) st->done = sst[1];
+    return;
+  }
   const int64_t stride = (int64_t)sg.nb * BVB;
   const int tid = threadIdx.x;
   const int64_t i0 = j * (int64_t)BVB + tid;
@@ -179,14 +186,12 @@ __global__ __launch_bounds__(BVB) void kb_update(const Segment *__restrict__ seg
   for (int k = 0; k < KA; ++k) av[k] = tid + k * BVB < na ? av[k] : 0.0;
 #pragma unroll
   for (int k = 0; k < KB; ++k) bv[k] = tid + k * BVB < sg.nb ? bv[k] : 0.0;
-  TV xv[PF], pv[PF], zv[PF], zzv[PF], wv[PF], wwv[PF], dv[PF];
+  TV zv[PF], zzv[PF], wv[PF], wwv[PF], dv[PF];
 #pragma unroll
   for (int t = 0; t < PF; ++t) {
     const int64_t li = i0 + t * stride;
     const bool ok = li < sg.n;
     const int64_t i = sg.off + (ok ? li : 0);
-    xv[t] = ok ? x[i] : TV(0);
-    pv[t] = ok ? p[i] : TV(0);
     zv[t] = ok ? z[i] : TV(0);
     zzv[t] = ok ? zz[i] : TV(0);
     wv[t] = ok ? w[i] : TV(0);
@@ -214,16 +219,18 @@ __global__ __launch_bounds__(BVB) void kb_update(const Segment *__restrict__ seg
   for (int t = 0; t < PF; ++t) {
     const int64_t li = i0 + t * stride;
     if (li < sg.n)
-      update_z<TV>(sg.off + li, a, xv[t], pv[t], zv[t], zzv[t], wv[t], wwv[t], dv[t], x,
-                   z, zz, sr, sb);
+      update_z<TV>(sg.off + li, a, zv[t], zzv[t], wv[t], wwv[t], dv[t], z, zz, sr, sb);
   }
   for (int64_t li = i0 + PF * stride; li < sg.n; li += stride) {
     const int64_t i = sg.off + li;
-    update_z<TV>(i, a, x[i], p[i], z[i], zz[i], w[i], ww[i], diag[i], x, z, zz, sr, sb);
+    update_z<TV>(i, a, z[i], zz[i], w[i], ww[i], diag[i], z, zz, sr, sb);
   }
   block_partial_at(sr, pr + q * NB, sg.vb0 + j, red);
   block_partial_at(sb, pb + q * NB, sg.vb0 + j, red);
-  if (j == 0 && threadIdx.x == 0) st->iter = st->iter + 1;
+  if (j == 0 && threadIdx.x == 0) {
+    st->alpha = ak;
+    st->iter = st->iter + 1;
+  }
 }
 
 // Operator callbacks of the batched engine.  The direction-and-dual operator
@@ -551,6 +558,7 @@ struct BBicg {
     dd.ppo = pp[q ^ 1];
     dd.pn = p[q];
     dd.ppn = pp[q];
+    dd.x = x;
     dd.pr = pr;
     dd.pb = pb;
     dd.pbn = pbn;
@@ -563,8 +571,7 @@ struct BBicg {
     int rc = dual(z, zz, dd, w, ww, pa, a, s);
     if (rc) return rc;
     hipLaunchKernelGGL((kb_update<TV>), dim3(a.nvb), dim3(BVB), 0, s, seg, a.act,
-                       a.actvb, a.nact, NB, q, pa, pr, pb, st, x, p[q], w, ww, z, zz,
-                       diag);
+                       a.actvb, a.nact, NB, q, pa, pr, pb, st, w, ww, z, zz, diag);
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
   }

@@ -78,9 +78,14 @@ __device__ __forceinline__ int xcd_tile(int b, int nb) {
 struct SolveStatus {
   int done;   // 0 running, 1 converged, 2 itmax, 3 breakdown, 4 diverged
   int iter;   // completed iterations
+  int pend;   // batched engine: done code found by an iteration launch, made
+              // `done` by the update launch that follows (so no workgroup of
+              // the iteration launch sees it change)
+  int pad;
   double err; // ||r||/||b|| of the last test
   double bnrm2;
-  double pad;
+  double alpha;  // alpha of the last update (batched engine: x += alpha p
+                 // is applied by the next iteration launch)
 };
 
 // One system of a batch: its node range in the concatenated batch vectors,
@@ -161,11 +166,14 @@ bool owner_fits(const Grid3 &g);
 // previous iteration and beta from the reduction partials, then
 // p = z + beta po and pp = zz + beta ppo formed as the nodes are loaded
 // (iteration 1: p = z, pp = zz); the new directions of the nodes a tile owns
-// go to pn / ppn, the operator results to the launch's outputs.
+// go to pn / ppn, the operator results to the launch's outputs, and the
+// previous update's x += alpha po (alpha from the status) to x — also when
+// the test ends the solve, so x is complete when the system is done.
 template <typename TIO>
 struct DirData {
   const TIO *po = nullptr, *ppo = nullptr;  // previous directions
   TIO *pn = nullptr, *ppn = nullptr;        // new directions
+  TIO *x = nullptr;  // solution: x += alpha po of the previous update (owned nodes)
   const double *pr = nullptr, *pb = nullptr, *pbn = nullptr;  // r.r, z.rr, b.b partials
   SolveStatus *stv = nullptr;
   double tol = 0, divtol = 0;

@@ -229,7 +229,7 @@ __device__ __forceinline__ void load_rec(Res r, int v, int s, Coef<T> &c) {
 // each unit's owned nodes (per thread over the unit's planes in order, then
 // over the block in a fixed order): a balanced launch stores one partial per
 // unit (they do not depend on the grid), a z-chunk launch one per (chunk,
-// tile) group, the sum of the group's units in plane order.
+// tile) group (per thread over the group's planes, then over the block).
 // ---------------------------------------------------------------------------
 // With DIR (dual only) the launch also does the BiCG direction update of the
 // iteration: x1, x2 are z, zz; the plane loads also read the previous
@@ -240,6 +240,7 @@ template <typename TIO, bool DIR>
 struct PlanePair {
   TIO a0, a1, b0, b1;
   TIO c0, c1, d0, d1;  // DIR: previous directions of the two nodes
+  TIO e0, e1;          // DIR: x of the nodes the march owns (else 0)
   uint8_t m0, m1;
 };
 
@@ -264,6 +265,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
   __shared__ double red[2][NW];
   __shared__ double rsum[4][NW];
   __shared__ int sflag[2];
+  __shared__ double salpha;
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
@@ -341,7 +343,6 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const Res rrc = make_res(op.rec, (int64_t)eplane * ez * SR);
     const Res rx1z = make_res(x1, 0);  // every access out of range: zeros
     double *dm = want_dot ? dot + ub + (gidx >= 0 ? gidx : (int64_t)tile * ngz) : nullptr;
-    double gacc = 0.0;  // z-chunk launch: the march's plane sums in order
 
     const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
     const int gy0 = (tile / ntx) * (TY - 1) - 1;
@@ -376,6 +377,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     };
     const int vn0 = (pc0 >= 0 && owned_col(lx0, ly0)) ? vp0 : kOob;
     const int vn1 = (pc1 >= 0 && owned_col(lx1, ly1)) ? vp1 : kOob;
+    const Res rxs = make_res(DIR ? dd.x + off : y1 + off, op.g.N * SI);
+    TIO ap = TIO(0);  // DIR: alpha of the previous update (x += ap po)
     // DIR: beta and the first-iteration flag, known after the status test
     // below; dir() is only evaluated after it
     TIO bt = TIO(0);
@@ -396,7 +399,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       r.b0 = DUAL ? bload<TIO>(rb, vp0, sv) : TIO(0);
       r.c0 = DIR ? bload<TIO>(rc, vp0, sv) : TIO(0);
       r.d0 = DIR ? bload<TIO>(rd, vp0, sv) : TIO(0);
-      r.a1 = r.b1 = r.c1 = r.d1 = TIO(0);
+      // x of owned nodes of the march's own planes only
+      const Res re = gz < k1 ? rxs : rx1z;
+      r.e0 = DIR ? bload<TIO>(re, vn0, sv) : TIO(0);
+      r.a1 = r.b1 = r.c1 = r.d1 = r.e1 = TIO(0);
       r.m1 = 1;
       if (has1) {
         r.m1 = bload<uint8_t>(rk, vm1, sm);
@@ -405,6 +411,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
         if (DIR) {
           r.c1 = bload<TIO>(rc, vp1, sv);
           r.d1 = bload<TIO>(rd, vp1, sv);
+          r.e1 = bload<TIO>(re, vn1, sv);
         }
       }
     };
@@ -421,9 +428,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
           const int sv = uni(gz * plane * SI);
           bstore(a0, rpn, vn0, sv);
           bstore(b0, rppn, vn0, sv);
+          if (!first) bstore(r.e0 + ap * r.c0, rxs, vn0, sv);
           if (has1) {
             bstore(a1, rpn, vn1, sv);
             bstore(b1, rppn, vn1, sv);
+            if (!first) bstore(r.e1 + ap * r.c1, rxs, vn1, sv);
           }
         }
       }
@@ -449,9 +458,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     // red[unit & 1] and thread 0 stores it after the next barrier
     double dacc = 0.0;
     int pend = -1;  // unit whose block sum waits for the next barrier
+    // a z-chunk launch stores one partial per group: the per-thread sum over
+    // the march is reduced once at its end
     auto dot_emit = [&](double d, int k) {
       dacc += d;
-      if (k % kUnitPlanes == kUnitPlanes - 1 || k == naz - 1) {
+      if (gidx < 0 && (k % kUnitPlanes == kUnitPlanes - 1 || k == naz - 1)) {
         const double v = wave_sum_lane63(dacc);
         if ((tid & 63) == 63) red[(k / kUnitPlanes) & 1][tid >> 6] = v;
         dacc = 0.0;
@@ -461,14 +472,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     // every thread reads the wave sums (LDS broadcast, no branch around the
     // reads, so they issue with the layer's other LDS reads); thread 0 stores
     auto dot_flush = [&]() {
+      if (gidx >= 0) return;
       const int pb = pend < 0 ? 0 : pend;
       double t = 0.0;
       for (int w = 0; w < NW; ++w) t += red[pb & 1][w];
-      if (gidx >= 0) {
-        if (pend >= 0) gacc += t;
-      } else if (pend >= 0 && tid == 0) {
-        dm[pend] = t;
-      }
+      if (pend >= 0 && tid == 0) dm[pend] = t;
       pend = -1;
     };
 
@@ -500,12 +508,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     coef_fetch(k0 - 1, cn);
 
     if (DIR) {
-      // the status of the previous iteration, read once per block: one block
-      // of this launch may be writing it (its err, and done when this
-      // launch's test ends the solve, which every block then finds too)
+      // the status of the previous iteration, read once per block and
+      // broadcast (one block of this launch writes its err and pend; done,
+      // iter and alpha change only in the update launches)
       if (tid == 0) {
         sflag[0] = dd.stv[m].done;
         sflag[1] = dd.stv[m].iter;
+        salpha = dd.stv[m].alpha;
       }
       const Segment sg = BATCH ? md[m].seg : m0.seg;
       const int q = dd.q;
@@ -515,6 +524,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       double sums[4];
       blk_sum4(parts, sg.nb, sums);  // its barriers also publish sflag
       const int pdone = sflag[0], k = sflag[1] + 1;
+      ap = TIO(salpha);
       __syncthreads();  // sflag is rewritten by the block's next march
       if (pdone) return;
       const double rn2 = sums[0], bn2 = sums[1];
@@ -533,10 +543,25 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       if (tile == 0 && k0 == 0 && tid == 0) {  // one block per system
         dd.stv[m].err = err;
         dd.stv[m].bnrm2 = bn2;
-        if (done) dd.stv[m].done = done;
+        if (done) dd.stv[m].pend = done;
       }
-      if (done) return;
       first = k == 1;
+      if (done) {
+        // the solve ends here: complete x with the last update's alpha po
+        // on the nodes this march owns, then stop
+        if (!first) {
+          for (int gz = k0; gz < k1; ++gz) {
+            const int sv = uni(gz * plane * SI);
+            const TIO x0 = bload<TIO>(rxs, vn0, sv), p0 = bload<TIO>(rpo, vn0, sv);
+            bstore(x0 + ap * p0, rxs, vn0, sv);
+            if (has1) {
+              const TIO x1v = bload<TIO>(rxs, vn1, sv), p1v = bload<TIO>(rpo, vn1, sv);
+              bstore(x1v + ap * p1v, rxs, vn1, sv);
+            }
+          }
+        }
+        return;
+      }
       bt = TIO(beta);
     }
     T fb1[4], fb2[4];
@@ -689,8 +714,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     }
     lds_barrier();  // buffers are refilled by the next march
     if (want_dot) {
-      dot_flush();
-      if (gidx >= 0 && tid == 0) dm[0] = gacc;
+      if (gidx >= 0) {
+        const double v = wave_sum_lane63(dacc);
+        if ((tid & 63) == 63) red[0][tid >> 6] = v;
+        lds_barrier();
+        if (tid == 0) {
+          double t = 0.0;
+          for (int w = 0; w < NW; ++w) t += red[0][w];
+          dm[0] = t;
+        }
+      } else {
+        dot_flush();
+      }
     }
   };
 
