@@ -54,7 +54,10 @@ extern "C" {
 /* precisions */
 #define VGPU_FP64 0
 #define VGPU_FP32 1             /* FP32 element math, FP64 node accumulation  */
-#define VGPU_MIXED 2            /* solve: FP32 inner BiCG + FP64 refinement   */
+#define VGPU_MIXED 2            /* solve: FP32 inner BiCGSTAB + FP64 refinement */
+/* Krylov methods of vgpu_solve (FP64 / FP32 precisions) */
+#define VGPU_METHOD_BICG 0      /* Jacobi-BiCG, the reference's dbcg_simple    */
+#define VGPU_METHOD_BICGSTAB 1  /* Jacobi-BiCGSTAB (right preconditioned)      */
 
 typedef struct vgpu_ctx vgpu_ctx;
 
@@ -120,6 +123,14 @@ int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
  * shape (bitwise); the owner scatter needs 8 N and 40 E below 2^31 bytes
  * (else VGPU_ERR_ARG; such systems default to the coloured scatter). */
 int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk, int tile);
+/* Krylov method of the FP64 and FP32 solves (and of the FP64 passes of a
+ * mixed solve): VGPU_METHOD_BICG (default; the reference's method,
+ * VANADIS.FOR:977-1055) or VGPU_METHOD_BICGSTAB (van der Vorst's BiCGSTAB
+ * with the same Jacobi preconditioner and stopping rule, on the owner
+ * kernel's launches whatever the scatter mode; needs 8 N and 40 E below
+ * 2^31 bytes).  No counterpart in the reference: it adds a method with two
+ * single operator applies per iteration and no transposed apply. */
+int vgpu_set_method(vgpu_ctx *ctx, int method);
 
 /* y = (I-D) A (I-D) x  (transpose=1: A^T), host vectors of length N.
  * precision VGPU_FP64 or VGPU_FP32 (x,y still FP64 on the host). */
@@ -138,11 +149,14 @@ int vgpu_diag(vgpu_ctx *ctx, double *diag);
 int vgpu_assemble_rhs(vgpu_ctx *ctx, const double *q_in, const double *t_old,
                       double *b);
 /* Solve A x = b.  x is in/out (warm start, zeroed at masked nodes).
- * precision VGPU_FP64, VGPU_FP32 (FP32 vectors and operator, FP64 scalars)
- * or VGPU_MIXED (inner FP32 BiCG to inner_rtol, 0 = 1e-2, with FP64
- * true-residual refinement to rtol; if a refinement step fails to halve the
- * true residual the solve finishes in FP64 BiCG).  iters = total BiCG iterations,
- * err = final relative residual (recursive for FP64, true for MIXED). */
+ * precision VGPU_FP64, VGPU_FP32 (FP32 vectors and operator, FP64 scalars),
+ * both with the context's method (vgpu_set_method), or VGPU_MIXED (inner
+ * FP32 BiCGSTAB to inner_rtol, 0 = 1e-4, on r/|r| with FP64 true-residual
+ * refinement x += |r| dx to rtol; FP32 BiCG inside for systems too large for
+ * the owner launches; if a refinement step fails to halve the true residual
+ * the solve finishes in FP64 with the context's method).  iters = total
+ * Krylov iterations, err = final relative residual (recursive for FP64 and
+ * FP32, true for MIXED). */
 int vgpu_solve(vgpu_ctx *ctx, int precision, const double *b, double *x,
                double rtol, int itmax, double inner_rtol, int *iters,
                double *err);

@@ -153,6 +153,60 @@ def dbcg_simple(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
     return x, it.value, err.value, rc
 
 
+def bicgstab_jacobi(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
+    """Jacobi-BiCGSTAB, right preconditioned (van der Vorst 1992; the
+    preconditioned BiCGSTAB of Barrett et al., Templates, 1994, sec. 2.3.8 with
+    M = D), with the stopping rule of dbcg_simple (recursive ||r||/||b|| <= tol,
+    VANADIS.FOR:977-1055, cudatest.cu:491-553) tested before each iteration.
+    Not in the reference: the checker of the library's BiCGSTAB method.
+    Shadow residual r~ = r0.  Returns (x, iterations, err, status) with
+    status 1 converged, 2 itmax, 3 breakdown."""
+    A = lambda v: matvec_0(v, iss, bb, nax, nay, naz, mask)
+    d = np.asarray(diag, np.float64)
+    x = np.array(x0, np.float64, copy=True)
+    x[np.asarray(mask) == 1] = 0.0
+    r = np.asarray(b, np.float64) - A(x)
+    rt = r.copy()
+    bn = np.linalg.norm(b)
+    rho_old = alpha = omega = 1.0
+    p = np.zeros_like(r)
+    v = np.zeros_like(r)
+    k = 0
+    while True:
+        err = np.linalg.norm(r) / bn
+        if err <= tol:
+            return x, k, err, 1
+        if k >= itmax:
+            return x, k, err, 2
+        if not np.isfinite(err):
+            return x, k, err, 3
+        rho = rt @ r
+        if k == 0:
+            p = r.copy()
+        else:
+            beta = (rho / rho_old) * (alpha / omega)
+            if not np.isfinite(beta) or rho == 0.0 or omega == 0.0:
+                return x, k, err, 3
+            p = r + beta * (p - omega * v)
+        ph = p / d
+        v = A(ph)
+        gam = rt @ v
+        if gam == 0.0 or not np.isfinite(rho / gam):
+            return x, k, err, 3
+        alpha = rho / gam
+        s_ = r - alpha * v
+        sh = s_ / d
+        t = A(sh)
+        tt = t @ t
+        if tt == 0.0 or not np.isfinite((t @ s_) / tt):
+            return x, k, err, 3
+        omega = (t @ s_) / tt
+        x = x + alpha * ph + omega * sh
+        r = s_ - omega * t
+        rho_old = rho
+        k += 1
+
+
 def mass(iss, xx, yy, zz, c, nelem):
     return lib().orc_integrate_mass_hex8(nelem, iss, xx, yy, zz,
                                          np.ascontiguousarray(c, np.float64))

@@ -180,6 +180,37 @@ def test_bicg_converges_to_direct_solution(pkg, po):
     assert np.all(x[m] == 0.0)
 
 
+@pytest.mark.parametrize("dims,seed,lam", [((6, 5, 4), 5, 2e-2), ((9, 7, 5), 3, 3e-3)])
+def test_bicgstab_converges_to_direct_solution(pkg, po, dims, seed, lam):
+    """The oracle's Jacobi-BiCGSTAB reaches the direct solution of the masked
+    system, and its first iterates are those of scipy's preconditioned
+    BiCGSTAB (the same algorithm, Templates sec. 2.3.8) to rounding (BiCGSTAB
+    amplifies rounding differences from iteration to iteration; scipy also
+    stops half an iteration early, on |s|, so the counts are not compared)."""
+    p = pkg.problem.unit_box(*dims, vel_seed=seed, lam=lam)
+    bb, b, d = po.assemble(p)
+    b, d, x0 = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    x, it, err, st = po.bicgstab_jacobi(b, x0, 1e-12, 2000, bb, p.iss, d, p.nax, p.nay,
+                                        p.naz, p.mask)
+    assert st == 1 and err <= 1e-12
+    Dm = sp.diags(1.0 - p.mask)
+    Asys = (Dm @ csr_from_bb(p, bb) @ Dm + sp.diags(p.mask.astype(float))).tocsc()
+    xd = spla.spsolve(Asys, b)
+    assert np.linalg.norm(x - xd) / np.linalg.norm(xd) < 1e-9
+    assert np.all(x[p.mask == 1] == 0.0)
+    # scipy on the same system, M = D
+    M = spla.LinearOperator(Asys.shape, matvec=lambda v: v / d)
+    hist = []
+    spla.bicgstab(Asys, b, rtol=1e-30, atol=0.0, maxiter=5, M=M,
+                  callback=lambda xk: hist.append(xk.copy()))
+    assert len(hist) == 5
+    for k in (1, 2, 5):
+        xk, itk, _, _ = po.bicgstab_jacobi(b, x0, 1e-30, k, bb, p.iss, d, p.nax, p.nay,
+                                           p.naz, p.mask)
+        assert itk == k
+        assert np.linalg.norm(xk - hist[k - 1]) <= 1e-11 * np.linalg.norm(hist[k - 1])
+
+
 def test_theta_scheme_assembly(pkg, po):
     """BB = 2H + (3/dt)C and b = 3F - (H - (3/dt)C) T_old per element
     (VANADIS.FOR:723-748), checked against el3d outputs."""

@@ -19,6 +19,8 @@ SCATTER_OWNER, SCATTER_COLORED, SCATTER_ATOMIC = 0, 1, 2
 DR_V1, DR_V2, DR_V3 = 1, 2, 3
 SCHEME_THETA, SCHEME_STEADY, SCHEME_EULER = 0, 1, 2
 FP64, FP32, MIXED = 0, 1, 2
+METHOD_BICG, METHOD_BICGSTAB = 0, 1
+_METHOD_NAMES = {"bicg": METHOD_BICG, "bicgstab": METHOD_BICGSTAB}
 
 _SCATTER_NAMES = {"owner": SCATTER_OWNER, "colored": SCATTER_COLORED, "atomic": SCATTER_ATOMIC}
 
@@ -36,6 +38,7 @@ EXPORTED = (
     "vgpu_net_predict_dev", "vgpu_net_train_step_dev", "vgpu_net_reset_optimizer",
     "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
     "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
+    "vgpu_set_method",
 )
 
 
@@ -82,6 +85,7 @@ def load():
                                   C.c_double]
     L.vgpu_set_scheme.argtypes = [_vp, C.c_int, C.c_double]
     L.vgpu_set_options.argtypes = [_vp, C.c_int, C.c_int, C.c_int]
+    L.vgpu_set_method.argtypes = [_vp, C.c_int]
     L.vgpu_apply.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
     L.vgpu_apply_dev.argtypes = [_vp, C.c_int, C.c_int, _vp, _vp]
     L.vgpu_diag.argtypes = [_vp, _dp]
@@ -221,6 +225,12 @@ class Context:
         chunk); owner tile shape (-1 keep)."""
         check(load().vgpu_set_options(self.h, scatter_code(scatter), int(zchunk), int(tile)),
               "set_options")
+
+    def set_method(self, method="bicg"):
+        """Krylov method of the FP64 / FP32 solves: "bicg" (the reference's
+        dbcg_simple) or "bicgstab" (Jacobi-BiCGSTAB on the owner launches)."""
+        code = _METHOD_NAMES[method] if isinstance(method, str) else int(method)
+        check(load().vgpu_set_method(self.h, code), "set_method")
 
     def apply(self, x, scatter="owner", precision=FP64, transpose=False):
         y = np.zeros(self.N)

@@ -271,8 +271,40 @@ int solve_owner(vgpu_ctx *c, BBicg<TV> &eng, const TV *b, TV *x, const TV *diag,
                    iters, err, status, divtol);
 }
 
+// One system by BiCGSTAB on the owner launches (whatever the context's
+// scatter mode: the fused launches exist for the owner kernel only).
+template <typename TV>
+int solve_stab(vgpu_ctx *c, Stab<TV> &eng, const TV *b, TV *x, const TV *diag, double tol,
+               int itmax, int *iters, double *err, int *status, double divtol = 1e300) {
+  if (!owner_fits(c->g)) {
+    set_error("BiCGSTAB: system too large for the owner launches (8 N and 40 E must "
+              "stay below 2^31 bytes)");
+    return VGPU_ERR_ARG;
+  }
+  const int64_t na = c->zchunk > 0 ? owner_groups(c->g, c->tile, c->zchunk)
+                                   : owner_units(c->g, c->tile);
+  int rc = eng.alloc(c->g.N, (int)na);
+  if (rc) return rc;
+  StabSingleOp<TV> single = [c](const TV *xx, TV *y, hipStream_t s) {
+    OpData<TV> op = op_for<TV>(c);
+    return launch_apply<TV, TV>(op, VGPU_SCATTER_OWNER, c->zchunk, xx, y, nullptr, nullptr,
+                                nullptr, s);
+  };
+  StabOwnerOp<TV> owner = [c](int mode, const StabData<TV> &sd, const TV *r, TV *y,
+                              double *dot, hipStream_t s) {
+    OpData<TV> op = op_for<TV>(c);
+    return launch_owner_stab<TV, TV>(op, mode, c->zchunk, sd, r, y, dot, s);
+  };
+  const uint64_t key = (c->version << 8) | (uint64_t)(c->zchunk % 64) |
+                       (sizeof(TV) == 4 ? (1ull << 7) : 0);
+  return eng.solve(key, single, owner, c->mask, b, x, diag, tol, itmax, divtol, c->stream,
+                   iters, err, status);
+}
+
 int solve_fp64_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                    int itmax, int *iters, double *err, int *status) {
+  if (c->method == VGPU_METHOD_BICGSTAB)
+    return solve_stab<double>(c, c->stab64, b, x, c->diag, rtol, itmax, iters, err, status);
   const int scatter = c->scatter;
   if (scatter == VGPU_SCATTER_OWNER)
     return solve_owner<double>(c, c->bb64, b, x, c->diag, rtol, itmax, iters, err,
@@ -313,7 +345,10 @@ int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                      c->stream, N, x, c->fx);
   VGPU_HIP(hipGetLastError());
   int rc = VGPU_OK;
-  if (scatter == VGPU_SCATTER_OWNER) {
+  if (c->method == VGPU_METHOD_BICGSTAB) {
+    rc = solve_stab<float>(c, c->stab32, c->fb, c->fx, c->fdiag, rtol, itmax, iters, err,
+                           status);
+  } else if (scatter == VGPU_SCATTER_OWNER) {
     rc = solve_owner<float>(c, c->bb32, c->fb, c->fx, c->fdiag, rtol, itmax, iters,
                             err, status);
   } else {
@@ -338,7 +373,10 @@ int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
 // Inner iterations allowed per refinement step of the mixed solve.
 constexpr int kInnerCap = 1000;
 
-// Mixed precision: FP64 true residual + refinement, FP32 inner BiCG
+// Mixed precision: FP64 true residual + refinement, FP32 inner BiCGSTAB on
+// the owner launches (FP32 BiCG loses its recurrences across the large
+// residual peaks of advection-dominated systems; BiCGSTAB's residual stays
+// within a few orders of |b|), FP32 BiCG for systems too large for them
 int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
                     int itmax, double inner_rtol, int *iters, double *err,
                     int *status) {
@@ -430,12 +468,12 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
     VGPU_HIP(hipGetLastError());
     int it = 0;
     double e = 0;
-    // inner FP32 BiCG on the scaled residual: capped, and stopped if its
-    // recursive residual grows 100-fold (FP32 BiCG loses its recurrences
-    // after large residual peaks); a stopped inner solve is not applied
-    if (scatter == VGPU_SCATTER_OWNER)
-      rc = solve_owner<float>(c, c->bb32, c->fb, c->fx, c->fdiag, inner_rtol,
-                              kInnerCap, &it, &e, &st_inner, 1e2);
+    // inner FP32 solve on the scaled residual: capped, and stopped if its
+    // recursive residual grows 1e4-fold (BiCGSTAB) or 100-fold (BiCG); a
+    // stopped inner solve is not applied
+    if (owner_fits(c->g))
+      rc = solve_stab<float>(c, c->stab32, c->fb, c->fx, c->fdiag, inner_rtol, kInnerCap,
+                             &it, &e, &st_inner, 1e4);
     else
       rc = c->bicg32.solve(key, dual, single, c->mask, c->fb, c->fx, c->fdiag,
                            inner_rtol, kInnerCap, true, c->stream, &it, &e,
@@ -617,6 +655,8 @@ int vgpu_destroy(vgpu_ctx *c) {
   c->bicg32.release();
   c->bb64.release();
   c->bb32.release();
+  c->stab64.release();
+  c->stab32.release();
   free_dev(c->xc);
   free_dev(c->yc);
   free_dev(c->zc);
@@ -720,6 +760,19 @@ int vgpu_set_options(vgpu_ctx *c, int scatter, int zchunk, int tile) {
   if (zchunk >= 0) c->zchunk = zchunk;
   if (tile >= 0) c->tile = tile;
   c->version++;
+  return VGPU_OK;
+}
+
+int vgpu_set_method(vgpu_ctx *c, int method) {
+  if (!c || (method != VGPU_METHOD_BICG && method != VGPU_METHOD_BICGSTAB)) {
+    set_error("vgpu_set_method: method must be VGPU_METHOD_BICG or VGPU_METHOD_BICGSTAB");
+    return VGPU_ERR_ARG;
+  }
+  if (method == VGPU_METHOD_BICGSTAB && !owner_fits(c->g)) {
+    set_error("vgpu_set_method: BiCGSTAB runs on the owner launches; system too large");
+    return VGPU_ERR_ARG;
+  }
+  c->method = method;
   return VGPU_OK;
 }
 
@@ -957,7 +1010,7 @@ int vgpu_solve_dev(vgpu_ctx *c, int precision, const double *b, double *x,
     rc = solve_fp32_dev(c, b, x, rtol, itmax, iters, err, &status);
   else if (precision == VGPU_MIXED)
     rc = solve_mixed_dev(c, b, x, rtol, itmax,
-                         inner_rtol > 0 ? inner_rtol : 1e-2, iters, err, &status);
+                         inner_rtol > 0 ? inner_rtol : 1e-4, iters, err, &status);
   else {
     set_error("vgpu_solve: precision must be FP64, FP32 or MIXED");
     return VGPU_ERR_ARG;

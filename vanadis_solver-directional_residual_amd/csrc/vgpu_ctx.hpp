@@ -4,6 +4,7 @@
 #include <vector>
 
 #include "vgpu_bbicg.hpp"
+#include "vgpu_stab.hpp"
 
 using namespace vgpu;
 
@@ -37,6 +38,9 @@ struct vgpu_ctx {
   Bicg<float> bicg32;
   BBicg<double> bb64;   // owner scatter: batched engine, one system
   BBicg<float> bb32;
+  Stab<double> stab64;  // BiCGSTAB (owner launches)
+  Stab<float> stab32;
+  int method = VGPU_METHOD_BICG;
   uint64_t version = 1;
   std::vector<void *> user_allocs;
 

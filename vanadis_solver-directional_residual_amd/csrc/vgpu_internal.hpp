@@ -86,6 +86,8 @@ struct SolveStatus {
   double bnrm2;
   double alpha;  // alpha of the last update (batched engine: x += alpha p
                  // is applied by the next iteration launch)
+  double rho;    // BiCGSTAB: rho of the last iteration (r~ . r)
+  double omega;  // BiCGSTAB: omega of the last iteration
 };
 
 // One system of a batch: its node range in the concatenated batch vectors,
@@ -187,6 +189,41 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
                      bool batch, int tile, int zchunk, bool robin, const DirData<TIO> &dd,
                      const TIO *z, const TIO *zz, TIO *y1, TIO *y2, double *dot,
                      hipStream_t s);
+// Owner launches of one Jacobi-BiCGSTAB iteration (one system, owner
+// scatter), written for the scaled vectors R = D^-1 r, P = D^-1 p,
+// V = D^-1 v (D the diagonal, right preconditioning) and the shadow
+// RT = D r~ (so that r~ . r = RT . R):
+//  * kStabDir: the convergence test of the previous iteration and
+//    beta = (rho/rho_old)(alpha/omega) from the partials; the plane loads form
+//    P = R + beta (po - omega vo) (iteration 1: P = R), store it at the nodes
+//    the tile owns and apply the operator; output V = (A P)/d; dot partials
+//    of RT . V;
+//  * kStabS: alpha = rho / (RT . V) from the partials; the plane loads form
+//    S = R - alpha V; output t = A S (raw); dot partials of t . s (s = d S) and
+//    t . t.
+// The vector update between them (omega, x, R and the partials of RT . R and
+// |r|^2) is a vector kernel.
+enum StabMode { kStabDir = 2, kStabS = 3 };
+template <typename TIO>
+struct StabData {
+  const TIO *po = nullptr, *vo = nullptr;  // kStabDir: previous P, V
+  TIO *pn = nullptr;                       // kStabDir: new P (owned nodes)
+  const TIO *v = nullptr;                  // kStabS: V of this iteration
+  const TIO *d = nullptr;                  // diagonal (owned nodes)
+  const TIO *rt = nullptr;                 // kStabDir: shadow RT (owned nodes)
+  // vector-kernel partials [NB]: RT . R, |r|^2, |b|^2
+  const double *prt = nullptr, *prr = nullptr, *pbn = nullptr;
+  const double *pga = nullptr;  // kStabS: the kStabDir launch's RT . V partials [na]
+  double *dot2 = nullptr;       // kStabS: t . t partials (t . s go to `dot`)
+  SolveStatus *st = nullptr;
+  double tol = 0, divtol = 0;
+  int itmax = 0, NB = 0, na = 0;
+};
+// One BiCGSTAB owner launch (mode kStabDir or kStabS) on R (x1) into y1,
+// dot partials per unit (zchunk 0) or per (chunk, tile) group (zchunk > 0).
+template <typename T, typename TIO>
+int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<TIO> &sd,
+                      const TIO *r, TIO *y1, double *dot, hipStream_t s);
 // Owner operator over the active systems of a batch (units split evenly over
 // as.nu): for every active member m whose status is not done, y1 = A_m x1 (and
 // y2 = A_m^T x2) on the member's node range of the batch vectors.  dot (dual

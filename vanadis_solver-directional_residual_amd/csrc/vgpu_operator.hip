@@ -236,23 +236,34 @@ __device__ __forceinline__ void load_rec(Res r, int v, int s, Coef<T> &c) {
 // directions, form p = z + beta po, pp = zz + beta ppo (masked in LDS as the
 // operator's inputs), and the nodes of the tile's own columns and planes are
 // stored to pn, ppn.
+//
+// With kStabDir / kStabS (single operator) the launch is one of the two
+// operator launches of a BiCGSTAB iteration (see StabData): the plane loads
+// form P = R + beta (po - omega vo) or S = R - alpha V, and the owned node's
+// output is scaled or reduced with values loaded at that node.
 template <typename TIO, bool DIR>
 struct PlanePair {
   TIO a0, a1, b0, b1;
-  TIO c0, c1, d0, d1;  // DIR: previous directions of the two nodes
+  TIO c0, c1, d0, d1;  // DIR: previous directions of the two nodes; BiCGSTAB:
+                       // po, vo (kStabDir) or V (kStabS)
   TIO e0, e1;          // DIR: x of the nodes the march owns (else 0)
+  TIO f, g;            // BiCGSTAB: d and RT at the thread's owned node
   uint8_t m0, m1;
 };
 
-template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, bool DIR, int TX, int TY>
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int DM, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 : 4)))) void k_owner(
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
+                                                  : ((DUAL || DM >= 2) ? 3 : 4)))) void k_owner(
     Member<T> m0, const Member<T> *__restrict__ md, const int *__restrict__ act,
     const int64_t *__restrict__ actu, const int64_t *__restrict__ actg, int nact,
     int64_t U, int zc, const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
     TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
-    double *__restrict__ dot, DirData<TIO> dd) {
+    double *__restrict__ dot, DirData<TIO> dd, StabData<TIO> sd) {
+  constexpr bool DIR = DM == 1;  // BiCG iteration launch
+  constexpr bool SA = DM == kStabDir, SB = DM == kStabS, STAB = SA || SB;
   static_assert(DUAL || !DIR, "the direction update needs the dual operator");
+  static_assert(!(DUAL && STAB), "the BiCGSTAB launches use the single operator");
   constexpr int NP = (TX + 1) * (TY + 1);
   constexpr int NT = TX * TY;
   constexpr int NW = NT / 64;
@@ -263,14 +274,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
   __shared__ T st1[2][4 * NT];
   __shared__ T st2[DUAL ? 2 : 1][DUAL ? 4 * NT : 1];
   __shared__ double red[2][NW];
+  __shared__ double red2[SB ? 2 : 1][NW];
   __shared__ double rsum[4][NW];
   __shared__ int sflag[2];
   __shared__ double salpha;
+  __shared__ double sstab[3];  // BiCGSTAB: rho, alpha, omega of the status
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int64_t G = gridDim.x, lb = xcd_tile(blockIdx.x, gridDim.x);
-  const bool want_dot = DUAL && dot != nullptr;
+  const bool want_dot = (DUAL || STAB) && dot != nullptr;
 
   const int q0 = tid, q1 = tid + NT;
   const int lx0 = q0 % (TX + 1), ly0 = q0 / (TX + 1);
@@ -293,15 +306,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     return lo;
   };
 
-  // Block-wide sums of four partial arrays a[j][0..n) in one pass, each in a
+  // Block-wide sums of four partial arrays a[j][0..n[j]) in one pass, each in a
   // fixed order (per thread strided, DPP within waves, waves in order); the
   // results in every thread.
-  auto blk_sum4 = [&](const double *const a[4], int n, double out[4]) {
+  auto blk_sum4 = [&](const double *const a[4], const int n[4], double out[4]) {
     double v[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = tid; i < n; i += NT) {
+    const int nmax = max(max(n[0], n[1]), max(n[2], n[3]));
+    for (int i = tid; i < nmax; i += NT) {
       double x[4];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) x[j] = a[j][i];
+      for (int j = 0; j < 4; ++j) x[j] = i < n[j] ? a[j][i] : 0.0;
 #pragma unroll
       for (int j = 0; j < 4; ++j) v[j] += x[j];
     }
@@ -326,7 +340,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
   // slot gidx of the member's partials, the sum of its planes in order;
   // otherwise one partial per (tile, plane) unit.
   auto march = [&](int m, int tile, int k0, int k1, int64_t gidx) {
-    if (!DIR && st && st[m].done) return;
+    if (!DIR && !STAB && st && st[m].done) return;
     const OpData<T> op = BATCH ? md[m].op : m0.op;
     const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
     const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
@@ -342,7 +356,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const Res rmk = make_res(op.mask, op.g.N);
     const Res rrc = make_res(op.rec, (int64_t)eplane * ez * SR);
     const Res rx1z = make_res(x1, 0);  // every access out of range: zeros
-    double *dm = want_dot ? dot + ub + (gidx >= 0 ? gidx : (int64_t)tile * ngz) : nullptr;
+    const int64_t dslot = ub + (gidx >= 0 ? gidx : (int64_t)tile * ngz);
+    double *dm = want_dot ? dot + dslot : nullptr;
+    double *dm2 = (SB && want_dot) ? sd.dot2 + dslot : nullptr;
 
     const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
     const int gy0 = (tile / ntx) * (TY - 1) - 1;
@@ -368,9 +384,15 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const int vm0 = pc0 >= 0 ? pc0 : kOob, vm1 = pc1 >= 0 ? pc1 : kOob;
     // DIR: previous and new directions; a plane node of the tile's own node
     // columns stores its new direction (kOob elsewhere)
-    const Res rpo = make_res(DIR ? dd.po + off : x1 + off, op.g.N * SI);
-    const Res rppo = make_res(DIR ? dd.ppo + off : x1 + off, op.g.N * SI);
-    const Res rpn = make_res(DIR ? dd.pn + off : y1 + off, op.g.N * SI);
+    // BiCGSTAB: c = po (kStabDir) or V (kStabS), d = vo; new P of owned nodes
+    // to rpn; d and RT read at the owned node column vo
+    const TIO *cptr = DIR ? dd.po : SA ? sd.po : SB ? sd.v : x1;
+    const TIO *dptr = DIR ? dd.ppo : SA ? sd.vo : x1;
+    const Res rpo = make_res(cptr + off, op.g.N * SI);
+    const Res rppo = make_res(dptr + off, op.g.N * SI);
+    const Res rpn = make_res(DIR ? dd.pn + off : SA ? sd.pn + off : y1 + off, op.g.N * SI);
+    const Res rdg = make_res(STAB ? sd.d + off : x1 + off, op.g.N * SI);
+    const Res rrt = make_res(SA ? sd.rt + off : x1 + off, op.g.N * SI);
     const Res rppn = make_res(DIR ? dd.ppn + off : y1 + off, op.g.N * SI);
     auto owned_col = [&](int lx, int ly) {
       return lx >= 1 && lx <= TX - 1 && ly >= 1 && ly <= TY - 1;
@@ -380,10 +402,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     const Res rxs = make_res(DIR ? dd.x + off : y1 + off, op.g.N * SI);
     TIO ap = TIO(0);  // DIR: alpha of the previous update (x += ap po)
     // DIR: beta and the first-iteration flag, known after the status test
-    // below; dir() is only evaluated after it
-    TIO bt = TIO(0);
+    // below; dir() is only evaluated after it.  BiCGSTAB: beta, omega
+    // (kStabDir) or alpha (kStabS) and the first-iteration flag.
+    TIO bt = TIO(0), om = TIO(0), al = TIO(0);
     bool first = false;
     auto dir = [&](TIO zv, TIO pv) { return first ? zv : zv + bt * pv; };
+    // operator input of a plane node from its loads a (R), c, d
+    auto stab_in = [&](TIO a, TIO c, TIO d) {
+      if (SA) return first ? a : a + bt * (c - om * d);
+      return a - al * c;
+    };
 
     // this thread's two plane nodes of plane gz (raw values and flags; value
     // 0 outside the grid: a plane above the top loads through resources of
@@ -397,8 +425,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       r.m0 = bload<uint8_t>(rk, vm0, sm);
       r.a0 = bload<TIO>(ra, vp0, sv);
       r.b0 = DUAL ? bload<TIO>(rb, vp0, sv) : TIO(0);
-      r.c0 = DIR ? bload<TIO>(rc, vp0, sv) : TIO(0);
-      r.d0 = DIR ? bload<TIO>(rd, vp0, sv) : TIO(0);
+      r.c0 = (DIR || STAB) ? bload<TIO>(rc, vp0, sv) : TIO(0);
+      r.d0 = (DIR || SA) ? bload<TIO>(rd, vp0, sv) : TIO(0);
+      r.f = STAB ? bload<TIO>(in ? rdg : rx1z, vo, sv) : TIO(0);
+      r.g = SA ? bload<TIO>(in ? rrt : rx1z, vo, sv) : TIO(0);
       // x of owned nodes of the march's own planes only
       const Res re = gz < k1 ? rxs : rx1z;
       r.e0 = DIR ? bload<TIO>(re, vn0, sv) : TIO(0);
@@ -413,6 +443,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
           r.d1 = bload<TIO>(rd, vp1, sv);
           r.e1 = bload<TIO>(re, vn1, sv);
         }
+        if (STAB) r.c1 = bload<TIO>(rc, vp1, sv);
+        if (SA) r.d1 = bload<TIO>(rd, vp1, sv);
       }
     };
     // plane gz into LDS; DIR: the directions are formed first and those of
@@ -436,6 +468,15 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
           }
         }
       }
+      if (STAB) {
+        a0 = stab_in(r.a0, r.c0, r.d0);
+        a1 = stab_in(r.a1, r.c1, r.d1);
+        if (SA && gz >= k0 && gz < k1) {
+          const int sv = uni(gz * plane * SI);
+          bstore(a0, rpn, vn0, sv);
+          if (has1) bstore(a1, rpn, vn1, sv);
+        }
+      }
       p1[q0] = r.m0 ? T(0) : T(a0);
       if (DUAL) p2[q0] = r.m0 ? T(0) : T(b0);
       pk[q0] = r.m0;
@@ -456,16 +497,22 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     // dot terms: each thread sums its owned nodes of the current unit (its
     // planes in order); at the unit's last plane the block sum goes to
     // red[unit & 1] and thread 0 stores it after the next barrier
-    double dacc = 0.0;
+    // (kStabS: a second sum, t . t, in dacc2 / red2 / dm2)
+    double dacc = 0.0, dacc2 = 0.0;
     int pend = -1;  // unit whose block sum waits for the next barrier
     // a z-chunk launch stores one partial per group: the per-thread sum over
     // the march is reduced once at its end
-    auto dot_emit = [&](double d, int k) {
+    auto dot_emit = [&](double d, double d2, int k) {
       dacc += d;
+      if (SB) dacc2 += d2;
       if (gidx < 0 && (k % kUnitPlanes == kUnitPlanes - 1 || k == naz - 1)) {
         const double v = wave_sum_lane63(dacc);
         if ((tid & 63) == 63) red[(k / kUnitPlanes) & 1][tid >> 6] = v;
-        dacc = 0.0;
+        if (SB) {
+          const double v2 = wave_sum_lane63(dacc2);
+          if ((tid & 63) == 63) red2[(k / kUnitPlanes) & 1][tid >> 6] = v2;
+        }
+        dacc = dacc2 = 0.0;
         pend = k / kUnitPlanes;
       }
     };
@@ -474,9 +521,14 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
     auto dot_flush = [&]() {
       if (gidx >= 0) return;
       const int pb = pend < 0 ? 0 : pend;
-      double t = 0.0;
+      double t = 0.0, t2 = 0.0;
       for (int w = 0; w < NW; ++w) t += red[pb & 1][w];
-      if (pend >= 0 && tid == 0) dm[pend] = t;
+      if (SB)
+        for (int w = 0; w < NW; ++w) t2 += red2[pb & 1][w];
+      if (pend >= 0 && tid == 0) {
+        dm[pend] = t;
+        if (SB) dm2[pend] = t2;
+      }
       pend = -1;
     };
 
@@ -498,8 +550,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
         cm[c] = bload<uint8_t>(rk, vc, sm);
         ca[c] = bload<TIO>(ra, vcs, sv);
         cb[c] = DUAL ? bload<TIO>(rb, vcs, sv) : TIO(0);
-        cc[c] = DIR ? bload<TIO>(rc, vcs, sv) : TIO(0);
-        cd[c] = DIR ? bload<TIO>(rd, vcs, sv) : TIO(0);
+        cc[c] = (DIR || STAB) ? bload<TIO>(rc, vcs, sv) : TIO(0);
+        cd[c] = (DIR || SA) ? bload<TIO>(rd, vcs, sv) : TIO(0);
       }
     }
     PlanePair<TIO, DIR> pr;
@@ -522,7 +574,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
                                       dd.pb + (q ^ 1) * dd.NB + sg.vb0,
                                       dd.pb + q * dd.NB + sg.vb0};
       double sums[4];
-      blk_sum4(parts, sg.nb, sums);  // its barriers also publish sflag
+      const int ns[4] = {sg.nb, sg.nb, sg.nb, sg.nb};
+      blk_sum4(parts, ns, sums);  // its barriers also publish sflag
       const int pdone = sflag[0], k = sflag[1] + 1;
       ap = TIO(salpha);
       __syncthreads();  // sflag is rewritten by the block's next march
@@ -564,12 +617,81 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       }
       bt = TIO(beta);
     }
+    if (STAB) {
+      // the status of the previous launch, read once per block and broadcast
+      // (one block of this launch writes it; done is committed by the kStabS
+      // launch after a kStabDir launch found it, so no block of a launch sees
+      // it change)
+      SolveStatus *stm = sd.st;
+      if (tid == 0) {
+        sflag[0] = stm->done;
+        sflag[1] = SA ? stm->iter : stm->pend;
+        sstab[0] = stm->rho;
+        sstab[1] = stm->alpha;
+        sstab[2] = stm->omega;
+      }
+      const bool writer = tile == 0 && k0 == 0 && tid == 0;  // one block
+      double sums[4];
+      if (SA) {
+        const double *const parts[4] = {sd.prt, sd.prr, sd.pbn, sd.pbn};
+        const int ns[4] = {sd.NB, sd.NB, sd.NB, 0};
+        blk_sum4(parts, ns, sums);
+      } else {
+        const double *const parts[4] = {sd.prt, sd.pga, sd.prt, sd.prt};
+        const int ns[4] = {sd.NB, sd.na, 0, 0};
+        blk_sum4(parts, ns, sums);
+      }
+      const int pdone = sflag[0], fl = sflag[1];
+      const double rho_old = sstab[0], al_old = sstab[1], om_old = sstab[2];
+      __syncthreads();  // sflag is rewritten by the block's next march
+      if (pdone) return;
+      if (SA) {
+        const int k = fl + 1;
+        const double rho = sums[0];
+        const double err = sqrt(sums[1] / sums[2]);
+        int done = 0;
+        if (err <= sd.tol) done = 1;
+        else if (k > sd.itmax) done = 2;
+        else if (!(err == err)) done = 3;
+        else if (err > sd.divtol) done = 4;
+        double beta = 0.0;
+        if (!done && k > 1) {
+          beta = (rho / rho_old) * (al_old / om_old);
+          if (!(beta == beta) || rho == 0.0 || om_old == 0.0) done = 3;
+        }
+        if (writer) {
+          stm->err = err;
+          stm->bnrm2 = sums[2];
+          if (done) stm->pend = done;
+        }
+        if (done) return;
+        first = k == 1;
+        bt = TIO(beta);
+        om = TIO(om_old);
+      } else {
+        if (fl) {  // the kStabDir launch ended the solve
+          if (writer) stm->done = fl;
+          return;
+        }
+        const double rho = sums[0], gam = sums[1];
+        const double alpha = rho / gam;
+        if (!(alpha == alpha) || gam == 0.0) {
+          if (writer) stm->done = 3;
+          return;
+        }
+        if (writer) {
+          stm->rho = rho;
+          stm->alpha = alpha;
+        }
+        al = TIO(alpha);
+      }
+    }
     T fb1[4], fb2[4];
     {
       T n1[4], n2[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const TIO a = DIR ? dir(ca[c], cc[c]) : ca[c];
+        const TIO a = DIR ? dir(ca[c], cc[c]) : STAB ? stab_in(ca[c], cc[c], cd[c]) : ca[c];
         const TIO b = DIR ? dir(cb[c], cd[c]) : cb[c];
         n1[c] = cm[c] ? T(0) : T(a);
         n2[c] = cm[c] ? T(0) : T(b);
@@ -582,7 +704,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
 #pragma unroll
     for (int c = 0; c < 4; ++c) ot1[c] = ot2[c] = T(0);
     uint8_t omk[2] = {1, 1};    // flag of the owned node of the plane read 2 layers ago
-    double xok[2] = {0.0, 0.0};  // its x2 value (for the dot)
+    double xok[2] = {0.0, 0.0};  // its x2 value (for the dot; kStabS: its S)
+    TIO fok[2] = {TIO(0), TIO(0)}, gok[2] = {TIO(0), TIO(0)};  // BiCGSTAB: its d, RT
     int kl = k0 - 1;
 
     // output of plane kl-1 (summed from the stage written in the previous
@@ -601,20 +724,28 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
         }
       }
       const int so = uni((kl - 1) * plane * SI);
-      const TIO o1 = omk[P] ? TIO(0) : TIO(v1);
+      // kStabDir: V = (A P)/d (d is 0 only where the node is not stored)
+      const double dg = (double)fok[P];
+      const TIO o1 = omk[P] ? TIO(0) : SA ? TIO(dg != 0.0 ? v1 / dg : 0.0) : TIO(v1);
       bstore(o1, ry1, vo, so);
-      double d = 0.0;
+      double d = 0.0, d2 = 0.0;
       if (DUAL) {
         bstore(omk[P] ? TIO(0) : TIO(v2), ry2, vo, so);
         d = (double)o1 * xok[P];
       }
-      if (want_dot) dot_emit(d, kl - 1);
+      if (SA) d = (double)o1 * (double)gok[P];              // RT . V
+      if (SB) {
+        d = (double)o1 * (dg * xok[P]);                     // t . s, s = d S
+        d2 = (double)o1 * (double)o1;                       // t . t
+      }
+      if (want_dot) dot_emit(d, d2, kl - 1);
     };
 
     // one element layer kl with plane kl+1 in buffer P
     auto layer = [&](auto Pc) {
       constexpr int P = decltype(Pc)::value;
       plane_put(pr, kl + 1, pl1[P], DUAL ? pl2[DUAL ? P : 0] : nullptr, pm[P]);
+      const TIO fon = pr.f, gon = pr.g;  // owned node of plane kl+1
       cf = cn;
       plane_fetch(kl + 2, pr);  // prefetch plane kl+2 and layer kl+1
       coef_fetch(kl + 1, cn);
@@ -636,11 +767,14 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
         face_fwd<T>(xt, ft2);
       }
       const uint8_t omn = pm[P][oq];
-      const double xon = (DUAL && own) ? (double)pl2[DUAL ? P : 0][oq] : 0.0;
+      const double xon = (DUAL && own) ? (double)pl2[DUAL ? P : 0][oq]
+                         : (SB && own) ? (double)pl1[P][oq] : 0.0;
       // owner sums and output of plane kl-1
       if (kl - 1 >= k0) emit_plane(std::integral_constant<int, P ^ 1>(), Pc);
       omk[P] = omn;
       xok[P] = xon;
+      fok[P] = fon;
+      gok[P] = gon;
       // element layer kl; a layer outside the grid has zero fields and a
       // zero mass weight, so its result is zero
       const bool lv = kl >= 0 && kl < ez;
@@ -717,11 +851,20 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (DUAL ? 2 : 3) : (DUAL ? 3 :
       if (gidx >= 0) {
         const double v = wave_sum_lane63(dacc);
         if ((tid & 63) == 63) red[0][tid >> 6] = v;
+        if (SB) {
+          const double v2 = wave_sum_lane63(dacc2);
+          if ((tid & 63) == 63) red2[0][tid >> 6] = v2;
+        }
         lds_barrier();
         if (tid == 0) {
           double t = 0.0;
           for (int w = 0; w < NW; ++w) t += red[0][w];
           dm[0] = t;
+          if (SB) {
+            double t2 = 0.0;
+            for (int w = 0; w < NW; ++w) t2 += red2[0][w];
+            dm2[0] = t2;
+          }
         }
       } else {
         dot_flush();
@@ -954,9 +1097,9 @@ bool owner_fits(const Grid3 &g) {
 }
 
 namespace {
-template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, bool DIR, int TX, int TY>
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int DM, int TX, int TY>
 const void *owner_kernel() {
-  return reinterpret_cast<const void *>(&k_owner<T, TIO, DUAL, ROBIN, BATCH, DIR, TX, TY>);
+  return reinterpret_cast<const void *>(&k_owner<T, TIO, DUAL, ROBIN, BATCH, DM, TX, TY>);
 }
 
 // Resident workgroups of a kernel on the whole device (cached per kernel).
@@ -1002,46 +1145,51 @@ struct OwnerLaunch {
   TIO *y2;
   double *dot;
   DirData<TIO> dd;
+  StabData<TIO> sd;
 };
 
-template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, bool DIR, int TX, int TY>
+template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int DM, int TX, int TY>
 void launch_owner_k(const OwnerLaunch<T, TIO> &L, hipStream_t s) {
-  const void *fn = owner_kernel<T, TIO, DUAL, ROBIN, BATCH, DIR, TX, TY>();
+  const void *fn = owner_kernel<T, TIO, DUAL, ROBIN, BATCH, DM, TX, TY>();
   const int nb = owner_grid(fn, TX * TY, L.utot, L.gtot, L.zchunk);
-  hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, DIR, TX, TY>), dim3(nb),
+  hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, DM, TX, TY>), dim3(nb),
                      dim3(TX * TY), 0, s, L.m0, L.md, L.as.act, L.as.actu, L.as.actg,
                      L.as.nact, L.utot, L.zchunk > 0 ? L.zchunk : 0, L.st, L.x1, L.y1,
-                     L.x2, L.y2, L.dot, L.dd);
+                     L.x2, L.y2, L.dot, L.dd, L.sd);
 }
 
-template <typename T, typename TIO, bool BATCH, bool DIR, int TX, int TY>
+// DM: 0 plain operator, 1 BiCG iteration launch (dual), kStabDir / kStabS
+// BiCGSTAB launches (single)
+template <typename T, typename TIO, bool BATCH, int DM, int TX, int TY>
 void launch_owner_shape(const OwnerLaunch<T, TIO> &L, bool robin, hipStream_t s) {
   if (L.x2) {
+    if constexpr (DM <= 1) {
+      if (robin)
+        launch_owner_k<T, TIO, true, true, BATCH, DM, TX, TY>(L, s);
+      else
+        launch_owner_k<T, TIO, true, false, BATCH, DM, TX, TY>(L, s);
+    }
+  } else if constexpr (DM != 1) {
     if (robin)
-      launch_owner_k<T, TIO, true, true, BATCH, DIR, TX, TY>(L, s);
+      launch_owner_k<T, TIO, false, true, BATCH, DM, TX, TY>(L, s);
     else
-      launch_owner_k<T, TIO, true, false, BATCH, DIR, TX, TY>(L, s);
-  } else if (!DIR) {
-    if (robin)
-      launch_owner_k<T, TIO, false, true, BATCH, false, TX, TY>(L, s);
-    else
-      launch_owner_k<T, TIO, false, false, BATCH, false, TX, TY>(L, s);
+      launch_owner_k<T, TIO, false, false, BATCH, DM, TX, TY>(L, s);
   }
 }
 
 // utot units (balanced launch) or gtot groups (z-chunk launch, zchunk > 0)
-template <typename T, typename TIO, bool BATCH, bool DIR = false>
+template <typename T, typename TIO, bool BATCH, int DM = 0>
 int launch_owner_any(const OwnerLaunch<T, TIO> &L, int tile, bool robin, hipStream_t s) {
   if (L.zchunk > 0 ? L.gtot <= 0 : L.utot <= 0) return VGPU_OK;
   switch (tile) {
     case 1:
-      launch_owner_shape<T, TIO, BATCH, DIR, 16, 16>(L, robin, s);
+      launch_owner_shape<T, TIO, BATCH, DM, 16, 16>(L, robin, s);
       break;
     case 2:
-      launch_owner_shape<T, TIO, BATCH, DIR, 64, 4>(L, robin, s);
+      launch_owner_shape<T, TIO, BATCH, DM, 64, 4>(L, robin, s);
       break;
     default:
-      launch_owner_shape<T, TIO, BATCH, DIR, 32, 8>(L, robin, s);
+      launch_owner_shape<T, TIO, BATCH, DM, 32, 8>(L, robin, s);
       break;
   }
   VGPU_HIP(hipGetLastError());
@@ -1097,7 +1245,7 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
     L.as = as;
     L.utot = as.nu;
     L.gtot = as.ng;
-    return launch_owner_any<T, TIO, true, true>(L, tile, robin, s);
+    return launch_owner_any<T, TIO, true, 1>(L, tile, robin, s);
   }
   if (!owner_fits(m0.op.g)) {
     set_error("owner scatter: system too large for 32-bit offsets");
@@ -1106,7 +1254,7 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
   L.m0 = m0;
   L.utot = m0.seg.nu;
   L.gtot = owner_groups(m0.op.g, tile, zchunk);
-  return launch_owner_any<T, TIO, false, true>(L, tile, robin, s);
+  return launch_owner_any<T, TIO, false, 1>(L, tile, robin, s);
 }
 
 template int launch_owner_dir<double, double>(const Member<double> &, const Member<double> *,
@@ -1119,6 +1267,41 @@ template int launch_owner_dir<float, float>(const Member<float> &, const Member<
                                             const DirData<float> &, const float *,
                                             const float *, float *, float *, double *,
                                             hipStream_t);
+
+template <typename T, typename TIO>
+int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<TIO> &sd,
+                      const TIO *r, TIO *y1, double *dot, hipStream_t s) {
+  const Grid3 &g = op.g;
+  if (!owner_fits(g)) {
+    set_error("owner scatter: system too large for 32-bit offsets");
+    return VGPU_ERR_ARG;
+  }
+  if (mode != kStabDir && mode != kStabS) {
+    set_error("BiCGSTAB launch: unknown mode");
+    return VGPU_ERR_ARG;
+  }
+  OwnerLaunch<T, TIO> L{};
+  L.m0.op = op;
+  L.m0.seg.n = g.N;
+  L.m0.seg.nu = owner_units(g, op.tile);
+  L.utot = L.m0.seg.nu;
+  L.gtot = owner_groups(g, op.tile, zchunk);
+  L.zchunk = zchunk;
+  L.x1 = r;
+  L.y1 = y1;
+  L.dot = dot;
+  L.sd = sd;
+  if (mode == kStabDir)
+    return launch_owner_any<T, TIO, false, kStabDir>(L, op.tile, op.ra != 0.0, s);
+  return launch_owner_any<T, TIO, false, kStabS>(L, op.tile, op.ra != 0.0, s);
+}
+
+template int launch_owner_stab<double, double>(const OpData<double> &, int, int,
+                                               const StabData<double> &, const double *,
+                                               double *, double *, hipStream_t);
+template int launch_owner_stab<float, float>(const OpData<float> &, int, int,
+                                             const StabData<float> &, const float *,
+                                             float *, double *, hipStream_t);
 
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,

@@ -1,0 +1,372 @@
+// Jacobi-preconditioned BiCGSTAB (van der Vorst 1992, right preconditioning
+// by the diagonal D) for one system on the owner operator, with FP64
+// reductions and scalars whatever the vector type TV.  The reference solves
+// with BiCG (dbcg_simple, VANADIS.FOR:977-1055); BiCGSTAB is the inner
+// method of the mixed-precision solve and an optional method of the FP64 and
+// FP32 solves, with the reference's stopping rule (recursive
+// ||r||/||b|| <= tol, cudatest.cu:491-553).
+//
+// The recurrences run on the scaled vectors R = D^-1 r, P = D^-1 p,
+// V = D^-1 A P, S = D^-1 s and the shadow RT = D r0 (r0 the initial
+// residual, so r0 . r = RT . R):
+//   rho = RT . R;  beta = (rho / rho_old)(alpha / omega)
+//   P = R + beta (P - omega V);  V = D^-1 A P;  alpha = rho / (RT . V)
+//   S = R - alpha V;  t = A S;  omega = (t . s) / (t . t), s = D S
+//   x += alpha P + omega S;  R = S - omega D^-1 t;  |r|^2 = sum (d R)^2
+// One iteration = the kStabDir owner launch (test, beta, P formed as the
+// nodes are loaded, V, RT . V partials), the kStabS owner launch (alpha,
+// S formed on load, t, t . s and t . t partials) and ks_update; pairs of
+// iterations are captured in one hipGraph and replayed until the status
+// reports done.  Directions P and V alternate between two buffers by
+// iteration parity: a launch reads the previous ones at its tiles' halo
+// while it stores the new ones.
+#pragma once
+#include <algorithm>
+#include <functional>
+
+#include "vgpu_bicg.hpp"
+
+namespace vgpu {
+
+constexpr int SVB = 256;   // threads of the vector workgroups
+constexpr int SNB = 1024;  // at most this many vector workgroups (= partials)
+
+inline int stab_blocks(int64_t n) {
+  return (int)std::min<int64_t>(SNB, std::max<int64_t>(1, (n + 4 * SVB - 1) / (4 * SVB)));
+}
+
+// Block sums of two per-thread values (shuffles within waves, waves in
+// order), written by thread 0 to a[slot], b[slot].
+__device__ __forceinline__ void stab_partial2(double u, double v, double *a, double *b,
+                                              int slot) {
+  __shared__ double red[2][SVB / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    u += __shfl_down(u, off, 64);
+    v += __shfl_down(v, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = u;
+    red[1][threadIdx.x >> 6] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double su = 0.0, sv = 0.0;
+    for (int w = 0; w < SVB / 64; ++w) {
+      su += red[0][w];
+      sv += red[1][w];
+    }
+    a[slot] = su;
+    b[slot] = sv;
+  }
+}
+
+// r = b - A x0 (ax = A x0), R = r / d, RT = d r (shadow r0 = r); x zeroed on
+// the mask; partials of RT . R, |r|^2 and |b|^2.
+template <typename TV>
+__global__ __launch_bounds__(SVB) void ks_init(int64_t n, const uint8_t *__restrict__ mask,
+                                              const TV *__restrict__ b, TV *__restrict__ x,
+                                              const TV *__restrict__ ax,
+                                              const TV *__restrict__ diag, TV *__restrict__ R,
+                                              TV *__restrict__ RT, double *__restrict__ prt,
+                                              double *__restrict__ prr,
+                                              double *__restrict__ pbn) {
+  __shared__ double red[SVB / 64];
+  double srt = 0, sr = 0, sb = 0;
+  const int64_t stride = (int64_t)gridDim.x * SVB;
+  for (int64_t i = blockIdx.x * (int64_t)SVB + threadIdx.x; i < n; i += stride) {
+    if (mask[i]) x[i] = TV(0);
+    const TV bi = b[i], di = diag[i];
+    const TV ri = bi - ax[i];
+    const TV Ri = ri / di, RTi = ri * di;
+    R[i] = Ri;
+    RT[i] = RTi;
+    srt += (double)RTi * (double)Ri;
+    sr += (double)ri * (double)ri;
+    sb += (double)bi * (double)bi;
+  }
+  stab_partial2(srt, sr, prt, prr, blockIdx.x);
+  for (int off = 32; off > 0; off >>= 1) sb += __shfl_down(sb, off, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sb;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < SVB / 64; ++w) t += red[w];
+    pbn[blockIdx.x] = t;
+  }
+}
+
+// omega = (t . s) / (t . t) from the kStabS launch's partials (na of each),
+// then S = R - alpha V, x += alpha P + omega S, R = S - omega t / d and the
+// partials of RT . R and |r|^2 = (d R)^2.  alpha comes from the status
+// (written by the kStabS launch); one workgroup writes omega and the count.
+template <typename TV>
+__global__ __launch_bounds__(SVB) void ks_update(
+    int64_t n, SolveStatus *__restrict__ st, int na, const double *__restrict__ pts,
+    const double *__restrict__ ptt, TV *__restrict__ R, const TV *__restrict__ V,
+    const TV *__restrict__ P, const TV *__restrict__ t, const TV *__restrict__ diag,
+    const TV *__restrict__ RT, TV *__restrict__ x, double *__restrict__ prt,
+    double *__restrict__ prr) {
+  __shared__ double red2[2][SVB / 64];
+  __shared__ int sdone;
+  __shared__ double salpha;
+  const int tid = threadIdx.x;
+  if (tid == 0) {
+    sdone = st->done;
+    salpha = st->alpha;
+  }
+  // the partials, per thread in index order, then over the block
+  double a = 0.0, c = 0.0;
+  for (int i = tid; i < na; i += SVB) {
+    a += pts[i];
+    c += ptt[i];
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    c += __shfl_down(c, off, 64);
+  }
+  if ((tid & 63) == 0) {
+    red2[0][tid >> 6] = a;
+    red2[1][tid >> 6] = c;
+  }
+  __syncthreads();
+  if (sdone) return;
+  double ts = 0.0, tt = 0.0;
+  for (int w = 0; w < SVB / 64; ++w) {
+    ts += red2[0][w];
+    tt += red2[1][w];
+  }
+  const double omega = ts / tt;
+  const bool w0 = blockIdx.x == 0 && tid == 0;
+  if (!(omega == omega) || tt == 0.0) {
+    if (w0) st->done = 3;
+    return;
+  }
+  const TV al = TV(salpha), om = TV(omega);
+  double sr = 0.0, sn = 0.0;
+  const int64_t stride = (int64_t)gridDim.x * SVB;
+  for (int64_t i = blockIdx.x * (int64_t)SVB + tid; i < n; i += stride) {
+    const TV di = diag[i];
+    const TV Si = R[i] - al * V[i];
+    x[i] = x[i] + al * P[i] + om * Si;
+    const TV Rn = Si - om * (t[i] / di);
+    R[i] = Rn;
+    const double rn = (double)Rn * (double)di;
+    sr += (double)RT[i] * (double)Rn;
+    sn += rn * rn;
+  }
+  stab_partial2(sr, sn, prt, prr, blockIdx.x);
+  if (w0) {
+    st->omega = omega;
+    st->iter = st->iter + 1;
+  }
+}
+
+// Operator callbacks: y = A x of the system (ignores the status), and one
+// BiCGSTAB owner launch (mode kStabDir or kStabS).
+template <typename TV>
+using StabSingleOp = std::function<int(const TV *x, TV *y, hipStream_t s)>;
+template <typename TV>
+using StabOwnerOp = std::function<int(int mode, const StabData<TV> &sd, const TV *r, TV *y,
+                                      double *dot, hipStream_t s)>;
+
+template <typename TV>
+struct Stab {
+  int64_t n = 0;
+  int NB = 0, na = 0;
+  TV *ax = nullptr, *R = nullptr, *RT = nullptr, *t = nullptr;
+  TV *P[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr};
+  double *prt = nullptr, *prr = nullptr, *pbn = nullptr;
+  double *pga = nullptr, *pts = nullptr, *ptt = nullptr;
+  SolveStatus *st = nullptr, *st_host = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph = nullptr;
+  int pairs_per_graph = 4;
+
+  int cap = 0;  // capacity of the owner partial arrays
+
+  // n nodes, na owner dot partials per launch (the launches' count: units or
+  // (chunk, tile) groups)
+  int alloc(int64_t nn, int nna) {
+    nna = std::max(1, nna);
+    if (R && nn == n && nna <= cap) {
+      if (nna != na) drop_graph();
+      na = nna;
+      return VGPU_OK;
+    }
+    release();
+    n = nn;
+    NB = stab_blocks(n);
+    na = cap = nna;
+    TV **v[] = {&ax, &R, &RT, &t, &P[0], &P[1], &V[0], &V[1]};
+    for (auto pv : v) {
+      VGPU_HIP(hipMalloc(pv, std::max<int64_t>(1, n) * sizeof(TV)));
+      VGPU_HIP(hipMemset(*pv, 0, std::max<int64_t>(1, n) * sizeof(TV)));
+    }
+    double **pn[] = {&prt, &prr, &pbn};
+    for (auto pd : pn) VGPU_HIP(hipMalloc(pd, NB * sizeof(double)));
+    double **pa[] = {&pga, &pts, &ptt};
+    for (auto pd : pa) VGPU_HIP(hipMalloc(pd, cap * sizeof(double)));
+    VGPU_HIP(hipMalloc(&st, sizeof(SolveStatus)));
+    VGPU_HIP(hipHostMalloc(&st_host, sizeof(SolveStatus), hipHostMallocDefault));
+    return VGPU_OK;
+  }
+  void drop_graph() {
+    if (gexec) hipGraphExecDestroy(gexec);
+    if (graph) hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+    key_valid = false;
+  }
+  void release() {
+    drop_graph();
+    TV **v[] = {&ax, &R, &RT, &t, &P[0], &P[1], &V[0], &V[1]};
+    for (auto pv : v)
+      if (*pv) {
+        hipFree(*pv);
+        *pv = nullptr;
+      }
+    double **d[] = {&prt, &prr, &pbn, &pga, &pts, &ptt};
+    for (auto pd : d)
+      if (*pd) {
+        hipFree(*pd);
+        *pd = nullptr;
+      }
+    if (st) hipFree(st);
+    st = nullptr;
+    if (st_host) hipHostFree(st_host);
+    st_host = nullptr;
+    n = 0;
+    NB = na = cap = 0;
+  }
+  ~Stab() { release(); }
+
+  // Solve A x = b (device vectors; x in/out, zeroed on the mask).  The graph
+  // is reused while op_key, the vectors, tol, itmax and divtol are unchanged.
+  int solve(uint64_t op_key, const StabSingleOp<TV> &single, const StabOwnerOp<TV> &owner,
+            const uint8_t *mask, const TV *b, TV *x, const TV *diag, double tol, int itmax,
+            double divtol, hipStream_t s, int *iters, double *err, int *status) {
+    int rc = single(x, ax, s);  // A x0 (x0 masked on load)
+    if (rc) return rc;
+    hipLaunchKernelGGL((ks_init<TV>), dim3(NB), dim3(SVB), 0, s, n, mask, b, x, ax, diag,
+                       R, RT, prt, prr, pbn);
+    VGPU_HIP(hipGetLastError());
+    SolveStatus s0{};
+    s0.rho = s0.alpha = s0.omega = 1.0;
+    VGPU_HIP(hipMemcpyAsync(st, &s0, sizeof(SolveStatus), hipMemcpyHostToDevice, s));
+    const bool same = key_valid && key_op == op_key && key_b == b && key_x == x &&
+                      key_diag == diag && key_tol == tol && key_itmax == itmax &&
+                      key_div == divtol;
+    if (!same) {
+      rc = capture(op_key, owner, diag, b, x, tol, itmax, divtol, s);
+      if (rc) return rc;
+    }
+    // replay until done, at most two replays in flight
+    const int max_replays = itmax / (2 * pairs_per_graph) + 2;
+    hipEvent_t ev[2];
+    VGPU_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
+    VGPU_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
+    int result = VGPU_OK, launched = 0, completed = 0;
+    for (;;) {
+      while (launched - completed < 2 && launched < max_replays) {
+        if (hipGraphLaunch(gexec, s) != hipSuccess ||
+            hipMemcpyAsync(st_host, st, sizeof(SolveStatus), hipMemcpyDeviceToHost, s) !=
+                hipSuccess ||
+            hipEventRecord(ev[launched & 1], s) != hipSuccess) {
+          set_error("BiCGSTAB graph replay failed");
+          result = VGPU_ERR_HIP;
+          break;
+        }
+        ++launched;
+      }
+      if (result || completed == launched) break;
+      if (hipEventSynchronize(ev[completed & 1]) != hipSuccess) {
+        set_error("BiCGSTAB event wait failed");
+        result = VGPU_ERR_HIP;
+        break;
+      }
+      ++completed;
+      if (st_host->done) break;
+    }
+    if (hipStreamSynchronize(s) != hipSuccess && !result) {
+      set_error("BiCGSTAB stream sync failed");
+      result = VGPU_ERR_HIP;
+    }
+    hipEventDestroy(ev[0]);
+    hipEventDestroy(ev[1]);
+    if (result) return result;
+    *iters = st_host->iter;
+    *err = st_host->err;
+    *status = st_host->done;
+    return VGPU_OK;
+  }
+
+ private:
+  bool key_valid = false;
+  uint64_t key_op = 0;
+  const void *key_b = nullptr, *key_x = nullptr, *key_diag = nullptr;
+  double key_tol = 0, key_div = 0;
+  int key_itmax = 0;
+
+  int capture(uint64_t op_key, const StabOwnerOp<TV> &owner, const TV *diag, const TV *b,
+              TV *x, double tol, int itmax, double divtol, hipStream_t s) {
+    drop_graph();
+    VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int crc = VGPU_OK;
+    for (int pi = 0; pi < pairs_per_graph && crc == VGPU_OK; ++pi)
+      for (int q = 1; q >= 0 && crc == VGPU_OK; --q)
+        crc = enqueue_iteration(owner, diag, x, tol, itmax, divtol, q, s);
+    hipError_t ce = hipStreamEndCapture(s, &graph);
+    if (crc) {
+      drop_graph();
+      return crc;
+    }
+    VGPU_HIP(ce);
+    VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    key_valid = true;
+    key_op = op_key;
+    key_b = b;
+    key_x = x;
+    key_diag = diag;
+    key_tol = tol;
+    key_itmax = itmax;
+    key_div = divtol;
+    return VGPU_OK;
+  }
+
+  int enqueue_iteration(const StabOwnerOp<TV> &owner, const TV *diag, TV *x, double tol,
+                        int itmax, double divtol, int q, hipStream_t s) {
+    StabData<TV> sa;
+    sa.po = P[q ^ 1];
+    sa.vo = V[q ^ 1];
+    sa.pn = P[q];
+    sa.d = diag;
+    sa.rt = RT;
+    sa.prt = prt;
+    sa.prr = prr;
+    sa.pbn = pbn;
+    sa.st = st;
+    sa.tol = tol;
+    sa.divtol = divtol;
+    sa.itmax = itmax;
+    sa.NB = NB;
+    sa.na = na;
+    int rc = owner(kStabDir, sa, R, V[q], pga, s);
+    if (rc) return rc;
+    StabData<TV> sb = sa;
+    sb.po = sb.vo = nullptr;
+    sb.pn = nullptr;
+    sb.rt = nullptr;
+    sb.v = V[q];
+    sb.pga = pga;
+    sb.dot2 = ptt;
+    rc = owner(kStabS, sb, R, t, pts, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL((ks_update<TV>), dim3(NB), dim3(SVB), 0, s, n, st, na, pts, ptt, R,
+                       V[q], P[q], t, diag, RT, x, prt, prr);
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  }
+};
+
+}  // namespace vgpu
