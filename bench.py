@@ -8,8 +8,9 @@ resident in HBM before timing.
 A step is one plain-FP64 Jacobi-BiCG solve to rtol 1e-8 from x0 = 0
 (time-to-rtol).  `value` is the operator throughput in GDOF/s (nodes / time of
 one FP64 matrix-free apply, owner-computes scatter), measured with HIP events
-on the library's stream.  The line also carries time-to-rtol of FP64 and
-mixed precision, the other scatter modes, the roofline of the dominant kernel
+on the library's stream.  The line also carries time-to-rtol of FP64 BiCG,
+FP64 BiCGSTAB and mixed precision (FP32 BiCGSTAB inner solves, FP64
+refinement), the other scatter modes, the roofline of the dominant kernel
 (the owner launch of every BiCG iteration: direction update p = z + beta p,
 pp = zz + beta pp fused into the dual apply A p, A^T pp), that of the plain
 dual apply, and the reference's CPU operator timed on the host.
@@ -99,8 +100,9 @@ def measured_traffic(tuned):
 
 def config3(pkg):
     """256x256x64 elements (4.29M nodes), per-element diffusivity log-uniform
-    in [1e-4, 1e-2]: FP64 BiCG and mixed precision (FP32 inner BiCG, rtol 1e-4,
-    FP64 refinement) to rtol 1e-8; rel-L2 of the mixed solution vs FP64."""
+    in [1e-4, 1e-2]: FP64 BiCG, FP64 BiCGSTAB and mixed precision (FP32 inner
+    BiCGSTAB, rtol 1e-4, FP64 refinement) to rtol 1e-8; rel-L2 of the
+    BiCGSTAB and mixed solutions vs FP64 BiCG."""
     progress("config 3: 256x256x64")
     p = pkg.problem.config_problem("c3")
     ctx = pkg.Context.from_problem(p)
@@ -112,11 +114,14 @@ def config3(pkg):
     out = {"workload": "HEX8 unit box 256x256x64 elements, 4293185 nodes, lambda log-uniform "
                        "[1e-4, 1e-2] per element, V1 tau, rtol 1e-8 from x0=0"}
     xs = {}
-    for name, prec, inner in (("fp64", pkg.FP64, 0.0), ("mixed", pkg.MIXED, 1e-4)):
+    for name, prec, inner in (("fp64", pkg.FP64, 0.0), ("fp64_bicgstab", pkg.FP64, 0.0),
+                              ("mixed", pkg.MIXED, 1e-4)):
+        ctx.set_method("bicgstab" if name == "fp64_bicgstab" else "bicg")
         for rep in range(2):  # first solve captures the graph
             ctx.h2d(x_dev, zeros)
             it, err, sec, rc = ctx.solve_dev(b_dev, x_dev, rtol=1e-8, itmax=50000,
                                              precision=prec, inner_rtol=inner)
+        ctx.set_method("bicg")
         xs[name] = ctx.d2h(x_dev, p.N)
         out[name] = {"iterations": it, "seconds": round(sec, 4), "status": rc,
                      "true_rel_residual": ctx.true_residual(b, xs[name])}
@@ -124,6 +129,10 @@ def config3(pkg):
     out["mixed_rel_l2_vs_fp64"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
                                         np.linalg.norm(xs["fp64"]))
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    out["bicgstab_rel_l2_vs_fp64"] = float(np.linalg.norm(xs["fp64_bicgstab"] - xs["fp64"]) /
+                                           np.linalg.norm(xs["fp64"]))
+    out["speedup_bicgstab_vs_fp64"] = round(out["fp64"]["seconds"] /
+                                            out["fp64_bicgstab"]["seconds"], 3)
     out["dual_apply_ms"] = {"fp64": round(ctx.time_kernel(1, "owner", pkg.FP64, 20), 5),
                             "fp32": round(ctx.time_kernel(1, "owner", pkg.FP32, 20), 5)}
     ctx.close()
@@ -133,24 +142,36 @@ def config3(pkg):
 def learned_guess(pkg):
     """Implicit Euler (dt 0.01, 200 steps, helix source of seed 0) on the
     128x128x64 box, every step to rtol 1e-8, from the previous-step guess
-    (the plain method), linear extrapolation + 2 Jacobi sweeps, and the
-    trained network (committed parameters, trained on seeds 1-10 by
-    learned.main) + 2 Jacobi sweeps."""
+    with BiCG (the plain method), and with linear extrapolation + 2 Jacobi
+    sweeps and the trained network (committed parameters, trained on seeds
+    1-10 by learned.main) + 2 Jacobi sweeps, each solved by BiCG and by
+    BiCGSTAB; speedups against the plain method."""
     path = os.path.join(ROOT, "vanadis_solver-directional_residual_amd", "data",
                         "learned_guess_128x128x64.npz")
     if not os.path.exists(path):
         return {"status": "no trained parameters"}
     params = pkg.learned.load_params(path)
-    progress("learned guess: 3 x 200 implicit-Euler steps")
-    res, _ = pkg.learned.experiment(pkg, params=params,
-                                    modes=("previous", "extrap+jacobi", "net+jacobi"))
+    modes = ("previous", "extrap+jacobi", "net+jacobi", "previous@bicgstab",
+             "extrap+jacobi@bicgstab", "net+jacobi@bicgstab")
+    progress(f"learned guess: {len(modes)} x 200 implicit-Euler steps")
+    res, _ = pkg.learned.experiment(pkg, params=params, modes=modes)
     out = {"workload": "HEX8 unit box 128x128x64, implicit Euler dt=0.01, 200 steps, "
-                       "helix source seed 0, BiCG rtol 1e-8 each step"}
-    for m in ("previous", "extrap+jacobi", "net+jacobi"):
+                       "helix source seed 0, rtol 1e-8 each step; mode guess[+jacobi]"
+                       "[@method], default method BiCG"}
+    for m in modes:
         out[m] = {"iterations": res[m]["iterations"], "seconds": res[m]["seconds"],
                   "fallbacks": res[m]["fallbacks"]}
+        if m != "previous":
+            out[m]["speedup_vs_previous"] = res[f"{m}_speedup_vs_previous"]
     out["net_speedup_vs_previous"] = res["net+jacobi_speedup_vs_previous"]
     out["extrap_speedup_vs_previous"] = res["extrap+jacobi_speedup_vs_previous"]
+    out["net_bicgstab_speedup_vs_previous"] = res["net+jacobi@bicgstab_speedup_vs_previous"]
+    out["extrap_bicgstab_speedup_vs_previous"] = \
+        res["extrap+jacobi@bicgstab_speedup_vs_previous"]
+    progress(f"learned: net {out['net_speedup_vs_previous']}x, extrap "
+             f"{out['extrap_speedup_vs_previous']}x, net+bicgstab "
+             f"{out['net_bicgstab_speedup_vs_previous']}x, extrap+bicgstab "
+             f"{out['extrap_bicgstab_speedup_vs_previous']}x vs previous-step guess")
     return out
 
 
@@ -206,6 +227,7 @@ def main():
                     help="only launch the dual apply and copy kernels (counter runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-stab", action="store_true", help="skip the FP64 BiCGSTAB solves")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default owner tile (32x8) instead of the autotuner's")
     ap.add_argument("--tile", type=int, default=-1,
@@ -299,6 +321,29 @@ def main():
     else:
         mixed_status = "skipped"
 
+    # FP64 BiCGSTAB (same preconditioner, stopping rule and rtol; reported,
+    # never fatal for the line)
+    ssec, sits, rel_l2_stab, true_res_stab, stab_status = [], 0, None, None, "ok"
+    if not a.no_stab:
+        try:
+            ctx.set_method("bicgstab")
+            solve(pkg.FP64)
+            for _ in range(max(1, min(3, a.steps))):
+                it, err_s, sec = solve(pkg.FP64)
+                ssec.append(sec)
+                sits = it
+            xs = ctx.d2h(x_dev, N)
+            rel_l2_stab = float(np.linalg.norm(xs - x64) / np.linalg.norm(x64))
+            true_res_stab = ctx.true_residual(b, xs)
+            progress(f"bicgstab: {sits} iterations, {1e3 * np.mean(ssec):.2f} ms")
+        except RuntimeError as exc:
+            stab_status = f"failed: {exc}"
+            progress(f"bicgstab {stab_status}")
+        finally:
+            ctx.set_method("bicg")
+    else:
+        stab_status = "skipped"
+
     # kernels, HIP events on the library stream
     reps = 50
     t_single = ctx.time_kernel(0, "owner", pkg.FP64, reps)
@@ -361,8 +406,13 @@ def main():
                                "to rtol 1e-8 from x0=0, owner-computes scatter",
                    "elements": E, "nodes": N},
         "time_to_rtol_s": {"fp64": round(float(np.mean(secs)), 5),
-                           "mixed": round(float(np.mean(msec)), 5) if msec else None},
-        "iterations": {"fp64": iters[-1], "mixed_inner_total": mits},
+                           "mixed": round(float(np.mean(msec)), 5) if msec else None,
+                           "fp64_bicgstab": round(float(np.mean(ssec)), 5) if ssec else None},
+        "iterations": {"fp64": iters[-1], "mixed_inner_total": mits, "fp64_bicgstab": sits},
+        "speedup_bicgstab_vs_fp64": (round(float(np.mean(secs)) / float(np.mean(ssec)), 3)
+                                     if ssec else None),
+        "bicgstab_status": stab_status,
+        "bicgstab_rel_l2_vs_fp64": rel_l2_stab, "bicgstab_true_rel_residual": true_res_stab,
         "speedup_mixed_vs_fp64": (round(float(np.mean(secs)) / float(np.mean(msec)), 3)
                                   if msec else None),
         "mixed_status": mixed_status,

@@ -86,14 +86,16 @@ class Stepper:
     def step(self, centre, sigma, x1, x2, out, mode="previous", net=None, rtol=1e-8,
              itmax=20000, sweeps=2, omega=0.8):
         """Solve step n into `out` from the guess of `mode`; x1 = x_{n-1},
-        x2 = x_{n-2} (device).  Returns (iterations, seconds guess + solve,
-        fallback used)."""
+        x2 = x_{n-2} (device).  mode is guess[+jacobi][@method]: guess
+        "previous", "extrap" or "net", method "bicg" (default) or "bicgstab".
+        Returns (iterations, seconds guess + solve, fallback used)."""
         ctx = self.ctx
         self.source(centre, sigma)
         ctx.assemble_rhs_dev(self.q, x1, self.b)
         ctx.residual_norm_dev(self.b, x1)  # synchronise before timing
         t0 = time.perf_counter()
         fallback = False
+        mode, _, method = mode.partition("@")
         base, _, post = mode.partition("+")
         if base == "previous":
             ctx.copy_dev(out, x1, 8 * self.N)
@@ -110,7 +112,13 @@ class Stepper:
             ctx.jacobi_dev(self.b, out, sweeps, omega)
         elif post:
             raise ValueError(mode)
-        it, err, sec, rc = ctx.solve_dev(self.b, out, rtol=rtol, itmax=itmax)
+        if method:
+            ctx.set_method(method)
+        try:
+            it, err, sec, rc = ctx.solve_dev(self.b, out, rtol=rtol, itmax=itmax)
+        finally:
+            if method:
+                ctx.set_method("bicg")
         t1 = time.perf_counter()
         if rc != _lib.OK:
             raise _lib.VgpuError(rc, f"step solve failed ({mode}): it={it} err={err}")
