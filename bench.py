@@ -42,10 +42,11 @@ def algorithmic_bytes(N, E, dual, fp32=False, fused=False):
     """Compulsory bytes of one matrix-free apply with this library's layout:
     input/output vectors, the uint8 Dirichlet mask and five per-element
     fields (vx, vy, vz, lambda, tau); axis coordinates are O(n^(1/3)).
-    fused: the iteration launch reads z, zz and the previous directions and
-    writes the new directions and A p, A^T pp (8 vectors)."""
+    fused: the BiCG iteration launch reads z, zz and the previous directions,
+    writes the new directions and A p, A^T pp, and reads and writes x
+    (x += alpha p of the previous update): 10 vectors."""
     w = 4 if fp32 else 8
-    vec = (8 if fused else 4 if dual else 2) * w * N
+    vec = (10 if fused else 4 if dual else 2) * w * N
     return vec + N + 5 * w * E
 
 
@@ -80,21 +81,22 @@ def cpu_baseline(pkg, seconds=12.0):
             "ms_per_apply": round(dt * 1e3, 3)}
 
 
-def measured_traffic(tuned):
-    """HBM bytes per dual-apply launch from the committed rocprofv3 --pmc
-    summary of the same kernel and tile (FETCH_SIZE and WRITE_SIZE, each scaled
-    by the copy-kernel calibration), or None."""
+def measured_traffic(tuned, kernel="iteration_launch"):
+    """HBM bytes per launch of `kernel` (iteration_launch, dual_apply,
+    bicgstab_direction_launch, bicgstab_s_launch) from the committed
+    rocprofv3 --pmc summary of the same kernel, tile and z chunk (FETCH_SIZE
+    and WRITE_SIZE, each scaled by the copy-kernel calibration), or None."""
     path = os.path.join(ROOT, "profiles", "traffic_dual_apply_fp64_128x128x64.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         t = json.load(f)
     key = "0/0" if tuned is None else f"{tuned['tile']}/{tuned['zchunk']}"
-    e = t.get("iteration_launch", {}).get(key)
+    e = t.get(kernel, {}).get(key)
     if not e:
         return None
     return {"bytes_per_launch": e["bytes_per_launch"],
-            "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json iteration_launch "
+            "source": f"profiles/traffic_dual_apply_fp64_128x128x64.json {kernel} "
                       f"tile/zchunk {key}"}
 
 
@@ -224,7 +226,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel-only", action="store_true",
-                    help="only launch the dual apply and copy kernels (counter runs)")
+                    help="only launch the dual apply, the BiCG and BiCGSTAB owner launches "
+                         "and the copy kernel (counter runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-stab", action="store_true", help="skip the FP64 BiCGSTAB solves")
@@ -274,6 +277,8 @@ def main():
         for _ in range(3):
             ctx.time_kernel(1, "owner", pkg.FP64, reps=5)
             ctx.time_kernel(3, "owner", pkg.FP64, reps=5)
+            ctx.time_kernel(5, "owner", pkg.FP64, reps=5)
+            ctx.time_kernel(6, "owner", pkg.FP64, reps=5)
             ctx.time_kernel(2, "owner", pkg.FP64, reps=5)
         progress("kernel-only done")
         print(json.dumps({"kernel_only": True, "N": N, "E": E}))
@@ -351,6 +356,8 @@ def main():
     t_iter = ctx.time_kernel(3, "owner", pkg.FP64, reps)
     t_copy = ctx.time_kernel(2, "owner", pkg.FP64, reps)
     t_copy8 = ctx.time_kernel(4, "owner", pkg.FP64, 20)
+    t_sdir = ctx.time_kernel(5, "owner", pkg.FP64, reps)
+    t_ss = ctx.time_kernel(6, "owner", pkg.FP64, reps)
     modes = {}
     for sc in ("owner", "colored", "atomic"):
         for prec, pn in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32")):
@@ -363,9 +370,25 @@ def main():
     achieved = iter_bytes / (t_iter * 1e-3) / 1e9
     copy_gbs = 16.0 * N / (t_copy * 1e-3) / 1e9
     copy8_gbs = 128.0 * N / (t_copy8 * 1e-3) / 1e9
+    # BiCGSTAB owner launches: direction (R, po, vo in; P, V out; d, RT at the
+    # owned nodes; mask; records) and S (R, V in; t out; d; mask; records)
+    sdir_bytes = 57 * N + 40 * E
+    ss_bytes = 33 * N + 40 * E
+    stab_roof = {
+        "direction_launch": {"ms": round(t_sdir, 5), "algorithmic_bytes_per_launch": sdir_bytes,
+                             "achieved": round(sdir_bytes / (t_sdir * 1e-3) / 1e9, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                             "frac": round(sdir_bytes / (t_sdir * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                             "kernel": "k_owner<double,double,single,kStabDir>"},
+        "s_launch": {"ms": round(t_ss, 5), "algorithmic_bytes_per_launch": ss_bytes,
+                     "achieved": round(ss_bytes / (t_ss * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ss_bytes / (t_ss * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "kernel": "k_owner<double,double,single,kStabS>"}}
     progress(f"apply {t_single * 1e3:.1f} us ({gdofs:.1f} GDOF/s), dual {t_dual * 1e3:.1f} us "
              f"({dual_gbs:.0f} GB/s), iteration launch {t_iter * 1e3:.1f} us "
-             f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s (8 N: {copy8_gbs:.0f} GB/s)")
+             f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s (8 N: {copy8_gbs:.0f} GB/s), "
+             f"BiCGSTAB launches {t_sdir * 1e3:.1f} / {t_ss * 1e3:.1f} us")
 
     batch = None
     if not a.no_batch:
@@ -428,6 +451,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_owner<double,double,dual,direction>",
                      "algorithmic_bytes_per_launch": iter_bytes},
+        "roofline_bicgstab": stab_roof,
         "roofline_dual_apply": {"bound": "hbm", "achieved": round(dual_gbs, 1),
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(dual_gbs / HBM_PEAK_GBS, 4),
@@ -443,6 +467,10 @@ def main():
     if traffic:
         out["roofline"]["traffic"] = traffic["bytes_per_launch"]
         out["roofline"]["traffic_source"] = traffic["source"]
+    for key, kname in (("direction_launch", "bicgstab_direction_launch"),
+                       ("s_launch", "bicgstab_s_launch")):
+        tr = measured_traffic(tuned, kname)
+        stab_roof[key]["traffic"] = tr["bytes_per_launch"] if tr else None
     print(json.dumps(out), flush=True)
     ctx.close()
 

@@ -195,7 +195,9 @@ int vgpu_memcpy_d2h(vgpu_ctx *ctx, void *dst, const void *src_dev, int64_t bytes
  *           test, beta and p = z + beta p, pp = zz + beta pp fused into the
  *           dual apply (owner scatter only),
  *       4 = device copy of 8 N doubles (the size of an iteration's working
- *           set; buffers allocated for the call). */
+ *           set; buffers allocated for the call),
+ *       5 / 6 = the BiCGSTAB direction / S owner launch (FP64, owner scatter;
+ *           the context's BiCGSTAB buffers, a status that never stops). */
 int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
                      int reps, double *avg_ms);
 

@@ -1529,9 +1529,12 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
 
 int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                      int reps, double *avg_ms) {
-  if (!c || !avg_ms || reps < 1 || what < 0 || what > 4) return VGPU_ERR_ARG;
+  if (!c || !avg_ms || reps < 1 || what < 0 || what > 6) return VGPU_ERR_ARG;
   if (what != 2 && what != 4 && !c->have_coeffs) return VGPU_ERR_STATE;
   if (what == 3 && scatter != VGPU_SCATTER_OWNER) return VGPU_ERR_ARG;
+  if (what >= 5 && (scatter != VGPU_SCATTER_OWNER || precision != VGPU_FP64 ||
+                    !owner_fits(c->g)))
+    return VGPU_ERR_ARG;
   const int64_t N = c->g.N;
   VGPU_HIP(hipSetDevice(c->device));
   // deterministic non-trivial inputs
@@ -1612,12 +1615,65 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                                             c->tdir + 2 * N, c->tdir + 3 * N, c->tpart,
                                             c->stream);
   };
+  // what 5 / 6: the BiCGSTAB direction / S launch (FP64) on the BiCGSTAB
+  // engine's buffers, inputs from the deterministic vectors above, every
+  // reduction partial 1 and a status that never stops (tol 0), so every
+  // workgroup runs its march
+  if (what >= 5) {
+    int rc = ensure_diag(c);
+    if (rc) return rc;
+    const int64_t na = c->zchunk > 0 ? owner_groups(c->g, c->tile, c->zchunk)
+                                     : owner_units(c->g, c->tile);
+    rc = c->stab64.alloc(N, (int)na);
+    if (rc) return rc;
+    Stab<double> &e = c->stab64;
+    double *fills[] = {e.prt, e.prr, e.pbn, e.pga, e.pts, e.ptt};
+    const int64_t counts[] = {e.NB, e.NB, e.NB, e.cap, e.cap, e.cap};
+    for (int i = 0; i < 6; ++i)
+      hipLaunchKernelGGL(k_fill, grid_for(counts[i]), dim3(256), 0, c->stream, counts[i], 1.0,
+                         fills[i]);
+    SolveStatus s0{};
+    s0.iter = 1;
+    s0.rho = s0.alpha = s0.omega = 1.0;
+    VGPU_HIP(hipMemcpyAsync(e.st, &s0, sizeof(SolveStatus), hipMemcpyHostToDevice,
+                            c->stream));
+    VGPU_HIP(hipStreamSynchronize(c->stream));
+  }
+  auto launch_stab = [&]() -> int {
+    Stab<double> &e = c->stab64;
+    StabData<double> sd;
+    sd.d = c->diag;
+    sd.prt = e.prt;
+    sd.prr = e.prr;
+    sd.pbn = e.pbn;
+    sd.st = e.st;
+    sd.tol = 0.0;
+    sd.divtol = 1e300;
+    sd.itmax = 1 << 30;
+    sd.NB = e.NB;
+    sd.na = e.na;
+    OpData<double> op = c->op64(c->cH, c->cC);
+    if (what == 5) {
+      sd.po = c->vt;
+      sd.vo = c->vt2;
+      sd.pn = e.P[0];
+      sd.rt = c->vb;
+      return launch_owner_stab<double, double>(op, kStabDir, c->zchunk, sd, c->vx_, e.V[0],
+                                               e.pga, c->stream);
+    }
+    sd.v = c->vt;
+    sd.pga = e.pga;
+    sd.dot2 = e.ptt;
+    return launch_owner_stab<double, double>(op, kStabS, c->zchunk, sd, c->vx_, e.t, e.pts,
+                                             c->stream);
+  };
   // what 4: a copy of 8 N doubles, the size class of a BiCG iteration's
   // working set (buffers held for this call only)
   double *big = nullptr;
   if (what == 4) VGPU_HIP(hipMalloc(&big, 16 * N * sizeof(double)));
   auto launch = [&]() -> int {
     if (what == 3) return launch_dir();
+    if (what >= 5) return launch_stab();
     if (what == 4) {
       hipLaunchKernelGGL(k_copy, grid_for(8 * N, 256), dim3(256), 0, c->stream, 8 * N, big,
                          big + 8 * N);
@@ -1655,20 +1711,35 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     int rc = launch();
     if (rc) return rc;
   }
+  // the timed launches are captured in one graph, so the events bracket GPU
+  // work only (host enqueue gaps between launches do not count)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  VGPU_HIP(hipStreamSynchronize(c->stream));
+  VGPU_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int lrc = VGPU_OK;
+  for (int i = 0; i < reps && lrc == VGPU_OK; ++i) lrc = launch();
+  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
+  if (lrc) {
+    if (graph) hipGraphDestroy(graph);
+    return lrc;
+  }
+  VGPU_HIP(ce);
+  VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
   hipEvent_t e0, e1;
   VGPU_HIP(hipEventCreate(&e0));
   VGPU_HIP(hipEventCreate(&e1));
+  VGPU_HIP(hipGraphLaunch(gexec, c->stream));  // warm-up replay
   VGPU_HIP(hipEventRecord(e0, c->stream));
-  for (int i = 0; i < reps; ++i) {
-    int rc = launch();
-    if (rc) return rc;
-  }
+  VGPU_HIP(hipGraphLaunch(gexec, c->stream));
   VGPU_HIP(hipEventRecord(e1, c->stream));
   VGPU_HIP(hipEventSynchronize(e1));
   float ms = 0;
   VGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
   hipEventDestroy(e0);
   hipEventDestroy(e1);
+  hipGraphExecDestroy(gexec);
+  hipGraphDestroy(graph);
   if (big) VGPU_HIP(hipFree(big));
   *avg_ms = ms / reps;
   return VGPU_OK;

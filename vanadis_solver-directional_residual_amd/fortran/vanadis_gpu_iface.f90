@@ -17,6 +17,7 @@ module vanadis_gpu
   integer(c_int), parameter :: VGPU_SCHEME_THETA = 0, VGPU_SCHEME_STEADY = 1, &
                                VGPU_SCHEME_EULER = 2
   integer(c_int), parameter :: VGPU_FP64 = 0, VGPU_FP32 = 1, VGPU_MIXED = 2
+  integer(c_int), parameter :: VGPU_METHOD_BICG = 0, VGPU_METHOD_BICGSTAB = 1
 
   interface
     integer(c_int) function vgpu_create(nax, nay, naz, xx, yy, zz, iss, bc, ctx) &
@@ -47,6 +48,12 @@ module vanadis_gpu
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: scatter, zchunk, tile
+    end function
+
+    integer(c_int) function vgpu_set_method(ctx, method) bind(C, name="vgpu_set_method")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: method
     end function
 
     integer(c_int) function vgpu_assemble_rhs(ctx, q_in, t_old, b) &
