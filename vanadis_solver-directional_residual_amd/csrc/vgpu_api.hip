@@ -291,9 +291,9 @@ int solve_stab(vgpu_ctx *c, Stab<TV> &eng, const TV *b, TV *x, const TV *diag, d
                                 nullptr, s);
   };
   StabOwnerOp<TV> owner = [c](int mode, const StabData<TV> &sd, const TV *r, TV *y,
-                              double *dot, hipStream_t s) {
+                              hipStream_t s) {
     OpData<TV> op = op_for<TV>(c);
-    return launch_owner_stab<TV, TV>(op, mode, c->zchunk, sd, r, y, dot, s);
+    return launch_owner_stab<TV, TV>(op, mode, c->zchunk, sd, r, y, s);
   };
   const uint64_t key = (c->version << 8) | (uint64_t)(c->zchunk % 64) |
                        (sizeof(TV) == 4 ? (1ull << 7) : 0);
@@ -1616,8 +1616,8 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                                             c->stream);
   };
   // what 5 / 6: the BiCGSTAB direction / S launch (FP64) on the BiCGSTAB
-  // engine's buffers, inputs from the deterministic vectors above, every
-  // reduction partial 1 and a status that never stops (tol 0), so every
+  // engine's buffers, inputs from the deterministic vectors above, fixed
+  // reduction partials and a status that never stops (tol 0), so every
   // workgroup runs its march
   if (what >= 5) {
     int rc = ensure_diag(c);
@@ -1627,45 +1627,31 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     rc = c->stab64.alloc(N, (int)na);
     if (rc) return rc;
     Stab<double> &e = c->stab64;
-    double *fills[] = {e.prt, e.prr, e.pbn, e.pga, e.pts, e.ptt};
-    const int64_t counts[] = {e.NB, e.NB, e.NB, e.cap, e.cap, e.cap};
-    for (int i = 0; i < 6; ++i)
-      hipLaunchKernelGGL(k_fill, grid_for(counts[i]), dim3(256), 0, c->stream, counts[i], 1.0,
-                         fills[i]);
+    // partials 1 (r~.s: 2, so that rho = r~.s - omega r~.t is not 0)
+    double *fills[] = {e.prt, e.prr, e.pbn, e.pga, e.prn, e.pts, e.ptt, e.prs, e.prq};
+    const int64_t counts[] = {e.NB, e.NB, e.NB, e.cap, e.cap, e.cap, e.cap, e.cap, e.cap};
+    for (int i = 0; i < 9; ++i)
+      hipLaunchKernelGGL(k_fill, grid_for(counts[i]), dim3(256), 0, c->stream, counts[i],
+                         fills[i] == e.prs ? 2.0 : 1.0, fills[i]);
+    // iteration 1 of a solve that never stops (tol 0): both launches march
     SolveStatus s0{};
-    s0.iter = 1;
-    s0.rho = s0.alpha = s0.omega = 1.0;
+    s0.iter = s0.pad = 1;
+    s0.rho = s0.alpha = s0.omega = s0.rho_next = 1.0;
     VGPU_HIP(hipMemcpyAsync(e.st, &s0, sizeof(SolveStatus), hipMemcpyHostToDevice,
                             c->stream));
     VGPU_HIP(hipStreamSynchronize(c->stream));
   }
   auto launch_stab = [&]() -> int {
     Stab<double> &e = c->stab64;
-    StabData<double> sd;
-    sd.d = c->diag;
-    sd.prt = e.prt;
-    sd.prr = e.prr;
-    sd.pbn = e.pbn;
-    sd.st = e.st;
-    sd.tol = 0.0;
-    sd.divtol = 1e300;
-    sd.itmax = 1 << 30;
-    sd.NB = e.NB;
-    sd.na = e.na;
     OpData<double> op = c->op64(c->cH, c->cC);
-    if (what == 5) {
-      sd.po = c->vt;
-      sd.vo = c->vt2;
-      sd.pn = e.P[0];
-      sd.rt = c->vb;
-      return launch_owner_stab<double, double>(op, kStabDir, c->zchunk, sd, c->vx_, e.V[0],
-                                               e.pga, c->stream);
-    }
-    sd.v = c->vt;
-    sd.pga = e.pga;
-    sd.dot2 = e.ptt;
-    return launch_owner_stab<double, double>(op, kStabS, c->zchunk, sd, c->vx_, e.t, e.pts,
-                                             c->stream);
+    // inputs from the deterministic vectors: R = vx_, x = vt2
+    if (what == 5)
+      return launch_owner_stab<double, double>(
+          op, kStabDir, c->zchunk, e.dir_data(0, c->diag, c->vt2, 0.0, 1 << 30, 1e300),
+          c->vx_, e.V[0], c->stream);
+    return launch_owner_stab<double, double>(
+        op, kStabS, c->zchunk, e.s_data(0, c->diag, 0.0, 1 << 30, 1e300), c->vx_, e.T,
+        c->stream);
   };
   // what 4: a copy of 8 N doubles, the size class of a BiCG iteration's
   // working set (buffers held for this call only)

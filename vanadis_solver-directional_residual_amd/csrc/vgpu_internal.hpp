@@ -81,13 +81,14 @@ struct SolveStatus {
   int pend;   // batched engine: done code found by an iteration launch, made
               // `done` by the update launch that follows (so no workgroup of
               // the iteration launch sees it change)
-  int pad;
+  int pad;    // BiCGSTAB: index of the iteration whose direction launch ran
   double err; // ||r||/||b|| of the last test
   double bnrm2;
   double alpha;  // alpha of the last update (batched engine: x += alpha p
                  // is applied by the next iteration launch)
-  double rho;    // BiCGSTAB: rho of the last iteration (r~ . r)
-  double omega;  // BiCGSTAB: omega of the last iteration
+  double rho;       // BiCGSTAB: rho of the last S launch's iteration (r~ . r)
+  double omega;     // BiCGSTAB: omega of the last completed iteration
+  double rho_next;  // BiCGSTAB: rho of the iteration whose direction launch ran
 };
 
 // One system of a batch: its node range in the concatenated batch vectors,
@@ -189,41 +190,49 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
                      bool batch, int tile, int zchunk, bool robin, const DirData<TIO> &dd,
                      const TIO *z, const TIO *zz, TIO *y1, TIO *y2, double *dot,
                      hipStream_t s);
-// Owner launches of one Jacobi-BiCGSTAB iteration (one system, owner
+// Owner launches of one Jacobi-BiCGSTAB iteration j (one system, owner
 // scatter), written for the scaled vectors R = D^-1 r, P = D^-1 p,
-// V = D^-1 v (D the diagonal, right preconditioning) and the shadow
-// RT = D r~ (so that r~ . r = RT . R):
-//  * kStabDir: the convergence test of the previous iteration and
-//    beta = (rho/rho_old)(alpha/omega) from the partials; the plane loads form
-//    P = R + beta (po - omega vo) (iteration 1: P = R), store it at the nodes
-//    the tile owns and apply the operator; output V = (A P)/d; dot partials
-//    of RT . V;
-//  * kStabS: alpha = rho / (RT . V) from the partials; the plane loads form
-//    S = R - alpha V; output t = A S (raw); dot partials of t . s (s = d S) and
-//    t . t.
-// The vector update between them (omega, x, R and the partials of RT . R and
-// |r|^2) is a vector kernel.
+// V = D^-1 A P, S = D^-1 s, T = D^-1 t (D the diagonal, right
+// preconditioning) and the shadow RT = D r0 (so that r0 . r = RT . R):
+//  * kStabDir: omega_{j-1} = t.s / t.t and rho_j = r~.s - omega r~.t from the
+//    previous S launch's partials, beta_j = (rho_j / rho_{j-1})
+//    (alpha_{j-1} / omega_{j-1}); the plane loads (R, V, P of iteration j-1
+//    and T) form S = R - alpha V, R_j = S - omega T and
+//    P_j = R_j + beta (P - omega V) (j = 0: R_j = R, P_j = R); the nodes the
+//    tile owns store R_j, P_j and x += alpha P + omega S; output
+//    V_j = (A P_j)/d; partials of RT . V_j and |r_j|^2 = sum (d R_j)^2;
+//  * kStabS: the convergence test of r_j (|r_j|/|b| from those partials),
+//    alpha_j = rho_j / (RT . V_j); the plane loads form S = R_j - alpha V_j;
+//    output T = (A S)/d; partials of t.s, t.t, r~.s and r~.t (t = d T,
+//    s = d S).
+// Two launches per iteration, no vector pass in between.
 enum StabMode { kStabDir = 2, kStabS = 3 };
 template <typename TIO>
 struct StabData {
-  const TIO *po = nullptr, *vo = nullptr;  // kStabDir: previous P, V
-  TIO *pn = nullptr;                       // kStabDir: new P (owned nodes)
-  const TIO *v = nullptr;                  // kStabS: V of this iteration
+  const TIO *po = nullptr, *vo = nullptr;  // kStabDir: P, V of iteration j-1
+  const TIO *to = nullptr;                 // kStabDir: T of iteration j-1
+  TIO *pn = nullptr, *rn = nullptr;        // kStabDir: P_j, R_j (owned nodes)
+  TIO *x = nullptr;                        // kStabDir: solution (owned nodes)
+  const TIO *v = nullptr;                  // kStabS: V_j
   const TIO *d = nullptr;                  // diagonal (owned nodes)
-  const TIO *rt = nullptr;                 // kStabDir: shadow RT (owned nodes)
-  // vector-kernel partials [NB]: RT . R, |r|^2, |b|^2
-  const double *prt = nullptr, *prr = nullptr, *pbn = nullptr;
-  const double *pga = nullptr;  // kStabS: the kStabDir launch's RT . V partials [na]
-  double *dot2 = nullptr;       // kStabS: t . t partials (t . s go to `dot`)
+  const TIO *rt = nullptr;                 // shadow RT (owned nodes)
+  // partial arrays the launch's prologue sums (in a fixed order) and their
+  // counts: kStabDir t.s, t.t, r~.s, r~.t; kStabS RT.V, |r|^2, |b|^2
+  const double *pin[4] = {nullptr, nullptr, nullptr, nullptr};
+  int nin[4] = {0, 0, 0, 0};
+  const double *prt0 = nullptr;  // kStabDir, iteration 0: r0 . r0 partials
+  int nb0 = 0;
+  // partials the launch writes, one per unit (zchunk 0) or (chunk, tile)
+  // group (zchunk > 0): kStabDir RT.V, |r|^2; kStabS t.s, t.t, r~.s, r~.t
+  double *pout[4] = {nullptr, nullptr, nullptr, nullptr};
   SolveStatus *st = nullptr;
   double tol = 0, divtol = 0;
-  int itmax = 0, NB = 0, na = 0;
+  int itmax = 0;
 };
-// One BiCGSTAB owner launch (mode kStabDir or kStabS) on R (x1) into y1,
-// dot partials per unit (zchunk 0) or per (chunk, tile) group (zchunk > 0).
+// One BiCGSTAB owner launch (mode kStabDir or kStabS) on R (x1) into y1.
 template <typename T, typename TIO>
 int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<TIO> &sd,
-                      const TIO *r, TIO *y1, double *dot, hipStream_t s);
+                      const TIO *r, TIO *y1, hipStream_t s);
 // Owner operator over the active systems of a batch (units split evenly over
 // as.nu): for every active member m whose status is not done, y1 = A_m x1 (and
 // y2 = A_m^T x2) on the member's node range of the batch vectors.  dot (dual

@@ -239,14 +239,15 @@ __device__ __forceinline__ void load_rec(Res r, int v, int s, Coef<T> &c) {
 //
 // With kStabDir / kStabS (single operator) the launch is one of the two
 // operator launches of a BiCGSTAB iteration (see StabData): the plane loads
-// form P = R + beta (po - omega vo) or S = R - alpha V, and the owned node's
-// output is scaled or reduced with values loaded at that node.
+// form R, P (and x at owned nodes) or S, and the owned node's output is
+// scaled by 1/d and reduced with values loaded at that node.
 template <typename TIO, bool DIR>
 struct PlanePair {
   TIO a0, a1, b0, b1;
   TIO c0, c1, d0, d1;  // DIR: previous directions of the two nodes; BiCGSTAB:
-                       // po, vo (kStabDir) or V (kStabS)
-  TIO e0, e1;          // DIR: x of the nodes the march owns (else 0)
+                       // c = V, d = previous P (kStabDir)
+  TIO e0, e1;          // DIR, kStabDir: x of the nodes the march owns (else 0)
+  TIO t0, t1;          // kStabDir: T of the previous S launch
   TIO f, g;            // BiCGSTAB: d and RT at the thread's owned node
   uint8_t m0, m1;
 };
@@ -273,17 +274,20 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
   __shared__ uint8_t pm[2][NP];
   __shared__ T st1[2][4 * NT];
   __shared__ T st2[DUAL ? 2 : 1][DUAL ? 4 * NT : 1];
-  __shared__ double red[2][NW];
-  __shared__ double red2[SB ? 2 : 1][NW];
+  // dot sums per launch: the dual apply's w.pp; kStabDir RT.V and |r|^2;
+  // kStabS t.s, t.t, r~.s and r~.t
+  constexpr int ND = DUAL ? 1 : SA ? 2 : SB ? 4 : 1;
+  __shared__ double red[ND][2][NW];
   __shared__ double rsum[4][NW];
-  __shared__ int sflag[2];
+  __shared__ T plR[SA ? 2 : 1][SA ? NP : 1];  // kStabDir: the new R of the plane
+  __shared__ int sflag[3];
   __shared__ double salpha;
-  __shared__ double sstab[3];  // BiCGSTAB: rho, alpha, omega of the status
+  __shared__ double sstab[2];  // BiCGSTAB: scalars of the status
 
   const int tid = threadIdx.x;
   const int tx = tid % TX, ty = tid / TX;
   const int64_t G = gridDim.x, lb = xcd_tile(blockIdx.x, gridDim.x);
-  const bool want_dot = (DUAL || STAB) && dot != nullptr;
+  const bool want_dot = (DUAL && dot != nullptr) || STAB;
 
   const int q0 = tid, q1 = tid + NT;
   const int lx0 = q0 % (TX + 1), ly0 = q0 / (TX + 1);
@@ -357,8 +361,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     const Res rrc = make_res(op.rec, (int64_t)eplane * ez * SR);
     const Res rx1z = make_res(x1, 0);  // every access out of range: zeros
     const int64_t dslot = ub + (gidx >= 0 ? gidx : (int64_t)tile * ngz);
-    double *dm = want_dot ? dot + dslot : nullptr;
-    double *dm2 = (SB && want_dot) ? sd.dot2 + dslot : nullptr;
+    double *dm[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i)
+      dm[i] = STAB ? sd.pout[i] + dslot : want_dot ? dot + dslot : nullptr;
 
     const int gx0 = (tile % ntx) * (TX - 1) - 1;  // tile node/element origin
     const int gy0 = (tile / ntx) * (TY - 1) - 1;
@@ -384,22 +390,25 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     const int vm0 = pc0 >= 0 ? pc0 : kOob, vm1 = pc1 >= 0 ? pc1 : kOob;
     // DIR: previous and new directions; a plane node of the tile's own node
     // columns stores its new direction (kOob elsewhere)
-    // BiCGSTAB: c = po (kStabDir) or V (kStabS), d = vo; new P of owned nodes
-    // to rpn; d and RT read at the owned node column vo
-    const TIO *cptr = DIR ? dd.po : SA ? sd.po : SB ? sd.v : x1;
-    const TIO *dptr = DIR ? dd.ppo : SA ? sd.vo : x1;
+    // BiCGSTAB: c = V (previous V for kStabDir), d = previous P, t = T of the
+    // previous S launch; new P, R and x of owned nodes to rpn, rrn, rxs; d
+    // and RT read at the owned node column vo
+    const TIO *cptr = DIR ? dd.po : SA ? sd.vo : SB ? sd.v : x1;
+    const TIO *dptr = DIR ? dd.ppo : SA ? sd.po : x1;
     const Res rpo = make_res(cptr + off, op.g.N * SI);
     const Res rppo = make_res(dptr + off, op.g.N * SI);
     const Res rpn = make_res(DIR ? dd.pn + off : SA ? sd.pn + off : y1 + off, op.g.N * SI);
+    const Res rrn = make_res(SA ? sd.rn + off : y1 + off, op.g.N * SI);
+    const Res rto = make_res(SA ? sd.to + off : x1 + off, op.g.N * SI);
     const Res rdg = make_res(STAB ? sd.d + off : x1 + off, op.g.N * SI);
-    const Res rrt = make_res(SA ? sd.rt + off : x1 + off, op.g.N * SI);
+    const Res rrt = make_res(STAB ? sd.rt + off : x1 + off, op.g.N * SI);
     const Res rppn = make_res(DIR ? dd.ppn + off : y1 + off, op.g.N * SI);
     auto owned_col = [&](int lx, int ly) {
       return lx >= 1 && lx <= TX - 1 && ly >= 1 && ly <= TY - 1;
     };
     const int vn0 = (pc0 >= 0 && owned_col(lx0, ly0)) ? vp0 : kOob;
     const int vn1 = (pc1 >= 0 && owned_col(lx1, ly1)) ? vp1 : kOob;
-    const Res rxs = make_res(DIR ? dd.x + off : y1 + off, op.g.N * SI);
+    const Res rxs = make_res(DIR ? dd.x + off : SA ? sd.x + off : y1 + off, op.g.N * SI);
     TIO ap = TIO(0);  // DIR: alpha of the previous update (x += ap po)
     // DIR: beta and the first-iteration flag, known after the status test
     // below; dir() is only evaluated after it.  BiCGSTAB: beta, omega
@@ -407,10 +416,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     TIO bt = TIO(0), om = TIO(0), al = TIO(0);
     bool first = false;
     auto dir = [&](TIO zv, TIO pv) { return first ? zv : zv + bt * pv; };
-    // operator input of a plane node from its loads a (R), c, d
-    auto stab_in = [&](TIO a, TIO c, TIO d) {
-      if (SA) return first ? a : a + bt * (c - om * d);
-      return a - al * c;
+    // kStabDir, one node from its loads R, V, P, T of the previous iteration:
+    // S = R - alpha V, the new R = S - omega T and P = R + beta (P - omega V)
+    // (iteration 0: R and P = R as loaded)
+    auto stab_rp = [&](TIO a, TIO c, TIO d, TIO t, TIO &sv, TIO &rv, TIO &pv) {
+      sv = a - al * c;
+      rv = first ? a : sv - om * t;
+      pv = first ? a : rv + bt * (d - om * c);
     };
 
     // this thread's two plane nodes of plane gz (raw values and flags; value
@@ -425,31 +437,33 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       r.m0 = bload<uint8_t>(rk, vm0, sm);
       r.a0 = bload<TIO>(ra, vp0, sv);
       r.b0 = DUAL ? bload<TIO>(rb, vp0, sv) : TIO(0);
+      const Res rt_ = in ? rto : rx1z;
       r.c0 = (DIR || STAB) ? bload<TIO>(rc, vp0, sv) : TIO(0);
       r.d0 = (DIR || SA) ? bload<TIO>(rd, vp0, sv) : TIO(0);
+      r.t0 = SA ? bload<TIO>(rt_, vp0, sv) : TIO(0);
       r.f = STAB ? bload<TIO>(in ? rdg : rx1z, vo, sv) : TIO(0);
-      r.g = SA ? bload<TIO>(in ? rrt : rx1z, vo, sv) : TIO(0);
+      r.g = STAB ? bload<TIO>(in ? rrt : rx1z, vo, sv) : TIO(0);
       // x of owned nodes of the march's own planes only
       const Res re = gz < k1 ? rxs : rx1z;
-      r.e0 = DIR ? bload<TIO>(re, vn0, sv) : TIO(0);
-      r.a1 = r.b1 = r.c1 = r.d1 = r.e1 = TIO(0);
+      r.e0 = (DIR || SA) ? bload<TIO>(re, vn0, sv) : TIO(0);
+      r.a1 = r.b1 = r.c1 = r.d1 = r.e1 = r.t1 = TIO(0);
       r.m1 = 1;
       if (has1) {
         r.m1 = bload<uint8_t>(rk, vm1, sm);
         r.a1 = bload<TIO>(ra, vp1, sv);
         if (DUAL) r.b1 = bload<TIO>(rb, vp1, sv);
-        if (DIR) {
-          r.c1 = bload<TIO>(rc, vp1, sv);
+        if (DIR || STAB) r.c1 = bload<TIO>(rc, vp1, sv);
+        if (DIR || SA) {
           r.d1 = bload<TIO>(rd, vp1, sv);
           r.e1 = bload<TIO>(re, vn1, sv);
         }
-        if (STAB) r.c1 = bload<TIO>(rc, vp1, sv);
-        if (SA) r.d1 = bload<TIO>(rd, vp1, sv);
+        if (SA) r.t1 = bload<TIO>(rt_, vp1, sv);
       }
     };
     // plane gz into LDS; DIR: the directions are formed first and those of
     // the march's own planes stored
-    auto plane_put = [&](const PlanePair<TIO, DIR> &r, int gz, T *p1, T *p2, uint8_t *pk) {
+    auto plane_put = [&](const PlanePair<TIO, DIR> &r, int gz, T *p1, T *p2, uint8_t *pk,
+                         T *prr) {
       TIO a0 = r.a0, b0 = r.b0, a1 = r.a1, b1 = r.b1;
       if (DIR) {
         a0 = dir(r.a0, r.c0);
@@ -468,14 +482,27 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
           }
         }
       }
-      if (STAB) {
-        a0 = stab_in(r.a0, r.c0, r.d0);
-        a1 = stab_in(r.a1, r.c1, r.d1);
-        if (SA && gz >= k0 && gz < k1) {
+      if (SA) {
+        TIO s0, r0, s1, r1;
+        stab_rp(r.a0, r.c0, r.d0, r.t0, s0, r0, a0);
+        stab_rp(r.a1, r.c1, r.d1, r.t1, s1, r1, a1);
+        if (gz >= k0 && gz < k1) {
           const int sv = uni(gz * plane * SI);
           bstore(a0, rpn, vn0, sv);
-          if (has1) bstore(a1, rpn, vn1, sv);
+          bstore(r0, rrn, vn0, sv);
+          if (!first) bstore(r.e0 + al * r.d0 + om * s0, rxs, vn0, sv);
+          if (has1) {
+            bstore(a1, rpn, vn1, sv);
+            bstore(r1, rrn, vn1, sv);
+            if (!first) bstore(r.e1 + al * r.d1 + om * s1, rxs, vn1, sv);
+          }
         }
+        prr[q0] = r.m0 ? T(0) : T(r0);
+        if (has1) prr[q1] = r.m1 ? T(0) : T(r1);
+      }
+      if (SB) {
+        a0 = r.a0 - al * r.c0;
+        a1 = r.a1 - al * r.c1;
       }
       p1[q0] = r.m0 ? T(0) : T(a0);
       if (DUAL) p2[q0] = r.m0 ? T(0) : T(b0);
@@ -494,25 +521,25 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       load_rec<T>(kz == kc ? rrc : rx1z, vr, uni(kc * eplane * SR), c);
     };
 
-    // dot terms: each thread sums its owned nodes of the current unit (its
-    // planes in order); at the unit's last plane the block sum goes to
-    // red[unit & 1] and thread 0 stores it after the next barrier
-    // (kStabS: a second sum, t . t, in dacc2 / red2 / dm2)
-    double dacc = 0.0, dacc2 = 0.0;
+    // dot terms (ND sums): each thread sums its owned nodes of the current
+    // unit (its planes in order); at the unit's last plane the block sums go
+    // to red[.][unit & 1] and thread 0 stores them after the next barrier
+    double dacc[ND];
+#pragma unroll
+    for (int i = 0; i < ND; ++i) dacc[i] = 0.0;
     int pend = -1;  // unit whose block sum waits for the next barrier
     // a z-chunk launch stores one partial per group: the per-thread sum over
     // the march is reduced once at its end
-    auto dot_emit = [&](double d, double d2, int k) {
-      dacc += d;
-      if (SB) dacc2 += d2;
+    auto dot_emit = [&](const double (&dv)[ND], int k) {
+#pragma unroll
+      for (int i = 0; i < ND; ++i) dacc[i] += dv[i];
       if (gidx < 0 && (k % kUnitPlanes == kUnitPlanes - 1 || k == naz - 1)) {
-        const double v = wave_sum_lane63(dacc);
-        if ((tid & 63) == 63) red[(k / kUnitPlanes) & 1][tid >> 6] = v;
-        if (SB) {
-          const double v2 = wave_sum_lane63(dacc2);
-          if ((tid & 63) == 63) red2[(k / kUnitPlanes) & 1][tid >> 6] = v2;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) {
+          const double v = wave_sum_lane63(dacc[i]);
+          if ((tid & 63) == 63) red[i][(k / kUnitPlanes) & 1][tid >> 6] = v;
+          dacc[i] = 0.0;
         }
-        dacc = dacc2 = 0.0;
         pend = k / kUnitPlanes;
       }
     };
@@ -521,13 +548,15 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     auto dot_flush = [&]() {
       if (gidx >= 0) return;
       const int pb = pend < 0 ? 0 : pend;
-      double t = 0.0, t2 = 0.0;
-      for (int w = 0; w < NW; ++w) t += red[pb & 1][w];
-      if (SB)
-        for (int w = 0; w < NW; ++w) t2 += red2[pb & 1][w];
+      double t[ND];
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        t[i] = 0.0;
+        for (int w = 0; w < NW; ++w) t[i] += red[i][pb & 1][w];
+      }
       if (pend >= 0 && tid == 0) {
-        dm[pend] = t;
-        if (SB) dm2[pend] = t2;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) dm[i][pend] = t[i];
       }
       pend = -1;
     };
@@ -535,14 +564,14 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     // prologue: the bottom face of layer k0-1 (plane k0-1) straight from
     // memory, plane k0 and the fields of layer k0-1 ahead; the loads are
     // issued before the status test so that they overlap its reductions
-    TIO ca[4], cb[4], cc[4], cd[4];
+    TIO ca[4], cb[4], cc[4], cd[4], ct[4];
     uint8_t cm[4];
     {
       const int gz = max(k0 - 1, 0);
       const int sm = uni(gz * plane), sv = uni(gz * plane * SI);
       const bool zin = k0 >= 1;
       const Res ra = zin ? rx1 : rx1z, rb = zin ? rx2 : rx1z, rk = zin ? rmk : rx1z;
-      const Res rc = zin ? rpo : rx1z, rd = zin ? rppo : rx1z;
+      const Res rc = zin ? rpo : rx1z, rd = zin ? rppo : rx1z, rt_ = zin ? rto : rx1z;
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int col = colofs(exg + (c == 1 || c == 2), eyg + (c >= 2));
@@ -552,6 +581,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
         cb[c] = DUAL ? bload<TIO>(rb, vcs, sv) : TIO(0);
         cc[c] = (DIR || STAB) ? bload<TIO>(rc, vcs, sv) : TIO(0);
         cd[c] = (DIR || SA) ? bload<TIO>(rd, vcs, sv) : TIO(0);
+        ct[c] = SA ? bload<TIO>(rt_, vcs, sv) : TIO(0);
       }
     }
     PlanePair<TIO, DIR> pr;
@@ -618,71 +648,81 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       bt = TIO(beta);
     }
     if (STAB) {
-      // the status of the previous launch, read once per block and broadcast
-      // (one block of this launch writes it; done is committed by the kStabS
-      // launch after a kStabDir launch found it, so no block of a launch sees
-      // it change)
+      // the status and the partials of the previous launch, read once per
+      // block and broadcast.  Each launch writes only fields the other one
+      // reads (kStabDir: rho_next, omega, pad, pend; kStabS: rho, alpha,
+      // iter, err, done), so no block sees a field change that it reads.
       SolveStatus *stm = sd.st;
       if (tid == 0) {
         sflag[0] = stm->done;
-        sflag[1] = SA ? stm->iter : stm->pend;
-        sstab[0] = stm->rho;
+        sflag[1] = SA ? stm->iter : stm->pad;  // index j of this iteration
+        sflag[2] = stm->pend;
+        sstab[0] = SA ? stm->rho : stm->rho_next;
         sstab[1] = stm->alpha;
-        sstab[2] = stm->omega;
       }
       const bool writer = tile == 0 && k0 == 0 && tid == 0;  // one block
       double sums[4];
-      if (SA) {
-        const double *const parts[4] = {sd.prt, sd.prr, sd.pbn, sd.pbn};
-        const int ns[4] = {sd.NB, sd.NB, sd.NB, 0};
-        blk_sum4(parts, ns, sums);
-      } else {
-        const double *const parts[4] = {sd.prt, sd.pga, sd.prt, sd.prt};
-        const int ns[4] = {sd.NB, sd.na, 0, 0};
-        blk_sum4(parts, ns, sums);
-      }
-      const int pdone = sflag[0], fl = sflag[1];
-      const double rho_old = sstab[0], al_old = sstab[1], om_old = sstab[2];
+      blk_sum4(sd.pin, sd.nin, sums);  // its barriers also publish sflag
+      const int pdone = sflag[0], j = sflag[1], ppend = sflag[2];
+      const double rho_a = sstab[0], al_a = sstab[1];
       __syncthreads();  // sflag is rewritten by the block's next march
       if (pdone) return;
       if (SA) {
-        const int k = fl + 1;
-        const double rho = sums[0];
+        // sums: t.s, t.t, r~.s, r~.t of the previous S launch
+        first = j == 0;
+        double rho = 0.0, omega = 0.0, beta = 0.0;
+        if (first) {
+          // rho_0 = r0 . r0 from the initial partials
+          double s0[4];
+          const double *const p0[4] = {sd.prt0, sd.prt0, sd.prt0, sd.prt0};
+          const int n0[4] = {sd.nb0, 0, 0, 0};
+          blk_sum4(p0, n0, s0);
+          rho = s0[0];
+        } else {
+          omega = sums[0] / sums[1];
+          rho = sums[2] - omega * sums[3];
+          beta = (rho / rho_a) * (al_a / omega);
+          if (!(beta == beta) || !(omega == omega) || omega == 0.0 || rho == 0.0 ||
+              sums[1] == 0.0) {
+            if (writer) stm->pend = 3;
+            return;
+          }
+        }
+        if (writer) {
+          stm->rho_next = rho;
+          stm->omega = omega;
+          stm->pad = j;
+        }
+        bt = TIO(beta);
+        om = TIO(omega);
+        al = TIO(al_a);
+      } else {
+        // sums: RT.V and |r_j|^2 of the kStabDir launch, |b|^2
+        if (ppend) {  // the kStabDir launch found a breakdown
+          if (writer) stm->done = ppend;
+          return;
+        }
         const double err = sqrt(sums[1] / sums[2]);
+        const double gam = sums[0];
+        const double alpha = rho_a / gam;
         int done = 0;
         if (err <= sd.tol) done = 1;
-        else if (k > sd.itmax) done = 2;
+        else if (j >= sd.itmax) done = 2;
         else if (!(err == err)) done = 3;
         else if (err > sd.divtol) done = 4;
-        double beta = 0.0;
-        if (!done && k > 1) {
-          beta = (rho / rho_old) * (al_old / om_old);
-          if (!(beta == beta) || rho == 0.0 || om_old == 0.0) done = 3;
-        }
+        else if (!(alpha == alpha) || gam == 0.0) done = 3;
         if (writer) {
           stm->err = err;
           stm->bnrm2 = sums[2];
-          if (done) stm->pend = done;
+          stm->iter = done ? j : j + 1;
+          if (done) {
+            stm->done = done;
+          } else {
+            stm->rho = rho_a;
+            stm->alpha = alpha;
+          }
         }
         if (done) return;
-        first = k == 1;
-        bt = TIO(beta);
-        om = TIO(om_old);
-      } else {
-        if (fl) {  // the kStabDir launch ended the solve
-          if (writer) stm->done = fl;
-          return;
-        }
-        const double rho = sums[0], gam = sums[1];
-        const double alpha = rho / gam;
-        if (!(alpha == alpha) || gam == 0.0) {
-          if (writer) stm->done = 3;
-          return;
-        }
-        if (writer) {
-          stm->rho = rho;
-          stm->alpha = alpha;
-        }
         al = TIO(alpha);
       }
     }
@@ -691,7 +731,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       T n1[4], n2[4];
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
-        const TIO a = DIR ? dir(ca[c], cc[c]) : STAB ? stab_in(ca[c], cc[c], cd[c]) : ca[c];
+        TIO a = DIR ? dir(ca[c], cc[c]) : ca[c];
+        if (SA) {
+          TIO sv_, rv_;
+          stab_rp(ca[c], cc[c], cd[c], ct[c], sv_, rv_, a);
+        }
+        if (SB) a = ca[c] - al * cc[c];
         const TIO b = DIR ? dir(cb[c], cd[c]) : cb[c];
         n1[c] = cm[c] ? T(0) : T(a);
         n2[c] = cm[c] ? T(0) : T(b);
@@ -704,7 +749,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
 #pragma unroll
     for (int c = 0; c < 4; ++c) ot1[c] = ot2[c] = T(0);
     uint8_t omk[2] = {1, 1};    // flag of the owned node of the plane read 2 layers ago
-    double xok[2] = {0.0, 0.0};  // its x2 value (for the dot; kStabS: its S)
+    double xok[2] = {0.0, 0.0};  // its x2 value (for the dot; kStabS: its S;
+                                 // kStabDir: its new R)
     TIO fok[2] = {TIO(0), TIO(0)}, gok[2] = {TIO(0), TIO(0)};  // BiCGSTAB: its d, RT
     int kl = k0 - 1;
 
@@ -724,27 +770,39 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
         }
       }
       const int so = uni((kl - 1) * plane * SI);
-      // kStabDir: V = (A P)/d (d is 0 only where the node is not stored)
+      // BiCGSTAB: V = (A P)/d, T = (A S)/d (d is 0 only where the node is
+      // not stored)
       const double dg = (double)fok[P];
-      const TIO o1 = omk[P] ? TIO(0) : SA ? TIO(dg != 0.0 ? v1 / dg : 0.0) : TIO(v1);
+      const TIO o1 = omk[P] ? TIO(0) : STAB ? TIO(dg != 0.0 ? v1 / dg : 0.0) : TIO(v1);
       bstore(o1, ry1, vo, so);
-      double d = 0.0, d2 = 0.0;
+      double dv[ND];
+#pragma unroll
+      for (int i = 0; i < ND; ++i) dv[i] = 0.0;
       if (DUAL) {
         bstore(omk[P] ? TIO(0) : TIO(v2), ry2, vo, so);
-        d = (double)o1 * xok[P];
+        dv[0] = (double)o1 * xok[P];
       }
-      if (SA) d = (double)o1 * (double)gok[P];              // RT . V
+      const double rtv = (double)gok[P];
+      if (SA) {
+        dv[0] = (double)o1 * rtv;                 // RT . V
+        const double rn = dg * xok[P];
+        dv[ND > 1 ? 1 : 0] = rn * rn;             // |r|^2, r = d R
+      }
       if (SB) {
-        d = (double)o1 * (dg * xok[P]);                     // t . s, s = d S
-        d2 = (double)o1 * (double)o1;                       // t . t
+        const double tv = dg * (double)o1, sv = dg * xok[P];  // t, s at the node
+        dv[0] = tv * sv;                          // t . s
+        dv[ND > 1 ? 1 : 0] = tv * tv;             // t . t
+        dv[ND > 2 ? 2 : 0] = rtv * xok[P];        // r~ . s = RT . S
+        dv[ND > 3 ? 3 : 0] = rtv * (double)o1;    // r~ . t = RT . T
       }
-      if (want_dot) dot_emit(d, d2, kl - 1);
+      if (want_dot) dot_emit(dv, kl - 1);
     };
 
     // one element layer kl with plane kl+1 in buffer P
     auto layer = [&](auto Pc) {
       constexpr int P = decltype(Pc)::value;
-      plane_put(pr, kl + 1, pl1[P], DUAL ? pl2[DUAL ? P : 0] : nullptr, pm[P]);
+      plane_put(pr, kl + 1, pl1[P], DUAL ? pl2[DUAL ? P : 0] : nullptr, pm[P],
+                plR[SA ? P : 0]);
       const TIO fon = pr.f, gon = pr.g;  // owned node of plane kl+1
       cf = cn;
       plane_fetch(kl + 2, pr);  // prefetch plane kl+2 and layer kl+1
@@ -768,7 +826,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       }
       const uint8_t omn = pm[P][oq];
       const double xon = (DUAL && own) ? (double)pl2[DUAL ? P : 0][oq]
-                         : (SB && own) ? (double)pl1[P][oq] : 0.0;
+                         : (SB && own) ? (double)pl1[P][oq]
+                         : (SA && own) ? (double)plR[SA ? P : 0][oq] : 0.0;
       // owner sums and output of plane kl-1
       if (kl - 1 >= k0) emit_plane(std::integral_constant<int, P ^ 1>(), Pc);
       omk[P] = omn;
@@ -849,21 +908,18 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     lds_barrier();  // buffers are refilled by the next march
     if (want_dot) {
       if (gidx >= 0) {
-        const double v = wave_sum_lane63(dacc);
-        if ((tid & 63) == 63) red[0][tid >> 6] = v;
-        if (SB) {
-          const double v2 = wave_sum_lane63(dacc2);
-          if ((tid & 63) == 63) red2[0][tid >> 6] = v2;
+#pragma unroll
+        for (int i = 0; i < ND; ++i) {
+          const double v = wave_sum_lane63(dacc[i]);
+          if ((tid & 63) == 63) red[i][0][tid >> 6] = v;
         }
         lds_barrier();
         if (tid == 0) {
-          double t = 0.0;
-          for (int w = 0; w < NW; ++w) t += red[0][w];
-          dm[0] = t;
-          if (SB) {
-            double t2 = 0.0;
-            for (int w = 0; w < NW; ++w) t2 += red2[0][w];
-            dm2[0] = t2;
+#pragma unroll
+          for (int i = 0; i < ND; ++i) {
+            double t = 0.0;
+            for (int w = 0; w < NW; ++w) t += red[i][0][w];
+            dm[i][0] = t;
           }
         }
       } else {
@@ -1270,7 +1326,7 @@ template int launch_owner_dir<float, float>(const Member<float> &, const Member<
 
 template <typename T, typename TIO>
 int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<TIO> &sd,
-                      const TIO *r, TIO *y1, double *dot, hipStream_t s) {
+                      const TIO *r, TIO *y1, hipStream_t s) {
   const Grid3 &g = op.g;
   if (!owner_fits(g)) {
     set_error("owner scatter: system too large for 32-bit offsets");
@@ -1289,7 +1345,6 @@ int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<
   L.zchunk = zchunk;
   L.x1 = r;
   L.y1 = y1;
-  L.dot = dot;
   L.sd = sd;
   if (mode == kStabDir)
     return launch_owner_any<T, TIO, false, kStabDir>(L, op.tile, op.ra != 0.0, s);
@@ -1298,10 +1353,10 @@ int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<
 
 template int launch_owner_stab<double, double>(const OpData<double> &, int, int,
                                                const StabData<double> &, const double *,
-                                               double *, double *, hipStream_t);
+                                               double *, hipStream_t);
 template int launch_owner_stab<float, float>(const OpData<float> &, int, int,
-                                             const StabData<float> &, const float *,
-                                             float *, double *, hipStream_t);
+                                             const StabData<float> &, const float *, float *,
+                                             hipStream_t);
 
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,

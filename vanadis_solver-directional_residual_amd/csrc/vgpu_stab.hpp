@@ -7,19 +7,20 @@
 // ||r||/||b|| <= tol, cudatest.cu:491-553).
 //
 // The recurrences run on the scaled vectors R = D^-1 r, P = D^-1 p,
-// V = D^-1 A P, S = D^-1 s and the shadow RT = D r0 (r0 the initial
-// residual, so r0 . r = RT . R):
-//   rho = RT . R;  beta = (rho / rho_old)(alpha / omega)
-//   P = R + beta (P - omega V);  V = D^-1 A P;  alpha = rho / (RT . V)
-//   S = R - alpha V;  t = A S;  omega = (t . s) / (t . t), s = D S
-//   x += alpha P + omega S;  R = S - omega D^-1 t;  |r|^2 = sum (d R)^2
-// One iteration = the kStabDir owner launch (test, beta, P formed as the
-// nodes are loaded, V, RT . V partials), the kStabS owner launch (alpha,
-// S formed on load, t, t . s and t . t partials) and ks_update; pairs of
+// V = D^-1 A P, S = D^-1 s, T = D^-1 t and the shadow RT = D r0 (r0 the
+// initial residual, so r0 . r = RT . R), iteration j:
+//   rho_j = r~.s - omega r~.t (= RT . R_j);  beta = (rho_j / rho_{j-1}) (alpha / omega)
+//   S = R - alpha V;  R_j = S - omega T;  x += alpha P + omega S
+//   P_j = R_j + beta (P - omega V);  V_j = D^-1 A P_j;  alpha_j = rho_j / (RT . V_j)
+//   S_j = R_j - alpha_j V_j;  T_j = D^-1 A S_j;  omega_j = (t . s) / (t . t)
+// with t = d T, s = d S.  One iteration = the kStabDir owner launch (R_j,
+// P_j and x formed as the plane nodes are loaded, V_j, partials of RT . V_j
+// and |r_j|^2) and the kStabS owner launch (the test of |r_j|, alpha_j, S_j
+// formed on load, T_j, partials of t.s, t.t, r~.s, r~.t); pairs of
 // iterations are captured in one hipGraph and replayed until the status
-// reports done.  Directions P and V alternate between two buffers by
-// iteration parity: a launch reads the previous ones at its tiles' halo
-// while it stores the new ones.
+// reports done.  R, P and V alternate between two buffers by iteration
+// parity: a launch reads the previous ones at its tiles' halo while it
+// stores the new ones.
 #pragma once
 #include <algorithm>
 #include <functional>
@@ -61,7 +62,7 @@ __device__ __forceinline__ void stab_partial2(double u, double v, double *a, dou
 }
 
 // r = b - A x0 (ax = A x0), R = r / d, RT = d r (shadow r0 = r); x zeroed on
-// the mask; partials of RT . R, |r|^2 and |b|^2.
+// the mask; partials of RT . R (= r0 . r0), |r|^2 and |b|^2.
 template <typename TV>
 __global__ __launch_bounds__(SVB) void ks_init(int64_t n, const uint8_t *__restrict__ mask,
                                               const TV *__restrict__ b, TV *__restrict__ x,
@@ -96,88 +97,25 @@ __global__ __launch_bounds__(SVB) void ks_init(int64_t n, const uint8_t *__restr
   }
 }
 
-// omega = (t . s) / (t . t) from the kStabS launch's partials (na of each),
-// then S = R - alpha V, x += alpha P + omega S, R = S - omega t / d and the
-// partials of RT . R and |r|^2 = (d R)^2.  alpha comes from the status
-// (written by the kStabS launch); one workgroup writes omega and the count.
-template <typename TV>
-__global__ __launch_bounds__(SVB) void ks_update(
-    int64_t n, SolveStatus *__restrict__ st, int na, const double *__restrict__ pts,
-    const double *__restrict__ ptt, TV *__restrict__ R, const TV *__restrict__ V,
-    const TV *__restrict__ P, const TV *__restrict__ t, const TV *__restrict__ diag,
-    const TV *__restrict__ RT, TV *__restrict__ x, double *__restrict__ prt,
-    double *__restrict__ prr) {
-  __shared__ double red2[2][SVB / 64];
-  __shared__ int sdone;
-  __shared__ double salpha;
-  const int tid = threadIdx.x;
-  if (tid == 0) {
-    sdone = st->done;
-    salpha = st->alpha;
-  }
-  // the partials, per thread in index order, then over the block
-  double a = 0.0, c = 0.0;
-  for (int i = tid; i < na; i += SVB) {
-    a += pts[i];
-    c += ptt[i];
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off, 64);
-    c += __shfl_down(c, off, 64);
-  }
-  if ((tid & 63) == 0) {
-    red2[0][tid >> 6] = a;
-    red2[1][tid >> 6] = c;
-  }
-  __syncthreads();
-  if (sdone) return;
-  double ts = 0.0, tt = 0.0;
-  for (int w = 0; w < SVB / 64; ++w) {
-    ts += red2[0][w];
-    tt += red2[1][w];
-  }
-  const double omega = ts / tt;
-  const bool w0 = blockIdx.x == 0 && tid == 0;
-  if (!(omega == omega) || tt == 0.0) {
-    if (w0) st->done = 3;
-    return;
-  }
-  const TV al = TV(salpha), om = TV(omega);
-  double sr = 0.0, sn = 0.0;
-  const int64_t stride = (int64_t)gridDim.x * SVB;
-  for (int64_t i = blockIdx.x * (int64_t)SVB + tid; i < n; i += stride) {
-    const TV di = diag[i];
-    const TV Si = R[i] - al * V[i];
-    x[i] = x[i] + al * P[i] + om * Si;
-    const TV Rn = Si - om * (t[i] / di);
-    R[i] = Rn;
-    const double rn = (double)Rn * (double)di;
-    sr += (double)RT[i] * (double)Rn;
-    sn += rn * rn;
-  }
-  stab_partial2(sr, sn, prt, prr, blockIdx.x);
-  if (w0) {
-    st->omega = omega;
-    st->iter = st->iter + 1;
-  }
-}
-
 // Operator callbacks: y = A x of the system (ignores the status), and one
 // BiCGSTAB owner launch (mode kStabDir or kStabS).
 template <typename TV>
 using StabSingleOp = std::function<int(const TV *x, TV *y, hipStream_t s)>;
 template <typename TV>
 using StabOwnerOp = std::function<int(int mode, const StabData<TV> &sd, const TV *r, TV *y,
-                                      double *dot, hipStream_t s)>;
+                                      hipStream_t s)>;
 
 template <typename TV>
 struct Stab {
   int64_t n = 0;
   int NB = 0, na = 0;
-  TV *ax = nullptr, *R = nullptr, *RT = nullptr, *t = nullptr;
-  TV *P[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr};
-  double *prt = nullptr, *prr = nullptr, *pbn = nullptr;
-  double *pga = nullptr, *pts = nullptr, *ptt = nullptr;
+  TV *ax = nullptr, *RT = nullptr, *T = nullptr;
+  TV *R[2] = {nullptr, nullptr}, *P[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr};
+  double *prt = nullptr, *prr = nullptr, *pbn = nullptr;  // initial partials [NB]
+  // direction launch partials (RT.V, |r|^2), S launch partials (t.s, t.t,
+  // r~.s, r~.t) [cap]
+  double *pga = nullptr, *prn = nullptr;
+  double *pts = nullptr, *ptt = nullptr, *prs = nullptr, *prq = nullptr;
   SolveStatus *st = nullptr, *st_host = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipGraph_t graph = nullptr;
@@ -189,7 +127,7 @@ struct Stab {
   // (chunk, tile) groups)
   int alloc(int64_t nn, int nna) {
     nna = std::max(1, nna);
-    if (R && nn == n && nna <= cap) {
+    if (RT && nn == n && nna <= cap) {
       if (nna != na) drop_graph();
       na = nna;
       return VGPU_OK;
@@ -198,15 +136,18 @@ struct Stab {
     n = nn;
     NB = stab_blocks(n);
     na = cap = nna;
-    TV **v[] = {&ax, &R, &RT, &t, &P[0], &P[1], &V[0], &V[1]};
+    TV **v[] = {&ax, &RT, &T, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
     for (auto pv : v) {
       VGPU_HIP(hipMalloc(pv, std::max<int64_t>(1, n) * sizeof(TV)));
       VGPU_HIP(hipMemset(*pv, 0, std::max<int64_t>(1, n) * sizeof(TV)));
     }
     double **pn[] = {&prt, &prr, &pbn};
     for (auto pd : pn) VGPU_HIP(hipMalloc(pd, NB * sizeof(double)));
-    double **pa[] = {&pga, &pts, &ptt};
-    for (auto pd : pa) VGPU_HIP(hipMalloc(pd, cap * sizeof(double)));
+    double **pa[] = {&pga, &prn, &pts, &ptt, &prs, &prq};
+    for (auto pd : pa) {
+      VGPU_HIP(hipMalloc(pd, cap * sizeof(double)));
+      VGPU_HIP(hipMemset(*pd, 0, cap * sizeof(double)));
+    }
     VGPU_HIP(hipMalloc(&st, sizeof(SolveStatus)));
     VGPU_HIP(hipHostMalloc(&st_host, sizeof(SolveStatus), hipHostMallocDefault));
     return VGPU_OK;
@@ -220,13 +161,13 @@ struct Stab {
   }
   void release() {
     drop_graph();
-    TV **v[] = {&ax, &R, &RT, &t, &P[0], &P[1], &V[0], &V[1]};
+    TV **v[] = {&ax, &RT, &T, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
     for (auto pv : v)
       if (*pv) {
         hipFree(*pv);
         *pv = nullptr;
       }
-    double **d[] = {&prt, &prr, &pbn, &pga, &pts, &ptt};
+    double **d[] = {&prt, &prr, &pbn, &pga, &prn, &pts, &ptt, &prs, &prq};
     for (auto pd : d)
       if (*pd) {
         hipFree(*pd);
@@ -248,8 +189,9 @@ struct Stab {
             double divtol, hipStream_t s, int *iters, double *err, int *status) {
     int rc = single(x, ax, s);  // A x0 (x0 masked on load)
     if (rc) return rc;
+    // R_0 goes to the buffer the first captured iteration (parity 1) reads
     hipLaunchKernelGGL((ks_init<TV>), dim3(NB), dim3(SVB), 0, s, n, mask, b, x, ax, diag,
-                       R, RT, prt, prr, pbn);
+                       R[0], RT, prt, prr, pbn);
     VGPU_HIP(hipGetLastError());
     SolveStatus s0{};
     s0.rho = s0.alpha = s0.omega = 1.0;
@@ -336,36 +278,57 @@ struct Stab {
 
   int enqueue_iteration(const StabOwnerOp<TV> &owner, const TV *diag, TV *x, double tol,
                         int itmax, double divtol, int q, hipStream_t s) {
+    int rc = owner(kStabDir, dir_data(q, diag, x, tol, itmax, divtol), R[q ^ 1], V[q], s);
+    if (rc) return rc;
+    return owner(kStabS, s_data(q, diag, tol, itmax, divtol), R[q], T, s);
+  }
+
+ public:
+  // Launch data of the direction launch (input R[q^1], output V[q]) and of
+  // the S launch (input R[q], output T) of an iteration of parity q.
+  StabData<TV> dir_data(int q, const TV *diag, TV *x, double tol, int itmax,
+                        double divtol) const {
     StabData<TV> sa;
     sa.po = P[q ^ 1];
     sa.vo = V[q ^ 1];
+    sa.to = T;
     sa.pn = P[q];
+    sa.rn = R[q];
+    sa.x = x;
     sa.d = diag;
     sa.rt = RT;
-    sa.prt = prt;
-    sa.prr = prr;
-    sa.pbn = pbn;
+    double *const ain[4] = {pts, ptt, prs, prq};
+    for (int i = 0; i < 4; ++i) {
+      sa.pin[i] = ain[i];
+      sa.nin[i] = na;
+    }
+    sa.prt0 = prt;
+    sa.nb0 = NB;
+    sa.pout[0] = pga;
+    sa.pout[1] = prn;
     sa.st = st;
     sa.tol = tol;
     sa.divtol = divtol;
     sa.itmax = itmax;
-    sa.NB = NB;
-    sa.na = na;
-    int rc = owner(kStabDir, sa, R, V[q], pga, s);
-    if (rc) return rc;
-    StabData<TV> sb = sa;
-    sb.po = sb.vo = nullptr;
-    sb.pn = nullptr;
-    sb.rt = nullptr;
+    return sa;
+  }
+  StabData<TV> s_data(int q, const TV *diag, double tol, int itmax, double divtol) const {
+    StabData<TV> sb;
     sb.v = V[q];
-    sb.pga = pga;
-    sb.dot2 = ptt;
-    rc = owner(kStabS, sb, R, t, pts, s);
-    if (rc) return rc;
-    hipLaunchKernelGGL((ks_update<TV>), dim3(NB), dim3(SVB), 0, s, n, st, na, pts, ptt, R,
-                       V[q], P[q], t, diag, RT, x, prt, prr);
-    VGPU_HIP(hipGetLastError());
-    return VGPU_OK;
+    sb.d = diag;
+    sb.rt = RT;
+    sb.pin[0] = pga;
+    sb.pin[1] = prn;
+    sb.pin[2] = pbn;
+    sb.nin[0] = sb.nin[1] = na;
+    sb.nin[2] = NB;
+    double *const bout[4] = {pts, ptt, prs, prq};
+    for (int i = 0; i < 4; ++i) sb.pout[i] = bout[i];
+    sb.st = st;
+    sb.tol = tol;
+    sb.divtol = divtol;
+    sb.itmax = itmax;
+    return sb;
   }
 };
 
