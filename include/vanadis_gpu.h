@@ -160,7 +160,10 @@ int vgpu_assemble_rhs(vgpu_ctx *ctx, const double *q_in, const double *t_old,
 int vgpu_solve(vgpu_ctx *ctx, int precision, const double *b, double *x,
                double rtol, int itmax, double inner_rtol, int *iters,
                double *err);
-/* Same with b, x already on the device (FP64 [N]); diagnostic timings. */
+/* Same with b, x already on the device (FP64 [N]).  seconds (optional): wall
+ * time of the solve between two stream synchronisations; NULL skips the timer
+ * and the synchronisation before the solve (the solve queues behind the
+ * caller's work on the context's stream). */
 int vgpu_solve_dev(vgpu_ctx *ctx, int precision, const double *b_dev,
                    double *x_dev, double rtol, int itmax, double inner_rtol,
                    int *iters, double *err, double *seconds);

@@ -296,13 +296,16 @@ class Context:
         return a
 
     def solve_dev(self, b_dev, x_dev, rtol=1e-8, itmax=20000, precision=FP64,
-                  inner_rtol=0.0):
+                  inner_rtol=0.0, timed=True):
+        """Returns (iterations, err, seconds, status code); timed=False skips
+        the solve's own timer (and the stream synchronisation before it):
+        seconds is then 0."""
         it = C.c_int(0)
         err = C.c_double(0.0)
         sec = C.c_double(0.0)
         rc = load().vgpu_solve_dev(self.h, int(precision), b_dev, x_dev, float(rtol),
                                    int(itmax), float(inner_rtol), C.byref(it), C.byref(err),
-                                   C.byref(sec))
+                                   C.byref(sec) if timed else None)
         return it.value, err.value, sec.value, rc
 
     def time_kernel(self, what=0, scatter="owner", precision=FP64, reps=20):

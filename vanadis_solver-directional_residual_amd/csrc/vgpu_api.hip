@@ -271,6 +271,21 @@ int solve_owner(vgpu_ctx *c, BBicg<TV> &eng, const TV *b, TV *x, const TV *diag,
                    iters, err, status, divtol);
 }
 
+// Z chunk of the BiCGSTAB launches: the context's, or with none set (0, the
+// balanced launch, whose per-plane dot partials every workgroup of both
+// launches would re-sum) the smallest chunk that keeps every (chunk, tile)
+// group of a launch resident at once (two 256-thread workgroups per CU).
+int stab_zchunk(const vgpu_ctx *c) {
+  if (c->zchunk > 0) return c->zchunk;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) !=
+          hipSuccess || cus <= 0)
+    cus = 256;
+  const int64_t tiles = owner_groups(c->g, c->tile, c->g.naz);  // one chunk per tile
+  const int64_t chunks = std::max<int64_t>(1, 2 * (int64_t)cus / std::max<int64_t>(1, tiles));
+  return (int)((c->g.naz + chunks - 1) / chunks);
+}
+
 // One system by BiCGSTAB on the owner launches (whatever the context's
 // scatter mode: the fused launches exist for the owner kernel only).
 template <typename TV>
@@ -281,8 +296,8 @@ int solve_stab(vgpu_ctx *c, Stab<TV> &eng, const TV *b, TV *x, const TV *diag, d
               "stay below 2^31 bytes)");
     return VGPU_ERR_ARG;
   }
-  const int64_t na = c->zchunk > 0 ? owner_groups(c->g, c->tile, c->zchunk)
-                                   : owner_units(c->g, c->tile);
+  const int zc = stab_zchunk(c);
+  const int64_t na = owner_groups(c->g, c->tile, zc);
   int rc = eng.alloc(c->g.N, (int)na);
   if (rc) return rc;
   StabSingleOp<TV> single = [c](const TV *xx, TV *y, hipStream_t s) {
@@ -290,12 +305,12 @@ int solve_stab(vgpu_ctx *c, Stab<TV> &eng, const TV *b, TV *x, const TV *diag, d
     return launch_apply<TV, TV>(op, VGPU_SCATTER_OWNER, c->zchunk, xx, y, nullptr, nullptr,
                                 nullptr, s);
   };
-  StabOwnerOp<TV> owner = [c](int mode, const StabData<TV> &sd, const TV *r, TV *y,
-                              hipStream_t s) {
+  StabOwnerOp<TV> owner = [c, zc](int mode, const StabData<TV> &sd, const TV *r, TV *y,
+                                  hipStream_t s) {
     OpData<TV> op = op_for<TV>(c);
-    return launch_owner_stab<TV, TV>(op, mode, c->zchunk, sd, r, y, s);
+    return launch_owner_stab<TV, TV>(op, mode, zc, sd, r, y, s);
   };
-  const uint64_t key = (c->version << 8) | (uint64_t)(c->zchunk % 64) |
+  const uint64_t key = (c->version << 8) | (uint64_t)(zc % 64) |
                        (sizeof(TV) == 4 ? (1ull << 7) : 0);
   return eng.solve(key, single, owner, c->mask, b, x, diag, tol, itmax, divtol, c->stream,
                    iters, err, status);
@@ -1001,7 +1016,9 @@ int vgpu_solve_dev(vgpu_ctx *c, int precision, const double *b, double *x,
   VGPU_HIP(hipSetDevice(c->device));
   int rc = ensure_diag(c);
   if (rc) return rc;
-  VGPU_HIP(hipStreamSynchronize(c->stream));
+  // the timer brackets the solve only when asked for (a sync before it);
+  // without `seconds` the solve queues behind the caller's work
+  if (seconds) VGPU_HIP(hipStreamSynchronize(c->stream));
   const auto t0 = std::chrono::steady_clock::now();
   int status = 0;
   if (precision == VGPU_FP64)
@@ -1622,9 +1639,7 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   if (what >= 5) {
     int rc = ensure_diag(c);
     if (rc) return rc;
-    const int64_t na = c->zchunk > 0 ? owner_groups(c->g, c->tile, c->zchunk)
-                                     : owner_units(c->g, c->tile);
-    rc = c->stab64.alloc(N, (int)na);
+    rc = c->stab64.alloc(N, (int)owner_groups(c->g, c->tile, stab_zchunk(c)));
     if (rc) return rc;
     Stab<double> &e = c->stab64;
     // partials 1 (r~.s: 2, so that rho = r~.s - omega r~.t is not 0)
@@ -1647,10 +1662,10 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     // inputs from the deterministic vectors: R = vx_, x = vt2
     if (what == 5)
       return launch_owner_stab<double, double>(
-          op, kStabDir, c->zchunk, e.dir_data(0, c->diag, c->vt2, 0.0, 1 << 30, 1e300),
+          op, kStabDir, stab_zchunk(c), e.dir_data(0, c->diag, c->vt2, 0.0, 1 << 30, 1e300),
           c->vx_, e.V[0], c->stream);
     return launch_owner_stab<double, double>(
-        op, kStabS, c->zchunk, e.s_data(0, c->diag, 0.0, 1 << 30, 1e300), c->vx_, e.T,
+        op, kStabS, stab_zchunk(c), e.s_data(0, c->diag, 0.0, 1 << 30, 1e300), c->vx_, e.T,
         c->stream);
   };
   // what 4: a copy of 8 N doubles, the size class of a BiCG iteration's

@@ -65,13 +65,15 @@ constexpr int BVB = 256;
 constexpr int BNB = 1024;
 
 // r = b - A x0 (ax holds A x0), rr = r; stores only the preconditioned
-// z = r/d, zz = rr/d (the engine carries z, zz: r = d z), x zeroed on the
-// mask; partials of r.r, z.rr and b.b (cudatest.cu:464-491,
-// VANADIS.FOR:991-1000).
+// z = r/d, zz = rr/d (the engine carries z, zz: r = d z) and the engine's
+// iterate xw = x0 zeroed on the mask; partials of r.r, z.rr and b.b
+// (cudatest.cu:464-491, VANADIS.FOR:991-1000); each system's initial (zero)
+// status, written on the device.
 template <typename TV>
-__global__ __launch_bounds__(BVB) void kb_init(const Segment *__restrict__ seg, VGPU_ACT_ARGS,
+__global__ __launch_bounds__(BVB) void kb_init(SolveStatus *__restrict__ stv,
+                                              const Segment *__restrict__ seg, VGPU_ACT_ARGS,
                                               const uint8_t *mask, const TV *b,
-                                              TV *x, const TV *ax, TV *z,
+                                              const TV *x, TV *xw, const TV *ax, TV *z,
                                               TV *zz, const TV *diag,
                                               double *pr, double *pb,
                                               double *pbn) {
@@ -79,11 +81,12 @@ __global__ __launch_bounds__(BVB) void kb_init(const Segment *__restrict__ seg, 
   const BlockOf bo = block_of(act, actvb, nact, blockIdx.x);
   const Segment sg = seg[bo.m];
   const int j = bo.j;
+  if (j == 0 && threadIdx.x == 0) stv[bo.m] = SolveStatus{};
   double sr = 0, sb = 0, sn = 0;
   for (int64_t li = j * (int64_t)BVB + threadIdx.x; li < sg.n;
        li += (int64_t)sg.nb * BVB) {
     const int64_t i = sg.off + li;
-    if (mask[i]) x[i] = TV(0);
+    xw[i] = mask[i] ? TV(0) : x[i];
     const double bi = b[i];
     const TV ri = TV(bi - (double)ax[i]);
     const TV zi = TV((double)ri / (double)diag[i]);
@@ -96,6 +99,14 @@ __global__ __launch_bounds__(BVB) void kb_init(const Segment *__restrict__ seg, 
   block_partial_at(sr, pr, sg.vb0 + j, red);
   block_partial_at(sb, pb, sg.vb0 + j, red);
   block_partial_at(sn, pbn, sg.vb0 + j, red);
+}
+
+// y = x over n values
+template <typename TV>
+__global__ __launch_bounds__(BVB) void kb_copy(int64_t n, const TV *__restrict__ x,
+                                              TV *__restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * BVB;
+  for (int64_t i = blockIdx.x * (int64_t)BVB + threadIdx.x; i < n; i += stride) y[i] = x[i];
 }
 
 // Block-wide sums of two per-thread values, each in the order block_sum
@@ -338,6 +349,7 @@ struct BBicg {
   // A x0 at the start
   TV *ax = nullptr, *p[2] = {nullptr, nullptr}, *pp[2] = {nullptr, nullptr},
      *z = nullptr, *zz = nullptr, *w = nullptr, *ww = nullptr;
+  TV *xw = nullptr;  // the iterates (copied to the caller's x at the end)
   double *pr = nullptr, *pb = nullptr, *pbn = nullptr, *pa = nullptr;
   SolveStatus *st = nullptr;       // device [nm]
   SolveStatus *st_host = nullptr;  // pinned [nm]
@@ -373,7 +385,7 @@ struct BBicg {
     rc = full.set(hseg, all);
     if (!rc) rc = cur.set(hseg, all);
     if (rc) return rc;
-    TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
+    TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww, &xw};
     for (auto pv : v) {
       VGPU_HIP(hipMalloc(pv, ntot * sizeof(TV)));
       VGPU_HIP(hipMemset(*pv, 0, ntot * sizeof(TV)));
@@ -396,7 +408,7 @@ struct BBicg {
   }
   void release() {
     drop_graph();
-    TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww};
+    TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww, &xw};
     for (auto pv : v)
       if (*pv) {
         hipFree(*pv);
@@ -422,8 +434,10 @@ struct BBicg {
   ~BBicg() { release(); }
 
   // Solve every system with x, b, diag, mask on the device (batch layout).
-  // op_key identifies the operators; the graph is reused while the key,
-  // buffers, tol, itmax, divtol and the active set are unchanged.  When a
+  // op_key identifies the operators; the iterations run on the engine's own
+  // buffers (x0 copied in by the init kernel, the result copied out at the
+  // end), so the graph is reused while the key, diag, tol, itmax, divtol and
+  // the active set are unchanged, whatever b and x.  When a
   // quarter of the launched systems have finished, the launches are narrowed
   // to the running ones (graph re-captured between replays).  Per-system
   // results go to iters/err/status [nm].
@@ -433,13 +447,10 @@ struct BBicg {
             double divtol = 1e300) {
     int rc = single(x, ax, full.view, s);  // A x0 (x0 masked on load)
     if (rc) return rc;
-    hipLaunchKernelGGL((kb_init<TV>), dim3(full.view.nvb), dim3(BVB), 0, s, seg,
-                       full.view.act, full.view.actvb, full.view.nact, mask, b, x, ax,
+    hipLaunchKernelGGL((kb_init<TV>), dim3(full.view.nvb), dim3(BVB), 0, s, st, seg,
+                       full.view.act, full.view.actvb, full.view.nact, mask, b, x, xw, ax,
                        z, zz, diag, pr, pb, pbn);
     VGPU_HIP(hipGetLastError());
-    std::vector<SolveStatus> s0(nm, SolveStatus{});
-    VGPU_HIP(hipMemcpyAsync(st, s0.data(), nm * sizeof(SolveStatus),
-                            hipMemcpyHostToDevice, s));
     if ((int)cur.members.size() != nm) {  // start with every system launched
       VGPU_HIP(hipStreamSynchronize(s));
       drop_graph();
@@ -448,11 +459,10 @@ struct BBicg {
       rc = cur.set(hseg, all);
       if (rc) return rc;
     }
-    const bool same = key_valid && key_op == op_key && key_b == b && key_x == x &&
-                      key_diag == diag && key_tol == tol && key_itmax == itmax &&
-                      key_div == divtol;
+    const bool same = key_valid && key_op == op_key && key_diag == diag && key_tol == tol &&
+                      key_itmax == itmax && key_div == divtol;
     if (!same) {
-      rc = capture(op_key, dual, diag, b, x, tol, itmax, divtol, s);
+      rc = capture(op_key, dual, diag, tol, itmax, divtol, s);
       if (rc) return rc;
     }
     // replay until every system is done, at most two replays in flight
@@ -497,8 +507,16 @@ struct BBicg {
         if (run.empty()) break;
         result = cur.set(hseg, run);
         if (!result)
-          result = capture(op_key, dual, diag, b, x, tol, itmax, divtol, s);
+          result = capture(op_key, dual, diag, tol, itmax, divtol, s);
         if (result) break;
+      }
+    }
+    if (!result) {  // the iterates to the caller (stream order: after the replays)
+      const int64_t nx = hseg.back().off + hseg.back().n;
+      hipLaunchKernelGGL((kb_copy<TV>), dim3(BNB), dim3(BVB), 0, s, nx, xw, x);
+      if (hipGetLastError() != hipSuccess) {
+        set_error("batched BiCG result copy failed");
+        result = VGPU_ERR_HIP;
       }
     }
     if (hipStreamSynchronize(s) != hipSuccess && !result) {
@@ -519,19 +537,19 @@ struct BBicg {
  private:
   bool key_valid = false;
   uint64_t key_op = 0;
-  const void *key_b = nullptr, *key_x = nullptr, *key_diag = nullptr;
+  const void *key_diag = nullptr;
   double key_tol = 0;
   int key_itmax = 0;
   double key_div = 0;
 
-  int capture(uint64_t op_key, const BDirOp<TV> &dual, const TV *diag, const TV *b,
-              TV *x, double tol, int itmax, double divtol, hipStream_t s) {
+  int capture(uint64_t op_key, const BDirOp<TV> &dual, const TV *diag, double tol,
+              int itmax, double divtol, hipStream_t s) {
     drop_graph();
     VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int crc = VGPU_OK;
     for (int pi = 0; pi < pairs_per_graph && crc == VGPU_OK; ++pi)
       for (int q = 1; q >= 0 && crc == VGPU_OK; --q)
-        crc = enqueue_iteration(dual, diag, x, tol, itmax, divtol, q, s);
+        crc = enqueue_iteration(dual, diag, xw, tol, itmax, divtol, q, s);
     hipError_t ce = hipStreamEndCapture(s, &graph);
     if (crc) {
       drop_graph();
@@ -541,8 +559,6 @@ struct BBicg {
     VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
     key_valid = true;
     key_op = op_key;
-    key_b = b;
-    key_x = x;
     key_diag = diag;
     key_tol = tol;
     key_itmax = itmax;

@@ -61,21 +61,30 @@ __device__ __forceinline__ void stab_partial2(double u, double v, double *a, dou
   }
 }
 
-// r = b - A x0 (ax = A x0), R = r / d, RT = d r (shadow r0 = r); x zeroed on
-// the mask; partials of RT . R (= r0 . r0), |r|^2 and |b|^2.
+// r = b - A x0 (ax = A x0), R = r / d, RT = d r (shadow r0 = r); the
+// engine's iterate xw = x0 zeroed on the mask; partials of RT . R
+// (= r0 . r0), |r|^2 and |b|^2; the solve's initial status (written on the
+// device: no host copy on the stream).
 template <typename TV>
-__global__ __launch_bounds__(SVB) void ks_init(int64_t n, const uint8_t *__restrict__ mask,
-                                              const TV *__restrict__ b, TV *__restrict__ x,
+__global__ __launch_bounds__(SVB) void ks_init(SolveStatus *__restrict__ st, int64_t n,
+                                              const uint8_t *__restrict__ mask,
+                                              const TV *__restrict__ b,
+                                              const TV *__restrict__ x, TV *__restrict__ xw,
                                               const TV *__restrict__ ax,
                                               const TV *__restrict__ diag, TV *__restrict__ R,
                                               TV *__restrict__ RT, double *__restrict__ prt,
                                               double *__restrict__ prr,
                                               double *__restrict__ pbn) {
   __shared__ double red[SVB / 64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    SolveStatus s0{};
+    s0.rho = s0.alpha = s0.omega = 1.0;
+    *st = s0;
+  }
   double srt = 0, sr = 0, sb = 0;
   const int64_t stride = (int64_t)gridDim.x * SVB;
   for (int64_t i = blockIdx.x * (int64_t)SVB + threadIdx.x; i < n; i += stride) {
-    if (mask[i]) x[i] = TV(0);
+    xw[i] = mask[i] ? TV(0) : x[i];
     const TV bi = b[i], di = diag[i];
     const TV ri = bi - ax[i];
     const TV Ri = ri / di, RTi = ri * di;
@@ -97,6 +106,14 @@ __global__ __launch_bounds__(SVB) void ks_init(int64_t n, const uint8_t *__restr
   }
 }
 
+// y = x over n values
+template <typename TV>
+__global__ __launch_bounds__(SVB) void ks_copy(int64_t n, const TV *__restrict__ x,
+                                              TV *__restrict__ y) {
+  const int64_t stride = (int64_t)gridDim.x * SVB;
+  for (int64_t i = blockIdx.x * (int64_t)SVB + threadIdx.x; i < n; i += stride) y[i] = x[i];
+}
+
 // Operator callbacks: y = A x of the system (ignores the status), and one
 // BiCGSTAB owner launch (mode kStabDir or kStabS).
 template <typename TV>
@@ -110,6 +127,7 @@ struct Stab {
   int64_t n = 0;
   int NB = 0, na = 0;
   TV *ax = nullptr, *RT = nullptr, *T = nullptr;
+  TV *xw = nullptr;  // the iterate (copied to the caller's x at the end)
   TV *R[2] = {nullptr, nullptr}, *P[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr};
   double *prt = nullptr, *prr = nullptr, *pbn = nullptr;  // initial partials [NB]
   // direction launch partials (RT.V, |r|^2), S launch partials (t.s, t.t,
@@ -117,9 +135,14 @@ struct Stab {
   double *pga = nullptr, *prn = nullptr;
   double *pts = nullptr, *ptt = nullptr, *prs = nullptr, *prq = nullptr;
   SolveStatus *st = nullptr, *st_host = nullptr;
-  hipGraphExec_t gexec = nullptr;
-  hipGraph_t graph = nullptr;
-  int pairs_per_graph = 4;
+  // graphs of 1, 2 and 4 iteration pairs, captured on demand: a solve replays
+  // the size that the previous solve's iteration count suggests, so short
+  // (time-step) solves launch few no-op iterations past their end and long
+  // ones few graph launches
+  static constexpr int kSizes = 3;
+  hipGraphExec_t gexec[kSizes] = {nullptr, nullptr, nullptr};
+  hipGraph_t graph[kSizes] = {nullptr, nullptr, nullptr};
+  int last_iters = -1;
 
   int cap = 0;  // capacity of the owner partial arrays
 
@@ -136,7 +159,7 @@ struct Stab {
     n = nn;
     NB = stab_blocks(n);
     na = cap = nna;
-    TV **v[] = {&ax, &RT, &T, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
+    TV **v[] = {&ax, &RT, &T, &xw, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
     for (auto pv : v) {
       VGPU_HIP(hipMalloc(pv, std::max<int64_t>(1, n) * sizeof(TV)));
       VGPU_HIP(hipMemset(*pv, 0, std::max<int64_t>(1, n) * sizeof(TV)));
@@ -153,15 +176,18 @@ struct Stab {
     return VGPU_OK;
   }
   void drop_graph() {
-    if (gexec) hipGraphExecDestroy(gexec);
-    if (graph) hipGraphDestroy(graph);
-    gexec = nullptr;
-    graph = nullptr;
+    for (int i = 0; i < kSizes; ++i) {
+      if (gexec[i]) hipGraphExecDestroy(gexec[i]);
+      if (graph[i]) hipGraphDestroy(graph[i]);
+      gexec[i] = nullptr;
+      graph[i] = nullptr;
+    }
     key_valid = false;
   }
+  static int pairs_of(int size) { return 1 << size; }
   void release() {
     drop_graph();
-    TV **v[] = {&ax, &RT, &T, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
+    TV **v[] = {&ax, &RT, &T, &xw, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
     for (auto pv : v)
       if (*pv) {
         hipFree(*pv);
@@ -182,36 +208,37 @@ struct Stab {
   }
   ~Stab() { release(); }
 
-  // Solve A x = b (device vectors; x in/out, zeroed on the mask).  The graph
-  // is reused while op_key, the vectors, tol, itmax and divtol are unchanged.
+  // Solve A x = b (device vectors; x in/out, zeroed on the mask).  The
+  // iterations run on the engine's own buffers (x0 copied in by the init
+  // kernel, the result copied out at the end), so the graph is reused while
+  // op_key, diag, tol, itmax and divtol are unchanged, whatever b and x.
   int solve(uint64_t op_key, const StabSingleOp<TV> &single, const StabOwnerOp<TV> &owner,
             const uint8_t *mask, const TV *b, TV *x, const TV *diag, double tol, int itmax,
             double divtol, hipStream_t s, int *iters, double *err, int *status) {
     int rc = single(x, ax, s);  // A x0 (x0 masked on load)
     if (rc) return rc;
     // R_0 goes to the buffer the first captured iteration (parity 1) reads
-    hipLaunchKernelGGL((ks_init<TV>), dim3(NB), dim3(SVB), 0, s, n, mask, b, x, ax, diag,
-                       R[0], RT, prt, prr, pbn);
+    hipLaunchKernelGGL((ks_init<TV>), dim3(NB), dim3(SVB), 0, s, st, n, mask, b, x, xw, ax,
+                       diag, R[0], RT, prt, prr, pbn);
     VGPU_HIP(hipGetLastError());
-    SolveStatus s0{};
-    s0.rho = s0.alpha = s0.omega = 1.0;
-    VGPU_HIP(hipMemcpyAsync(st, &s0, sizeof(SolveStatus), hipMemcpyHostToDevice, s));
-    const bool same = key_valid && key_op == op_key && key_b == b && key_x == x &&
-                      key_diag == diag && key_tol == tol && key_itmax == itmax &&
-                      key_div == divtol;
-    if (!same) {
-      rc = capture(op_key, owner, diag, b, x, tol, itmax, divtol, s);
+    const bool same = key_valid && key_op == op_key && key_diag == diag && key_tol == tol &&
+                      key_itmax == itmax && key_div == divtol;
+    if (!same) drop_graph();
+    const int size = last_iters < 0 ? 2 : last_iters <= 6 ? 0 : last_iters <= 24 ? 1 : 2;
+    if (!gexec[size]) {
+      rc = capture(op_key, owner, diag, tol, itmax, divtol, size, s);
       if (rc) return rc;
     }
+    hipGraphExec_t gx = gexec[size];
     // replay until done, at most two replays in flight
-    const int max_replays = itmax / (2 * pairs_per_graph) + 2;
+    const int max_replays = itmax / (2 * pairs_of(size)) + 2;
     hipEvent_t ev[2];
     VGPU_HIP(hipEventCreateWithFlags(&ev[0], hipEventDisableTiming));
     VGPU_HIP(hipEventCreateWithFlags(&ev[1], hipEventDisableTiming));
     int result = VGPU_OK, launched = 0, completed = 0;
     for (;;) {
       while (launched - completed < 2 && launched < max_replays) {
-        if (hipGraphLaunch(gexec, s) != hipSuccess ||
+        if (hipGraphLaunch(gx, s) != hipSuccess ||
             hipMemcpyAsync(st_host, st, sizeof(SolveStatus), hipMemcpyDeviceToHost, s) !=
                 hipSuccess ||
             hipEventRecord(ev[launched & 1], s) != hipSuccess) {
@@ -230,6 +257,13 @@ struct Stab {
       ++completed;
       if (st_host->done) break;
     }
+    if (!result) {  // the iterate to the caller (stream order: after the replays)
+      hipLaunchKernelGGL((ks_copy<TV>), dim3(NB), dim3(SVB), 0, s, n, xw, x);
+      if (hipGetLastError() != hipSuccess) {
+        set_error("BiCGSTAB result copy failed");
+        result = VGPU_ERR_HIP;
+      }
+    }
     if (hipStreamSynchronize(s) != hipSuccess && !result) {
       set_error("BiCGSTAB stream sync failed");
       result = VGPU_ERR_HIP;
@@ -240,35 +274,33 @@ struct Stab {
     *iters = st_host->iter;
     *err = st_host->err;
     *status = st_host->done;
+    last_iters = st_host->iter;
     return VGPU_OK;
   }
 
  private:
   bool key_valid = false;
   uint64_t key_op = 0;
-  const void *key_b = nullptr, *key_x = nullptr, *key_diag = nullptr;
+  const void *key_diag = nullptr;
   double key_tol = 0, key_div = 0;
   int key_itmax = 0;
 
-  int capture(uint64_t op_key, const StabOwnerOp<TV> &owner, const TV *diag, const TV *b,
-              TV *x, double tol, int itmax, double divtol, hipStream_t s) {
-    drop_graph();
+  int capture(uint64_t op_key, const StabOwnerOp<TV> &owner, const TV *diag, double tol,
+              int itmax, double divtol, int size, hipStream_t s) {
     VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
     int crc = VGPU_OK;
-    for (int pi = 0; pi < pairs_per_graph && crc == VGPU_OK; ++pi)
+    for (int pi = 0; pi < pairs_of(size) && crc == VGPU_OK; ++pi)
       for (int q = 1; q >= 0 && crc == VGPU_OK; --q)
-        crc = enqueue_iteration(owner, diag, x, tol, itmax, divtol, q, s);
-    hipError_t ce = hipStreamEndCapture(s, &graph);
+        crc = enqueue_iteration(owner, diag, xw, tol, itmax, divtol, q, s);
+    hipError_t ce = hipStreamEndCapture(s, &graph[size]);
     if (crc) {
       drop_graph();
       return crc;
     }
     VGPU_HIP(ce);
-    VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+    VGPU_HIP(hipGraphInstantiate(&gexec[size], graph[size], nullptr, nullptr, 0));
     key_valid = true;
     key_op = op_key;
-    key_b = b;
-    key_x = x;
     key_diag = diag;
     key_tol = tol;
     key_itmax = itmax;

@@ -84,11 +84,13 @@ class Stepper:
         self.ctx.gaussian_dev(centre, sigma, self.q, self.src)
 
     def step(self, centre, sigma, x1, x2, out, mode="previous", net=None, rtol=1e-8,
-             itmax=20000, sweeps=2, omega=0.8):
+             itmax=20000, sweeps=2, omega=0.8, x3=None):
         """Solve step n into `out` from the guess of `mode`; x1 = x_{n-1},
-        x2 = x_{n-2} (device).  mode is guess[+jacobi][@method]: guess
-        "previous", "extrap" or "net", method "bicg" (default) or "bicgstab".
-        Returns (iterations, seconds guess + solve, fallback used)."""
+        x2 = x_{n-2}, x3 = x_{n-3} or None (device).  mode is
+        guess[+jacobi][@method]: guess "previous", "extrap" (2 x1 - x2),
+        "extrap2" (3 x1 - 3 x2 + x3; linear while x3 is None) or "net", method
+        "bicg" (default) or "bicgstab".  Returns (iterations, seconds guess +
+        solve, fallback used)."""
         ctx = self.ctx
         self.source(centre, sigma)
         ctx.assemble_rhs_dev(self.q, x1, self.b)
@@ -99,8 +101,11 @@ class Stepper:
         base, _, post = mode.partition("+")
         if base == "previous":
             ctx.copy_dev(out, x1, 8 * self.N)
-        elif base == "extrap":
+        elif base == "extrap" or (base == "extrap2" and x3 is None):
             ctx.axpby_dev(2.0, x1, -1.0, x2, out)
+        elif base == "extrap2":
+            ctx.axpby_dev(3.0, x1, -3.0, x2, out)
+            ctx.axpby_dev(1.0, x3, 1.0, out, out)
         elif base == "net":
             net.predict_dev(x1, x2, self.src, out)
             if ctx.residual_norm_dev(self.b, x1) < ctx.residual_norm_dev(self.b, out):
@@ -115,7 +120,7 @@ class Stepper:
         if method:
             ctx.set_method(method)
         try:
-            it, err, sec, rc = ctx.solve_dev(self.b, out, rtol=rtol, itmax=itmax)
+            it, err, sec, rc = ctx.solve_dev(self.b, out, rtol=rtol, itmax=itmax, timed=False)
         finally:
             if method:
                 ctx.set_method("bicg")
@@ -131,23 +136,25 @@ def run(stepper, seed, steps, mode="previous", net=None, states=None, rtol=1e-8)
     used).  Returns per-step iterations, seconds and fallbacks."""
     centres, sigma = helix(seed, steps)
     own = states is None
-    buf = stepper.alloc_states(3) if own else states
+    buf = stepper.alloc_states(4) if own else states
     its, secs, fbs = [], [], []
     for n in range(1, steps + 1):
         if own:
-            x1, x2, out = (stepper.ptr(buf, (n - 1) % 3), stepper.ptr(buf, (n - 2) % 3),
-                           stepper.ptr(buf, n % 3))
+            x1, x2, x3, out = (stepper.ptr(buf, (n - 1) % 4), stepper.ptr(buf, (n - 2) % 4),
+                               stepper.ptr(buf, (n - 3) % 4), stepper.ptr(buf, n % 4))
             if n == 1:
                 x2 = x1
         else:
-            x1, x2, out = (stepper.ptr(buf, n - 1), stepper.ptr(buf, max(n - 2, 0)),
-                           stepper.ptr(buf, n))
-        it, sec, fb = stepper.step(centres[n - 1], sigma, x1, x2, out, mode, net, rtol)
+            x1, x2, x3, out = (stepper.ptr(buf, n - 1), stepper.ptr(buf, max(n - 2, 0)),
+                               stepper.ptr(buf, max(n - 3, 0)), stepper.ptr(buf, n))
+        if n < 3:
+            x3 = None
+        it, sec, fb = stepper.step(centres[n - 1], sigma, x1, x2, out, mode, net, rtol, x3=x3)
         its.append(it)
         secs.append(sec)
         fbs.append(fb)
     return {"iterations": its, "seconds": secs, "fallbacks": int(sum(fbs)),
-            "last": stepper.ptr(buf, steps % 3 if own else steps)}
+            "last": stepper.ptr(buf, steps % 4 if own else steps)}
 
 
 def init_params(seed=0):
@@ -219,17 +226,23 @@ def train(net, stepper, seeds, trajectories, steps, epochs=3, lr=5e-4, shuffle_s
 def experiment(pkg, shape=(128, 128, 64), dt=0.01, steps=200, train_seeds=range(1, 11),
                eval_seed=0, epochs=3, lr=5e-4, extrap=1.0, log=None, params=None,
                loss_kind=1, smoothing=2,
-               modes=("previous", "extrap", "extrap+jacobi", "net", "net+jacobi")):
+               modes=("previous", "extrap", "extrap+jacobi", "net", "net+jacobi"), tune=False):
     """Generate training trajectories with the plain method, train the network
     (unless params are given) on the initial-residual loss (loss_kind 1) or the
     increment loss (0), then run the evaluation seed with the three guesses.
-    Returns a dict of results and the trained parameters."""
+    tune: run the autotuner once on the implicit-Euler system first (the
+    owner tile and z chunk every solve then uses).  Returns a dict of
+    results and the trained parameters."""
     p = pkg.problem.unit_box(*shape, vel_seed=0, lam=1e-3, scheme=_lib.SCHEME_EULER, dt=dt)
     ctx = pkg.Context.from_problem(p)
     st = Stepper(ctx, p, dt)
+    if tune:
+        res_tune = ctx.autotune(_lib.FP64, reps=5)
     net = _lib.Net(ctx)
     vx, vy, vz = nodal_velocity(p)
     res = {"shape": list(shape), "nodes": p.N, "dt": dt, "steps": steps}
+    if tune:
+        res["autotune"] = res_tune
     t0 = time.time()
     trajs = []
     if params is None:
