@@ -144,17 +144,17 @@ def config3(pkg):
 def learned_guess(pkg):
     """Implicit Euler (dt 0.01, 200 steps, helix source of seed 0) on the
     128x128x64 box, every step to rtol 1e-8, from the previous-step guess
-    with BiCG (the plain method), and with linear extrapolation + 2 Jacobi
-    sweeps and the trained network (committed parameters, trained on seeds
-    1-10 by learned.main) + 2 Jacobi sweeps, each solved by BiCG and by
-    BiCGSTAB; speedups against the plain method."""
+    with BiCG (the plain method), and with linear and quadratic
+    extrapolation + 2 Jacobi sweeps and the trained network (committed
+    parameters, trained on seeds 1-10 by learned.main) + 2 Jacobi sweeps,
+    each solved by BiCG and by BiCGSTAB; speedups against the plain method."""
     path = os.path.join(ROOT, "vanadis_solver-directional_residual_amd", "data",
                         "learned_guess_128x128x64.npz")
     if not os.path.exists(path):
         return {"status": "no trained parameters"}
     params = pkg.learned.load_params(path)
-    modes = ("previous", "extrap+jacobi", "net+jacobi", "previous@bicgstab",
-             "extrap+jacobi@bicgstab", "net+jacobi@bicgstab")
+    modes = ("previous", "extrap+jacobi", "extrap2+jacobi", "net+jacobi", "previous@bicgstab",
+             "extrap+jacobi@bicgstab", "extrap2+jacobi@bicgstab", "net+jacobi@bicgstab")
     progress(f"learned guess: {len(modes)} x 200 implicit-Euler steps")
     res, _ = pkg.learned.experiment(pkg, params=params, modes=modes, tune=True)
     out = {"workload": "HEX8 unit box 128x128x64, implicit Euler dt=0.01, 200 steps, "
@@ -172,10 +172,13 @@ def learned_guess(pkg):
     out["net_bicgstab_speedup_vs_previous"] = res["net+jacobi@bicgstab_speedup_vs_previous"]
     out["extrap_bicgstab_speedup_vs_previous"] = \
         res["extrap+jacobi@bicgstab_speedup_vs_previous"]
+    out["extrap2_bicgstab_speedup_vs_previous"] = \
+        res["extrap2+jacobi@bicgstab_speedup_vs_previous"]
     progress(f"learned: net {out['net_speedup_vs_previous']}x, extrap "
              f"{out['extrap_speedup_vs_previous']}x, net+bicgstab "
              f"{out['net_bicgstab_speedup_vs_previous']}x, extrap+bicgstab "
-             f"{out['extrap_bicgstab_speedup_vs_previous']}x vs previous-step guess")
+             f"{out['extrap_bicgstab_speedup_vs_previous']}x, quadratic extrap+bicgstab "
+             f"{out['extrap2_bicgstab_speedup_vs_previous']}x vs previous-step guess")
     return out
 
 
