@@ -166,3 +166,32 @@ def test_trajectory_store_round_trip(pkg, tmp_path):
     assert np.abs(ctx.d2h(st.ptr(buf, 5), p.N)).max() > 0
     ctx.close()
 
+
+
+def test_warm_starts_and_methods_reach_the_same_states(pkg):
+    """20 implicit-Euler steps on a small box from every guess (previous,
+    linear and quadratic extrapolation, with Jacobi sweeps) and both methods:
+    each step is solved to rtol 1e-8 of its own right-hand side, so the final
+    states agree to the solver tolerance; the extrapolated guesses need fewer
+    iterations than the previous-step guess, and quadratic extrapolation
+    fewer than linear with BiCGSTAB."""
+    p = pkg.problem.unit_box(24, 20, 12, vel_seed=0, lam=1e-3, scheme=pkg.SCHEME_EULER,
+                             dt=0.01)
+    ctx = pkg.Context.from_problem(p)
+    st = pkg.learned.Stepper(ctx, p, 0.01)
+    finals, its = {}, {}
+    modes = ("previous", "extrap+jacobi", "extrap2+jacobi", "previous@bicgstab",
+             "extrap@bicgstab", "extrap2@bicgstab", "extrap2+jacobi@bicgstab")
+    for mode in modes:
+        r = pkg.learned.run(st, 0, 20, mode)
+        finals[mode] = ctx.d2h(r["last"], p.N)
+        its[mode] = sum(r["iterations"])
+        log(f"{mode}: {its[mode]} iterations")
+    ref = finals["previous"]
+    for mode in modes:
+        assert np.linalg.norm(finals[mode] - ref) <= 1e-6 * np.linalg.norm(ref), mode
+    assert its["extrap+jacobi"] < its["previous"]
+    assert its["extrap2+jacobi"] < its["previous"]
+    assert its["extrap@bicgstab"] < its["previous@bicgstab"]
+    assert its["extrap2@bicgstab"] <= its["extrap@bicgstab"]
+    ctx.close()

@@ -155,3 +155,30 @@ def test_mixed_precision_bicgstab_inner(pkg, gpu):
         assert ctx.true_residual(b, xm) <= 1e-8
         assert rel(xm, x64) <= 1e-6
         assert itm <= 3 * its
+
+
+@pytest.mark.parametrize("method", ["bicg", "bicgstab"])
+def test_device_solves_on_rotating_buffers(pkg, gpu, method):
+    """Solves of one system into different device buffers (as a time stepper
+    rotates its states), timed and untimed, give bitwise the result of the
+    first solve (the captured graphs do not depend on the caller's x)."""
+    p = pkg.problem.unit_box(30, 22, 14, vel_seed=3, lam=2e-3)
+    log(f"rotating buffers {method}")
+    with pkg.Context.from_problem(p) as ctx:
+        ctx.set_options(scatter="owner", zchunk=5, tile=1)
+        ctx.set_method(method)
+        b = ctx.assemble_rhs(p.q_in)
+        bd = ctx.dev_alloc(8 * p.N)
+        ctx.h2d(bd, b)
+        xs = [ctx.dev_alloc(8 * p.N) for _ in range(3)]
+        res = []
+        for rep in range(2):
+            for xd in xs:
+                ctx.h2d(xd, np.zeros(p.N))
+                it, err, _, rc = ctx.solve_dev(bd, xd, rtol=1e-10, itmax=5000, timed=(rep == 0))
+                assert rc == 0 and err <= 1e-10
+                res.append((it, err, ctx.d2h(xd, p.N)))
+        ctx.set_method("bicg")
+        for it, err, x in res[1:]:
+            assert it == res[0][0] and err == res[0][1] and np.array_equal(x, res[0][2])
+        assert ctx.true_residual(b, res[0][2]) <= 1e-8
