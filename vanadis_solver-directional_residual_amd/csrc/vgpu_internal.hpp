@@ -228,6 +228,10 @@ struct StabData {
   SolveStatus *st = nullptr;
   double tol = 0, divtol = 0;
   int itmax = 0;
+  // 1: every workgroup reads the status first and returns at once when the
+  // solve has ended (cheap launches past the end of a short solve, one load
+  // latency more for the others; a kernel variant of its own)
+  int early = 0;
 };
 // One BiCGSTAB owner launch (mode kStabDir or kStabS) on R (x1) into y1.
 template <typename T, typename TIO>

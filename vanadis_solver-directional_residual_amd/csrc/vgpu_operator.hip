@@ -262,7 +262,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
     double *__restrict__ dot, DirData<TIO> dd, StabData<TIO> sd) {
   constexpr bool DIR = DM == 1;  // BiCG iteration launch
-  constexpr bool SA = DM == kStabDir, SB = DM == kStabS, STAB = SA || SB;
+  // BiCGSTAB launches; DM + 2: the same with the early status test
+  constexpr bool SA = DM == kStabDir || DM == kStabDir + 2;
+  constexpr bool SB = DM == kStabS || DM == kStabS + 2, STAB = SA || SB;
+  constexpr bool EARLY = DM >= 4;
   static_assert(DUAL || !DIR, "the direction update needs the dual operator");
   static_assert(!(DUAL && STAB), "the BiCGSTAB launches use the single operator");
   constexpr int NP = (TX + 1) * (TY + 1);
@@ -345,6 +348,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
   // otherwise one partial per (tile, plane) unit.
   auto march = [&](int m, int tile, int k0, int k1, int64_t gidx) {
     if (!DIR && !STAB && st && st[m].done) return;
+    if (EARLY && sd.st->done) return;
     const OpData<T> op = BATCH ? md[m].op : m0.op;
     const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
     const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
@@ -1346,6 +1350,11 @@ int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<
   L.x1 = r;
   L.y1 = y1;
   L.sd = sd;
+  if (sd.early) {
+    if (mode == kStabDir)
+      return launch_owner_any<T, TIO, false, kStabDir + 2>(L, op.tile, op.ra != 0.0, s);
+    return launch_owner_any<T, TIO, false, kStabS + 2>(L, op.tile, op.ra != 0.0, s);
+  }
   if (mode == kStabDir)
     return launch_owner_any<T, TIO, false, kStabDir>(L, op.tile, op.ra != 0.0, s);
   return launch_owner_any<T, TIO, false, kStabS>(L, op.tile, op.ra != 0.0, s);

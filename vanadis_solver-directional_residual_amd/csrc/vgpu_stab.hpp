@@ -291,7 +291,7 @@ struct Stab {
     int crc = VGPU_OK;
     for (int pi = 0; pi < pairs_of(size) && crc == VGPU_OK; ++pi)
       for (int q = 1; q >= 0 && crc == VGPU_OK; --q)
-        crc = enqueue_iteration(owner, diag, xw, tol, itmax, divtol, q, s);
+        crc = enqueue_iteration(owner, diag, xw, tol, itmax, divtol, q, size < 2, s);
     hipError_t ce = hipStreamEndCapture(s, &graph[size]);
     if (crc) {
       drop_graph();
@@ -308,11 +308,15 @@ struct Stab {
     return VGPU_OK;
   }
 
+  // early: the launches of short solves' graphs return at once after the end
   int enqueue_iteration(const StabOwnerOp<TV> &owner, const TV *diag, TV *x, double tol,
-                        int itmax, double divtol, int q, hipStream_t s) {
-    int rc = owner(kStabDir, dir_data(q, diag, x, tol, itmax, divtol), R[q ^ 1], V[q], s);
+                        int itmax, double divtol, int q, bool early, hipStream_t s) {
+    StabData<TV> a = dir_data(q, diag, x, tol, itmax, divtol);
+    StabData<TV> b = s_data(q, diag, tol, itmax, divtol);
+    a.early = b.early = early ? 1 : 0;
+    int rc = owner(kStabDir, a, R[q ^ 1], V[q], s);
     if (rc) return rc;
-    return owner(kStabS, s_data(q, diag, tol, itmax, divtol), R[q], T, s);
+    return owner(kStabS, b, R[q], T, s);
   }
 
  public:
