@@ -225,11 +225,16 @@ int vgpu_assemble_rhs_dev(vgpu_ctx *ctx, const double *q_dev, const double *t_ol
 int vgpu_gaussian_dev(vgpu_ctx *ctx, double cx, double cy, double cz, double sigma,
                       double *q_elem_dev, double *src_node_dev);
 /* Device vector helpers on the context's stream: copy; out = a x + b y (zero on
- * Dirichlet nodes; y may be NULL when b = 0); |b - A x| / |b| (FP64 owner
- * apply, synchronous). */
+ * Dirichlet nodes; y may be NULL when b = 0); out = a x + b y + c z (zero on
+ * Dirichlet nodes; e.g. the quadratic extrapolation 3 x1 - 3 x2 + x3 of a
+ * time stepper's initial guess); |b - A x| / |b| (FP64 owner apply,
+ * synchronous).  No counterpart in the reference, whose host keeps the
+ * previous step in host memory (VANADIS.FOR:809-810). */
 int vgpu_copy_dev(vgpu_ctx *ctx, void *dst, const void *src, int64_t bytes);
 int vgpu_axpby_dev(vgpu_ctx *ctx, double a, const double *x, double b, const double *y,
                    double *out);
+int vgpu_axpbypcz_dev(vgpu_ctx *ctx, double a, const double *x, double b, const double *y,
+                      double c, const double *z, double *out);
 int vgpu_residual_norm_dev(vgpu_ctx *ctx, const double *b, const double *x, double *rel);
 /* `sweeps` damped Jacobi sweeps x += omega D^-1 (b - A x) on the device (D the
  * solver's Jacobi diagonal): smooths the rough part of an initial guess. */

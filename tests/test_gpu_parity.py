@@ -221,3 +221,28 @@ def test_ddf_reproduces_transcript(pkg, gpu, golden_log):
     with pkg.Context.from_problem(p) as ctx:
         d, _ = ctx.ddf()
     assert d == pytest.approx(float(golden_log["steps"][0]["ddf_text"]), rel=1e-12)
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_jacobi_sweeps_match_oracle(case, po, sweeps):
+    """Damped Jacobi sweeps x <- x + omega (b - A x) / d (0 on the mask), the
+    update fused into the owner launch, against the same sweeps on the
+    oracle's operator and diagonal."""
+    name, p, bb, b, d, ctx = case
+    log(f"jacobi {name} sweeps {sweeps}")
+    b, d, _ = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    if np.linalg.norm(b) == 0.0:
+        b = np.random.default_rng(17).standard_normal(p.N) * (1 - p.mask)
+    x0 = np.random.default_rng(18).standard_normal(p.N) * (1 - p.mask)
+    omega = 0.8
+    xr = x0.copy()
+    for _ in range(sweeps):
+        ax = po.matvec_0(xr, p.iss, bb, p.nax, p.nay, p.naz, p.mask)
+        xr = np.where(p.mask == 1, 0.0, xr + omega * (b - ax) / d)
+    bd = ctx.dev_alloc(8 * p.N)
+    xd = ctx.dev_alloc(8 * p.N)
+    ctx.h2d(bd, b)
+    ctx.h2d(xd, x0)
+    ctx.jacobi_dev(bd, xd, sweeps, omega)
+    x = ctx.d2h(xd, p.N)
+    assert rel(x, xr) <= 1e-13

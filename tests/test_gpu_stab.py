@@ -182,3 +182,24 @@ def test_device_solves_on_rotating_buffers(pkg, gpu, method):
         for it, err, x in res[1:]:
             assert it == res[0][0] and err == res[0][1] and np.array_equal(x, res[0][2])
         assert ctx.true_residual(b, res[0][2]) <= 1e-8
+
+
+def test_three_term_combination(pkg, gpu):
+    """vgpu_axpbypcz_dev: out = a x + b y + c z on free nodes, 0 on the mask,
+    also in place (out = z); to rounding (the device may fuse the products
+    into multiply-adds)."""
+    p = pkg.problem.unit_box(9, 8, 7)
+    rng = np.random.default_rng(5)
+    with pkg.Context.from_problem(p) as ctx:
+        vs = [rng.standard_normal(p.N) for _ in range(3)]
+        ds = [ctx.dev_alloc(8 * p.N) for _ in range(4)]
+        for d_, v in zip(ds, vs):
+            ctx.h2d(d_, v)
+        free = p.mask == 0
+        for (a, b, c), out in (((3.0, -3.0, 1.0), 3), ((0.5, 0.25, 2.0), 2)):
+            ctx.axpbypcz_dev(a, ds[0], b, ds[1], c, ds[2], ds[out])
+            ref = a * vs[0] + b * vs[1] + c * vs[2]
+            scale = np.abs(a * vs[0]) + np.abs(b * vs[1]) + np.abs(c * vs[2])
+            got = ctx.d2h(ds[out], p.N)
+            assert np.all(got[~free] == 0.0)
+            assert np.all(np.abs(got[free] - ref[free]) <= 4e-16 * scale[free])

@@ -38,7 +38,7 @@ EXPORTED = (
     "vgpu_net_predict_dev", "vgpu_net_train_step_dev", "vgpu_net_reset_optimizer",
     "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
     "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
-    "vgpu_set_method",
+    "vgpu_set_method", "vgpu_axpbypcz_dev",
 )
 
 
@@ -120,6 +120,8 @@ def load():
     L.vgpu_net_set_static.argtypes = [_vp, _dp, _dp, _dp, _fp, C.c_double, C.c_double]
     L.vgpu_copy_dev.argtypes = [_vp, _vp, _vp, C.c_int64]
     L.vgpu_axpby_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp, _vp]
+    L.vgpu_axpbypcz_dev.argtypes = [_vp, C.c_double, _vp, C.c_double, _vp, C.c_double, _vp,
+                                    _vp]
     L.vgpu_residual_norm_dev.argtypes = [_vp, _vp, _vp, _pd]
     L.vgpu_jacobi_dev.argtypes = [_vp, _vp, _vp, C.c_int, C.c_double]
     L.vgpu_ddf.argtypes = [_vp, _pd, _pi]
@@ -322,6 +324,10 @@ class Context:
 
     def axpby_dev(self, a, x, b, y, out):
         check(load().vgpu_axpby_dev(self.h, float(a), x, float(b), y, out), "axpby_dev")
+
+    def axpbypcz_dev(self, a, x, b, y, c, z, out):
+        check(load().vgpu_axpbypcz_dev(self.h, float(a), x, float(b), y, float(c), z, out),
+              "axpbypcz_dev")
 
     def residual_norm_dev(self, b, x):
         r = C.c_double(0)

@@ -34,6 +34,14 @@ __global__ void k_lincomb(int64_t n, const uint8_t *mask, double a,
     out[i] = mask[i] ? 0.0 : a * u[i] + (v ? c * v[i] : 0.0);
 }
 
+// out = a u + b v + c w on free nodes, 0 on Dirichlet nodes
+__global__ void k_lincomb3(int64_t n, const uint8_t *mask, double a, const double *u, double b,
+                           const double *v, double c, const double *w, double *out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = mask[i] ? 0.0 : a * u[i] + b * v[i] + c * w[i];
+}
+
 // r = b - r; partials of r.r
 __global__ __launch_bounds__(VB) void k_resid(int64_t n, const double *b,
                                               double *r, double *part) {
@@ -946,6 +954,17 @@ int vgpu_axpby_dev(vgpu_ctx *c, double a, const double *x, double b, const doubl
   return VGPU_OK;
 }
 
+int vgpu_axpbypcz_dev(vgpu_ctx *c, double a, const double *x, double b, const double *y,
+                      double cz, const double *z, double *out) {
+  if (!c || !x || !y || !z || !out) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(c->device));
+  const int64_t N = c->g.N;
+  hipLaunchKernelGGL(k_lincomb3, grid_for(N), dim3(256), 0, c->stream, N, c->mask, a, x, b, y,
+                     cz, z, out);
+  VGPU_HIP(hipGetLastError());
+  return VGPU_OK;
+}
+
 int vgpu_residual_norm_dev(vgpu_ctx *c, const double *b, const double *x, double *rel) {
   if (!c || !b || !x || !rel) return VGPU_ERR_ARG;
   if (!c->have_coeffs) return VGPU_ERR_STATE;
@@ -976,9 +995,25 @@ int vgpu_jacobi_dev(vgpu_ctx *c, const double *b, double *x, int sweeps, double 
   if (rc) return rc;
   const int64_t N = c->g.N;
   OpData<double> op = c->op64(c->cH, c->cC);
+  if (owner_fits(c->g)) {
+    // the update fused into the owner launch, alternating between x and vt
+    // (other tiles read a sweep's input at their halo while it writes)
+    for (int k = 0; k < sweeps; ++k) {
+      const double *in = (k & 1) ? c->vt : x;
+      double *out = (k & 1) ? x : c->vt;
+      rc = launch_owner_jacobi<double, double>(op, c->zchunk, omega, in, b, c->diag, out,
+                                               c->stream);
+      if (rc) return rc;
+    }
+    if (sweeps & 1) {
+      hipLaunchKernelGGL(k_copy, grid_for(N), dim3(256), 0, c->stream, N, c->vt, x);
+      VGPU_HIP(hipGetLastError());
+    }
+    return VGPU_OK;
+  }
   for (int k = 0; k < sweeps; ++k) {
-    rc = launch_apply<double, double>(op, VGPU_SCATTER_OWNER, c->zchunk, x, c->vt, nullptr,
-                                      nullptr, nullptr, c->stream);
+    rc = launch_apply<double, double>(op, c->scatter, c->zchunk, x, c->vt, nullptr, nullptr,
+                                      nullptr, c->stream);
     if (rc) return rc;
     hipLaunchKernelGGL(k_jacobi, grid_for(N), dim3(256), 0, c->stream, N, c->mask, b, c->vt,
                        c->diag, omega, x);

@@ -207,6 +207,8 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
 //    s = d S).
 // Two launches per iteration, no vector pass in between.
 enum StabMode { kStabDir = 2, kStabS = 3 };
+// (4, 5: the same with the early status test; 6: a damped Jacobi sweep)
+constexpr int kJacobi = 6;
 template <typename TIO>
 struct StabData {
   const TIO *po = nullptr, *vo = nullptr;  // kStabDir: P, V of iteration j-1
@@ -232,7 +234,14 @@ struct StabData {
   // solve has ended (cheap launches past the end of a short solve, one load
   // latency more for the others; a kernel variant of its own)
   int early = 0;
+  double jomega = 0;  // Jacobi sweep: damping (d, rt hold the diagonal and b)
 };
+// One damped Jacobi sweep on the owner kernel: xn = x + omega (b - A x) / d
+// (0 on the mask), the update fused into the operator's output stage; xn must
+// not alias x (other tiles read x at their halo).
+template <typename T, typename TIO>
+int launch_owner_jacobi(const OpData<T> &op, int zchunk, double omega, const TIO *x,
+                        const TIO *b, const TIO *d, TIO *xn, hipStream_t s);
 // One BiCGSTAB owner launch (mode kStabDir or kStabS) on R (x1) into y1.
 template <typename T, typename TIO>
 int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<TIO> &sd,

@@ -265,9 +265,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
   // BiCGSTAB launches; DM + 2: the same with the early status test
   constexpr bool SA = DM == kStabDir || DM == kStabDir + 2;
   constexpr bool SB = DM == kStabS || DM == kStabS + 2, STAB = SA || SB;
-  constexpr bool EARLY = DM >= 4;
+  constexpr bool EARLY = DM == kStabDir + 2 || DM == kStabS + 2;
+  // damped Jacobi sweep: output x + omega (b - A x) / d at the owned nodes
+  constexpr bool JAC = DM == kJacobi;
   static_assert(DUAL || !DIR, "the direction update needs the dual operator");
-  static_assert(!(DUAL && STAB), "the BiCGSTAB launches use the single operator");
+  static_assert(!(DUAL && (STAB || JAC)), "the BiCGSTAB and Jacobi launches are single");
   constexpr int NP = (TX + 1) * (TY + 1);
   constexpr int NT = TX * TY;
   constexpr int NW = NT / 64;
@@ -404,8 +406,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     const Res rpn = make_res(DIR ? dd.pn + off : SA ? sd.pn + off : y1 + off, op.g.N * SI);
     const Res rrn = make_res(SA ? sd.rn + off : y1 + off, op.g.N * SI);
     const Res rto = make_res(SA ? sd.to + off : x1 + off, op.g.N * SI);
-    const Res rdg = make_res(STAB ? sd.d + off : x1 + off, op.g.N * SI);
-    const Res rrt = make_res(STAB ? sd.rt + off : x1 + off, op.g.N * SI);
+    // owned-node values: d and RT (BiCGSTAB), d and b (Jacobi sweep)
+    const Res rdg = make_res((STAB || JAC) ? sd.d + off : x1 + off, op.g.N * SI);
+    const Res rrt = make_res((STAB || JAC) ? sd.rt + off : x1 + off, op.g.N * SI);
     const Res rppn = make_res(DIR ? dd.ppn + off : y1 + off, op.g.N * SI);
     auto owned_col = [&](int lx, int ly) {
       return lx >= 1 && lx <= TX - 1 && ly >= 1 && ly <= TY - 1;
@@ -445,8 +448,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       r.c0 = (DIR || STAB) ? bload<TIO>(rc, vp0, sv) : TIO(0);
       r.d0 = (DIR || SA) ? bload<TIO>(rd, vp0, sv) : TIO(0);
       r.t0 = SA ? bload<TIO>(rt_, vp0, sv) : TIO(0);
-      r.f = STAB ? bload<TIO>(in ? rdg : rx1z, vo, sv) : TIO(0);
-      r.g = STAB ? bload<TIO>(in ? rrt : rx1z, vo, sv) : TIO(0);
+      r.f = (STAB || JAC) ? bload<TIO>(in ? rdg : rx1z, vo, sv) : TIO(0);
+      r.g = (STAB || JAC) ? bload<TIO>(in ? rrt : rx1z, vo, sv) : TIO(0);
       // x of owned nodes of the march's own planes only
       const Res re = gz < k1 ? rxs : rx1z;
       r.e0 = (DIR || SA) ? bload<TIO>(re, vn0, sv) : TIO(0);
@@ -777,7 +780,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       // BiCGSTAB: V = (A P)/d, T = (A S)/d (d is 0 only where the node is
       // not stored)
       const double dg = (double)fok[P];
-      const TIO o1 = omk[P] ? TIO(0) : STAB ? TIO(dg != 0.0 ? v1 / dg : 0.0) : TIO(v1);
+      TIO o1 = omk[P] ? TIO(0) : STAB ? TIO(dg != 0.0 ? v1 / dg : 0.0) : TIO(v1);
+      if (JAC)  // the input x of the node is its masked LDS value
+        o1 = omk[P] ? TIO(0)
+                    : TIO(xok[P] + sd.jomega * (((double)gok[P] - v1) / (dg != 0.0 ? dg : 1.0)));
       bstore(o1, ry1, vo, so);
       double dv[ND];
 #pragma unroll
@@ -830,7 +836,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       }
       const uint8_t omn = pm[P][oq];
       const double xon = (DUAL && own) ? (double)pl2[DUAL ? P : 0][oq]
-                         : (SB && own) ? (double)pl1[P][oq]
+                         : ((SB || JAC) && own) ? (double)pl1[P][oq]
                          : (SA && own) ? (double)plR[SA ? P : 0][oq] : 0.0;
       // owner sums and output of plane kl-1
       if (kl - 1 >= k0) emit_plane(std::integral_constant<int, P ^ 1>(), Pc);
@@ -1359,6 +1365,31 @@ int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<
     return launch_owner_any<T, TIO, false, kStabDir>(L, op.tile, op.ra != 0.0, s);
   return launch_owner_any<T, TIO, false, kStabS>(L, op.tile, op.ra != 0.0, s);
 }
+
+template <typename T, typename TIO>
+int launch_owner_jacobi(const OpData<T> &op, int zchunk, double omega, const TIO *x, const TIO *b,
+                        const TIO *d, TIO *xn, hipStream_t s) {
+  if (!owner_fits(op.g)) {
+    set_error("owner scatter: system too large for 32-bit offsets");
+    return VGPU_ERR_ARG;
+  }
+  OwnerLaunch<T, TIO> L{};
+  L.m0.op = op;
+  L.m0.seg.n = op.g.N;
+  L.m0.seg.nu = owner_units(op.g, op.tile);
+  L.utot = L.m0.seg.nu;
+  L.gtot = owner_groups(op.g, op.tile, zchunk);
+  L.zchunk = zchunk;
+  L.x1 = x;
+  L.y1 = xn;
+  L.sd.d = d;
+  L.sd.rt = b;
+  L.sd.jomega = omega;
+  return launch_owner_any<T, TIO, false, kJacobi>(L, op.tile, op.ra != 0.0, s);
+}
+template int launch_owner_jacobi<double, double>(const OpData<double> &, int, double,
+                                                 const double *, const double *,
+                                                 const double *, double *, hipStream_t);
 
 template int launch_owner_stab<double, double>(const OpData<double> &, int, int,
                                                const StabData<double> &, const double *,

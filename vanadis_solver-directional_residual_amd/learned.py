@@ -87,7 +87,8 @@ class Stepper:
              itmax=20000, sweeps=2, omega=0.8, x3=None):
         """Solve step n into `out` from the guess of `mode`; x1 = x_{n-1},
         x2 = x_{n-2}, x3 = x_{n-3} or None (device).  mode is
-        guess[+jacobi][@method]: guess "previous", "extrap" (2 x1 - x2),
+        guess[+jacobi[N]][@method] (N damped Jacobi sweeps, default `sweeps`):
+        guess "previous", "extrap" (2 x1 - x2),
         "extrap2" (3 x1 - 3 x2 + x3; linear while x3 is None) or "net", method
         "bicg" (default) or "bicgstab".  Returns (iterations, seconds guess +
         solve, fallback used)."""
@@ -104,8 +105,7 @@ class Stepper:
         elif base == "extrap" or (base == "extrap2" and x3 is None):
             ctx.axpby_dev(2.0, x1, -1.0, x2, out)
         elif base == "extrap2":
-            ctx.axpby_dev(3.0, x1, -3.0, x2, out)
-            ctx.axpby_dev(1.0, x3, 1.0, out, out)
+            ctx.axpbypcz_dev(3.0, x1, -3.0, x2, 1.0, x3, out)
         elif base == "net":
             net.predict_dev(x1, x2, self.src, out)
             if ctx.residual_norm_dev(self.b, x1) < ctx.residual_norm_dev(self.b, out):
@@ -113,8 +113,9 @@ class Stepper:
                 fallback = True
         else:
             raise ValueError(mode)
-        if post == "jacobi":
-            ctx.jacobi_dev(self.b, out, sweeps, omega)
+        if post.startswith("jacobi"):
+            n_sw = int(post[6:]) if len(post) > 6 else sweeps
+            ctx.jacobi_dev(self.b, out, n_sw, omega)
         elif post:
             raise ValueError(mode)
         if method:
