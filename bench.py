@@ -154,12 +154,14 @@ def learned_guess(pkg):
         return {"status": "no trained parameters"}
     params = pkg.learned.load_params(path)
     modes = ("previous", "extrap+jacobi", "extrap2+jacobi", "net+jacobi", "previous@bicgstab",
-             "extrap+jacobi@bicgstab", "extrap2+jacobi@bicgstab", "net+jacobi@bicgstab")
+             "extrap+jacobi@bicgstab", "extrap2+jacobi@bicgstab", "extrap2+jacobi4@bicgstab",
+             "net+jacobi@bicgstab")
     progress(f"learned guess: {len(modes)} x 200 implicit-Euler steps")
     res, _ = pkg.learned.experiment(pkg, params=params, modes=modes, tune=True)
     out = {"workload": "HEX8 unit box 128x128x64, implicit Euler dt=0.01, 200 steps, "
-                       "helix source seed 0, rtol 1e-8 each step; mode guess[+jacobi]"
-                       "[@method], default method BiCG; owner tile and z chunk autotuned "
+                       "helix source seed 0, rtol 1e-8 each step; mode guess[+jacobi[N]]"
+                       "[@method] (N Jacobi sweeps, default 2), default method BiCG; owner tile "
+                       "and z chunk autotuned "
                        "once on the implicit-Euler system",
            "autotune": res.get("autotune")}
     for m in modes:
@@ -174,11 +176,14 @@ def learned_guess(pkg):
         res["extrap+jacobi@bicgstab_speedup_vs_previous"]
     out["extrap2_bicgstab_speedup_vs_previous"] = \
         res["extrap2+jacobi@bicgstab_speedup_vs_previous"]
+    out["extrap2_jacobi4_bicgstab_speedup_vs_previous"] = \
+        res["extrap2+jacobi4@bicgstab_speedup_vs_previous"]
     progress(f"learned: net {out['net_speedup_vs_previous']}x, extrap "
              f"{out['extrap_speedup_vs_previous']}x, net+bicgstab "
              f"{out['net_bicgstab_speedup_vs_previous']}x, extrap+bicgstab "
              f"{out['extrap_bicgstab_speedup_vs_previous']}x, quadratic extrap+bicgstab "
-             f"{out['extrap2_bicgstab_speedup_vs_previous']}x vs previous-step guess")
+             f"{out['extrap2_bicgstab_speedup_vs_previous']}x (4 sweeps "
+             f"{out['extrap2_jacobi4_bicgstab_speedup_vs_previous']}x) vs previous-step guess")
     return out
 
 
