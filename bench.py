@@ -241,6 +241,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-stab", action="store_true", help="skip the FP64 BiCGSTAB solves")
+    ap.add_argument("--no-scatter-solves", action="store_true",
+                    help="skip the BiCG solves with the colored and atomic scatters")
     ap.add_argument("--no-tune", action="store_true",
                     help="keep the default owner tile (32x8) instead of the autotuner's")
     ap.add_argument("--tile", type=int, default=-1,
@@ -336,6 +338,33 @@ def main():
     else:
         mixed_status = "skipped"
 
+    # the two scatter flavours of the reference's solver (8 colour launches;
+    # FP64 atomics), FP64 BiCG to the same rtol (reported, never fatal)
+    own_zchunk = tuned["zchunk"] if tuned else max(a.zchunk, 0)
+    own_tile = tuned["tile"] if tuned else max(a.tile, 0)
+    scatter_solves = {}
+    if not a.no_scatter_solves:
+        for sc in ("colored", "atomic"):
+            ctx.set_options(sc, 0, own_tile)
+            try:
+                solve(pkg.FP64)
+                csec = []
+                for _ in range(2):
+                    it_c, _, sec = solve(pkg.FP64)
+                    csec.append(sec)
+                xc = ctx.d2h(x_dev, N)
+                scatter_solves[sc] = {
+                    "iterations": it_c, "ms": round(1e3 * float(np.mean(csec)), 3),
+                    "speedup_vs_owner": round(ms_step / (1e3 * float(np.mean(csec))), 3),
+                    "rel_l2_vs_owner": float(np.linalg.norm(xc - x64) / np.linalg.norm(x64)),
+                    "true_rel_residual": ctx.true_residual(b, xc)}
+                progress(f"{sc} scatter: {it_c} iterations, {1e3 * np.mean(csec):.2f} ms")
+            except RuntimeError as exc:
+                scatter_solves[sc] = {"status": f"failed: {exc}"}
+                progress(f"{sc} scatter solve failed: {exc}")
+            finally:
+                ctx.set_options("owner", own_zchunk, own_tile)
+
     # FP64 BiCGSTAB (same preconditioner, stopping rule and rtol; reported,
     # never fatal for the line)
     ssec, sits, rel_l2_stab, true_res_stab, stab_status = [], 0, None, None, "ok"
@@ -370,9 +399,14 @@ def main():
     t_ss = ctx.time_kernel(6, "owner", pkg.FP64, reps)
     modes = {}
     for sc in ("owner", "colored", "atomic"):
+        # brick scatters: the library's brick depth (z chunk 0), not the
+        # owner launches' tuned chunk
+        if sc != "owner":
+            ctx.set_options(sc, 0, own_tile)
         for prec, pn in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32")):
             ms = ctx.time_kernel(0, sc, prec, 20)
             modes[f"{sc}_{pn}"] = {"ms": round(ms, 5), "GDOF/s": round(N / ms / 1e6, 2)}
+        ctx.set_options("owner", own_zchunk, own_tile)
     gdofs = N / t_single / 1e6
     dual_bytes = algorithmic_bytes(N, E, dual=True)
     dual_gbs = dual_bytes / (t_dual * 1e-3) / 1e9
@@ -456,6 +490,7 @@ def main():
                      "iteration_launch_fp64": round(t_iter, 5)},
         "ms_per_iteration_fp64": round(ms_step / iters[-1], 5),
         "scatter_modes": modes,
+        "scatter_solves_fp64": scatter_solves,
         "copy_GBps": round(copy_gbs, 1),
         "copy_8N_GBps": round(copy8_gbs, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,

@@ -114,8 +114,10 @@ int vgpu_set_coeffs(vgpu_ctx *ctx, const double *lam, const double *vel,
 int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
 /* Options: scatter mode for the solver, z-chunk of the z-marching tiles
  * (-1 = keep; 0 = owner launches size their grid to the device's resident
- * workgroups and split the node planes evenly over them, brick launches use
- * 8; n > 0 = owner launches give each (chunk of n node planes, tile) group
+ * workgroups and split the node planes evenly over them, brick launches take
+ * the deepest brick of at most 8 element layers that still gives every
+ * compute unit two workgroups per launch (per colour launch for COLORED);
+ * n > 0 = owner launches give each (chunk of n node planes, tile) group
  * one workgroup, groups numbered chunk by chunk so that neighbouring tiles
  * march the same planes together; brick launches use n planes per brick)
  * and owner tile shape (0 = 32x8, 1 = 16x16, 2 = 64x4 element columns;

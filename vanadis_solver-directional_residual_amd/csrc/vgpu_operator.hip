@@ -1186,6 +1186,40 @@ int resident_blocks(const void *fn, int threads) {
   return r;
 }
 
+// Compute units of the current device (cached per device).
+int device_cus() {
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(dev);
+  if (it != cache.end()) return it->second;
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      cus <= 0)
+    cus = 256;
+  cache[dev] = cus;
+  return cus;
+}
+
+// Element layers per brick of the COLORED / ATOMIC launches when the caller
+// leaves the choice to the library: the deepest brick (at most 8 layers)
+// whose launch (each colour's launch for COLORED) still gives every compute
+// unit two workgroups, else 1.  A colour launch holds an eighth of the
+// bricks, so at 128x128x64 COLORED takes 1 layer (512 bricks per colour) and
+// ATOMIC 8 (512 bricks).
+int brick_depth(const Grid3 &g, bool colored, int bx, int by) {
+  const int64_t ntx = (g.ex + bx - 1) / bx, nty = (g.ey + by - 1) / by;
+  const int64_t cols = colored ? ((ntx + 1) / 2) * ((nty + 1) / 2) : ntx * nty;
+  const int64_t want = 2 * (int64_t)device_cus();
+  for (int zc = 8; zc > 1; --zc) {
+    const int64_t ntz = (g.ez + zc - 1) / zc;
+    if (cols * (colored ? (ntz + 1) / 2 : ntz) >= want) return zc;
+  }
+  return 1;
+}
+
 // Grid of an owner launch: zchunk > 0 one workgroup per (z chunk, tile)
 // group (gtot of them), zchunk = 0 one workgroup per resident slot over utot
 // units.
@@ -1432,7 +1466,7 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
   }
   // brick modes need zeroed outputs; bricks are 32 x 8 element columns
   constexpr int BX = 32, BY = 8;
-  if (zc <= 0) zc = 8;
+  if (zc <= 0) zc = brick_depth(g, scatter == VGPU_SCATTER_COLORED, BX, BY);
   hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
                      dual ? y2 : nullptr, g.N);
   const int ntx = (g.ex + BX - 1) / BX, nty = (g.ey + BY - 1) / BY;
