@@ -32,6 +32,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BiCG time-to-rtol 1e-8 and operator GDOF/s; mixed/learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # dense BF16 MFMA of the MI355X
 
 
 def progress(msg):
@@ -138,6 +139,42 @@ def config3(pkg):
     out["dual_apply_ms"] = {"fp64": round(ctx.time_kernel(1, "owner", pkg.FP64, 20), 5),
                             "fp32": round(ctx.time_kernel(1, "owner", pkg.FP32, 20), 5)}
     ctx.close()
+    return out
+
+
+def conv_layers(pkg, ctx, N):
+    """The learned guess's layer kernels on this node grid (BF16 MFMA, FP32
+    accumulation, 16 channels, 3x3x3 taps): GPU time per launch and the two
+    rooflines.  A layer does 2 x 27 x 16 x 16 flops per node; it moves 32 bytes
+    per node per BF16 tensor read or written (hidden forward: X in, Y out;
+    backward data: dZ and Yprev in, dZprev out; weight gradients: X and dZ in)."""
+    flops = 2.0 * 27 * 16 * 16 * N
+    out = {"flops_per_launch": flops, "peak_mfma_bf16_TFLOPs": MFMA_BF16_PEAK_TFLOPS,
+           "peak_hbm_GBps": HBM_PEAK_GBS}
+    net = pkg.Net(ctx)
+    try:
+        for what, name, tensors in ((0, "hidden_forward", 2), (1, "backward_data", 3),
+                                    (2, "weight_gradient", 2)):
+            ms = net.time_layers(what, reps=20)
+            nbytes = 32.0 * tensors * N
+            tf = flops / (ms * 1e-3) / 1e12
+            gbs = nbytes / (ms * 1e-3) / 1e9
+            t_mfma = flops / (MFMA_BF16_PEAK_TFLOPS * 1e12)
+            t_hbm = nbytes / (HBM_PEAK_GBS * 1e9)
+            out[name] = {"ms": round(ms, 5), "TFLOPs": round(tf, 1),
+                         "frac_mfma": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
+                         "algorithmic_bytes": nbytes, "GBps": round(gbs, 1),
+                         "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+                         "bound": "hbm" if t_hbm > t_mfma else "mfma",
+                         "frac_of_bound": round(max(t_mfma, t_hbm) / (ms * 1e-3), 4)}
+        out["prediction_ms"] = round(net.time_layers(3, reps=10), 5)
+    finally:
+        net.close()
+    progress(f"conv layers: forward {out['hidden_forward']['ms'] * 1e3:.1f} us "
+             f"({out['hidden_forward']['TFLOPs']} TFLOP/s), backward data "
+             f"{out['backward_data']['ms'] * 1e3:.1f} us, weight gradient "
+             f"{out['weight_gradient']['ms'] * 1e3:.1f} us, prediction "
+             f"{out['prediction_ms']:.3f} ms")
     return out
 
 
@@ -397,6 +434,11 @@ def main():
     t_copy8 = ctx.time_kernel(4, "owner", pkg.FP64, 20)
     t_sdir = ctx.time_kernel(5, "owner", pkg.FP64, reps)
     t_ss = ctx.time_kernel(6, "owner", pkg.FP64, reps)
+    try:
+        conv = conv_layers(pkg, ctx, N)
+    except Exception as exc:  # reported, never fatal for the line
+        conv = {"status": f"failed: {exc}"}
+        progress(f"conv layers {conv['status']}")
     modes = {}
     for sc in ("owner", "colored", "atomic"):
         # brick scatters: the library's brick depth (z chunk 0), not the
@@ -498,6 +540,7 @@ def main():
                      "traffic": None, "kernel": "k_owner<double,double,dual,direction>",
                      "algorithmic_bytes_per_launch": iter_bytes},
         "roofline_bicgstab": stab_roof,
+        "conv_layers_bf16": conv,
         "roofline_dual_apply": {"bound": "hbm", "achieved": round(dual_gbs, 1),
                                 "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                 "frac": round(dual_gbs / HBM_PEAK_GBS, 4),

@@ -312,6 +312,12 @@ int vgpu_net_set_loss(vgpu_net *net, int kind);
  * that BF16 rounding leaves in the correction, the slowest modes of a
  * mass-dominated operator for BiCG. */
 int vgpu_net_set_smoothing(vgpu_net *net, int passes);
+/* Average GPU time (HIP events around a captured graph of reps launches on
+ * the context's stream) of: 0 a hidden forward layer (16 -> 16 channels,
+ * leaky ReLU), 1 a backward-data layer, 2 a layer's weight and bias
+ * gradients, 3 the whole prediction network (input packing + 5 layers).
+ * Timing only: runs on the network's own buffers. */
+int vgpu_net_time_layers(vgpu_net *net, int what, int reps, double *avg_ms);
 
 #ifdef __cplusplus
 }

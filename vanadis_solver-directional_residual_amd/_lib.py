@@ -38,7 +38,7 @@ EXPORTED = (
     "vgpu_net_predict_dev", "vgpu_net_train_step_dev", "vgpu_net_reset_optimizer",
     "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
     "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
-    "vgpu_set_method", "vgpu_axpbypcz_dev",
+    "vgpu_set_method", "vgpu_axpbypcz_dev", "vgpu_net_time_layers",
 )
 
 
@@ -131,6 +131,7 @@ def load():
     L.vgpu_net_reset_optimizer.argtypes = [_vp]
     L.vgpu_net_set_loss.argtypes = [_vp, C.c_int]
     L.vgpu_net_set_smoothing.argtypes = [_vp, C.c_int]
+    L.vgpu_net_time_layers.argtypes = [_vp, C.c_int, C.c_int, _pd]
     _lib = L
     return L
 
@@ -490,6 +491,14 @@ class Net:
     def set_loss(self, kind):
         """0: increment L2 loss; 1: initial-residual loss through the operator."""
         check(load().vgpu_net_set_loss(self.h, int(kind)), "net_set_loss")
+
+    def time_layers(self, what=0, reps=20):
+        """Average ms of 0 a hidden forward layer, 1 a backward-data layer,
+        2 a layer's weight gradients, 3 the whole prediction network."""
+        ms = C.c_double(0)
+        check(load().vgpu_net_time_layers(self.h, int(what), int(reps), C.byref(ms)),
+              "net_time_layers")
+        return ms.value
 
 
 def write_tfile(path, xx, yy, zz, c):

@@ -13,11 +13,12 @@
 // guess = x_{n-1} + extrap (x_{n-1} - x_{n-2}) + out_scale * net.
 //
 // Tensors are node-major, 16 BF16 channels per node ([N][16], 32 bytes).
-// One wave computes 16 consecutive nodes of an x row for all 16 output
-// channels: per tap one v_mfma_f32_16x16x16_bf16 with A = the tap's weights
-// (output x input channels, held in registers) and B = the 16 nodes' input
-// channels (one 8-byte load per lane).  MFMA operand layout (16x16x16,
-// lane l): A[l%16][4(l/16)+i], B[4(l/16)+i][l%16], D[4(l/16)+i][l%16].
+// A wave computes 16 consecutive nodes of an x row for all 16 output
+// channels, row after row up z: per tap one v_mfma_f32_16x16x16_bf16 with
+// A = the tap's weights (output x input channels, held in registers) and
+// B = the 16 nodes' input channels (8 bytes per lane, from a register window
+// of three node planes).  MFMA operand layout (16x16x16, lane l):
+// A[l%16][4(l/16)+i], B[4(l/16)+i][l%16], D[4(l/16)+i][l%16].
 #include <cmath>
 #include <cstring>
 
@@ -65,84 +66,143 @@ __device__ __forceinline__ bool inside(const Grid3 &g, int x, int y, int z) {
   return x >= 0 && y >= 0 && z >= 0 && x < g.nax && y < g.nay && z < g.naz;
 }
 
-// Forward layer: Y[n][co] = act(b[co] + sum_t sum_ci W[t][co][ci] X[n+off_t][ci]).
-// LAST: no activation, channel 0 to out[n] in FP32.
-template <bool RELU, bool LAST>
-__global__ __launch_bounds__(256) void k_conv_fwd(Grid3 g, const uint16_t *__restrict__ X,
-                                                  const uint16_t *__restrict__ W,
-                                                  const float *__restrict__ bias,
-                                                  uint16_t *__restrict__ Y,
-                                                  float *__restrict__ out) {
+// Layer kernels marching up z.  One wave owns a column of 16-node x row
+// segments (segment, y) over a chunk of zc node planes; its 27 taps' weights
+// stay in registers for the whole march, and the input operands of the
+// three planes z-1, z, z+1 around the current output plane form a register
+// window (per plane the 9 (x, y) offsets of the tap stencil), so each plane
+// step loads the 9 operands of one new plane (issued one step ahead)
+// instead of 27 operands and 27 weight fragments.  The MFMAs of an output
+// row run in tap order t = 0..26 as in a one-row kernel, so the results do
+// not depend on the chunking.
+//   FWD_HIDDEN: Y[n][co] = leaky(b[co] + sum_t sum_ci W[t][co][ci] X[n+off_t][ci])
+//   FWD_LAST:   out[n] = b[0] + sum_t sum_ci W[t][0][ci] X[n+off_t][ci] (FP32)
+//   BWD_DATA:   dZprev[n][ci] = leaky'(Yprev[n][ci]) sum_t sum_co WT[t][ci][co]
+//               dZ[n-off_t][co]
+// (off_t = (t%3-1, t/3%3-1, t/9-1)).  Waves are numbered column-fastest
+// within a chunk and workgroups mapped XCD-aware, so the waves of one L2
+// march neighbouring rows over the same planes.
+enum ConvMode { FWD_HIDDEN = 0, FWD_LAST = 1, BWD_DATA = 2 };
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv_z(Grid3 g, int zc, const uint16_t *__restrict__ X,
+                                                const uint16_t *__restrict__ W,
+                                                const float *__restrict__ bias,
+                                                const uint16_t *__restrict__ Yprev,
+                                                uint16_t *__restrict__ Y,
+                                                float *__restrict__ out) {
   const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nseg = (int64_t)((g.nax + 15) >> 4) * g.nay * g.naz;
-  if (wave >= nseg) return;  // whole waves only
-  const RowSeg rs = row_seg(g, wave);
+  const int64_t wave =
+      (int64_t)xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int nsx = (g.nax + 15) >> 4;
+  const int64_t ncol = (int64_t)nsx * g.nay;
+  const int nzc = (g.naz + zc - 1) / zc;
+  if (wave >= ncol * nzc) return;  // whole waves only
+  const int64_t cidx = wave % ncol;
+  const int chunk = (int)(wave / ncol);
+  const int x0 = (int)(cidx % nsx) * 16, y = (int)(cidx / nsx);
+  const int z0 = chunk * zc, z1 = min(z0 + zc, g.naz);
   const int col = lane & 15, cg = lane >> 4;
-  const int x = rs.x0 + col;
+  const int x = x0 + col;
   v4s a[NTAP];
 #pragma unroll
   for (int t = 0; t < NTAP; ++t) a[t] = load4(W + (t * NC + col) * NC + 4 * cg);
-  v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int t = 0; t < NTAP; ++t) {
-    const int xx = x + t % 3 - 1, yy = rs.y + (t / 3) % 3 - 1, zz = rs.z + t / 9 - 1;
+  // operand of stencil offset q = (ox+1) + 3 (oy+1) in plane zz
+  auto ld = [&](int q, int zz) -> v4s {
+    const int xx = x + q % 3 - 1, yy = y + q / 3 - 1;
     v4s b = {0, 0, 0, 0};
     if (inside(g, xx, yy, zz))
       b = load4(X + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
-  }
-  if (x >= g.nax) return;
-  const int64_t n = x + (int64_t)g.nax * (rs.y + (int64_t)g.nay * rs.z);
-  if (LAST) {
-    if (cg == 0) out[n] = acc[0] + bias[0];
-    return;
-  }
-  v4s o;
+    return b;
+  };
+  v4s w0[9], w1[9], w2[9], wn[9];  // planes z-1, z, z+1, z+2
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    float v = acc[i] + bias[4 * cg + i];
-    if (RELU && !(v > 0.f)) v *= SLOPE;
-    o[i] = (short)f2bf(v);
+  for (int q = 0; q < 9; ++q) {
+    w0[q] = ld(q, z0 - 1);
+    w1[q] = ld(q, z0);
+    w2[q] = ld(q, z0 + 1);
   }
-  *reinterpret_cast<v4s *>(Y + n * NC + 4 * cg) = o;
+  for (int z = z0; z < z1; ++z) {
+    if (z + 1 < z1) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) wn[q] = ld(q, z + 2);
+    }
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < NTAP; ++t) {
+      // forward: X[n + off_t] = window (t/9, t%9); backward: dZ[n - off_t]
+      // = window (2 - t/9, 8 - t%9)
+      const int p = MODE == BWD_DATA ? 2 - t / 9 : t / 9;
+      const int q = MODE == BWD_DATA ? 8 - t % 9 : t % 9;
+      const v4s b = p == 0 ? w0[q] : p == 1 ? w1[q] : w2[q];
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
+    }
+    if (x < g.nax) {
+      const int64_t n = x + (int64_t)g.nax * (y + (int64_t)g.nay * z);
+      if (MODE == FWD_LAST) {
+        if (cg == 0) out[n] = acc[0] + bias[0];
+      } else if (MODE == FWD_HIDDEN) {
+        v4s o;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          float v = acc[i] + bias[4 * cg + i];
+          if (!(v > 0.f)) v *= SLOPE;
+          o[i] = (short)f2bf(v);
+        }
+        *reinterpret_cast<v4s *>(Y + n * NC + 4 * cg) = o;
+      } else {
+        const v4s yp = load4(Yprev + n * NC + 4 * cg);
+        v4s o;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+          o[i] = (short)f2bf(bf2f((uint16_t)yp[i]) > 0.f ? acc[i] : SLOPE * acc[i]);
+        *reinterpret_cast<v4s *>(Y + n * NC + 4 * cg) = o;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      w0[q] = w1[q];
+      w1[q] = w2[q];
+      w2[q] = wn[q];
+    }
+  }
 }
 
-// Backward through a layer's input: dX[n][ci] = sum_t sum_co W[t][co][ci]
-// dZ[n-off_t][co] (WT = W transposed per tap, [t][ci][co]); through the
-// leaky ReLU of the previous layer: dZprev = dX where Yprev > 0, else SLOPE dX.
-__global__ __launch_bounds__(256) void k_conv_bwd_data(Grid3 g,
-                                                       const uint16_t *__restrict__ dZ,
-                                                       const uint16_t *__restrict__ WT,
-                                                       const uint16_t *__restrict__ Yprev,
-                                                       uint16_t *__restrict__ dZprev) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6;
-  const int64_t nseg = (int64_t)((g.nax + 15) >> 4) * g.nay * g.naz;
-  if (wave >= nseg) return;
-  const RowSeg rs = row_seg(g, wave);
-  const int col = lane & 15, cg = lane >> 4;
-  const int x = rs.x0 + col;
-  v4s a[NTAP];
-#pragma unroll
-  for (int t = 0; t < NTAP; ++t) a[t] = load4(WT + (t * NC + col) * NC + 4 * cg);
-  v4f acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-  for (int t = 0; t < NTAP; ++t) {
-    const int xx = x - (t % 3 - 1), yy = rs.y - ((t / 3) % 3 - 1), zz = rs.z - (t / 9 - 1);
-    v4s b = {0, 0, 0, 0};
-    if (inside(g, xx, yy, zz))
-      b = load4(dZ + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
+// z chunk of the layer kernels.  A wave's time grows with the planes it
+// marches plus the two halo planes it loads, and the waves run in rounds of
+// the device's resident slots, so the chunk count minimises
+// ceil(waves / slots) x (planes per chunk + 2).
+inline int conv_zchunk(const Grid3 &g) {
+  static int slots = 0;
+  if (slots == 0) {
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+            hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &per_cu, reinterpret_cast<const void *>(&k_conv_z<FWD_HIDDEN>), 256, 0) !=
+            hipSuccess)
+      return g.naz;
+    slots = std::max(1, per_cu) * 4 * std::max(1, cus);  // waves
   }
-  if (x >= g.nax) return;
-  const int64_t n = x + (int64_t)g.nax * (rs.y + (int64_t)g.nay * rs.z);
-  const v4s yp = load4(Yprev + n * NC + 4 * cg);
-  v4s o;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-    o[i] = (short)f2bf(bf2f((uint16_t)yp[i]) > 0.f ? acc[i] : SLOPE * acc[i]);
-  *reinterpret_cast<v4s *>(dZprev + n * NC + 4 * cg) = o;
+  const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
+  int best = g.naz;
+  int64_t best_cost = -1;
+  for (int nch = 1; nch <= g.naz; ++nch) {
+    const int zc = (g.naz + nch - 1) / nch;
+    const int64_t waves = ncol * ((g.naz + zc - 1) / zc);
+    const int64_t cost = ((waves + slots - 1) / slots) * (zc + 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = zc;
+    }
+  }
+  return best;
+}
+inline dim3 conv_grid(const Grid3 &g, int zc) {
+  const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
+  const int64_t waves = ncol * ((g.naz + zc - 1) / zc);
+  return dim3((unsigned)((waves + 3) / 4));
 }
 
 // Weight and bias gradients of a layer, per wave over a fixed set of row
@@ -412,15 +472,16 @@ struct vgpu_net {
     hipLaunchKernelGGL(k_pack, grid1(N), dim3(256), 0, s, N, x1, x2, src, vx, vy, vz,
                        in_scale[0], in_scale[1], in_scale[2], in_scale[3], in_scale[4],
                        in_scale[5], X0);
-    const dim3 grid((unsigned)((nseg + 3) / 4)), blk(256);
+    const int zc = conv_zchunk(g);
+    const dim3 grid = conv_grid(g, zc), blk(256);
     const uint16_t *in = X0;
     for (int l = 0; l < NL - 1; ++l) {
-      hipLaunchKernelGGL((k_conv_fwd<true, false>), grid, blk, 0, s, g, in, Wb + l * WL,
-                         B + l * NC, Y[l], nullptr);
+      hipLaunchKernelGGL((k_conv_z<FWD_HIDDEN>), grid, blk, 0, s, g, zc, in, Wb + l * WL,
+                         B + l * NC, nullptr, Y[l], nullptr);
       in = Y[l];
     }
-    hipLaunchKernelGGL((k_conv_fwd<false, true>), grid, blk, 0, s, g, in,
-                       Wb + (NL - 1) * WL, B + (NL - 1) * NC, nullptr, out);
+    hipLaunchKernelGGL((k_conv_z<FWD_LAST>), grid, blk, 0, s, g, zc, in, Wb + (NL - 1) * WL,
+                       B + (NL - 1) * NC, nullptr, nullptr, out);
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
   }
@@ -625,7 +686,8 @@ int vgpu_net_train_step_dev(vgpu_net *n, const double *x1, const double *x2,
   hipLaunchKernelGGL(k_pack_grad, grid1(n->N), dim3(256), 0, s, n->N, gout, n->dZ4);
   VGPU_HIP(hipGetLastError());
   // backward: layer 4 (no activation), then 3..0 through the ReLUs
-  const dim3 grid((unsigned)((n->nseg + 3) / 4)), blk(256);
+  const int zc = conv_zchunk(n->c->g);
+  const dim3 grid = conv_grid(n->c->g, zc), blk(256);
   const uint16_t *dz = n->dZ4;
   uint16_t *bufs[2] = {n->dZa, n->dZb};
   for (int l = NL - 1; l >= 0; --l) {
@@ -634,8 +696,8 @@ int vgpu_net_train_step_dev(vgpu_net *n, const double *x1, const double *x2,
     if (rc) return rc;
     if (l > 0) {
       uint16_t *dprev = bufs[l & 1];
-      hipLaunchKernelGGL(k_conv_bwd_data, grid, blk, 0, s, n->c->g, dz, n->WbT + l * WL,
-                         n->Y[l - 1], dprev);
+      hipLaunchKernelGGL((k_conv_z<BWD_DATA>), grid, blk, 0, s, n->c->g, zc, dz,
+                         n->WbT + l * WL, nullptr, n->Y[l - 1], dprev, nullptr);
       dz = dprev;
     }
   }
@@ -675,6 +737,69 @@ int vgpu_net_reset_optimizer(vgpu_net *n) {
   for (float *p : {n->mW, n->vW}) VGPU_HIP(hipMemset(p, 0, NL * WL * sizeof(float)));
   for (float *p : {n->mB, n->vB}) VGPU_HIP(hipMemset(p, 0, NL * NC * sizeof(float)));
   n->adam_t = 0;
+  return VGPU_OK;
+}
+
+int vgpu_net_time_layers(vgpu_net *n, int what, int reps, double *avg_ms) {
+  if (!n || !avg_ms || reps < 1 || what < 0 || what > 3) return VGPU_ERR_ARG;
+  VGPU_HIP(hipSetDevice(n->c->device));
+  hipStream_t s = n->c->stream;
+  const Grid3 &g = n->c->g;
+  const int zc = conv_zchunk(g);
+  const dim3 grid = conv_grid(g, zc), blk(256);
+  // layer 1 (16 -> 16 channels) on the network's own buffers
+  auto launch = [&]() -> int {
+    switch (what) {
+      case 0:
+        hipLaunchKernelGGL((k_conv_z<FWD_HIDDEN>), grid, blk, 0, s, g, zc, n->Y[0],
+                           n->Wb + WL, n->B + NC, nullptr, n->Y[1], nullptr);
+        break;
+      case 1:
+        hipLaunchKernelGGL((k_conv_z<BWD_DATA>), grid, blk, 0, s, g, zc, n->dZa,
+                           n->WbT + WL, nullptr, n->Y[0], n->dZb, nullptr);
+        break;
+      case 2:
+        return n->weight_grad(1, n->Y[0], n->dZa);
+      default:  // the whole prediction network on zero states
+        return n->forward(n->e, n->e0, n->r);
+    }
+    VGPU_HIP(hipGetLastError());
+    return VGPU_OK;
+  };
+  for (int i = 0; i < 2; ++i) {
+    int rc = launch();
+    if (rc) return rc;
+  }
+  // the timed launches are captured in one graph, so the events bracket GPU
+  // work only
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  VGPU_HIP(hipStreamSynchronize(s));
+  VGPU_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int lrc = VGPU_OK;
+  for (int i = 0; i < reps && lrc == VGPU_OK; ++i) lrc = launch();
+  const hipError_t ce = hipStreamEndCapture(s, &graph);
+  if (lrc) {
+    if (graph) hipGraphDestroy(graph);
+    return lrc;
+  }
+  VGPU_HIP(ce);
+  VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+  hipEvent_t e0, e1;
+  VGPU_HIP(hipEventCreate(&e0));
+  VGPU_HIP(hipEventCreate(&e1));
+  VGPU_HIP(hipGraphLaunch(gexec, s));  // warm-up replay
+  VGPU_HIP(hipEventRecord(e0, s));
+  VGPU_HIP(hipGraphLaunch(gexec, s));
+  VGPU_HIP(hipEventRecord(e1, s));
+  VGPU_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  VGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipGraphExecDestroy(gexec);
+  hipGraphDestroy(graph);
+  *avg_ms = ms / reps;
   return VGPU_OK;
 }
 
