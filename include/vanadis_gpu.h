@@ -312,6 +312,10 @@ int vgpu_net_set_loss(vgpu_net *net, int kind);
  * that BF16 rounding leaves in the correction, the slowest modes of a
  * mass-dominated operator for BiCG. */
 int vgpu_net_set_smoothing(vgpu_net *net, int passes);
+/* Node planes each wave of the layer kernels marches (0, the default: the
+ * count that balances the waves over the device's resident slots; n > 0: n).
+ * The outputs do not depend on it (bitwise). */
+int vgpu_net_set_zchunk(vgpu_net *net, int zchunk);
 /* Average GPU time (HIP events around a captured graph of reps launches on
  * the context's stream) of: 0 a hidden forward layer (16 -> 16 channels,
  * leaky ReLU), 1 a backward-data layer, 2 a layer's weight and bias

@@ -87,6 +87,46 @@ def test_gradients_match_oracle(pkg, po, extrap):
     ctx.close()
 
 
+def test_layer_chunking_is_bitwise_invisible(pkg, po):
+    """The layer kernels march up z over chunks of node planes; the
+    prediction, the loss and the gradients are bitwise the same for one plane
+    per wave, several (uneven last chunk) and the whole grid in one march,
+    and match the oracle's forward pass."""
+    p, ctx, net, vec, dev = _setup(pkg, (20, 17, 9), 4)
+    rng = np.random.default_rng(6)
+    w = (rng.standard_normal((L, T, C, C)) * 0.12).astype(np.float32)
+    b = (rng.standard_normal((L, C)) * 0.05).astype(np.float32)
+    scale = np.array([1.0, 0.8, 1.2, 1, 1, 1], np.float32)
+    net.set_params(w, b)
+    net.set_static(vec["vx"], vec["vy"], vec["vz"], scale, 0.7, 1.0)
+    res = []
+    for zc in (1, 3, 4, 10, 0):
+        log(f"net z chunk {zc}")
+        net.set_zchunk(zc)
+        net.predict_dev(dev["x1"], dev["x2"], dev["src"], dev["guess"])
+        g = ctx.d2h(dev["guess"], p.N)
+        loss = net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 0.0)
+        gw, gb = net.get_grads()
+        res.append((g, loss, gw, gb))
+    for g, loss, gw, gb in res[1:]:
+        assert np.array_equal(g, res[0][0]) and loss == res[0][1]
+        assert np.array_equal(gw, res[0][2]) and np.array_equal(gb, res[0][3])
+    _, _, _, out = po.net_grads(p.nax, p.nay, p.naz, vec["x1"], vec["x2"], vec["src"],
+                                vec["vx"], vec["vy"], vec["vz"], vec["tgt"], p.mask, scale,
+                                0.7, w, b, quant=True, extrap=1.0)
+    base = vec["x1"] + (vec["x1"] - vec["x2"])
+    pred = (res[0][0] - base) / 0.7
+    assert np.linalg.norm(pred - out) <= 1e-2 * np.linalg.norm(out)
+    with pytest.raises(pkg.VgpuError):
+        net.set_zchunk(-1)
+    for what in range(4):
+        assert net.time_layers(what, reps=2) > 0.0
+    with pytest.raises(pkg.VgpuError):
+        net.time_layers(4, reps=2)
+    net.close()
+    ctx.close()
+
+
 def test_residual_loss_gradient_matches_finite_difference(pkg):
     """Initial-residual loss |A (t - g)|^2 / |A (t - base)|^2: its gradient
     w.r.t. the output bias (the sum of dL/dout) against central differences."""

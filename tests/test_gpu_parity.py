@@ -87,6 +87,25 @@ def test_owner_tile_shapes(case, po, tile, zchunk):
         ctx.set_options("owner", 0, 0)
 
 
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+@pytest.mark.parametrize("zchunk", [1, 3, 8])
+def test_brick_depths(case, po, scatter, zchunk):
+    """The colored and atomic scatters at explicit brick depths (element
+    layers per brick; 0 above lets the library choose) give the operator,
+    forward and transpose."""
+    name, p, bb, _, _, ctx = case
+    log(f"brick depth {zchunk} {scatter} {name}")
+    x = np.random.default_rng(19).standard_normal(p.N)
+    ctx.set_options(scatter, zchunk, 0)
+    try:
+        for tr in (False, True):
+            ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+            y = ctx.apply(x, scatter=scatter, transpose=tr)
+            assert rel(y, ref) <= 1e-13, (tr, rel(y, ref))
+    finally:
+        ctx.set_options("owner", 0, 0)
+
+
 @pytest.mark.parametrize("scatter", SCATTERS)
 def test_apply_fp32_matches_oracle(case, po, pkg, scatter):
     name, p, bb, _, _, ctx = case

@@ -39,6 +39,7 @@ EXPORTED = (
     "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
     "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
     "vgpu_set_method", "vgpu_axpbypcz_dev", "vgpu_net_time_layers",
+    "vgpu_net_set_zchunk",
 )
 
 
@@ -132,6 +133,7 @@ def load():
     L.vgpu_net_set_loss.argtypes = [_vp, C.c_int]
     L.vgpu_net_set_smoothing.argtypes = [_vp, C.c_int]
     L.vgpu_net_time_layers.argtypes = [_vp, C.c_int, C.c_int, _pd]
+    L.vgpu_net_set_zchunk.argtypes = [_vp, C.c_int]
     _lib = L
     return L
 
@@ -491,6 +493,10 @@ class Net:
     def set_loss(self, kind):
         """0: increment L2 loss; 1: initial-residual loss through the operator."""
         check(load().vgpu_net_set_loss(self.h, int(kind)), "net_set_loss")
+
+    def set_zchunk(self, zchunk=0):
+        """Node planes per wave of the layer kernels (0: automatic)."""
+        check(load().vgpu_net_set_zchunk(self.h, int(zchunk)), "net_set_zchunk")
 
     def time_layers(self, what=0, reps=20):
         """Average ms of 0 a hidden forward layer, 1 a backward-data layer,

@@ -455,6 +455,8 @@ struct vgpu_net {
   int adam_t = 0;
   int loss_kind = 0;  // 0: increment L2, 1: operator residual
   int smooth = 0;     // passes of the [1 4 1]^3 / 216 filter on the output
+  int zchunk = 0;     // node planes per wave of the layer kernels (0: automatic)
+  int layer_zc() const { return zchunk > 0 ? std::min(zchunk, c->g.naz) : conv_zchunk(c->g); }
   double *e = nullptr, *e0 = nullptr, *r = nullptr, *r0 = nullptr;
   float *fo = nullptr, *ft = nullptr;  // filtered output / gradient, scratch
 
@@ -472,7 +474,7 @@ struct vgpu_net {
     hipLaunchKernelGGL(k_pack, grid1(N), dim3(256), 0, s, N, x1, x2, src, vx, vy, vz,
                        in_scale[0], in_scale[1], in_scale[2], in_scale[3], in_scale[4],
                        in_scale[5], X0);
-    const int zc = conv_zchunk(g);
+    const int zc = layer_zc();
     const dim3 grid = conv_grid(g, zc), blk(256);
     const uint16_t *in = X0;
     for (int l = 0; l < NL - 1; ++l) {
@@ -686,7 +688,7 @@ int vgpu_net_train_step_dev(vgpu_net *n, const double *x1, const double *x2,
   hipLaunchKernelGGL(k_pack_grad, grid1(n->N), dim3(256), 0, s, n->N, gout, n->dZ4);
   VGPU_HIP(hipGetLastError());
   // backward: layer 4 (no activation), then 3..0 through the ReLUs
-  const int zc = conv_zchunk(n->c->g);
+  const int zc = n->layer_zc();
   const dim3 grid = conv_grid(n->c->g, zc), blk(256);
   const uint16_t *dz = n->dZ4;
   uint16_t *bufs[2] = {n->dZa, n->dZb};
@@ -725,6 +727,12 @@ int vgpu_net_set_smoothing(vgpu_net *n, int passes) {
   return VGPU_OK;
 }
 
+int vgpu_net_set_zchunk(vgpu_net *n, int zchunk) {
+  if (!n || zchunk < 0) return VGPU_ERR_ARG;
+  n->zchunk = zchunk;
+  return VGPU_OK;
+}
+
 int vgpu_net_set_loss(vgpu_net *n, int kind) {
   if (!n || kind < 0 || kind > 1) return VGPU_ERR_ARG;
   n->loss_kind = kind;
@@ -745,7 +753,7 @@ int vgpu_net_time_layers(vgpu_net *n, int what, int reps, double *avg_ms) {
   VGPU_HIP(hipSetDevice(n->c->device));
   hipStream_t s = n->c->stream;
   const Grid3 &g = n->c->g;
-  const int zc = conv_zchunk(g);
+  const int zc = n->layer_zc();
   const dim3 grid = conv_grid(g, zc), blk(256);
   // layer 1 (16 -> 16 channels) on the network's own buffers
   auto launch = [&]() -> int {
