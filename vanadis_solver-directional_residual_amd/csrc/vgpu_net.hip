@@ -34,7 +34,8 @@ constexpr int NC = 16;            // channels
 constexpr int NL = 5;             // layers
 constexpr int NTAP = 27;          // 3x3x3 taps, t = dx + 3 dy + 9 dz, offset d-1
 constexpr int WL = NTAP * NC * NC;  // weights per layer
-constexpr int NWAVE_W = 1024;     // waves of the weight-gradient kernel
+constexpr int NWAVE_W = 2048;     // waves of the weight-gradient kernel (one round at 2 per SIMD)
+constexpr int NBLK_W = NWAVE_W / 4;  // its workgroups (= partials)
 constexpr float SLOPE = 0.1f;     // leaky ReLU slope for negative inputs
 
 // round to nearest even (NaN kept quiet)
@@ -205,71 +206,136 @@ inline dim3 conv_grid(const Grid3 &g, int zc) {
   return dim3((unsigned)((waves + 3) / 4));
 }
 
-// Weight and bias gradients of a layer, per wave over a fixed set of row
-// segments (s = wave, wave + NWAVE_W, ...):
+// Weight and bias gradients of a layer:
 //   dW[t][co][ci] = sum_n dZ[n][co] X[n+off_t][ci],  db[co] = sum_n dZ[n][co]
-// as MFMAs over 16 nodes (K).  part[w][t][co][ci], partb[w][lane].
+// as MFMAs over 16 nodes (K): A[co][k] = dZ[node k][co], B[k][ci] =
+// X[node k + off_t][ci].  A lane needs one channel of 4 consecutive nodes,
+// while a node's channels are contiguous in memory, so each wave stages the
+// rows of its 16-node segment in its own LDS area with 8-byte loads (the
+// 3 x 3 (y, z) neighbourhood of X rows with one halo node either side, and
+// the dZ row) and reads the operands back transposed.  The node stride in
+// LDS is 40 bytes, so the four lane groups (4 nodes apart) fall on disjoint
+// banks.  Waves take row segments s = wave, wave + NWAVE_W, ...; the four
+// waves of a workgroup then add their accumulators in wave order through LDS
+// and write one partial per workgroup: part[blk][t][co][ci],
+// partb[blk][lane].
+constexpr int WS_RS = 20;  // LDS node stride (bf16): 16 channels + 4 pad
+constexpr int WS_RN = 18;  // staged nodes per X row: x0 - 1 .. x0 + 16
+constexpr int WS_ROW = WS_RN * WS_RS;
 __global__ __launch_bounds__(256) void k_conv_bwd_w(Grid3 g, const uint16_t *__restrict__ X,
                                                     const uint16_t *__restrict__ dZ,
                                                     float *__restrict__ part,
                                                     float *__restrict__ partb) {
-  const int lane = threadIdx.x & 63;
-  const int w = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  // per wave: 9 X rows and the dZ row; reused for the workgroup's sum
+  __shared__ __align__(16) uint16_t lds[4][10][WS_ROW];
+  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
+  const int w = blockIdx.x * 4 + wl;
   const int64_t nseg = (int64_t)((g.nax + 15) >> 4) * g.nay * g.naz;
   const int col = lane & 15, cg = lane >> 4;
+  uint16_t(*L)[WS_ROW] = lds[wl];
   v4f acc[NTAP];
 #pragma unroll
   for (int t = 0; t < NTAP; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
   float bsum = 0.f;
+  auto node = [&](const uint16_t *T, int xx, int yy, int zz) -> v4s {
+    v4s v = {0, 0, 0, 0};
+    if (inside(g, xx, yy, zz))
+      v = load4(T + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
+    return v;
+  };
   for (int64_t s = w; s < nseg; s += NWAVE_W) {
     const RowSeg rs = row_seg(g, s);
-    // A[co = col][k = node 4cg+i]
+    // stage: row q = (dy+1) + 3 (dz+1) of X at staged node col+1 (halo
+    // nodes 0 and 17 by lanes col 0 and 1), the dZ row at node col
+    v4s xr[9], xh[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      const int yy = rs.y + q % 3 - 1, zz = rs.z + q / 3 - 1;
+      xr[q] = node(X, rs.x0 + col, yy, zz);
+      xh[q] = col < 2 ? node(X, col == 0 ? rs.x0 - 1 : rs.x0 + 16, yy, zz) : v4s{0, 0, 0, 0};
+    }
+    const v4s zr = node(dZ, rs.x0 + col, rs.y, rs.z);
+#pragma unroll
+    for (int q = 0; q < 9; ++q) {
+      *reinterpret_cast<v4s *>(&L[q][(col + 1) * WS_RS + 4 * cg]) = xr[q];
+      if (col < 2)
+        *reinterpret_cast<v4s *>(&L[q][(col == 0 ? 0 : WS_RN - 1) * WS_RS + 4 * cg]) = xh[q];
+    }
+    *reinterpret_cast<v4s *>(&L[9][col * WS_RS + 4 * cg]) = zr;
+    // other lanes of this wave read what these lanes wrote: LDS operations
+    // of one wave complete in order; the fence and wave barrier keep the
+    // compiler from moving the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // A[co = col][k = node 4cg+i] (zero beyond the grid: the row load
+    // returned zeros there)
     v4s a;
-    int xi[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      xi[i] = rs.x0 + 4 * cg + i;
-      uint16_t h = 0;
-      if (xi[i] < g.nax)
-        h = dZ[(xi[i] + (int64_t)g.nax * (rs.y + (int64_t)g.nay * rs.z)) * NC + col];
+      const uint16_t h = L[9][(4 * cg + i) * WS_RS + col];
       a[i] = (short)h;
       bsum += bf2f(h);
     }
 #pragma unroll
     for (int t = 0; t < NTAP; ++t) {
-      const int dx = t % 3 - 1, yy = rs.y + (t / 3) % 3 - 1, zz = rs.z + t / 9 - 1;
+      const int dx = t % 3 - 1, q = t / 3;
       v4s b;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int xx = xi[i] + dx;
-        uint16_t h = 0;
-        if (xi[i] < g.nax && inside(g, xx, yy, zz))
-          h = X[(xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + col];
-        b[i] = (short)h;
-      }
+      for (int i = 0; i < 4; ++i) b[i] = (short)L[q][(4 * cg + i + dx + 1) * WS_RS + col];
       acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
     }
+    // the next segment's stores stay behind this segment's reads
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
-  float *pw = part + (int64_t)w * WL;
+  // the workgroup's sum in wave order through LDS (the staging area is free
+  // once every wave has passed the barrier)
+  float *red = reinterpret_cast<float *>(&lds[0][0][0]);  // [NTAP][4][64] + [64]
+  static_assert(sizeof(lds) >= (NTAP * 4 * 64 + 64) * sizeof(float), "LDS sum area");
+  __syncthreads();
+  for (int v = 0; v < 4; ++v) {
+    if (wl == v) {
 #pragma unroll
-  for (int t = 0; t < NTAP; ++t)
+      for (int t = 0; t < NTAP; ++t)
 #pragma unroll
-    for (int i = 0; i < 4; ++i) pw[(t * NC + 4 * cg + i) * NC + col] = acc[t][i];
-  partb[(int64_t)w * 64 + lane] = bsum;
+        for (int i = 0; i < 4; ++i) {
+          float &r = red[(t * 4 + i) * 64 + lane];
+          r = v == 0 ? acc[t][i] : r + acc[t][i];
+        }
+      float &rb = red[NTAP * 4 * 64 + lane];
+      rb = v == 0 ? bsum : rb + bsum;
+    }
+    __syncthreads();
+  }
+  float *pw = part + (int64_t)blockIdx.x * WL;
+  for (int e = threadIdx.x; e < NTAP * 4 * 64; e += blockDim.x) {
+    const int t = e / 256, i = (e / 64) % 4, l = e % 64;
+    pw[(t * NC + 4 * (l >> 4) + i) * NC + (l & 15)] = red[e];
+  }
+  if (threadIdx.x < 64) partb[(int64_t)blockIdx.x * 64 + threadIdx.x] = red[NTAP * 4 * 64 + threadIdx.x];
 }
 
-// dW[i] = sum over waves in order; db[co] = sum over waves and lane groups
+// dW[i] = sum over the workgroup partials in order (eight interleaved
+// running sums, added in a fixed order); db[co] = sum over partials and lane
+// groups
 __global__ void k_wsum(const float *__restrict__ part, const float *__restrict__ partb,
                        float *__restrict__ dW, float *__restrict__ db) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < WL) {
+    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (int w = 0; w < NBLK_W; w += 8)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] += part[(int64_t)(w + k) * WL + i];
     float s = 0.f;
-    for (int w = 0; w < NWAVE_W; ++w) s += part[(int64_t)w * WL + i];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s += acc[k];
     dW[i] = s;
   } else if (i < WL + NC) {
     const int co = i - WL;
     float s = 0.f;
-    for (int w = 0; w < NWAVE_W; ++w)
+    for (int w = 0; w < NBLK_W; ++w)
       for (int c = 0; c < 4; ++c) s += partb[(int64_t)w * 64 + c * 16 + co];
     db[co] = s;
   }
@@ -509,7 +575,7 @@ struct vgpu_net {
 
   int weight_grad(int l, const uint16_t *Xin, const uint16_t *dZl) {
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(NWAVE_W / 4), dim3(256), 0, s, c->g, Xin, dZl,
+    hipLaunchKernelGGL(k_conv_bwd_w, dim3(NBLK_W), dim3(256), 0, s, c->g, Xin, dZl,
                        part, partb);
     hipLaunchKernelGGL(k_wsum, dim3((WL + NC + 255) / 256), dim3(256), 0, s, part, partb,
                        dW + l * WL, dB + l * NC);
@@ -545,8 +611,8 @@ int vgpu_net_create(vgpu_ctx *c, vgpu_net **out) {
   A((void **)&n->ft, n->N * sizeof(float));
   for (float **p : {&n->W, &n->mW, &n->vW, &n->dW}) A((void **)p, NL * WL * sizeof(float));
   for (float **p : {&n->B, &n->mB, &n->vB, &n->dB}) A((void **)p, NL * NC * sizeof(float));
-  A((void **)&n->part, (size_t)NWAVE_W * WL * sizeof(float));
-  A((void **)&n->partb, (size_t)NWAVE_W * 64 * sizeof(float));
+  A((void **)&n->part, (size_t)NBLK_W * WL * sizeof(float));
+  A((void **)&n->partb, (size_t)NBLK_W * 64 * sizeof(float));
   A((void **)&n->Wb, NL * WL * sizeof(uint16_t));
   A((void **)&n->WbT, NL * WL * sizeof(uint16_t));
   for (double **p : {&n->vx, &n->vy, &n->vz, &n->e, &n->e0, &n->r, &n->r0})
