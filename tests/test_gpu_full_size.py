@@ -68,3 +68,23 @@ def test_mixed_full_size(c2, pkg):
     print(f"  mixed inner iterations {it}, true rel residual {err:.2e}", flush=True)
     assert err <= 1e-8 and ctx.true_residual(b, x) <= 1e-8
     assert np.linalg.norm(x) == pytest.approx(g["x_norm2"], rel=1e-8)
+
+
+def test_bicgstab_full_size(c2, pkg):
+    """FP64 BiCGSTAB (same preconditioner and stopping rule) reaches the
+    oracle's BiCG solution at the benchmark size."""
+    p, bb, ctx = c2
+    g = json.load(open(os.path.join(HERE, "golden", "oracle_c2.json")))
+    b = ctx.assemble_rhs(p.q_in)
+    ctx.set_options("owner", 0, 0)
+    ctx.set_method("bicgstab")
+    try:
+        x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=5000)
+    finally:
+        ctx.set_method("bicg")
+    print(f"  bicgstab iterations {it}, recursive {err:.2e}", flush=True)
+    assert rc == 0 and err <= 1e-8
+    assert ctx.true_residual(b, x) <= 1e-8
+    assert np.linalg.norm(x) == pytest.approx(g["x_norm2"], rel=1e-8)
+    assert ctx.mass(x) == pytest.approx(g["mass"], rel=1e-8)
+    assert x.sum() == pytest.approx(g["x_sum"], rel=1e-8)
