@@ -200,6 +200,24 @@ inline int conv_zchunk(const Grid3 &g) {
   }
   return best;
 }
+// z chunk of the weight-gradient march: the (column, chunk) items run in
+// rounds of NWAVE_W waves, so the chunk count minimises
+// ceil(items / NWAVE_W) x (planes per chunk + 2)
+inline int wgrad_zchunk(const Grid3 &g) {
+  const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
+  int best = g.naz;
+  int64_t best_cost = -1;
+  for (int nch = 1; nch <= g.naz; ++nch) {
+    const int zc = (g.naz + nch - 1) / nch;
+    const int64_t items = ncol * ((g.naz + zc - 1) / zc);
+    const int64_t cost = ((items + NWAVE_W - 1) / NWAVE_W) * (zc + 2);
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = zc;
+    }
+  }
+  return best;
+}
 inline dim3 conv_grid(const Grid3 &g, int zc) {
   const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
   const int64_t waves = ncol * ((g.naz + zc - 1) / zc);
@@ -210,27 +228,33 @@ inline dim3 conv_grid(const Grid3 &g, int zc) {
 //   dW[t][co][ci] = sum_n dZ[n][co] X[n+off_t][ci],  db[co] = sum_n dZ[n][co]
 // as MFMAs over 16 nodes (K): A[co][k] = dZ[node k][co], B[k][ci] =
 // X[node k + off_t][ci].  A lane needs one channel of 4 consecutive nodes,
-// while a node's channels are contiguous in memory, so each wave stages the
-// rows of its 16-node segment in its own LDS area with 8-byte loads (the
-// 3 x 3 (y, z) neighbourhood of X rows with one halo node either side, and
-// the dZ row) and reads the operands back transposed.  The node stride in
-// LDS is 40 bytes, so the four lane groups (4 nodes apart) fall on disjoint
-// banks.  Waves take row segments s = wave, wave + NWAVE_W, ...; the four
-// waves of a workgroup then add their accumulators in wave order through LDS
-// and write one partial per workgroup: part[blk][t][co][ci],
-// partb[blk][lane].
+// while a node's channels are contiguous in memory, so each wave stages rows
+// in its own LDS area with 8-byte loads and reads the operands back
+// transposed.  A wave marches a column of 16-node x-row segments (segment,
+// y) up a chunk of zc planes: the 3 X rows (y-1, y, y+1) of planes z-1, z,
+// z+1 sit in a ring of three plane slots (plane p in slot p mod 3), each with
+// one halo node either side, so a plane step stages one new plane (loaded a
+// step ahead) and the dZ row.  The node stride in LDS is 40 bytes, so the
+// four lane groups (4 nodes apart) fall on disjoint banks.  Waves take
+// (column, chunk) items w, w + NWAVE_W, ...; the four waves of a workgroup
+// then add their accumulators in wave order through LDS and write one partial
+// per workgroup: part[blk][t][co][ci], partb[blk][lane].
 constexpr int WS_RS = 20;  // LDS node stride (bf16): 16 channels + 4 pad
 constexpr int WS_RN = 18;  // staged nodes per X row: x0 - 1 .. x0 + 16
 constexpr int WS_ROW = WS_RN * WS_RS;
-__global__ __launch_bounds__(256) void k_conv_bwd_w(Grid3 g, const uint16_t *__restrict__ X,
+__global__ __launch_bounds__(256) void k_conv_bwd_w(Grid3 g, int zc,
+                                                    const uint16_t *__restrict__ X,
                                                     const uint16_t *__restrict__ dZ,
                                                     float *__restrict__ part,
                                                     float *__restrict__ partb) {
-  // per wave: 9 X rows and the dZ row; reused for the workgroup's sum
+  // per wave: 3 plane slots x 3 X rows and the dZ row; reused for the
+  // workgroup's sum
   __shared__ __align__(16) uint16_t lds[4][10][WS_ROW];
   const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
   const int w = blockIdx.x * 4 + wl;
-  const int64_t nseg = (int64_t)((g.nax + 15) >> 4) * g.nay * g.naz;
+  const int nsx = (g.nax + 15) >> 4;
+  const int64_t ncol = (int64_t)nsx * g.nay;
+  const int64_t nitem = ncol * ((g.naz + zc - 1) / zc);
   const int col = lane & 15, cg = lane >> 4;
   uint16_t(*L)[WS_ROW] = lds[wl];
   v4f acc[NTAP];
@@ -243,52 +267,76 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(Grid3 g, const uint16_t *__r
       v = load4(T + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
     return v;
   };
-  for (int64_t s = w; s < nseg; s += NWAVE_W) {
-    const RowSeg rs = row_seg(g, s);
-    // stage: row q = (dy+1) + 3 (dz+1) of X at staged node col+1 (halo
-    // nodes 0 and 17 by lanes col 0 and 1), the dZ row at node col
-    v4s xr[9], xh[9];
+  for (int64_t it = w; it < nitem; it += NWAVE_W) {
+    const int64_t cidx = it % ncol;
+    const int x0 = (int)(cidx % nsx) * 16, y = (int)(cidx / nsx);
+    const int z0 = (int)(it / ncol) * zc, z1 = min(z0 + zc, g.naz);
+    // rows of the next plane (staged node col+1; halo nodes 0 and 17 by
+    // lanes col 0 and 1) and the next dZ row (node col)
+    v4s nr[3], nh[3], nz;
+    auto load_plane = [&](int zz) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      const int yy = rs.y + q % 3 - 1, zz = rs.z + q / 3 - 1;
-      xr[q] = node(X, rs.x0 + col, yy, zz);
-      xh[q] = col < 2 ? node(X, col == 0 ? rs.x0 - 1 : rs.x0 + 16, yy, zz) : v4s{0, 0, 0, 0};
+      for (int r = 0; r < 3; ++r) {
+        nr[r] = node(X, x0 + col, y + r - 1, zz);
+        nh[r] = col < 2 ? node(X, col == 0 ? x0 - 1 : x0 + 16, y + r - 1, zz)
+                        : v4s{0, 0, 0, 0};
+      }
+    };
+    auto stage_plane = [&](int zz) {
+      const int sl = ((zz % 3) + 3) % 3;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        *reinterpret_cast<v4s *>(&L[sl * 3 + r][(col + 1) * WS_RS + 4 * cg]) = nr[r];
+        if (col < 2)
+          *reinterpret_cast<v4s *>(&L[sl * 3 + r][(col == 0 ? 0 : WS_RN - 1) * WS_RS + 4 * cg]) =
+              nh[r];
+      }
+    };
+    load_plane(z0 - 1);
+    stage_plane(z0 - 1);
+    load_plane(z0);
+    stage_plane(z0);
+    load_plane(z0 + 1);
+    nz = node(dZ, x0 + col, y, z0);
+    for (int z = z0; z < z1; ++z) {
+      stage_plane(z + 1);  // the slot of plane z - 2
+      *reinterpret_cast<v4s *>(&L[9][col * WS_RS + 4 * cg]) = nz;
+      if (z + 1 < z1) {  // the next step's plane and dZ row, one step ahead
+        load_plane(z + 2);
+        nz = node(dZ, x0 + col, y, z + 1);
+      }
+      // other lanes of this wave read what these lanes wrote: LDS
+      // operations of one wave complete in order; the fence and wave
+      // barrier keep the compiler from moving the reads above the writes
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      // A[co = col][k = node 4cg+i] (zero beyond the grid: the row load
+      // returned zeros there)
+      v4s a;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint16_t h = L[9][(4 * cg + i) * WS_RS + col];
+        a[i] = (short)h;
+        bsum += bf2f(h);
+      }
+      const uint16_t *pl[3];  // slots of planes z-1, z, z+1
+#pragma unroll
+      for (int d = 0; d < 3; ++d) pl[d] = &L[(((z + d - 1) % 3 + 3) % 3) * 3][0];
+#pragma unroll
+      for (int t = 0; t < NTAP; ++t) {
+        const int dx = t % 3 - 1, r = (t / 3) % 3, d = t / 9;
+        const uint16_t *row = pl[d] + r * WS_ROW;
+        v4s b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) b[i] = (short)row[(4 * cg + i + dx + 1) * WS_RS + col];
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+      }
+      // the next step's stores stay behind this step's reads
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
-    const v4s zr = node(dZ, rs.x0 + col, rs.y, rs.z);
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      *reinterpret_cast<v4s *>(&L[q][(col + 1) * WS_RS + 4 * cg]) = xr[q];
-      if (col < 2)
-        *reinterpret_cast<v4s *>(&L[q][(col == 0 ? 0 : WS_RN - 1) * WS_RS + 4 * cg]) = xh[q];
-    }
-    *reinterpret_cast<v4s *>(&L[9][col * WS_RS + 4 * cg]) = zr;
-    // other lanes of this wave read what these lanes wrote: LDS operations
-    // of one wave complete in order; the fence and wave barrier keep the
-    // compiler from moving the reads above the writes
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // A[co = col][k = node 4cg+i] (zero beyond the grid: the row load
-    // returned zeros there)
-    v4s a;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const uint16_t h = L[9][(4 * cg + i) * WS_RS + col];
-      a[i] = (short)h;
-      bsum += bf2f(h);
-    }
-#pragma unroll
-    for (int t = 0; t < NTAP; ++t) {
-      const int dx = t % 3 - 1, q = t / 3;
-      v4s b;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = (short)L[q][(4 * cg + i + dx + 1) * WS_RS + col];
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
-    }
-    // the next segment's stores stay behind this segment's reads
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
   // the workgroup's sum in wave order through LDS (the staging area is free
   // once every wave has passed the barrier)
@@ -575,7 +623,8 @@ struct vgpu_net {
 
   int weight_grad(int l, const uint16_t *Xin, const uint16_t *dZl) {
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(NBLK_W), dim3(256), 0, s, c->g, Xin, dZl,
+    hipLaunchKernelGGL(k_conv_bwd_w, dim3(NBLK_W), dim3(256), 0, s, c->g, wgrad_zchunk(c->g),
+                       Xin, dZl,
                        part, partb);
     hipLaunchKernelGGL(k_wsum, dim3((WL + NC + 255) / 256), dim3(256), 0, s, part, partb,
                        dW + l * WL, dB + l * NC);
