@@ -365,26 +365,41 @@ __global__ __launch_bounds__(256) void k_conv_bwd_w(Grid3 g, int zc,
   if (threadIdx.x < 64) partb[(int64_t)blockIdx.x * 64 + threadIdx.x] = red[NTAP * 4 * 64 + threadIdx.x];
 }
 
-// dW[i] = sum over the workgroup partials in order (eight interleaved
-// running sums, added in a fixed order); db[co] = sum over partials and lane
-// groups
-__global__ void k_wsum(const float *__restrict__ part, const float *__restrict__ partb,
-                       float *__restrict__ dW, float *__restrict__ db) {
+// dW = sum of the workgroup partials in two fixed-order stages: stage 1
+// sums slice j of WS_SPLIT consecutive partials of each element (eight
+// interleaved running sums) into part2[j], the bias lanes likewise; stage 2
+// adds the slices in order (db[co] also over the four lane groups).
+constexpr int WS_SPLIT = 16;
+constexpr int WS_P2 = WL + 64;  // elements of one slice sum (weights, bias lanes)
+__global__ void k_wsum1(const float *__restrict__ part, const float *__restrict__ partb,
+                        float *__restrict__ part2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  constexpr int per = NBLK_W / WS_SPLIT;
+  static_assert(NBLK_W % (WS_SPLIT * 8) == 0, "slices of whole groups of 8");
+  if (i >= WS_P2) return;
+  float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  for (int w = j * per; w < (j + 1) * per; w += 8)
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      acc[k] += i < WL ? part[(int64_t)(w + k) * WL + i] : partb[(int64_t)(w + k) * 64 + i - WL];
+  float s = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) s += acc[k];
+  part2[(int64_t)j * WS_P2 + i] = s;
+}
+
+__global__ void k_wsum2(const float *__restrict__ part2, float *__restrict__ dW,
+                        float *__restrict__ db) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < WL) {
-    float acc[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    for (int w = 0; w < NBLK_W; w += 8)
-#pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] += part[(int64_t)(w + k) * WL + i];
     float s = 0.f;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) s += acc[k];
+    for (int j = 0; j < WS_SPLIT; ++j) s += part2[(int64_t)j * WS_P2 + i];
     dW[i] = s;
   } else if (i < WL + NC) {
     const int co = i - WL;
     float s = 0.f;
-    for (int w = 0; w < NBLK_W; ++w)
-      for (int c = 0; c < 4; ++c) s += partb[(int64_t)w * 64 + c * 16 + co];
+    for (int j = 0; j < WS_SPLIT; ++j)
+      for (int c = 0; c < 4; ++c) s += part2[(int64_t)j * WS_P2 + WL + c * 16 + co];
     db[co] = s;
   }
 }
@@ -561,7 +576,8 @@ struct vgpu_net {
   uint16_t *X0 = nullptr, *Y[NL - 1] = {}, *dZ4 = nullptr, *dZa = nullptr, *dZb = nullptr;
   float *out = nullptr;
   float *W = nullptr, *B = nullptr, *mW = nullptr, *vW = nullptr, *mB = nullptr,
-        *vB = nullptr, *dW = nullptr, *dB = nullptr, *part = nullptr, *partb = nullptr;
+        *vB = nullptr, *dW = nullptr, *dB = nullptr, *part = nullptr, *partb = nullptr,
+        *part2 = nullptr;
   uint16_t *Wb = nullptr, *WbT = nullptr;
   double *vx = nullptr, *vy = nullptr, *vz = nullptr, *lpart = nullptr;
   float in_scale[6] = {1, 1, 1, 1, 1, 1};
@@ -576,7 +592,7 @@ struct vgpu_net {
 
   ~vgpu_net() {
     void *ptrs[] = {X0, Y[0], Y[1], Y[2], Y[3], dZ4, dZa, dZb, out, W,  B, mW, vW, mB,
-                    vB, dW, dB, part, partb, Wb, WbT, vx, vy, vz, lpart, e, e0, r, r0,
+                    vB, dW, dB, part, partb, part2, Wb, WbT, vx, vy, vz, lpart, e, e0, r, r0,
                     fo, ft};
     for (void *p : ptrs)
       if (p) hipFree(p);
@@ -626,7 +642,9 @@ struct vgpu_net {
     hipLaunchKernelGGL(k_conv_bwd_w, dim3(NBLK_W), dim3(256), 0, s, c->g, wgrad_zchunk(c->g),
                        Xin, dZl,
                        part, partb);
-    hipLaunchKernelGGL(k_wsum, dim3((WL + NC + 255) / 256), dim3(256), 0, s, part, partb,
+    hipLaunchKernelGGL(k_wsum1, dim3((WS_P2 + 255) / 256, WS_SPLIT), dim3(256), 0, s, part,
+                       partb, part2);
+    hipLaunchKernelGGL(k_wsum2, dim3((WL + NC + 255) / 256), dim3(256), 0, s, part2,
                        dW + l * WL, dB + l * NC);
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
@@ -662,6 +680,7 @@ int vgpu_net_create(vgpu_ctx *c, vgpu_net **out) {
   for (float **p : {&n->B, &n->mB, &n->vB, &n->dB}) A((void **)p, NL * NC * sizeof(float));
   A((void **)&n->part, (size_t)NBLK_W * WL * sizeof(float));
   A((void **)&n->partb, (size_t)NBLK_W * 64 * sizeof(float));
+  A((void **)&n->part2, (size_t)WS_SPLIT * WS_P2 * sizeof(float));
   A((void **)&n->Wb, NL * WL * sizeof(uint16_t));
   A((void **)&n->WbT, NL * WL * sizeof(uint16_t));
   for (double **p : {&n->vx, &n->vy, &n->vz, &n->e, &n->e0, &n->r, &n->r0})
