@@ -432,6 +432,62 @@ void orc_matvec_0(double *xin, double *xout, const int *iss, const double *bb,
     if (mask[j] == 1) xout[j] = 0.0;
 }
 
+/* matvec_0 with the elements in 8 parity colours, colour (i&1) | (j&1)<<1 |
+ * (k&1)<<2 of the element's lowest node (i, j, k): elements of one colour
+ * share no node, so a colour's elements run in parallel (OpenMP, `threads`
+ * threads) and each node receives its contributions colour by colour -- the
+ * CPU counterpart of the reference's coloured scatter (coloring.cu:50-106,
+ * 289-360) over the stored element matrices of VANADIS.FOR:2149-2193.  Same
+ * result as orc_matvec_0 up to the order of a node's additions, and the same
+ * bits for every thread count.  Returns 0, or -1 if out of memory. */
+int orc_matvec_0_colored(double *xin, double *xout, const int *iss, const double *bb,
+                         int nax, int nay, int naz, const int *mask, int threads) {
+  const int E = (nax - 1) * (nay - 1) * (naz - 1);
+  const int N = nax * nay * naz;
+  int *list = (int *)malloc(sizeof(int) * (size_t)(E > 0 ? E : 1));
+  if (!list) return -1;
+  int start[9] = {0};
+  for (int e = 0; e < E; e++) {
+    const int n0 = iss[e] - 1;
+    const int c = (n0 % nax & 1) | ((n0 / nax) % nay & 1) << 1 | (n0 / (nax * nay) & 1) << 2;
+    start[c + 1]++;
+  }
+  for (int c = 0; c < 8; c++) start[c + 1] += start[c];
+  {
+    int fill[8];
+    for (int c = 0; c < 8; c++) fill[c] = start[c];
+    for (int e = 0; e < E; e++) {
+      const int n0 = iss[e] - 1;
+      const int c = (n0 % nax & 1) | ((n0 / nax) % nay & 1) << 1 | (n0 / (nax * nay) & 1) << 2;
+      list[fill[c]++] = e;
+    }
+  }
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int j = 0; j < N; j++) {
+    xout[j] = 0.0;
+    if (mask[j] == 1) xin[j] = 0.0;
+  }
+  for (int c = 0; c < 8; c++) {
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int q = start[c]; q < start[c + 1]; q++) {
+      const int e = list[q];
+      int nd[8];
+      for (int j = 0; j < 8; j++) nd[j] = iss[j * E + e] - 1;
+      const double *b3 = bb + 64 * (size_t)e; /* b3(i,j) = b3[8i + j] */
+      for (int i = 0; i < 8; i++) {
+        double s = xout[nd[i]];
+        for (int j = 0; j < 8; j++) s = s + b3[8 * i + j] * xin[nd[j]];
+        xout[nd[i]] = s;
+      }
+    }
+  }
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int j = 0; j < N; j++)
+    if (mask[j] == 1) xout[j] = 0.0;
+  free(list);
+  return 0;
+}
+
 /* mttvec_0, VANADIS.FOR:2195-2245: y = (I-D) A^T (I-D) x */
 void orc_mttvec_0(double *xin, double *xout, const int *iss, const double *bb,
                   int nax, int nay, int naz, const int *mask) {

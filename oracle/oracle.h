@@ -63,6 +63,9 @@ int orc_assemble(int variant, int scheme, double dt, int nax, int nay, int naz,
 
 void orc_matvec_0(double *xin, double *xout, const int *iss, const double *bb,
                   int nax, int nay, int naz, const int *mask);
+/* matvec_0 over 8 parity colours of elements, OpenMP within a colour */
+int orc_matvec_0_colored(double *xin, double *xout, const int *iss, const double *bb,
+                         int nax, int nay, int naz, const int *mask, int threads);
 void orc_mttvec_0(double *xin, double *xout, const int *iss, const double *bb,
                   int nax, int nay, int naz, const int *mask);
 double orc_dnrm2(int n, const double *x);

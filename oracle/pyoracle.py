@@ -50,6 +50,9 @@ def lib():
         L.orc_assemble.restype = C.c_int
         for f in (L.orc_matvec_0, L.orc_mttvec_0):
             f.argtypes = [_dp, _dp, _ip, _dp, C.c_int, C.c_int, C.c_int, _ip]
+        L.orc_matvec_0_colored.argtypes = [_dp, _dp, _ip, _dp, C.c_int, C.c_int, C.c_int, _ip,
+                                           C.c_int]
+        L.orc_matvec_0_colored.restype = C.c_int
         L.orc_dbcg_simple.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _pi, _pd, _dp,
                                       _ip, _dp, C.c_int, C.c_int, C.c_int, _ip]
         L.orc_dbcg_simple.restype = C.c_int
@@ -140,6 +143,16 @@ def matvec_0(x, iss, bb, nax, nay, naz, mask, transpose=False):
     y = np.zeros_like(xin)
     f = lib().orc_mttvec_0 if transpose else lib().orc_matvec_0
     f(xin, y, iss, bb, nax, nay, naz, mask)
+    return y
+
+
+def matvec_0_colored(x, iss, bb, nax, nay, naz, mask, threads=1):
+    """matvec_0 over 8 parity colours of elements, `threads` OpenMP threads
+    within a colour (oracle.c orc_matvec_0_colored)."""
+    xin = np.array(x, np.float64, copy=True)
+    y = np.zeros_like(xin)
+    if lib().orc_matvec_0_colored(xin, y, iss, bb, nax, nay, naz, mask, int(threads)) != 0:
+        raise MemoryError("orc_matvec_0_colored")
     return y
 
 

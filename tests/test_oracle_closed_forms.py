@@ -163,6 +163,26 @@ def test_matvec0_equals_masked_csr(pkg, po):
     assert np.allclose(d, A.diagonal(), rtol=1e-15, atol=0)
 
 
+@pytest.mark.parametrize("dims", [(5, 4, 3), (12, 9, 7)])
+def test_colored_matvec0_equals_masked_csr(pkg, po, dims):
+    """The 8-colour OpenMP matvec_0 (the CPU baseline's parallel operator)
+    is the masked CSR product, bitwise the same for any thread count, and
+    within rounding of the serial matvec_0 (only a node's addition order
+    differs)."""
+    p = pkg.problem.unit_box(*dims, vel_seed=4, lam=1e-2)
+    bb, b, d = po.assemble(p)
+    A = csr_from_bb(p, bb)
+    D = sp.diags(1.0 - p.mask)
+    ref = D @ A @ D @ (x := np.random.default_rng(2).standard_normal(p.N))
+    y1 = po.matvec_0_colored(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, threads=1)
+    y4 = po.matvec_0_colored(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, threads=4)
+    ys = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask)
+    assert np.array_equal(y1, y4)
+    assert np.allclose(y1, ref, rtol=0, atol=1e-14 * np.abs(ref).max())
+    assert np.abs(y1 - ys).max() <= 1e-15 * np.abs(ys).max() * 8
+    assert np.all(y1[p.mask == 1] == 0.0)
+
+
 def test_bicg_converges_to_direct_solution(pkg, po):
     p = pkg.problem.unit_box(6, 5, 4, vel_seed=5, lam=2e-2)
     bb, b, d = po.assemble(p)
