@@ -299,9 +299,53 @@ def batched_solves(pkg, tuned):
                      "bitwise_equal_to_single": same,
                      "note": "host-to-device copies of b and x included in both"}
         progress(f"batched {name}: {t_batch * 1e3:.1f} ms vs {t_seq * 1e3:.1f} ms one by one")
+        if not res[name]["all_converged"]:
+            res[name]["with_bicgstab_fallback"] = batched_fallback(pkg, ctxs, bs)
         for c in ctxs:
             c.close()
     return res
+
+
+def batched_fallback(pkg, ctxs, bs, cap=2000, itmax=20000):
+    """The same batch with the BiCG iterations capped at `cap` and the
+    members left unconverged finished by BiCGSTAB on their own contexts
+    (vgpu_batch_set_fallback), against the same policy one system at a time."""
+    with pkg.Batch(ctxs) as bt:
+        bt.set_fallback("bicgstab", itmax=itmax)
+        bt.solve(bs, rtol=1e-8, itmax=cap, raise_on_fail=False)  # warm-up, capture
+        t0 = time.time()
+        xs, its, errs, st, rc = bt.solve(bs, rtol=1e-8, itmax=cap, raise_on_fail=False)
+        t_batch = time.time() - t0
+
+    def one(c, bb):
+        x, it, err, rc1 = c.solve(bb, rtol=1e-8, itmax=cap, raise_on_fail=False)
+        if rc1 != pkg.OK:
+            c.set_method("bicgstab")
+            try:
+                x, it2, err, rc1 = c.solve(bb, x, rtol=1e-8, itmax=itmax, raise_on_fail=False)
+            finally:
+                c.set_method("bicg")
+            it += it2
+        return x, it, rc1
+
+    for c, bb in zip(ctxs, bs):
+        one(c, bb)  # warm-up
+    t0 = time.time()
+    same = True
+    for m, (c, bb) in enumerate(zip(ctxs, bs)):
+        x, it, _ = one(c, bb)
+        same = same and it == its[m] and bool(np.array_equal(x, xs[m]))
+    t_seq = time.time() - t0
+    fell_back = int((its > cap).sum())
+    progress(f"  with BiCGSTAB fallback ({fell_back} members): {t_batch * 1e3:.1f} ms vs "
+             f"{t_seq * 1e3:.1f} ms one by one, all converged {bool((st == 1).all())}")
+    return {"bicg_cap": cap, "members_finished_by_bicgstab": fell_back,
+            "iterations_min_max": [int(its.min()), int(its.max())],
+            "all_converged": bool((st == 1).all()),
+            "true_rel_residual_max": float(max(c.true_residual(bb, x)
+                                               for c, bb, x in zip(ctxs, bs, xs))),
+            "batched_s": round(t_batch, 4), "one_by_one_s": round(t_seq, 4),
+            "speedup": round(t_seq / t_batch, 3), "bitwise_equal_to_single": same}
 
 
 def main():

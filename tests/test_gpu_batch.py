@@ -147,3 +147,36 @@ def test_batch_matches_oracle_solution(pkg, po):
         assert r <= 1e-7, (m, r)
     for c in ctxs:
         c.close()
+
+
+def test_fallback_finishes_unconverged_members(pkg, gpu):
+    """vgpu_batch_set_fallback: members that the batched BiCG leaves at itmax
+    are finished by BiCGSTAB on their own contexts from their current
+    iterate and reach the solution; members that converge in the batch are
+    untouched (bitwise the result without the fallback)."""
+    probs = [pkg.problem.unit_box(14, 11, 9, vel_seed=s, lam=2e-3) for s in (1, 2, 3)]
+    ctxs = [_ctx(pkg, q) for q in probs]
+    bs = [c.assemble_rhs(q.q_in) for c, q in zip(ctxs, probs)]
+    with pkg.Batch(ctxs) as bt:
+        log("batch fallback: itmax 3 without the fallback")
+        _, it0, _, st0, rc0 = bt.solve(bs, rtol=1e-10, itmax=3, raise_on_fail=False)
+        assert (st0 == 2).all() and (it0 == 3).all() and rc0 == 5  # VGPU_NOT_CONVERGED
+        bt.set_fallback("bicgstab", itmax=5000)
+        log("batch fallback: itmax 3 with the BiCGSTAB fallback")
+        xs, its, ers, st, rc = bt.solve(bs, rtol=1e-10, itmax=3)
+        assert rc == 0 and (st == 1).all() and (its > 3).all() and (ers <= 1e-10).all()
+        for c, b, x in zip(ctxs, bs, xs):
+            xr, _, _, _ = c.solve(b, rtol=1e-12, itmax=20000)
+            assert c.true_residual(b, x) <= 1e-8
+            assert np.linalg.norm(x - xr) <= 1e-7 * np.linalg.norm(xr)
+        log("batch fallback: converged members untouched")
+        xa, ita, era, sta, _ = bt.solve(bs, rtol=1e-8, itmax=20000)
+        bt.set_fallback(None)
+        xb, itb, erb, stb, _ = bt.solve(bs, rtol=1e-8, itmax=20000)
+        assert (sta == 1).all() and (stb == 1).all()
+        assert np.array_equal(ita, itb) and np.array_equal(era, erb)
+        assert all(np.array_equal(a, b) for a, b in zip(xa, xb))
+        with pytest.raises(pkg.VgpuError):
+            bt.set_fallback("bicgstab", itmax=-1)
+    for c in ctxs:
+        c.close()

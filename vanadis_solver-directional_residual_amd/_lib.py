@@ -39,7 +39,7 @@ EXPORTED = (
     "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
     "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
     "vgpu_set_method", "vgpu_axpbypcz_dev", "vgpu_net_time_layers",
-    "vgpu_net_set_zchunk",
+    "vgpu_net_set_zchunk", "vgpu_batch_set_fallback",
 )
 
 
@@ -134,6 +134,7 @@ def load():
     L.vgpu_net_set_smoothing.argtypes = [_vp, C.c_int]
     L.vgpu_net_time_layers.argtypes = [_vp, C.c_int, C.c_int, _pd]
     L.vgpu_net_set_zchunk.argtypes = [_vp, C.c_int]
+    L.vgpu_batch_set_fallback.argtypes = [_vp, C.c_int, C.c_int]
     _lib = L
     return L
 
@@ -388,6 +389,12 @@ class Batch:
         tot = C.c_int64(0)
         check(load().vgpu_batch_layout(self.h, off, C.byref(tot)), "batch_layout")
         return list(off), tot.value
+
+    def set_fallback(self, method=None, itmax=20000):
+        """FP64 members left at itmax or in breakdown by the batched BiCG are
+        finished by BiCGSTAB on their own contexts ("bicgstab"; None: off)."""
+        code = METHOD_BICG if method is None else _METHOD_NAMES[method]
+        check(load().vgpu_batch_set_fallback(self.h, int(code), int(itmax)), "batch_set_fallback")
 
     def solve(self, bs, x0s=None, rtol=1e-8, itmax=20000, precision=FP64,
               raise_on_fail=True):

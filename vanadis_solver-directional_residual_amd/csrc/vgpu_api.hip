@@ -1121,6 +1121,8 @@ struct vgpu_batch {
   BBicg<double> e64;
   BBicg<float> e32;
   uint64_t gen = 0;
+  int fallback = VGPU_METHOD_BICG;  // VGPU_METHOD_BICGSTAB: finish unconverged members
+  int fallback_itmax = 0;  // iterations allowed to each member's fallback solve
 
   ~vgpu_batch() {
     e64.release();
@@ -1313,6 +1315,26 @@ int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double 
   }
   if (rc) return rc;
   VGPU_HIP(hipStreamSynchronize(bt->stream));
+  // FP64 members the batched BiCG left at itmax or in breakdown: BiCGSTAB on
+  // the member's own context from its current iterate, to the same rtol
+  if (precision == VGPU_FP64 && bt->fallback == VGPU_METHOD_BICGSTAB) {
+    for (size_t i = 0; i < bt->m.size(); ++i) {
+      if (status[i] != 2 && status[i] != 3) continue;
+      vgpu_ctx *c = bt->m[i];
+      const Segment &sg = bt->seg[i];
+      int it = 0, st = 0;
+      double e = 0;
+      VGPU_HIP(hipSetDevice(c->device));
+      rc = solve_stab<double>(c, c->stab64, b + sg.off, x + sg.off, c->diag, rtol,
+                              bt->fallback_itmax, &it, &e, &st);
+      if (rc) return rc;
+      VGPU_HIP(hipStreamSynchronize(c->stream));
+      iters[i] += it;
+      errs[i] = e;
+      status[i] = st;
+    }
+    VGPU_HIP(hipSetDevice(bt->device));
+  }
   const auto t1 = std::chrono::steady_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
   for (size_t i = 0; i < bt->m.size(); ++i) {
@@ -1326,6 +1348,25 @@ int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double 
       set_error("batched BiCG: system " + std::to_string(i) + " did not reach rtol");
       return VGPU_NOT_CONVERGED;
     }
+  return VGPU_OK;
+}
+
+int vgpu_batch_set_fallback(vgpu_batch *bt, int method, int itmax) {
+  if (!bt || (method != VGPU_METHOD_BICG && method != VGPU_METHOD_BICGSTAB) || itmax < 0) {
+    set_error("vgpu_batch_set_fallback: method must be VGPU_METHOD_BICG (off) or "
+              "VGPU_METHOD_BICGSTAB, itmax >= 0");
+    return VGPU_ERR_ARG;
+  }
+  if (method == VGPU_METHOD_BICGSTAB) {
+    for (vgpu_ctx *c : bt->m)
+      if (!owner_fits(c->g)) {
+        set_error("vgpu_batch_set_fallback: a member is too large for the BiCGSTAB owner "
+                  "launches (8 N and 40 E must stay below 2^31 bytes)");
+        return VGPU_ERR_ARG;
+      }
+  }
+  bt->fallback = method;
+  bt->fallback_itmax = itmax;
   return VGPU_OK;
 }
 
