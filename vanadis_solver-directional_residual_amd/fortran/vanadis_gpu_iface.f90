@@ -142,5 +142,14 @@ module vanadis_gpu
       integer(c_int), intent(out) :: iters(*), status(*)
       real(c_double), intent(out) :: errs(*)
     end function
+
+    ! members left at itmax / in breakdown by the batched BiCG are finished by
+    ! BiCGSTAB on their own contexts (VGPU_METHOD_BICGSTAB; VGPU_METHOD_BICG: off)
+    integer(c_int) function vgpu_batch_set_fallback(batch, method, itmax) &
+        bind(C, name="vgpu_batch_set_fallback")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: batch
+      integer(c_int), value :: method, itmax
+    end function
   end interface
 end module vanadis_gpu
