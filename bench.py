@@ -218,8 +218,8 @@ def conv_layers(pkg, ctx, N):
 def learned_guess(pkg):
     """Implicit Euler (dt 0.01, 200 steps, helix source of seed 0) on the
     128x128x64 box, every step to rtol 1e-8, from the previous-step guess
-    with BiCG (the plain method), and with linear and quadratic
-    extrapolation + 2 Jacobi sweeps and the trained network (committed
+    with BiCG (the plain method), and with linear, quadratic and cubic
+    extrapolation + Jacobi sweeps and the trained network (committed
     parameters, trained on seeds 1-10 by learned.main) + 2 Jacobi sweeps,
     each solved by BiCG and by BiCGSTAB; speedups against the plain method."""
     path = os.path.join(ROOT, "vanadis_solver-directional_residual_amd", "data",
@@ -229,7 +229,7 @@ def learned_guess(pkg):
     params = pkg.learned.load_params(path)
     modes = ("previous", "extrap+jacobi", "extrap2+jacobi", "net+jacobi", "previous@bicgstab",
              "extrap+jacobi@bicgstab", "extrap2+jacobi@bicgstab", "extrap2+jacobi4@bicgstab",
-             "net+jacobi@bicgstab")
+             "extrap3+jacobi8@bicgstab", "net+jacobi@bicgstab")
     progress(f"learned guess: {len(modes)} x 200 implicit-Euler steps")
     res, _ = pkg.learned.experiment(pkg, params=params, modes=modes, tune=True)
     out = {"workload": "HEX8 unit box 128x128x64, implicit Euler dt=0.01, 200 steps, "
@@ -252,12 +252,16 @@ def learned_guess(pkg):
         res["extrap2+jacobi@bicgstab_speedup_vs_previous"]
     out["extrap2_jacobi4_bicgstab_speedup_vs_previous"] = \
         res["extrap2+jacobi4@bicgstab_speedup_vs_previous"]
+    out["extrap3_jacobi8_bicgstab_speedup_vs_previous"] = \
+        res["extrap3+jacobi8@bicgstab_speedup_vs_previous"]
     progress(f"learned: net {out['net_speedup_vs_previous']}x, extrap "
              f"{out['extrap_speedup_vs_previous']}x, net+bicgstab "
              f"{out['net_bicgstab_speedup_vs_previous']}x, extrap+bicgstab "
              f"{out['extrap_bicgstab_speedup_vs_previous']}x, quadratic extrap+bicgstab "
              f"{out['extrap2_bicgstab_speedup_vs_previous']}x (4 sweeps "
-             f"{out['extrap2_jacobi4_bicgstab_speedup_vs_previous']}x) vs previous-step guess")
+             f"{out['extrap2_jacobi4_bicgstab_speedup_vs_previous']}x), cubic extrap + 8 sweeps "
+             f"+ bicgstab {out['extrap3_jacobi8_bicgstab_speedup_vs_previous']}x vs "
+             f"previous-step guess")
     return out
 
 

@@ -210,7 +210,7 @@ def test_trajectory_store_round_trip(pkg, tmp_path):
 
 def test_warm_starts_and_methods_reach_the_same_states(pkg):
     """20 implicit-Euler steps on a small box from every guess (previous,
-    linear and quadratic extrapolation, with Jacobi sweeps) and both methods:
+    linear, quadratic and cubic extrapolation, with Jacobi sweeps) and both methods:
     each step is solved to rtol 1e-8 of its own right-hand side, so the final
     states agree to the solver tolerance; the extrapolated guesses need fewer
     iterations than the previous-step guess, and quadratic extrapolation
@@ -221,7 +221,8 @@ def test_warm_starts_and_methods_reach_the_same_states(pkg):
     st = pkg.learned.Stepper(ctx, p, 0.01)
     finals, its = {}, {}
     modes = ("previous", "extrap+jacobi", "extrap2+jacobi", "previous@bicgstab",
-             "extrap@bicgstab", "extrap2@bicgstab", "extrap2+jacobi@bicgstab")
+             "extrap@bicgstab", "extrap2@bicgstab", "extrap2+jacobi@bicgstab",
+             "extrap3@bicgstab", "extrap3+jacobi8@bicgstab")
     for mode in modes:
         r = pkg.learned.run(st, 0, 20, mode)
         finals[mode] = ctx.d2h(r["last"], p.N)
@@ -234,4 +235,5 @@ def test_warm_starts_and_methods_reach_the_same_states(pkg):
     assert its["extrap2+jacobi"] < its["previous"]
     assert its["extrap@bicgstab"] < its["previous@bicgstab"]
     assert its["extrap2@bicgstab"] <= its["extrap@bicgstab"]
+    assert its["extrap3+jacobi8@bicgstab"] < its["extrap2+jacobi@bicgstab"]
     ctx.close()

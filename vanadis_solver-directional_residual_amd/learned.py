@@ -84,12 +84,13 @@ class Stepper:
         self.ctx.gaussian_dev(centre, sigma, self.q, self.src)
 
     def step(self, centre, sigma, x1, x2, out, mode="previous", net=None, rtol=1e-8,
-             itmax=20000, sweeps=2, omega=0.8, x3=None):
+             itmax=20000, sweeps=2, omega=0.8, x3=None, x4=None):
         """Solve step n into `out` from the guess of `mode`; x1 = x_{n-1},
         x2 = x_{n-2}, x3 = x_{n-3} or None (device).  mode is
         guess[+jacobi[N]][@method] (N damped Jacobi sweeps, default `sweeps`):
         guess "previous", "extrap" (2 x1 - x2),
-        "extrap2" (3 x1 - 3 x2 + x3; linear while x3 is None) or "net", method
+        "extrap2" (3 x1 - 3 x2 + x3; linear while x3 is None), "extrap3"
+        (4 x1 - 6 x2 + 4 x3 - x4; quadratic while x4 is None) or "net", method
         "bicg" (default) or "bicgstab".  Returns (iterations, seconds guess +
         solve, fallback used)."""
         ctx = self.ctx
@@ -104,8 +105,13 @@ class Stepper:
             ctx.copy_dev(out, x1, 8 * self.N)
         elif base == "extrap" or (base == "extrap2" and x3 is None):
             ctx.axpby_dev(2.0, x1, -1.0, x2, out)
-        elif base == "extrap2":
+        elif base == "extrap2" or (base == "extrap3" and x4 is None and x3 is not None):
             ctx.axpbypcz_dev(3.0, x1, -3.0, x2, 1.0, x3, out)
+        elif base == "extrap3" and x3 is None:
+            ctx.axpby_dev(2.0, x1, -1.0, x2, out)
+        elif base == "extrap3":
+            ctx.axpbypcz_dev(4.0, x1, -6.0, x2, 4.0, x3, out)
+            ctx.axpby_dev(1.0, out, -1.0, x4, out)
         elif base == "net":
             net.predict_dev(x1, x2, self.src, out)
             if ctx.residual_norm_dev(self.b, x1) < ctx.residual_norm_dev(self.b, out):
@@ -137,25 +143,31 @@ def run(stepper, seed, steps, mode="previous", net=None, states=None, rtol=1e-8)
     used).  Returns per-step iterations, seconds and fallbacks."""
     centres, sigma = helix(seed, steps)
     own = states is None
-    buf = stepper.alloc_states(4) if own else states
+    R = 5  # ring: x_{n-1} .. x_{n-4} and the output
+    buf = stepper.alloc_states(R) if own else states
     its, secs, fbs = [], [], []
     for n in range(1, steps + 1):
         if own:
-            x1, x2, x3, out = (stepper.ptr(buf, (n - 1) % 4), stepper.ptr(buf, (n - 2) % 4),
-                               stepper.ptr(buf, (n - 3) % 4), stepper.ptr(buf, n % 4))
+            x1, x2, x3, x4, out = (stepper.ptr(buf, (n - 1) % R), stepper.ptr(buf, (n - 2) % R),
+                                   stepper.ptr(buf, (n - 3) % R), stepper.ptr(buf, (n - 4) % R),
+                                   stepper.ptr(buf, n % R))
             if n == 1:
                 x2 = x1
         else:
-            x1, x2, x3, out = (stepper.ptr(buf, n - 1), stepper.ptr(buf, max(n - 2, 0)),
-                               stepper.ptr(buf, max(n - 3, 0)), stepper.ptr(buf, n))
+            x1, x2, x3, x4, out = (stepper.ptr(buf, n - 1), stepper.ptr(buf, max(n - 2, 0)),
+                                   stepper.ptr(buf, max(n - 3, 0)),
+                                   stepper.ptr(buf, max(n - 4, 0)), stepper.ptr(buf, n))
         if n < 3:
             x3 = None
-        it, sec, fb = stepper.step(centres[n - 1], sigma, x1, x2, out, mode, net, rtol, x3=x3)
+        if n < 4:
+            x4 = None
+        it, sec, fb = stepper.step(centres[n - 1], sigma, x1, x2, out, mode, net, rtol, x3=x3,
+                                   x4=x4)
         its.append(it)
         secs.append(sec)
         fbs.append(fb)
     return {"iterations": its, "seconds": secs, "fallbacks": int(sum(fbs)),
-            "last": stepper.ptr(buf, steps % 4 if own else steps)}
+            "last": stepper.ptr(buf, steps % R if own else steps)}
 
 
 def init_params(seed=0):
