@@ -21,6 +21,8 @@
 // A[l%16][4(l/16)+i], B[4(l/16)+i][l%16], D[4(l/16)+i][l%16].
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "vgpu_ctx.hpp"
 
@@ -174,17 +176,27 @@ __global__ __launch_bounds__(256) void k_conv_z(Grid3 g, int zc, const uint16_t 
 // the device's resident slots, so the chunk count minimises
 // ceil(waves / slots) x (planes per chunk + 2).
 inline int conv_zchunk(const Grid3 &g) {
-  static int slots = 0;
+  // resident waves of the layer kernel on the current device (cached per device)
+  static std::mutex mu;
+  static std::map<int, int> cache;
+  int dev = 0, slots = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return g.naz;
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(dev);
+    if (it != cache.end()) slots = it->second;
+  }
   if (slots == 0) {
-    int dev = 0, cus = 0, per_cu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
             hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
             &per_cu, reinterpret_cast<const void *>(&k_conv_z<FWD_HIDDEN>), 256, 0) !=
             hipSuccess)
       return g.naz;
     slots = std::max(1, per_cu) * 4 * std::max(1, cus);  // waves
+    std::lock_guard<std::mutex> lk(mu);
+    cache[dev] = slots;
   }
   const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
   int best = g.naz;
