@@ -13,7 +13,11 @@ FP64 BiCGSTAB and mixed precision (FP32 BiCGSTAB inner solves, FP64
 refinement), the other scatter modes, the roofline of the dominant kernel
 (the owner launch of every BiCG iteration: direction update p = z + beta p,
 pp = zz + beta pp fused into the dual apply A p, A^T pp), that of the plain
-dual apply, and the reference's CPU operator timed on the host.
+dual apply, the 32x32x16 and 256x256x64 workloads, the batched solves (FP64
+and mixed precision), the implicit-Euler learned-guess run and, in the CPU
+leg, the oracle's serial and OpenMP BiCG timed on the host, the rel-L2 of every
+GPU solution against the oracle's FP64 solve to 1e-12, and the legacy drop-in
+kernel_dbcg_simple_ on the oracle-assembled element matrices.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--kernel-only] [--no-cpu]
 """
@@ -51,52 +55,158 @@ def algorithmic_bytes(N, E, dual, fp32=False, fused=False):
     return vec + N + 5 * w * E
 
 
-def cpu_baseline(pkg, seconds=12.0):
-    """The reference's own CPU operator (matvec_0 of VANADIS.FOR built by
-    flang into oracle/_ref; the C oracle port when that build is absent) on a
-    128x128x8-element slab of the workload, single thread (the reference's
-    solver is serial), and beside it the C oracle's 8-colour OpenMP
-    matvec_0 on the host cores."""
+def host_cpu():
+    """Model, physical cores (unique core/socket pairs of `lscpu -p`), logical
+    CPUs, the CPUs this process may run on and OMP_NUM_THREADS; the baseline
+    threads = physical cores, capped by the affinity set and OMP_NUM_THREADS
+    when the environment sets it (a host shared with other jobs)."""
+    import subprocess
+    info = {"model": None, "physical_cores": None, "logical_cpus": os.cpu_count(),
+            "affinity_cpus": len(os.sched_getaffinity(0)),
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                info["model"] = line.split(":", 1)[1].strip()
+        pairs = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True,
+                               timeout=20).stdout
+        info["physical_cores"] = len({l for l in pairs.splitlines() if l and l[0] != "#"})
+    except (OSError, subprocess.SubprocessError):
+        pass
+    t = info["physical_cores"] or info["affinity_cpus"]
+    t = min(t, info["affinity_cpus"])
+    if info["omp_num_threads_env"] and info["omp_num_threads_env"].isdigit():
+        t = min(t, int(info["omp_num_threads_env"]))
+    info["threads_used"] = max(1, t)
+    return info
+
+
+def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
+    """The CPU leg: the oracle (oracle/, the C restatement of the reference's
+    dbcg_simple + matvec_0/mttvec_0, VANADIS.FOR:977-1055, 2149-2245, over
+    element matrices assembled like przest3d, 713-771) timed on the host at
+    the 128x128x64 workload, serial (as the reference's solver) and with the
+    8-colour OpenMP products; the oracle as the checker of the GPU solutions
+    (rel-L2 against its FP64 BiCG run to 1e-12: live for 32x32x16 and
+    128x128x64, from the committed sketches for 256x256x64, the batches and
+    the time stepping); and the legacy drop-in kernel_dbcg_simple_ on the
+    GPU, fed with the oracle's element matrices as the Fortran host feeds it.
+    sols: {(config, name): solution}."""
     import __graft_entry__ as ge
     po = ge.load_oracle()
-    p = pkg.problem.unit_box(128, 128, 8, vel_seed=0, lam=1e-3, extent=(1.0, 1.0, 0.125))
-    t0 = time.time()
-    bb, _, _ = po.assemble(p)
-    t_asm = time.time() - t0
-    kind = "port"
-    apply = lambda x: po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask)  # noqa: E731
-    if po.RefLib.available("v1"):
-        R = po.RefLib("v1")
-        R.set_sizes(p.E, p.N)
-        apply = lambda x: R.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask)  # noqa: E731
-        kind = "reference"
-    x = np.random.default_rng(0).standard_normal(p.N)
-    apply(x)
-    reps, t0 = 0, time.time()
-    while time.time() - t0 < seconds:
-        apply(x)
-        reps += 1
-    dt = (time.time() - t0) / reps
-    out = {"value": round(p.N / dt / 1e9, 5), "unit": "GDOF/s", "cores": 1, "kind": kind,
-           "sample": f"matvec_0 on stored element matrices, 128x128x8 elements "
-                     f"({p.N} nodes), {reps} applies in {seconds:.0f} s after "
-                     f"{t_asm:.1f} s assembly; serial like the reference solver",
-           "ms_per_apply": round(dt * 1e3, 3)}
-    # the same operator over 8 parity colours of elements, OpenMP within a
-    # colour, on the host cores this process may use (at most 16)
-    threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    po.matvec_0_colored(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, threads)
-    reps, t0 = 0, time.time()
-    while time.time() - t0 < seconds / 2:
-        po.matvec_0_colored(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, threads)
-        reps += 1
-    dt = (time.time() - t0) / reps
-    out["openmp"] = {"value": round(p.N / dt / 1e9, 5), "unit": "GDOF/s", "cores": threads,
-                     "kind": "port",
-                     "sample": f"8-colour OpenMP matvec_0 of the C oracle on the same slab, "
-                               f"{reps} applies in {seconds / 2:.0f} s",
-                     "ms_per_apply": round(dt * 1e3, 3)}
-    return out
+    host = host_cpu()
+    T = host["threads_used"]
+    out = {"host": host}
+    progress(f"cpu: {host['model']}, {host['physical_cores']} physical cores, {T} threads")
+    # --- 128x128x64: assembly, serial sample, OpenMP solve, checker solve
+    t0 = time.perf_counter()
+    bb, b, d = po.assemble(p2, threads=T)
+    t_asm = time.perf_counter() - t0
+    b, d, x0 = po.impose_mask(p2.mask, b, d, np.zeros(p2.N))
+    args = (bb, p2.iss, d, p2.nax, p2.nay, p2.naz, p2.mask)
+    x = np.random.default_rng(0).standard_normal(p2.N)
+    po.matvec_0(x, p2.iss, bb, p2.nax, p2.nay, p2.naz, p2.mask)
+    t0 = time.perf_counter()
+    for _ in range(3):
+        po.matvec_0(x, p2.iss, bb, p2.nax, p2.nay, p2.naz, p2.mask)
+    t_ser_apply = (time.perf_counter() - t0) / 3
+    t0 = time.perf_counter()
+    _, it_s, _, _ = po.dbcg_simple(b, x0, 1e-30, serial_iters, *args)
+    t_ser = time.perf_counter() - t0
+    with open(os.path.join(ROOT, "tests", "golden", "oracle_c2.json")) as f:
+        ser_its = json.load(f)["iterations"]  # the serial oracle's count to 1e-8
+    ms_it = 1e3 * t_ser / it_s
+    po.matvec_0_colored(x, p2.iss, bb, p2.nax, p2.nay, p2.naz, p2.mask, T)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        po.matvec_0_colored(x, p2.iss, bb, p2.nax, p2.nay, p2.naz, p2.mask, T)
+    t_omp_apply = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    x_omp, it_omp, err_omp, rc_omp = po.dbcg_simple_omp(b, x0, 1e-8, 20000, *args, T)
+    t_omp = time.perf_counter() - t0
+    progress(f"cpu 128x128x64: serial {ms_it:.1f} ms/iteration ({it_s} sampled), OpenMP "
+             f"{it_omp} iterations in {t_omp:.2f} s on {T} threads")
+    x_o, it_o, _, rc_o = po.dbcg_simple_omp(b, x0, 1e-12, 20000, *args, T)
+    N = p2.N
+    out.update({
+        "value": round(N / t_ser_apply / 1e9, 5), "unit": "GDOF/s", "cores": 1, "kind": "port",
+        "sample": f"the oracle's serial matvec_0 on stored element matrices at 128x128x64 "
+                  f"({N} nodes), 3 applies; its serial dbcg_simple sampled over {it_s} "
+                  f"iterations ({t_ser:.1f} s) and projected to its {ser_its} iterations to "
+                  f"rtol 1e-8; assembly (OpenMP, {T} threads) {t_asm:.1f} s, reported apart",
+        "ms_per_apply": round(1e3 * t_ser_apply, 3),
+        "ms_per_iteration": round(ms_it, 3),
+        "iterations_to_rtol": ser_its,
+        "time_to_rtol_s_projected": round(ms_it * ser_its / 1e3, 3),
+        "assembly_s": round(t_asm, 3),
+        "openmp": {"value": round(N / t_omp_apply / 1e9, 5), "unit": "GDOF/s", "cores": T,
+                   "kind": "port", "time_to_rtol_s": round(t_omp, 3), "iterations": it_omp,
+                   "converged": rc_omp == 0,
+                   "ms_per_iteration": round(1e3 * t_omp / max(1, it_omp), 3),
+                   "sample": f"the oracle's dbcg_simple with 8-colour OpenMP products and "
+                             f"fixed-order OpenMP reductions, whole solve to rtol 1e-8 from "
+                             f"x0 = 0 on {T} threads, OMP_PROC_BIND "
+                             f"{os.environ.get('OMP_PROC_BIND', 'unset')}"}})
+    # --- the checker: rel-L2 of every GPU solution against the oracle
+    rel = {}
+    n2 = np.linalg.norm(x_o)
+    for (cfg, name), xs in sols.items():
+        if cfg == "c2":
+            rel[f"c2/{name}"] = float(np.linalg.norm(xs - x_o) / n2)
+    rel["c2/cpu_openmp_bicg"] = float(np.linalg.norm(x_omp - x_o) / n2)
+    checker = {"oracle_c2": f"oracle FP64 BiCG to 1e-12 ({it_o} iterations, OpenMP), live"}
+    # legacy drop-in: kernel_dbcg_simple_ on these element matrices (the
+    # unchanged host's call), whole call and its phases
+    leg = {}
+    for sc in ("colored", "atomic"):
+        times = []
+        for _ in range(legacy_reps):
+            t0 = time.perf_counter()
+            xl, itl, errl = pkg.kernel_dbcg_simple(N, b, x0, 1e-8, 20000, bb, p2.iss, d, p2.nax,
+                                                   p2.nay, p2.naz, p2.mask, scatter=sc)
+            times.append((time.perf_counter() - t0, pkg._lib.legacy_last_times()))
+        wall, (up, so, down) = times[-1]
+        leg[sc] = {"iterations": itl, "err": errl, "whole_call_s": round(wall, 4),
+                   "upload_s": round(up, 4), "solve_s": round(so, 4),
+                   "download_s": round(down, 4),
+                   "rel_l2_vs_oracle": float(np.linalg.norm(xl - x_o) / n2)}
+        progress(f"legacy {sc}: {itl} iterations, call {wall * 1e3:.1f} ms (upload "
+                 f"{up * 1e3:.1f}, solve {so * 1e3:.1f}, download {down * 1e3:.1f})")
+    pkg._lib.load().vgpu_legacy_release()
+    del bb
+    # --- 32x32x16: serial and OpenMP to rtol, live checker
+    p1 = pkg.problem.config_problem("c1")
+    bb1, b1, d1 = po.assemble(p1, threads=T)
+    b1, d1, z1 = po.impose_mask(p1.mask, b1, d1, np.zeros(p1.N))
+    a1 = (bb1, p1.iss, d1, p1.nax, p1.nay, p1.naz, p1.mask)
+    t0 = time.perf_counter()
+    _, it1s, _, _ = po.dbcg_simple(b1, z1, 1e-8, 20000, *a1)
+    t1s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    _, it1o, _, _ = po.dbcg_simple_omp(b1, z1, 1e-8, 20000, *a1, T)
+    t1o = time.perf_counter() - t0
+    x1o, _, _, _ = po.dbcg_simple_omp(b1, z1, 1e-12, 20000, *a1, T)
+    out["config1"] = {"serial": {"time_to_rtol_s": round(t1s, 4), "iterations": it1s},
+                      "openmp": {"time_to_rtol_s": round(t1o, 4), "iterations": it1o,
+                                 "cores": T}}
+    for (cfg, name), xs in sols.items():
+        if cfg == "c1":
+            rel[f"c1/{name}"] = float(np.linalg.norm(xs - x1o) / np.linalg.norm(x1o))
+    checker["oracle_c1"] = "oracle FP64 BiCG to 1e-12 (OpenMP), live"
+    # --- sketches of the oracle's solutions of the large workloads
+    sk = po.load_sketches()
+    for (cfg, name), xs in sols.items():
+        if cfg in ("c3", "c5") and cfg in sk:
+            rel[f"{cfg}/{name}"] = po.sketch_rel_l2(xs, sk[cfg])
+        if cfg in ("c4a", "c4b") and cfg in sk:
+            est = [po.sketch_rel_l2(xm, e) for xm, e in zip(xs, sk[cfg]["members"])]
+            rel[f"{cfg}/{name}_max_over_members"] = max(est)
+    checker["sketched"] = ("256x256x64, the batches and the time stepping: estimates from 32 "
+                           "(batch members: 16) Gaussian projections of the oracle's FP64 "
+                           "solution to 1e-12 (tests/golden/oracle_sketch.json; +-27 % / "
+                           "+-41 % at 99 %)")
+    return out, {"rel_l2": rel, "checker": checker}, leg
 
 
 def measured_traffic(tuned, kernel="iteration_launch"):
@@ -118,7 +228,7 @@ def measured_traffic(tuned, kernel="iteration_launch"):
                       f"tile/zchunk {key}"}
 
 
-def config3(pkg):
+def config3(pkg, sols):
     """256x256x64 elements (4.29M nodes), per-element diffusivity log-uniform
     in [1e-4, 1e-2]: FP64 BiCG, FP64 BiCGSTAB and mixed precision (FP32 inner
     BiCGSTAB, rtol 1e-4, FP64 refinement) to rtol 1e-8; rel-L2 of the
@@ -143,6 +253,7 @@ def config3(pkg):
                                              precision=prec, inner_rtol=inner)
         ctx.set_method("bicg")
         xs[name] = ctx.d2h(x_dev, p.N)
+        sols[("c3", name)] = xs[name]
         out[name] = {"iterations": it, "seconds": round(sec, 4), "status": rc,
                      "true_rel_residual": ctx.true_residual(b, xs[name])}
         progress(f"config 3 {name}: {it} iterations, {sec * 1e3:.1f} ms, rc {rc}")
@@ -176,6 +287,34 @@ def config3(pkg):
              f"{roof['bicg_iteration_launch']['frac']}, BiCGSTAB "
              f"{roof['bicgstab_direction_launch']['frac']} / {roof['bicgstab_s_launch']['frac']}")
     ctx.close()
+    return out
+
+
+def config1(pkg, sols):
+    """32x32x16 elements (18,513 nodes): FP64 BiCG, FP64 BiCGSTAB and mixed
+    precision to rtol 1e-8 from x0 = 0, time-to-rtol with the data resident
+    (the second of two solves)."""
+    p = pkg.problem.config_problem("c1")
+    out = {"workload": "HEX8 unit box 32x32x16 elements, 18513 nodes, lambda 1e-3, V1 tau, "
+                       "rtol 1e-8 from x0=0"}
+    with pkg.Context.from_problem(p) as ctx:
+        b = ctx.assemble_rhs(p.q_in)
+        b_dev, x_dev = ctx.dev_alloc(8 * p.N), ctx.dev_alloc(8 * p.N)
+        ctx.h2d(b_dev, b)
+        for name, prec, method in (("fp64_bicg_owner", pkg.FP64, "bicg"),
+                                   ("fp64_bicgstab", pkg.FP64, "bicgstab"),
+                                   ("mixed", pkg.MIXED, "bicg")):
+            ctx.set_method(method)
+            for _ in range(2):
+                ctx.h2d(x_dev, np.zeros(p.N))
+                it, err, sec, rc = ctx.solve_dev(b_dev, x_dev, rtol=1e-8, itmax=20000,
+                                                 precision=prec)
+            ctx.set_method("bicg")
+            sols[("c1", name)] = ctx.d2h(x_dev, p.N)
+            out[name] = {"iterations": it, "seconds": round(sec, 5), "status": rc}
+    progress(f"config 1: fp64 {out['fp64_bicg_owner']['seconds'] * 1e3:.2f} ms, bicgstab "
+             f"{out['fp64_bicgstab']['seconds'] * 1e3:.2f} ms, mixed "
+             f"{out['mixed']['seconds'] * 1e3:.2f} ms")
     return out
 
 
@@ -215,7 +354,7 @@ def conv_layers(pkg, ctx, N):
     return out
 
 
-def learned_guess(pkg):
+def learned_guess(pkg, sols):
     """Implicit Euler (dt 0.01, 200 steps, helix source of seed 0) on the
     128x128x64 box, every step to rtol 1e-8, from the previous-step guess
     with BiCG (the plain method), and with linear, quadratic and cubic
@@ -231,7 +370,9 @@ def learned_guess(pkg):
              "extrap+jacobi@bicgstab", "extrap2+jacobi@bicgstab", "extrap2+jacobi4@bicgstab",
              "extrap3+jacobi8@bicgstab", "net+jacobi@bicgstab")
     progress(f"learned guess: {len(modes)} x 200 implicit-Euler steps")
-    res, _ = pkg.learned.experiment(pkg, params=params, modes=modes, tune=True)
+    res, _ = pkg.learned.experiment(pkg, params=params, modes=modes, tune=True, keep_final=True)
+    for m, xf in res.pop("finals").items():
+        sols[("c5", m)] = xf
     out = {"workload": "HEX8 unit box 128x128x64, implicit Euler dt=0.01, 200 steps, "
                        "helix source seed 0, rtol 1e-8 each step; mode guess[+jacobi[N]]"
                        "[@method] (N Jacobi sweeps, default 2), default method BiCG; owner tile "
@@ -265,19 +406,22 @@ def learned_guess(pkg):
     return out
 
 
-def batched_solves(pkg, tuned):
-    """Batched FP64 solves to rtol 1e-8 against the same systems solved one by
-    one: 32 right-hand sides (own diffusivity and source) on one 96x96x48 mesh,
-    and 16 boxes of n^3 elements, n uniform in 24..96."""
+def batched_solves(pkg, tuned, sols):
+    """Batched FP64 and mixed-precision solves to rtol 1e-8 against the same
+    systems solved one by one: 32 right-hand sides (own diffusivity and
+    source) on one 96x96x48 mesh, and 16 boxes of n^3 elements, n uniform in
+    24..96.  The owner tile and z chunk are the autotuner's for the
+    128x128x64 workload (a z chunk > 0, so that each batch member's inner
+    BiCGSTAB sums like its single-context solve)."""
     res = {}
     for name, gen in (("rhs32_96x96x48", lambda: pkg.problem.batch_rhs_problems()),
                       ("subdomains16_24to96", lambda: pkg.problem.batch_subdomain_problems())):
         progress(f"batched {name}: generating")
         probs = gen()
         ctxs = [pkg.Context.from_problem(q) for q in probs]
+        zc = tuned["zchunk"] if tuned and tuned["zchunk"] > 0 else 8
         for c in ctxs:
-            c.set_options("owner", 0 if tuned is None else tuned["zchunk"],
-                          0 if tuned is None else tuned["tile"])
+            c.set_options("owner", zc, 0 if tuned is None else tuned["tile"])
         bs = [c.assemble_rhs(q.q_in) for c, q in zip(ctxs, probs)]
         itmax = 20000
         with pkg.Batch(ctxs) as bt:
@@ -305,9 +449,40 @@ def batched_solves(pkg, tuned):
         progress(f"batched {name}: {t_batch * 1e3:.1f} ms vs {t_seq * 1e3:.1f} ms one by one")
         if not res[name]["all_converged"]:
             res[name]["with_bicgstab_fallback"] = batched_fallback(pkg, ctxs, bs)
+        res[name]["mixed"] = batched_mixed(pkg, ctxs, bs)
+        sols[("c4a" if name.startswith("rhs") else "c4b", "mixed")] = res[name]["mixed"].pop("xs")
         for c in ctxs:
             c.close()
     return res
+
+
+def batched_mixed(pkg, ctxs, bs, itmax=20000):
+    """Mixed precision (FP32 inner BiCGSTAB to 1e-4, FP64 refinement) to rtol
+    1e-8: the batch (one FP64 owner launch per refinement for all members,
+    the inner solves together in one pair of owner launches per iteration)
+    against each member's mixed solve on its own context."""
+    with pkg.Batch(ctxs) as bt:
+        bt.solve(bs, rtol=1e-8, itmax=itmax, precision=pkg.MIXED, raise_on_fail=False)
+        t0 = time.time()
+        xs, its, errs, st, rc = bt.solve(bs, rtol=1e-8, itmax=itmax, precision=pkg.MIXED,
+                                         raise_on_fail=False)
+        t_batch = time.time() - t0
+    for c, bb in zip(ctxs, bs):
+        c.solve(bb, rtol=1e-8, itmax=itmax, precision=pkg.MIXED, raise_on_fail=False)
+    t0 = time.time()
+    same = True
+    for m, (c, bb) in enumerate(zip(ctxs, bs)):
+        x, it, err, _ = c.solve(bb, rtol=1e-8, itmax=itmax, precision=pkg.MIXED,
+                                raise_on_fail=False)
+        same = same and it == its[m] and bool(np.array_equal(x, xs[m]))
+    t_seq = time.time() - t0
+    tres = max(c.true_residual(bb, x) for c, bb, x in zip(ctxs, bs, xs))
+    progress(f"  mixed: {t_batch * 1e3:.1f} ms batched vs {t_seq * 1e3:.1f} ms one by one, "
+             f"all converged {bool((st == 1).all())}, bitwise {same}")
+    return {"inner_iterations_min_max": [int(its.min()), int(its.max())],
+            "all_converged": bool((st == 1).all()), "true_rel_residual_max": float(tres),
+            "batched_s": round(t_batch, 4), "one_by_one_s": round(t_seq, 4),
+            "speedup": round(t_seq / t_batch, 3), "bitwise_equal_to_single": same, "xs": xs}
 
 
 def batched_fallback(pkg, ctxs, bs, cap=2000, itmax=20000):
@@ -360,7 +535,8 @@ def main():
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dual apply, the BiCG and BiCGSTAB owner launches "
                          "and the copy kernel (counter runs)")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline, the oracle checks and the legacy drop-in leg")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-stab", action="store_true", help="skip the FP64 BiCGSTAB solves")
     ap.add_argument("--no-scatter-solves", action="store_true",
@@ -426,6 +602,7 @@ def main():
             raise RuntimeError(f"solve failed rc={rc} it={it} err={err}")
         return it, err, sec
 
+    sols = {}  # (config, name) -> solution, for the oracle checker at the end
     progress("warmup")
     for _ in range(a.warmup):
         solve(pkg.FP64)
@@ -438,6 +615,7 @@ def main():
         iters.append(it)
     wall = time.time() - t0
     x64 = ctx.d2h(x_dev, N)
+    sols[("c2", "fp64_bicg_owner")] = x64
     ms_step = 1e3 * float(np.mean(secs))
     progress(f"fp64: {iters[-1]} iterations, {ms_step:.2f} ms to rtol")
 
@@ -451,6 +629,7 @@ def main():
                 msec.append(sec)
                 mits = it
             xm = ctx.d2h(x_dev, N)
+            sols[("c2", "mixed")] = xm
             rel_l2_mixed = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
             true_res_mixed = ctx.true_residual(b, xm)
             progress(f"mixed: {mits} inner iterations, {1e3 * np.mean(msec):.2f} ms")
@@ -475,6 +654,7 @@ def main():
                     it_c, _, sec = solve(pkg.FP64)
                     csec.append(sec)
                 xc = ctx.d2h(x_dev, N)
+                sols[("c2", f"fp64_bicg_{sc}")] = xc
                 scatter_solves[sc] = {
                     "iterations": it_c, "ms": round(1e3 * float(np.mean(csec)), 3),
                     "speedup_vs_owner": round(ms_step / (1e3 * float(np.mean(csec))), 3),
@@ -499,6 +679,7 @@ def main():
                 ssec.append(sec)
                 sits = it
             xs = ctx.d2h(x_dev, N)
+            sols[("c2", "fp64_bicgstab")] = xs
             rel_l2_stab = float(np.linalg.norm(xs - x64) / np.linalg.norm(x64))
             true_res_stab = ctx.true_residual(b, xs)
             progress(f"bicgstab: {sits} iterations, {1e3 * np.mean(ssec):.2f} ms")
@@ -517,6 +698,7 @@ def main():
     t_iter = ctx.time_kernel(3, "owner", pkg.FP64, reps)
     t_copy = ctx.time_kernel(2, "owner", pkg.FP64, reps)
     t_copy8 = ctx.time_kernel(4, "owner", pkg.FP64, 20)
+    t_copyh = ctx.time_kernel(7, "owner", pkg.FP64, 10)
     t_sdir = ctx.time_kernel(5, "owner", pkg.FP64, reps)
     t_ss = ctx.time_kernel(6, "owner", pkg.FP64, reps)
     try:
@@ -541,6 +723,7 @@ def main():
     achieved = iter_bytes / (t_iter * 1e-3) / 1e9
     copy_gbs = 16.0 * N / (t_copy * 1e-3) / 1e9
     copy8_gbs = 128.0 * N / (t_copy8 * 1e-3) / 1e9
+    copyh_gbs = 1024.0 * N / (t_copyh * 1e-3) / 1e9  # 64 N doubles in and out
     # BiCGSTAB owner launches: direction (R, V, P, T in; R, P, V out; x in
     # and out, d and RT at the owned nodes; mask; records) and S (R, V in;
     # T out; d and RT at the owned nodes; mask; records)
@@ -562,10 +745,17 @@ def main():
              f"({achieved:.0f} GB/s), copy {copy_gbs:.0f} GB/s (8 N: {copy8_gbs:.0f} GB/s), "
              f"BiCGSTAB launches {t_sdir * 1e3:.1f} / {t_ss * 1e3:.1f} us")
 
+    c1 = None
+    try:
+        c1 = config1(pkg, sols)
+    except Exception as exc:  # reported, never fatal for the line
+        c1 = {"status": f"failed: {exc}"}
+        progress(f"c1 {c1['status']}")
+
     batch = None
     if not a.no_batch:
         try:
-            batch = batched_solves(pkg, tuned)
+            batch = batched_solves(pkg, tuned, sols)
         except Exception as exc:  # reported, never fatal for the line
             batch = {"status": f"failed: {exc}"}
             progress(f"batched {batch['status']}")
@@ -573,7 +763,7 @@ def main():
     c3 = None
     if not a.no_c3:
         try:
-            c3 = config3(pkg)
+            c3 = config3(pkg, sols)
         except Exception as exc:  # reported, never fatal for the line
             c3 = {"status": f"failed: {exc}"}
             progress(f"c3 {c3['status']}")
@@ -581,15 +771,15 @@ def main():
     learned = None
     if not a.no_learned:
         try:
-            learned = learned_guess(pkg)
+            learned = learned_guess(pkg, sols)
         except Exception as exc:  # reported, never fatal for the line
             learned = {"status": f"failed: {exc}"}
             progress(f"learned {learned['status']}")
 
-    cpu = None
+    cpu, checks, legacy = None, None, None
     if not a.no_cpu:
-        progress("cpu baseline")
-        cpu = cpu_baseline(pkg)
+        progress("cpu baseline, oracle checks and the legacy drop-in")
+        cpu, checks, legacy = cpu_baseline_and_checks(pkg, p, sols)
 
     out = {
         "metric": METRIC, "value": round(gdofs, 3), "unit": "GDOF/s", "n_gpus": 1,
@@ -620,6 +810,11 @@ def main():
         "scatter_solves_fp64": scatter_solves,
         "copy_GBps": round(copy_gbs, 1),
         "copy_8N_GBps": round(copy8_gbs, 1),
+        "copy_hbm_GBps": round(copyh_gbs, 1),
+        "practical_hbm_roof": {"GBps": round(copyh_gbs, 1), "frac_of_peak":
+                               round(copyh_gbs / HBM_PEAK_GBS, 4),
+                               "note": "16-byte copy of 64 N doubles (0.55 GB each way), far "
+                                       "above the 256 MB last-level cache"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None, "kernel": "k_owner<double,double,dual,direction>",
@@ -632,6 +827,9 @@ def main():
                                 "kernel": "k_owner<double,double,dual>",
                                 "algorithmic_bytes_per_launch": dual_bytes},
         "cpu_baseline": cpu,
+        "rel_l2_vs_oracle": checks,
+        "legacy_drop_in": legacy,
+        "config1": c1,
         "autotune": tuned,
         "batched": batch,
         "learned_guess": learned,

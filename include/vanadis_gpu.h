@@ -88,6 +88,23 @@ void kernel_dbcg_simple_(int *N, double *B, double *X, double *TOL, int *ITMAX,
 int vgpu_legacy_set_scatter(int scatter);
 /* Release the legacy path's cached device buffers. */
 int vgpu_legacy_release(void);
+/* The colouring and per-colour element lists of the colour variant, as the
+ * legacy path builds them (generate_hex8_colors / build_color_element_lists,
+ * src_v2026_2_1/cudatest_coloring_gtx1060_optimized.cu:50-106): color[E],
+ * elems[E] (0-based element ids, stable by colour), offsets[9], counts[8].
+ * Host computation; no device work. */
+int vgpu_legacy_colors(int nax, int nay, int naz, int *color, int *elems, int *offsets,
+                       int *counts);
+/* The element lists the legacy path holds on the device after a call
+ * (read back), with their offsets and counts; capacity = length of elems
+ * (at least E of the last call).  VGPU_ERR_STATE before the first call. */
+int vgpu_legacy_get_color_lists(int64_t capacity, int *elems, int *offsets, int *counts);
+/* Wall times (seconds) of the last kernel_dbcg_simple_ call: host-to-device
+ * uploads of b, x, BB, ISS, DIAG and the mask (with the buffer set-up when
+ * the size changed), the BiCG solve on the device (first kernel to the
+ * convergence readback), and the download of X.  No counterpart in the
+ * reference, whose host times the whole call (VANADIS.FOR:807-812). */
+int vgpu_legacy_last_times(double *upload_s, double *solve_s, double *download_s);
 
 /* ------------------------------------------------------------------------ *
  * Context API: matrix-free operator, device-resident across time steps.
@@ -202,7 +219,10 @@ int vgpu_memcpy_d2h(vgpu_ctx *ctx, void *dst, const void *src_dev, int64_t bytes
  *       4 = device copy of 8 N doubles (the size of an iteration's working
  *           set; buffers allocated for the call),
  *       5 / 6 = the BiCGSTAB direction / S owner launch (FP64, owner scatter;
- *           the context's BiCGSTAB buffers, a status that never stops). */
+ *           the context's BiCGSTAB buffers, a status that never stops),
+ *       7 = device copy of 64 N doubles with 16-byte accesses (the practical
+ *           HBM roof: buffers far above the 256 MB last-level cache,
+ *           allocated for the call). */
 int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
                      int reps, double *avg_ms);
 

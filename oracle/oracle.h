@@ -74,6 +74,23 @@ int orc_dbcg_simple(int n, const double *b, double *x, double tol, int itmax,
                     const double *diag, int nax, int nay, int naz,
                     const int *mask);
 
+/* OpenMP CPU path (ompbase.c): mttvec_0 over the 8 parity colours; the
+ * reference BiCG with colour-parallel products and fixed-order OpenMP
+ * reductions (bitwise reproducible for a fixed thread count); the assembly
+ * with parallel element systems (bitwise orc_assemble). */
+int orc_mttvec_0_colored(double *xin, double *xout, const int *iss, const double *bb,
+                         int nax, int nay, int naz, const int *mask, int threads);
+int orc_dbcg_simple_omp(int n, const double *b, double *x, double tol, int itmax,
+                        int *iter, double *err, const double *bb, const int *iss,
+                        const double *diag, int nax, int nay, int naz, const int *mask,
+                        int threads);
+int orc_assemble_omp(int variant, int scheme, double dt, int nax, int nay, int naz,
+                     const double *xx, const double *yy, const double *zz,
+                     const int *iss, const int *kod, const double *lam, double P,
+                     const double *vel, const double *q_in, double alpha, double qs,
+                     const double *t_old, double *bb, double *fmv3, double *diag,
+                     int threads);
+
 double orc_integrate_mass_hex8(int nelem, const int *iss, const double *xx,
                                const double *yy, const double *zz,
                                const double *c);

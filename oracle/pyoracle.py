@@ -56,6 +56,18 @@ def lib():
         L.orc_dbcg_simple.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _pi, _pd, _dp,
                                       _ip, _dp, C.c_int, C.c_int, C.c_int, _ip]
         L.orc_dbcg_simple.restype = C.c_int
+        L.orc_mttvec_0_colored.argtypes = [_dp, _dp, _ip, _dp, C.c_int, C.c_int, C.c_int, _ip,
+                                           C.c_int]
+        L.orc_mttvec_0_colored.restype = C.c_int
+        L.orc_dbcg_simple_omp.argtypes = [C.c_int, _dp, _dp, C.c_double, C.c_int, _pi, _pd,
+                                          _dp, _ip, _dp, C.c_int, C.c_int, C.c_int, _ip,
+                                          C.c_int]
+        L.orc_dbcg_simple_omp.restype = C.c_int
+        L.orc_assemble_omp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_int, C.c_int,
+                                       C.c_int, _dp, _dp, _dp, _ip, _ip, _dp, C.c_double, _dp,
+                                       _dp, C.c_double, C.c_double, C.c_void_p, _dp, _dp, _dp,
+                                       C.c_int]
+        L.orc_assemble_omp.restype = C.c_int
         L.orc_integrate_mass_hex8.argtypes = [C.c_int, _ip, _dp, _dp, _dp, _dp]
         L.orc_integrate_mass_hex8.restype = C.c_double
         L.orc_diagonal_dominance_factor.argtypes = [C.c_int, C.c_int, _ip, _dp]
@@ -110,19 +122,26 @@ def el3d(variant, xe, ye, ze, lam3, P, v3, q_in=0.0, robin=0, alpha=0.0, qs=0.0)
     return H.reshape(8, 8), Cm.reshape(8, 8), F
 
 
-def assemble(p, t_old=None, scheme=None, dt=None):
-    """BB [64E], b [N], diag [N] of problem p (no mask imposition)."""
+def assemble(p, t_old=None, scheme=None, dt=None, threads=0):
+    """BB [64E], b [N], diag [N] of problem p (no mask imposition).
+    threads > 0: element systems in parallel (orc_assemble_omp, bitwise the
+    serial result)."""
     N, E = p.N, p.E
-    bb = np.zeros(64 * E)
+    bb = np.empty(64 * E)
     b = np.zeros(N)
     d = np.zeros(N)
     t = None if t_old is None else np.ascontiguousarray(t_old, np.float64)
-    lib().orc_assemble(p.variant, p.scheme if scheme is None else scheme,
-                       p.dt if dt is None else dt, p.nax, p.nay, p.naz,
-                       p.xx, p.yy, p.zz, p.iss, p.kod, np.ascontiguousarray(p.lam, np.float64),
-                       p.P, np.ascontiguousarray(p.vel, np.float64),
-                       np.ascontiguousarray(p.q_in, np.float64), p.alpha, p.qs,
-                       None if t is None else t.ctypes.data_as(C.c_void_p), bb, b, d)
+    args = (p.variant, p.scheme if scheme is None else scheme,
+            p.dt if dt is None else dt, p.nax, p.nay, p.naz,
+            p.xx, p.yy, p.zz, p.iss, p.kod, np.ascontiguousarray(p.lam, np.float64),
+            p.P, np.ascontiguousarray(p.vel, np.float64),
+            np.ascontiguousarray(p.q_in, np.float64), p.alpha, p.qs,
+            None if t is None else t.ctypes.data_as(C.c_void_p), bb, b, d)
+    if threads > 0:
+        if lib().orc_assemble_omp(*args, int(threads)) != 0:
+            raise MemoryError("orc_assemble_omp")
+    else:
+        lib().orc_assemble(*args)
     return bb, b, d
 
 
@@ -156,6 +175,30 @@ def matvec_0_colored(x, iss, bb, nax, nay, naz, mask, threads=1):
     return y
 
 
+def mttvec_0_colored(x, iss, bb, nax, nay, naz, mask, threads=1):
+    """mttvec_0 over 8 parity colours (ompbase.c orc_mttvec_0_colored)."""
+    xin = np.array(x, np.float64, copy=True)
+    y = np.zeros_like(xin)
+    if lib().orc_mttvec_0_colored(xin, y, iss, bb, nax, nay, naz, mask, int(threads)) != 0:
+        raise MemoryError("orc_mttvec_0_colored")
+    return y
+
+
+def dbcg_simple_omp(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask, threads):
+    """dbcg_simple with colour-parallel products and fixed-order OpenMP
+    reductions (ompbase.c); returns (x, iterations, err, rc)."""
+    x = np.array(x0, np.float64, copy=True)
+    it = C.c_int(0)
+    err = C.c_double(0.0)
+    rc = lib().orc_dbcg_simple_omp(len(b), np.ascontiguousarray(b, np.float64), x, tol,
+                                   itmax, C.byref(it), C.byref(err), bb, iss,
+                                   np.ascontiguousarray(diag, np.float64), nax, nay, naz,
+                                   mask, int(threads))
+    if rc < 0:
+        raise MemoryError("orc_dbcg_simple_omp")
+    return x, it.value, err.value, rc
+
+
 def dbcg_simple(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
     x = np.array(x0, np.float64, copy=True)
     it = C.c_int(0)
@@ -166,7 +209,7 @@ def dbcg_simple(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
     return x, it.value, err.value, rc
 
 
-def bicgstab_jacobi(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
+def bicgstab_jacobi(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask, threads=0):
     """Jacobi-BiCGSTAB, right preconditioned (van der Vorst 1992; the
     preconditioned BiCGSTAB of Barrett et al., Templates, 1994, sec. 2.3.8 with
     M = D), with the stopping rule of dbcg_simple (recursive ||r||/||b|| <= tol,
@@ -174,7 +217,10 @@ def bicgstab_jacobi(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask):
     Not in the reference: the checker of the library's BiCGSTAB method.
     Shadow residual r~ = r0.  Returns (x, iterations, err, status) with
     status 1 converged, 2 itmax, 3 breakdown."""
-    A = lambda v: matvec_0(v, iss, bb, nax, nay, naz, mask)
+    if threads > 0:
+        A = lambda v: matvec_0_colored(v, iss, bb, nax, nay, naz, mask, threads)  # noqa: E731
+    else:
+        A = lambda v: matvec_0(v, iss, bb, nax, nay, naz, mask)  # noqa: E731
     d = np.asarray(diag, np.float64)
     x = np.array(x0, np.float64, copy=True)
     x[np.asarray(mask) == 1] = 0.0
@@ -299,6 +345,19 @@ class RefLib:
         cma_n = skl[136 + 64 * 8: 136 + 64 * 9].reshape(8, 8, order="F").copy()
         return hma, cma_n, fmv
 
+    def build_bc_nodes(self, iss, kod, nax, nay, naz):
+        """The reference's Dirichlet mask (build_bc_nodes, VANADIS.FOR:2026-2117;
+        V3 omp_timing.for:1719-1811); set_sizes first."""
+        E = (nax - 1) * (nay - 1) * (naz - 1)
+        m = np.full(nax * nay * naz, -7, np.int32)
+        iss_c = np.ascontiguousarray(iss, np.int32)
+        kod_c = np.ascontiguousarray(kod, np.int32)
+        assert iss_c.size == 8 * E and kod_c.size == 6 * E
+        nx, ny, nz = C.c_int(nax), C.c_int(nay), C.c_int(naz)
+        self.L.build_bc_nodes_(iss_c.ctypes.data_as(_pi), kod_c.ctypes.data_as(_pi),
+                               m.ctypes.data_as(_pi), C.byref(nx), C.byref(ny), C.byref(nz))
+        return m
+
     def matvec_0(self, x, iss, bb, nax, nay, naz, mask, transpose=False):
         xin = np.array(x, np.float64, copy=True)
         y = np.zeros_like(xin)
@@ -379,3 +438,31 @@ def net_grads(nx, ny, nz, x1, x2, src, vx, vy, vz, target, mask, in_scale, out_s
         raise MemoryError("orc_net_grads")
     return loss.value, dw, db, out
 
+
+
+# ---- solution sketches of the large workloads (tests/golden/oracle_sketch.json)
+def sketch_projections(x, k, seed0):
+    """g_i . x for the seeded Gaussian vectors g_i = default_rng(seed0 + i)
+    .standard_normal(len(x)), i < k (tests/golden/make_oracle_sketch.py)."""
+    x = np.ascontiguousarray(x, np.float64)
+    return np.array([np.dot(np.random.default_rng(seed0 + i).standard_normal(x.size), x)
+                     for i in range(k)])
+
+
+def sketch_rel_l2(x, entry):
+    """Estimate of |x - x_o| / |x_o| for the oracle solution x_o that `entry`
+    sketches: sqrt(mean_i (g_i . x - g_i . x_o)^2) / |x_o|.  E (g . v)^2 =
+    |v|^2, so with k projections the squared estimate is |v|^2 chi2_k / k
+    (k = 32: within -27 % / +27 % of the true value with 99 % probability;
+    k = 16: -40 % / +41 %)."""
+    s = np.asarray(entry["sketch"], np.float64)
+    d = sketch_projections(x, len(s), entry["sketch_seed0"]) - s
+    return float(np.sqrt(np.mean(d * d)) / entry["x_norm2"])
+
+
+def load_sketches(path=None):
+    import json
+    if path is None:
+        path = os.path.join(os.path.dirname(HERE), "tests", "golden", "oracle_sketch.json")
+    with open(path) as f:
+        return json.load(f)

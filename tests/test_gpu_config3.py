@@ -1,9 +1,10 @@
 """256x256x64 elements with per-element diffusivity log-uniform in
 [1e-4, 1e-2] (4.29M nodes): the FP64 solve reaches rtol 1e-8 with a true
 residual at the solver's attainable accuracy, the operator agrees between
-scatter modes and precisions, and mixed precision (FP32 inner BiCG at rtol
-1e-4, FP64 refinement) reaches the FP64 solution within 1e-8 rel-L2 scaled by
-the conditioning the two stopping rules allow."""
+scatter modes and precisions, and the FP64 BiCG, FP64 BiCGSTAB and mixed
+precision (FP32 inner BiCGSTAB at rtol 1e-4, FP64 refinement) solutions are
+within 1e-8 rel-L2 of the CPU oracle's FP64 BiCG run to 1e-12 (estimated from
+the committed sketch of its solution, tests/golden/oracle_sketch.json)."""
 import numpy as np
 import pytest
 
@@ -29,7 +30,7 @@ def test_c3_operator_modes_agree(c3):
     assert np.linalg.norm(yf - ref) <= 2e-6 * np.linalg.norm(ref)
 
 
-def test_c3_fp64_and_mixed_solves(c3, pkg):
+def test_c3_fp64_and_mixed_solves(c3, pkg, po):
     p, ctx = c3
     b = ctx.assemble_rhs(p.q_in)
     x64, it64, err64, rc = ctx.solve(b, rtol=1e-8, itmax=50000, raise_on_fail=False)
@@ -44,6 +45,19 @@ def test_c3_fp64_and_mixed_solves(c3, pkg):
     print(f"  c3 mixed: rc {rcm} inner iterations {itm} true residual {rm:.2e} "
           f"rel-L2 vs fp64 {d:.2e}", flush=True)
     assert rcm == pkg.OK and rm <= 1e-8
-    # both iterates have true residual <= ~5e-8; their difference is bounded by
-    # cond * residual, measured ~1e-8 here
-    assert d <= 1e-6
+    # both against the oracle's FP64 BiCG run to 1e-12 (tests/golden/
+    # oracle_sketch.json: 32 Gaussian projections of its solution)
+    sk = po.load_sketches()["c3"]
+    e64, em = po.sketch_rel_l2(x64, sk), po.sketch_rel_l2(xm, sk)
+    print(f"  c3 rel-L2 vs oracle (sketch): fp64 {e64:.2e}, mixed {em:.2e}", flush=True)
+    assert e64 <= 1e-8 and em <= 1e-8
+    assert d <= 2e-8
+    ctx.set_method("bicgstab")
+    try:
+        xs, its, errs, rcs = ctx.solve(b, rtol=1e-8, itmax=50000, raise_on_fail=False)
+    finally:
+        ctx.set_method("bicg")
+    es = po.sketch_rel_l2(xs, sk)
+    print(f"  c3 bicgstab: rc {rcs} iterations {its} rel-L2 vs oracle (sketch) {es:.2e}",
+          flush=True)
+    assert rcs == pkg.OK and es <= 1e-8

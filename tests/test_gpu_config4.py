@@ -1,0 +1,98 @@
+"""Batched solves in mixed precision (the 32-right-hand-side and 16-subdomain
+workloads): one FP64 owner launch per refinement for every member's true
+residual, the members' inner FP32 BiCGSTAB solves in one pair of owner
+launches per iteration over the whole batch, captured in a hipGraph.
+
+Bars: every member reaches a true relative residual <= 1e-8; each member's
+solution and iteration count are bitwise those of the same mixed solve on its
+own context (same tile shape and z chunk); the solutions are within 1e-8
+rel-L2 of the CPU oracle's FP64 solve (run to 1e-12), live for small members
+and through the committed sketches (tests/golden/oracle_sketch.json) at the
+workload sizes."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+TILE, ZCHUNK = 1, 8  # 16x16 tiles, 8-plane chunks (any z chunk > 0 is bitwise-portable)
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def run_batch_and_singles(pkg, probs, label):
+    ctxs = [pkg.Context.from_problem(q) for q in probs]
+    try:
+        for c in ctxs:
+            c.set_options("owner", ZCHUNK, TILE)
+        bs = [c.assemble_rhs(q.q_in) for c, q in zip(ctxs, probs)]
+        log(f"{label}: batched mixed solve of {len(probs)} systems")
+        with pkg.Batch(ctxs) as bt:
+            xs, its, errs, st, rc = bt.solve(bs, rtol=1e-8, itmax=20000, precision=pkg.MIXED,
+                                             raise_on_fail=False)
+        log(f"{label}: rc {rc}, inner iterations {its.min()}..{its.max()}, "
+            f"status {sorted(set(st.tolist()))}")
+        singles = []
+        for m, (c, b) in enumerate(zip(ctxs, bs)):
+            x1, it1, e1, rc1 = c.solve(b, rtol=1e-8, itmax=20000, precision=pkg.MIXED,
+                                       raise_on_fail=False)
+            singles.append((x1, it1, e1, rc1))
+        tres = [c.true_residual(b, x) for c, b, x in zip(ctxs, bs, xs)]
+        return xs, its, errs, st, rc, singles, tres
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+def check(pkg, po, xs, its, errs, st, rc, singles, tres, sketch=None):
+    assert rc == pkg.OK and (st == 1).all(), st
+    assert max(errs) <= 1e-8 and max(tres) <= 1e-8, (max(errs), max(tres))
+    for m, (x1, it1, e1, rc1) in enumerate(singles):
+        assert rc1 == pkg.OK
+        assert it1 == its[m] and e1 == errs[m], (m, it1, its[m], e1, errs[m])
+        assert np.array_equal(x1, xs[m]), m
+    if sketch is not None:
+        est = [po.sketch_rel_l2(x, e) for x, e in zip(xs, sketch["members"])]
+        log(f"  rel-L2 vs the oracle (sketch estimates): max {max(est):.2e}")
+        assert max(est) <= 1e-8, est
+
+
+def test_mixed_batch_small_members_vs_oracle(pkg, po, gpu):
+    """Three uneven small systems with their own diffusivity field and
+    source: the batched mixed solution against the oracle's FP64 BiCG run to
+    1e-12 on the same inputs."""
+    probs = [pkg.problem.unit_box(10, 10, 10, vel_seed=1, lam=3e-3),
+             pkg.problem.unit_box(14, 9, 7, vel_seed=2, lam_range=(1e-4, 1e-2), lam_seed=5),
+             pkg.problem.unit_box(12, 12, 12, vel_seed=3, lam=1e-4, src_center=(0.5, 0.4, 0.6),
+                                  src_sigma=0.15)]
+    xs, its, errs, st, rc, singles, tres = run_batch_and_singles(pkg, probs, "small mixed")
+    check(pkg, po, xs, its, errs, st, rc, singles, tres)
+    for q, x in zip(probs, xs):
+        bb, b, d = po.assemble(q)
+        b, d, x0 = po.impose_mask(q.mask, b, d, np.zeros(q.N))
+        xo, ito, _, rco = po.dbcg_simple(b, x0, 1e-12, 20000, bb, q.iss, d, q.nax, q.nay,
+                                         q.naz, q.mask)
+        if rco != 0:  # BiCG stalled before 1e-12: the oracle's BiCGSTAB
+            xo, _, _, _ = po.bicgstab_jacobi(b, x0, 1e-13, 20000, bb, q.iss, d, q.nax, q.nay,
+                                             q.naz, q.mask)
+        r = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+        log(f"  N {q.N}: rel-L2 vs oracle {r:.2e}")
+        assert r <= 1e-8
+
+
+def test_mixed_batch_rhs32(pkg, po, gpu):
+    """32 right-hand sides with their own diffusivity and source on one
+    96x96x48 mesh."""
+    probs = pkg.problem.batch_rhs_problems()
+    sk = po.load_sketches().get("c4a")
+    out = run_batch_and_singles(pkg, probs, "rhs32 mixed")
+    check(pkg, po, *out, sketch=sk)
+
+
+def test_mixed_batch_subdomains16(pkg, po, gpu):
+    """16 independent boxes of n^3 elements, n drawn from 24..96."""
+    probs = pkg.problem.batch_subdomain_problems()
+    sk = po.load_sketches().get("c4b")
+    out = run_batch_and_singles(pkg, probs, "subdomains16 mixed")
+    check(pkg, po, *out, sketch=sk)

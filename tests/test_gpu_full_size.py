@@ -1,7 +1,11 @@
 """Parity at the benchmark size (128x128x64 elements, 1,081,665 nodes): the
 operator against the oracle's matvec_0/mttvec_0 on oracle-assembled element
-matrices, and the BiCG solution against the oracle's own solve (summary in
-tests/golden/oracle_c2.json, made by tests/golden/make_oracle_c2.py)."""
+matrices, and the solutions against the oracle's own FP64 solves: its BiCG to
+1e-8 (summary in tests/golden/oracle_c2.json, made by
+tests/golden/make_oracle_c2.py) and its BiCG to 1e-12, through 32 Gaussian
+projections (tests/golden/oracle_sketch.json, made by
+tests/golden/make_oracle_sketch.py): every solve to rtol 1e-8 lands within
+1e-8 rel-L2 of it."""
 import json
 import os
 
@@ -58,19 +62,25 @@ def test_solve_full_size_matches_oracle(c2, pkg, po, scatter):
     assert np.linalg.norm(x) == pytest.approx(g["x_norm2"], rel=1e-8)
     assert ctx.mass(x) == pytest.approx(g["mass"], rel=1e-8)
     assert x.sum() == pytest.approx(g["x_sum"], rel=1e-8)
+    est = po.sketch_rel_l2(x, po.load_sketches()["c2"])
+    print(f"  rel-L2 vs the oracle's 1e-12 solve (sketch estimate) {est:.2e}", flush=True)
+    assert est <= 1e-8
 
 
-def test_mixed_full_size(c2, pkg):
+def test_mixed_full_size(c2, pkg, po):
     p, bb, ctx = c2
     g = json.load(open(os.path.join(HERE, "golden", "oracle_c2.json")))
     b = ctx.assemble_rhs(p.q_in)
     x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=20000, precision=pkg.MIXED)
-    print(f"  mixed inner iterations {it}, true rel residual {err:.2e}", flush=True)
+    est = po.sketch_rel_l2(x, po.load_sketches()["c2"])
+    print(f"  mixed inner iterations {it}, true rel residual {err:.2e}, rel-L2 vs oracle "
+          f"(sketch) {est:.2e}", flush=True)
     assert err <= 1e-8 and ctx.true_residual(b, x) <= 1e-8
     assert np.linalg.norm(x) == pytest.approx(g["x_norm2"], rel=1e-8)
+    assert est <= 1e-8
 
 
-def test_bicgstab_full_size(c2, pkg):
+def test_bicgstab_full_size(c2, pkg, po):
     """FP64 BiCGSTAB (same preconditioner and stopping rule) reaches the
     oracle's BiCG solution at the benchmark size."""
     p, bb, ctx = c2
@@ -88,3 +98,24 @@ def test_bicgstab_full_size(c2, pkg):
     assert np.linalg.norm(x) == pytest.approx(g["x_norm2"], rel=1e-8)
     assert ctx.mass(x) == pytest.approx(g["mass"], rel=1e-8)
     assert x.sum() == pytest.approx(g["x_sum"], rel=1e-8)
+    assert po.sketch_rel_l2(x, po.load_sketches()["c2"]) <= 1e-8
+
+
+def test_colored_then_atomic_solves_one_context(c2, pkg):
+    """A COLORED FP64 BiCG solve followed by an ATOMIC one on the same
+    context (the drop and re-capture of the solver graph on a scatter change),
+    then the owner scatter again: each reaches rtol."""
+    p, bb, ctx = c2
+    b = ctx.assemble_rhs(p.q_in)
+    res = {}
+    try:
+        for sc in ("colored", "atomic", "owner"):
+            print(f"full-size solve sequence: {sc}", flush=True)
+            ctx.set_options(sc, 0, 0)
+            x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=5000)
+            res[sc] = x
+            assert rc == 0 and err <= 1e-8, (sc, rc, err)
+    finally:
+        ctx.set_options("owner", 0, 0)
+    for sc in ("colored", "atomic"):
+        assert np.linalg.norm(res[sc] - res["owner"]) <= 1e-8 * np.linalg.norm(res["owner"])

@@ -59,6 +59,18 @@ def test_coloring_bit_exact_and_disjoint(pkg, po, dims):
         assert list(cnt) == [300, 300, 270, 270, 270, 270, 243, 243]
 
 
+@pytest.mark.parametrize("dims", [(7, 20, 20), (33, 33, 17), (6, 5, 4), (2, 2, 2)])
+def test_library_colour_lists_bit_exact(pkg, po, dims):
+    """The colouring and per-colour element lists the library's legacy colour
+    path builds (vgpu_legacy_colors; host code, no device) equal the oracle's
+    generate_hex8_colors / build_color_element_lists bit for bit."""
+    nax, nay, naz = dims
+    col, el, off, cnt = pkg._lib.legacy_colors(nax, nay, naz)
+    ocol, oel, ooff, ocnt = po.colors(nax, nay, naz)
+    assert np.array_equal(col, ocol) and np.array_equal(el, oel)
+    assert np.array_equal(off, ooff) and np.array_equal(cnt, ocnt)
+
+
 def test_splitmix_determinism(pkg):
     a = pkg.problem.splitmix64(42, 10)
     b = pkg.problem.splitmix64(42, 20)[:10]

@@ -85,3 +85,24 @@ def test_el3d_unit_box_fields(pkg, po):
                             0, 0.0, 0.0)
         worst = max(worst, np.abs(Ho - Hr).max() / np.abs(Hr).max())
     assert worst <= 1e-15, worst
+
+
+@pytest.mark.parametrize("which,facemap", [("v1", 1), ("v3", 3)])
+@pytest.mark.parametrize("dims", [(7, 20, 20), (4, 6, 5), (9, 5, 7)])
+def test_dirichlet_mask_bitwise(pkg, po, which, facemap, dims):
+    """The host mirror's Dirichlet mask and the oracle's against the
+    reference's own build_bc_nodes, bit for bit: V1's face table
+    (VANADIS.FOR:2063-2117) and V3's, whose faces 5 and 6 take the swapped
+    node sets (omp_timing.for:1793-1807), so a missing or extra swap fails."""
+    R = ref(po, which)
+    nax, nay, naz = dims
+    E, N = (nax - 1) * (nay - 1) * (naz - 1), nax * nay * naz
+    R.set_sizes(E, N)
+    iss = po.w_el(nax, nay, naz)
+    kod = po.kod_el_dr(nax, nay, naz)
+    mr = R.build_bc_nodes(iss, kod, nax, nay, naz)
+    assert set(np.unique(mr)) <= {0, 1}
+    assert np.array_equal(po.build_bc_nodes(facemap, iss, kod, nax, nay, naz), mr)
+    assert np.array_equal(pkg.problem.build_bc_nodes(iss, kod, nax, nay, naz, facemap), mr)
+    other = 3 if facemap == 1 else 1
+    assert not np.array_equal(po.build_bc_nodes(other, iss, kod, nax, nay, naz), mr)

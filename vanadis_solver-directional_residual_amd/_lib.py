@@ -39,7 +39,8 @@ EXPORTED = (
     "vgpu_copy_dev", "vgpu_axpby_dev", "vgpu_residual_norm_dev", "vgpu_net_set_loss",
     "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
     "vgpu_set_method", "vgpu_axpbypcz_dev", "vgpu_net_time_layers",
-    "vgpu_net_set_zchunk", "vgpu_batch_set_fallback",
+    "vgpu_net_set_zchunk", "vgpu_batch_set_fallback", "vgpu_legacy_colors",
+    "vgpu_legacy_get_color_lists", "vgpu_legacy_last_times",
 )
 
 
@@ -78,6 +79,9 @@ def load():
                                       _pi, _pi, _ip]
     L.kernel_dbcg_simple_.restype = None
     L.vgpu_legacy_set_scatter.argtypes = [C.c_int]
+    L.vgpu_legacy_colors.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip]
+    L.vgpu_legacy_get_color_lists.argtypes = [C.c_int64, _ip, _ip, _ip]
+    L.vgpu_legacy_last_times.argtypes = [_pd, _pd, _pd]
     L.vgpu_create.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _vp, _ip,
                               C.POINTER(_vp)]
     L.vgpu_destroy.argtypes = [_vp]
@@ -175,6 +179,35 @@ def kernel_dbcg_simple(non, fmv3, tnw2, tol, il_i, bb, iss, diag, nax, nay, naz,
                           C.byref(err), _f64(bb), _i32(iss), _f64(diag), C.byref(nx),
                           C.byref(ny), C.byref(nz), _i32(bc_dirichlet))
     return x, it.value, err.value
+
+
+def legacy_colors(nax, nay, naz):
+    """The legacy path's colouring (color[E], elems[E], offsets[9],
+    counts[8]), computed by the library on the host."""
+    E = (nax - 1) * (nay - 1) * (naz - 1)
+    color = np.zeros(E, np.int32)
+    elems = np.zeros(E, np.int32)
+    off = np.zeros(9, np.int32)
+    cnt = np.zeros(8, np.int32)
+    check(load().vgpu_legacy_colors(nax, nay, naz, color, elems, off, cnt), "legacy colors")
+    return color, elems, off, cnt
+
+
+def legacy_last_times():
+    """(upload_s, solve_s, download_s) of the last kernel_dbcg_simple call."""
+    u, s_, d = C.c_double(), C.c_double(), C.c_double()
+    check(load().vgpu_legacy_last_times(C.byref(u), C.byref(s_), C.byref(d)), "legacy times")
+    return u.value, s_.value, d.value
+
+
+def legacy_device_color_lists(E):
+    """The element lists the legacy path holds on the device (read back after
+    a kernel_dbcg_simple call with the colour scatter), offsets, counts."""
+    elems = np.zeros(E, np.int32)
+    off = np.zeros(9, np.int32)
+    cnt = np.zeros(8, np.int32)
+    check(load().vgpu_legacy_get_color_lists(E, elems, off, cnt), "legacy color lists")
+    return elems, off, cnt
 
 
 class Context:

@@ -123,6 +123,26 @@ __global__ void k_copy(int64_t n, const double *a, double *b) {
     b[i] = a[i];
 }
 
+// Copy of n2 16-byte words, four per thread per step (the HBM-sized
+// calibration copy of vgpu_time_kernel, what = 7).
+__global__ __launch_bounds__(256) void k_copy_wide(int64_t n2, const double2 *__restrict__ a,
+                                                   double2 *__restrict__ b) {
+  const int64_t stride = (int64_t)gridDim.x * 1024;
+  for (int64_t i0 = blockIdx.x * (int64_t)1024 + threadIdx.x; i0 < n2; i0 += stride) {
+    double2 v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = i0 + 256 * k;
+      v[k] = i < n2 ? a[i] : double2{0.0, 0.0};
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const int64_t i = i0 + 256 * k;
+      if (i < n2) b[i] = v[k];
+    }
+  }
+}
+
 inline dim3 grid_for(int64_t n, int bs = 256) {
   int64_t g = (n + bs - 1) / bs;
   if (g > 4096) g = 4096;
@@ -1120,6 +1140,10 @@ struct vgpu_batch {
   float *fdiag = nullptr, *fb = nullptr, *fx = nullptr;
   BBicg<double> e64;
   BBicg<float> e32;
+  BStab<float> es32;  // inner solves of the mixed-precision batch
+  double *vr = nullptr;  // mixed precision: true residuals (batch layout)
+  double *bred = nullptr;  // mixed precision: per-member reduction partials
+  int bred_n = 0;
   uint64_t gen = 0;
   int fallback = VGPU_METHOD_BICG;  // VGPU_METHOD_BICGSTAB: finish unconverged members
   int fallback_itmax = 0;  // iterations allowed to each member's fallback solve
@@ -1127,7 +1151,8 @@ struct vgpu_batch {
   ~vgpu_batch() {
     e64.release();
     e32.release();
-    void *ptrs[] = {md64, md32, mask, diag, b, x, fdiag, fb, fx};
+    es32.release();
+    void *ptrs[] = {md64, md32, mask, diag, b, x, fdiag, fb, fx, vr, bred};
     for (void *q : ptrs)
       if (q) hipFree(q);
     if (stream) hipStreamDestroy(stream);
@@ -1197,6 +1222,201 @@ struct vgpu_batch {
                      errs, status);
   }
 };
+
+namespace {
+
+// Mixed precision over a batch: for every member the steps of
+// solve_mixed_dev (FP64 true residual r = b - A x and refinement
+// x += |r| dx, inner FP32 BiCGSTAB on r/|r| to inner_rtol, capped at
+// kInnerCap iterations and stopped on a 1e4-fold residual growth; a member
+// whose refinement fails to halve its true residual continues with FP64
+// passes of its context's method), with the FP64 operator applied to all
+// members in one owner launch and the inner solves of all members that need
+// one run together by the batched BiCGSTAB (two owner launches per
+// iteration over all members, captured in one hipGraph).  The vector
+// kernels run per member with the grids of the single-context solve, so with
+// the same owner tile shape and a common z chunk > 0 a member's result is
+// bitwise that of vgpu_solve(VGPU_MIXED) on its context.
+int batch_solve_mixed(vgpu_batch *bt, const double *b, double *x, double rtol, int itmax,
+                      double inner_rtol, int *iters, double *errs, int *status) {
+  const int nm = (int)bt->m.size();
+  hipStream_t s = bt->stream;
+  int rc = bt->es32.alloc(bt->seg);
+  if (rc) return rc;
+  if (!bt->vr) VGPU_HIP(hipMalloc(&bt->vr, bt->ntot * sizeof(double)));
+  std::vector<int> np(nm), pofs(nm + 1, 0);
+  for (int m = 0; m < nm; ++m) {
+    np[m] = nparts_for(bt->seg[m].n);
+    pofs[m + 1] = pofs[m] + np[m];
+  }
+  if (bt->bred_n < pofs[nm]) {
+    if (bt->bred) hipFree(bt->bred);
+    bt->bred = nullptr;
+    bt->bred_n = 0;
+    VGPU_HIP(hipMalloc(&bt->bred, pofs[nm] * sizeof(double)));
+    bt->bred_n = pofs[nm];
+  }
+  std::vector<double> hpart(pofs[nm]);
+  // per-member sums of the partials, each in order (as host_sum)
+  auto member_sums = [&](std::vector<double> &out) -> int {
+    VGPU_HIP(hipMemcpyAsync(hpart.data(), bt->bred, pofs[nm] * sizeof(double),
+                            hipMemcpyDeviceToHost, s));
+    VGPU_HIP(hipStreamSynchronize(s));
+    for (int m = 0; m < nm; ++m) {
+      double t = 0;
+      for (int i = 0; i < np[m]; ++i) t += hpart[pofs[m] + i];
+      out[m] = t;
+    }
+    return VGPU_OK;
+  };
+  const ActiveSet &all = bt->es32.all.view;
+  std::vector<double> bn(nm), rn(nm), rel(nm, 0.0), prev_rel(nm, 0.0), sums(nm);
+  std::vector<int> total(nm, 0), st_inner(nm, 0), fp64(nm, 0), running(nm, 1), run(nm);
+  for (int m = 0; m < nm; ++m) {
+    const Segment &sg = bt->seg[m];
+    status[m] = 2;
+    hipLaunchKernelGGL((k_dot<double>), dim3(np[m]), dim3(VB), 0, s, sg.n, nullptr,
+                       b + sg.off, b + sg.off, bt->bred + pofs[m]);
+    hipLaunchKernelGGL(k_lincomb, grid_for(sg.n), dim3(256), 0, s, sg.n, bt->mask + sg.off,
+                       1.0, x + sg.off, 0.0, nullptr, x + sg.off);
+  }
+  VGPU_HIP(hipGetLastError());
+  rc = member_sums(sums);
+  if (rc) return rc;
+  for (int m = 0; m < nm; ++m) bn[m] = std::sqrt(sums[m]);
+  // true residuals r = b - A x of every member (in vr) and their norms
+  auto true_rel = [&]() -> int {
+    int r = launch_owner_batch<double, double>(bt->md64, all, bt->tile, bt->zchunk, bt->robin,
+                                               nullptr, x, bt->vr, nullptr, nullptr, nullptr,
+                                               s);
+    if (r) return r;
+    for (int m = 0; m < nm; ++m) {
+      const Segment &sg = bt->seg[m];
+      hipLaunchKernelGGL(k_resid, dim3(np[m]), dim3(VB), 0, s, sg.n, b + sg.off,
+                         bt->vr + sg.off, bt->bred + pofs[m]);
+    }
+    VGPU_HIP(hipGetLastError());
+    r = member_sums(sums);
+    if (r) return r;
+    for (int m = 0; m < nm; ++m) {
+      rn[m] = std::sqrt(sums[m]);
+      rel[m] = rn[m] / bn[m];
+    }
+    return VGPU_OK;
+  };
+  BStabSingleOp<float> single = [bt, &all](const float *xi, float *y, hipStream_t st) {
+    return launch_owner_batch<float, float>(bt->md32, all, bt->tile, bt->zchunk, bt->robin,
+                                            nullptr, xi, y, nullptr, nullptr, nullptr, st);
+  };
+  BStabOwnerOp<float> owner = [bt, &all](int mode, const StabData<float> &sd, const float *r,
+                                         float *y, hipStream_t st) {
+    return launch_owner_stab_batch<float, float>(bt->md32, all, bt->tile, bt->zchunk,
+                                                 bt->robin, mode, sd, r, y, st);
+  };
+  const uint64_t key = (bt->gen << 8) | (uint64_t)(bt->zchunk % 64) | (1ull << 7);
+  std::vector<int> it_in(nm), stin(nm);
+  std::vector<double> e_in(nm);
+  for (int outer = 0; outer < 64; ++outer) {
+    bool any = false;
+    for (int m = 0; m < nm; ++m) any = any || running[m];
+    if (!any) break;
+    rc = true_rel();
+    if (rc) return rc;
+    for (int m = 0; m < nm; ++m) {
+      if (!running[m]) continue;
+      if (rel[m] <= rtol) {
+        status[m] = 1;
+        running[m] = 0;
+      } else if (!(rel[m] == rel[m])) {
+        status[m] = 3;
+        running[m] = 0;
+      } else if (total[m] >= itmax) {
+        status[m] = 2;
+        running[m] = 0;
+      } else {
+        if (!fp64[m] && outer > 0 && (rel[m] > 0.5 * prev_rel[m] || st_inner[m] >= 3))
+          fp64[m] = 1;
+        prev_rel[m] = rel[m];
+      }
+    }
+    // members in FP64 mode: one pass of the context's method from the
+    // current iterate (their b and x are the batch vectors' ranges)
+    bool need_rel = false;
+    for (int m = 0; m < nm; ++m) {
+      if (!running[m] || !fp64[m]) continue;
+      vgpu_ctx *c = bt->m[m];
+      const Segment &sg = bt->seg[m];
+      VGPU_HIP(hipStreamSynchronize(s));
+      VGPU_HIP(hipSetDevice(c->device));
+      int it64 = 0, st64 = 0;
+      double e64 = 0;
+      rc = solve_fp64_dev(c, b + sg.off, x + sg.off, rtol, itmax - total[m], &it64, &e64,
+                          &st64);
+      if (rc) return rc;
+      VGPU_HIP(hipStreamSynchronize(c->stream));
+      VGPU_HIP(hipSetDevice(bt->device));
+      total[m] += it64;
+      if (st64 >= 3) {  // breakdown or divergence of the FP64 pass
+        status[m] = st64;
+        running[m] = 0;
+        need_rel = true;
+      }
+    }
+    if (need_rel) {  // the final true residual of those members
+      std::vector<double> keep = rel;
+      rc = true_rel();
+      if (rc) return rc;
+      for (int m = 0; m < nm; ++m)
+        if (running[m] || status[m] < 3) rel[m] = keep[m];
+    }
+    // inner FP32 solves of the other running members, together
+    bool any_inner = false;
+    for (int m = 0; m < nm; ++m) {
+      run[m] = running[m] && !fp64[m];
+      any_inner = any_inner || run[m];
+    }
+    if (!any_inner) continue;
+    for (int m = 0; m < nm; ++m) {
+      if (!run[m]) continue;
+      const Segment &sg = bt->seg[m];
+      hipLaunchKernelGGL(k_scale_to_f32, grid_for(sg.n), dim3(256), 0, s, sg.n,
+                         bt->vr + sg.off, 1.0 / rn[m], bt->fb + sg.off, bt->fx + sg.off);
+    }
+    VGPU_HIP(hipGetLastError());
+    rc = bt->es32.solve(key, single, owner, bt->mask, bt->fb, bt->fx, bt->fdiag, inner_rtol,
+                        kInnerCap, 1e4, run.data(), s, it_in.data(), e_in.data(),
+                        stin.data());
+    if (rc) return rc;
+    for (int m = 0; m < nm; ++m) {
+      if (!run[m]) continue;
+      total[m] += it_in[m];
+      st_inner[m] = stin[m];
+      if (stin[m] >= 3) continue;  // next pass falls back to FP64
+      const Segment &sg = bt->seg[m];
+      hipLaunchKernelGGL(k_refine, grid_for(sg.n), dim3(256), 0, s, sg.n, x + sg.off, rn[m],
+                         bt->fx + sg.off);
+    }
+    VGPU_HIP(hipGetLastError());
+  }
+  // pass limit or itmax: report the final iterate's true residual
+  bool any2 = false;
+  for (int m = 0; m < nm; ++m) any2 = any2 || status[m] == 2;
+  if (any2) {
+    std::vector<double> keep = rel;
+    rc = true_rel();
+    if (rc) return rc;
+    for (int m = 0; m < nm; ++m)
+      if (status[m] != 2) rel[m] = keep[m];
+  }
+  VGPU_HIP(hipStreamSynchronize(s));
+  for (int m = 0; m < nm; ++m) {
+    iters[m] = total[m];
+    errs[m] = rel[m];
+  }
+  return VGPU_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -1287,8 +1507,8 @@ int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double 
     set_error("vgpu_batch_solve: bad arguments");
     return VGPU_ERR_ARG;
   }
-  if (precision != VGPU_FP64 && precision != VGPU_FP32) {
-    set_error("vgpu_batch_solve: precision must be FP64 or FP32");
+  if (precision != VGPU_FP64 && precision != VGPU_FP32 && precision != VGPU_MIXED) {
+    set_error("vgpu_batch_solve: precision must be FP64, FP32 or MIXED");
     return VGPU_ERR_ARG;
   }
   VGPU_HIP(hipSetDevice(bt->device));
@@ -1300,6 +1520,8 @@ int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double 
   if (precision == VGPU_FP64) {
     rc = bt->solve_tv<double>(bt->e64, bt->md64, b, x, bt->diag, rtol, itmax, iters,
                               errs, status);
+  } else if (precision == VGPU_MIXED) {
+    rc = batch_solve_mixed(bt, b, x, rtol, itmax, 1e-4, iters, errs, status);
   } else {
     hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
                        bt->stream, N, b, bt->fb);
@@ -1622,8 +1844,8 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
 
 int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                      int reps, double *avg_ms) {
-  if (!c || !avg_ms || reps < 1 || what < 0 || what > 6) return VGPU_ERR_ARG;
-  if (what != 2 && what != 4 && !c->have_coeffs) return VGPU_ERR_STATE;
+  if (!c || !avg_ms || reps < 1 || what < 0 || what > 7) return VGPU_ERR_ARG;
+  if (what != 2 && what != 4 && what != 7 && !c->have_coeffs) return VGPU_ERR_STATE;
   if (what == 3 && scatter != VGPU_SCATTER_OWNER) return VGPU_ERR_ARG;
   if (what >= 5 && (scatter != VGPU_SCATTER_OWNER || precision != VGPU_FP64 ||
                     !owner_fits(c->g)))
@@ -1746,10 +1968,22 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   };
   // what 4: a copy of 8 N doubles, the size class of a BiCG iteration's
   // working set (buffers held for this call only)
+  // what 7: a copy of 64 N doubles (0.55 GB each way at 128x128x64, well
+  // above the 256 MB last-level cache): the practical HBM roof
   double *big = nullptr;
   if (what == 4) VGPU_HIP(hipMalloc(&big, 16 * N * sizeof(double)));
+  if (what == 7) VGPU_HIP(hipMalloc(&big, 128 * N * sizeof(double)));
   auto launch = [&]() -> int {
     if (what == 3) return launch_dir();
+    if (what == 7) {
+      const int64_t n2 = 32 * N;  // 64 N doubles
+      const int64_t g = std::min<int64_t>(32768, (n2 + 1023) / 1024);
+      hipLaunchKernelGGL(k_copy_wide, dim3((unsigned)g), dim3(256), 0, c->stream, n2,
+                         reinterpret_cast<const double2 *>(big),
+                         reinterpret_cast<double2 *>(big + 64 * N));
+      VGPU_HIP(hipGetLastError());
+      return VGPU_OK;
+    }
     if (what >= 5) return launch_stab();
     if (what == 4) {
       hipLaunchKernelGGL(k_copy, grid_for(8 * N, 256), dim3(256), 0, c->stream, 8 * N, big,

@@ -68,14 +68,17 @@ __device__ __forceinline__ void block_partial(double v, double *out,
 }
 
 // r = b - r (r holds A x0), rr = r, z = r/d, zz = rr/d, x zeroed on the mask;
-// partials of r.r, z.rr and b.b (cudatest.cu:464-491, VANADIS.FOR:991-1000).
+// partials of r.r, z.rr and b.b (cudatest.cu:464-491, VANADIS.FOR:991-1000);
+// the solve's status starts at zero (written here, on the device, so a solve
+// queues no host-to-device copy).
 template <typename TV>
 __global__ __launch_bounds__(VB) void k_init(int64_t n, const uint8_t *mask,
                                              const TV *b, TV *x, TV *r, TV *rr,
                                              TV *z, TV *zz, const TV *diag,
                                              double *pr, double *pb,
-                                             double *pbn) {
+                                             double *pbn, SolveStatus *st) {
   __shared__ double red[VB / 64];
+  if (blockIdx.x == 0 && threadIdx.x == 0) *st = SolveStatus{};
   double sr = 0, sb = 0, sn = 0;
   for (int64_t i = blockIdx.x * (int64_t)VB + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * VB) {
@@ -338,10 +341,8 @@ struct Bicg {
     int rc = single(x, r, s);  // r = A x0 (x0 masked on load)
     if (rc) return rc;
     hipLaunchKernelGGL((k_init<TV>), dim3(nb), dim3(VB), 0, s, n, mask, b, x,
-                       r, rr, z, zz, diag, pr, pb, pbn);
+                       r, rr, z, zz, diag, pr, pb, pbn, st);
     VGPU_HIP(hipGetLastError());
-    SolveStatus s0{};
-    VGPU_HIP(hipMemcpyAsync(st, &s0, sizeof s0, hipMemcpyHostToDevice, s));
     const bool same = key_valid && key_op == op_key && key_b == b &&
                       key_x == x && key_diag == diag && key_tol == tol &&
                       key_itmax == itmax && key_zero == zero_out &&

@@ -236,6 +236,16 @@ struct StabData {
   int early = 0;
   double jomega = 0;  // Jacobi sweep: damping (d, rt hold the diagonal and b)
 };
+// The same two launches over the members of a batch (md, as as
+// launch_owner_batch): sd.st points at the members' statuses (indexed by
+// member), sd.pin / sd.pout / sd.prt0 at the batch-wide partial arrays, of
+// which member m reads and writes its own ranges (owner partials from
+// seg.u0, vector-workgroup partials from seg.vb0); sd.nin / sd.nb0 are not
+// used.  A member whose status is done at the launch skips its marches.
+template <typename T, typename TIO>
+int launch_owner_stab_batch(const Member<T> *md, const ActiveSet &as, int tile, int zchunk,
+                            bool robin, int mode, const StabData<TIO> &sd, const TIO *r,
+                            TIO *y1, hipStream_t s);
 // One damped Jacobi sweep on the owner kernel: xn = x + omega (b - A x) / d
 // (0 on the mask), the update fused into the operator's output stage; xn must
 // not alias x (other tiles read x at their halo).

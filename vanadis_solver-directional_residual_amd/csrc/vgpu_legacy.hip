@@ -9,6 +9,7 @@
 // (generate_hex8_colors / build_color_element_lists, lines 50-106); the BiCG
 // is the graph-captured engine of vgpu_bicg.hpp; device buffers persist
 // between calls of the same size instead of 19 allocations per call.
+#include <chrono>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -94,6 +95,29 @@ __global__ void k_mask_from_int(int64_t n, const int *bc, uint8_t *m) {
     m[i] = bc[i] == 1 ? 1 : 0;
 }
 
+// Colouring and compact per-colour element lists of the colour variant
+// (generate_hex8_colors / build_color_element_lists, coloring.cu:50-106):
+// colour (i&1) + 2(j&1) + 4(k&1) of element (i, j, k), numbered
+// e = i + (nax-1) j + (nax-1)(nay-1) k; the lists are a stable counting sort
+// by colour.  color and elems hold E entries, coff 9, ccnt 8.
+void colour_lists(int nax, int nay, int naz, int *color, int *elems, int coff[9],
+                  int ccnt[8]) {
+  const int64_t E = (int64_t)(nax - 1) * (nay - 1) * (naz - 1);
+  for (int k = 0; k < naz - 1; ++k)
+    for (int j = 0; j < nay - 1; ++j)
+      for (int i = 0; i < nax - 1; ++i) {
+        const int64_t e = i + (int64_t)(nax - 1) * j + (int64_t)(nax - 1) * (nay - 1) * k;
+        color[e] = (i & 1) + 2 * (j & 1) + 4 * (k & 1);
+      }
+  int cursor[8];
+  for (int c = 0; c < 8; ++c) ccnt[c] = 0;
+  for (int64_t e = 0; e < E; ++e) ccnt[color[e]]++;
+  coff[0] = 0;
+  for (int c = 0; c < 8; ++c) coff[c + 1] = coff[c] + ccnt[c];
+  for (int c = 0; c < 8; ++c) cursor[c] = coff[c];
+  for (int64_t e = 0; e < E; ++e) elems[cursor[color[e]]++] = (int)e;
+}
+
 struct Legacy {
   int N = 0;
   int64_t E = 0;
@@ -105,6 +129,9 @@ struct Legacy {
   int coff[9] = {0}, ccnt[8] = {0};
   Bicg<double> bicg;
   int scatter = -1;  // -1 = not chosen yet (env / default)
+  // wall times of the last call: uploads (incl. the buffer set-up of a new
+  // size), the solve, the solution download
+  double t_upload = 0, t_solve = 0, t_download = 0;
 
   void release() {
     bicg.release();
@@ -146,22 +173,8 @@ struct Legacy {
     VGPU_HIP(hipMalloc(&bc, N * sizeof(int)));
     VGPU_HIP(hipMalloc(&mask, N));
     VGPU_HIP(hipMalloc(&celems, E * sizeof(int)));
-    // colouring and compact lists: generate_hex8_colors /
-    // build_color_element_lists (coloring.cu:50-106)
     std::vector<int> color(E), elems(E);
-    for (int k = 0; k < naz - 1; ++k)
-      for (int j = 0; j < nay - 1; ++j)
-        for (int i = 0; i < nax - 1; ++i) {
-          const int64_t e = i + (int64_t)(nax - 1) * j + (int64_t)(nax - 1) * (nay - 1) * k;
-          color[e] = (i & 1) + 2 * (j & 1) + 4 * (k & 1);
-        }
-    int cursor[8];
-    for (int c = 0; c < 8; ++c) ccnt[c] = 0;
-    for (int64_t e = 0; e < E; ++e) ccnt[color[e]]++;
-    coff[0] = 0;
-    for (int c = 0; c < 8; ++c) coff[c + 1] = coff[c] + ccnt[c];
-    for (int c = 0; c < 8; ++c) cursor[c] = coff[c];
-    for (int64_t e = 0; e < E; ++e) elems[cursor[color[e]]++] = (int)e;
+    colour_lists(nax, nay, naz, color.data(), elems.data(), coff, ccnt);
     VGPU_HIP(hipMemcpy(celems, elems.data(), E * sizeof(int), hipMemcpyHostToDevice));
     return bicg.alloc(N, 512);
   }
@@ -216,6 +229,8 @@ int legacy_solve(int N, const double *B, double *X, double tol, int itmax,
     return VGPU_ERR_ARG;
   }
   Legacy &L = legacy();
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
   int rc = L.ensure(N, nax, nay, naz);
   if (rc) return rc;
   if (L.scatter < 0) L.scatter = legacy_scatter_default();
@@ -229,6 +244,8 @@ int legacy_solve(int N, const double *B, double *X, double tol, int itmax,
   hipLaunchKernelGGL(k_mask_from_int, dim3(256), dim3(256), 0, s, (int64_t)N,
                      L.bc, L.mask);
   VGPU_HIP(hipGetLastError());
+  VGPU_HIP(hipStreamSynchronize(s));
+  const auto t1 = clk::now();
   DualOp<double> dual = [&L](const double *p, double *z, const double *pp,
                              double *zz, double *pa, int *na, const int *done,
                              hipStream_t st) {
@@ -250,8 +267,13 @@ int legacy_solve(int N, const double *B, double *X, double tol, int itmax,
   rc = L.bicg.solve((uint64_t)(100 + L.scatter), dual, single, L.mask, L.b, L.x,
                     L.diag, tol, itmax, true, s, iter, err, &status);
   if (rc) return rc;
+  const auto t2 = clk::now();
   VGPU_HIP(hipMemcpyAsync(X, L.x, N * sizeof(double), hipMemcpyDeviceToHost, s));
   VGPU_HIP(hipStreamSynchronize(s));
+  const auto t3 = clk::now();
+  L.t_upload = std::chrono::duration<double>(t1 - t0).count();
+  L.t_solve = std::chrono::duration<double>(t2 - t1).count();
+  L.t_download = std::chrono::duration<double>(t3 - t2).count();
   return VGPU_OK;
 }
 
@@ -289,6 +311,41 @@ int vgpu_legacy_set_scatter(int scatter) {
   }
   vgpu::legacy().scatter = scatter;
   vgpu::legacy().bicg.drop_graph();
+  return VGPU_OK;
+}
+
+int vgpu_legacy_colors(int nax, int nay, int naz, int *color, int *elems, int *offsets,
+                       int *counts) {
+  if (nax < 2 || nay < 2 || naz < 2 || !color || !elems || !offsets || !counts ||
+      (int64_t)(nax - 1) * (nay - 1) * (naz - 1) > INT32_MAX) {
+    vgpu::set_error("vgpu_legacy_colors: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  vgpu::colour_lists(nax, nay, naz, color, elems, offsets, counts);
+  return VGPU_OK;
+}
+
+int vgpu_legacy_get_color_lists(int64_t capacity, int *elems, int *offsets, int *counts) {
+  vgpu::Legacy &L = vgpu::legacy();
+  if (!L.celems || L.E == 0) {
+    vgpu::set_error("vgpu_legacy_get_color_lists: no legacy call has run");
+    return VGPU_ERR_STATE;
+  }
+  if (!elems || !offsets || !counts || capacity < L.E) {
+    vgpu::set_error("vgpu_legacy_get_color_lists: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  VGPU_HIP(hipMemcpy(elems, L.celems, L.E * sizeof(int), hipMemcpyDeviceToHost));
+  for (int c = 0; c < 9; ++c) offsets[c] = L.coff[c];
+  for (int c = 0; c < 8; ++c) counts[c] = L.ccnt[c];
+  return VGPU_OK;
+}
+
+int vgpu_legacy_last_times(double *upload_s, double *solve_s, double *download_s) {
+  const vgpu::Legacy &L = vgpu::legacy();
+  if (upload_s) *upload_s = L.t_upload;
+  if (solve_s) *solve_s = L.t_solve;
+  if (download_s) *download_s = L.t_download;
   return VGPU_OK;
 }
 

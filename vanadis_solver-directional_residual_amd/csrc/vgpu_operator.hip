@@ -350,7 +350,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
   // otherwise one partial per (tile, plane) unit.
   auto march = [&](int m, int tile, int k0, int k1, int64_t gidx) {
     if (!DIR && !STAB && st && st[m].done) return;
-    if (EARLY && sd.st->done) return;
+    if (EARLY && (BATCH ? sd.st[m].done : sd.st->done)) return;
     const OpData<T> op = BATCH ? md[m].op : m0.op;
     const int64_t off = BATCH ? md[m].seg.off : m0.seg.off;
     const int64_t ub = BATCH ? md[m].seg.u0 : m0.seg.u0;
@@ -659,7 +659,10 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       // block and broadcast.  Each launch writes only fields the other one
       // reads (kStabDir: rho_next, omega, pad, pend; kStabS: rho, alpha,
       // iter, err, done), so no block sees a field change that it reads.
-      SolveStatus *stm = sd.st;
+      // BATCH: member m's status, and its ranges of the partial arrays: the
+      // owner partials at its units / groups (from seg.u0), the vector
+      // workgroup partials (|b|^2, r0 . r0) at its workgroups (from seg.vb0)
+      SolveStatus *stm = BATCH ? sd.st + m : sd.st;
       if (tid == 0) {
         sflag[0] = stm->done;
         sflag[1] = SA ? stm->iter : stm->pad;  // index j of this iteration
@@ -669,7 +672,21 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       }
       const bool writer = tile == 0 && k0 == 0 && tid == 0;  // one block
       double sums[4];
-      blk_sum4(sd.pin, sd.nin, sums);  // its barriers also publish sflag
+      if (BATCH) {
+        const Segment sg = md[m].seg;
+        const int no = (int)(zc > 0 ? sg.ng : sg.nu);
+        const double *pin[4];
+        int nin[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const bool vec = SB && i == 2;  // |b|^2: vector-workgroup partials
+          pin[i] = sd.pin[i] ? sd.pin[i] + (vec ? (int64_t)sg.vb0 : sg.u0) : nullptr;
+          nin[i] = sd.pin[i] ? (vec ? sg.nb : no) : 0;
+        }
+        blk_sum4(pin, nin, sums);  // its barriers also publish sflag
+      } else {
+        blk_sum4(sd.pin, sd.nin, sums);
+      }
       const int pdone = sflag[0], j = sflag[1], ppend = sflag[2];
       const double rho_a = sstab[0], al_a = sstab[1];
       __syncthreads();  // sflag is rewritten by the block's next march
@@ -681,8 +698,9 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
         if (first) {
           // rho_0 = r0 . r0 from the initial partials
           double s0[4];
-          const double *const p0[4] = {sd.prt0, sd.prt0, sd.prt0, sd.prt0};
-          const int n0[4] = {sd.nb0, 0, 0, 0};
+          const double *p0b = BATCH ? sd.prt0 + md[m].seg.vb0 : sd.prt0;
+          const double *const p0[4] = {p0b, p0b, p0b, p0b};
+          const int n0[4] = {BATCH ? md[m].seg.nb : sd.nb0, 0, 0, 0};
           blk_sum4(p0, n0, s0);
           rho = s0[0];
         } else {
@@ -1399,6 +1417,32 @@ int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<
     return launch_owner_any<T, TIO, false, kStabDir>(L, op.tile, op.ra != 0.0, s);
   return launch_owner_any<T, TIO, false, kStabS>(L, op.tile, op.ra != 0.0, s);
 }
+
+template <typename T, typename TIO>
+int launch_owner_stab_batch(const Member<T> *md, const ActiveSet &as, int tile, int zchunk,
+                            bool robin, int mode, const StabData<TIO> &sd, const TIO *r,
+                            TIO *y1, hipStream_t s) {
+  if (mode != kStabDir && mode != kStabS) {
+    set_error("BiCGSTAB launch: unknown mode");
+    return VGPU_ERR_ARG;
+  }
+  OwnerLaunch<T, TIO> L{};
+  L.md = md;
+  L.as = as;
+  L.utot = as.nu;
+  L.gtot = as.ng;
+  L.zchunk = zchunk;
+  L.x1 = r;
+  L.y1 = y1;
+  L.sd = sd;
+  if (mode == kStabDir) return launch_owner_any<T, TIO, true, kStabDir>(L, tile, robin, s);
+  return launch_owner_any<T, TIO, true, kStabS>(L, tile, robin, s);
+}
+
+template int launch_owner_stab_batch<float, float>(const Member<float> *, const ActiveSet &,
+                                                   int, int, bool, int,
+                                                   const StabData<float> &, const float *,
+                                                   float *, hipStream_t);
 
 template <typename T, typename TIO>
 int launch_owner_jacobi(const OpData<T> &op, int zchunk, double omega, const TIO *x, const TIO *b,

@@ -137,15 +137,18 @@ class Stepper:
         return it, t1 - t0, fallback
 
 
-def run(stepper, seed, steps, mode="previous", net=None, states=None, rtol=1e-8):
+def run(stepper, seed, steps, mode="previous", net=None, states=None, rtol=1e-8,
+        true_residuals=False):
     """Run `steps` steps from x_0 = 0.  states: device buffer of steps+1
     vectors that receives x_0..x_steps (else two rotating work vectors are
-    used).  Returns per-step iterations, seconds and fallbacks."""
+    used).  Returns per-step iterations, seconds and fallbacks (and with
+    true_residuals the true relative residual |b - A x_n| / |b| of every
+    step's solution, computed after the step's timer)."""
     centres, sigma = helix(seed, steps)
     own = states is None
     R = 5  # ring: x_{n-1} .. x_{n-4} and the output
     buf = stepper.alloc_states(R) if own else states
-    its, secs, fbs = [], [], []
+    its, secs, fbs, trs = [], [], [], []
     for n in range(1, steps + 1):
         if own:
             x1, x2, x3, x4, out = (stepper.ptr(buf, (n - 1) % R), stepper.ptr(buf, (n - 2) % R),
@@ -163,11 +166,16 @@ def run(stepper, seed, steps, mode="previous", net=None, states=None, rtol=1e-8)
             x4 = None
         it, sec, fb = stepper.step(centres[n - 1], sigma, x1, x2, out, mode, net, rtol, x3=x3,
                                    x4=x4)
+        if true_residuals:
+            trs.append(stepper.ctx.residual_norm_dev(stepper.b, out))
         its.append(it)
         secs.append(sec)
         fbs.append(fb)
-    return {"iterations": its, "seconds": secs, "fallbacks": int(sum(fbs)),
-            "last": stepper.ptr(buf, steps % R if own else steps)}
+    out = {"iterations": its, "seconds": secs, "fallbacks": int(sum(fbs)),
+           "last": stepper.ptr(buf, steps % R if own else steps)}
+    if true_residuals:
+        out["true_residuals"] = trs
+    return out
 
 
 def init_params(seed=0):
@@ -239,13 +247,15 @@ def train(net, stepper, seeds, trajectories, steps, epochs=3, lr=5e-4, shuffle_s
 def experiment(pkg, shape=(128, 128, 64), dt=0.01, steps=200, train_seeds=range(1, 11),
                eval_seed=0, epochs=3, lr=5e-4, extrap=1.0, log=None, params=None,
                loss_kind=1, smoothing=2,
-               modes=("previous", "extrap", "extrap+jacobi", "net", "net+jacobi"), tune=False):
+               modes=("previous", "extrap", "extrap+jacobi", "net", "net+jacobi"), tune=False,
+               keep_final=False):
     """Generate training trajectories with the plain method, train the network
     (unless params are given) on the initial-residual loss (loss_kind 1) or the
     increment loss (0), then run the evaluation seed with the three guesses.
     tune: run the autotuner once on the implicit-Euler system first (the
-    owner tile and z chunk every solve then uses).  Returns a dict of
-    results and the trained parameters."""
+    owner tile and z chunk every solve then uses).  keep_final: the last
+    state of every mode (host array) in res["finals"][mode].  Returns a dict
+    of results and the trained parameters."""
     p = pkg.problem.unit_box(*shape, vel_seed=0, lam=1e-3, scheme=_lib.SCHEME_EULER, dt=dt)
     ctx = pkg.Context.from_problem(p)
     st = Stepper(ctx, p, dt)
@@ -283,9 +293,13 @@ def experiment(pkg, shape=(128, 128, 64), dt=0.01, steps=200, train_seeds=range(
                        float(params["extrap"]))
         net.set_smoothing(int(params["smoothing"]))
     res["training_samples"] = len(list(train_seeds)) * steps
+    if keep_final:
+        res["finals"] = {}
     for mode in modes:
         r = run(st, eval_seed, steps, mode, net)
-        r.pop("last")
+        last = r.pop("last")
+        if keep_final:
+            res["finals"][mode] = ctx.d2h(last, p.N)
         res[mode] = {"iterations": int(sum(r["iterations"])),
                      "seconds": round(float(sum(r["seconds"])), 4),
                      "fallbacks": r["fallbacks"],
