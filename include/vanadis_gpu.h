@@ -173,7 +173,8 @@ int vgpu_assemble_rhs(vgpu_ctx *ctx, const double *q_in, const double *t_old,
  * FP32 BiCGSTAB to inner_rtol, 0 = 1e-4, on r/|r| with FP64 true-residual
  * refinement x += |r| dx to rtol; FP32 BiCG inside for systems too large for
  * the owner launches; if a refinement step fails to halve the true residual
- * the solve finishes in FP64 with the context's method).  iters = total
+ * the solve finishes with FP64 BiCGSTAB passes, or the context's method for
+ * systems too large for the owner launches).  iters = total
  * Krylov iterations, err = final relative residual (recursive for FP64 and
  * FP32, true for MIXED). */
 int vgpu_solve(vgpu_ctx *ctx, int precision, const double *b, double *x,
@@ -276,9 +277,16 @@ int vgpu_jacobi_dev(vgpu_ctx *ctx, const double *b, double *x, int sweeps, doubl
 typedef struct vgpu_batch vgpu_batch;
 int vgpu_batch_create(int nm, vgpu_ctx *const *ctxs, vgpu_batch **out);
 int vgpu_batch_destroy(vgpu_batch *batch);
-/* b[m], x[m]: host vectors of system m (x in/out).  precision VGPU_FP64 or
- * VGPU_FP32.  iters/errs/status [nm]: status 1 converged, 2 itmax,
- * 3 breakdown.  Returns VGPU_OK when every system converged. */
+/* b[m], x[m]: host vectors of system m (x in/out).  precision VGPU_FP64,
+ * VGPU_FP32 or VGPU_MIXED.  VGPU_MIXED: each member as vgpu_solve's
+ * VGPU_MIXED (inner rtol 1e-4), with one FP64 owner launch per refinement for
+ * every member's true residual and the members' inner FP32 BiCGSTAB solves
+ * run together (two owner launches per iteration over the whole batch, one
+ * hipGraph); with the same tile shape and a z chunk > 0 on every member,
+ * each member's result is bitwise vgpu_solve's on its context; iters = inner
+ * + FP64 iterations, errs = true relative residuals.  iters/errs/status
+ * [nm]: status 1 converged, 2 itmax, 3 breakdown.  Returns VGPU_OK when
+ * every system converged. */
 int vgpu_batch_solve(vgpu_batch *batch, int precision, const double *const *b,
                      double *const *x, double rtol, int itmax, int *iters,
                      double *errs, int *status);

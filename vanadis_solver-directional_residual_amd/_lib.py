@@ -325,6 +325,10 @@ class Context:
         check(load().vgpu_dev_alloc(self.h, int(nbytes), C.byref(p)), "dev_alloc")
         return p
 
+    def dev_free(self, ptr):
+        """Free a buffer of dev_alloc (also freed with the context)."""
+        check(load().vgpu_dev_free(self.h, ptr), "dev_free")
+
     def h2d(self, dst, arr):
         a = np.ascontiguousarray(arr)
         check(load().vgpu_memcpy_h2d(self.h, dst, a.ctypes.data_as(_vp), a.nbytes), "h2d")

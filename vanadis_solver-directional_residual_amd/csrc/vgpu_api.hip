@@ -416,6 +416,17 @@ int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
 // Inner iterations allowed per refinement step of the mixed solve.
 constexpr int kInnerCap = 1000;
 
+// FP64 pass of a mixed solve whose FP32 inner solves stopped reducing the
+// true residual: BiCGSTAB on the owner launches (the method of the inner
+// solves; plain BiCG can stagnate on the systems where the FP32 inner solves
+// give up), the context's method for systems too large for them.
+int mixed_fp64_pass(vgpu_ctx *c, const double *b, double *x, double rtol, int itmax,
+                    int *iters, double *err, int *status) {
+  if (owner_fits(c->g))
+    return solve_stab<double>(c, c->stab64, b, x, c->diag, rtol, itmax, iters, err, status);
+  return solve_fp64_dev(c, b, x, rtol, itmax, iters, err, status);
+}
+
 // Mixed precision: FP64 true residual + refinement, FP32 inner BiCGSTAB on
 // the owner launches (FP32 BiCG loses its recurrences across the large
 // residual peaks of advection-dominated systems; BiCGSTAB's residual stays
@@ -486,14 +497,14 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
       break;
     }
     // the FP32 inner solve stopped reducing the true residual: continue with
-    // FP64 BiCG passes from the current iterate (the fallback of the mixed
-    // mode); each pass restarts from the true residual checked above
+    // FP64 passes from the current iterate (the fallback of the mixed mode);
+    // each pass restarts from the true residual checked above
     if (!fp64 && outer > 0 && (rel > 0.5 * prev_rel || st_inner >= 3)) fp64 = true;
     prev_rel = rel;
     if (fp64) {
       int it64 = 0, st64 = 0;
       double e64 = 0;
-      rc = solve_fp64_dev(c, b, x, rtol, itmax - total, &it64, &e64, &st64);
+      rc = mixed_fp64_pass(c, b, x, rtol, itmax - total, &it64, &e64, &st64);
       if (rc) return rc;
       total += it64;
       if (st64 >= 3) {  // breakdown or divergence of the FP64 pass
@@ -1230,7 +1241,7 @@ namespace {
 // x += |r| dx, inner FP32 BiCGSTAB on r/|r| to inner_rtol, capped at
 // kInnerCap iterations and stopped on a 1e4-fold residual growth; a member
 // whose refinement fails to halve its true residual continues with FP64
-// passes of its context's method), with the FP64 operator applied to all
+// BiCGSTAB passes, mixed_fp64_pass), with the FP64 operator applied to all
 // members in one owner launch and the inner solves of all members that need
 // one run together by the batched BiCGSTAB (two owner launches per
 // iteration over all members, captured in one hipGraph).  The vector
@@ -1339,8 +1350,8 @@ int batch_solve_mixed(vgpu_batch *bt, const double *b, double *x, double rtol, i
         prev_rel[m] = rel[m];
       }
     }
-    // members in FP64 mode: one pass of the context's method from the
-    // current iterate (their b and x are the batch vectors' ranges)
+    // members in FP64 mode: one FP64 pass from the current iterate (their b
+    // and x are the batch vectors' ranges)
     bool need_rel = false;
     for (int m = 0; m < nm; ++m) {
       if (!running[m] || !fp64[m]) continue;
@@ -1350,8 +1361,8 @@ int batch_solve_mixed(vgpu_batch *bt, const double *b, double *x, double rtol, i
       VGPU_HIP(hipSetDevice(c->device));
       int it64 = 0, st64 = 0;
       double e64 = 0;
-      rc = solve_fp64_dev(c, b + sg.off, x + sg.off, rtol, itmax - total[m], &it64, &e64,
-                          &st64);
+      rc = mixed_fp64_pass(c, b + sg.off, x + sg.off, rtol, itmax - total[m], &it64, &e64,
+                           &st64);
       if (rc) return rc;
       VGPU_HIP(hipStreamSynchronize(c->stream));
       VGPU_HIP(hipSetDevice(bt->device));
