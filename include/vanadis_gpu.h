@@ -106,6 +106,29 @@ int vgpu_legacy_get_color_lists(int64_t capacity, int *elems, int *offsets, int 
  * reference, whose host times the whole call (VANADIS.FOR:807-812). */
 int vgpu_legacy_last_times(double *upload_s, double *solve_s, double *download_s);
 
+/* Allocation-free form of the legacy drop-in.  No counterpart in the
+ * reference (whose solver makes 19 device allocations per call,
+ * cudatest.cu:408-450): for callers that cannot allocate device memory at
+ * solve time (tight time-stepping or serving loops, harnesses that forbid
+ * allocation).  Same algorithm, kernels and launch shapes as
+ * kernel_dbcg_simple_ (bitwise the same iterates with the colored scatter),
+ * but every array is a device pointer (b, x in/out, bb [64 E], iss [8 E]
+ * column-major 1-based, diag, bc [N] int), scalars by value, iter / err on
+ * the host, scatter VGPU_SCATTER_COLORED or VGPU_SCATTER_ATOMIC, stream a
+ * hipStream_t (NULL: the null stream); the call returns when the solve is
+ * done.  The BiCG work vectors, Dirichlet flags, colour lists, partials and
+ * status live in one fixed device array of the library (about 14 MB per
+ * device, held until the process ends): N may not exceed
+ * vgpu_noalloc_max_nodes() (2^18; larger N: VGPU_ERR_ARG).  Calls on one
+ * device share that array: calls that overlap on one device (other host
+ * threads or streams) race.  Returns VGPU_OK (converged), VGPU_NOT_CONVERGED
+ * (itmax), VGPU_BREAKDOWN or VGPU_ERR_ARG / VGPU_ERR_HIP. */
+int kernel_dbcg_simple_noalloc(int n, const double *b_dev, double *x_dev, double tol,
+                               int itmax, int *iter, double *err, const double *bb_dev,
+                               const int *iss_dev, const double *diag_dev, int nax, int nay,
+                               int naz, const int *bc_dev, int scatter, void *stream);
+int vgpu_noalloc_max_nodes(int64_t *max_nodes);
+
 /* ------------------------------------------------------------------------ *
  * Context API: matrix-free operator, device-resident across time steps.
  * ------------------------------------------------------------------------ */

@@ -40,7 +40,8 @@ EXPORTED = (
     "vgpu_jacobi_dev", "vgpu_net_set_smoothing", "vgpu_ddf", "vgpu_write_tfile",
     "vgpu_set_method", "vgpu_axpbypcz_dev", "vgpu_net_time_layers",
     "vgpu_net_set_zchunk", "vgpu_batch_set_fallback", "vgpu_legacy_colors",
-    "vgpu_legacy_get_color_lists", "vgpu_legacy_last_times",
+    "vgpu_legacy_get_color_lists", "vgpu_legacy_last_times", "kernel_dbcg_simple_noalloc",
+    "vgpu_noalloc_max_nodes",
 )
 
 
@@ -82,6 +83,10 @@ def load():
     L.vgpu_legacy_colors.argtypes = [C.c_int, C.c_int, C.c_int, _ip, _ip, _ip, _ip]
     L.vgpu_legacy_get_color_lists.argtypes = [C.c_int64, _ip, _ip, _ip]
     L.vgpu_legacy_last_times.argtypes = [_pd, _pd, _pd]
+    L.kernel_dbcg_simple_noalloc.argtypes = [C.c_int, _vp, _vp, C.c_double, C.c_int, _pi, _pd,
+                                             _vp, _vp, _vp, C.c_int, C.c_int, C.c_int, _vp,
+                                             C.c_int, _vp]
+    L.vgpu_noalloc_max_nodes.argtypes = [C.POINTER(C.c_int64)]
     L.vgpu_create.argtypes = [C.c_int, C.c_int, C.c_int, _dp, _dp, _dp, _vp, _ip,
                               C.POINTER(_vp)]
     L.vgpu_destroy.argtypes = [_vp]

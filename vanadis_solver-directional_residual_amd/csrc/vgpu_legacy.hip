@@ -14,6 +14,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <mutex>
+#include <string>
 #include <vector>
 
 #include "vgpu_bicg.hpp"
@@ -277,10 +279,208 @@ int legacy_solve(int N, const double *B, double *X, double tol, int itmax,
   return VGPU_OK;
 }
 
+// ---------------------------------------------------------------------------
+// The allocation-free form of the drop-in (kernel_dbcg_simple_noalloc): the
+// caller's device buffers and, for everything else the solve needs, one
+// file-scope device array of fixed size.  Layout of the array (each part
+// 256-byte aligned): r, rr, p, pp, z, zz [kNaMaxNodes doubles each], the
+// Dirichlet flags [kNaMaxNodes bytes], the colour lists [kNaMaxNodes ints],
+// the reduction partials (pr 2 x 256, pb 2 x 256, pbn 256, pa 256 doubles)
+// and the solve status.
+// ---------------------------------------------------------------------------
+constexpr int64_t kNaMaxNodes = int64_t(1) << 18;
+constexpr int64_t kNaAlign = 256;
+constexpr int64_t na_round(int64_t b) { return (b + kNaAlign - 1) / kNaAlign * kNaAlign; }
+constexpr int64_t kNaVecBytes = 6 * kNaMaxNodes * (int64_t)sizeof(double);
+constexpr int64_t kNaMaskOff = kNaVecBytes;
+constexpr int64_t kNaElemOff = kNaMaskOff + na_round(kNaMaxNodes);
+constexpr int64_t kNaPartOff = kNaElemOff + na_round(kNaMaxNodes * (int64_t)sizeof(int));
+constexpr int64_t kNaPartCount = 2 * 256 + 2 * 256 + 256 + 256;
+constexpr int64_t kNaStatOff = kNaPartOff + na_round(kNaPartCount * (int64_t)sizeof(double));
+constexpr int64_t kNaScratchBytes = kNaStatOff + na_round(sizeof(SolveStatus));
+
+// Element lists of the colour variant, computed on the device in closed form:
+// element e = i + ex j + ex ey k has colour c = (i&1) + 2(j&1) + 4(k&1)
+// (generate_hex8_colors, coloring.cu:50-74); the stable counting sort of
+// build_color_element_lists (coloring.cu:82-106) puts it at
+// offset_c + (k>>1) ni nj + (j>>1) ni + (i>>1), with ni (nj) the number of
+// i in [0, ex) (j in [0, ey)) of its parity.  off[c] as colour_counts.
+__global__ void k_colour_lists(int ex, int ey, int ez, int o0, int o1, int o2, int o3, int o4,
+                               int o5, int o6, int o7, int *elems) {
+  const int off[8] = {o0, o1, o2, o3, o4, o5, o6, o7};
+  const int64_t E = (int64_t)ex * ey * ez;
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < E;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int i = (int)(e % ex), j = (int)((e / ex) % ey), k = (int)(e / ((int64_t)ex * ey));
+    const int pi = i & 1, pj = j & 1, pk = k & 1;
+    const int ni = (ex - pi + 1) >> 1, nj = (ey - pj + 1) >> 1;
+    const int64_t rank = (int64_t)(k >> 1) * ni * nj + (int64_t)(j >> 1) * ni + (i >> 1);
+    elems[off[pi + 2 * pj + 4 * pk] + rank] = (int)e;
+  }
+}
+
+// Colour counts and offsets of an ex x ey x ez element grid (the same as
+// colour_lists computes by counting).
+void colour_counts(int ex, int ey, int ez, int coff[9], int ccnt[8]) {
+  coff[0] = 0;
+  for (int c = 0; c < 8; ++c) {
+    const int pi = c & 1, pj = (c >> 1) & 1, pk = (c >> 2) & 1;
+    ccnt[c] = ((ex - pi + 1) >> 1) * ((ey - pj + 1) >> 1) * ((ez - pk + 1) >> 1);
+    coff[c + 1] = coff[c] + ccnt[c];
+  }
+}
+
+}  // namespace
+
+// The scratch array of kernel_dbcg_simple_noalloc (one copy per device,
+// resident for the life of the process).
+static __device__ __align__(256) unsigned char scratch[kNaScratchBytes];
+
+namespace {
+
+std::mutex g_na_mu;
+void *g_na_ptr[64] = {};  // device address of `scratch` per device
+
+// The device address of `scratch` on the current device, looked up once per
+// device (host code must never pass the array's name to a kernel: on the host
+// it is a stand-in, not a device address).
+int noalloc_scratch(void **out) {
+  int dev = 0;
+  VGPU_HIP(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    set_error("kernel_dbcg_simple_noalloc: device index above 63");
+    return VGPU_ERR_ARG;
+  }
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    void *p = nullptr;
+    VGPU_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_na_ptr[dev] = p;
+  }
+  *out = g_na_ptr[dev];
+  return VGPU_OK;
+}
+
+int legacy_solve_noalloc(int N, const double *b, double *x, double tol, int itmax, int *iter,
+                         double *err, const double *bb, const int *iss, const double *diag,
+                         int nax, int nay, int naz, const int *bc, int scatter,
+                         hipStream_t s) {
+  if (N != nax * nay * naz || nax < 2 || nay < 2 || naz < 2 || itmax < 0 || !(tol >= 0) ||
+      !b || !x || !iter || !err || !bb || !iss || !diag || !bc) {
+    set_error("kernel_dbcg_simple_noalloc: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  if ((int64_t)N > kNaMaxNodes) {
+    set_error("kernel_dbcg_simple_noalloc: " + std::to_string(N) + " nodes exceed the " +
+              std::to_string(kNaMaxNodes) + " of its fixed scratch array");
+    return VGPU_ERR_ARG;
+  }
+  if (scatter != VGPU_SCATTER_COLORED && scatter != VGPU_SCATTER_ATOMIC) {
+    set_error("kernel_dbcg_simple_noalloc: scatter must be colored or atomic");
+    return VGPU_ERR_ARG;
+  }
+  void *base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc) return rc;
+  unsigned char *sc = static_cast<unsigned char *>(base);
+  const int64_t n = N, E = (int64_t)(nax - 1) * (nay - 1) * (naz - 1);
+  double *v = reinterpret_cast<double *>(sc);
+  double *r = v, *rr = v + kNaMaxNodes, *p = v + 2 * kNaMaxNodes, *pp = v + 3 * kNaMaxNodes,
+         *z = v + 4 * kNaMaxNodes, *zz = v + 5 * kNaMaxNodes;
+  uint8_t *mask = sc + kNaMaskOff;
+  int *elems = reinterpret_cast<int *>(sc + kNaElemOff);
+  double *part = reinterpret_cast<double *>(sc + kNaPartOff);
+  double *pr = part, *pb = part + 512, *pbn = part + 1024, *pa = part + 1280;
+  SolveStatus *st = reinterpret_cast<SolveStatus *>(sc + kNaStatOff);
+  // the legacy path's launch shapes (bitwise the same iterates)
+  const int nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));
+  int coff[9], ccnt[8];
+  colour_counts(nax - 1, nay - 1, naz - 1, coff, ccnt);
+  hipLaunchKernelGGL(k_mask_from_int, dim3(256), dim3(256), 0, s, n, bc, mask);
+  if (scatter == VGPU_SCATTER_COLORED)
+    hipLaunchKernelGGL(k_colour_lists, dim3((unsigned)std::min<int64_t>(4096, (E + 255) / 256)),
+                       dim3(256), 0, s, nax - 1, nay - 1, naz - 1, coff[0], coff[1], coff[2],
+                       coff[3], coff[4], coff[5], coff[6], coff[7], elems);
+  VGPU_HIP(hipGetLastError());
+  // y1 (+)= A x1, y2 (+)= A^T x2 (y2 == nullptr: forward only) on stored BB
+  auto dual = [&](const double *x1, double *y1, const double *x2, double *y2,
+                  const int *done) {
+    const bool two = x2 != nullptr;
+    if (scatter == VGPU_SCATTER_ATOMIC) {
+      const unsigned g = (unsigned)std::min<int64_t>(4096, (E + 255) / 256);
+      if (two)
+        hipLaunchKernelGGL((k_bb<true, true>), dim3(g), dim3(256), 0, s, E, iss, bb, mask,
+                           nullptr, E, x1, y1, x2, y2, done);
+      else
+        hipLaunchKernelGGL((k_bb<true, false>), dim3(g), dim3(256), 0, s, E, iss, bb, mask,
+                           nullptr, E, x1, y1, x2, y2, done);
+    } else {
+      for (int c = 0; c < 8; ++c) {
+        if (ccnt[c] == 0) continue;
+        const unsigned g = (unsigned)std::min<int64_t>(4096, (ccnt[c] + 255) / 256);
+        if (two)
+          hipLaunchKernelGGL((k_bb<false, true>), dim3(g), dim3(256), 0, s, E, iss, bb, mask,
+                             elems + coff[c], (int64_t)ccnt[c], x1, y1, x2, y2, done);
+        else
+          hipLaunchKernelGGL((k_bb<false, false>), dim3(g), dim3(256), 0, s, E, iss, bb, mask,
+                             elems + coff[c], (int64_t)ccnt[c], x1, y1, x2, y2, done);
+      }
+    }
+  };
+  // r = A x0, then the initial residuals, partials and (zero) status
+  hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, nullptr, r, nullptr, n);
+  dual(x, r, nullptr, nullptr, nullptr);
+  hipLaunchKernelGGL((k_init<double>), dim3(nb), dim3(VB), 0, s, n, mask, b, x, r, rr, z, zz,
+                     diag, pr, pb, pbn, st);
+  VGPU_HIP(hipGetLastError());
+  // iterations in pairs (partials by parity), the status read after each
+  // pair; kernels of iterations past the end return at once
+  const int max_pairs = itmax / 2 + 2;
+  SolveStatus hs{};
+  for (int pair = 0; pair < max_pairs; ++pair) {
+    for (int q = 1; q >= 0; --q) {
+      hipLaunchKernelGGL((k_dir<double>), dim3(nb), dim3(VB), 0, s, n, nb, q, pr, pb, pbn, st,
+                         tol, itmax, 1e300, p, pp, z, zz, 1);
+      dual(p, z, pp, zz, &st->done);
+      hipLaunchKernelGGL((k_dot<double>), dim3(nb), dim3(VB), 0, s, n, st, z, pp, pa);
+      hipLaunchKernelGGL((k_update<double>), dim3(nb), dim3(VB), 0, s, n, nb, nb, q, pa, pr,
+                         pb, st, x, p, r, rr, z, zz, diag);
+    }
+    VGPU_HIP(hipGetLastError());
+    VGPU_HIP(hipMemcpyAsync(&hs, st, sizeof hs, hipMemcpyDeviceToHost, s));
+    VGPU_HIP(hipStreamSynchronize(s));
+    if (hs.done) break;
+  }
+  *iter = hs.iter;
+  *err = hs.err;
+  if (hs.done == 1) return VGPU_OK;
+  if (hs.done == 3) {
+    set_error("kernel_dbcg_simple_noalloc: BiCG breakdown");
+    return VGPU_BREAKDOWN;
+  }
+  set_error("kernel_dbcg_simple_noalloc: itmax reached");
+  return VGPU_NOT_CONVERGED;
+}
+
 }  // namespace
 }  // namespace vgpu
 
 extern "C" {
+
+int kernel_dbcg_simple_noalloc(int n, const double *b_dev, double *x_dev, double tol,
+                               int itmax, int *iter, double *err, const double *bb_dev,
+                               const int *iss_dev, const double *diag_dev, int nax, int nay,
+                               int naz, const int *bc_dev, int scatter, void *stream) {
+  return vgpu::legacy_solve_noalloc(n, b_dev, x_dev, tol, itmax, iter, err, bb_dev, iss_dev,
+                                    diag_dev, nax, nay, naz, bc_dev, scatter,
+                                    static_cast<hipStream_t>(stream));
+}
+
+int vgpu_noalloc_max_nodes(int64_t *max_nodes) {
+  if (!max_nodes) return VGPU_ERR_ARG;
+  *max_nodes = vgpu::kNaMaxNodes;
+  return VGPU_OK;
+}
 
 void kernel_dbcg_simple_(int *N, double *B, double *X, double *TOL, int *ITMAX,
                          int *ITER, double *ERR, double *BB, int *ISS,
