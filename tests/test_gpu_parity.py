@@ -213,6 +213,24 @@ def test_legacy_drop_in_reproduces_transcript(pkg, po, gpu, golden_log):
             assert m == pytest.approx(float(s["mass_text"]), rel=1e-8), k
 
 
+def test_legacy_device_colour_lists(pkg, po, gpu):
+    """The per-colour element lists, offsets and counts the colour path holds
+    on the device after a call (read back) equal the oracle's
+    generate_hex8_colors / build_color_element_lists bit for bit; the call's
+    phase timers are filled."""
+    p = pkg.problem.reference_default_case()
+    bb, b, d = po.assemble(p)
+    b, d, t = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    _, it, err = pkg.kernel_dbcg_simple(p.N, b, t, 1e-8, 1000, bb, p.iss, d, p.nax, p.nay,
+                                        p.naz, p.mask, scatter="colored")
+    assert err <= 1e-8
+    el, off, cnt = pkg._lib.legacy_device_color_lists(p.E)
+    _, oel, ooff, ocnt = po.colors(p.nax, p.nay, p.naz)
+    assert np.array_equal(el, oel) and np.array_equal(off, ooff) and np.array_equal(cnt, ocnt)
+    up, so, down = pkg._lib.legacy_last_times()
+    assert up > 0 and so > 0 and down > 0
+
+
 def test_config1_solutions_vs_oracle(pkg, po, gpu):
     """The 32x32x16 workload at full size: FP64 BiCG (owner, colored and
     atomic scatters), FP64 BiCGSTAB and mixed precision, each to rtol 1e-8,

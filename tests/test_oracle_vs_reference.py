@@ -1,6 +1,8 @@
 """The oracle against the reference's own Fortran, built by `make -C oracle ref`
 with flang from the sources (V1: VANADIS.FOR, V3: the OpenMP timing build).
 Skipped when oracle/_ref has not been built (e.g. no reference tree)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -106,3 +108,33 @@ def test_dirichlet_mask_bitwise(pkg, po, which, facemap, dims):
     assert np.array_equal(pkg.problem.build_bc_nodes(iss, kod, nax, nay, naz, facemap), mr)
     other = 3 if facemap == 1 else 1
     assert not np.array_equal(po.build_bc_nodes(other, iss, kod, nax, nay, naz), mr)
+
+
+def test_unchanged_host_links_against_the_library(pkg):
+    """VANADIS.FOR, unchanged, compiled by flang and linked against
+    libvanadis_gpu.so in place of the CUDA objects (makefile:298-302):
+    `make -C oracle link`.  The object needs kernel_dbcg_simple_ and the
+    linked executable takes it from the library (dynamic symbol, no
+    unresolved references)."""
+    import shutil
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    if not os.path.isdir("/root/reference/src_v2026_2_1") or shutil.which("flang") is None:
+        pytest.skip("needs the reference sources and flang")
+    r = subprocess.run(["make", "-B", "-C", os.path.join(root, "oracle"), "link"],
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stderr[-2000:]
+    ref = os.path.join(root, "oracle", "_ref")
+    obj = subprocess.run(["nm", "-u", os.path.join(ref, "vanadis_host.o")],
+                         capture_output=True, text=True).stdout
+    assert "kernel_dbcg_simple_" in obj
+    exe = os.path.join(ref, "vanadis_gpu_host")
+    dyn = subprocess.run(["nm", "-D", "-u", exe], capture_output=True, text=True).stdout
+    assert "kernel_dbcg_simple_" in dyn  # resolved at run time ...
+    ldd = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libvanadis_gpu.so => " + os.path.join(root, "vanadis_solver-directional_residual_amd")\
+        in ldd  # ... from the library
+    assert "not found" not in ldd
+    exports = subprocess.run(["nm", "-D", "--defined-only", pkg._lib.LIB_PATH],
+                             capture_output=True, text=True).stdout
+    assert " T kernel_dbcg_simple_" in exports
