@@ -527,6 +527,14 @@ def batched_fallback(pkg, ctxs, bs, cap=2000, itmax=20000):
             "speedup": round(t_seq / t_batch, 3), "bitwise_equal_to_single": same}
 
 
+def own_zchunk_default(tuned, a):
+    return tuned["zchunk"] if tuned else max(a.zchunk, 0)
+
+
+def own_tile_default(tuned, a):
+    return tuned["tile"] if tuned else max(a.tile, 0)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -641,12 +649,23 @@ def main():
 
     # the two scatter flavours of the reference's solver (8 colour launches;
     # FP64 atomics), FP64 BiCG to the same rtol (reported, never fatal)
-    own_zchunk = tuned["zchunk"] if tuned else max(a.zchunk, 0)
-    own_tile = tuned["tile"] if tuned else max(a.tile, 0)
+    own_zchunk = own_zchunk_default(tuned, a)
+    own_tile = own_tile_default(tuned, a)
+    # brick shape and depth of the two brick scatters from the brick tuner
+    brick_tuned = {}
+    for sc in ("colored", "atomic"):
+        if a.no_tune:
+            brick_tuned[sc] = {"tile": 0, "zchunk": 0, "ms": None, "source": "library rule"}
+        else:
+            brick_tuned[sc] = ctx.autotune_bricks(sc, pkg.FP64, reps=10)
+            brick_tuned[sc]["source"] = "vgpu_autotune_bricks"
+            progress(f"brick tuner {sc}: shape {brick_tuned[sc]['tile']} depth "
+                     f"{brick_tuned[sc]['zchunk']} ({brick_tuned[sc]['ms'] * 1e3:.1f} us dual)")
+        ctx.set_options("owner", own_zchunk_default(tuned, a), own_tile_default(tuned, a))
     scatter_solves = {}
     if not a.no_scatter_solves:
         for sc in ("colored", "atomic"):
-            ctx.set_options(sc, 0, own_tile)
+            ctx.set_options(sc, brick_tuned[sc]["zchunk"], brick_tuned[sc]["tile"])
             try:
                 solve(pkg.FP64)
                 csec = []
@@ -708,10 +727,9 @@ def main():
         progress(f"conv layers {conv['status']}")
     modes = {}
     for sc in ("owner", "colored", "atomic"):
-        # brick scatters: the library's brick depth (z chunk 0), not the
-        # owner launches' tuned chunk
+        # brick scatters: the brick tuner's shape and depth
         if sc != "owner":
-            ctx.set_options(sc, 0, own_tile)
+            ctx.set_options(sc, brick_tuned[sc]["zchunk"], brick_tuned[sc]["tile"])
         for prec, pn in ((pkg.FP64, "fp64"), (pkg.FP32, "fp32")):
             ms = ctx.time_kernel(0, sc, prec, 20)
             modes[f"{sc}_{pn}"] = {"ms": round(ms, 5), "GDOF/s": round(N / ms / 1e6, 2)}
@@ -808,6 +826,7 @@ def main():
         "ms_per_iteration_fp64": round(ms_step / iters[-1], 5),
         "scatter_modes": modes,
         "scatter_solves_fp64": scatter_solves,
+        "brick_autotune": brick_tuned,
         "copy_GBps": round(copy_gbs, 1),
         "copy_8N_GBps": round(copy8_gbs, 1),
         "copy_hbm_GBps": round(copyh_gbs, 1),

@@ -160,8 +160,8 @@ int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
  * n > 0 = owner launches give each (chunk of n node planes, tile) group
  * one workgroup, groups numbered chunk by chunk so that neighbouring tiles
  * march the same planes together; brick launches use n planes per brick)
- * and owner tile shape (0 = 32x8, 1 = 16x16, 2 = 64x4 element columns;
- * -1 = keep).  The owner results do not depend on the z chunk or the tile
+ * and tile shape (0 = 32x8, 1 = 16x16, 2 = 64x4 element columns: the owner
+ * tiles, and the bricks of the colored / atomic scatters; -1 = keep).  The owner results do not depend on the z chunk or the tile
  * shape (bitwise); the owner scatter needs 8 N and 40 E below 2^31 bytes
  * (else VGPU_ERR_ARG; such systems default to the coloured scatter). */
 int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk, int tile);
@@ -259,6 +259,20 @@ int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
  * the choice does not change any result bit. */
 int vgpu_autotune(vgpu_ctx *ctx, int precision, int reps, int *scatter,
                   int *tile, int *zchunk, double *best_ms);
+
+/* Brick tuner of the reference's two solver flavours (colour-block sizes of
+ * the colored scatter, coloring.cu:289-360; brick sizes of the atomic one,
+ * cudatest.cu:184-227, whose launches use fixed 256-thread blocks): times the
+ * dual apply (vgpu_time_kernel what = 1) of `scatter` (VGPU_SCATTER_COLORED
+ * or VGPU_SCATTER_ATOMIC) for every brick shape (tile 0 = 32 x 8, 1 = 16 x 16,
+ * 2 = 64 x 4 element columns) and brick depth in {library rule (0), 1, 2, 3,
+ * 4, 6, 8, 12, 16} element layers, keeps the fastest on the context (scatter,
+ * tile, zchunk: the solver then uses that scatter) and returns it.  Colored
+ * results are bitwise reproducible for a fixed choice and agree with the
+ * oracle to <= 1e-13 for every choice (the order of a brick-face node's sums
+ * depends on the bricks). */
+int vgpu_autotune_bricks(vgpu_ctx *ctx, int scatter, int precision, int reps, int *tile,
+                         int *zchunk, double *best_ms);
 
 /* Device-resident forms for time stepping: b_dev = right-hand side of the
  * scheme from element sources q_dev [E] and the previous solution t_old_dev

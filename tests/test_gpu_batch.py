@@ -180,3 +180,34 @@ def test_fallback_finishes_unconverged_members(pkg, gpu):
             bt.set_fallback("bicgstab", itmax=-1)
     for c in ctxs:
         c.close()
+
+
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+def test_brick_autotune_keeps_parity_and_cache_roundtrips(pkg, po, tmp_path, scatter):
+    """The brick tuner of the colored / atomic scatter picks a brick shape and
+    depth among the candidates; the operator with the choice still matches
+    the oracle to 1e-13, the context then solves with that scatter, colored
+    results are bitwise repeatable, and the JSON cache round-trips."""
+    p = pkg.problem.unit_box(40, 24, 20, lam=1e-3, vel_seed=2)
+    bb, b, d = po.assemble(p)
+    ctx = _ctx(pkg, p)
+    x = np.random.default_rng(8).standard_normal(p.N)
+    cache = str(tmp_path / "tune.json")
+    log(f"brick autotune {scatter}")
+    r = pkg.autotune.tune(ctx, pkg.FP64, cache_path=cache, reps=3, scatter=scatter)
+    assert not r["cached"] and r["tile"] in (0, 1, 2)
+    assert r["zchunk"] in (0, 1, 2, 3, 4, 6, 8, 12, 16) and r["ms"] > 0
+    for tr in (False, True):
+        ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+        y = ctx.apply(x, scatter=scatter, transpose=tr)
+        assert np.linalg.norm(y - ref) <= 1e-13 * np.linalg.norm(ref)
+    if scatter == "colored":
+        assert np.array_equal(ctx.apply(x, scatter=scatter), ctx.apply(x, scatter=scatter))
+    bo, do, x0 = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    xs, it, err, rc = ctx.solve(bo, rtol=1e-10, itmax=5000)  # the tuned scatter
+    assert rc == 0 and err <= 1e-10
+    r2 = pkg.autotune.tune(ctx, pkg.FP64, cache_path=cache, scatter=scatter)
+    assert r2["cached"] and (r2["tile"], r2["zchunk"]) == (r["tile"], r["zchunk"])
+    entry = json.loads(open(cache).read())
+    assert any(k.endswith("|" + scatter) for k in entry)
+    ctx.close()

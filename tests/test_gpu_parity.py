@@ -96,15 +96,15 @@ def test_owner_tile_shapes(case, po, tile, zchunk):
 
 
 @pytest.mark.parametrize("scatter", ["colored", "atomic"])
-@pytest.mark.parametrize("zchunk", [1, 3, 8])
-def test_brick_depths(case, po, scatter, zchunk):
+@pytest.mark.parametrize("zchunk,tile", [(1, 0), (3, 0), (8, 0), (0, 1), (2, 1), (0, 2), (5, 2)])
+def test_brick_depths(case, po, scatter, zchunk, tile):
     """The colored and atomic scatters at explicit brick depths (element
-    layers per brick; 0 above lets the library choose) give the operator,
-    forward and transpose."""
+    layers per brick; 0 lets the library choose) and brick shapes (32x8,
+    16x16, 64x4) give the operator, forward and transpose."""
     name, p, bb, _, _, ctx = case
-    log(f"brick depth {zchunk} {scatter} {name}")
+    log(f"brick depth {zchunk} shape {tile} {scatter} {name}")
     x = np.random.default_rng(19).standard_normal(p.N)
-    ctx.set_options(scatter, zchunk, 0)
+    ctx.set_options(scatter, zchunk, tile)
     try:
         for tr in (False, True):
             ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)

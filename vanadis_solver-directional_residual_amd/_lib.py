@@ -41,7 +41,7 @@ EXPORTED = (
     "vgpu_set_method", "vgpu_axpbypcz_dev", "vgpu_net_time_layers",
     "vgpu_net_set_zchunk", "vgpu_batch_set_fallback", "vgpu_legacy_colors",
     "vgpu_legacy_get_color_lists", "vgpu_legacy_last_times", "kernel_dbcg_simple_noalloc",
-    "vgpu_noalloc_max_nodes",
+    "vgpu_noalloc_max_nodes", "vgpu_autotune_bricks",
 )
 
 
@@ -111,6 +111,7 @@ def load():
     L.vgpu_memcpy_d2h.argtypes = [_vp, _vp, _vp, C.c_int64]
     L.vgpu_time_kernel.argtypes = [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _pd]
     L.vgpu_autotune.argtypes = [_vp, C.c_int, C.c_int, _pi, _pi, _pi, _pd]
+    L.vgpu_autotune_bricks.argtypes = [_vp, C.c_int, C.c_int, C.c_int, _pi, _pi, _pd]
     L.vgpu_batch_create.argtypes = [C.c_int, C.POINTER(_vp), C.POINTER(_vp)]
     L.vgpu_batch_destroy.argtypes = [_vp]
     L.vgpu_batch_solve.argtypes = [_vp, C.c_int, C.POINTER(_vp), C.POINTER(_vp), C.c_double,
@@ -396,6 +397,19 @@ class Context:
         check(load().vgpu_autotune(self.h, int(precision), int(reps), C.byref(sc), C.byref(t),
                                    C.byref(z), C.byref(ms)), "autotune")
         return {"scatter": sc.value, "tile": t.value, "zchunk": z.value, "ms": ms.value}
+
+
+    def autotune_bricks(self, scatter="colored", precision=FP64, reps=20):
+        """Time every brick shape / depth of the colored or atomic scatter's
+        dual apply, keep the fastest (the context then solves with that
+        scatter); returns {"scatter", "tile", "zchunk", "ms"}."""
+        t, z = C.c_int(0), C.c_int(0)
+        ms = C.c_double(0)
+        check(load().vgpu_autotune_bricks(self.h, scatter_code(scatter), int(precision),
+                                          int(reps), C.byref(t), C.byref(z), C.byref(ms)),
+              "autotune_bricks")
+        return {"scatter": scatter_code(scatter), "tile": t.value, "zchunk": z.value,
+                "ms": ms.value}
 
 
 class Batch:

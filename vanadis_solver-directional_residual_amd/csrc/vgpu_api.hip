@@ -1853,6 +1853,48 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
   return VGPU_OK;
 }
 
+int vgpu_autotune_bricks(vgpu_ctx *c, int scatter, int precision, int reps, int *tile,
+                         int *zchunk, double *best_ms) {
+  if (!c || reps < 1 || (precision != VGPU_FP64 && precision != VGPU_FP32) ||
+      (scatter != VGPU_SCATTER_COLORED && scatter != VGPU_SCATTER_ATOMIC)) {
+    set_error("vgpu_autotune_bricks: bad arguments");
+    return VGPU_ERR_ARG;
+  }
+  if (!c->have_coeffs) return VGPU_ERR_STATE;
+  // brick depths: the library's rule (0) and fixed layer counts
+  const int depths[] = {0, 1, 2, 3, 4, 6, 8, 12, 16};
+  const int save_tile = c->tile, save_zc = c->zchunk, save_sc = c->scatter;
+  double best = 1e300;
+  int bt = -1, bz = 0;
+  for (int t = 0; t < 3; ++t)
+    for (int zc : depths) {
+      if (zc > c->g.ez) continue;
+      c->tile = t;
+      c->zchunk = zc;
+      double ms = 0;
+      const int rc = vgpu_time_kernel(c, 1, scatter, precision, reps, &ms);
+      if (rc) {
+        c->tile = save_tile;
+        c->zchunk = save_zc;
+        c->scatter = save_sc;
+        return rc;
+      }
+      if (ms < best) {
+        best = ms;
+        bt = t;
+        bz = zc;
+      }
+    }
+  c->scatter = scatter;
+  c->tile = bt;
+  c->zchunk = bz;
+  c->version++;
+  if (tile) *tile = bt;
+  if (zchunk) *zchunk = bz;
+  if (best_ms) *best_ms = best;
+  return VGPU_OK;
+}
+
 int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                      int reps, double *avg_ms) {
   if (!c || !avg_ms || reps < 1 || what < 0 || what > 7) return VGPU_ERR_ARG;

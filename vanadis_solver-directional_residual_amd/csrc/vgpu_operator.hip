@@ -1476,6 +1476,37 @@ template int launch_owner_stab<float, float>(const OpData<float> &, int, int,
                                              const StabData<float> &, const float *, float *,
                                              hipStream_t);
 
+// The COLORED / ATOMIC brick launches of one apply for bricks of BX x BY
+// element columns and zc element layers (zc <= 0: brick_depth's rule).
+template <typename T, typename TIO, int BX, int BY>
+void launch_bricks(const OpData<T> &op, int scatter, int zc, const TIO *x1, TIO *y1,
+                   const TIO *x2, TIO *y2, hipStream_t s) {
+  const Grid3 &g = op.g;
+  const bool dual = x2 != nullptr;
+  if (zc <= 0) zc = brick_depth(g, scatter == VGPU_SCATTER_COLORED, BX, BY);
+  const int ntx = (g.ex + BX - 1) / BX, nty = (g.ey + BY - 1) / BY;
+  const int ntz = (g.ez + zc - 1) / zc;
+  if (scatter == VGPU_SCATTER_ATOMIC) {
+    dim3 grid(ntx * nty * ntz);
+    if (dual)
+      hipLaunchKernelGGL((k_brick<T, TIO, true, true, BX, BY>), grid, dim3(BX * BY), 0, s,
+                         op, zc, 0, x1, y1, x2, y2);
+    else
+      hipLaunchKernelGGL((k_brick<T, TIO, false, true, BX, BY>), grid, dim3(BX * BY), 0, s,
+                         op, zc, 0, x1, y1, x2, y2);
+    return;
+  }
+  dim3 grid((ntx + 1) / 2, (nty + 1) / 2, (ntz + 1) / 2);
+  for (int c = 0; c < 8; ++c) {
+    if (dual)
+      hipLaunchKernelGGL((k_brick<T, TIO, true, false, BX, BY>), grid, dim3(BX * BY), 0, s,
+                         op, zc, c, x1, y1, x2, y2);
+    else
+      hipLaunchKernelGGL((k_brick<T, TIO, false, false, BX, BY>), grid, dim3(BX * BY), 0, s,
+                         op, zc, c, x1, y1, x2, y2);
+  }
+}
+
 template <typename T, typename TIO>
 int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
                  TIO *y1, const TIO *x2, TIO *y2, double *dot_part,
@@ -1508,34 +1539,24 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
     L.dot = dot_part;
     return launch_owner_any<T, TIO, false>(L, op.tile, op.ra != 0.0, s);
   }
-  // brick modes need zeroed outputs; bricks are 32 x 8 element columns
-  constexpr int BX = 32, BY = 8;
-  if (zc <= 0) zc = brick_depth(g, scatter == VGPU_SCATTER_COLORED, BX, BY);
-  hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
-                     dual ? y2 : nullptr, g.N);
-  const int ntx = (g.ex + BX - 1) / BX, nty = (g.ey + BY - 1) / BY;
-  const int ntz = (g.ez + zc - 1) / zc;
-  if (scatter == VGPU_SCATTER_ATOMIC) {
-    dim3 grid(ntx * nty * ntz);
-    if (dual)
-      hipLaunchKernelGGL((k_brick<T, TIO, true, true, BX, BY>), grid, dim3(BX * BY),
-                         0, s, op, zc, 0, x1, y1, x2, y2);
-    else
-      hipLaunchKernelGGL((k_brick<T, TIO, false, true, BX, BY>), grid,
-                         dim3(BX * BY), 0, s, op, zc, 0, x1, y1, x2, y2);
-  } else if (scatter == VGPU_SCATTER_COLORED) {
-    dim3 grid((ntx + 1) / 2, (nty + 1) / 2, (ntz + 1) / 2);
-    for (int c = 0; c < 8; ++c) {
-      if (dual)
-        hipLaunchKernelGGL((k_brick<T, TIO, true, false, BX, BY>), grid,
-                           dim3(BX * BY), 0, s, op, zc, c, x1, y1, x2, y2);
-      else
-        hipLaunchKernelGGL((k_brick<T, TIO, false, false, BX, BY>), grid,
-                           dim3(BX * BY), 0, s, op, zc, c, x1, y1, x2, y2);
-    }
-  } else {
+  // brick modes need zeroed outputs; the brick shape follows op.tile
+  // (0 = 32 x 8, 1 = 16 x 16, 2 = 64 x 4 element columns)
+  if (scatter != VGPU_SCATTER_ATOMIC && scatter != VGPU_SCATTER_COLORED) {
     set_error("unknown scatter mode " + std::to_string(scatter));
     return VGPU_ERR_ARG;
+  }
+  hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
+                     dual ? y2 : nullptr, g.N);
+  switch (op.tile) {
+    case 1:
+      launch_bricks<T, TIO, 16, 16>(op, scatter, zc, x1, y1, x2, y2, s);
+      break;
+    case 2:
+      launch_bricks<T, TIO, 64, 4>(op, scatter, zc, x1, y1, x2, y2, s);
+      break;
+    default:
+      launch_bricks<T, TIO, 32, 8>(op, scatter, zc, x1, y1, x2, y2, s);
+      break;
   }
   VGPU_HIP(hipGetLastError());
   return VGPU_OK;
