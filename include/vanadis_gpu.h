@@ -40,7 +40,7 @@ extern "C" {
 /* ---- enums ---- */
 /* scatter modes of the element operator */
 #define VGPU_SCATTER_OWNER 0    /* owner-computes tiles, deterministic, 1 pass */
-#define VGPU_SCATTER_COLORED 1  /* 8 brick colours, RMW, deterministic        */
+#define VGPU_SCATTER_COLORED 1  /* 2 colours of z-slabs, deterministic        */
 #define VGPU_SCATTER_ATOMIC 2   /* hardware FP64/FP32 atomics on tile faces   */
 /* DR variants (gauss_point_hex8DR): V1 VANADIS.FOR:1868-2023,
  * V2 src_updates/gauss_point_hex8DR.for, V3 vanadis_full_DR_OpenMP_timing.for:1567-1716 */
@@ -152,18 +152,26 @@ int vgpu_set_coeffs(vgpu_ctx *ctx, const double *lam, const double *vel,
                     double dt);
 /* Change only the scheme / dt (e.g. steady vs implicit Euler). */
 int vgpu_set_scheme(vgpu_ctx *ctx, int scheme, double dt);
-/* Options: scatter mode for the solver, z-chunk of the z-marching tiles
- * (-1 = keep; 0 = owner launches size their grid to the device's resident
- * workgroups and split the node planes evenly over them, brick launches take
- * the deepest brick of at most 8 element layers that still gives every
- * compute unit two workgroups per launch (per colour launch for COLORED);
- * n > 0 = owner launches give each (chunk of n node planes, tile) group
- * one workgroup, groups numbered chunk by chunk so that neighbouring tiles
- * march the same planes together; brick launches use n planes per brick)
- * and tile shape (0 = 32x8, 1 = 16x16, 2 = 64x4 element columns: the owner
- * tiles, and the bricks of the colored / atomic scatters; -1 = keep).  The owner results do not depend on the z chunk or the tile
- * shape (bitwise); the owner scatter needs 8 N and 40 E below 2^31 bytes
- * (else VGPU_ERR_ARG; such systems default to the coloured scatter). */
+/* Options: scatter mode for the solver, z chunk and tile shape (-1 = keep).
+ *   OWNER:   tiles of TX x TY element columns (tile 0 = 32x8, 1 = 16x16,
+ *            2 = 64x4) overlapping by one column, each node written by one
+ *            thread; zchunk 0 = the device's resident workgroups split the
+ *            node planes evenly, n > 0 = one workgroup per (chunk of n node
+ *            planes, tile) group, groups numbered chunk by chunk so that
+ *            neighbouring tiles march the same planes together.  Results do
+ *            not depend on the z chunk or the tile shape (bitwise).  Needs
+ *            8 N and 40 E below 2^31 bytes (else VGPU_ERR_ARG; such systems
+ *            default to the coloured scatter).
+ *   COLORED: the same tiles over z-slabs of zchunk element layers (0 = the
+ *            deepest slab of at most 8 layers whose colour launch still
+ *            gives every compute unit two workgroups); slabs coloured by
+ *            parity, two launches, each slab's face planes summed in colour
+ *            order: deterministic for a fixed choice, <= 1e-13 of the
+ *            oracle for every choice.
+ *   ATOMIC:  disjoint bricks of the tile shape and zchunk element layers
+ *            (0 = the deepest brick of at most 8 layers that still gives
+ *            every compute unit two workgroups), brick faces added with
+ *            hardware atomics (sum order not reproducible). */
 int vgpu_set_options(vgpu_ctx *ctx, int scatter, int zchunk, int tile);
 /* Krylov method of the FP64 and FP32 solves (and of the FP64 passes of a
  * mixed solve): VGPU_METHOD_BICG (default; the reference's method,

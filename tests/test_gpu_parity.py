@@ -114,6 +114,26 @@ def test_brick_depths(case, po, scatter, zchunk, tile):
         ctx.set_options("owner", 0, 0)
 
 
+@pytest.mark.parametrize("zchunk,tile", [(0, 0), (3, 1), (1, 2)])
+def test_colored_large_system_path(case, po, monkeypatch, zchunk, tile):
+    """The colored scatter's kernel for systems too large for the owner
+    kernel's 32-bit offsets (k_slab), forced on these meshes with
+    VGPU_COLORED_LARGE_PATH=1: the operator, forward and transpose."""
+    name, p, bb, _, _, ctx = case
+    log(f"colored large-system path {name} zc {zchunk} tile {tile}")
+    monkeypatch.setenv("VGPU_COLORED_LARGE_PATH", "1")
+    x = np.random.default_rng(23).standard_normal(p.N)
+    ctx.set_options("colored", zchunk, tile)
+    try:
+        for tr in (False, True):
+            ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+            y = ctx.apply(x, scatter="colored", transpose=tr)
+            assert rel(y, ref) <= 1e-13, (tr, rel(y, ref))
+        assert np.array_equal(ctx.apply(x, scatter="colored"), ctx.apply(x, scatter="colored"))
+    finally:
+        ctx.set_options("owner", 0, 0)
+
+
 @pytest.mark.parametrize("scatter", SCATTERS)
 def test_apply_fp32_matches_oracle(case, po, pkg, scatter):
     name, p, bb, _, _, ctx = case

@@ -207,8 +207,10 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
 //    s = d S).
 // Two launches per iteration, no vector pass in between.
 enum StabMode { kStabDir = 2, kStabS = 3 };
-// (4, 5: the same with the early status test; 6: a damped Jacobi sweep)
+// (4, 5: the same with the early status test; 6: a damped Jacobi sweep;
+// 7: the operator over the z-slabs of one colour of the colored scatter)
 constexpr int kJacobi = 6;
+constexpr int kSlab = 7;
 template <typename TIO>
 struct StabData {
   const TIO *po = nullptr, *vo = nullptr;  // kStabDir: P, V of iteration j-1

@@ -18,6 +18,7 @@
 #include "vgpu_internal.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 #include <map>
 #include <mutex>
@@ -254,13 +255,14 @@ struct PlanePair {
 
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int DM, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
-                                                  : ((DUAL || DM >= 2) ? 3 : 4)))) void k_owner(
+__attribute__((amdgpu_waves_per_eu(
+    sizeof(T) == 8 ? ((DUAL || (DM >= 2 && DM != kSlab)) ? 2 : 3)
+                   : ((DUAL || (DM >= 2 && DM != kSlab)) ? 3 : 4)))) void k_owner(
     Member<T> m0, const Member<T> *__restrict__ md, const int *__restrict__ act,
     const int64_t *__restrict__ actu, const int64_t *__restrict__ actg, int nact,
     int64_t U, int zc, const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
     TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
-    double *__restrict__ dot, DirData<TIO> dd, StabData<TIO> sd) {
+    double *__restrict__ dot, DirData<TIO> dd, StabData<TIO> sd, int slab_color) {
   constexpr bool DIR = DM == 1;  // BiCG iteration launch
   // BiCGSTAB launches; DM + 2: the same with the early status test
   constexpr bool SA = DM == kStabDir || DM == kStabDir + 2;
@@ -268,6 +270,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
   constexpr bool EARLY = DM == kStabDir + 2 || DM == kStabS + 2;
   // damped Jacobi sweep: output x + omega (b - A x) / d at the owned nodes
   constexpr bool JAC = DM == kJacobi;
+  // colored scatter (z-chunk launch only): the march of a z-slab of element
+  // layers [k0, k1 - 1) with the layers just below and above it taken as
+  // zero (no recomputed halo layer), so its bottom and top planes get the
+  // slab's partial sums: the colour-0 launch (slab_color 0, the even slabs)
+  // stores them, the colour-1 launch adds its own to them (the grid's top
+  // plane, which has no slab above, is stored)
+  constexpr bool SLAB = DM == kSlab;
   static_assert(DUAL || !DIR, "the direction update needs the dual operator");
   static_assert(!(DUAL && (STAB || JAC)), "the BiCGSTAB and Jacobi launches are single");
   constexpr int NP = (TX + 1) * (TY + 1);
@@ -436,7 +445,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     // 0 outside the grid: a plane above the top loads through resources of
     // zero size, at a clamped scalar offset)
     auto plane_fetch = [&](int gz, PlanePair<TIO, DIR> &r) {
-      const bool in = gz < naz;
+      const bool in = gz < (SLAB ? k1 : naz);
       const Res ra = in ? rx1 : rx1z, rb = in ? rx2 : rx1z, rk = in ? rmk : rx1z;
       const Res rc = in ? rpo : rx1z, rd = in ? rppo : rx1z;
       const int gc = min(gz, naz - 1);
@@ -525,7 +534,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       const int kc = min(max(kz, 0), ez - 1);
       c.hz = op.hz[kc];
       c.ihz = op.ihz[kc];
-      load_rec<T>(kz == kc ? rrc : rx1z, vr, uni(kc * eplane * SR), c);
+      const bool live = kz == kc && (!SLAB || (kz >= k0 && kz < k1 - 1));
+      load_rec<T>(live ? rrc : rx1z, vr, uni(kc * eplane * SR), c);
     };
 
     // dot terms (ND sums): each thread sums its owned nodes of the current
@@ -576,7 +586,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     {
       const int gz = max(k0 - 1, 0);
       const int sm = uni(gz * plane), sv = uni(gz * plane * SI);
-      const bool zin = k0 >= 1;
+      const bool zin = !SLAB && k0 >= 1;
       const Res ra = zin ? rx1 : rx1z, rb = zin ? rx2 : rx1z, rk = zin ? rmk : rx1z;
       const Res rc = zin ? rpo : rx1z, rd = zin ? rppo : rx1z, rt_ = zin ? rto : rx1z;
 #pragma unroll
@@ -802,12 +812,19 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       if (JAC)  // the input x of the node is its masked LDS value
         o1 = omk[P] ? TIO(0)
                     : TIO(xok[P] + sd.jomega * (((double)gok[P] - v1) / (dg != 0.0 ? dg : 1.0)));
+      TIO o2 = omk[P] ? TIO(0) : TIO(v2);
+      if (SLAB && slab_color == 1 && !omk[P] &&
+          (kl - 1 == k0 || (kl - 1 == k1 - 1 && kl - 1 < naz - 1))) {
+        // colour 1: a slab face plane adds to the colour-0 slab's sums
+        o1 = TIO((double)bload<TIO>(ry1, vo, so) + v1);
+        if (DUAL) o2 = TIO((double)bload<TIO>(ry2, vo, so) + v2);
+      }
       bstore(o1, ry1, vo, so);
       double dv[ND];
 #pragma unroll
       for (int i = 0; i < ND; ++i) dv[i] = 0.0;
       if (DUAL) {
-        bstore(omk[P] ? TIO(0) : TIO(v2), ry2, vo, so);
+        bstore(o2, ry2, vo, so);
         dv[0] = (double)o1 * xok[P];
       }
       const double rtv = (double)gok[P];
@@ -864,7 +881,7 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
       gok[P] = gon;
       // element layer kl; a layer outside the grid has zero fields and a
       // zero mass weight, so its result is zero
-      const bool lv = kl >= 0 && kl < ez;
+      const bool lv = kl >= 0 && kl < ez && (!SLAB || (kl >= k0 && kl < k1 - 1));
       T ob1[4], on1[4], ob2[4], on2[4];
       {
         ElemConst<T> k;
@@ -967,6 +984,11 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? ((DUAL || DM >= 2) ? 2 : 3)
     const int nt = ((g.nax + TX - 2) / (TX - 1)) * ((g.nay + TY - 2) / (TY - 1));
     const int64_t lg = lb - (BATCH ? actg[ia] : (int64_t)0);
     const int chunk = (int)(lg / nt), tile = (int)(lg - (int64_t)chunk * nt);
+    if (SLAB) {  // slab 2 chunk + colour: element layers [e0, e1), planes e0..e1
+      const int e0 = (2 * chunk + slab_color) * zc;
+      if (e0 < g.ez) march(m, tile, e0, min(g.ez, e0 + zc) + 1, lg);
+      return;
+    }
     const int k0 = chunk * zc;
     if (k0 < g.naz) march(m, tile, k0, min(g.naz, k0 + zc), lg);
     return;
@@ -1021,14 +1043,13 @@ __device__ __forceinline__ void elem_stage(const OpData<T> &op, bool valid,
 }
 
 // ---------------------------------------------------------------------------
-// COLORED / ATOMIC kernel: disjoint tiles of 32 x 8 elements x zc layers.
-// Nodes strictly inside the tile (x and y) of planes strictly inside the
-// chunk are stored; all other tile nodes are added (RMW for COLORED, atomic
-// for ATOMIC).  The output must be zero on entry.  COLORED runs one launch
-// per colour: tile parity (bx&1) | (by&1)<<1 | (bz&1)<<2 == color.
+// ATOMIC kernel: disjoint bricks of TX x TY element columns x zc layers (one
+// launch, XCD-aware brick order).  Nodes strictly inside the brick (x and y)
+// of planes strictly inside it are stored; all other brick nodes are added
+// with hardware atomics.  The output must be zero on entry.
 // ---------------------------------------------------------------------------
-template <typename T, typename TIO, bool DUAL, bool ATOM, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int color,
+template <typename T, typename TIO, bool DUAL, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc,
                                                 const TIO *__restrict__ x1,
                                                 TIO *__restrict__ y1,
                                                 const TIO *__restrict__ x2,
@@ -1040,18 +1061,9 @@ __global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int col
   __shared__ T st2[DUAL ? 8 * (TX * TY) : 1];
 
   if (op.done && *op.done) return;
-  int bx, by, bz;
-  if (ATOM) {  // 1-D grid, XCD-aware
-    const int ntx = (op.g.ex + TX - 1) / TX, nty = (op.g.ey + TY - 1) / TY;
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    bx = t % ntx;
-    by = (t / ntx) % nty;
-    bz = t / (ntx * nty);
-  } else {
-    bx = 2 * blockIdx.x + (color & 1);
-    by = 2 * blockIdx.y + ((color >> 1) & 1);
-    bz = 2 * blockIdx.z + ((color >> 2) & 1);
-  }
+  const int ntbx = (op.g.ex + TX - 1) / TX, ntby = (op.g.ey + TY - 1) / TY;
+  const int t = xcd_tile(blockIdx.x, gridDim.x);
+  const int bx = t % ntbx, by = (t / ntbx) % ntby, bz = t / (ntbx * ntby);
   const int kz0 = bz * zc;
   if (bx * TX >= op.g.ex || by * TY >= op.g.ey || kz0 >= op.g.ez) return;
   const int kz1 = min(kz0 + zc, op.g.ez);  // layers [kz0, kz1)
@@ -1081,13 +1093,10 @@ __global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int col
     if (masked || gx >= op.g.nax || gy >= op.g.nay || kz >= op.g.naz) return;
     const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * kz);
     const bool inner = complete_z && lx >= 1 && lx < TX && ly >= 1 && ly < TY;
-    if (inner) {
+    if (inner)
       y[n] = TIO(v);
-    } else if (ATOM) {
+    else
       atomicAdd(&y[n], TIO(v));
-    } else {
-      y[n] = TIO((double)y[n] + v);
-    }
   };
 
   PlaneRegs<TIO, DUAL> pr;
@@ -1135,6 +1144,123 @@ __global__ __launch_bounds__(TX * TY) void k_brick(OpData<T> op, int zc, int col
     const bool mk = pm[cl][h ? q1 : q0] != 0;
     emit(y1, lx, ly, kz1, a1[h], mk, false);
     if (DUAL) emit(y2, lx, ly, kz1, a2[h], mk, false);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// COLORED kernel: two colours of z-slabs of zc element layers (colour = slab
+// index parity), x-y tiles of TX x TY element columns overlapping by one
+// column, every node column owned by one tile (the owner kernel's x-y
+// split: a tile's owned nodes see all four in-plane elements).  A workgroup
+// marches one (tile, slab): the owned nodes of the planes inside the slab get
+// complete sums and plain stores; the slab's bottom and top planes get its
+// partial sums, which colour 0 stores and colour 1 adds to what colour 0
+// stored (its bottom plane always, its top plane unless it is the grid's
+// top).  Slabs of one colour share no node, the colour-0 launch runs before
+// the colour-1 launch, so every node's sum has a fixed order: deterministic,
+// no atomics, no zeroing pass.
+// ---------------------------------------------------------------------------
+template <typename T, typename TIO, bool DUAL, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void k_slab(OpData<T> op, int zc, int color,
+                                               const TIO *__restrict__ x1,
+                                               TIO *__restrict__ y1,
+                                               const TIO *__restrict__ x2,
+                                               TIO *__restrict__ y2) {
+  constexpr int NP = (TX + 1) * (TY + 1), NT = TX * TY;
+  __shared__ T pl1[2][NP];
+  __shared__ T pl2[DUAL ? 2 : 1][DUAL ? NP : 1];
+  __shared__ uint8_t pm[2][NP];
+  __shared__ T st1[8 * NT];
+  __shared__ T st2[DUAL ? 8 * NT : 1];
+
+  if (op.done && *op.done) return;
+  const int ntx = (op.g.nax + TX - 2) / (TX - 1), nty = (op.g.nay + TY - 2) / (TY - 1);
+  const int ntiles = ntx * nty;
+  // XCD-aware: each XCD takes a contiguous range of tiles of one slab row
+  const int lin = xcd_tile(blockIdx.x, gridDim.x);
+  const int tile = lin % ntiles, bz = 2 * (lin / ntiles) + color;
+  const int kz0 = bz * zc;
+  if (kz0 >= op.g.ez) return;
+  const int kz1 = min(kz0 + zc, op.g.ez);  // element layers [kz0, kz1)
+
+  const int tid = threadIdx.x;
+  const int tx = tid % TX, ty = tid / TX;
+  const int gx0 = (tile % ntx) * (TX - 1) - 1, gy0 = (tile / ntx) * (TY - 1) - 1;
+  const int exg = gx0 + tx, eyg = gy0 + ty;
+  const bool colok = exg >= 0 && eyg >= 0 && exg < op.g.ex && eyg < op.g.ey;
+  ColConst<T> cq;
+  col_setup<T>(colok ? op.hx[exg] : T(1), colok ? op.hy[eyg] : T(1),
+               colok ? op.ihx[exg] : T(1), colok ? op.ihy[eyg] : T(1), cq);
+  const bool robin = colok && exg == 0 && op.ra != 0.0;
+
+  const int q0 = tid, q1 = tid + NT;
+  const int lx0 = q0 % (TX + 1), ly0 = q0 / (TX + 1);
+  const int lx1 = q1 % (TX + 1), ly1 = q1 / (TX + 1);
+  const bool has1 = q1 < NP;
+
+  auto eidx = [&](int k) {
+    return exg + (int64_t)op.g.ex * (eyg + (int64_t)op.g.ey * k);
+  };
+  // owned node (lx, ly) of plane kz: store v (add to the stored value when
+  // add); Dirichlet nodes get 0
+  auto emit = [&](TIO *y, int lx, int ly, int kz, double v, bool masked, bool add) {
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    if (lx < 1 || ly < 1 || lx > TX - 1 || ly > TY - 1) return;
+    if (gx >= op.g.nax || gy >= op.g.nay || kz >= op.g.naz) return;
+    const int64_t n = gx + (int64_t)op.g.nax * (gy + (int64_t)op.g.nay * kz);
+    if (masked)
+      y[n] = TIO(0);
+    else if (add)
+      y[n] = TIO((double)y[n] + v);
+    else
+      y[n] = TIO(v);
+  };
+
+  PlaneRegs<TIO, DUAL> pr;
+  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kz0, lx0, ly0, lx1, ly1, has1, pr);
+  plane_store<T, TIO, DUAL>(pr, pl1[0], DUAL ? pl2[0] : nullptr, pm[0], q0, q1, has1);
+  plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kz0 + 1, lx0, ly0, lx1, ly1, has1, pr);
+  Coef<T> cn = load_coef<T>(op, colok, eidx(kz0), kz0);
+  double a1[2] = {0.0, 0.0}, a2[2] = {0.0, 0.0};
+  for (int kl = kz0; kl < kz1; ++kl) {
+    const int cb = (kl - kz0) & 1, ct = cb ^ 1;
+    plane_store<T, TIO, DUAL>(pr, pl1[ct], DUAL ? pl2[ct] : nullptr, pm[ct], q0, q1, has1);
+    const Coef<T> cf = cn;
+    if (kl + 1 < kz1) {
+      plane_load<T, TIO, DUAL>(op, x1, x2, gx0, gy0, kl + 2, lx0, ly0, lx1, ly1, has1, pr);
+      cn = load_coef<T>(op, colok, eidx(kl + 1), kl + 1);
+    }
+    lds_barrier();
+    const uint8_t mk0 = pm[cb][q0], mk1 = has1 ? pm[cb][q1] : uint8_t(1);
+    elem_stage<T, DUAL, TX, TY>(op, colok, robin, cq, cf, pl1[cb], pl1[ct],
+                                DUAL ? pl2[cb] : nullptr, DUAL ? pl2[ct] : nullptr, tx, ty,
+                                st1, DUAL ? st2 : nullptr, tid);
+    lds_barrier();
+    // plane kl: complete inside the slab, partial at its bottom
+    const bool add = kl == kz0 && color == 1;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h && !has1) break;
+      const int lx = h ? lx1 : lx0, ly = h ? ly1 : ly0;
+      const bool mk = h ? mk1 : mk0;
+      emit(y1, lx, ly, kl, a1[h] + node_sum<T, TX, TY>(st1, lx, ly, false), mk, add);
+      a1[h] = node_sum<T, TX, TY>(st1, lx, ly, true);
+      if (DUAL) {
+        emit(y2, lx, ly, kl, a2[h] + node_sum<T, TX, TY>(st2, lx, ly, false), mk, add);
+        a2[h] = node_sum<T, TX, TY>(st2, lx, ly, true);
+      }
+    }
+  }
+  // top plane of the slab (its flags are in the buffer written last)
+  const int cl = (kz1 - kz0) & 1;
+  const bool add = color == 1 && kz1 < op.g.ez;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h && !has1) break;
+    const int lx = h ? lx1 : lx0, ly = h ? ly1 : ly0;
+    const bool mk = pm[cl][h ? q1 : q0] != 0;
+    emit(y1, lx, ly, kz1, a1[h], mk, add);
+    if (DUAL) emit(y2, lx, ly, kz1, a2[h], mk, add);
   }
 }
 
@@ -1221,19 +1347,36 @@ int device_cus() {
   return cus;
 }
 
-// Element layers per brick of the COLORED / ATOMIC launches when the caller
-// leaves the choice to the library: the deepest brick (at most 8 layers)
-// whose launch (each colour's launch for COLORED) still gives every compute
-// unit two workgroups, else 1.  A colour launch holds an eighth of the
-// bricks, so at 128x128x64 COLORED takes 1 layer (512 bricks per colour) and
-// ATOMIC 8 (512 bricks).
-int brick_depth(const Grid3 &g, bool colored, int bx, int by) {
-  const int64_t ntx = (g.ex + bx - 1) / bx, nty = (g.ey + by - 1) / by;
-  const int64_t cols = colored ? ((ntx + 1) / 2) * ((nty + 1) / 2) : ntx * nty;
+// Element layers per brick of the ATOMIC launch when the caller leaves the
+// choice to the library: the deepest brick (at most 8 layers) whose launch
+// still gives every compute unit two workgroups, else 1 (at 128x128x64 with
+// 32 x 8 bricks: 8 layers, 512 bricks).
+int brick_depth(const Grid3 &g, int bx, int by) {
+  const int64_t cols = (int64_t)((g.ex + bx - 1) / bx) * ((g.ey + by - 1) / by);
+  const int64_t want = 2 * (int64_t)device_cus();
+  for (int zc = 8; zc > 1; --zc)
+    if (cols * ((g.ez + zc - 1) / zc) >= want) return zc;
+  return 1;
+}
+
+// VGPU_COLORED_LARGE_PATH=1 sends every COLORED apply to k_slab, the path of
+// systems too large for the owner kernel, so that it can be checked on
+// small meshes (read at every launch).
+bool colored_large_path() {
+  const char *e = std::getenv("VGPU_COLORED_LARGE_PATH");
+  return e && e[0] == '1';
+}
+
+// Element layers per slab of the COLORED launches when the caller leaves the
+// choice to the library: the deepest slab (at most 8 layers) whose colour
+// launches (each with half of the slabs) still give every compute unit two
+// workgroups, else 1.
+int slab_depth(const Grid3 &g, int tx, int ty) {
+  const int64_t tiles = (int64_t)((g.nax + tx - 2) / (tx - 1)) * ((g.nay + ty - 2) / (ty - 1));
   const int64_t want = 2 * (int64_t)device_cus();
   for (int zc = 8; zc > 1; --zc) {
-    const int64_t ntz = (g.ez + zc - 1) / zc;
-    if (cols * (colored ? (ntz + 1) / 2 : ntz) >= want) return zc;
+    const int64_t nsl = (g.ez + zc - 1) / zc;
+    if (tiles * (nsl / 2) >= want) return zc;
   }
   return 1;
 }
@@ -1264,6 +1407,7 @@ struct OwnerLaunch {
   double *dot;
   DirData<TIO> dd;
   StabData<TIO> sd;
+  int slab_color = 0;
 };
 
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int DM, int TX, int TY>
@@ -1273,7 +1417,7 @@ void launch_owner_k(const OwnerLaunch<T, TIO> &L, hipStream_t s) {
   hipLaunchKernelGGL((k_owner<T, TIO, DUAL, ROBIN, BATCH, DM, TX, TY>), dim3(nb),
                      dim3(TX * TY), 0, s, L.m0, L.md, L.as.act, L.as.actu, L.as.actg,
                      L.as.nact, L.utot, L.zchunk > 0 ? L.zchunk : 0, L.st, L.x1, L.y1,
-                     L.x2, L.y2, L.dot, L.dd, L.sd);
+                     L.x2, L.y2, L.dot, L.dd, L.sd, L.slab_color);
 }
 
 // DM: 0 plain operator, 1 BiCG iteration launch (dual), kStabDir / kStabS
@@ -1281,7 +1425,7 @@ void launch_owner_k(const OwnerLaunch<T, TIO> &L, hipStream_t s) {
 template <typename T, typename TIO, bool BATCH, int DM, int TX, int TY>
 void launch_owner_shape(const OwnerLaunch<T, TIO> &L, bool robin, hipStream_t s) {
   if (L.x2) {
-    if constexpr (DM <= 1) {
+    if constexpr (DM <= 1 || DM == kSlab) {
       if (robin)
         launch_owner_k<T, TIO, true, true, BATCH, DM, TX, TY>(L, s);
       else
@@ -1476,34 +1620,86 @@ template int launch_owner_stab<float, float>(const OpData<float> &, int, int,
                                              const StabData<float> &, const float *, float *,
                                              hipStream_t);
 
-// The COLORED / ATOMIC brick launches of one apply for bricks of BX x BY
-// element columns and zc element layers (zc <= 0: brick_depth's rule).
+// The ATOMIC brick launch of one apply for bricks of BX x BY element columns
+// and zc element layers (zc <= 0: brick_depth's rule); the outputs must be
+// zero on entry.
 template <typename T, typename TIO, int BX, int BY>
-void launch_bricks(const OpData<T> &op, int scatter, int zc, const TIO *x1, TIO *y1,
-                   const TIO *x2, TIO *y2, hipStream_t s) {
+void launch_bricks(const OpData<T> &op, int zc, const TIO *x1, TIO *y1, const TIO *x2,
+                   TIO *y2, hipStream_t s) {
   const Grid3 &g = op.g;
   const bool dual = x2 != nullptr;
-  if (zc <= 0) zc = brick_depth(g, scatter == VGPU_SCATTER_COLORED, BX, BY);
+  if (zc <= 0) zc = brick_depth(g, BX, BY);
   const int ntx = (g.ex + BX - 1) / BX, nty = (g.ey + BY - 1) / BY;
   const int ntz = (g.ez + zc - 1) / zc;
-  if (scatter == VGPU_SCATTER_ATOMIC) {
+  {
     dim3 grid(ntx * nty * ntz);
     if (dual)
-      hipLaunchKernelGGL((k_brick<T, TIO, true, true, BX, BY>), grid, dim3(BX * BY), 0, s,
-                         op, zc, 0, x1, y1, x2, y2);
+      hipLaunchKernelGGL((k_brick<T, TIO, true, BX, BY>), grid, dim3(BX * BY), 0, s, op, zc,
+                         x1, y1, x2, y2);
     else
-      hipLaunchKernelGGL((k_brick<T, TIO, false, true, BX, BY>), grid, dim3(BX * BY), 0, s,
-                         op, zc, 0, x1, y1, x2, y2);
-    return;
+      hipLaunchKernelGGL((k_brick<T, TIO, false, BX, BY>), grid, dim3(BX * BY), 0, s, op, zc,
+                         x1, y1, x2, y2);
   }
-  dim3 grid((ntx + 1) / 2, (nty + 1) / 2, (ntz + 1) / 2);
-  for (int c = 0; c < 8; ++c) {
+}
+
+// The two colour launches of the COLORED scatter on the owner kernel
+// (kSlab): tiles of the context's owner shape, slabs of zc element layers
+// (zc <= 0: slab_depth's rule), one workgroup per (slab, tile) of a colour.
+template <typename T, typename TIO>
+int launch_owner_slabs(const OpData<T> &op, int zc, const TIO *x1, TIO *y1, const TIO *x2,
+                       TIO *y2, hipStream_t s) {
+  int tx, ty;
+  owner_shape(op.tile, &tx, &ty);
+  const Grid3 &g = op.g;
+  if (zc <= 0) zc = slab_depth(g, tx, ty);
+  const int64_t tiles =
+      (int64_t)((g.nax + tx - 2) / (tx - 1)) * ((g.nay + ty - 2) / (ty - 1));
+  const int nsl = (g.ez + zc - 1) / zc;
+  for (int c = 0; c < 2; ++c) {
+    const int nslc = (nsl - c + 1) / 2;  // slabs of parity c
+    if (nslc <= 0) continue;
+    OwnerLaunch<T, TIO> L{};
+    L.m0.op = op;
+    L.m0.seg.n = g.N;
+    L.m0.seg.nu = owner_units(g, op.tile);
+    L.utot = L.m0.seg.nu;
+    L.gtot = tiles * nslc;
+    L.zchunk = zc;
+    // the solver's done flag is the first member of its status
+    L.st = reinterpret_cast<const SolveStatus *>(op.done);
+    L.x1 = x1;
+    L.y1 = y1;
+    L.x2 = x2;
+    L.y2 = y2;
+    L.slab_color = c;
+    const int rc = launch_owner_any<T, TIO, false, kSlab>(L, op.tile, op.ra != 0.0, s);
+    if (rc) return rc;
+  }
+  return VGPU_OK;
+}
+
+// The two colour launches of the COLORED scatter with k_slab (systems too
+// large for the owner kernel's 32-bit offsets) for tiles of BX x BY element
+// columns and slabs of zc element layers (zc <= 0: slab_depth's rule).
+template <typename T, typename TIO, int BX, int BY>
+void launch_slabs(const OpData<T> &op, int zc, const TIO *x1, TIO *y1, const TIO *x2,
+                  TIO *y2, hipStream_t s) {
+  const Grid3 &g = op.g;
+  const bool dual = x2 != nullptr;
+  if (zc <= 0) zc = slab_depth(g, BX, BY);
+  const int64_t tiles =
+      (int64_t)((g.nax + BX - 2) / (BX - 1)) * ((g.nay + BY - 2) / (BY - 1));
+  const int nsl = (g.ez + zc - 1) / zc;
+  for (int c = 0; c < 2; ++c) {
+    const int nslc = (nsl - c + 1) / 2;  // slabs of parity c
+    if (nslc <= 0) continue;
+    const dim3 grid((unsigned)(tiles * nslc));
     if (dual)
-      hipLaunchKernelGGL((k_brick<T, TIO, true, false, BX, BY>), grid, dim3(BX * BY), 0, s,
-                         op, zc, c, x1, y1, x2, y2);
+      hipLaunchKernelGGL((k_slab<T, TIO, true, BX, BY>), grid, dim3(BX * BY), 0, s, op, zc,
+                         c, x1, y1, x2, y2);
     else
-      hipLaunchKernelGGL((k_brick<T, TIO, false, false, BX, BY>), grid, dim3(BX * BY), 0, s,
-                         op, zc, c, x1, y1, x2, y2);
+      hipLaunchKernelGGL((k_slab<T, TIO, false, BX, BY>), grid, dim3(BX * BY), 0, s, op, zc,
+                         c, x1, y1, x2, y2);
   }
 }
 
@@ -1545,18 +1741,36 @@ int launch_apply(const OpData<T> &op, int scatter, int zc, const TIO *x1,
     set_error("unknown scatter mode " + std::to_string(scatter));
     return VGPU_ERR_ARG;
   }
-  hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
-                     dual ? y2 : nullptr, g.N);
-  switch (op.tile) {
-    case 1:
-      launch_bricks<T, TIO, 16, 16>(op, scatter, zc, x1, y1, x2, y2, s);
-      break;
-    case 2:
-      launch_bricks<T, TIO, 64, 4>(op, scatter, zc, x1, y1, x2, y2, s);
-      break;
-    default:
-      launch_bricks<T, TIO, 32, 8>(op, scatter, zc, x1, y1, x2, y2, s);
-      break;
+  if (scatter == VGPU_SCATTER_COLORED && owner_fits(g) && !colored_large_path()) {
+    return launch_owner_slabs<T, TIO>(op, zc, x1, y1, x2, y2, s);
+  } else if (scatter == VGPU_SCATTER_COLORED) {
+    // systems whose vectors exceed the owner kernel's 32-bit offsets (or
+    // any system with VGPU_COLORED_LARGE_PATH=1)
+    switch (op.tile) {
+      case 1:
+        launch_slabs<T, TIO, 16, 16>(op, zc, x1, y1, x2, y2, s);
+        break;
+      case 2:
+        launch_slabs<T, TIO, 64, 4>(op, zc, x1, y1, x2, y2, s);
+        break;
+      default:
+        launch_slabs<T, TIO, 32, 8>(op, zc, x1, y1, x2, y2, s);
+        break;
+    }
+  } else {
+    hipLaunchKernelGGL((k_zero2<TIO>), dim3(1024), dim3(256), 0, s, op.done, y1,
+                       dual ? y2 : nullptr, g.N);
+    switch (op.tile) {
+      case 1:
+        launch_bricks<T, TIO, 16, 16>(op, zc, x1, y1, x2, y2, s);
+        break;
+      case 2:
+        launch_bricks<T, TIO, 64, 4>(op, zc, x1, y1, x2, y2, s);
+        break;
+      default:
+        launch_bricks<T, TIO, 32, 8>(op, zc, x1, y1, x2, y2, s);
+        break;
+    }
   }
   VGPU_HIP(hipGetLastError());
   return VGPU_OK;
