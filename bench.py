@@ -542,7 +542,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--kernel-only", action="store_true",
                     help="only launch the dual apply, the BiCG and BiCGSTAB owner launches "
-                         "and the copy kernel (counter runs)")
+                         "and the copy kernels (counter runs)")
     ap.add_argument("--no-cpu", action="store_true",
                     help="skip the CPU baseline, the oracle checks and the legacy drop-in leg")
     ap.add_argument("--no-mixed", action="store_true")
@@ -598,6 +598,7 @@ def main():
             ctx.time_kernel(5, "owner", pkg.FP64, reps=5)
             ctx.time_kernel(6, "owner", pkg.FP64, reps=5)
             ctx.time_kernel(2, "owner", pkg.FP64, reps=5)
+            ctx.time_kernel(7, "owner", pkg.FP64, reps=5)
         progress("kernel-only done")
         print(json.dumps({"kernel_only": True, "N": N, "E": E}))
         return
