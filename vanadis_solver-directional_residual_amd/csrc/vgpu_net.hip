@@ -13,12 +13,11 @@
 // guess = x_{n-1} + extrap (x_{n-1} - x_{n-2}) + out_scale * net.
 //
 // Tensors are node-major, 16 BF16 channels per node ([N][16], 32 bytes).
-// A wave computes 16 consecutive nodes of an x row for all 16 output
-// channels, row after row up z: per tap one v_mfma_f32_16x16x16_bf16 with
-// A = the tap's weights (output x input channels, held in registers) and
-// B = the 16 nodes' input channels (8 bytes per lane, from a register window
-// of three node planes).  MFMA operand layout (16x16x16, lane l):
-// A[l%16][4(l/16)+i], B[4(l/16)+i][l%16], D[4(l/16)+i][l%16].
+// The layer kernels march tiles of x rows up z through an LDS ring of node
+// planes, one v_mfma_f32_16x16x32_bf16 per pair of taps; the weight-gradient
+// kernel reduces over nodes with v_mfma_f32_16x16x16_bf16, whose operand
+// layout (lane l, i = 0..3) is A[l%16][4(l/16)+i], B[4(l/16)+i][l%16],
+// D[4(l/16)+i][l%16].
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -69,114 +68,280 @@ __device__ __forceinline__ bool inside(const Grid3 &g, int x, int y, int z) {
   return x >= 0 && y >= 0 && z >= 0 && x < g.nax && y < g.nay && z < g.naz;
 }
 
-// Layer kernels marching up z.  One wave owns a column of 16-node x row
-// segments (segment, y) over a chunk of zc node planes; its 27 taps' weights
-// stay in registers for the whole march, and the input operands of the
-// three planes z-1, z, z+1 around the current output plane form a register
-// window (per plane the 9 (x, y) offsets of the tap stencil), so each plane
-// step loads the 9 operands of one new plane (issued one step ahead)
-// instead of 27 operands and 27 weight fragments.  The MFMAs of an output
-// row run in tap order t = 0..26 as in a one-row kernel, so the results do
-// not depend on the chunking.
+// Layer kernels.  A workgroup (4 waves) owns a tile of CT_TY x rows of 16
+// nodes, (x0 .. x0+15) x (y0 .. y0+CT_TY-1), over a chunk of zc node planes,
+// and marches it up z.  The input planes z-1, z, z+1 around the output plane
+// z, each with one halo node or row on every side of the tile, sit in LDS in
+// a ring of four plane slots.  Each step the workgroup writes plane z+1,
+// which it loaded two steps earlier, into the slot of plane z-3 and issues
+// the global loads of plane z+3 into the registers just freed, so a load has
+// two steps of MFMAs to arrive; one barrier per step orders the writes
+// against the reads of the step before.  A wave computes two rows of the
+// tile for all 16 output channels: per pair of taps one
+// v_mfma_f32_16x16x32_bf16 with A = the two taps' weights (16 output x 2 x 16
+// input channels, in registers for the whole march) and B = the 16 nodes'
+// input channels at the two taps' offsets, 16 bytes per lane from LDS.  The
+// 27 taps make 13 pairs and tap 26 alone (zero weights beside it).  An output
+// row's MFMAs run in the same order whatever the chunking, so the results do
+// not depend on it.
 //   FWD_HIDDEN: Y[n][co] = leaky(b[co] + sum_t sum_ci W[t][co][ci] X[n+off_t][ci])
 //   FWD_LAST:   out[n] = b[0] + sum_t sum_ci W[t][0][ci] X[n+off_t][ci] (FP32)
 //   BWD_DATA:   dZprev[n][ci] = leaky'(Yprev[n][ci]) sum_t sum_co WT[t][ci][co]
 //               dZ[n-off_t][co]
-// (off_t = (t%3-1, t/3%3-1, t/9-1)).  Waves are numbered column-fastest
-// within a chunk and workgroups mapped XCD-aware, so the waves of one L2
-// march neighbouring rows over the same planes.
+// (off_t = (t%3-1, t/3%3-1, t/9-1)).  MFMA operand layout (16x16x32, lane l,
+// i = 0..7): A[l%16][8(l/16)+i], B[8(l/16)+i][l%16], D[4(l/16)+i][l%16]; the
+// reduction index k = 8(l/16)+i stands for input channel 8((l/16)%2)+i of
+// tap 2j+(l/16)/2 in A and B alike.  In LDS a node takes its 32 bytes of
+// channels, unpadded: the 16-byte reads of 16 consecutive nodes by each
+// quarter wave run free of bank conflicts on gfx950 (its LDS conflict counter
+// reads zero; a 48-byte node stride doubled the LDS busy cycles).  Global
+// accesses are buffer operations on one node plane each (planes up to
+// CT_OOB bytes).  Tiles are numbered x-fastest within a chunk
+// and workgroups mapped XCD-aware, so the workgroups of one L2 march
+// neighbouring tiles over the same planes.
 enum ConvMode { FWD_HIDDEN = 0, FWD_LAST = 1, BWD_DATA = 2 };
 
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+constexpr int CT_RW = 2;                     // x rows per wave
+constexpr int CT_TY = 4 * CT_RW;             // x rows per tile
+constexpr int CT_ROWS = CT_TY + 2;           // staged rows: y0 - 1 .. y0 + CT_TY
+constexpr int CT_NS = 18;                    // staged nodes per row: x0 - 1 .. x0 + 16
+constexpr int CT_NB = 2 * NC;                // LDS bytes per node
+constexpr int CT_RB = CT_NS * CT_NB;         // LDS bytes per staged row
+constexpr int CT_SB = CT_ROWS * CT_RB;       // LDS bytes per plane slot
+constexpr int CT_NPC = CT_ROWS * CT_NS * 2;  // 16-byte pieces per staged plane
+constexpr int CT_NP = (NTAP + 1) / 2;        // tap pairs
+constexpr int CT_NQ = (CT_NPC + 255) / 256;  // pieces per thread and plane
+
+__device__ __forceinline__ v8s load8(const uint16_t *p) {
+  return *reinterpret_cast<const v8s *>(p);
+}
+// operand of tap t (forward X[n + off_t], backward dZ[n - off_t]) relative
+// to the lower corner (-1, -1, -1) of the output node's 3x3x3 neighbourhood
 template <int MODE>
-__global__ __launch_bounds__(256) void k_conv_z(Grid3 g, int zc, const uint16_t *__restrict__ X,
+__device__ __forceinline__ constexpr int ct_dz(int t) {
+  return MODE == BWD_DATA ? 2 - t / 9 : t / 9;
+}
+template <int MODE>
+__device__ __forceinline__ constexpr int ct_dy(int t) {
+  return MODE == BWD_DATA ? 2 - (t / 3) % 3 : (t / 3) % 3;
+}
+template <int MODE>
+__device__ __forceinline__ constexpr int ct_dx(int t) {
+  return MODE == BWD_DATA ? 2 - t % 3 : t % 3;
+}
+
+typedef unsigned int v4u __attribute__((ext_vector_type(4)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+constexpr int CT_OOB = 0x40000000;  // buffer offset past every plane: loads 0, stores dropped
+// buffer resource over node plane p of a [N][16] BF16 tensor (or an FP32
+// node vector when bytes_per_node is 4); no records outside the grid, so
+// every access to such a plane reads zeros or is dropped
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void *base, const Grid3 &g,
+                                                            int p, int bytes_per_node) {
+  const int pb = g.nax * g.nay * bytes_per_node;
+  const bool in = p >= 0 && p < g.naz;
+  return __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<char *>(static_cast<const char *>(base)) + (in ? (int64_t)p * pb : 0), 0,
+      in ? pb : 0, 0x00020000);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_conv_t(Grid3 g, int zc, const uint16_t *__restrict__ X,
                                                 const uint16_t *__restrict__ W,
                                                 const float *__restrict__ bias,
                                                 const uint16_t *__restrict__ Yprev,
                                                 uint16_t *__restrict__ Y,
                                                 float *__restrict__ out) {
-  const int lane = threadIdx.x & 63;
-  const int64_t wave =
-      (int64_t)xcd_tile(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-  const int nsx = (g.nax + 15) >> 4;
-  const int64_t ncol = (int64_t)nsx * g.nay;
-  const int nzc = (g.naz + zc - 1) / zc;
-  if (wave >= ncol * nzc) return;  // whole waves only
-  const int64_t cidx = wave % ncol;
-  const int chunk = (int)(wave / ncol);
-  const int x0 = (int)(cidx % nsx) * 16, y = (int)(cidx / nsx);
+  __shared__ __align__(16) unsigned char lds[4 * CT_SB];
+  const int tid = threadIdx.x, lane = tid & 63, wl = tid >> 6;
+  const int ntx = (g.nax + 15) >> 4, nty = (g.nay + CT_TY - 1) / CT_TY;
+  const int tpc = ntx * nty;
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);  // the grid is exactly tpc x chunks
+  const int chunk = tile / tpc, tr = tile % tpc;
+  const int x0 = (tr % ntx) * 16, y0 = (tr / ntx) * CT_TY;
   const int z0 = chunk * zc, z1 = min(z0 + zc, g.naz);
-  const int col = lane & 15, cg = lane >> 4;
-  const int x = x0 + col;
-  v4s a[NTAP];
+  const int col = lane & 15, cg = lane >> 4, hi = cg >> 1;
+  const v8s zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
+  // A: weights of tap 2j + hi, input channels 8 (cg % 2) .. + 7
+  v8s a[CT_NP];
 #pragma unroll
-  for (int t = 0; t < NTAP; ++t) a[t] = load4(W + (t * NC + col) * NC + 4 * cg);
-  // operand of stencil offset q = (ox+1) + 3 (oy+1) in plane zz
-  auto ld = [&](int q, int zz) -> v4s {
-    const int xx = x + q % 3 - 1, yy = y + q / 3 - 1;
-    v4s b = {0, 0, 0, 0};
-    if (inside(g, xx, yy, zz))
-      b = load4(X + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
-    return b;
-  };
-  v4s w0[9], w1[9], w2[9], wn[9];  // planes z-1, z, z+1, z+2
-#pragma unroll
-  for (int q = 0; q < 9; ++q) {
-    w0[q] = ld(q, z0 - 1);
-    w1[q] = ld(q, z0);
-    w2[q] = ld(q, z0 + 1);
+  for (int j = 0; j < CT_NP; ++j) {
+    const int t = 2 * j + hi;
+    a[j] = t < NTAP ? load8(W + (t * NC + col) * NC + 8 * (cg & 1)) : zero8;
   }
-  for (int z = z0; z < z1; ++z) {
-    if (z + 1 < z1) {
+  float bs[4] = {0.f, 0.f, 0.f, 0.f};
+  if (MODE == FWD_HIDDEN)
 #pragma unroll
-      for (int q = 0; q < 9; ++q) wn[q] = ld(q, z + 2);
+    for (int i = 0; i < 4; ++i) bs[i] = bias[4 * cg + i];
+  else if (MODE == FWD_LAST)
+    bs[0] = bias[0];
+  // The two 16-byte pieces this thread stages per plane: piece c is channel
+  // half c % 2 of staged node (c % 36) / 2 in staged row c / 36.  Global
+  // accesses are buffer operations whose range check supplies the zero
+  // padding (pieces outside the grid get an offset past the plane, planes
+  // outside the grid or past the chunk no records) and drops the stores of
+  // nodes outside the grid, so every access is issued and the waits before
+  // the LDS writes count exactly the accesses issued since.
+  int soff[CT_NQ], goff[CT_NQ];
+  bool pin[CT_NQ];
+#pragma unroll
+  for (int q = 0; q < CT_NQ; ++q) {
+    const int c = tid + 256 * q;
+    const int r = c / (2 * CT_NS), s = (c % (2 * CT_NS)) >> 1, hh = c & 1;
+    const int xx = x0 - 1 + s, yy = y0 - 1 + r;
+    pin[q] = c < CT_NPC;
+    goff[q] = pin[q] && xx >= 0 && xx < g.nax && yy >= 0 && yy < g.nay
+                  ? (xx + g.nax * yy) * (2 * NC) + 16 * hh
+                  : CT_OOB;
+    soff[q] = r * CT_RB + s * CT_NB + 16 * hh;
+  }
+  // plane p's pieces (zeros outside the grid; planes past z1 are never
+  // written, so they are not read either)
+  auto ld = [&](v4u *R, int p) {
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(X, g, p <= z1 ? p : -1, 2 * NC);
+#pragma unroll
+    for (int q = 0; q < CT_NQ; ++q) R[q] = __builtin_amdgcn_raw_buffer_load_b128(rs, goff[q], 0, 0);
+    __builtin_amdgcn_sched_barrier(0);  // issued here, ahead of the step's MFMAs
+  };
+  auto st = [&](const v4u *R, int p) {
+#pragma unroll
+    for (int q = 0; q < CT_NQ; ++q)
+      if (pin[q]) *reinterpret_cast<v4u *>(lds + ((p - z0 + 1) & 3) * CT_SB + soff[q]) = R[q];
+  };
+  // the wave's rows y0 + wr, y0 + wr + 1: the lane's node offsets in a plane
+  // (past the plane for nodes outside the grid) and its B operand position
+  const int wr = CT_RW * wl, x = x0 + col;
+  int noff[CT_RW];
+#pragma unroll
+  for (int rr = 0; rr < CT_RW; ++rr) {
+    const int y = y0 + wr + rr;
+    noff[rr] = x < g.nax && y < g.nay ? x + g.nax * y : -1;
+  }
+  int colx[3];  // byte offset of staged node col + d, channel half cg % 2
+#pragma unroll
+  for (int d = 0; d < 3; ++d) colx[d] = (col + d) * CT_NB + 16 * (cg & 1);
+  const int lbase = wr * CT_RB;
+  // BWD_DATA: Yprev of the wave's two rows, planes two steps apart
+  v2u yp[2][CT_RW];
+  auto ld_yp = [&](v2u *dst, int z) {
+    const __amdgpu_buffer_rsrc_t rs = plane_rsrc(Yprev, g, z < z1 ? z : -1, 2 * NC);
+#pragma unroll
+    for (int rr = 0; rr < CT_RW; ++rr)
+      dst[rr] = __builtin_amdgcn_raw_buffer_load_b64(
+          rs, noff[rr] >= 0 ? noff[rr] * (2 * NC) + 8 * cg : CT_OOB, 0, 0);
+  };
+  auto plane = [&](int z, const v2u *ypz) {
+    int sb[3];  // slots of planes z - 1, z, z + 1
+#pragma unroll
+    for (int d = 0; d < 3; ++d) sb[d] = ((z - z0 + d) & 3) * CT_SB;
+    // B of pair j: rows wr and wr + 1 at the offset of tap 2j + hi
+    auto bread = [&](int j, v8s *b) {
+      const int t0 = 2 * j, t1 = min(2 * j + 1, NTAP - 1);
+      const int o = hi ? sb[ct_dz<MODE>(t1)] + ct_dy<MODE>(t1) * CT_RB + colx[ct_dx<MODE>(t1)]
+                       : sb[ct_dz<MODE>(t0)] + ct_dy<MODE>(t0) * CT_RB + colx[ct_dx<MODE>(t0)];
+      const unsigned char *p = lds + lbase + o;
+#pragma unroll
+      for (int rr = 0; rr < CT_RW; ++rr) b[rr] = *reinterpret_cast<const v8s *>(p + rr * CT_RB);
+    };
+    v4f acc[CT_RW];
+#pragma unroll
+    for (int rr = 0; rr < CT_RW; ++rr) acc[rr] = v4f{0.f, 0.f, 0.f, 0.f};
+    // B operands one pair ahead (the compiler interleaves the reads and
+    // MFMAs at its own register budget)
+    v8s bq[2][CT_RW];
+    bread(0, bq[0]);
+#pragma unroll
+    for (int j = 0; j < CT_NP; ++j) {
+      if (j + 1 < CT_NP) bread(j + 1, bq[(j + 1) & 1]);
+      const v8s *b = bq[j & 1];
+      const v8bf aj = __builtin_bit_cast(v8bf, a[j]);
+#pragma unroll
+      for (int rr = 0; rr < CT_RW; ++rr)
+        acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aj, __builtin_bit_cast(v8bf, b[rr]),
+                                                          acc[rr], 0, 0, 0);
     }
-    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    if (MODE == FWD_LAST) {
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(out, g, z, 4);
 #pragma unroll
-    for (int t = 0; t < NTAP; ++t) {
-      // forward: X[n + off_t] = window (t/9, t%9); backward: dZ[n - off_t]
-      // = window (2 - t/9, 8 - t%9)
-      const int p = MODE == BWD_DATA ? 2 - t / 9 : t / 9;
-      const int q = MODE == BWD_DATA ? 8 - t % 9 : t % 9;
-      const v4s b = p == 0 ? w0[q] : p == 1 ? w1[q] : w2[q];
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[t], b, acc, 0, 0, 0);
-    }
-    if (x < g.nax) {
-      const int64_t n = x + (int64_t)g.nax * (y + (int64_t)g.nay * z);
-      if (MODE == FWD_LAST) {
-        if (cg == 0) out[n] = acc[0] + bias[0];
-      } else if (MODE == FWD_HIDDEN) {
-        v4s o;
+      for (int rr = 0; rr < CT_RW; ++rr)
+        __builtin_amdgcn_raw_buffer_store_b32(
+            __float_as_uint(acc[rr][0] + bs[0]), rs,
+            noff[rr] >= 0 && cg == 0 ? noff[rr] * 4 : CT_OOB, 0, 0);
+    } else {
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(Y, g, z, 2 * NC);
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          float v = acc[i] + bias[4 * cg + i];
-          if (!(v > 0.f)) v *= SLOPE;
-          o[i] = (short)f2bf(v);
+      for (int rr = 0; rr < CT_RW; ++rr) {
+        v4f v;
+        if (MODE == FWD_HIDDEN) {
+#pragma unroll
+          for (int i = 0; i < 4; ++i) {
+            const float u = acc[rr][i] + bs[i];
+            v[i] = fmaxf(u, SLOPE * u);  // leaky ReLU (slope < 1)
+          }
+        } else {
+          const v4s yv = __builtin_bit_cast(v4s, ypz[rr]);
+#pragma unroll
+          for (int i = 0; i < 4; ++i)
+            v[i] = bf2f((uint16_t)yv[i]) > 0.f ? acc[rr][i] : SLOPE * acc[rr][i];
         }
-        *reinterpret_cast<v4s *>(Y + n * NC + 4 * cg) = o;
-      } else {
-        const v4s yp = load4(Yprev + n * NC + 4 * cg);
-        v4s o;
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-          o[i] = (short)f2bf(bf2f((uint16_t)yp[i]) > 0.f ? acc[i] : SLOPE * acc[i]);
-        *reinterpret_cast<v4s *>(Y + n * NC + 4 * cg) = o;
+        // round to nearest even, two values per conversion
+        v2u o;
+        o[0] = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{v[0], v[1]}, v2bf));
+        o[1] = __builtin_bit_cast(unsigned, __builtin_convertvector(v2f{v[2], v[3]}, v2bf));
+        __builtin_amdgcn_raw_buffer_store_b64(o, rs,
+                                              noff[rr] >= 0 ? noff[rr] * (2 * NC) + 8 * cg
+                                                            : CT_OOB,
+                                              0, 0);
       }
     }
-#pragma unroll
-    for (int q = 0; q < 9; ++q) {
-      w0[q] = w1[q];
-      w1[q] = w2[q];
-      w2[q] = wn[q];
-    }
+  };
+  // register sets: set s holds the planes p with (p - z0 + 1) % 2 == s
+  v4u R0[CT_NQ], R1[CT_NQ];
+  ld(R0, z0 - 1);
+  ld(R1, z0);
+  st(R0, z0 - 1);
+  st(R1, z0);
+  ld(R0, z0 + 1);
+  ld(R1, z0 + 2);
+  if (MODE == BWD_DATA) {
+    ld_yp(yp[0], z0);
+    ld_yp(yp[1], z0 + 1);
   }
+  // Step z: plane z + 1 into its slot, barrier, loads of plane z + 3 (and
+  // Yprev of plane z + 2), output plane z.  A trip runs two steps, starting
+  // after the first step's barrier, so that the register sets are named
+  // statically and every path into a wait before an LDS write has the same
+  // accesses in flight: the waits count only the accesses issued since the
+  // set was loaded.  An odd last step follows the loop.
+  st(R0, z0 + 1);
+  __syncthreads();
+  int z = z0;
+  for (; z + 1 < z1; z += 2) {
+    ld(R0, z + 3);
+    plane(z, yp[0]);
+    if (MODE == BWD_DATA) ld_yp(yp[0], z + 2);
+    st(R1, z + 2);
+    __syncthreads();
+    ld(R1, z + 4);
+    plane(z + 1, yp[1]);
+    if (MODE == BWD_DATA) ld_yp(yp[1], z + 3);
+    st(R0, z + 3);  // past the chunk (no step follows): zeros into a slot no step reads
+    __syncthreads();
+  }
+  if (z < z1) plane(z, yp[0]);
 }
 
-// z chunk of the layer kernels.  A wave's time grows with the planes it
-// marches plus the two halo planes it loads, and the waves run in rounds of
-// the device's resident slots, so the chunk count minimises
-// ceil(waves / slots) x (planes per chunk + 2).
+// z chunk of the layer kernels.  A workgroup's time grows with the planes it
+// marches plus the two halo planes it loads first, and the workgroups run in
+// rounds of the device's resident slots, so the chunk count minimises
+// ceil(workgroups / slots) x (planes per chunk + 2).
 inline int conv_zchunk(const Grid3 &g) {
-  // resident waves of the layer kernel on the current device (cached per device)
+  // resident workgroups of the layer kernel on the current device (cached per device)
   static std::mutex mu;
   static std::map<int, int> cache;
   int dev = 0, slots = 0;
@@ -191,20 +356,20 @@ inline int conv_zchunk(const Grid3 &g) {
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
             hipSuccess ||
         hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &per_cu, reinterpret_cast<const void *>(&k_conv_z<FWD_HIDDEN>), 256, 0) !=
+            &per_cu, reinterpret_cast<const void *>(&k_conv_t<FWD_HIDDEN>), 256, 0) !=
             hipSuccess)
       return g.naz;
-    slots = std::max(1, per_cu) * 4 * std::max(1, cus);  // waves
+    slots = std::max(1, per_cu) * std::max(1, cus);
     std::lock_guard<std::mutex> lk(mu);
     cache[dev] = slots;
   }
-  const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
+  const int64_t tpc = (int64_t)((g.nax + 15) / 16) * ((g.nay + CT_TY - 1) / CT_TY);
   int best = g.naz;
   int64_t best_cost = -1;
   for (int nch = 1; nch <= g.naz; ++nch) {
     const int zc = (g.naz + nch - 1) / nch;
-    const int64_t waves = ncol * ((g.naz + zc - 1) / zc);
-    const int64_t cost = ((waves + slots - 1) / slots) * (zc + 2);
+    const int64_t wgs = tpc * ((g.naz + zc - 1) / zc);
+    const int64_t cost = ((wgs + slots - 1) / slots) * (zc + 2);
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
       best = zc;
@@ -212,6 +377,7 @@ inline int conv_zchunk(const Grid3 &g) {
   }
   return best;
 }
+
 // z chunk of the weight-gradient march: the (column, chunk) items run in
 // rounds of NWAVE_W waves, so the chunk count minimises
 // ceil(items / NWAVE_W) x (planes per chunk + 2)
@@ -231,9 +397,8 @@ inline int wgrad_zchunk(const Grid3 &g) {
   return best;
 }
 inline dim3 conv_grid(const Grid3 &g, int zc) {
-  const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
-  const int64_t waves = ncol * ((g.naz + zc - 1) / zc);
-  return dim3((unsigned)((waves + 3) / 4));
+  const int64_t tpc = (int64_t)((g.nax + 15) / 16) * ((g.nay + CT_TY - 1) / CT_TY);
+  return dim3((unsigned)(tpc * ((g.naz + zc - 1) / zc)));
 }
 
 // Weight and bias gradients of a layer:
@@ -620,11 +785,11 @@ struct vgpu_net {
     const dim3 grid = conv_grid(g, zc), blk(256);
     const uint16_t *in = X0;
     for (int l = 0; l < NL - 1; ++l) {
-      hipLaunchKernelGGL((k_conv_z<FWD_HIDDEN>), grid, blk, 0, s, g, zc, in, Wb + l * WL,
+      hipLaunchKernelGGL((k_conv_t<FWD_HIDDEN>), grid, blk, 0, s, g, zc, in, Wb + l * WL,
                          B + l * NC, nullptr, Y[l], nullptr);
       in = Y[l];
     }
-    hipLaunchKernelGGL((k_conv_z<FWD_LAST>), grid, blk, 0, s, g, zc, in, Wb + (NL - 1) * WL,
+    hipLaunchKernelGGL((k_conv_t<FWD_LAST>), grid, blk, 0, s, g, zc, in, Wb + (NL - 1) * WL,
                        B + (NL - 1) * NC, nullptr, nullptr, out);
     VGPU_HIP(hipGetLastError());
     return VGPU_OK;
@@ -668,6 +833,11 @@ extern "C" {
 int vgpu_net_create(vgpu_ctx *c, vgpu_net **out) {
   if (!c || !out) return VGPU_ERR_ARG;
   *out = nullptr;
+  // the layer kernels address one node plane per buffer resource
+  if ((int64_t)c->g.nax * c->g.nay * NC * (int64_t)sizeof(uint16_t) >= CT_OOB) {
+    set_error("vgpu_net_create: node planes above 2^30 bytes of 16 BF16 channels");
+    return VGPU_ERR_ARG;
+  }
   VGPU_HIP(hipSetDevice(c->device));
   auto *n = new (std::nothrow) vgpu_net();
   if (!n) return VGPU_ERR_NOMEM;
@@ -844,7 +1014,7 @@ int vgpu_net_train_step_dev(vgpu_net *n, const double *x1, const double *x2,
     if (rc) return rc;
     if (l > 0) {
       uint16_t *dprev = bufs[l & 1];
-      hipLaunchKernelGGL((k_conv_z<BWD_DATA>), grid, blk, 0, s, n->c->g, zc, dz,
+      hipLaunchKernelGGL((k_conv_t<BWD_DATA>), grid, blk, 0, s, n->c->g, zc, dz,
                          n->WbT + l * WL, nullptr, n->Y[l - 1], dprev, nullptr);
       dz = dprev;
     }
@@ -905,11 +1075,11 @@ int vgpu_net_time_layers(vgpu_net *n, int what, int reps, double *avg_ms) {
   auto launch = [&]() -> int {
     switch (what) {
       case 0:
-        hipLaunchKernelGGL((k_conv_z<FWD_HIDDEN>), grid, blk, 0, s, g, zc, n->Y[0],
+        hipLaunchKernelGGL((k_conv_t<FWD_HIDDEN>), grid, blk, 0, s, g, zc, n->Y[0],
                            n->Wb + WL, n->B + NC, nullptr, n->Y[1], nullptr);
         break;
       case 1:
-        hipLaunchKernelGGL((k_conv_z<BWD_DATA>), grid, blk, 0, s, g, zc, n->dZa,
+        hipLaunchKernelGGL((k_conv_t<BWD_DATA>), grid, blk, 0, s, g, zc, n->dZa,
                            n->WbT + WL, nullptr, n->Y[0], n->dZb, nullptr);
         break;
       case 2:
