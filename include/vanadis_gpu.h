@@ -362,6 +362,9 @@ int vgpu_batch_solve_dev(vgpu_batch *batch, int precision, const double *b,
  * output channel, input channel), b [5][16], FP32 on the host. */
 typedef struct vgpu_net vgpu_net;
 int vgpu_net_create(vgpu_ctx *ctx, vgpu_net **out);
+/* may follow vgpu_destroy of the network's context (it waits for the device
+ * and frees the network's own buffers only); every other vgpu_net_* call
+ * needs the context alive */
 int vgpu_net_destroy(vgpu_net *net);
 int vgpu_net_set_params(vgpu_net *net, const float *w, const float *b);
 int vgpu_net_get_params(vgpu_net *net, float *w, float *b);

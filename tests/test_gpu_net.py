@@ -70,15 +70,17 @@ def test_gradients_match_oracle(pkg, po, extrap):
     for layer in range(L):
         r = np.linalg.norm(gw[layer] - dw[layer]) / np.linalg.norm(dw[layer])
         log(f"layer {layer}: |dW gpu - oracle| / |dW| = {r:.2e}")
-        assert r <= 2e-2, (layer, r)
+        assert r <= 5e-3, (layer, r)  # measured 5e-5 (layer 4) .. 2.0e-3 (layer 0)
     rb = np.linalg.norm(gb - db) / np.linalg.norm(db)
-    assert rb <= 2e-2, rb
+    assert rb <= 1e-3, rb  # measured 2.1e-4
     # the prediction is the oracle's forward pass
     net.predict_dev(dev["x1"], dev["x2"], dev["src"], dev["guess"])
     g = ctx.d2h(dev["guess"], p.N)
     base = vec["x1"] + extrap * (vec["x1"] - vec["x2"])
     pred = (g - base) / 0.7
-    assert np.linalg.norm(pred - out) <= 1e-2 * np.linalg.norm(out)
+    rp = np.linalg.norm(pred - out) / np.linalg.norm(out)
+    log(f"prediction: |out gpu - oracle| / |out| = {rp:.2e}")
+    assert rp <= 2e-3  # measured 4.6e-4
     # deterministic: a second pass gives the same gradients bitwise
     net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 0.0)
     gw2, gb2 = net.get_grads()
@@ -237,3 +239,20 @@ def test_warm_starts_and_methods_reach_the_same_states(pkg):
     assert its["extrap2@bicgstab"] <= its["extrap@bicgstab"]
     assert its["extrap3+jacobi8@bicgstab"] < its["extrap2+jacobi@bicgstab"]
     ctx.close()
+
+
+def test_net_destroy_after_its_context(pkg, gpu):
+    """vgpu_net_destroy after vgpu_destroy of the network's context (the order a
+    garbage collector may pick): it waits for the device and frees the
+    network's own buffers only.  Closing a Context closes its networks first."""
+    p = pkg.problem.unit_box(8, 6, 5, lam=1e-3)
+    ctx = pkg.Context.from_problem(p)
+    net = pkg.Net(ctx)
+    assert gpu.vgpu_destroy(ctx.h) == 0
+    ctx.h = None
+    assert gpu.vgpu_net_destroy(net.h) == 0
+    net.h = None
+    ctx2 = pkg.Context.from_problem(p)
+    net2 = pkg.Net(ctx2)
+    ctx2.close()
+    assert net2.h is None and ctx2.h is None

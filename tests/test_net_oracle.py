@@ -98,3 +98,42 @@ def test_bf16_mode_close_to_fp64(po):
     assert abs(l1 - l0) <= 0.05 * l0
     assert np.linalg.norm(o1 - o0) <= 0.05 * np.linalg.norm(o0)
     assert np.linalg.norm(dw1 - dw0) <= 0.1 * np.linalg.norm(dw0)
+
+
+def test_gpu_bars_catch_a_dropped_tap(po):
+    """The GPU gradient test holds each layer's dW to 5e-3 relative L2, the bias
+    gradient to 1e-3 and the prediction to 2e-3 of this oracle (BF16 roundings
+    that fall on the other side of a rounding boundary from the FP64-summed
+    oracle give 5e-5..2e-3 for dW, 2e-4 for the bias, 5e-4 for the prediction).
+    A kernel that dropped one tap of one layer, or mirrored a tap, fails at
+    least one of those bars: planted here in the oracle's weights, on the GPU
+    test's inputs."""
+    nx, ny, nz = 20, 17, 9
+    N = nx * ny * nz
+    rng = np.random.default_rng(2)
+    vec = {k: rng.standard_normal(N) for k in ("x1", "x2", "src", "vx", "vy", "vz", "tgt")}
+    mask = np.zeros(N, dtype=np.int32)
+    rng = np.random.default_rng(3)
+    w = (rng.standard_normal((L, T, C, C)) * 0.12).astype(np.float32)
+    b = (rng.standard_normal((L, C)) * 0.05).astype(np.float32)
+    scale = np.array([1.0, 0.8, 1.2, 1, 1, 1], np.float32)
+    args = (nx, ny, nz, vec["x1"], vec["x2"], vec["src"], vec["vx"], vec["vy"], vec["vz"],
+            vec["tgt"], mask, scale, 0.7)
+    _, dw, db, out = po.net_grads(*args, w, b, quant=True, extrap=1.0)
+    dw = dw.reshape(L, -1)
+
+    def worst(w2):
+        _, dw2, db2, out2 = po.net_grads(*args, w2, b, quant=True, extrap=1.0)
+        dw2 = dw2.reshape(L, -1)
+        r = max(np.linalg.norm(dw2[l] - dw[l]) / np.linalg.norm(dw[l]) / 5e-3 for l in range(L))
+        r = max(r, np.linalg.norm(db2 - db) / np.linalg.norm(db) / 1e-3)
+        return max(r, np.linalg.norm(out2 - out) / np.linalg.norm(out) / 2e-3)
+
+    for layer in range(L):
+        for t in (0, 13, 26):
+            w2 = w.copy()
+            w2[layer, t] = 0.0
+            assert worst(w2) > 1.0, ("dropped", layer, t)
+        w2 = w.copy()
+        w2[layer] = w[layer, ::-1]  # taps mirrored (forward read as backward)
+        assert worst(w2) > 1.0, ("mirrored", layer)

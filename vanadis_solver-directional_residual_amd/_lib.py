@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -228,6 +229,7 @@ class Context:
                             None if iss_arr is None else iss_arr.ctypes.data_as(_vp),
                             _i32(bc), C.byref(h)), "vgpu_create")
         self._keep = iss_arr
+        self._children = weakref.WeakSet()  # nets and batches built on this context
         self.h = h
         self.nax, self.nay, self.naz = nax, nay, naz
         self.N = nax * nay * naz
@@ -241,6 +243,10 @@ class Context:
         return c
 
     def close(self):
+        # networks and batches built on this context go first (a collector
+        # may finalise the context before them)
+        for child in list(getattr(self, "_children", ())):
+            child.close()
         if getattr(self, "h", None):
             load().vgpu_destroy(self.h)
             self.h = None
@@ -422,6 +428,8 @@ class Batch:
         h = _vp()
         check(load().vgpu_batch_create(len(self.ctxs), arr, C.byref(h)), "batch_create")
         self.h = h
+        for c in self.ctxs:
+            c._children.add(self)
 
     def close(self):
         if getattr(self, "h", None):
@@ -503,6 +511,7 @@ class Net:
         check(load().vgpu_net_create(ctx.h, C.byref(h)), "net_create")
         self.h = h
         self.ctx = ctx
+        ctx._children.add(self)
 
     def close(self):
         if getattr(self, "h", None):

@@ -749,6 +749,7 @@ inline dim3 grid1(int64_t n, int bs = 256) {
 
 struct vgpu_net {
   vgpu_ctx *c = nullptr;
+  int device = 0;  // the context's device (destroy does not touch the context)
   int64_t N = 0, nseg = 0, count = 0;
   uint16_t *X0 = nullptr, *Y[NL - 1] = {}, *dZ4 = nullptr, *dZa = nullptr, *dZb = nullptr;
   float *out = nullptr;
@@ -842,6 +843,7 @@ int vgpu_net_create(vgpu_ctx *c, vgpu_net **out) {
   auto *n = new (std::nothrow) vgpu_net();
   if (!n) return VGPU_ERR_NOMEM;
   n->c = c;
+  n->device = c->device;
   n->N = c->g.N;
   n->nseg = (int64_t)((c->g.nax + 15) / 16) * c->g.nay * c->g.naz;
   const size_t T = (size_t)n->N * NC * sizeof(uint16_t);
@@ -888,8 +890,10 @@ int vgpu_net_create(vgpu_ctx *c, vgpu_net **out) {
 
 int vgpu_net_destroy(vgpu_net *n) {
   if (!n) return VGPU_ERR_ARG;
-  hipSetDevice(n->c->device);
-  hipStreamSynchronize(n->c->stream);
+  // the context may already be destroyed (its stream with it): wait for the
+  // device instead of the context's stream
+  hipSetDevice(n->device);
+  hipDeviceSynchronize();
   delete n;
   return VGPU_OK;
 }
