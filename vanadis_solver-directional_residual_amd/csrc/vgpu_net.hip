@@ -35,8 +35,7 @@ constexpr int NC = 16;            // channels
 constexpr int NL = 5;             // layers
 constexpr int NTAP = 27;          // 3x3x3 taps, t = dx + 3 dy + 9 dz, offset d-1
 constexpr int WL = NTAP * NC * NC;  // weights per layer
-constexpr int NWAVE_W = 2048;     // waves of the weight-gradient kernel (one round at 2 per SIMD)
-constexpr int NBLK_W = NWAVE_W / 4;  // its workgroups (= partials)
+constexpr int NBLK_W = 512;       // workgroups of the weight-gradient kernel (= partials)
 constexpr float SLOPE = 0.1f;     // leaky ReLU slope for negative inputs
 
 // round to nearest even (NaN kept quiet)
@@ -378,17 +377,28 @@ inline int conv_zchunk(const Grid3 &g) {
   return best;
 }
 
-// z chunk of the weight-gradient march: the (column, chunk) items run in
-// rounds of NWAVE_W waves, so the chunk count minimises
-// ceil(items / NWAVE_W) x (planes per chunk + 2)
+// tile of the weight-gradient kernel (below)
+constexpr int WG_TY = 8;                      // x rows per tile (two per wave)
+constexpr int WG_ROWS = WG_TY + 2;            // staged X rows: y0 - 1 .. y0 + 8
+constexpr int WG_CB = 16 * 2;                 // LDS bytes per (row, channel): 16 nodes
+constexpr int WG_CPY = NC * WG_CB;            // bytes per shifted copy of a staged row
+constexpr int WG_RB = 3 * WG_CPY;             // bytes per staged X row (three copies)
+constexpr int WG_SB = WG_ROWS * WG_RB;        // bytes per X plane slot
+constexpr int WG_DB = WG_TY * WG_CPY;         // bytes per staged dZ plane
+constexpr int WG_XP = WG_ROWS * CT_NS * 2;    // 16-byte X pieces per plane
+constexpr int WG_XQ = (WG_XP + 255) / 256;    // X pieces per thread and plane
+static_assert(WG_TY * 16 * 2 == 256, "one dZ piece per thread and plane");
+// z chunk of the weight-gradient march: the (tile, chunk) items run in
+// rounds of the NBLK_W workgroups, so the chunk count minimises
+// ceil(items / NBLK_W) x (planes per chunk + 2)
 inline int wgrad_zchunk(const Grid3 &g) {
-  const int64_t ncol = (int64_t)((g.nax + 15) / 16) * g.nay;
+  const int64_t tpc = (int64_t)((g.nax + 15) / 16) * ((g.nay + WG_TY - 1) / WG_TY);
   int best = g.naz;
   int64_t best_cost = -1;
   for (int nch = 1; nch <= g.naz; ++nch) {
     const int zc = (g.naz + nch - 1) / nch;
-    const int64_t items = ncol * ((g.naz + zc - 1) / zc);
-    const int64_t cost = ((items + NWAVE_W - 1) / NWAVE_W) * (zc + 2);
+    const int64_t items = tpc * ((g.naz + zc - 1) / zc);
+    const int64_t cost = ((items + NBLK_W - 1) / NBLK_W) * (zc + 2);
     if (best_cost < 0 || cost < best_cost) {
       best_cost = cost;
       best = zc;
@@ -403,122 +413,154 @@ inline dim3 conv_grid(const Grid3 &g, int zc) {
 
 // Weight and bias gradients of a layer:
 //   dW[t][co][ci] = sum_n dZ[n][co] X[n+off_t][ci],  db[co] = sum_n dZ[n][co]
-// as MFMAs over 16 nodes (K): A[co][k] = dZ[node k][co], B[k][ci] =
-// X[node k + off_t][ci].  A lane needs one channel of 4 consecutive nodes,
-// while a node's channels are contiguous in memory, so each wave stages rows
-// in its own LDS area with 8-byte loads and reads the operands back
-// transposed.  A wave marches a column of 16-node x-row segments (segment,
-// y) up a chunk of zc planes: the 3 X rows (y-1, y, y+1) of planes z-1, z,
-// z+1 sit in a ring of three plane slots (plane p in slot p mod 3), each with
-// one halo node either side, so a plane step stages one new plane (loaded a
-// step ahead) and the dZ row.  The node stride in LDS is 40 bytes, so the
-// four lane groups (4 nodes apart) fall on disjoint banks.  Waves take
-// (column, chunk) items w, w + NWAVE_W, ...; the four waves of a workgroup
-// then add their accumulators in wave order through LDS and write one partial
-// per workgroup: part[blk][t][co][ci], partb[blk][lane].
-constexpr int WS_RS = 20;  // LDS node stride (bf16): 16 channels + 4 pad
-constexpr int WS_RN = 18;  // staged nodes per X row: x0 - 1 .. x0 + 16
-constexpr int WS_ROW = WS_RN * WS_RS;
-__global__ __launch_bounds__(256) void k_conv_bwd_w(Grid3 g, int zc,
-                                                    const uint16_t *__restrict__ X,
-                                                    const uint16_t *__restrict__ dZ,
-                                                    float *__restrict__ part,
-                                                    float *__restrict__ partb) {
-  // per wave: 3 plane slots x 3 X rows and the dZ row; reused for the
-  // workgroup's sum
-  __shared__ __align__(16) uint16_t lds[4][10][WS_ROW];
-  const int lane = threadIdx.x & 63, wl = threadIdx.x >> 6;
-  const int w = blockIdx.x * 4 + wl;
-  const int nsx = (g.nax + 15) >> 4;
-  const int64_t ncol = (int64_t)nsx * g.nay;
-  const int64_t nitem = ncol * ((g.naz + zc - 1) / zc);
-  const int col = lane & 15, cg = lane >> 4;
-  uint16_t(*L)[WS_ROW] = lds[wl];
+// as v_mfma_f32_16x16x32_bf16 over 32 nodes (K) at a time, two x-rows of 16:
+// A[co][k] = dZ[node k][co], B[k][ci] = X[node k + off_t][ci].  A lane needs
+// one channel of 8 consecutive nodes, while a node's channels are contiguous
+// in memory, so the planes go to LDS transposed: per staged row and channel
+// the 16 nodes as 32 bytes, the X rows in three copies shifted by one node
+// (dx = -1, 0, +1), so that every operand is one aligned 16-byte read.  A
+// workgroup (4 waves, two rows each) marches a tile of 8 x-rows of 16 nodes
+// up a chunk of zc planes like the layer kernels: X planes z-1, z, z+1 in an
+// LDS ring of four slots, written one step after their loads were issued
+// (two steps of work for a load to arrive), the dZ plane of the step in one
+// of two slots, one barrier per step.  A persistent grid of NBLK_W
+// workgroups takes the (tile, chunk) items blockIdx, blockIdx + NBLK_W, ...;
+// its four waves add their accumulators in wave order through LDS and write
+// one partial per workgroup: part[blk][t][co][ci], partb[blk][lane].
+__global__ __launch_bounds__(256) void k_conv_wg(Grid3 g, int zc,
+                                                 const uint16_t *__restrict__ X,
+                                                 const uint16_t *__restrict__ dZ,
+                                                 float *__restrict__ part,
+                                                 float *__restrict__ partb) {
+  __shared__ __align__(16) unsigned char lds[4 * WG_SB + 2 * WG_DB];
+  static_assert(sizeof(lds) >= (NTAP * 4 * 64 + 64) * sizeof(float), "LDS sum area");
+  const int tid = threadIdx.x, lane = tid & 63, wl = tid >> 6;
+  const int col = lane & 15, grp = lane >> 4;
+  const int ntx = (g.nax + 15) >> 4, nty = (g.nay + WG_TY - 1) / WG_TY;
+  const int tpc = ntx * nty;
+  const int nitem = tpc * ((g.naz + zc - 1) / zc);
   v4f acc[NTAP];
 #pragma unroll
   for (int t = 0; t < NTAP; ++t) acc[t] = v4f{0.f, 0.f, 0.f, 0.f};
   float bsum = 0.f;
-  auto node = [&](const uint16_t *T, int xx, int yy, int zz) -> v4s {
-    v4s v = {0, 0, 0, 0};
-    if (inside(g, xx, yy, zz))
-      v = load4(T + (xx + (int64_t)g.nax * (yy + (int64_t)g.nay * zz)) * NC + 4 * cg);
-    return v;
-  };
-  for (int64_t it = w; it < nitem; it += NWAVE_W) {
-    const int64_t cidx = it % ncol;
-    const int x0 = (int)(cidx % nsx) * 16, y = (int)(cidx / nsx);
-    const int z0 = (int)(it / ncol) * zc, z1 = min(z0 + zc, g.naz);
-    // rows of the next plane (staged node col+1; halo nodes 0 and 17 by
-    // lanes col 0 and 1) and the next dZ row (node col)
-    v4s nr[3], nh[3], nz;
-    auto load_plane = [&](int zz) {
+  // the wave's operand positions: rows 2 wl + grp / 2, channel col, nodes
+  // 8 (grp % 2) .. + 7
+  const int orow = 2 * wl + (grp >> 1);
+  const int obyte = col * WG_CB + 16 * (grp & 1);
+  for (int it = xcd_tile(blockIdx.x, gridDim.x); it < nitem; it += gridDim.x) {
+    const int chunk = it / tpc, tr = it % tpc;
+    const int x0 = (tr % ntx) * 16, y0 = (tr / ntx) * WG_TY;
+    const int z0 = chunk * zc, z1 = min(z0 + zc, g.naz);
+    // X pieces: piece c is channel half c % 2 of staged node (c % 36) / 2
+    // (node x0 - 1 + s) in staged row c / 36; dZ piece: channel half tid % 2
+    // of node x0 + (tid % 32) / 2 in row y0 + tid / 32.  Offsets past the
+    // plane for nodes outside the grid (loads return zeros).
+    int goff[WG_XQ], xs[WG_XQ], xr[WG_XQ], xh[WG_XQ];
+    bool pin[WG_XQ];
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        nr[r] = node(X, x0 + col, y + r - 1, zz);
-        nh[r] = col < 2 ? node(X, col == 0 ? x0 - 1 : x0 + 16, y + r - 1, zz)
-                        : v4s{0, 0, 0, 0};
+    for (int q = 0; q < WG_XQ; ++q) {
+      const int c = tid + 256 * q;
+      xr[q] = c / (2 * CT_NS);
+      xs[q] = (c % (2 * CT_NS)) >> 1;
+      xh[q] = c & 1;
+      const int xx = x0 - 1 + xs[q], yy = y0 - 1 + xr[q];
+      pin[q] = c < WG_XP;
+      goff[q] = pin[q] && xx >= 0 && xx < g.nax && yy >= 0 && yy < g.nay
+                    ? (xx + g.nax * yy) * (2 * NC) + 16 * xh[q]
+                    : CT_OOB;
+    }
+    const int dr = tid >> 5, dsn = (tid & 31) >> 1, dh = tid & 1;
+    const int doff = x0 + dsn < g.nax && y0 + dr < g.nay
+                         ? ((x0 + dsn) + g.nax * (y0 + dr)) * (2 * NC) + 16 * dh
+                         : CT_OOB;
+    auto ldx = [&](v4u *R, int p) {
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(X, g, p <= z1 ? p : -1, 2 * NC);
+#pragma unroll
+      for (int q = 0; q < WG_XQ; ++q)
+        R[q] = __builtin_amdgcn_raw_buffer_load_b128(rs, goff[q], 0, 0);
+    };
+    auto ldd = [&](v4u &D, int p) {
+      const __amdgpu_buffer_rsrc_t rs = plane_rsrc(dZ, g, p < z1 ? p : -1, 2 * NC);
+      D = __builtin_amdgcn_raw_buffer_load_b128(rs, doff, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);  // issued here, ahead of the step's MFMAs
+    };
+    // transposed writes: channel 8 h + c of staged node s to each copy d
+    // that holds it (copy d, position j = s - d, 0 <= j < 16)
+    auto stx = [&](const v4u *R, int p) {
+      unsigned char *base = lds + ((p - z0 + 1) & 3) * WG_SB;
+#pragma unroll
+      for (int q = 0; q < WG_XQ; ++q) {
+        if (!pin[q]) continue;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const int j = xs[q] - d;
+          if (j < 0 || j > 15) continue;
+          uint16_t *dst = reinterpret_cast<uint16_t *>(base + xr[q] * WG_RB + d * WG_CPY +
+                                                       8 * xh[q] * WG_CB) + j;
+#pragma unroll
+          for (int c = 0; c < 8; ++c) dst[c * (WG_CB / 2)] = (uint16_t)(R[q][c >> 1] >> (16 * (c & 1)));
+        }
       }
     };
-    auto stage_plane = [&](int zz) {
-      const int sl = ((zz % 3) + 3) % 3;
+    auto std_ = [&](const v4u &D, int p) {
+      uint16_t *dst = reinterpret_cast<uint16_t *>(lds + 4 * WG_SB + ((p - z0) & 1) * WG_DB +
+                                                   dr * WG_CPY + 8 * dh * WG_CB) + dsn;
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        *reinterpret_cast<v4s *>(&L[sl * 3 + r][(col + 1) * WS_RS + 4 * cg]) = nr[r];
-        if (col < 2)
-          *reinterpret_cast<v4s *>(&L[sl * 3 + r][(col == 0 ? 0 : WS_RN - 1) * WS_RS + 4 * cg]) =
-              nh[r];
-      }
+      for (int c = 0; c < 8; ++c) dst[c * (WG_CB / 2)] = (uint16_t)(D[c >> 1] >> (16 * (c & 1)));
     };
-    load_plane(z0 - 1);
-    stage_plane(z0 - 1);
-    load_plane(z0);
-    stage_plane(z0);
-    load_plane(z0 + 1);
-    nz = node(dZ, x0 + col, y, z0);
-    for (int z = z0; z < z1; ++z) {
-      stage_plane(z + 1);  // the slot of plane z - 2
-      *reinterpret_cast<v4s *>(&L[9][col * WS_RS + 4 * cg]) = nz;
-      if (z + 1 < z1) {  // the next step's plane and dZ row, one step ahead
-        load_plane(z + 2);
-        nz = node(dZ, x0 + col, y, z + 1);
-      }
-      // other lanes of this wave read what these lanes wrote: LDS
-      // operations of one wave complete in order; the fence and wave
-      // barrier keep the compiler from moving the reads above the writes
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      // A[co = col][k = node 4cg+i] (zero beyond the grid: the row load
-      // returned zeros there)
-      v4s a;
+    auto step = [&](int z) {
+      const v8s a = *reinterpret_cast<const v8s *>(lds + 4 * WG_SB + ((z - z0) & 1) * WG_DB +
+                                                   orow * WG_CPY + obyte);
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const uint16_t h = L[9][(4 * cg + i) * WS_RS + col];
-        a[i] = (short)h;
-        bsum += bf2f(h);
-      }
-      const uint16_t *pl[3];  // slots of planes z-1, z, z+1
+      for (int i = 0; i < 8; ++i) bsum += bf2f((uint16_t)a[i]);
+      const v8bf av = __builtin_bit_cast(v8bf, a);
+      int sb[3];  // slots of planes z - 1, z, z + 1
 #pragma unroll
-      for (int d = 0; d < 3; ++d) pl[d] = &L[(((z + d - 1) % 3 + 3) % 3) * 3][0];
+      for (int d = 0; d < 3; ++d) sb[d] = ((z - z0 + d) & 3) * WG_SB;
 #pragma unroll
       for (int t = 0; t < NTAP; ++t) {
-        const int dx = t % 3 - 1, r = (t / 3) % 3, d = t / 9;
-        const uint16_t *row = pl[d] + r * WS_ROW;
-        v4s b;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) b[i] = (short)row[(4 * cg + i + dx + 1) * WS_RS + col];
-        acc[t] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[t], 0, 0, 0);
+        const int dx = t % 3, dy = (t / 3) % 3, dz = t / 9;
+        const v8s b = *reinterpret_cast<const v8s *>(lds + sb[dz] + (orow + dy) * WG_RB +
+                                                     dx * WG_CPY + obyte);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, __builtin_bit_cast(v8bf, b),
+                                                         acc[t], 0, 0, 0);
       }
-      // the next step's stores stay behind this step's reads
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    v4u R0[WG_XQ], R1[WG_XQ], D0, D1;
+    ldx(R0, z0 - 1);
+    ldx(R1, z0);
+    ldd(D0, z0);
+    ldd(D1, z0 + 1);
+    __syncthreads();  // the previous item's reads are done
+    stx(R0, z0 - 1);
+    stx(R1, z0);
+    ldx(R0, z0 + 1);
+    ldx(R1, z0 + 2);
+    stx(R0, z0 + 1);
+    std_(D0, z0);
+    __syncthreads();
+    // step z: loads of X plane z + 3 and dZ plane z + 2, the MFMAs of plane
+    // z, then X plane z + 2 and dZ plane z + 1 into their slots and the
+    // barrier; two steps per trip name the register sets statically
+    int z = z0;
+    for (; z + 1 < z1; z += 2) {
+      ldx(R0, z + 3);
+      ldd(D0, z + 2);
+      step(z);
+      stx(R1, z + 2);
+      std_(D1, z + 1);
+      __syncthreads();
+      ldx(R1, z + 4);
+      ldd(D1, z + 3);
+      step(z + 1);
+      stx(R0, z + 3);  // past the chunk (no step follows): zeros into a slot no step reads
+      std_(D0, z + 2);
+      __syncthreads();
     }
+    if (z < z1) step(z);
   }
-  // the workgroup's sum in wave order through LDS (the staging area is free
-  // once every wave has passed the barrier)
-  float *red = reinterpret_cast<float *>(&lds[0][0][0]);  // [NTAP][4][64] + [64]
-  static_assert(sizeof(lds) >= (NTAP * 4 * 64 + 64) * sizeof(float), "LDS sum area");
+  // the workgroup's sum in wave order through LDS
+  float *red = reinterpret_cast<float *>(lds);  // [NTAP][4][64] + [64]
   __syncthreads();
   for (int v = 0; v < 4; ++v) {
     if (wl == v) {
@@ -817,9 +859,8 @@ struct vgpu_net {
 
   int weight_grad(int l, const uint16_t *Xin, const uint16_t *dZl) {
     hipStream_t s = c->stream;
-    hipLaunchKernelGGL(k_conv_bwd_w, dim3(NBLK_W), dim3(256), 0, s, c->g, wgrad_zchunk(c->g),
-                       Xin, dZl,
-                       part, partb);
+    hipLaunchKernelGGL(k_conv_wg, dim3(NBLK_W), dim3(256), 0, s, c->g, wgrad_zchunk(c->g), Xin,
+                       dZl, part, partb);
     hipLaunchKernelGGL(k_wsum1, dim3((WS_P2 + 255) / 256, WS_SPLIT), dim3(256), 0, s, part,
                        partb, part2);
     hipLaunchKernelGGL(k_wsum2, dim3((WL + NC + 255) / 256), dim3(256), 0, s, part2,
