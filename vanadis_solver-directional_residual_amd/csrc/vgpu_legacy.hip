@@ -25,7 +25,16 @@ namespace vgpu {
 namespace {
 
 // Element products on stored BB (row-major per element: BB[64e + 8i + j]).
-// y1 += A_e x1 (rows), y2 += A_e^T x2 (columns); masked in and out.
+// y1 += A_e x1 (rows), y2 += A_e^T x2 (columns); masked in and out.  Eight
+// lanes per element: lane j of an element's group handles local node j.  It
+// loads the node's inputs, row j of A_e (four 16-byte loads: the group reads
+// the element's 512 bytes in order) and column j (eight 8-byte loads: the
+// group reads one 64-byte row per load, from cache), takes the other nodes'
+// inputs from its neighbours in the group, and forms
+//   y1_j = sum_i A[j][i] x1_i,   y2_j = sum_i A[i][j] x2_i   (i = 0..7 in order),
+// the same sums as the reference's per-element loops (MATVEC_C_cu /
+// MTTVEC_C_cu, cudatest.cu:184-227), so the coloured results do not depend on
+// the launch shape.
 template <bool ATOM, bool DUAL>
 __global__ __launch_bounds__(256) void k_bb(int64_t E, const int *__restrict__ iss,
                                             const double *__restrict__ bb,
@@ -35,51 +44,49 @@ __global__ __launch_bounds__(256) void k_bb(int64_t E, const int *__restrict__ i
                                             double *y1, const double *x2,
                                             double *y2, const int *done) {
   if (done && *done) return;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < count;
-       t += (int64_t)gridDim.x * blockDim.x) {
+  const int j = threadIdx.x & 7, gbase = threadIdx.x & 63 & ~7;
+  const int64_t step = (int64_t)gridDim.x * (blockDim.x >> 3);
+  // the eight lanes of a group share t, so they run the loop together
+  for (int64_t t = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 3; t < count;
+       t += step) {
     const int64_t e = elems ? elems[t] : t;
-    int nd[8];
-    bool mk[8];
-    double xa[8], xb[8], ya[8], yb[8];
+    const int nd = iss[j * E + e] - 1;
+    const bool mk = mask[nd] != 0;
+    const double xa = mk ? 0.0 : x1[nd];
+    const double xb = (DUAL && !mk) ? x2[nd] : 0.0;
+    const double *A = bb + 64 * e;
+    double r[8], c[8];
+    const double2 *row = reinterpret_cast<const double2 *>(A + 8 * j);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      nd[j] = iss[j * E + e] - 1;
-      mk[j] = mask[nd[j]] != 0;
-      xa[j] = mk[j] ? 0.0 : x1[nd[j]];
-      xb[j] = (DUAL && !mk[j]) ? x2[nd[j]] : 0.0;
-      yb[j] = 0.0;
+    for (int h = 0; h < 4; ++h) {
+      const double2 v = row[h];
+      r[2 * h] = v.x;
+      r[2 * h + 1] = v.y;
     }
-    const double2 *row = reinterpret_cast<const double2 *>(bb + 64 * e);
+    if (DUAL)
+#pragma unroll
+      for (int i = 0; i < 8; ++i) c[i] = A[8 * i + j];
+    double ya = 0.0, yb = 0.0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      double a[8];
-#pragma unroll
-      for (int h = 0; h < 4; ++h) {
-        const double2 v = row[4 * i + h];
-        a[2 * h] = v.x;
-        a[2 * h + 1] = v.y;
-      }
-      double s = 0.0;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) s += a[j] * xa[j];
-      ya[i] = s;
-      if (DUAL) {
-#pragma unroll
-        for (int j = 0; j < 8; ++j) yb[j] += a[j] * xb[i];
-      }
+      ya += r[i] * __shfl(xa, gbase + i, 64);
+      if (DUAL) yb += c[i] * __shfl(xb, gbase + i, 64);
     }
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      if (mk[i]) continue;
+    if (!mk) {
       if (ATOM) {
-        atomicAdd(&y1[nd[i]], ya[i]);
-        if (DUAL) atomicAdd(&y2[nd[i]], yb[i]);
+        atomicAdd(&y1[nd], ya);
+        if (DUAL) atomicAdd(&y2[nd], yb);
       } else {
-        y1[nd[i]] += ya[i];
-        if (DUAL) y2[nd[i]] += yb[i];
+        y1[nd] += ya;
+        if (DUAL) y2[nd] += yb;
       }
     }
   }
+}
+
+// workgroups of a k_bb launch over count elements (eight lanes each)
+inline unsigned bb_grid(int64_t count) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (8 * count + 255) / 256));
 }
 
 __global__ void k_zero(const int *done, double *a, double *b, int64_t n) {
@@ -184,7 +191,7 @@ struct Legacy {
                   const int *done, hipStream_t st) {
     const bool dual = pp != nullptr;
     if (scatter == VGPU_SCATTER_ATOMIC) {
-      const unsigned g = (unsigned)std::min<int64_t>(4096, (E + 255) / 256);
+      const unsigned g = bb_grid(E);
       if (dual)
         hipLaunchKernelGGL((k_bb<true, true>), dim3(g), dim3(256), 0, st, E, iss,
                            bb, mask, nullptr, E, p, z, pp, zz, done);
@@ -194,7 +201,7 @@ struct Legacy {
     } else {
       for (int c = 0; c < 8; ++c) {
         if (ccnt[c] == 0) continue;
-        const unsigned g = (unsigned)std::min<int64_t>(4096, (ccnt[c] + 255) / 256);
+        const unsigned g = bb_grid(ccnt[c]);
         if (dual)
           hipLaunchKernelGGL((k_bb<false, true>), dim3(g), dim3(256), 0, st, E,
                              iss, bb, mask, celems + coff[c], (int64_t)ccnt[c], p,
@@ -407,7 +414,7 @@ int legacy_solve_noalloc(int N, const double *b, double *x, double tol, int itma
                   const int *done) {
     const bool two = x2 != nullptr;
     if (scatter == VGPU_SCATTER_ATOMIC) {
-      const unsigned g = (unsigned)std::min<int64_t>(4096, (E + 255) / 256);
+      const unsigned g = bb_grid(E);
       if (two)
         hipLaunchKernelGGL((k_bb<true, true>), dim3(g), dim3(256), 0, s, E, iss, bb, mask,
                            nullptr, E, x1, y1, x2, y2, done);
@@ -417,7 +424,7 @@ int legacy_solve_noalloc(int N, const double *b, double *x, double tol, int itma
     } else {
       for (int c = 0; c < 8; ++c) {
         if (ccnt[c] == 0) continue;
-        const unsigned g = (unsigned)std::min<int64_t>(4096, (ccnt[c] + 255) / 256);
+        const unsigned g = bb_grid(ccnt[c]);
         if (two)
           hipLaunchKernelGGL((k_bb<false, true>), dim3(g), dim3(256), 0, s, E, iss, bb, mask,
                              elems + coff[c], (int64_t)ccnt[c], x1, y1, x2, y2, done);
