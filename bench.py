@@ -846,6 +846,22 @@ def main():
                                 "frac": round(dual_gbs / HBM_PEAK_GBS, 4),
                                 "kernel": "k_owner<double,double,dual>",
                                 "algorithmic_bytes_per_launch": dual_bytes},
+        # the same launches priced at the bytes a general unstructured layout
+        # moves (x, y, node coordinates, mask; 8-node connectivity and
+        # velocity + diffusivity per element: 41 N + 64 E per single apply,
+        # 193 N + 64 E per fused BiCG iteration); this structured kernel reads
+        # no coordinates or connectivity, so its own compulsory bytes (the
+        # rooflines above) are the lower, stricter count
+        "general_layout_bytes": {
+            "single_apply": {"bytes": 41 * N + 64 * E, "ms": round(t_single, 5),
+                             "GBps": round((41 * N + 64 * E) / (t_single * 1e-3) / 1e9, 1),
+                             "frac": round((41 * N + 64 * E) / (t_single * 1e-3) / 1e9
+                                           / HBM_PEAK_GBS, 4)},
+            "bicg_iteration": {"bytes": 193 * N + 64 * E, "ms": round(ms_step / iters[-1], 5),
+                               "GBps": round((193 * N + 64 * E) / (ms_step / iters[-1] * 1e-3)
+                                             / 1e9, 1),
+                               "frac": round((193 * N + 64 * E) / (ms_step / iters[-1] * 1e-3)
+                                             / 1e9 / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
         "rel_l2_vs_oracle": checks,
         "legacy_drop_in": legacy,
