@@ -87,7 +87,7 @@ def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
     dbcg_simple + matvec_0/mttvec_0, VANADIS.FOR:977-1055, 2149-2245, over
     element matrices assembled like przest3d, 713-771) timed on the host at
     the 128x128x64 workload, serial (as the reference's solver) and with the
-    8-colour OpenMP products; the oracle as the checker of the GPU solutions
+    8-colour OpenMP products; the oracle as the reference for the GPU solutions
     (rel-L2 against its FP64 BiCG run to 1e-12: live for 32x32x16 and
     128x128x64, from the committed sketches for 256x256x64, the batches and
     the time stepping); and the legacy drop-in kernel_dbcg_simple_ on the
@@ -99,7 +99,7 @@ def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
     T = host["threads_used"]
     out = {"host": host}
     progress(f"cpu: {host['model']}, {host['physical_cores']} physical cores, {T} threads")
-    # --- 128x128x64: assembly, serial sample, OpenMP solve, checker solve
+    # --- 128x128x64: assembly, serial sample, OpenMP solve, reference solve
     t0 = time.perf_counter()
     bb, b, d = po.assemble(p2, threads=T)
     t_asm = time.perf_counter() - t0
@@ -148,14 +148,14 @@ def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
                              f"fixed-order OpenMP reductions, whole solve to rtol 1e-8 from "
                              f"x0 = 0 on {T} threads, OMP_PROC_BIND "
                              f"{os.environ.get('OMP_PROC_BIND', 'unset')}"}})
-    # --- the checker: rel-L2 of every GPU solution against the oracle
+    # --- rel-L2 of every GPU solution against the oracle
     rel = {}
     n2 = np.linalg.norm(x_o)
     for (cfg, name), xs in sols.items():
         if cfg == "c2":
             rel[f"c2/{name}"] = float(np.linalg.norm(xs - x_o) / n2)
     rel["c2/cpu_openmp_bicg"] = float(np.linalg.norm(x_omp - x_o) / n2)
-    checker = {"oracle_c2": f"oracle FP64 BiCG to 1e-12 ({it_o} iterations, OpenMP), live"}
+    refs = {"oracle_c2": f"oracle FP64 BiCG to 1e-12 ({it_o} iterations, OpenMP), live"}
     # legacy drop-in: kernel_dbcg_simple_ on these element matrices (the
     # unchanged host's call), whole call and its phases
     leg = {}
@@ -175,7 +175,7 @@ def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
                  f"{up * 1e3:.1f}, solve {so * 1e3:.1f}, download {down * 1e3:.1f})")
     pkg._lib.load().vgpu_legacy_release()
     del bb
-    # --- 32x32x16: serial and OpenMP to rtol, live checker
+    # --- 32x32x16: serial and OpenMP to rtol, live reference solve
     p1 = pkg.problem.config_problem("c1")
     bb1, b1, d1 = po.assemble(p1, threads=T)
     b1, d1, z1 = po.impose_mask(p1.mask, b1, d1, np.zeros(p1.N))
@@ -193,7 +193,7 @@ def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
     for (cfg, name), xs in sols.items():
         if cfg == "c1":
             rel[f"c1/{name}"] = float(np.linalg.norm(xs - x1o) / np.linalg.norm(x1o))
-    checker["oracle_c1"] = "oracle FP64 BiCG to 1e-12 (OpenMP), live"
+    refs["oracle_c1"] = "oracle FP64 BiCG to 1e-12 (OpenMP), live"
     # --- sketches of the oracle's solutions of the large workloads
     sk = po.load_sketches()
     for (cfg, name), xs in sols.items():
@@ -202,11 +202,11 @@ def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
         if cfg in ("c4a", "c4b") and cfg in sk:
             est = [po.sketch_rel_l2(xm, e) for xm, e in zip(xs, sk[cfg]["members"])]
             rel[f"{cfg}/{name}_max_over_members"] = max(est)
-    checker["sketched"] = ("256x256x64, the batches and the time stepping: estimates from 32 "
+    refs["sketched"] = ("256x256x64, the batches and the time stepping: estimates from 32 "
                            "(batch members: 16) Gaussian projections of the oracle's FP64 "
                            "solution to 1e-12 (tests/golden/oracle_sketch.json; +-27 % / "
                            "+-41 % at 99 %)")
-    return out, {"rel_l2": rel, "checker": checker}, leg
+    return out, {"rel_l2": rel, "reference_solves": refs}, leg
 
 
 def measured_traffic(tuned, kernel="iteration_launch"):
@@ -611,7 +611,7 @@ def main():
             raise RuntimeError(f"solve failed rc={rc} it={it} err={err}")
         return it, err, sec
 
-    sols = {}  # (config, name) -> solution, for the oracle checker at the end
+    sols = {}  # (config, name) -> solution, compared with the oracle at the end
     progress("warmup")
     for _ in range(a.warmup):
         solve(pkg.FP64)

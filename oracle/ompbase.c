@@ -12,7 +12,7 @@
  * order (bitwise orc_assemble).
  *
  * TEST INFRASTRUCTURE ONLY: the OpenMP CPU baseline of the benchmark and a
- * faster checker for large systems; never part of the product path.
+ * faster reference for large systems; never part of the product path.
  */
 #include <math.h>
 #include <omp.h>

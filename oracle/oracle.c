@@ -2,7 +2,7 @@
  * CPU oracle: plain-C restatement of the reference's element operator,
  * assembly, masked EbE products, BiCG, mass integral and diagnostics.
  *
- * TEST INFRASTRUCTURE ONLY (see oracle.h): checker and CPU baseline, never part
+ * TEST INFRASTRUCTURE ONLY (see oracle.h): reference values for the tests and the CPU baseline, never part
  * of the product path.  Built with -ffp-contract=off so every expression
  * rounds like the reference built by gfortran/flang on baseline x86-64.
  */

@@ -2,7 +2,7 @@
  * CPU oracle for the Vanadis element-by-element DR-HEX8 operator and BiCG.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
+ * cpu_baseline leg of bench.py may load this library, and only as the reference the GPU results are compared with
  * (or the timed CPU baseline).  The product (libvanadis_gpu.so) never links it.
  *
  * Every routine restates one reference routine in plain C, in the reference's

@@ -2,7 +2,7 @@
 flang-built reference (oracle/_ref/*.so, when present).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker or the timed CPU baseline.  The
+cpu_baseline leg of bench.py, as the reference the GPU results are compared with or the timed CPU baseline.  The
 product path never imports this module.
 """
 from __future__ import annotations
@@ -214,7 +214,7 @@ def bicgstab_jacobi(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask, threa
     preconditioned BiCGSTAB of Barrett et al., Templates, 1994, sec. 2.3.8 with
     M = D), with the stopping rule of dbcg_simple (recursive ||r||/||b|| <= tol,
     VANADIS.FOR:977-1055, cudatest.cu:491-553) tested before each iteration.
-    Not in the reference: the checker of the library's BiCGSTAB method.
+    Not in the reference: the reference solution for the library's BiCGSTAB method.
     Shadow residual r~ = r0.  Returns (x, iterations, err, status) with
     status 1 converged, 2 itmax, 3 breakdown."""
     if threads > 0:
