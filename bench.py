@@ -147,7 +147,8 @@ def cpu_baseline_and_checks(pkg, p2, sols, legacy_reps=2, serial_iters=40):
                    "sample": f"the oracle's dbcg_simple with 8-colour OpenMP products and "
                              f"fixed-order OpenMP reductions, whole solve to rtol 1e-8 from "
                              f"x0 = 0 on {T} threads, OMP_PROC_BIND "
-                             f"{os.environ.get('OMP_PROC_BIND', 'unset')}"}})
+                             f"{os.environ.get('OMP_PROC_BIND', 'unset')}, OMP_PLACES "
+                             f"{os.environ.get('OMP_PLACES', 'unset')}"}})
     # --- rel-L2 of every GPU solution against the oracle
     rel = {}
     n2 = np.linalg.norm(x_o)
@@ -568,6 +569,10 @@ def main():
               "(the slab decomposition is a design, see the sharding notes)", file=sys.stderr)
         sys.exit(2)
 
+    # the CPU leg's OpenMP threads one per core, packed (read when the oracle's
+    # OpenMP runtime starts; a caller's own settings win)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     import __graft_entry__ as ge
     pkg = ge.load_package()
     pkg.load()
