@@ -80,9 +80,12 @@ __device__ __forceinline__ bool inside(const Grid3 &g, int x, int y, int z) {
 // v_mfma_f32_16x16x32_bf16 with A = the two taps' weights (16 output x 2 x 16
 // input channels, in registers for the whole march) and B = the 16 nodes'
 // input channels at the two taps' offsets, 16 bytes per lane from LDS.  The
-// 27 taps make 13 pairs and tap 26 alone (zero weights beside it).  An output
-// row's MFMAs run in the same order whatever the chunking, so the results do
-// not depend on it.
+// 27 taps make 13 pairs and tap 26 alone (zero weights beside it); in 12 of
+// the pairs both taps have the same dy, so row r at dy and row r + 1 at
+// dy - 1 read the same operand and a wave's two rows need 20 operand reads
+// instead of 28.  An
+// output row's MFMAs run in the same order whatever the chunking, so the
+// results do not depend on it.
 //   FWD_HIDDEN: Y[n][co] = leaky(b[co] + sum_t sum_ci W[t][co][ci] X[n+off_t][ci])
 //   FWD_LAST:   out[n] = b[0] + sum_t sum_ci W[t][0][ci] X[n+off_t][ci] (FP32)
 //   BWD_DATA:   dZprev[n][ci] = leaky'(Yprev[n][ci]) sum_t sum_co WT[t][ci][co]
@@ -90,7 +93,7 @@ __device__ __forceinline__ bool inside(const Grid3 &g, int x, int y, int z) {
 // (off_t = (t%3-1, t/3%3-1, t/9-1)).  MFMA operand layout (16x16x32, lane l,
 // i = 0..7): A[l%16][8(l/16)+i], B[8(l/16)+i][l%16], D[4(l/16)+i][l%16]; the
 // reduction index k = 8(l/16)+i stands for input channel 8((l/16)%2)+i of
-// tap 2j+(l/16)/2 in A and B alike.  In LDS a node takes its 32 bytes of
+// tap ct_lo(j) (l < 32) or ct_hi(j) (l >= 32) in A and B alike.  In LDS a node takes its 32 bytes of
 // channels, unpadded: the 16-byte reads of 16 consecutive nodes by each
 // quarter wave run free of bank conflicts on gfx950 (its LDS conflict counter
 // reads zero; a 48-byte node stride doubled the LDS busy cycles).  Global
@@ -123,6 +126,19 @@ template <int MODE>
 __device__ __forceinline__ constexpr int ct_dz(int t) {
   return MODE == BWD_DATA ? 2 - t / 9 : t / 9;
 }
+// tap pairs of one MFMA, in MFMA order (lo taps on lanes 0..31, hi taps on
+// 32..63; t = dx + 3 dy + 9 dz): j < 9 the z neighbours (dx, dy, 0) and
+// (dx, dy, 1) with dx = j / 3, dy = j % 3; j = 9..11 the x neighbours (0, dy, 2)
+// and (1, dy, 2) with dy = j - 9 -- in both, the two taps share dy, so row r at
+// dy and row r + 1 at dy - 1 read the same operand; j = 12 (2, 0, 2) and
+// (2, 1, 2); j = 13 (2, 2, 2) alone (-1: zero weights)
+__host__ __device__ constexpr int ct_lo(int j) {
+  return j < 9 ? j / 3 + 3 * (j % 3) : j < 12 ? 3 * (j - 9) + 18 : j == 12 ? 20 : 26;
+}
+__host__ __device__ constexpr int ct_hi(int j) {
+  return j < 9 ? ct_lo(j) + 9 : j < 12 ? ct_lo(j) + 1 : j == 12 ? 23 : -1;
+}
+constexpr int CT_NG = 4;  // groups of three pairs that share their operands
 template <int MODE>
 __device__ __forceinline__ constexpr int ct_dy(int t) {
   return MODE == BWD_DATA ? 2 - (t / 3) % 3 : (t / 3) % 3;
@@ -166,12 +182,13 @@ __global__ __launch_bounds__(256) void k_conv_t(Grid3 g, int zc, const uint16_t 
   const int z0 = chunk * zc, z1 = min(z0 + zc, g.naz);
   const int col = lane & 15, cg = lane >> 4, hi = cg >> 1;
   const v8s zero8 = {0, 0, 0, 0, 0, 0, 0, 0};
-  // A: weights of tap 2j + hi, input channels 8 (cg % 2) .. + 7
+  // A: weights of tap ct_hi(j) (lanes 32..63) or ct_lo(j), input channels
+  // 8 (cg % 2) .. + 7
   v8s a[CT_NP];
 #pragma unroll
   for (int j = 0; j < CT_NP; ++j) {
-    const int t = 2 * j + hi;
-    a[j] = t < NTAP ? load8(W + (t * NC + col) * NC + 8 * (cg & 1)) : zero8;
+    const int t = hi ? ct_hi(j) : ct_lo(j);
+    a[j] = t >= 0 ? load8(W + (t * NC + col) * NC + 8 * (cg & 1)) : zero8;
   }
   float bs[4] = {0.f, 0.f, 0.f, 0.f};
   if (MODE == FWD_HIDDEN)
@@ -238,31 +255,40 @@ __global__ __launch_bounds__(256) void k_conv_t(Grid3 g, int zc, const uint16_t 
     int sb[3];  // slots of planes z - 1, z, z + 1
 #pragma unroll
     for (int d = 0; d < 3; ++d) sb[d] = ((z - z0 + d) & 3) * CT_SB;
-    // B of pair j: rows wr and wr + 1 at the offset of tap 2j + hi
-    auto bread = [&](int j, v8s *b) {
-      const int t0 = 2 * j, t1 = min(2 * j + 1, NTAP - 1);
-      const int o = hi ? sb[ct_dz<MODE>(t1)] + ct_dy<MODE>(t1) * CT_RB + colx[ct_dx<MODE>(t1)]
-                       : sb[ct_dz<MODE>(t0)] + ct_dy<MODE>(t0) * CT_RB + colx[ct_dx<MODE>(t0)];
-      const unsigned char *p = lds + lbase + o;
-#pragma unroll
-      for (int rr = 0; rr < CT_RW; ++rr) b[rr] = *reinterpret_cast<const v8s *>(p + rr * CT_RB);
-    };
     v4f acc[CT_RW];
 #pragma unroll
     for (int rr = 0; rr < CT_RW; ++rr) acc[rr] = v4f{0.f, 0.f, 0.f, 0.f};
-    // B operands one pair ahead (the compiler interleaves the reads and
-    // MFMAs at its own register budget)
-    v8s bq[2][CT_RW];
-    bread(0, bq[0]);
+    auto mfma = [&](int j, int rr, const v8s &b) {
+      acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a[j]),
+                                                        __builtin_bit_cast(v8bf, b), acc[rr],
+                                                        0, 0, 0);
+    };
+    // pairs 3g .. 3g + 2 (g < CT_NG) differ only in dy: the B operands of
+    // the staged rows wr .. wr + 3 at each lane's plane and column serve all
+    // three pairs for both rows
 #pragma unroll
-    for (int j = 0; j < CT_NP; ++j) {
-      if (j + 1 < CT_NP) bread(j + 1, bq[(j + 1) & 1]);
-      const v8s *b = bq[j & 1];
-      const v8bf aj = __builtin_bit_cast(v8bf, a[j]);
+    for (int gx = 0; gx < CT_NG; ++gx) {
+      const int t = hi ? ct_hi(3 * gx) : ct_lo(3 * gx);
+      const int o = sb[ct_dz<MODE>(t)] + colx[ct_dx<MODE>(t)];
+      v8s b[CT_RW + 2];
+#pragma unroll
+      for (int k = 0; k < CT_RW + 2; ++k)
+        b[k] = *reinterpret_cast<const v8s *>(lds + lbase + o + k * CT_RB);
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy)
+#pragma unroll
+        for (int rr = 0; rr < CT_RW; ++rr)
+          mfma(3 * gx + dy, rr, b[rr + ct_dy<MODE>(ct_lo(3 * gx + dy))]);
+    }
+    // pairs 12, 13: each lane at its own tap's row and column
+#pragma unroll
+    for (int j = 3 * CT_NG; j < CT_NP; ++j) {
+      const int tl = ct_lo(j), th = ct_hi(j) >= 0 ? ct_hi(j) : tl;
+      const int o = hi ? sb[ct_dz<MODE>(th)] + ct_dy<MODE>(th) * CT_RB + colx[ct_dx<MODE>(th)]
+                       : sb[ct_dz<MODE>(tl)] + ct_dy<MODE>(tl) * CT_RB + colx[ct_dx<MODE>(tl)];
 #pragma unroll
       for (int rr = 0; rr < CT_RW; ++rr)
-        acc[rr] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(aj, __builtin_bit_cast(v8bf, b[rr]),
-                                                          acc[rr], 0, 0, 0);
+        mfma(j, rr, *reinterpret_cast<const v8s *>(lds + lbase + o + rr * CT_RB));
     }
     if (MODE == FWD_LAST) {
       const __amdgpu_buffer_rsrc_t rs = plane_rsrc(out, g, z, 4);
