@@ -300,7 +300,10 @@ class RefLib:
         path = os.path.join(REF_DIR, f"libvanadis_ref_{which}.so")
         if not os.path.exists(path):
             raise FileNotFoundError(path)
-        self.L = C.CDLL(path)
+        # lazy binding: the V1 build leaves VANADIS.FOR's call of the CUDA
+        # solver (kernel_dbcg_simple_, from przest3d) unresolved, and no
+        # harness calls przest3d
+        self.L = C.CDLL(path, mode=os.RTLD_LAZY)
         self.which = which
         self.L.par_3d_()
 
