@@ -5,7 +5,7 @@
 //
 // Layout.  The systems' vectors are concatenated (system m at nodes
 // [off_m, off_m + n_m)).  System m owns the vector workgroups
-// [vb0_m, vb0_m + nb_m), nb_m = min(1024, ceil(n_m / 1024)), each striding over
+// [vb0_m, vb0_m + nb_m), nb_m = min(256, ceil(n_m / 4096)), each striding over
 // the system's own range exactly as a solve of that system alone would, and
 // its reduction partials sit at those workgroup indices; the operator writes
 // its z.pp partials per owner unit.  Every scalar of system m is therefore
@@ -59,10 +59,13 @@ __device__ __forceinline__ void block_partial_at(double v, double *out, int slot
   const int *__restrict__ act, const int *__restrict__ actvb, int nact
 
 // Vector workgroups of the batched engine: BVB threads, at most BNB per
-// system, so a large system gets several workgroups per CU whose load and
-// store phases overlap (about four nodes per thread).
-constexpr int BVB = 256;
-constexpr int BNB = 1024;
+// system (about four nodes per thread; a large system gets one workgroup of
+// 16 waves per CU).  Every owner launch of an iteration re-sums a system's
+// nb partials of r.r, b.b and z.rr in each of its workgroups, all reading
+// the same lines at the launch's start, so few and large vector workgroups
+// keep that read short.
+constexpr int BVB = 1024;
+constexpr int BNB = 256;
 
 // r = b - A x0 (ax holds A x0), rr = r; stores only the preconditioned
 // z = r/d, zz = rr/d (the engine carries z, zz: r = d z) and the engine's
@@ -186,7 +189,7 @@ __global__ __launch_bounds__(BVB) void kb_update(const Segment *__restrict__ seg
   // not wait for the last nodes' loads
   const int na = (int)(sg.ng > 0 ? sg.ng : sg.nu);
   const double *pas = pa + sg.u0, *pbs = pb + (q ^ 1) * NB + sg.vb0;
-  constexpr int KA = 8, KB = 4;
+  constexpr int KA = 2, KB = 1;
   double av[KA], bv[KB];
   // unconditional loads at clamped indices (na, nb >= 1), then selects
 #pragma unroll

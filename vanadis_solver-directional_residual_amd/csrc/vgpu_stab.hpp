@@ -30,8 +30,10 @@
 
 namespace vgpu {
 
-constexpr int SVB = 256;   // threads of the vector workgroups
-constexpr int SNB = 1024;  // at most this many vector workgroups (= partials)
+// threads of the vector workgroups and at most this many of them (= partials,
+// re-summed by the owner launches: few and large, as in the BiCG engine)
+constexpr int SVB = 1024;
+constexpr int SNB = 256;
 
 inline int stab_blocks(int64_t n) {
   return (int)std::min<int64_t>(SNB, std::max<int64_t>(1, (n + 4 * SVB - 1) / (4 * SVB)));
@@ -372,7 +374,7 @@ struct Stab {
 // ---------------------------------------------------------------------------
 // Batched BiCGSTAB: nm systems in the batch layout of vgpu_bbicg.hpp (system
 // m at nodes [off_m, off_m + n_m), vector workgroups [vb0_m, vb0_m + nb_m)
-// with nb_m = min(1024, ceil(n_m / 1024)), the count a solve of the system
+// with nb_m = min(256, ceil(n_m / 4096)), the count a solve of the system
 // alone takes, owner partials from u0_m).  Every iteration is one kStabDir
 // and one kStabS owner launch over all systems; each system reads and writes
 // only its own partial ranges and status, so its iterates are bitwise those
