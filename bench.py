@@ -229,6 +229,103 @@ def measured_traffic(tuned, kernel="iteration_launch"):
                       f"tile/zchunk {key}"}
 
 
+def _owner_shape(tile):
+    return {0: (32, 8), 1: (16, 16), 2: (64, 4)}[int(tile)]
+
+
+# owner-kernel template arguments after <double, double, ...> per kernel
+_TRAFFIC_KERNELS = {"iteration_launch": "true, false, false, 1,",
+                    "dual_apply": "true, false, false, 0,",
+                    "bicgstab_direction_launch": "false, false, false, 2,",
+                    "bicgstab_s_launch": "false, false, false, 3,"}
+
+
+def live_traffic(tuned, N, E, timeout=180):
+    """HBM bytes per launch of the owner launches, measured in this run: two
+    child processes `rocprofv3 --pmc FETCH_SIZE` and `--pmc WRITE_SIZE` over
+    `bench.py --kernel-only` at the same tile and z chunk (one counter per
+    pass), each counter scaled by the factor that turns the same counter of
+    the copy kernel k_copy_wide (64 N doubles read and written per launch,
+    far above the last-level cache) into its known bytes.  Returns
+    {kernel: {...}, "calibration": {...}} or None (profiler missing or failed)."""
+    import csv
+    import glob
+    import shutil
+    import signal
+    import subprocess
+    import tempfile
+    if shutil.which("rocprofv3") is None:
+        return None
+    tile = 0 if tuned is None else int(tuned["tile"])
+    zc = 0 if tuned is None else int(tuned["zchunk"])
+    tx, ty = _owner_shape(tile)
+    med = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="vgpu_pmc_", dir=os.environ.get("TMPDIR", "/tmp"))
+        cmd = ["rocprofv3", "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "k",
+               "--", sys.executable, "-u", os.path.join(ROOT, "bench.py"), "--kernel-only",
+               "--tile", str(tile), "--zchunk", str(zc)]
+        progress(f"traffic: {counter} pass")
+        pr = subprocess.Popen(cmd, stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL,
+                              start_new_session=True)
+        try:
+            rc = pr.wait(timeout=timeout)
+        except subprocess.TimeoutExpired:
+            os.killpg(pr.pid, signal.SIGKILL)
+            pr.wait()
+            progress(f"traffic: {counter} pass timed out")
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+        if rc != 0 and not files:
+            progress(f"traffic: {counter} pass failed (rc {rc})")
+            shutil.rmtree(d, ignore_errors=True)
+            return None
+        vals = {}
+        for f in files:
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    name = row.get("Kernel_Name", "")
+                    if row.get("Counter_Name") != counter:
+                        continue
+                    key = None
+                    if "k_copy_wide" in name:
+                        key = "copy"
+                    elif "k_owner<double, double," in name and f", {tx}, {ty}>" in name:
+                        for k, sig in _TRAFFIC_KERNELS.items():
+                            if f"k_owner<double, double, {sig}" in name:
+                                key = k
+                    if key:
+                        vals.setdefault(key, []).append(float(row["Counter_Value"]))
+        shutil.rmtree(d, ignore_errors=True)
+        if "copy" not in vals:
+            progress(f"traffic: no copy-kernel {counter} values")
+            return None
+        med[counter] = {k: float(np.median(v)) for k, v in vals.items()}
+        med[counter + "_n"] = {k: len(v) for k, v in vals.items()}
+    copy_bytes = 64 * 8 * N  # read, and the same written, per launch
+    ff = copy_bytes / (med["FETCH_SIZE"]["copy"] * 1024.0)
+    fw = copy_bytes / (med["WRITE_SIZE"]["copy"] * 1024.0)
+    alg = {"iteration_launch": 81 * N + 40 * E, "dual_apply": 33 * N + 40 * E,
+           "bicgstab_direction_launch": 89 * N + 40 * E, "bicgstab_s_launch": 41 * N + 40 * E}
+    out = {"calibration": {"fetch_factor": round(ff, 5), "write_factor": round(fw, 5),
+                           "copy_bytes_each_way": copy_bytes,
+                           "copy_fetch_kib": med["FETCH_SIZE"]["copy"],
+                           "copy_write_kib": med["WRITE_SIZE"]["copy"]},
+           "tile": tile, "zchunk": zc}
+    for k in _TRAFFIC_KERNELS:
+        if k in med["FETCH_SIZE"] and k in med["WRITE_SIZE"]:
+            fb = med["FETCH_SIZE"][k] * 1024.0 * ff
+            wb = med["WRITE_SIZE"][k] * 1024.0 * fw
+            out[k] = {"bytes_per_launch": int(round(fb + wb)), "fetch_bytes": int(round(fb)),
+                      "write_bytes": int(round(wb)),
+                      "fetch_kib_counter": round(med["FETCH_SIZE"][k], 1),
+                      "write_kib_counter": round(med["WRITE_SIZE"][k], 1),
+                      "dispatches": med["FETCH_SIZE_n"][k],
+                      "ratio_to_algorithmic": round((fb + wb) / alg[k], 4)}
+    return out
+
+
 def config3(pkg, sols):
     """256x256x64 elements (4.29M nodes), per-element diffusivity log-uniform
     in [1e-4, 1e-2]: FP64 BiCG, FP64 BiCGSTAB and mixed precision (FP32 inner
@@ -563,6 +660,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched solves (32 right-hand sides, 16 subdomains)")
     ap.add_argument("--size", default="128,128,64")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter passes that measure the owner "
+                         "launches' HBM bytes (then the committed summary is reported)")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -876,14 +976,29 @@ def main():
         "learned_guess": learned,
         "config3": c3,
     }
-    traffic = measured_traffic(tuned)
-    if traffic:
-        out["roofline"]["traffic"] = traffic["bytes_per_launch"]
-        out["roofline"]["traffic_source"] = traffic["source"]
-    for key, kname in (("direction_launch", "bicgstab_direction_launch"),
-                       ("s_launch", "bicgstab_s_launch")):
-        tr = measured_traffic(tuned, kname)
-        stab_roof[key]["traffic"] = tr["bytes_per_launch"] if tr else None
+    live = None if a.no_traffic else live_traffic(tuned, N, E)
+    if live and "iteration_launch" in live:
+        out["traffic_live"] = live
+        src = (f"measured in this run: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE over bench.py "
+               f"--kernel-only --tile {live['tile']} --zchunk {live['zchunk']}, scaled by "
+               f"the copy kernel's factors {live['calibration']['fetch_factor']} / "
+               f"{live['calibration']['write_factor']}")
+        out["roofline"]["traffic"] = live["iteration_launch"]["bytes_per_launch"]
+        out["roofline"]["traffic_source"] = src
+        out["roofline_dual_apply"]["traffic"] = live.get("dual_apply", {}).get("bytes_per_launch")
+        for key, kname in (("direction_launch", "bicgstab_direction_launch"),
+                           ("s_launch", "bicgstab_s_launch")):
+            stab_roof[key]["traffic"] = live.get(kname, {}).get("bytes_per_launch")
+    else:
+        # the profiler is unavailable or was skipped: the committed summary
+        traffic = measured_traffic(tuned)
+        if traffic:
+            out["roofline"]["traffic"] = traffic["bytes_per_launch"]
+            out["roofline"]["traffic_source"] = traffic["source"] + " (committed, not this run)"
+        for key, kname in (("direction_launch", "bicgstab_direction_launch"),
+                           ("s_launch", "bicgstab_s_launch")):
+            tr = measured_traffic(tuned, kname)
+            stab_roof[key]["traffic"] = tr["bytes_per_launch"] if tr else None
     print(json.dumps(out), flush=True)
     ctx.close()
 
