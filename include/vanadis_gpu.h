@@ -337,11 +337,15 @@ int vgpu_batch_solve(vgpu_batch *batch, int precision, const double *const *b,
                      double *errs, int *status);
 /* Fallback for FP64 batch solves (no counterpart in the reference): with
  * method VGPU_METHOD_BICGSTAB, every member that the batched BiCG leaves at
- * itmax or in breakdown is finished by FP64 BiCGSTAB on its own context from
- * its current iterate, to the same rtol, with at most itmax iterations (its
- * iters then count both solves); VGPU_METHOD_BICG switches it off (the
- * default: members keep the batched BiCG's result).  Members
- * that converge in the batch are untouched (still bitwise vgpu_solve's). */
+ * itmax or in breakdown, or stops on its recursive residual while its true
+ * relative residual |b - A x| / |b| is above rtol, is finished by FP64
+ * BiCGSTAB on its own context from its current iterate (from zero when that
+ * iterate's true residual is above 1), to the same rtol, with at most itmax
+ * iterations per pass and up to three passes while the true residual stays
+ * above rtol (status 2 if it still does; iters count every solve); VGPU_METHOD_BICG switches it
+ * off (the default: members keep the batched BiCG's result).  Members that
+ * converge in the batch with a true residual within rtol are untouched (still
+ * bitwise vgpu_solve's). */
 int vgpu_batch_set_fallback(vgpu_batch *batch, int method, int itmax);
 /* Device-resident form: b, x in the batch layout (system m at offsets[m]
  * of a vector of `total` doubles; padding between systems is ignored). */

@@ -109,3 +109,35 @@ def test_mixed_batch_subdomains16(pkg, po, sub16, zchunk):
     sk = po.load_sketches().get("c4b")
     out = run_batch_and_singles(pkg, sub16, f"subdomains16 mixed zchunk {zchunk}", zchunk)
     check(pkg, po, *out, sketch=sk)
+
+
+@pytest.mark.parametrize("which", ["c4a", "c4b"])
+def test_fp64_batch_with_fallback(pkg, po, rhs32, sub16, which):
+    """FP64 batches of the same workloads: plain BiCG capped at 2000
+    iterations, then the BiCGSTAB fallback for every member left at the cap,
+    in breakdown, or stopped on a recursive residual whose true residual is
+    above rtol.  Every member ends with a true relative residual <= 1e-8 and
+    within 1e-8 rel-L2 of the oracle's solution (sketch)."""
+    probs = rhs32 if which == "c4a" else sub16
+    sk = po.load_sketches().get(which)
+    ctxs = [pkg.Context.from_problem(q) for q in probs]
+    try:
+        for c in ctxs:
+            c.set_options("owner", 11, TILE)
+        bs = [c.assemble_rhs(q.q_in) for c, q in zip(ctxs, probs)]
+        log(f"{which}: FP64 batch, BiCG cap 2000, BiCGSTAB fallback")
+        with pkg.Batch(ctxs) as bt:
+            bt.set_fallback("bicgstab", itmax=20000)
+            xs, its, errs, st, rc = bt.solve(bs, rtol=1e-8, itmax=2000, raise_on_fail=False)
+        tres = [c.true_residual(b, x) for c, b, x in zip(ctxs, bs, xs)]
+        log(f"  rc {rc}, iterations {its.min()}..{its.max()}, worst true residual "
+            f"{max(tres):.2e}")
+        assert rc == pkg.OK and (st == 1).all(), st
+        assert max(tres) <= 1e-8, max(tres)
+        if sk is not None:
+            est = [po.sketch_rel_l2(x, e) for x, e in zip(xs, sk["members"])]
+            log(f"  rel-L2 vs the oracle (sketch estimates): max {max(est):.2e}")
+            assert max(est) <= 1e-8, est
+    finally:
+        for c in ctxs:
+            c.close()

@@ -455,7 +455,8 @@ class Batch:
         return list(off), tot.value
 
     def set_fallback(self, method=None, itmax=20000):
-        """FP64 members left at itmax or in breakdown by the batched BiCG are
+        """FP64 members left at itmax or in breakdown by the batched BiCG, or
+        stopped on the recursive residual with a true residual above rtol, are
         finished by BiCGSTAB on their own contexts ("bicgstab"; None: off)."""
         code = METHOD_BICG if method is None else _METHOD_NAMES[method]
         check(load().vgpu_batch_set_fallback(self.h, int(code), int(itmax)), "batch_set_fallback")

@@ -1548,23 +1548,47 @@ int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double 
   }
   if (rc) return rc;
   VGPU_HIP(hipStreamSynchronize(bt->stream));
-  // FP64 members the batched BiCG left at itmax or in breakdown: BiCGSTAB on
-  // the member's own context from its current iterate, to the same rtol
+  // FP64 members the batched BiCG left at itmax or in breakdown, or stopped
+  // on its recursive residual while the true one, |b - A x| / |b|, is above
+  // rtol (the recurrence drifts from the truth on members whose residual
+  // peaks high): BiCGSTAB on the member's own context from its current
+  // iterate, or from zero if that iterate is worse than zero (true residual
+  // above 1 or not finite), to the same rtol
   if (precision == VGPU_FP64 && bt->fallback == VGPU_METHOD_BICGSTAB) {
     for (size_t i = 0; i < bt->m.size(); ++i) {
-      if (status[i] != 2 && status[i] != 3) continue;
+      if (status[i] != 1 && status[i] != 2 && status[i] != 3) continue;
       vgpu_ctx *c = bt->m[i];
       const Segment &sg = bt->seg[i];
       int it = 0, st = 0;
-      double e = 0;
+      double e = 0, tr = 0;
       VGPU_HIP(hipSetDevice(c->device));
-      rc = solve_stab<double>(c, c->stab64, b + sg.off, x + sg.off, c->diag, rtol,
-                              bt->fallback_itmax, &it, &e, &st);
+      rc = vgpu_residual_norm_dev(c, b + sg.off, x + sg.off, &tr);
       if (rc) return rc;
-      VGPU_HIP(hipStreamSynchronize(c->stream));
-      iters[i] += it;
-      errs[i] = e;
-      status[i] = st;
+      if (status[i] == 1 && tr <= rtol) continue;
+      if (!(tr <= 1.0)) {
+        hipLaunchKernelGGL(k_fill, grid_for(sg.n), dim3(256), 0, c->stream, (int64_t)sg.n, 0.0,
+                           x + sg.off);
+        VGPU_HIP(hipGetLastError());
+      }
+      // BiCGSTAB stops on its recursive residual too: a pass that ends with
+      // the true residual above rtol is followed by another from the current
+      // iterate (its recurrence restarts from the true residual), at most
+      // kFallbackPasses in all; a member still above rtol reports status 2
+      constexpr int kFallbackPasses = 3;
+      for (int pass = 0; pass < kFallbackPasses; ++pass) {
+        rc = solve_stab<double>(c, c->stab64, b + sg.off, x + sg.off, c->diag, rtol,
+                                bt->fallback_itmax, &it, &e, &st);
+        if (rc) return rc;
+        VGPU_HIP(hipStreamSynchronize(c->stream));
+        iters[i] += it;
+        errs[i] = e;
+        status[i] = st;
+        if (st != 1) break;
+        rc = vgpu_residual_norm_dev(c, b + sg.off, x + sg.off, &tr);
+        if (rc) return rc;
+        if (tr <= rtol) break;
+        if (pass == kFallbackPasses - 1) status[i] = 2;
+      }
     }
     VGPU_HIP(hipSetDevice(bt->device));
   }
