@@ -374,7 +374,6 @@ __attribute__((amdgpu_waves_per_eu(
     const Res ry2 = make_res(DUAL ? y2 + off : y1 + off, op.g.N * SI);
     const Res rmk = make_res(op.mask, op.g.N);
     const Res rrc = make_res(op.rec, (int64_t)eplane * ez * SR);
-    const Res rx1z = make_res(x1, 0);  // every access out of range: zeros
     const int64_t dslot = ub + (gidx >= 0 ? gidx : (int64_t)tile * ngz);
     double *dm[ND];
 #pragma unroll
@@ -442,38 +441,38 @@ __attribute__((amdgpu_waves_per_eu(
     };
 
     // this thread's two plane nodes of plane gz (raw values and flags; value
-    // 0 outside the grid: a plane above the top loads through resources of
-    // zero size, at a clamped scalar offset)
+    // 0 outside the grid: the lane offsets of a plane above the top get the
+    // out-of-range bit, at a clamped scalar offset, so every resource stays
+    // one and the same for the whole march)
     auto plane_fetch = [&](int gz, PlanePair<TIO, DIR> &r) {
-      const bool in = gz < (SLAB ? k1 : naz);
-      const Res ra = in ? rx1 : rx1z, rb = in ? rx2 : rx1z, rk = in ? rmk : rx1z;
-      const Res rc = in ? rpo : rx1z, rd = in ? rppo : rx1z;
+      const int ob = uni(gz < (SLAB ? k1 : naz) ? 0 : kOob);
       const int gc = min(gz, naz - 1);
       const int sm = uni(gc * plane), sv = uni(gc * plane * SI);
-      r.m0 = bload<uint8_t>(rk, vm0, sm);
-      r.a0 = bload<TIO>(ra, vp0, sv);
-      r.b0 = DUAL ? bload<TIO>(rb, vp0, sv) : TIO(0);
-      const Res rt_ = in ? rto : rx1z;
-      r.c0 = (DIR || STAB) ? bload<TIO>(rc, vp0, sv) : TIO(0);
-      r.d0 = (DIR || SA) ? bload<TIO>(rd, vp0, sv) : TIO(0);
-      r.t0 = SA ? bload<TIO>(rt_, vp0, sv) : TIO(0);
-      r.f = (STAB || JAC) ? bload<TIO>(in ? rdg : rx1z, vo, sv) : TIO(0);
-      r.g = (STAB || JAC) ? bload<TIO>(in ? rrt : rx1z, vo, sv) : TIO(0);
+      const int p0 = vp0 | ob;
+      r.m0 = bload<uint8_t>(rmk, vm0 | ob, sm);
+      r.a0 = bload<TIO>(rx1, p0, sv);
+      r.b0 = DUAL ? bload<TIO>(rx2, p0, sv) : TIO(0);
+      r.c0 = (DIR || STAB) ? bload<TIO>(rpo, p0, sv) : TIO(0);
+      r.d0 = (DIR || SA) ? bload<TIO>(rppo, p0, sv) : TIO(0);
+      r.t0 = SA ? bload<TIO>(rto, p0, sv) : TIO(0);
+      r.f = (STAB || JAC) ? bload<TIO>(rdg, vo | ob, sv) : TIO(0);
+      r.g = (STAB || JAC) ? bload<TIO>(rrt, vo | ob, sv) : TIO(0);
       // x of owned nodes of the march's own planes only
-      const Res re = gz < k1 ? rxs : rx1z;
-      r.e0 = (DIR || SA) ? bload<TIO>(re, vn0, sv) : TIO(0);
+      const int oe = uni(gz < k1 ? 0 : kOob);
+      r.e0 = (DIR || SA) ? bload<TIO>(rxs, vn0 | oe, sv) : TIO(0);
       r.a1 = r.b1 = r.c1 = r.d1 = r.e1 = r.t1 = TIO(0);
       r.m1 = 1;
       if (has1) {
-        r.m1 = bload<uint8_t>(rk, vm1, sm);
-        r.a1 = bload<TIO>(ra, vp1, sv);
-        if (DUAL) r.b1 = bload<TIO>(rb, vp1, sv);
-        if (DIR || STAB) r.c1 = bload<TIO>(rc, vp1, sv);
+        const int p1 = vp1 | ob;
+        r.m1 = bload<uint8_t>(rmk, vm1 | ob, sm);
+        r.a1 = bload<TIO>(rx1, p1, sv);
+        if (DUAL) r.b1 = bload<TIO>(rx2, p1, sv);
+        if (DIR || STAB) r.c1 = bload<TIO>(rpo, p1, sv);
         if (DIR || SA) {
-          r.d1 = bload<TIO>(rd, vp1, sv);
-          r.e1 = bload<TIO>(re, vn1, sv);
+          r.d1 = bload<TIO>(rppo, p1, sv);
+          r.e1 = bload<TIO>(rxs, vn1 | oe, sv);
         }
-        if (SA) r.t1 = bload<TIO>(rt_, vp1, sv);
+        if (SA) r.t1 = bload<TIO>(rto, p1, sv);
       }
     };
     // plane gz into LDS; DIR: the directions are formed first and those of
@@ -535,7 +534,7 @@ __attribute__((amdgpu_waves_per_eu(
       c.hz = op.hz[kc];
       c.ihz = op.ihz[kc];
       const bool live = kz == kc && (!SLAB || (kz >= k0 && kz < k1 - 1));
-      load_rec<T>(live ? rrc : rx1z, vr, uni(kc * eplane * SR), c);
+      load_rec<T>(rrc, vr | uni(live ? 0 : kOob), uni(kc * eplane * SR), c);
     };
 
     // dot terms (ND sums): each thread sums its owned nodes of the current
@@ -586,19 +585,17 @@ __attribute__((amdgpu_waves_per_eu(
     {
       const int gz = max(k0 - 1, 0);
       const int sm = uni(gz * plane), sv = uni(gz * plane * SI);
-      const bool zin = !SLAB && k0 >= 1;
-      const Res ra = zin ? rx1 : rx1z, rb = zin ? rx2 : rx1z, rk = zin ? rmk : rx1z;
-      const Res rc = zin ? rpo : rx1z, rd = zin ? rppo : rx1z, rt_ = zin ? rto : rx1z;
+      const int oz = uni(!SLAB && k0 >= 1 ? 0 : kOob);
 #pragma unroll
       for (int c = 0; c < 4; ++c) {
         const int col = colofs(exg + (c == 1 || c == 2), eyg + (c >= 2));
-        const int vc = col >= 0 ? col : kOob, vcs = col >= 0 ? col * SI : kOob;
-        cm[c] = bload<uint8_t>(rk, vc, sm);
-        ca[c] = bload<TIO>(ra, vcs, sv);
-        cb[c] = DUAL ? bload<TIO>(rb, vcs, sv) : TIO(0);
-        cc[c] = (DIR || STAB) ? bload<TIO>(rc, vcs, sv) : TIO(0);
-        cd[c] = (DIR || SA) ? bload<TIO>(rd, vcs, sv) : TIO(0);
-        ct[c] = SA ? bload<TIO>(rt_, vcs, sv) : TIO(0);
+        const int vc = (col >= 0 ? col : kOob) | oz, vcs = (col >= 0 ? col * SI : kOob) | oz;
+        cm[c] = bload<uint8_t>(rmk, vc, sm);
+        ca[c] = bload<TIO>(rx1, vcs, sv);
+        cb[c] = DUAL ? bload<TIO>(rx2, vcs, sv) : TIO(0);
+        cc[c] = (DIR || STAB) ? bload<TIO>(rpo, vcs, sv) : TIO(0);
+        cd[c] = (DIR || SA) ? bload<TIO>(rppo, vcs, sv) : TIO(0);
+        ct[c] = SA ? bload<TIO>(rto, vcs, sv) : TIO(0);
       }
     }
     PlanePair<TIO, DIR> pr;
