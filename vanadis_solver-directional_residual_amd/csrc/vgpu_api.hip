@@ -1943,17 +1943,32 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     f3 = reinterpret_cast<float *>(c->vt);
     f4 = reinterpret_cast<float *>(c->vt2);
   }
-  // fused iteration launch: z, zz = vx_, vb; previous directions vt, vt2;
-  // new directions, operator outputs and x in tdir; a status that never stops
-  // (tol 0, alpha 0) and unit partials, so beta = 1 and every system runs
+  // fused iteration launch on one block of nine vectors, as the BiCG engine
+  // lays them out (z, zz, previous directions, new directions, operator
+  // outputs, x: the launch takes the block-addressed kernel the solver uses):
+  // z, zz, po, ppo copied from the deterministic vectors vx_, vb, vt, vt2; a
+  // status that never stops (tol 0, alpha 0) and unit partials, so beta = 1
+  // and every system runs
   const Segment sg = make_segments({N}, {owner_units(c->g, c->tile)})[0];
   const int64_t tail = 2048;
   if (what == 3) {
     if (!c->tdir) {
-      VGPU_HIP(hipMalloc(&c->tdir, (5 * N + tail) * sizeof(double)));
-      VGPU_HIP(hipMemset(c->tdir, 0, 5 * N * sizeof(double)));
+      VGPU_HIP(hipMalloc(&c->tdir, (9 * N + tail) * sizeof(double)));
+      VGPU_HIP(hipMemset(c->tdir, 0, 9 * N * sizeof(double)));
     }
-    double *parts = c->tdir + 5 * N;
+    if (precision == VGPU_FP32) {
+      float *v = reinterpret_cast<float *>(c->tdir);
+      const float *src[] = {f1, f2, f3, f4};
+      for (int i = 0; i < 4; ++i)
+        VGPU_HIP(hipMemcpyAsync(v + i * N, src[i], N * sizeof(float),
+                                hipMemcpyDeviceToDevice, c->stream));
+    } else {
+      const double *src[] = {c->vx_, c->vb, c->vt, c->vt2};
+      for (int i = 0; i < 4; ++i)
+        VGPU_HIP(hipMemcpyAsync(c->tdir + i * N, src[i], N * sizeof(double),
+                                hipMemcpyDeviceToDevice, c->stream));
+    }
+    double *parts = c->tdir + 9 * N;
     hipLaunchKernelGGL(k_fill, grid_for(tail), dim3(256), 0, c->stream, tail, 1.0, parts);
     SolveStatus s0{};
     s0.iter = 1;
@@ -1962,16 +1977,16 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     VGPU_HIP(hipStreamSynchronize(c->stream));
   }
   auto launch_dir = [&]() -> int {
-    double *parts = c->tdir + 5 * N;
+    double *parts = c->tdir + 9 * N;
     const int ns = (int)((sizeof(SolveStatus) + 7) / 8);
     if (precision == VGPU_FP32) {
       float *v = reinterpret_cast<float *>(c->tdir);
       DirData<float> dd;
-      dd.po = f3;
-      dd.ppo = f4;
-      dd.pn = v;
-      dd.ppn = v + N;
-      dd.x = v + 4 * N;
+      dd.po = v + 2 * N;
+      dd.ppo = v + 3 * N;
+      dd.pn = v + 4 * N;
+      dd.ppn = v + 5 * N;
+      dd.x = v + 8 * N;
       dd.pr = dd.pb = dd.pbn = parts + ns;
       dd.stv = reinterpret_cast<SolveStatus *>(parts);
       dd.tol = 0.0;
@@ -1979,19 +1994,22 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
       dd.itmax = 1 << 30;
       dd.NB = sg.nb;
       dd.q = 0;
+      dd.blk = v;
+      dd.blk_bytes = 9 * N * (int64_t)sizeof(float);
       Member<float> m0{};
       m0.op = c->op32();
       m0.seg = sg;
       return launch_owner_dir<float, float>(m0, nullptr, ActiveSet{}, false, c->tile,
-                                            c->zchunk, m0.op.ra != 0.0, dd, f1, f2,
-                                            v + 2 * N, v + 3 * N, c->tpart, c->stream);
+                                            c->zchunk, m0.op.ra != 0.0, dd, v, v + N,
+                                            v + 6 * N, v + 7 * N, c->tpart, c->stream);
     }
+    double *v = c->tdir;
     DirData<double> dd;
-    dd.po = c->vt;
-    dd.ppo = c->vt2;
-    dd.pn = c->tdir;
-    dd.ppn = c->tdir + N;
-    dd.x = c->tdir + 4 * N;
+    dd.po = v + 2 * N;
+    dd.ppo = v + 3 * N;
+    dd.pn = v + 4 * N;
+    dd.ppn = v + 5 * N;
+    dd.x = v + 8 * N;
     dd.pr = dd.pb = dd.pbn = parts + ns;
     dd.stv = reinterpret_cast<SolveStatus *>(parts);
     dd.tol = 0.0;
@@ -1999,13 +2017,14 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     dd.itmax = 1 << 30;
     dd.NB = sg.nb;
     dd.q = 0;
+    dd.blk = v;
+    dd.blk_bytes = 9 * N * (int64_t)sizeof(double);
     Member<double> m0{};
     m0.op = c->op64(c->cH, c->cC);
     m0.seg = sg;
     return launch_owner_dir<double, double>(m0, nullptr, ActiveSet{}, false, c->tile,
-                                            c->zchunk, m0.op.ra != 0.0, dd, c->vx_, c->vb,
-                                            c->tdir + 2 * N, c->tdir + 3 * N, c->tpart,
-                                            c->stream);
+                                            c->zchunk, m0.op.ra != 0.0, dd, v, v + N,
+                                            v + 6 * N, v + 7 * N, c->tpart, c->stream);
   };
   // what 5 / 6: the BiCGSTAB direction / S launch (FP64) on the BiCGSTAB
   // engine's buffers, inputs from the deterministic vectors above, fixed
@@ -2029,18 +2048,24 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     s0.rho = s0.alpha = s0.omega = s0.rho_next = 1.0;
     VGPU_HIP(hipMemcpyAsync(e.st, &s0, sizeof(SolveStatus), hipMemcpyHostToDevice,
                             c->stream));
+    // inputs from the deterministic vectors, in the engine's own buffers (the
+    // launches then address its block as a solve does): R = vx_, x = vt2
+    const double *src[] = {c->vx_, c->vx_, c->vt2};
+    double *dst[] = {e.R[0], e.R[1], e.xw};
+    for (int i = 0; i < 3; ++i)
+      VGPU_HIP(hipMemcpyAsync(dst[i], src[i], N * sizeof(double), hipMemcpyDeviceToDevice,
+                              c->stream));
     VGPU_HIP(hipStreamSynchronize(c->stream));
   }
   auto launch_stab = [&]() -> int {
     Stab<double> &e = c->stab64;
     OpData<double> op = c->op64(c->cH, c->cC);
-    // inputs from the deterministic vectors: R = vx_, x = vt2
     if (what == 5)
       return launch_owner_stab<double, double>(
-          op, kStabDir, stab_zchunk(c), e.dir_data(0, c->diag, c->vt2, 0.0, 1 << 30, 1e300),
-          c->vx_, e.V[0], c->stream);
+          op, kStabDir, stab_zchunk(c), e.dir_data(0, c->diag, e.xw, 0.0, 1 << 30, 1e300),
+          e.R[1], e.V[0], c->stream);
     return launch_owner_stab<double, double>(
-        op, kStabS, stab_zchunk(c), e.s_data(0, c->diag, 0.0, 1 << 30, 1e300), c->vx_, e.T,
+        op, kStabS, stab_zchunk(c), e.s_data(0, c->diag, 0.0, 1 << 30, 1e300), e.R[0], e.T,
         c->stream);
   };
   // what 4: a copy of 8 N doubles, the size class of a BiCG iteration's

@@ -353,6 +353,7 @@ struct BBicg {
   TV *ax = nullptr, *p[2] = {nullptr, nullptr}, *pp[2] = {nullptr, nullptr},
      *z = nullptr, *zz = nullptr, *w = nullptr, *ww = nullptr;
   TV *xw = nullptr;  // the iterates (copied to the caller's x at the end)
+  VecBlock vblk;     // one allocation for the ten vectors above
   double *pr = nullptr, *pb = nullptr, *pbn = nullptr, *pa = nullptr;
   SolveStatus *st = nullptr;       // device [nm]
   SolveStatus *st_host = nullptr;  // pinned [nm]
@@ -389,9 +390,10 @@ struct BBicg {
     if (!rc) rc = cur.set(hseg, all);
     if (rc) return rc;
     TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww, &xw};
-    for (auto pv : v) {
-      VGPU_HIP(hipMalloc(pv, ntot * sizeof(TV)));
-      VGPU_HIP(hipMemset(*pv, 0, ntot * sizeof(TV)));
+    rc = vblk.alloc<TV>(v, 10, ntot);
+    if (rc) {
+      set_error("batched BiCG: vector allocation failed");
+      return rc;
     }
     VGPU_HIP(hipMalloc(&pr, 2 * NB * sizeof(double)));
     VGPU_HIP(hipMalloc(&pb, 2 * NB * sizeof(double)));
@@ -412,11 +414,8 @@ struct BBicg {
   void release() {
     drop_graph();
     TV **v[] = {&ax, &p[0], &p[1], &pp[0], &pp[1], &z, &zz, &w, &ww, &xw};
-    for (auto pv : v)
-      if (*pv) {
-        hipFree(*pv);
-        *pv = nullptr;
-      }
+    for (auto pv : v) *pv = nullptr;
+    vblk.release();
     double **d[] = {&pr, &pb, &pbn, &pa};
     for (auto pd : d)
       if (*pd) {
@@ -587,6 +586,8 @@ struct BBicg {
     dd.itmax = itmax;
     dd.NB = NB;
     dd.q = q;
+    dd.blk = vblk.base;
+    dd.blk_bytes = vblk.bytes;
     int rc = dual(z, zz, dd, w, ww, pa, a, s);
     if (rc) return rc;
     hipLaunchKernelGGL((kb_update<TV>), dim3(a.nvb), dim3(BVB), 0, s, seg, a.act,

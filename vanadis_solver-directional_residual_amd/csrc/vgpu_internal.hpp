@@ -4,6 +4,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
 #include <string>
@@ -181,6 +182,10 @@ struct DirData {
   SolveStatus *stv = nullptr;
   double tol = 0, divtol = 0;
   int itmax = 0, NB = 0, q = 0;
+  // the engine's vector block (VecBlock): when every vector of the launch
+  // lies in it, the launch addresses them through one buffer resource
+  const void *blk = nullptr;
+  int64_t blk_bytes = 0;
 };
 // y1 = A p, y2 = A^T pp with the fused direction update (inputs z, zz);
 // batch: the active members of md (else the single member m0); dot partials
@@ -211,6 +216,9 @@ enum StabMode { kStabDir = 2, kStabS = 3 };
 // 7: the operator over the z-slabs of one colour of the colored scatter)
 constexpr int kJacobi = 6;
 constexpr int kSlab = 7;
+// added to a BiCG / BiCGSTAB mode (1..5): the same launch with every vector
+// but the diagonal addressed through the engine's block (DirData::blk)
+constexpr int kBlk = 10;
 template <typename TIO>
 struct StabData {
   const TIO *po = nullptr, *vo = nullptr;  // kStabDir: P, V of iteration j-1
@@ -237,6 +245,40 @@ struct StabData {
   // latency more for the others; a kernel variant of its own)
   int early = 0;
   double jomega = 0;  // Jacobi sweep: damping (d, rt hold the diagonal and b)
+  // the engine's vector block, as in DirData (d is addressed on its own)
+  const void *blk = nullptr;
+  int64_t blk_bytes = 0;
+};
+
+// The vectors of a solver engine carved from one allocation (each padded to
+// a multiple of 128 values), so that the engine's owner launches can address
+// all of them through one buffer resource and a scalar offset each: fewer
+// scalar registers than a resource per vector (4 each).  The owner launches
+// need the block below 2^31 bytes; a larger block still works, through a
+// resource per vector.
+struct VecBlock {
+  void *base = nullptr;
+  int64_t bytes = 0;
+  // k vectors of n values of type TV into *v[0..k), zeroed
+  template <typename TV>
+  int alloc(TV **const *v, int k, int64_t n) {
+    release();
+    const int64_t np = (std::max<int64_t>(1, n) + 127) / 128 * 128;
+    bytes = k * np * (int64_t)sizeof(TV);
+    if (hipMalloc(&base, bytes) != hipSuccess) {
+      base = nullptr;
+      bytes = 0;
+      return VGPU_ERR_NOMEM;
+    }
+    if (hipMemset(base, 0, bytes) != hipSuccess) return VGPU_ERR_HIP;
+    for (int i = 0; i < k; ++i) *v[i] = static_cast<TV *>(base) + i * np;
+    return VGPU_OK;
+  }
+  void release() {
+    if (base) hipFree(base);
+    base = nullptr;
+    bytes = 0;
+  }
 };
 // The same two launches over the members of a batch (md, as as
 // launch_owner_batch): sd.st points at the members' statuses (indexed by

@@ -195,6 +195,24 @@ __device__ __forceinline__ void bstore(float x, Res r, int v, int s) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), r, v, s, 0);
 }
 
+// A vector of an owner launch: a resource of its own (offset 0), or the
+// engine's block resource and the vector's byte offset in it (added to the
+// scalar offset of every access)
+struct VRef {
+  Res r;
+  int o;
+};
+template <typename V>
+__device__ __forceinline__ V bload(const VRef &v, int vo, int so) {
+  return bload<V>(v.r, vo, so + v.o);
+}
+__device__ __forceinline__ void bstore(double x, const VRef &v, int vo, int so) {
+  bstore(x, v.r, vo, so + v.o);
+}
+__device__ __forceinline__ void bstore(float x, const VRef &v, int vo, int so) {
+  bstore(x, v.r, vo, so + v.o);
+}
+
 // element record (vx, vy, vz, lambda, tau) at byte offset v + s
 template <typename T>
 __device__ __forceinline__ void load_rec(Res r, int v, int s, Coef<T> &c) {
@@ -256,27 +274,32 @@ struct PlanePair {
 template <typename T, typename TIO, bool DUAL, bool ROBIN, bool BATCH, int DM, int TX, int TY>
 __global__ __launch_bounds__(TX * TY)
 __attribute__((amdgpu_waves_per_eu(
-    sizeof(T) == 8 ? ((DUAL || (DM >= 2 && DM != kSlab)) ? 2 : 3)
-                   : ((DUAL || (DM >= 2 && DM != kSlab)) ? 3 : 4)))) void k_owner(
+    sizeof(T) == 8 ? ((DUAL || (DM % kBlk >= 2 && DM % kBlk != kSlab)) ? 2 : 3)
+                   : ((DUAL || (DM % kBlk >= 2 && DM % kBlk != kSlab)) ? 3 : 4)))) void k_owner(
     Member<T> m0, const Member<T> *__restrict__ md, const int *__restrict__ act,
     const int64_t *__restrict__ actu, const int64_t *__restrict__ actg, int nact,
     int64_t U, int zc, const SolveStatus *__restrict__ st, const TIO *__restrict__ x1,
     TIO *__restrict__ y1, const TIO *__restrict__ x2, TIO *__restrict__ y2,
     double *__restrict__ dot, DirData<TIO> dd, StabData<TIO> sd, int slab_color) {
-  constexpr bool DIR = DM == 1;  // BiCG iteration launch
+  // BLK (DM + kBlk): the vectors of a BiCG or BiCGSTAB launch, but the
+  // diagonal, lie in the engine's block: one resource for all of them
+  constexpr bool BLK = DM >= kBlk;
+  constexpr int DMM = DM % kBlk;
+  constexpr bool DIR = DMM == 1;  // BiCG iteration launch
   // BiCGSTAB launches; DM + 2: the same with the early status test
-  constexpr bool SA = DM == kStabDir || DM == kStabDir + 2;
-  constexpr bool SB = DM == kStabS || DM == kStabS + 2, STAB = SA || SB;
-  constexpr bool EARLY = DM == kStabDir + 2 || DM == kStabS + 2;
+  constexpr bool SA = DMM == kStabDir || DMM == kStabDir + 2;
+  constexpr bool SB = DMM == kStabS || DMM == kStabS + 2, STAB = SA || SB;
+  constexpr bool EARLY = DMM == kStabDir + 2 || DMM == kStabS + 2;
   // damped Jacobi sweep: output x + omega (b - A x) / d at the owned nodes
-  constexpr bool JAC = DM == kJacobi;
+  constexpr bool JAC = DMM == kJacobi;
   // colored scatter (z-chunk launch only): the march of a z-slab of element
   // layers [k0, k1 - 1) with the layers just below and above it taken as
   // zero (no recomputed halo layer), so its bottom and top planes get the
   // slab's partial sums: the colour-0 launch (slab_color 0, the even slabs)
   // stores them, the colour-1 launch adds its own to them (the grid's top
   // plane, which has no slab above, is stored)
-  constexpr bool SLAB = DM == kSlab;
+  constexpr bool SLAB = DMM == kSlab;
+  static_assert(!BLK || DIR || STAB, "block addressing is for the solver launches");
   static_assert(DUAL || !DIR, "the direction update needs the dual operator");
   static_assert(!(DUAL && (STAB || JAC)), "the BiCGSTAB and Jacobi launches are single");
   constexpr int NP = (TX + 1) * (TY + 1);
@@ -368,10 +391,25 @@ __attribute__((amdgpu_waves_per_eu(
     const int ntx = (op.g.nax + TX - 2) / (TX - 1);
     const int nax = op.g.nax, nay = op.g.nay, ex = op.g.ex, ey = op.g.ey, ez = op.g.ez;
     const int plane = nax * nay, eplane = ex * ey;
-    const Res rx1 = make_res(x1 + off, op.g.N * SI);
-    const Res rx2 = make_res(DUAL ? x2 + off : x1 + off, op.g.N * SI);
-    const Res ry1 = make_res(y1 + off, op.g.N * SI);
-    const Res ry2 = make_res(DUAL ? y2 + off : y1 + off, op.g.N * SI);
+    // BLK: one resource over the engine's block, a scalar byte offset per
+    // vector (four scalar registers in all instead of four per vector)
+    const char *const vbase = BLK ? (const char *)(DIR ? dd.blk : sd.blk) : nullptr;
+    const Res rvb = make_res(vbase, BLK ? (DIR ? dd.blk_bytes : sd.blk_bytes) : 0);
+    auto vref = [&](const TIO *p) {
+      VRef v;
+      if constexpr (BLK) {
+        v.r = rvb;
+        v.o = (int)((const char *)p - vbase);
+      } else {
+        v.r = make_res(p, op.g.N * SI);
+        v.o = 0;
+      }
+      return v;
+    };
+    const VRef rx1 = vref(x1 + off);
+    const VRef rx2 = vref(DUAL ? x2 + off : x1 + off);
+    const VRef ry1 = vref(y1 + off);
+    const VRef ry2 = vref(DUAL ? y2 + off : y1 + off);
     const Res rmk = make_res(op.mask, op.g.N);
     const Res rrc = make_res(op.rec, (int64_t)eplane * ez * SR);
     const int64_t dslot = ub + (gidx >= 0 ? gidx : (int64_t)tile * ngz);
@@ -409,21 +447,22 @@ __attribute__((amdgpu_waves_per_eu(
     // and RT read at the owned node column vo
     const TIO *cptr = DIR ? dd.po : SA ? sd.vo : SB ? sd.v : x1;
     const TIO *dptr = DIR ? dd.ppo : SA ? sd.po : x1;
-    const Res rpo = make_res(cptr + off, op.g.N * SI);
-    const Res rppo = make_res(dptr + off, op.g.N * SI);
-    const Res rpn = make_res(DIR ? dd.pn + off : SA ? sd.pn + off : y1 + off, op.g.N * SI);
-    const Res rrn = make_res(SA ? sd.rn + off : y1 + off, op.g.N * SI);
-    const Res rto = make_res(SA ? sd.to + off : x1 + off, op.g.N * SI);
-    // owned-node values: d and RT (BiCGSTAB), d and b (Jacobi sweep)
+    const VRef rpo = vref(cptr + off);
+    const VRef rppo = vref(dptr + off);
+    const VRef rpn = vref(DIR ? dd.pn + off : SA ? sd.pn + off : y1 + off);
+    const VRef rrn = vref(SA ? sd.rn + off : y1 + off);
+    const VRef rto = vref(SA ? sd.to + off : x1 + off);
+    // owned-node values: d and RT (BiCGSTAB), d and b (Jacobi sweep); the
+    // diagonal is the context's, outside the engine's block
     const Res rdg = make_res((STAB || JAC) ? sd.d + off : x1 + off, op.g.N * SI);
-    const Res rrt = make_res((STAB || JAC) ? sd.rt + off : x1 + off, op.g.N * SI);
-    const Res rppn = make_res(DIR ? dd.ppn + off : y1 + off, op.g.N * SI);
+    const VRef rrt = vref((STAB || JAC) ? sd.rt + off : x1 + off);
+    const VRef rppn = vref(DIR ? dd.ppn + off : y1 + off);
     auto owned_col = [&](int lx, int ly) {
       return lx >= 1 && lx <= TX - 1 && ly >= 1 && ly <= TY - 1;
     };
     const int vn0 = (pc0 >= 0 && owned_col(lx0, ly0)) ? vp0 : kOob;
     const int vn1 = (pc1 >= 0 && owned_col(lx1, ly1)) ? vp1 : kOob;
-    const Res rxs = make_res(DIR ? dd.x + off : SA ? sd.x + off : y1 + off, op.g.N * SI);
+    const VRef rxs = vref(DIR ? dd.x + off : SA ? sd.x + off : y1 + off);
     TIO ap = TIO(0);  // DIR: alpha of the previous update (x += ap po)
     // DIR: beta and the first-iteration flag, known after the status test
     // below; dir() is only evaluated after it.  BiCGSTAB: beta, omega
@@ -1421,14 +1460,15 @@ void launch_owner_k(const OwnerLaunch<T, TIO> &L, hipStream_t s) {
 // BiCGSTAB launches (single)
 template <typename T, typename TIO, bool BATCH, int DM, int TX, int TY>
 void launch_owner_shape(const OwnerLaunch<T, TIO> &L, bool robin, hipStream_t s) {
+  constexpr int DMM = DM % kBlk;
   if (L.x2) {
-    if constexpr (DM <= 1 || DM == kSlab) {
+    if constexpr (DMM <= 1 || DMM == kSlab) {
       if (robin)
         launch_owner_k<T, TIO, true, true, BATCH, DM, TX, TY>(L, s);
       else
         launch_owner_k<T, TIO, true, false, BATCH, DM, TX, TY>(L, s);
     }
-  } else if constexpr (DM != 1) {
+  } else if constexpr (DMM != 1) {
     if (robin)
       launch_owner_k<T, TIO, false, true, BATCH, DM, TX, TY>(L, s);
     else
@@ -1486,11 +1526,30 @@ template int launch_owner_batch<float, float>(const Member<float> *, const Activ
                                               const float *, float *, const float *,
                                               float *, double *, hipStream_t);
 
+// Whether every vector p[0..k) of a solver launch lies in the engine's block
+// [blk, blk + bytes), with len bytes from its start (0: the start only, for
+// the batch vectors, which the engines carve whole from their block); the
+// launch then takes the block-addressed kernel (its resource needs the block
+// below 2^31 bytes).
+template <typename TIO>
+bool in_block(const void *blk, int64_t bytes, const TIO *const *p, int k, int64_t len) {
+  if (!blk || bytes <= 0 || bytes >= ((int64_t)1 << 31)) return false;
+  const char *b = static_cast<const char *>(blk);
+  for (int i = 0; i < k; ++i) {
+    const char *q = reinterpret_cast<const char *>(p[i]);
+    if (!q || q < b || q + std::max<int64_t>(len, 1) > b + bytes) return false;
+  }
+  return true;
+}
+
 template <typename T, typename TIO>
 int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &as,
                      bool batch, int tile, int zchunk, bool robin, const DirData<TIO> &dd,
                      const TIO *z, const TIO *zz, TIO *y1, TIO *y2, double *dot,
                      hipStream_t s) {
+  const TIO *const vs[] = {z, zz, y1, y2, dd.po, dd.ppo, dd.pn, dd.ppn, dd.x};
+  const bool blk = in_block(dd.blk, dd.blk_bytes, vs, 9,
+                            batch ? 0 : m0.op.g.N * (int64_t)sizeof(TIO));
   OwnerLaunch<T, TIO> L{};
   L.zchunk = zchunk;
   L.x1 = z;
@@ -1504,7 +1563,8 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
     L.as = as;
     L.utot = as.nu;
     L.gtot = as.ng;
-    return launch_owner_any<T, TIO, true, 1>(L, tile, robin, s);
+    return blk ? launch_owner_any<T, TIO, true, 1 + kBlk>(L, tile, robin, s)
+               : launch_owner_any<T, TIO, true, 1>(L, tile, robin, s);
   }
   if (!owner_fits(m0.op.g)) {
     set_error("owner scatter: system too large for 32-bit offsets");
@@ -1513,7 +1573,8 @@ int launch_owner_dir(const Member<T> &m0, const Member<T> *md, const ActiveSet &
   L.m0 = m0;
   L.utot = m0.seg.nu;
   L.gtot = owner_groups(m0.op.g, tile, zchunk);
-  return launch_owner_any<T, TIO, false, 1>(L, tile, robin, s);
+  return blk ? launch_owner_any<T, TIO, false, 1 + kBlk>(L, tile, robin, s)
+             : launch_owner_any<T, TIO, false, 1>(L, tile, robin, s);
 }
 
 template int launch_owner_dir<double, double>(const Member<double> &, const Member<double> *,
@@ -1549,14 +1610,23 @@ int launch_owner_stab(const OpData<T> &op, int mode, int zchunk, const StabData<
   L.x1 = r;
   L.y1 = y1;
   L.sd = sd;
+  const bool rb = op.ra != 0.0;
+  const int t = op.tile;
+  const TIO *const va[] = {r, y1, sd.po, sd.vo, sd.to, sd.pn, sd.rn, sd.x, sd.rt};
+  const TIO *const vb[] = {r, y1, sd.v, sd.rt};
+  const int64_t len = g.N * (int64_t)sizeof(TIO);
+  if (mode == kStabDir && in_block(sd.blk, sd.blk_bytes, va, 9, len))
+    return sd.early ? launch_owner_any<T, TIO, false, kStabDir + 2 + kBlk>(L, t, rb, s)
+                    : launch_owner_any<T, TIO, false, kStabDir + kBlk>(L, t, rb, s);
+  if (mode == kStabS && in_block(sd.blk, sd.blk_bytes, vb, 4, len))
+    return sd.early ? launch_owner_any<T, TIO, false, kStabS + 2 + kBlk>(L, t, rb, s)
+                    : launch_owner_any<T, TIO, false, kStabS + kBlk>(L, t, rb, s);
   if (sd.early) {
-    if (mode == kStabDir)
-      return launch_owner_any<T, TIO, false, kStabDir + 2>(L, op.tile, op.ra != 0.0, s);
-    return launch_owner_any<T, TIO, false, kStabS + 2>(L, op.tile, op.ra != 0.0, s);
+    if (mode == kStabDir) return launch_owner_any<T, TIO, false, kStabDir + 2>(L, t, rb, s);
+    return launch_owner_any<T, TIO, false, kStabS + 2>(L, t, rb, s);
   }
-  if (mode == kStabDir)
-    return launch_owner_any<T, TIO, false, kStabDir>(L, op.tile, op.ra != 0.0, s);
-  return launch_owner_any<T, TIO, false, kStabS>(L, op.tile, op.ra != 0.0, s);
+  if (mode == kStabDir) return launch_owner_any<T, TIO, false, kStabDir>(L, t, rb, s);
+  return launch_owner_any<T, TIO, false, kStabS>(L, t, rb, s);
 }
 
 template <typename T, typename TIO>
@@ -1576,6 +1646,12 @@ int launch_owner_stab_batch(const Member<T> *md, const ActiveSet &as, int tile, 
   L.x1 = r;
   L.y1 = y1;
   L.sd = sd;
+  const TIO *const va[] = {r, y1, sd.po, sd.vo, sd.to, sd.pn, sd.rn, sd.x, sd.rt};
+  const TIO *const vb[] = {r, y1, sd.v, sd.rt};
+  if (mode == kStabDir && in_block(sd.blk, sd.blk_bytes, va, 9, 0))
+    return launch_owner_any<T, TIO, true, kStabDir + kBlk>(L, tile, robin, s);
+  if (mode == kStabS && in_block(sd.blk, sd.blk_bytes, vb, 4, 0))
+    return launch_owner_any<T, TIO, true, kStabS + kBlk>(L, tile, robin, s);
   if (mode == kStabDir) return launch_owner_any<T, TIO, true, kStabDir>(L, tile, robin, s);
   return launch_owner_any<T, TIO, true, kStabS>(L, tile, robin, s);
 }

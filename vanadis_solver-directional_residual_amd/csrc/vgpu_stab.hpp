@@ -132,6 +132,7 @@ struct Stab {
   TV *ax = nullptr, *RT = nullptr, *T = nullptr;
   TV *xw = nullptr;  // the iterate (copied to the caller's x at the end)
   TV *R[2] = {nullptr, nullptr}, *P[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr};
+  VecBlock vblk;  // one allocation for the ten vectors above
   double *prt = nullptr, *prr = nullptr, *pbn = nullptr;  // initial partials [NB]
   // direction launch partials (RT.V, |r|^2), S launch partials (t.s, t.t,
   // r~.s, r~.t) [cap]
@@ -163,9 +164,9 @@ struct Stab {
     NB = stab_blocks(n);
     na = cap = nna;
     TV **v[] = {&ax, &RT, &T, &xw, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
-    for (auto pv : v) {
-      VGPU_HIP(hipMalloc(pv, std::max<int64_t>(1, n) * sizeof(TV)));
-      VGPU_HIP(hipMemset(*pv, 0, std::max<int64_t>(1, n) * sizeof(TV)));
+    if (vblk.alloc<TV>(v, 10, n) != VGPU_OK) {
+      set_error("BiCGSTAB: vector allocation failed");
+      return VGPU_ERR_NOMEM;
     }
     double **pn[] = {&prt, &prr, &pbn};
     for (auto pd : pn) VGPU_HIP(hipMalloc(pd, NB * sizeof(double)));
@@ -191,11 +192,8 @@ struct Stab {
   void release() {
     drop_graph();
     TV **v[] = {&ax, &RT, &T, &xw, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
-    for (auto pv : v)
-      if (*pv) {
-        hipFree(*pv);
-        *pv = nullptr;
-      }
+    for (auto pv : v) *pv = nullptr;
+    vblk.release();
     double **d[] = {&prt, &prr, &pbn, &pga, &prn, &pts, &ptt, &prs, &prq};
     for (auto pd : d)
       if (*pd) {
@@ -346,6 +344,8 @@ struct Stab {
     sa.pout[0] = pga;
     sa.pout[1] = prn;
     sa.st = st;
+    sa.blk = vblk.base;
+    sa.blk_bytes = vblk.bytes;
     sa.tol = tol;
     sa.divtol = divtol;
     sa.itmax = itmax;
@@ -364,6 +364,8 @@ struct Stab {
     double *const bout[4] = {pts, ptt, prs, prq};
     for (int i = 0; i < 4; ++i) sb.pout[i] = bout[i];
     sb.st = st;
+    sb.blk = vblk.base;
+    sb.blk_bytes = vblk.bytes;
     sb.tol = tol;
     sb.divtol = divtol;
     sb.itmax = itmax;
@@ -451,6 +453,7 @@ struct BStab {
   ActiveBuf all;  // every member, for the vector launches
   TV *ax = nullptr, *RT = nullptr, *T = nullptr, *xw = nullptr;
   TV *R[2] = {nullptr, nullptr}, *P[2] = {nullptr, nullptr}, *V[2] = {nullptr, nullptr};
+  VecBlock vblk;  // one allocation for the ten vectors above
   double *prt = nullptr, *prr = nullptr, *pbn = nullptr;
   double *pga = nullptr, *prn = nullptr, *pts = nullptr, *ptt = nullptr, *prs = nullptr,
          *prq = nullptr;
@@ -484,9 +487,9 @@ struct BStab {
     rc = all.set(sg, members);
     if (rc) return rc;
     TV **v[] = {&ax, &RT, &T, &xw, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
-    for (auto pv : v) {
-      VGPU_HIP(hipMalloc(pv, ntot * sizeof(TV)));
-      VGPU_HIP(hipMemset(*pv, 0, ntot * sizeof(TV)));
+    if (vblk.alloc<TV>(v, 10, ntot) != VGPU_OK) {
+      set_error("batched BiCGSTAB: vector allocation failed");
+      return VGPU_ERR_NOMEM;
     }
     double **pn[] = {&prt, &prr, &pbn};
     for (auto pd : pn) VGPU_HIP(hipMalloc(pd, NB * sizeof(double)));
@@ -511,11 +514,8 @@ struct BStab {
   void release() {
     drop_graph();
     TV **v[] = {&ax, &RT, &T, &xw, &R[0], &R[1], &P[0], &P[1], &V[0], &V[1]};
-    for (auto pv : v)
-      if (*pv) {
-        hipFree(*pv);
-        *pv = nullptr;
-      }
+    for (auto pv : v) *pv = nullptr;
+    vblk.release();
     double **d[] = {&prt, &prr, &pbn, &pga, &prn, &pts, &ptt, &prs, &prq};
     for (auto pd : d)
       if (*pd) {
@@ -656,6 +656,8 @@ struct BStab {
     sa.pout[0] = pga;
     sa.pout[1] = prn;
     sa.st = st;
+    sa.blk = vblk.base;
+    sa.blk_bytes = vblk.bytes;
     sa.tol = tol;
     sa.divtol = divtol;
     sa.itmax = itmax;
@@ -672,6 +674,8 @@ struct BStab {
     double *const bout[4] = {pts, ptt, prs, prq};
     for (int i = 0; i < 4; ++i) sb.pout[i] = bout[i];
     sb.st = st;
+    sb.blk = vblk.base;
+    sb.blk_bytes = vblk.bytes;
     sb.tol = tol;
     sb.divtol = divtol;
     sb.itmax = itmax;
