@@ -233,11 +233,16 @@ def _owner_shape(tile):
     return {0: (32, 8), 1: (16, 16), 2: (64, 4)}[int(tile)]
 
 
-# owner-kernel template arguments after <double, double, ...> per kernel
-_TRAFFIC_KERNELS = {"iteration_launch": "true, false, false, 1,",
-                    "dual_apply": "true, false, false, 0,",
-                    "bicgstab_direction_launch": "false, false, false, 2,",
-                    "bicgstab_s_launch": "false, false, false, 3,"}
+# owner-kernel template arguments after <double, double, ...> per kernel: the
+# solver launches in their block-addressed form (mode + 10, the form the
+# solvers and vgpu_time_kernel use) or, for systems whose engine block is too
+# large for it, the plain one
+_TRAFFIC_KERNELS = {"iteration_launch": ("true, false, false, 11,", "true, false, false, 1,"),
+                    "dual_apply": ("true, false, false, 0,",),
+                    "bicgstab_direction_launch": ("false, false, false, 12,",
+                                                  "false, false, false, 2,"),
+                    "bicgstab_s_launch": ("false, false, false, 13,",
+                                          "false, false, false, 3,")}
 
 
 def live_traffic(tuned, N, E, timeout=180):
@@ -292,8 +297,8 @@ def live_traffic(tuned, N, E, timeout=180):
                     if "k_copy_wide" in name:
                         key = "copy"
                     elif "k_owner<double, double," in name and f", {tx}, {ty}>" in name:
-                        for k, sig in _TRAFFIC_KERNELS.items():
-                            if f"k_owner<double, double, {sig}" in name:
+                        for k, sigs in _TRAFFIC_KERNELS.items():
+                            if any(f"k_owner<double, double, {sig}" in name for sig in sigs):
                                 key = k
                     if key:
                         vals.setdefault(key, []).append(float(row["Counter_Value"]))
@@ -1002,6 +1007,9 @@ def main():
             stab_roof[key]["traffic"] = live.get(kname, {}).get("bytes_per_launch")
     else:
         # the profiler is unavailable or was skipped: the committed summary
+        if not a.no_traffic:
+            progress("traffic: no live counter values for the iteration launch; the committed "
+                     "summary is reported")
         traffic = measured_traffic(tuned)
         if traffic:
             out["roofline"]["traffic"] = traffic["bytes_per_launch"]
