@@ -5,7 +5,7 @@
 //
 // Layout.  The systems' vectors are concatenated (system m at nodes
 // [off_m, off_m + n_m)).  System m owns the vector workgroups
-// [vb0_m, vb0_m + nb_m), nb_m = min(256, ceil(n_m / 4096)), each striding over
+// [vb0_m, vb0_m + nb_m), nb_m = min(512, ceil(n_m / 2048)), each striding over
 // the system's own range exactly as a solve of that system alone would, and
 // its reduction partials sit at those workgroup indices; the operator writes
 // its z.pp partials per owner unit.  Every scalar of system m is therefore
@@ -59,13 +59,15 @@ __device__ __forceinline__ void block_partial_at(double v, double *out, int slot
   const int *__restrict__ act, const int *__restrict__ actvb, int nact
 
 // Vector workgroups of the batched engine: BVB threads, at most BNB per
-// system (about four nodes per thread; a large system gets one workgroup of
-// 16 waves per CU).  Every owner launch of an iteration re-sums a system's
-// nb partials of r.r, b.b and z.rr in each of its workgroups, all reading
-// the same lines at the launch's start, so few and large vector workgroups
-// keep that read short.
-constexpr int BVB = 1024;
-constexpr int BNB = 256;
+// system (about four nodes per thread; a large system gets two workgroups of
+// 8 waves per CU).  Every owner launch of an iteration re-sums a system's nb
+// partials of r.r, b.b and z.rr in each of its workgroups, all reading the
+// same lines at the launch's start: fewer partials shorten that read, while
+// workgroups of 16 waves made the update launch itself slower (their block
+// reductions and barriers); 512 x 512 is the measured best of 256 x 1024,
+// 512 x 512 and 1024 x 256 per iteration.
+constexpr int BVB = 512;
+constexpr int BNB = 512;
 
 // r = b - A x0 (ax holds A x0), rr = r; stores only the preconditioned
 // z = r/d, zz = rr/d (the engine carries z, zz: r = d z) and the engine's
