@@ -1530,9 +1530,13 @@ template int launch_owner_batch<float, float>(const Member<float> *, const Activ
 // [blk, blk + bytes), with len bytes from its start (0: the start only, for
 // the batch vectors, which the engines carve whole from their block); the
 // launch then takes the block-addressed kernel (its resource needs the block
-// below 2^31 bytes).
+// below 2^31 bytes).  VGPU_NO_BLOCK_LAUNCH=1 sends every solver launch to the
+// form with a resource per vector, the path of engines whose block is too
+// large, so that it can be checked on small meshes (read at every launch).
 template <typename TIO>
 bool in_block(const void *blk, int64_t bytes, const TIO *const *p, int k, int64_t len) {
+  const char *e = std::getenv("VGPU_NO_BLOCK_LAUNCH");
+  if (e && e[0] == '1') return false;
   if (!blk || bytes <= 0 || bytes >= ((int64_t)1 << 31)) return false;
   const char *b = static_cast<const char *>(blk);
   for (int i = 0; i < k; ++i) {
