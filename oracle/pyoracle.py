@@ -257,9 +257,16 @@ def bicgstab_jacobi(b, x0, tol, itmax, bb, iss, diag, nax, nay, naz, mask, threa
         sh = s_ / d
         t = A(sh)
         tt = t @ t
-        if tt == 0.0 or not np.isfinite((t @ s_) / tt):
+        if tt == 0.0:
+            # s = 0 (t = A D^-1 s vanishes only with s for a regular A): the
+            # step solved the system; Templates' "s small: x += alpha p, stop"
+            # at s = 0, taken as omega = 0 so that r = s and the test above
+            # ends the loop (a singular A breaks down at beta)
+            omega = 0.0
+        elif not np.isfinite((t @ s_) / tt):
             return x, k, err, 3
-        omega = (t @ s_) / tt
+        else:
+            omega = (t @ s_) / tt
         x = x + alpha * ph + omega * sh
         r = s_ - omega * t
         rho_old = rho

@@ -1,0 +1,181 @@
+"""Degenerate and ragged meshes on the GPU against the CPU oracle: one
+element in one, two or three directions, long thin rows along x or y far
+wider than a tile, a single element layer, anisotropic element sizes, and
+every scheme.  On such meshes most nodes are Dirichlet nodes and a tile or
+z chunk is mostly outside the grid, which the interior cases do not reach.
+
+Bars (as for the interior cases): FP64 operator <= 1e-13 relative to the
+oracle's matvec_0 / mttvec_0 for every scatter, owner tile shape and z chunk
+(the owner results bitwise equal across them), zero at Dirichlet nodes;
+solves to rtol 1e-10 within 1e-8 rel-L2 of the oracle's FP64 solve to 1e-12
+for BiCG (every scatter) and within 1e-9 for BiCGSTAB and mixed precision;
+batched solves of all of them bitwise equal to the single solves."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = {
+    "1x1x1": dict(dims=(1, 1, 1)),
+    "1x1x7_theta": dict(dims=(1, 1, 7), scheme=0, dt=0.05, P=0.3),
+    "7x1x1": dict(dims=(7, 1, 1)),
+    "1x9x1_v3": dict(dims=(1, 9, 1), variant=3),
+    "2x2x2_euler": dict(dims=(2, 2, 2), scheme=2, dt=0.01),
+    "3x1x5_aniso": dict(dims=(3, 1, 5), extent=(3.0, 0.5, 0.1)),
+    "70x1x3": dict(dims=(70, 1, 3)),
+    "1x70x3": dict(dims=(1, 70, 3)),
+    "33x17x1_lamfield": dict(dims=(33, 17, 1), lam_range=(1e-4, 1e-2)),
+}
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def make(pkg, name):
+    c = dict(SHAPES[name])
+    dims = c.pop("dims")
+    lam = {} if "lam_range" in c else dict(lam=3e-3)
+    return pkg.problem.unit_box(*dims, vel_seed=5, **lam, **c)
+
+
+@pytest.fixture(scope="module", params=list(SHAPES))
+def shape(request, pkg, po, gpu):
+    name = request.param
+    p = make(pkg, name)
+    bb, b, d = po.assemble(p)
+    ctx = pkg.Context.from_problem(p)
+    yield name, p, bb, b, d, ctx
+    ctx.close()
+
+
+def rhs(po, p, b, d):
+    b, d, x0 = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    if np.linalg.norm(b) == 0.0:
+        b = np.random.default_rng(3).standard_normal(p.N) * (1 - p.mask)
+    return b, d, x0
+
+
+def true_res(po, p, bb, b, x):
+    r = b - po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, False)
+    return np.linalg.norm(r) / np.linalg.norm(b)
+
+
+def oracle_solution(po, p, bb, b, d, x0):
+    """The oracle's FP64 BiCG to 1e-12; where its recursive residual drifts
+    from the true one (the 70x1x3 row: true residual 4.5e-10), its BiCGSTAB
+    to 1e-13 when that is closer to b."""
+    x, it, _, rc = po.dbcg_simple(b, x0, 1e-12, 5000, bb, p.iss, d, p.nax, p.nay, p.naz,
+                                  p.mask)
+    if rc != 0 or true_res(po, p, bb, b, x) > 1e-11:
+        x2, it2, _, _ = po.bicgstab_jacobi(b, x0, 1e-13, 5000, bb, p.iss, d, p.nax, p.nay,
+                                           p.naz, p.mask)
+        if rc != 0 or true_res(po, p, bb, b, x2) < true_res(po, p, bb, b, x):
+            x, it = x2, it2
+    return x, it
+
+
+@pytest.mark.parametrize("scatter", ["owner", "colored", "atomic"])
+def test_apply_matches_oracle(shape, po, scatter):
+    name, p, bb, _, _, ctx = shape
+    log(f"apply {name} {scatter}")
+    x = np.random.default_rng(21).standard_normal(p.N)
+    for tr in (False, True):
+        ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+        y = ctx.apply(x, scatter=scatter, transpose=tr)
+        assert rel(y, ref) <= 1e-13, (tr, rel(y, ref))
+        assert np.all(y[p.mask == 1] == 0.0)
+
+
+def test_owner_tiles_and_chunks(shape, po):
+    """Every owner tile shape with z chunks of 1 and 2 node planes, more
+    planes than the mesh has, and the balanced launch: the same operator,
+    bitwise."""
+    name, p, bb, _, _, ctx = shape
+    x = np.random.default_rng(22).standard_normal(p.N)
+    ref = [po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr) for tr in (False, True)]
+    base = None
+    try:
+        for tile in (0, 1, 2):
+            for zc in (0, 1, 2, 64):
+                log(f"owner {name} tile {tile} zchunk {zc}")
+                ctx.set_options("owner", zc, tile)
+                y = [ctx.apply(x, scatter="owner", transpose=tr) for tr in (False, True)]
+                for tr in (0, 1):
+                    assert rel(y[tr], ref[tr]) <= 1e-13, (tile, zc, tr, rel(y[tr], ref[tr]))
+                if base is None:
+                    base = y
+                assert all(np.array_equal(a, c) for a, c in zip(y, base)), (tile, zc)
+    finally:
+        ctx.set_options("owner", 0, 0)
+
+
+@pytest.mark.parametrize("scatter", ["owner", "colored", "atomic"])
+def test_bicg_matches_oracle(shape, po, scatter):
+    name, p, bb, b, d, ctx = shape
+    b, d, x0 = rhs(po, p, b, d)
+    xo, ito = oracle_solution(po, p, bb, b, d, x0)
+    ctx.set_options(scatter=scatter)
+    try:
+        x, it, err, rc = ctx.solve(b, x0, rtol=1e-10, itmax=5000)
+    finally:
+        ctx.set_options(scatter="owner", zchunk=0, tile=0)
+    log(f"bicg {name} {scatter}: iterations {it} (oracle to 1e-12: {ito}), err {err:.2e}, "
+        f"rel-L2 vs oracle {rel(x, xo):.2e}")
+    assert rc == 0 and err <= 1e-10
+    # 1e-8 (the solution bar against the FP64 CPU solve): on the thin 70x1x3
+    # row BiCG's recursive residual drifts from the true one (the oracle's
+    # own BiCG: true residual 4.5e-10 when the recursive one is 1e-12), and
+    # each scatter's summation order lands elsewhere (1.2e-10..2.3e-9 here)
+    assert rel(x, xo) <= 1e-8
+    assert ctx.true_residual(b, x) <= 1e-8
+    assert np.all(x[p.mask == 1] == 0.0)
+
+
+def test_bicgstab_and_mixed_match_oracle(shape, po, pkg):
+    name, p, bb, b, d, ctx = shape
+    b, d, x0 = rhs(po, p, b, d)
+    xo, _ = oracle_solution(po, p, bb, b, d, x0)
+    ctx.set_method("bicgstab")
+    try:
+        x, it, err, rc = ctx.solve(b, x0, rtol=1e-10, itmax=5000)
+        log(f"bicgstab {name}: iterations {it}, rel-L2 vs oracle {rel(x, xo):.2e}")
+        assert rc == 0 and rel(x, xo) <= 1e-9
+    finally:
+        ctx.set_method("bicg")
+    x, it, err, rc = ctx.solve(b, x0, rtol=1e-10, itmax=5000, precision=pkg.MIXED)
+    log(f"mixed {name}: inner iterations {it}, rel-L2 vs oracle {rel(x, xo):.2e}")
+    assert rc == 0 and rel(x, xo) <= 1e-9
+    assert ctx.true_residual(b, x) <= 1e-9
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_batch_of_edge_shapes_bitwise(pkg, gpu, precision):
+    """All the shapes above in one batch (z-chunk launches, 16x16 tiles):
+    each member bitwise equal to its single solve on its own context."""
+    prec = pkg.FP64 if precision == "fp64" else pkg.MIXED
+    probs = [make(pkg, n) for n in SHAPES]
+    ctxs = [pkg.Context.from_problem(q) for q in probs]
+    try:
+        for c in ctxs:
+            c.set_options("owner", 3, 1)
+        bs = [c.assemble_rhs(q.q_in) for c, q in zip(ctxs, probs)]
+        singles = [c.solve(b, rtol=1e-9, itmax=5000, precision=prec, raise_on_fail=False)
+                   for c, b in zip(ctxs, bs)]
+        log(f"batch of {len(probs)} edge shapes, {precision}")
+        with pkg.Batch(ctxs) as bt:
+            xs, its, errs, st, rc = bt.solve(bs, rtol=1e-9, itmax=5000, precision=prec,
+                                             raise_on_fail=False)
+        log(f"  iterations {list(its)}, status {list(st)}")
+        assert rc == pkg.OK and (st == 1).all(), st
+        for m, (x1, it1, e1, rc1) in enumerate(singles):
+            assert rc1 == pkg.OK, m
+            assert its[m] == it1 and errs[m] == e1, (m, its[m], it1)
+            assert np.array_equal(xs[m], x1), m
+    finally:
+        for c in ctxs:
+            c.close()

@@ -231,6 +231,25 @@ def test_bicgstab_converges_to_direct_solution(pkg, po, dims, seed, lam):
         assert np.linalg.norm(xk - hist[k - 1]) <= 1e-11 * np.linalg.norm(hist[k - 1])
 
 
+def test_bicgstab_exact_step_converges(pkg, po):
+    """A mesh of one element with seven Dirichlet nodes leaves a 1x1 system:
+    BiCGSTAB's first half step solves it exactly (s = 0, so t = 0), which
+    must end the solve converged after one iteration with x = b / A (the
+    Templates' stop on small s), not report a breakdown on t . t = 0."""
+    p = pkg.problem.unit_box(1, 1, 1, vel_seed=5, lam=3e-3)
+    bb, b, d = po.assemble(p)
+    b, d, x0 = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    free = np.flatnonzero(p.mask == 0)
+    assert len(free) == 1
+    x, it, err, st = po.bicgstab_jacobi(b, x0, 1e-12, 50, bb, p.iss, d, p.nax, p.nay,
+                                        p.naz, p.mask)
+    Asys = csr_from_bb(p, bb)
+    f = free[0]
+    assert st == 1 and it == 1 and err == 0.0
+    assert x[f] == pytest.approx(b[f] / Asys[f, f], rel=1e-14)
+    assert np.all(x[p.mask == 1] == 0.0)
+
+
 def test_theta_scheme_assembly(pkg, po):
     """BB = 2H + (3/dt)C and b = 3F - (H - (3/dt)C) T_old per element
     (VANADIS.FOR:723-748), checked against el3d outputs."""

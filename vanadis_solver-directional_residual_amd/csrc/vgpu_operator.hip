@@ -750,11 +750,16 @@ __attribute__((amdgpu_waves_per_eu(
           blk_sum4(p0, n0, s0);
           rho = s0[0];
         } else {
-          omega = sums[0] / sums[1];
+          // t . t = 0: the previous step's S is zero, it solved the system
+          // exactly (t = D^-1 A S vanishes only with S for a regular A):
+          // omega = beta = 0, so R = S, x gains alpha P only, and the S
+          // launch's convergence test ends the solve; were A singular on S,
+          // that launch reports the breakdown (r~ . V = 0)
+          const bool sz = sums[1] == 0.0;
+          omega = sz ? 0.0 : sums[0] / sums[1];
           rho = sums[2] - omega * sums[3];
-          beta = (rho / rho_a) * (al_a / omega);
-          if (!(beta == beta) || !(omega == omega) || omega == 0.0 || rho == 0.0 ||
-              sums[1] == 0.0) {
+          beta = sz ? 0.0 : (rho / rho_a) * (al_a / omega);
+          if (!sz && (!(beta == beta) || !(omega == omega) || omega == 0.0 || rho == 0.0)) {
             if (writer) stm->pend = 3;
             return;
           }

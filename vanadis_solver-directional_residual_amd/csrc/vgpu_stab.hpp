@@ -13,7 +13,11 @@
 //   S = R - alpha V;  R_j = S - omega T;  x += alpha P + omega S
 //   P_j = R_j + beta (P - omega V);  V_j = D^-1 A P_j;  alpha_j = rho_j / (RT . V_j)
 //   S_j = R_j - alpha_j V_j;  T_j = D^-1 A S_j;  omega_j = (t . s) / (t . t)
-// with t = d T, s = d S.  One iteration = the kStabDir owner launch (R_j,
+// with t = d T, s = d S.  A step that solves the system exactly leaves S = 0
+// and t . t = 0: omega and beta are then taken as 0 (R_j = S = 0, x gains
+// alpha P), and the convergence test ends the solve (Barrett et al.,
+// Templates, 1994, sec. 2.3.8: stop when s is small, with x += alpha p).
+// One iteration = the kStabDir owner launch (R_j,
 // P_j and x formed as the plane nodes are loaded, V_j, partials of RT . V_j
 // and |r_j|^2) and the kStabS owner launch (the test of |r_j|, alpha_j, S_j
 // formed on load, T_j, partials of t.s, t.t, r~.s, r~.t); pairs of
