@@ -153,6 +153,28 @@ def test_bicgstab_and_mixed_match_oracle(shape, po, pkg):
     assert ctx.true_residual(b, x) <= 1e-9
 
 
+@pytest.mark.parametrize("scatter", ["colored", "atomic"])
+def test_legacy_drop_in(shape, po, pkg, scatter):
+    """kernel_dbcg_simple_ on the oracle's element matrices (the host's
+    role), where most of the eight colours hold no element: the solution
+    against the oracle's, and (colored) the device's colour lists against
+    the oracle's colouring bit for bit."""
+    name, p, bb, b, d, _ = shape
+    b, d, t = rhs(po, p, b, d)
+    xo, _ = oracle_solution(po, p, bb, b, d, t)
+    x, it, err = pkg.kernel_dbcg_simple(p.N, b, t, 1e-10, 5000, bb, p.iss, d, p.nax, p.nay,
+                                        p.naz, p.mask, scatter=scatter)
+    log(f"legacy {name} {scatter}: iterations {it}, err {err:.2e}, rel-L2 vs oracle "
+        f"{rel(x, xo):.2e}")
+    assert err <= 1e-10
+    assert rel(x, xo) <= 1e-8
+    if scatter == "colored":
+        el, off, cnt = pkg._lib.legacy_device_color_lists(p.E)
+        _, oel, ooff, ocnt = po.colors(p.nax, p.nay, p.naz)
+        assert np.array_equal(el, oel) and np.array_equal(off, ooff)
+        assert np.array_equal(cnt, ocnt)
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_batch_of_edge_shapes_bitwise(pkg, gpu, precision):
     """All the shapes above in one batch (z-chunk launches, 16x16 tiles):
