@@ -75,8 +75,10 @@ int vgpu_device_count(int *count);
  * (row-major per element, BB[64e+8i+j]), ISS column-major 1-based, stop when
  * ||r||/||b|| <= TOL on the recursive residual; X in/out, zeroed at masked
  * nodes.  Differences, by design: ITER/ERR are also written when x0 already
- * satisfies TOL (ITER=0), and a HIP failure sets ERR=+Inf so the host takes
- * its "stopping criterion not satisfied" path.  Scatter mode: environment
+ * satisfies TOL (ITER=0), a HIP failure sets ERR=+Inf so the host takes
+ * its "stopping criterion not satisfied" path, and B = 0 returns the exact
+ * X = 0 with ITER=0, ERR=0 (the reference's loop divides 0 by 0 there and
+ * leaves NaN in X).  Scatter mode: environment
  * VGPU_LEGACY_SCATTER=atomic|colored (default colored) or
  * vgpu_legacy_set_scatter().  Device buffers persist between calls with the
  * same sizes (one cache per process; not re-entrant, like the reference).
@@ -121,8 +123,9 @@ int vgpu_legacy_last_times(double *upload_s, double *solve_s, double *download_s
  * device, held until the process ends): N may not exceed
  * vgpu_noalloc_max_nodes() (2^18; larger N: VGPU_ERR_ARG).  Calls on one
  * device share that array: calls that overlap on one device (other host
- * threads or streams) race.  Returns VGPU_OK (converged), VGPU_NOT_CONVERGED
- * (itmax), VGPU_BREAKDOWN or VGPU_ERR_ARG / VGPU_ERR_HIP. */
+ * threads or streams) race.  Returns VGPU_OK (converged; b = 0: x = 0,
+ * iter 0, err 0), VGPU_NOT_CONVERGED (itmax), VGPU_BREAKDOWN or
+ * VGPU_ERR_ARG / VGPU_ERR_HIP. */
 int kernel_dbcg_simple_noalloc(int n, const double *b_dev, double *x_dev, double tol,
                                int itmax, int *iter, double *err, const double *bb_dev,
                                const int *iss_dev, const double *diag_dev, int nax, int nay,
@@ -207,7 +210,10 @@ int vgpu_assemble_rhs(vgpu_ctx *ctx, const double *q_in, const double *t_old,
  * the solve finishes with FP64 BiCGSTAB passes, or the context's method for
  * systems too large for the owner launches).  iters = total
  * Krylov iterations, err = final relative residual (recursive for FP64 and
- * FP32, true for MIXED). */
+ * FP32, true for MIXED).  VGPU_MIXED checks itmax between refinement steps,
+ * and an inner solve runs at most 1000 iterations, so iters can exceed itmax
+ * by up to 1000.  b = 0: x = 0, the exact solution from any start, with
+ * iters 0, err 0 and VGPU_OK (the relative residual is undefined there). */
 int vgpu_solve(vgpu_ctx *ctx, int precision, const double *b, double *x,
                double rtol, int itmax, double inner_rtol, int *iters,
                double *err);
@@ -330,8 +336,9 @@ int vgpu_batch_destroy(vgpu_batch *batch);
  * hipGraph); with the same tile shape and a z chunk > 0 on every member,
  * each member's result is bitwise vgpu_solve's on its context; iters = inner
  * + FP64 iterations, errs = true relative residuals.  iters/errs/status
- * [nm]: status 1 converged, 2 itmax, 3 breakdown.  Returns VGPU_OK when
- * every system converged. */
+ * [nm]: status 1 converged, 2 itmax, 3 breakdown; a member with b = 0 gets
+ * x = 0, status 1, iters 0, errs 0.  Returns VGPU_OK when every system
+ * converged. */
 int vgpu_batch_solve(vgpu_batch *batch, int precision, const double *const *b,
                      double *const *x, double rtol, int itmax, int *iters,
                      double *errs, int *status);

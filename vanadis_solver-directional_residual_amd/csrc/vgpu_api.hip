@@ -221,6 +221,29 @@ int nparts_for(int64_t n) {
   return (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));
 }
 
+// A zero right-hand side: x = 0 solves A x = b exactly from any start, while
+// the solvers' relative residual |r| / |b| is 0/0 or |r|/0 there and ends
+// them in a breakdown.  Called for a solve that ended in one: with |b| = 0
+// (b and x of n entries on the context's device and stream) x is zeroed and
+// *zero set, so that the caller reports the solve converged after 0
+// iterations with err 0.
+int zero_rhs_solution(vgpu_ctx *c, const double *b, double *x, int64_t n, bool *zero) {
+  const int np = nparts_for(n);
+  hipLaunchKernelGGL((k_dot<double>), dim3(np), dim3(VB), 0, c->stream, n, nullptr, b, b,
+                     c->red);
+  VGPU_HIP(hipGetLastError());
+  int rc = VGPU_OK;
+  const double bb = host_sum(c->red, np, c->stream, &rc);
+  if (rc) return rc;
+  *zero = bb == 0.0;
+  if (*zero) {
+    hipLaunchKernelGGL(k_fill, grid_for(n), dim3(256), 0, c->stream, n, 0.0, x);
+    VGPU_HIP(hipGetLastError());
+    VGPU_HIP(hipStreamSynchronize(c->stream));
+  }
+  return VGPU_OK;
+}
+
 // FP64 dual-operator callback for the BiCG on the context's operator
 DualOp<double> make_dual64(vgpu_ctx *c, int scatter) {
   return [c, scatter](const double *p, double *z, const double *pp, double *zz,
@@ -452,6 +475,15 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
   VGPU_HIP(hipGetLastError());
   const double bn = std::sqrt(host_sum(c->red, np, c->stream, &rc));
   if (rc) return rc;
+  if (bn == 0.0) {  // b = 0: x = 0 exactly, from any start
+    hipLaunchKernelGGL(k_fill, grid_for(N), dim3(256), 0, c->stream, N, 0.0, x);
+    VGPU_HIP(hipGetLastError());
+    VGPU_HIP(hipStreamSynchronize(c->stream));
+    *iters = 0;
+    *err = 0.0;
+    *status = 1;
+    return VGPU_OK;
+  }
   DualOp<float> dual = make_dual32(c, scatter);
   SingleOp<float> single = [c, scatter](const float *xx, float *y,
                                         hipStream_t s) {
@@ -1102,6 +1134,16 @@ int vgpu_solve_dev(vgpu_ctx *c, int precision, const double *b, double *x,
   VGPU_HIP(hipStreamSynchronize(c->stream));
   const auto t1 = std::chrono::steady_clock::now();
   if (seconds) *seconds = std::chrono::duration<double>(t1 - t0).count();
+  if (status >= 3) {  // a breakdown on b = 0 is the exact solution x = 0
+    bool zero = false;
+    rc = zero_rhs_solution(c, b, x, c->g.N, &zero);
+    if (rc) return rc;
+    if (zero) {
+      *iters = 0;
+      *err = 0.0;
+      status = 1;
+    }
+  }
   if (status == 1) return VGPU_OK;
   if (status == 3) {
     set_error("BiCG breakdown (zero or non-finite scalar)");
@@ -1294,7 +1336,17 @@ int batch_solve_mixed(vgpu_batch *bt, const double *b, double *x, double rtol, i
   VGPU_HIP(hipGetLastError());
   rc = member_sums(sums);
   if (rc) return rc;
-  for (int m = 0; m < nm; ++m) bn[m] = std::sqrt(sums[m]);
+  for (int m = 0; m < nm; ++m) {
+    bn[m] = std::sqrt(sums[m]);
+    if (bn[m] == 0.0) {  // b = 0: x = 0 exactly, from any start
+      const Segment &sg = bt->seg[m];
+      hipLaunchKernelGGL(k_fill, grid_for(sg.n), dim3(256), 0, s, (int64_t)sg.n, 0.0,
+                         x + sg.off);
+      status[m] = 1;
+      running[m] = 0;
+    }
+  }
+  VGPU_HIP(hipGetLastError());
   // true residuals r = b - A x of every member (in vr) and their norms
   auto true_rel = [&]() -> int {
     int r = launch_owner_batch<double, double>(bt->md64, all, bt->tile, bt->zchunk, bt->robin,
@@ -1311,7 +1363,7 @@ int batch_solve_mixed(vgpu_batch *bt, const double *b, double *x, double rtol, i
     if (r) return r;
     for (int m = 0; m < nm; ++m) {
       rn[m] = std::sqrt(sums[m]);
-      rel[m] = rn[m] / bn[m];
+      rel[m] = bn[m] > 0.0 ? rn[m] / bn[m] : 0.0;  // b = 0: x = 0, set above
     }
     return VGPU_OK;
   };
@@ -1548,6 +1600,25 @@ int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double 
   }
   if (rc) return rc;
   VGPU_HIP(hipStreamSynchronize(bt->stream));
+  // members that ended in a breakdown on b = 0: x = 0 is their exact
+  // solution, converged after 0 iterations
+  std::vector<char> zero(bt->m.size(), 0);
+  for (size_t i = 0; i < bt->m.size(); ++i) {
+    if (status[i] < 3) continue;
+    vgpu_ctx *c = bt->m[i];
+    const Segment &sg = bt->seg[i];
+    bool z = false;
+    VGPU_HIP(hipSetDevice(c->device));
+    rc = zero_rhs_solution(c, b + sg.off, x + sg.off, sg.n, &z);
+    if (rc) return rc;
+    if (z) {
+      zero[i] = 1;
+      iters[i] = 0;
+      errs[i] = 0.0;
+      status[i] = 1;
+    }
+  }
+  VGPU_HIP(hipSetDevice(bt->device));
   // FP64 members the batched BiCG left at itmax or in breakdown, or stopped
   // on its recursive residual while the true one, |b - A x| / |b|, is above
   // rtol (the recurrence drifts from the truth on members whose residual
@@ -1556,7 +1627,7 @@ int vgpu_batch_solve_dev(vgpu_batch *bt, int precision, const double *b, double 
   // above 1 or not finite), to the same rtol
   if (precision == VGPU_FP64 && bt->fallback == VGPU_METHOD_BICGSTAB) {
     for (size_t i = 0; i < bt->m.size(); ++i) {
-      if (status[i] != 1 && status[i] != 2 && status[i] != 3) continue;
+      if (zero[i] || (status[i] != 1 && status[i] != 2 && status[i] != 3)) continue;
       vgpu_ctx *c = bt->m[i];
       const Segment &sg = bt->seg[i];
       int it = 0, st = 0;

@@ -9,6 +9,7 @@
 // (generate_hex8_colors / build_color_element_lists, lines 50-106); the BiCG
 // is the graph-captured engine of vgpu_bicg.hpp; device buffers persist
 // between calls of the same size instead of 19 allocations per call.
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -276,6 +277,15 @@ int legacy_solve(int N, const double *B, double *X, double tol, int itmax,
   rc = L.bicg.solve((uint64_t)(100 + L.scatter), dual, single, L.mask, L.b, L.x,
                     L.diag, tol, itmax, true, s, iter, err, &status);
   if (rc) return rc;
+  // b = 0 ends BiCG in a breakdown (|r| / |b| = 0/0 or |r|/0; the reference's
+  // loop turns x into NaN there): x = 0 is the exact solution, returned as
+  // converged after 0 iterations with err 0
+  if (status >= 3 && std::all_of(B, B + N, [](double v) { return v == 0.0; })) {
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, nullptr, L.x, nullptr, (int64_t)N);
+    VGPU_HIP(hipGetLastError());
+    *iter = 0;
+    *err = 0.0;
+  }
   const auto t2 = clk::now();
   VGPU_HIP(hipMemcpyAsync(X, L.x, N * sizeof(double), hipMemcpyDeviceToHost, s));
   VGPU_HIP(hipStreamSynchronize(s));
@@ -460,6 +470,14 @@ int legacy_solve_noalloc(int N, const double *b, double *x, double tol, int itma
   }
   *iter = hs.iter;
   *err = hs.err;
+  if (hs.done >= 3 && hs.bnrm2 == 0.0) {  // b = 0: x = 0 exactly (as the allocating form)
+    hipLaunchKernelGGL(k_zero, dim3(256), dim3(256), 0, s, nullptr, x, nullptr, n);
+    VGPU_HIP(hipGetLastError());
+    VGPU_HIP(hipStreamSynchronize(s));
+    *iter = 0;
+    *err = 0.0;
+    return VGPU_OK;
+  }
   if (hs.done == 1) return VGPU_OK;
   if (hs.done == 3) {
     set_error("kernel_dbcg_simple_noalloc: BiCG breakdown");
