@@ -153,6 +153,29 @@ def test_bicgstab_and_mixed_match_oracle(shape, po, pkg):
     assert ctx.true_residual(b, x) <= 1e-9
 
 
+@pytest.mark.parametrize("scatter", ["owner", "colored", "atomic"])
+def test_autotuners(shape, po, pkg, tmp_path, scatter):
+    """The owner tuner (FP64 and FP32) and the brick tuners on these meshes,
+    where most tile shapes and z chunks cover little but the outside of the
+    grid: a valid choice, and the operator with it still the oracle's."""
+    name, p, bb, _, _, ctx = shape
+    x = np.random.default_rng(24).standard_normal(p.N)
+    cache = str(tmp_path / "tune.json")
+    try:
+        for prec in ((pkg.FP64, pkg.FP32) if scatter == "owner" else (pkg.FP64,)):
+            kw = {} if scatter == "owner" else dict(scatter=scatter)
+            r = pkg.autotune.tune(ctx, prec, cache_path=cache, reps=2, **kw)
+            log(f"autotune {name} {scatter} precision {prec}: tile {r['tile']} zchunk "
+                f"{r['zchunk']} {r['ms'] * 1e3:.1f} us")
+            assert r["tile"] in (0, 1, 2) and r["zchunk"] >= 0 and r["ms"] > 0
+            for tr in (False, True):
+                ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask, tr)
+                y = ctx.apply(x, scatter=scatter, transpose=tr)
+                assert rel(y, ref) <= 1e-13, (tr, rel(y, ref))
+    finally:
+        ctx.set_options("owner", 0, 0)
+
+
 @pytest.mark.parametrize("scatter", ["colored", "atomic"])
 def test_legacy_drop_in(shape, po, pkg, scatter):
     """kernel_dbcg_simple_ on the oracle's element matrices (the host's
