@@ -153,6 +153,40 @@ def test_bicgstab_and_mixed_match_oracle(shape, po, pkg):
     assert ctx.true_residual(b, x) <= 1e-9
 
 
+def test_assembly_and_diagnostics(shape, po):
+    """Diagonal, right-hand side with an old state, mass integral, diagonal
+    dominance factor and flag, and damped Jacobi sweeps on the device
+    against the oracle."""
+    name, p, bb, b, d, ctx = shape
+    log(f"assembly and diagnostics {name}")
+    _, dref, _ = po.impose_mask(p.mask, b, d, np.zeros(p.N))
+    assert rel(ctx.diag(), dref) <= 1e-14
+    t_old = np.random.default_rng(25).standard_normal(p.N) * (1 - p.mask)
+    _, bref, dref2 = po.assemble(p, t_old=t_old)
+    bref, _, _ = po.impose_mask(p.mask, bref, dref2, t_old)
+    assert rel(ctx.assemble_rhs(p.q_in, t_old), bref) <= 1e-13
+    c = np.random.default_rng(26).random(p.N)
+    assert ctx.mass(c) == pytest.approx(po.mass(p.iss, p.xx, p.yy, p.zz, c, p.E), rel=1e-13)
+    dd, dom = ctx.ddf()
+    assert dd == pytest.approx(po.ddf(p.E, p.N, p.iss, bb), rel=1e-12)
+    assert dom == po.is_dd(p.E, p.N, p.iss, bb)
+    bm, dm, _ = rhs(po, p, b, d)
+    x0 = np.random.default_rng(27).standard_normal(p.N) * (1 - p.mask)
+    xr = x0.copy()
+    for _ in range(2):
+        ax = po.matvec_0(xr, p.iss, bb, p.nax, p.nay, p.naz, p.mask)
+        xr = np.where(p.mask == 1, 0.0, xr + 0.8 * (bm - ax) / dm)
+    bd, xd = ctx.dev_alloc(8 * p.N), ctx.dev_alloc(8 * p.N)
+    try:
+        ctx.h2d(bd, bm)
+        ctx.h2d(xd, x0)
+        ctx.jacobi_dev(bd, xd, 2, 0.8)
+        assert rel(ctx.d2h(xd, p.N), xr) <= 1e-13
+    finally:
+        ctx.dev_free(bd)
+        ctx.dev_free(xd)
+
+
 @pytest.mark.parametrize("scatter", ["owner", "colored", "atomic"])
 def test_autotuners(shape, po, pkg, tmp_path, scatter):
     """The owner tuner (FP64 and FP32) and the brick tuners on these meshes,
