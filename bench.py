@@ -380,7 +380,7 @@ def config3(pkg, sols):
                                (3, "bicg_iteration_launch", 81 * N + 40 * E),
                                (5, "bicgstab_direction_launch", 89 * N + 40 * E),
                                (6, "bicgstab_s_launch", 41 * N + 40 * E)):
-        ms = ctx.time_kernel(what, "owner", pkg.FP64, 20)
+        ms = float(np.median([ctx.time_kernel(what, "owner", pkg.FP64, 20) for _ in range(3)]))
         gbs = nbytes / (ms * 1e-3) / 1e9
         roof[name] = {"ms": round(ms, 5), "algorithmic_bytes_per_launch": nbytes,
                       "achieved": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
@@ -832,16 +832,23 @@ def main():
     else:
         stab_status = "skipped"
 
-    # kernels, HIP events on the library stream
+    # kernels, HIP events on the library stream: each figure the median of
+    # three timings of `reps` back-to-back launches (one timing can catch a
+    # clock transient)
     reps = 50
-    t_single = ctx.time_kernel(0, "owner", pkg.FP64, reps)
-    t_dual = ctx.time_kernel(1, "owner", pkg.FP64, reps)
-    t_iter = ctx.time_kernel(3, "owner", pkg.FP64, reps)
-    t_copy = ctx.time_kernel(2, "owner", pkg.FP64, reps)
-    t_copy8 = ctx.time_kernel(4, "owner", pkg.FP64, 20)
-    t_copyh = ctx.time_kernel(7, "owner", pkg.FP64, 10)
-    t_sdir = ctx.time_kernel(5, "owner", pkg.FP64, reps)
-    t_ss = ctx.time_kernel(6, "owner", pkg.FP64, reps)
+
+    def timed(what, n_launch):
+        return float(np.median([ctx.time_kernel(what, "owner", pkg.FP64, n_launch)
+                                for _ in range(3)]))
+
+    t_single = timed(0, reps)
+    t_dual = timed(1, reps)
+    t_iter = timed(3, reps)
+    t_copy = timed(2, reps)
+    t_copy8 = timed(4, 20)
+    t_copyh = timed(7, 10)
+    t_sdir = timed(5, reps)
+    t_ss = timed(6, reps)
     try:
         conv = conv_layers(pkg, ctx, N)
     except Exception as exc:  # reported, never fatal for the line
