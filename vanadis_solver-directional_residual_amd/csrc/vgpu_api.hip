@@ -221,6 +221,18 @@ int nparts_for(int64_t n) {
   return (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));
 }
 
+// The context's FP32 right-hand side and iterate, allocated on first use;
+// one that failed is tried again by the next call (no half-set pair is used).
+int ensure_f32_vectors(vgpu_ctx *c) {
+  const int64_t N = c->g.N;
+  if (!c->fb && hipMalloc(&c->fb, N * sizeof(float)) != hipSuccess) c->fb = nullptr;
+  if (c->fb && !c->fx && hipMalloc(&c->fx, N * sizeof(float)) != hipSuccess) c->fx = nullptr;
+  if (c->fb && c->fx) return VGPU_OK;
+  (void)hipGetLastError();
+  set_error("FP32 vector allocation failed");
+  return VGPU_ERR_NOMEM;
+}
+
 // A zero right-hand side: x = 0 solves A x = b exactly from any start, while
 // the solvers' relative residual |r| / |b| is 0/0 or |r|/0 there and ends
 // them in a breakdown.  Called for a solve that ended in one: with |b| = 0
@@ -401,10 +413,7 @@ int solve_fp32_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
     int rc = c->bicg32.alloc(N, max_owner_tiles(c->g) + 256);
     if (rc) return rc;
   }
-  if (!c->fb) {
-    VGPU_HIP(hipMalloc(&c->fb, N * sizeof(float)));
-    VGPU_HIP(hipMalloc(&c->fx, N * sizeof(float)));
-  }
+  if (int r = ensure_f32_vectors(c)) return r;
   hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
                      c->stream, N, b, c->fb);
   hipLaunchKernelGGL((k_convert<double, float>), grid_for(N), dim3(256), 0,
@@ -463,10 +472,7 @@ int solve_mixed_dev(vgpu_ctx *c, const double *b, double *x, double rtol,
     int rc = c->bicg32.alloc(N, max_owner_tiles(c->g) + 256);
     if (rc) return rc;
   }
-  if (!c->fb) {
-    VGPU_HIP(hipMalloc(&c->fb, N * sizeof(float)));
-    VGPU_HIP(hipMalloc(&c->fx, N * sizeof(float)));
-  }
+  if (int r = ensure_f32_vectors(c)) return r;
   const int np = nparts_for(N);
   int rc = VGPU_OK;
   // ||b||
@@ -2003,9 +2009,8 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   // deterministic non-trivial inputs
   hipLaunchKernelGGL(k_lincomb, grid_for(N), dim3(256), 0, c->stream, N, c->mask,
                      0.0, c->vb, 0.0, nullptr, c->vt);
-  if (precision == VGPU_FP32 && !c->fb) {
-    VGPU_HIP(hipMalloc(&c->fb, N * sizeof(float)));
-    VGPU_HIP(hipMalloc(&c->fx, N * sizeof(float)));
+  if (precision == VGPU_FP32) {
+    if (int r = ensure_f32_vectors(c)) return r;
   }
   float *f1 = nullptr, *f2 = nullptr, *f3 = nullptr, *f4 = nullptr;
   if (precision == VGPU_FP32) {

@@ -373,8 +373,15 @@ struct BBicg {
     return true;
   }
 
+  // A failed allocation releases everything, so that the next call
+  // allocates again instead of finding a half-built engine.
   int alloc(const std::vector<Segment> &s) {
     if (seg && matches(s)) return VGPU_OK;
+    const int rc = alloc_parts(s);
+    if (rc) release();
+    return rc;
+  }
+  int alloc_parts(const std::vector<Segment> &s) {
     release();
     hseg = s;
     nm = (int)s.size();

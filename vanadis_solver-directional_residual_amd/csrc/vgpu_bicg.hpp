@@ -279,7 +279,14 @@ struct Bicg {
   hipGraph_t graph = nullptr;
   int pairs_per_graph = 4;
 
+  // (a failed allocation releases everything, n = 0: the next call starts
+  // again)
   int alloc(int64_t n_, int na_cap) {
+    const int rc = alloc_parts(n_, na_cap);
+    if (rc) release();
+    return rc;
+  }
+  int alloc_parts(int64_t n_, int na_cap) {
     release();
     n = n_;
     nb = (int)std::min<int64_t>(256, std::max<int64_t>(1, (n + VB - 1) / VB));

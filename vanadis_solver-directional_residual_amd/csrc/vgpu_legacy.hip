@@ -162,10 +162,21 @@ struct Legacy {
     N = 0;
     E = 0;
   }
+  // (a failed allocation releases everything, so that the next call, the
+  // host's next time step, allocates again instead of finding null buffers)
   int ensure(int N_, int nax_, int nay_, int naz_) {
     const int64_t E_ = (int64_t)(nax_ - 1) * (nay_ - 1) * (naz_ - 1);
     if (N_ == N && E_ == E && nax_ == nax && nay_ == nay && naz_ == naz && s)
       return VGPU_OK;
+    const int rc = ensure_parts(N_, nax_, nay_, naz_, E_);
+    if (rc) {
+      const int keep_scatter = scatter;
+      release();
+      scatter = keep_scatter;
+    }
+    return rc;
+  }
+  int ensure_parts(int N_, int nax_, int nay_, int naz_, int64_t E_) {
     const int keep_scatter = scatter;
     release();
     scatter = keep_scatter;

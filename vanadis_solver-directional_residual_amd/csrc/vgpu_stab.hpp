@@ -156,6 +156,7 @@ struct Stab {
 
   // n nodes, na owner dot partials per launch (the launches' count: units or
   // (chunk, tile) groups)
+  // (a failed allocation releases everything: the next call starts again)
   int alloc(int64_t nn, int nna) {
     nna = std::max(1, nna);
     if (RT && nn == n && nna <= cap) {
@@ -163,6 +164,11 @@ struct Stab {
       na = nna;
       return VGPU_OK;
     }
+    const int rc = alloc_parts(nn, nna);
+    if (rc) release();
+    return rc;
+  }
+  int alloc_parts(int64_t nn, int nna) {
     release();
     n = nn;
     NB = stab_blocks(n);
@@ -476,6 +482,12 @@ struct BStab {
                sg[i].ng == hseg[i].ng;
       if (same) return VGPU_OK;
     }
+    // a failed allocation releases everything: the next call starts again
+    const int rc = alloc_parts(sg);
+    if (rc) release();
+    return rc;
+  }
+  int alloc_parts(const std::vector<Segment> &sg) {
     release();
     hseg = sg;
     nm = (int)sg.size();
