@@ -70,3 +70,15 @@ def test_tfile_fixed_examples(pkg, tmp_path):
     lines = (tmp_path / "t.txt").read_text().splitlines()
     assert [ln[:12] for ln in lines] == [" 0.10000E+01", "-0.12346E-02", " 0.10000E+02",
                                         " 0.10000-119", " 0.00000E+00"]
+
+
+def test_tfile_special_values(pkg, tmp_path):
+    """NaN and infinities as gfortran writes them in e12.5 (the reference's
+    build compiles VANADIS.FOR with gfortran, makefile:296: "NaN",
+    "Infinity", "-Infinity", right-justified; flang would write "Inf");
+    subnormals and the largest double with 3-digit exponents."""
+    v = np.array([np.nan, np.inf, -np.inf, 5e-324, 1.7976931348623157e308, -1e-310])
+    pkg._lib.write_tfile(str(tmp_path / "t.txt"), v, v, v, np.zeros(6))
+    lines = (tmp_path / "t.txt").read_text().splitlines()
+    assert [ln[:12] for ln in lines] == ["         NaN", "    Infinity", "   -Infinity",
+                                        " 0.49407-323", " 0.17977+309", "-0.10000-309"]

@@ -1761,7 +1761,10 @@ int vgpu_true_residual(vgpu_ctx *c, const double *b, const double *x,
 namespace {
 // Fortran Ew.d edit descriptor with w = 12, d = 5: [-]0.ddddd E+ee, right
 // justified; exponents beyond 99 drop the E ([-]0.ddddd+eee); zero prints as
-// 0.00000E+00; NaN and infinities right-justified.
+// 0.00000E+00; NaN and infinities right-justified, spelled as gfortran
+// spells them in a 12-wide field (the reference's build compiles VANADIS.FOR
+// with gfortran, makefile:296): "NaN", "Infinity", "-Infinity" (flang writes
+// "Inf" / "-Inf" instead).
 void fortran_e12_5(double v, char out[13]) {
   char t[40];
   if (std::isnan(v)) {
