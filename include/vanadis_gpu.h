@@ -321,10 +321,12 @@ int vgpu_jacobi_dev(vgpu_ctx *ctx, const double *b, double *x, int sweeps, doubl
  * several contexts).  Every BiCG iteration runs one owner launch over all
  * systems (convergence test, direction update and dual operator) and one
  * vector launch (update), captured in one hipGraph; the systems stop
- * independently.  The batch uses the owner tile shape and z
- * chunk of ctxs[0]; with the same tile shape each system's iterates are
+ * independently.  The batch uses the owner tile shape and z chunk that
+ * ctxs[0] has when the batch is created (later vgpu_set_options calls do not
+ * change the batch's); with the same tile shape each system's iterates are
  * bitwise those of vgpu_solve on its context.  Contexts must outlive the
- * batch; coefficient or option changes are picked up at the next solve. */
+ * batch; coefficient, scheme and diagonal changes of any member are picked
+ * up at the next solve. */
 typedef struct vgpu_batch vgpu_batch;
 int vgpu_batch_create(int nm, vgpu_ctx *const *ctxs, vgpu_batch **out);
 int vgpu_batch_destroy(vgpu_batch *batch);
