@@ -2151,7 +2151,21 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   // working set (buffers held for this call only)
   // what 7: a copy of 64 N doubles (0.55 GB each way at 128x128x64, well
   // above the 256 MB last-level cache): the practical HBM roof
-  double *big = nullptr;
+  // the copy buffer, graph and events of this call, released on every return
+  struct Held {
+    double *big = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Held() {
+      if (gexec) hipGraphExecDestroy(gexec);
+      if (graph) hipGraphDestroy(graph);
+      if (e0) hipEventDestroy(e0);
+      if (e1) hipEventDestroy(e1);
+      if (big) hipFree(big);
+    }
+  } held;
+  double *&big = held.big;
   if (what == 4) VGPU_HIP(hipMalloc(&big, 16 * N * sizeof(double)));
   if (what == 7) VGPU_HIP(hipMalloc(&big, 128 * N * sizeof(double)));
   auto launch = [&]() -> int {
@@ -2205,34 +2219,23 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   }
   // the timed launches are captured in one graph, so the events bracket GPU
   // work only (host enqueue gaps between launches do not count)
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
   VGPU_HIP(hipStreamSynchronize(c->stream));
   VGPU_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int lrc = VGPU_OK;
   for (int i = 0; i < reps && lrc == VGPU_OK; ++i) lrc = launch();
-  const hipError_t ce = hipStreamEndCapture(c->stream, &graph);
-  if (lrc) {
-    if (graph) hipGraphDestroy(graph);
-    return lrc;
-  }
+  const hipError_t ce = hipStreamEndCapture(c->stream, &held.graph);
+  if (lrc) return lrc;
   VGPU_HIP(ce);
-  VGPU_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
-  hipEvent_t e0, e1;
-  VGPU_HIP(hipEventCreate(&e0));
-  VGPU_HIP(hipEventCreate(&e1));
-  VGPU_HIP(hipGraphLaunch(gexec, c->stream));  // warm-up replay
-  VGPU_HIP(hipEventRecord(e0, c->stream));
-  VGPU_HIP(hipGraphLaunch(gexec, c->stream));
-  VGPU_HIP(hipEventRecord(e1, c->stream));
-  VGPU_HIP(hipEventSynchronize(e1));
+  VGPU_HIP(hipGraphInstantiate(&held.gexec, held.graph, nullptr, nullptr, 0));
+  VGPU_HIP(hipEventCreate(&held.e0));
+  VGPU_HIP(hipEventCreate(&held.e1));
+  VGPU_HIP(hipGraphLaunch(held.gexec, c->stream));  // warm-up replay
+  VGPU_HIP(hipEventRecord(held.e0, c->stream));
+  VGPU_HIP(hipGraphLaunch(held.gexec, c->stream));
+  VGPU_HIP(hipEventRecord(held.e1, c->stream));
+  VGPU_HIP(hipEventSynchronize(held.e1));
   float ms = 0;
-  VGPU_HIP(hipEventElapsedTime(&ms, e0, e1));
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
-  hipGraphExecDestroy(gexec);
-  hipGraphDestroy(graph);
-  if (big) VGPU_HIP(hipFree(big));
+  VGPU_HIP(hipEventElapsedTime(&ms, held.e0, held.e1));
   *avg_ms = ms / reps;
   return VGPU_OK;
 }
