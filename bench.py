@@ -649,6 +649,42 @@ def own_tile_default(tuned, a):
     return tuned["tile"] if tuned else max(a.tile, 0)
 
 
+def shard_shares(pkg):
+    """One GPU's share of the 256x256x64 workload split over 8 GPUs, three
+    ways (z-slab 256x256x8, y-slab 256x32x64, pencil 128x64x64; the element
+    sizes of the whole mesh, the per-element diffusivity field), against the
+    whole mesh on one GPU: the owner launches at the autotuner's tile and z
+    chunk, HIP events, the median of three timings of 20 launches.  The
+    exchange of boundary planes a multi-GPU run adds is not part of it."""
+    out = {"note": "owner launches of one GPU's share of 256x256x64 over 8 GPUs; "
+                   "no halo exchange", "shares": {}}
+    for name, size, ext in (("whole_256x256x64", (256, 256, 64), (1.0, 1.0, 1.0)),
+                            ("z_slab_256x256x8", (256, 256, 8), (1.0, 1.0, 0.125)),
+                            ("y_slab_256x32x64", (256, 32, 64), (1.0, 0.125, 1.0)),
+                            ("pencil_128x64x64", (128, 64, 64), (0.5, 0.25, 1.0))):
+        p = pkg.problem.unit_box(*size, vel_seed=0, lam_range=(1e-4, 1e-2), extent=ext)
+        N, E = p.N, p.E
+        with pkg.Context.from_problem(p) as ctx:
+            t = ctx.autotune(pkg.FP64, reps=20)
+            r = {"nodes": N, "elements": E, "tile": t["tile"], "zchunk": t["zchunk"]}
+            for what, kind, nbytes in ((0, "single_apply", 17 * N + 40 * E),
+                                       (1, "dual_apply", 33 * N + 40 * E),
+                                       (3, "bicg_iteration_launch", 81 * N + 40 * E),
+                                       (5, "bicgstab_direction_launch", 89 * N + 40 * E),
+                                       (6, "bicgstab_s_launch", 41 * N + 40 * E)):
+                ms = float(np.median([ctx.time_kernel(what, "owner", pkg.FP64, 20)
+                                      for _ in range(3)]))
+                r[kind] = {"us": round(ms * 1e3, 2),
+                           "frac": round(nbytes / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            out["shares"][name] = r
+            progress(f"{name}: BiCG iteration launch {r['bicg_iteration_launch']['us']} us")
+    whole = out["shares"]["whole_256x256x64"]["bicg_iteration_launch"]["us"]
+    for name, r in out["shares"].items():
+        if name != "whole_256x256x64":
+            r["iteration_launch_vs_whole_over_8"] = round(whole / 8 / r["bicg_iteration_launch"]["us"], 3)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -679,6 +715,9 @@ def main():
     ap.add_argument("--no-traffic", action="store_true",
                     help="skip the two rocprofv3 counter passes that measure the owner "
                          "launches' HBM bytes (then the committed summary is reported)")
+    ap.add_argument("--shard-shares", action="store_true",
+                    help="only time the owner launches on one GPU's share of 256x256x64 "
+                         "over 8 GPUs (z-slab, y-slab, pencil) and print them as JSON")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -692,6 +731,9 @@ def main():
     import __graft_entry__ as ge
     pkg = ge.load_package()
     pkg.load()
+    if a.shard_shares:
+        print(json.dumps(shard_shares(pkg)))
+        return
     ex, ey, ez = (int(v) for v in a.size.split(","))
     progress(f"generating {ex}x{ey}x{ez} problem")
     p = pkg.problem.unit_box(ex, ey, ez, vel_seed=0, lam=1e-3)
