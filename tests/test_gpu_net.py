@@ -256,3 +256,38 @@ def test_net_destroy_after_its_context(pkg, gpu):
     net2 = pkg.Net(ctx2)
     ctx2.close()
     assert net2.h is None and ctx2.h is None
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 1, 2), (1, 17, 1), (33, 2, 1)])
+def test_degenerate_grids_match_oracle(pkg, po, shape):
+    """The layer kernels on node grids smaller than one tile in one, two or
+    three directions (2 nodes in a direction; a 34-node row over three tiles):
+    loss, gradients and prediction against the oracle, as on the interior
+    grids above."""
+    p, ctx, net, vec, dev = _setup(pkg, shape, 7)
+    rng = np.random.default_rng(8)
+    w = (rng.standard_normal((L, T, C, C)) * 0.12).astype(np.float32)
+    b = (rng.standard_normal((L, C)) * 0.05).astype(np.float32)
+    scale = np.array([1.0, 0.8, 1.2, 1, 1, 1], np.float32)
+    net.set_params(w, b)
+    net.set_static(vec["vx"], vec["vy"], vec["vz"], scale, 0.7, 1.0)
+    log(f"net on {shape}: N {p.N}")
+    loss = net.train_step_dev(dev["x1"], dev["x2"], dev["src"], dev["tgt"], 0.0)
+    gw, gb = net.get_grads()
+    lo, dw, db, out = po.net_grads(p.nax, p.nay, p.naz, vec["x1"], vec["x2"], vec["src"],
+                                   vec["vx"], vec["vy"], vec["vz"], vec["tgt"], p.mask, scale,
+                                   0.7, w, b, quant=True, extrap=1.0)
+    log(f"  loss gpu {loss:.6e} oracle {lo:.6e}")
+    assert loss == pytest.approx(lo, rel=2e-3)
+    gw = gw.reshape(L, -1)
+    dw = dw.reshape(L, -1)
+    for layer in range(L):
+        r = np.linalg.norm(gw[layer] - dw[layer]) / max(np.linalg.norm(dw[layer]), 1e-30)
+        assert r <= 5e-3, (layer, r)
+    assert np.linalg.norm(gb - db) <= 1e-3 * np.linalg.norm(db)
+    net.predict_dev(dev["x1"], dev["x2"], dev["src"], dev["guess"])
+    g = ctx.d2h(dev["guess"], p.N)
+    pred = (g - (2.0 * vec["x1"] - vec["x2"])) / 0.7
+    assert np.linalg.norm(pred - out) <= 2e-3 * np.linalg.norm(out)
+    net.close()
+    ctx.close()
