@@ -9,6 +9,8 @@ oracle's matvec_0 / mttvec_0 for every scatter, owner tile shape and z chunk
 (the owner results bitwise equal across them), zero at Dirichlet nodes;
 solves to rtol 1e-10 within 1e-8 rel-L2 of the oracle's FP64 solve to 1e-12
 for BiCG (every scatter) and within 1e-9 for BiCGSTAB and mixed precision;
+FP32 operator <= 2e-6 and FP32 solves to rtol 1e-5 within 1e-4 (plain BiCG
+not on the thin rows, where FP32 rounding at its residual peaks stalls it);
 batched solves of all of them bitwise equal to the single solves."""
 import numpy as np
 import pytest
@@ -89,6 +91,41 @@ def test_apply_matches_oracle(shape, po, scatter):
         y = ctx.apply(x, scatter=scatter, transpose=tr)
         assert rel(y, ref) <= 1e-13, (tr, rel(y, ref))
         assert np.all(y[p.mask == 1] == 0.0)
+
+
+@pytest.mark.parametrize("scatter", ["owner", "colored", "atomic"])
+def test_apply_fp32_matches_oracle(shape, po, pkg, scatter):
+    name, p, bb, _, _, ctx = shape
+    x = np.random.default_rng(28).standard_normal(p.N)
+    ref = po.matvec_0(x, p.iss, bb, p.nax, p.nay, p.naz, p.mask)
+    y = ctx.apply(x, scatter=scatter, precision=pkg.FP32)
+    log(f"apply fp32 {name} {scatter}: {rel(y, ref):.2e}")
+    assert rel(y, ref) <= 2e-6
+    assert np.all(y[p.mask == 1] == 0.0)
+
+
+@pytest.mark.parametrize("method", ["bicg", "bicgstab"])
+def test_fp32_solves(shape, po, pkg, method):
+    """FP32 solves (FP32 vectors and operator, FP64 scalars) to rtol 1e-5:
+    within 1e-4 rel-L2 of the oracle's FP64 solution."""
+    name, p, bb, b, d, ctx = shape
+    if name in ("70x1x3", "1x70x3") and method == "bicg":
+        # plain BiCG's residual peaks far above |b| on the thin rows (FP64:
+        # 150-270 iterations for 210 unknowns), and FP32 rounding at those
+        # peaks leaves it above 1e-5 at itmax (measured: rel-L2 5e-2 and 3e2)
+        pytest.skip("plain BiCG in FP32 does not reach 1e-5 on the thin rows; the FP32 "
+                    "BiCGSTAB solve covers them")
+    b, d, x0 = rhs(po, p, b, d)
+    xo, _ = oracle_solution(po, p, bb, b, d, x0)
+    ctx.set_method(method)
+    try:
+        x, it, err, rc = ctx.solve(b, x0, rtol=1e-5, itmax=5000, precision=pkg.FP32,
+                                   raise_on_fail=False)
+    finally:
+        ctx.set_method("bicg")
+    log(f"fp32 {method} {name}: rc {rc}, iterations {it}, rel-L2 vs oracle {rel(x, xo):.2e}")
+    assert rc == 0 and err <= 1e-5
+    assert rel(x, xo) <= 1e-4
 
 
 def test_owner_tiles_and_chunks(shape, po):
