@@ -347,8 +347,10 @@ def config3(pkg, sols):
     out = {"workload": "HEX8 unit box 256x256x64 elements, 4293185 nodes, lambda log-uniform "
                        "[1e-4, 1e-2] per element, V1 tau, rtol 1e-8 from x0=0"}
     xs = {}
+    # mixed: the inner rtol 1e-4 the workload specifies, and 1e-2, the inner
+    # tolerance that measured best over the configs and batch members
     for name, prec, inner in (("fp64", pkg.FP64, 0.0), ("fp64_bicgstab", pkg.FP64, 0.0),
-                              ("mixed", pkg.MIXED, 1e-4)):
+                              ("mixed", pkg.MIXED, 1e-4), ("mixed_inner_1e-2", pkg.MIXED, 1e-2)):
         ctx.set_method("bicgstab" if name == "fp64_bicgstab" else "bicg")
         for rep in range(2):  # first solve captures the graph
             ctx.h2d(x_dev, zeros)
@@ -363,6 +365,8 @@ def config3(pkg, sols):
     out["mixed_rel_l2_vs_fp64"] = float(np.linalg.norm(xs["mixed"] - xs["fp64"]) /
                                         np.linalg.norm(xs["fp64"]))
     out["speedup_mixed_vs_fp64"] = round(out["fp64"]["seconds"] / out["mixed"]["seconds"], 3)
+    out["speedup_mixed_inner_1e-2_vs_fp64"] = round(
+        out["fp64"]["seconds"] / out["mixed_inner_1e-2"]["seconds"], 3)
     out["bicgstab_rel_l2_vs_fp64"] = float(np.linalg.norm(xs["fp64_bicgstab"] - xs["fp64"]) /
                                            np.linalg.norm(xs["fp64"]))
     out["speedup_bicgstab_vs_fp64"] = round(out["fp64"]["seconds"] /
@@ -766,10 +770,10 @@ def main():
         print(json.dumps({"kernel_only": True, "N": N, "E": E}))
         return
 
-    def solve(precision, rtol=1e-8):
+    def solve(precision, rtol=1e-8, inner=0.0):
         ctx.h2d(x_dev, zeros)
         it, err, sec, rc = ctx.solve_dev(b_dev, x_dev, rtol=rtol, itmax=50000,
-                                         precision=precision)
+                                         precision=precision, inner_rtol=inner)
         if rc != pkg.OK:
             raise RuntimeError(f"solve failed rc={rc} it={it} err={err}")
         return it, err, sec
@@ -793,6 +797,7 @@ def main():
 
     # mixed precision (reported, never fatal for the line)
     msec, mits, rel_l2_mixed, true_res_mixed, mixed_status = [], 0, None, None, "ok"
+    mixed_inner2 = None
     if not a.no_mixed:
         try:
             solve(pkg.MIXED)
@@ -805,6 +810,19 @@ def main():
             rel_l2_mixed = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
             true_res_mixed = ctx.true_residual(b, xm)
             progress(f"mixed: {mits} inner iterations, {1e3 * np.mean(msec):.2f} ms")
+            # the same with inner rtol 1e-2 (the default is 1e-4, as the
+            # 256x256x64 workload specifies)
+            solve(pkg.MIXED, inner=1e-2)
+            m2 = []
+            for _ in range(max(1, min(3, a.steps))):
+                it2, _, sec2 = solve(pkg.MIXED, inner=1e-2)
+                m2.append(sec2)
+            x2 = ctx.d2h(x_dev, N)
+            sols[("c2", "mixed_inner_1e-2")] = x2
+            mixed_inner2 = {"iterations": it2, "seconds": round(float(np.mean(m2)), 5),
+                            "true_rel_residual": ctx.true_residual(b, x2)}
+            progress(f"mixed, inner rtol 1e-2: {it2} inner iterations, "
+                     f"{1e3 * np.mean(m2):.2f} ms")
         except RuntimeError as exc:
             mixed_status = f"failed: {exc}"
             progress(f"mixed {mixed_status}")
@@ -991,6 +1009,8 @@ def main():
                                   if msec else None),
         "mixed_status": mixed_status,
         "mixed_rel_l2_vs_fp64": rel_l2_mixed, "mixed_true_rel_residual": true_res_mixed,
+        "mixed_inner_1e-2": (dict(mixed_inner2, speedup_vs_fp64=round(
+            float(np.mean(secs)) / mixed_inner2["seconds"], 3)) if mixed_inner2 else None),
         "wall_s_timed_steps": round(wall, 3),
         "apply_ms": {"single_fp64": round(t_single, 5), "dual_fp64": round(t_dual, 5),
                      "iteration_launch_fp64": round(t_iter, 5)},
