@@ -355,7 +355,7 @@ def config3(pkg, sols):
     for name, prec, inner in (("fp64", pkg.FP64, 0.0), ("fp64_bicgstab", pkg.FP64, 0.0),
                               ("mixed", pkg.MIXED, 1e-4), ("mixed_inner_1e-2", pkg.MIXED, 1e-2)):
         if name == "mixed":
-            t32 = ctx.autotune(pkg.FP32, reps=5)
+            t32 = ctx.autotune(pkg.MIXED, reps=5)
             out["mixed_launch_shape"] = {"tile": t32["tile"], "zchunk": t32["zchunk"],
                                          "tuned_for": "FP32 iteration launch"}
         ctx.set_method("bicgstab" if name == "fp64_bicgstab" else "bicg")
@@ -814,7 +814,7 @@ def main():
             # (their inner FP32 launches are most of their work); the FP64
             # shape is restored after them
             if tuned is not None and tuned.get("source") == "autotuner":
-                t32 = ctx.autotune(pkg.FP32, reps=20)
+                t32 = ctx.autotune(pkg.MIXED, reps=20)
                 mixed_shape = {"tile": t32["tile"], "zchunk": t32["zchunk"],
                                "tuned_for": "FP32 iteration launch"}
             solve(pkg.MIXED)

@@ -266,11 +266,15 @@ int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
 
 /* Autotuner (no counterpart in the reference, whose CUDA launches use fixed
  * 256-thread blocks, cudatest.cu:339-347): times the solver's owner launch
- * (vgpu_time_kernel what = 3) of `precision`
+ * (vgpu_time_kernel what = 3) of `precision` (VGPU_MIXED: the FP32 launch,
+ * the work of a mixed solve's inner iterations, whose best shape can differ
+ * from the FP64 one: 33- against 22-plane chunks at 256x256x64)
  * for every owner tile shape and z chunk in {0, 4, 8, 16} and ceil(naz / k),
  * k = 1..8 (`reps` launches each), keeps the fastest on the context and
  * returns it.  Every owner node is summed by one thread in a fixed order, so
- * the choice does not change any result bit. */
+ * the choice does not change an operator apply's result bits; a solve's
+ * dot-product partials, and so its iterates, follow the tile shape and z
+ * chunk. */
 int vgpu_autotune(vgpu_ctx *ctx, int precision, int reps, int *scatter,
                   int *tile, int *zchunk, double *best_ms);
 

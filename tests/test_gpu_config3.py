@@ -70,7 +70,7 @@ def test_c3_mixed_loose_inner_tolerance(c3, pkg, po):
     solve (sketch)."""
     p, ctx = c3
     b = ctx.assemble_rhs(p.q_in)
-    t32 = ctx.autotune(pkg.FP32, reps=3)
+    t32 = ctx.autotune(pkg.MIXED, reps=3)
     try:
         x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=50000, precision=pkg.MIXED,
                                    inner_rtol=1e-2, raise_on_fail=False)

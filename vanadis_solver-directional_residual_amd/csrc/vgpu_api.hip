@@ -1905,11 +1905,14 @@ int vgpu_memcpy_d2h(vgpu_ctx *c, void *dst, const void *src, int64_t bytes) {
 
 int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
                   int *tile, int *zchunk, double *best_ms) {
-  if (!c || reps < 1 || (precision != VGPU_FP64 && precision != VGPU_FP32)) {
+  if (!c || reps < 1 ||
+      (precision != VGPU_FP64 && precision != VGPU_FP32 && precision != VGPU_MIXED)) {
     set_error("vgpu_autotune: bad arguments");
     return VGPU_ERR_ARG;
   }
   if (!c->have_coeffs) return VGPU_ERR_STATE;
+  // a mixed solve's work is its FP32 inner launches: tune those
+  const int tp = precision == VGPU_MIXED ? VGPU_FP32 : precision;
   // balanced launch, and z-chunk launches with chunks of naz / k planes (the
   // chunkings whose groups fill k rounds of workgroups evenly) and a few
   // fixed sizes
@@ -1926,7 +1929,7 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
       c->tile = t;
       c->zchunk = zc;
       double ms = 0;
-      const int rc = vgpu_time_kernel(c, 3, VGPU_SCATTER_OWNER, precision, reps, &ms);
+      const int rc = vgpu_time_kernel(c, 3, VGPU_SCATTER_OWNER, tp, reps, &ms);
       if (rc == VGPU_ERR_ARG) continue;  // more partials than the timing buffer
       if (rc) {
         c->tile = save_tile;
