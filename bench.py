@@ -357,7 +357,7 @@ def config3(pkg, sols):
         if name == "mixed":
             t32 = ctx.autotune(pkg.MIXED, reps=5)
             out["mixed_launch_shape"] = {"tile": t32["tile"], "zchunk": t32["zchunk"],
-                                         "tuned_for": "FP32 iteration launch"}
+                                         "tuned_for": "FP32 BiCGSTAB launches of the inner iterations"}
         ctx.set_method("bicgstab" if name == "fp64_bicgstab" else "bicg")
         for rep in range(2):  # first solve captures the graph
             ctx.h2d(x_dev, zeros)
@@ -816,7 +816,7 @@ def main():
             if tuned is not None and tuned.get("source") == "autotuner":
                 t32 = ctx.autotune(pkg.MIXED, reps=20)
                 mixed_shape = {"tile": t32["tile"], "zchunk": t32["zchunk"],
-                               "tuned_for": "FP32 iteration launch"}
+                               "tuned_for": "FP32 BiCGSTAB launches of the inner iterations"}
             solve(pkg.MIXED)
             for _ in range(max(1, min(3, a.steps))):
                 it, err_m, sec = solve(pkg.MIXED)

@@ -256,8 +256,9 @@ int vgpu_memcpy_d2h(vgpu_ctx *ctx, void *dst, const void *src_dev, int64_t bytes
  *           dual apply (owner scatter only),
  *       4 = device copy of 8 N doubles (the size of an iteration's working
  *           set; buffers allocated for the call),
- *       5 / 6 = the BiCGSTAB direction / S owner launch (FP64, owner scatter;
- *           the context's BiCGSTAB buffers, a status that never stops),
+ *       5 / 6 = the BiCGSTAB direction / S owner launch (FP64 or FP32, owner
+ *           scatter; the context's BiCGSTAB buffers of that precision, a
+ *           status that never stops),
  *       7 = device copy of 64 N doubles with 16-byte accesses (the practical
  *           HBM roof: buffers far above the 256 MB last-level cache,
  *           allocated for the call). */
@@ -266,9 +267,10 @@ int vgpu_time_kernel(vgpu_ctx *ctx, int what, int scatter, int precision,
 
 /* Autotuner (no counterpart in the reference, whose CUDA launches use fixed
  * 256-thread blocks, cudatest.cu:339-347): times the solver's owner launch
- * (vgpu_time_kernel what = 3) of `precision` (VGPU_MIXED: the FP32 launch,
- * the work of a mixed solve's inner iterations, whose best shape can differ
- * from the FP64 one: 33- against 22-plane chunks at 256x256x64)
+ * (vgpu_time_kernel what = 3) of `precision` (VGPU_MIXED: the FP32 BiCGSTAB
+ * direction and S launches, what = 5 + 6, the work of a mixed solve's inner
+ * iterations, whose best shape can differ from the FP64 one: 33- against
+ * 22-plane chunks at 256x256x64)
  * for every owner tile shape and z chunk in {0, 4, 8, 16} and ceil(naz / k),
  * k = 1..8 (`reps` launches each), keeps the fastest on the context and
  * returns it.  Every owner node is summed by one thread in a fixed order, so

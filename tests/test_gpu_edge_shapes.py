@@ -237,7 +237,7 @@ def test_autotuners(shape, po, pkg, tmp_path, scatter):
     x = np.random.default_rng(24).standard_normal(p.N)
     cache = str(tmp_path / "tune.json")
     try:
-        for prec in ((pkg.FP64, pkg.FP32) if scatter == "owner" else (pkg.FP64,)):
+        for prec in ((pkg.FP64, pkg.FP32, pkg.MIXED) if scatter == "owner" else (pkg.FP64,)):
             kw = {} if scatter == "owner" else dict(scatter=scatter)
             r = pkg.autotune.tune(ctx, prec, cache_path=cache, reps=2, **kw)
             log(f"autotune {name} {scatter} precision {prec}: tile {r['tile']} zchunk "

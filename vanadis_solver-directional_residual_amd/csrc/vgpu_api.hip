@@ -1911,8 +1911,11 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
     return VGPU_ERR_ARG;
   }
   if (!c->have_coeffs) return VGPU_ERR_STATE;
-  // a mixed solve's work is its FP32 inner launches: tune those
+  // a mixed solve's work is its FP32 inner launches: the direction and S
+  // launches of its inner BiCGSTAB (the FP32 BiCG iteration launch where the
+  // system exceeds the owner launches' limits)
   const int tp = precision == VGPU_MIXED ? VGPU_FP32 : precision;
+  const bool stab_pair = precision == VGPU_MIXED && owner_fits(c->g);
   // balanced launch, and z-chunk launches with chunks of naz / k planes (the
   // chunkings whose groups fill k rounds of workgroups evenly) and a few
   // fixed sizes
@@ -1929,7 +1932,15 @@ int vgpu_autotune(vgpu_ctx *c, int precision, int reps, int *scatter,
       c->tile = t;
       c->zchunk = zc;
       double ms = 0;
-      const int rc = vgpu_time_kernel(c, 3, VGPU_SCATTER_OWNER, tp, reps, &ms);
+      int rc = VGPU_OK;
+      if (stab_pair) {
+        double ms2 = 0;
+        rc = vgpu_time_kernel(c, 5, VGPU_SCATTER_OWNER, tp, reps, &ms);
+        if (!rc) rc = vgpu_time_kernel(c, 6, VGPU_SCATTER_OWNER, tp, reps, &ms2);
+        ms += ms2;
+      } else {
+        rc = vgpu_time_kernel(c, 3, VGPU_SCATTER_OWNER, tp, reps, &ms);
+      }
       if (rc == VGPU_ERR_ARG) continue;  // more partials than the timing buffer
       if (rc) {
         c->tile = save_tile;
@@ -2007,8 +2018,8 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
   if (!c || !avg_ms || reps < 1 || what < 0 || what > 7) return VGPU_ERR_ARG;
   if (what != 2 && what != 4 && what != 7 && !c->have_coeffs) return VGPU_ERR_STATE;
   if (what == 3 && scatter != VGPU_SCATTER_OWNER) return VGPU_ERR_ARG;
-  if (what >= 5 && (scatter != VGPU_SCATTER_OWNER || precision != VGPU_FP64 ||
-                    !owner_fits(c->g)))
+  if (what >= 5 && (scatter != VGPU_SCATTER_OWNER ||
+                    (precision != VGPU_FP64 && precision != VGPU_FP32) || !owner_fits(c->g)))
     return VGPU_ERR_ARG;
   const int64_t N = c->g.N;
   VGPU_HIP(hipSetDevice(c->device));
@@ -2108,16 +2119,15 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
                                             c->zchunk, m0.op.ra != 0.0, dd, v, v + N,
                                             v + 6 * N, v + 7 * N, c->tpart, c->stream);
   };
-  // what 5 / 6: the BiCGSTAB direction / S launch (FP64) on the BiCGSTAB
-  // engine's buffers, inputs from the deterministic vectors above, fixed
-  // reduction partials and a status that never stops (tol 0), so every
-  // workgroup runs its march
-  if (what >= 5) {
-    int rc = ensure_diag(c);
+  // what 5 / 6: the BiCGSTAB direction / S launch (FP64, or FP32 as the
+  // inner iterations of a mixed solve run it) on the BiCGSTAB engine's
+  // buffers, inputs from the deterministic vectors above, fixed reduction
+  // partials and a status that never stops (tol 0), so every workgroup runs
+  // its march
+  auto stab_setup = [&](auto &e, const auto *dg) -> int {
+    using TV = std::remove_const_t<std::remove_pointer_t<std::decay_t<decltype(dg)>>>;
+    int rc = e.alloc(N, (int)owner_groups(c->g, c->tile, stab_zchunk(c)));
     if (rc) return rc;
-    rc = c->stab64.alloc(N, (int)owner_groups(c->g, c->tile, stab_zchunk(c)));
-    if (rc) return rc;
-    Stab<double> &e = c->stab64;
     // partials 1 (r~.s: 2, so that rho = r~.s - omega r~.t is not 0)
     double *fills[] = {e.prt, e.prr, e.pbn, e.pga, e.prn, e.pts, e.ptt, e.prs, e.prq};
     const int64_t counts[] = {e.NB, e.NB, e.NB, e.cap, e.cap, e.cap, e.cap, e.cap, e.cap};
@@ -2133,22 +2143,41 @@ int vgpu_time_kernel(vgpu_ctx *c, int what, int scatter, int precision,
     // inputs from the deterministic vectors, in the engine's own buffers (the
     // launches then address its block as a solve does): R = vx_, x = vt2
     const double *src[] = {c->vx_, c->vx_, c->vt2};
-    double *dst[] = {e.R[0], e.R[1], e.xw};
+    TV *dst[] = {e.R[0], e.R[1], e.xw};
     for (int i = 0; i < 3; ++i)
-      VGPU_HIP(hipMemcpyAsync(dst[i], src[i], N * sizeof(double), hipMemcpyDeviceToDevice,
-                              c->stream));
+      hipLaunchKernelGGL((k_convert<double, TV>), grid_for(N), dim3(256), 0, c->stream, N,
+                         src[i], dst[i]);
+    VGPU_HIP(hipGetLastError());
     VGPU_HIP(hipStreamSynchronize(c->stream));
+    return VGPU_OK;
+  };
+  if (what >= 5) {
+    int rc = ensure_diag(c);
+    if (rc) return rc;
+    rc = precision == VGPU_FP32 ? stab_setup(c->stab32, c->fdiag)
+                                : stab_setup(c->stab64, c->diag);
+    if (rc) return rc;
   }
   auto launch_stab = [&]() -> int {
+    const int zc = stab_zchunk(c);
+    if (precision == VGPU_FP32) {
+      Stab<float> &e = c->stab32;
+      OpData<float> op = c->op32();
+      if (what == 5)
+        return launch_owner_stab<float, float>(
+            op, kStabDir, zc, e.dir_data(0, c->fdiag, e.xw, 0.0, 1 << 30, 1e300), e.R[1],
+            e.V[0], c->stream);
+      return launch_owner_stab<float, float>(
+          op, kStabS, zc, e.s_data(0, c->fdiag, 0.0, 1 << 30, 1e300), e.R[0], e.T, c->stream);
+    }
     Stab<double> &e = c->stab64;
     OpData<double> op = c->op64(c->cH, c->cC);
     if (what == 5)
       return launch_owner_stab<double, double>(
-          op, kStabDir, stab_zchunk(c), e.dir_data(0, c->diag, e.xw, 0.0, 1 << 30, 1e300),
-          e.R[1], e.V[0], c->stream);
+          op, kStabDir, zc, e.dir_data(0, c->diag, e.xw, 0.0, 1 << 30, 1e300), e.R[1],
+          e.V[0], c->stream);
     return launch_owner_stab<double, double>(
-        op, kStabS, stab_zchunk(c), e.s_data(0, c->diag, 0.0, 1 << 30, 1e300), e.R[0], e.T,
-        c->stream);
+        op, kStabS, zc, e.s_data(0, c->diag, 0.0, 1 << 30, 1e300), e.R[0], e.T, c->stream);
   };
   // what 4: a copy of 8 N doubles, the size class of a BiCG iteration's
   // working set (buffers held for this call only)
