@@ -7,7 +7,11 @@
   BiCGSTAB in FP64 and FP32, mixed precision, batch members, the legacy
   drop-in and its allocation-free form;
 * itmax = 0: not converged, x the masked start;
-* a start that already meets rtol: 0 iterations, x bitwise the start."""
+* a start that already meets rtol: 0 iterations, x bitwise the start;
+* a system on which plain Jacobi-BiCG diverges (a 256x256x8 unit box, 32:1
+  element aspect; the oracle's dbcg_simple reaches err 4.5e11 after 4000
+  iterations): BiCG reports it has not converged, BiCGSTAB converges (the
+  oracle's in 499 iterations)."""
 import ctypes as C
 
 import numpy as np
@@ -135,3 +139,22 @@ def test_itmax_zero_and_converged_start(pkg, sys1, method, prec):
             assert np.array_equal(x2, xs)
     finally:
         ctx.set_method("bicg")
+
+
+def test_diverging_bicg_reports_and_bicgstab_converges(pkg, gpu):
+    p = pkg.problem.unit_box(256, 256, 8, vel_seed=0, lam=1e-3)
+    ctx = pkg.Context.from_problem(p)
+    try:
+        b = ctx.assemble_rhs(p.q_in)
+        x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=4000, precision=pkg.FP64,
+                                   raise_on_fail=False)
+        log(f"256x256x8 BiCG: rc {rc}, iterations {it}, err {err:.3e}")
+        assert rc in (pkg.NOT_CONVERGED, pkg.BREAKDOWN) and not err <= 1e-8
+        ctx.set_method("bicgstab")
+        x, it, err, rc = ctx.solve(b, rtol=1e-8, itmax=4000, precision=pkg.FP64)
+        tr = ctx.true_residual(b, x)
+        log(f"256x256x8 BiCGSTAB: rc {rc}, iterations {it}, err {err:.3e}, true {tr:.3e}")
+        assert rc == pkg.OK and err <= 1e-8 and tr <= 1e-7
+    finally:
+        ctx.set_method("bicg")
+        ctx.close()
